@@ -1,0 +1,365 @@
+"""MI355X IFGF forward map for 2D Helmholtz boundary integral equations.
+
+Python host mirror of manuelarturosantana/IFGF2D's operator interface, calling
+the C ABI of libifgf2d.so (include/ifgf2d.h):
+
+* ``ForwardMap``      -- ForwardMap<Np, Formulation, Ps, Pang> (ForwardMap/ForwardMap.hpp:49-177)
+                          with precomps_and_setup folded into construction;
+                          ``compute_Ax_acc`` is the GPU forward map.
+* ``LinearOperator``  -- ForwardMap/LinearOperator.hpp:27-60: rows/cols plus a
+                          mapping(x, out) callable, applied by ``op * x``.
+* ``gmres``           -- the reference's Householder GMRES (opGMRES.hpp:68-269)
+                          on any LinearOperator (host), and ``ForwardMap.solve``
+                          for the device-resident GMRES.
+
+There is no CPU fallback: importing works without a GPU (plans can be built
+host-only for inspection), but every compute call runs the HIP kernels and
+raises if the library or the device is missing.
+"""
+import ctypes
+import math
+import os
+
+import numpy as np
+
+__all__ = [
+    "ForwardMap", "LinearOperator", "gmres", "hankel01", "IFGFError",
+    "circle", "kite", "star", "equal_parameter_lims", "SINGLE_LAYER", "DOUBLE_LAYER", "COMBINED",
+]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libifgf2d.so")
+
+SINGLE_LAYER, DOUBLE_LAYER, COMBINED = 0, 1, 2
+FP64, MIXED = 0, 1
+_FORMS = {"single": SINGLE_LAYER, "double": DOUBLE_LAYER, "combined": COMBINED,
+          SINGLE_LAYER: SINGLE_LAYER, DOUBLE_LAYER: DOUBLE_LAYER, COMBINED: COMBINED}
+_PREC = {"fp64": FP64, "mixed": MIXED, FP64: FP64, MIXED: MIXED}
+
+IFGF_OK, IFGF_ERR_NOT_CONVERGED = 0, 6
+IFGF_PLAN_HOST_ONLY, IFGF_PTR_DEVICE = 1, 1
+
+
+class IFGFError(RuntimeError):
+    pass
+
+
+class _Surface(ctypes.Structure):
+    _fields_ = [("curve_type", ctypes.c_int), ("curve_params", ctypes.POINTER(ctypes.c_double)),
+                ("ncurve_params", ctypes.c_int), ("patch_lims", ctypes.POINTER(ctypes.c_double)),
+                ("npatches", ctypes.c_int), ("Np", ctypes.c_int), ("delta", ctypes.c_double),
+                ("rp_p", ctypes.c_double)]
+
+
+class _Params(ctypes.Structure):
+    _fields_ = [("formulation", ctypes.c_int), ("precision", ctypes.c_int), ("k", ctypes.c_double),
+                ("eta", ctypes.c_double), ("Ps", ctypes.c_int), ("Pang", ctypes.c_int),
+                ("nlevels", ctypes.c_int), ("nrhs_max", ctypes.c_int), ("flags", ctypes.c_int)]
+
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_lp = ctypes.POINTER(ctypes.c_longlong)
+_ip = ctypes.POINTER(ctypes.c_int)
+APPLY_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, _dp, _dp)
+
+EXPORTS = [
+    "ifgf_plan_create", "ifgf_plan_destroy", "ifgf_last_error", "ifgf_plan_error", "ifgf_plan_size",
+    "ifgf_plan_points", "ifgf_plan_stats", "ifgf_plan_sorting", "ifgf_plan_level_cones",
+    "ifgf_plan_correction", "ifgf_apply", "ifgf_apply_direct", "ifgf_propagate", "ifgf_gmres",
+    "ifgf_gmres_host", "ifgf_hankel01", "ifgf_plan_profile", "ifgf_plan_stage_times",
+]
+
+_LIB = None
+
+
+def lib():
+    """Load libifgf2d.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise IFGFError(f"{LIB_PATH} is missing: build it with `make -C ifgf2d_amd` "
+                            "(or __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        L.ifgf_plan_create.argtypes = [ctypes.POINTER(_Surface), ctypes.POINTER(_Params),
+                                       ctypes.POINTER(ctypes.c_void_p)]
+        L.ifgf_plan_destroy.argtypes = [ctypes.c_void_p]
+        L.ifgf_last_error.restype = ctypes.c_char_p
+        L.ifgf_plan_error.argtypes = [ctypes.c_void_p]
+        L.ifgf_plan_error.restype = ctypes.c_char_p
+        L.ifgf_plan_size.argtypes = [ctypes.c_void_p]
+        L.ifgf_plan_size.restype = ctypes.c_longlong
+        L.ifgf_plan_points.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp]
+        L.ifgf_plan_stats.argtypes = [ctypes.c_void_p, _lp]
+        L.ifgf_plan_sorting.argtypes = [ctypes.c_void_p, _lp]
+        L.ifgf_plan_level_cones.argtypes = [ctypes.c_void_p, ctypes.c_int, _lp, _lp]
+        L.ifgf_plan_level_cones.restype = ctypes.c_longlong
+        L.ifgf_plan_correction.argtypes = [ctypes.c_void_p, _lp, _ip, _dp]
+        L.ifgf_plan_correction.restype = ctypes.c_longlong
+        L.ifgf_apply.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                 ctypes.c_longlong, ctypes.c_int, ctypes.c_void_p]
+        L.ifgf_apply_direct.argtypes = [ctypes.c_void_p, _dp, _lp, ctypes.c_longlong, _dp]
+        L.ifgf_propagate.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_longlong, _dp, _dp]
+        L.ifgf_gmres.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int, ctypes.c_double,
+                                 ctypes.c_int, _ip, _dp, _dp, ctypes.c_int]
+        L.ifgf_gmres_host.argtypes = [APPLY_FN, ctypes.c_void_p, ctypes.c_longlong, _dp, _dp,
+                                      ctypes.c_int, ctypes.c_double, ctypes.c_int, _ip, _dp]
+        L.ifgf_hankel01.argtypes = [_dp, ctypes.c_longlong, _dp, _dp]
+        L.ifgf_plan_profile.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.ifgf_plan_stage_times.argtypes = [ctypes.c_void_p, _dp, _lp]
+        _LIB = L
+    return _LIB
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _c(a):
+    return np.ascontiguousarray(a, dtype=np.complex128)
+
+
+# ------------------------------------------------------------------ curves ---
+def circle(cx=0.0, cy=0.0, R=1.0):
+    """Geometry/Curve.hpp:106-125."""
+    return (0, [cx, cy, R])
+
+
+def kite(A=1.0, B=0.65, C=-0.65, D=1.5):
+    """Geometry/Curve.hpp:128-146 (Colton-Kress kite)."""
+    return (1, [A, B, C, D])
+
+
+def star(cx=0.0, cy=0.0, R=1.0, a=(), b=()):
+    """r(t) = R (1 + sum_m a[m] cos(m t) + b[m] sin(m t)), m = 0.. (no reference class)."""
+    n = max(len(a), len(b))
+    a = list(a) + [0.0] * (n - len(a))
+    b = list(b) + [0.0] * (n - len(b))
+    return (2, [cx, cy, R, float(n)] + a + b)
+
+
+def equal_parameter_lims(npatches):
+    return np.linspace(0.0, 2.0 * np.pi, npatches + 1)
+
+
+def _curve_eval(curve, t):
+    typ, p = curve
+    if typ == 0:
+        return p[0] + p[2] * np.cos(t), p[1] + p[2] * np.sin(t)
+    if typ == 1:
+        return p[0] * np.cos(t) + p[1] * np.cos(2 * t) + p[2], p[3] * np.sin(t)
+    n = int(p[3])
+    r = np.ones_like(t)
+    for m in range(n):
+        r = r + p[4 + m] * np.cos(m * t) + p[4 + n + m] * np.sin(m * t)
+    return p[0] + p[2] * r * np.cos(t), p[1] + p[2] * r * np.sin(t)
+
+
+def default_delta(curve, lims, frac=0.75):
+    """Near-singular distance: frac x the shortest patch chord."""
+    x, y = _curve_eval(curve, np.asarray(lims, dtype=np.float64))
+    return frac * float(np.min(np.hypot(np.diff(x), np.diff(y))))
+
+
+# -------------------------------------------------------------- operators ---
+class ForwardMap:
+    """The IFGF-accelerated boundary-integral operator on one closed curve.
+
+    Mirrors ForwardMap<Np, Formulation, Ps, Pang>(delta, curves, ..., p) followed
+    by precomps_and_setup(k, nlevels) (ForwardMap.tpp:15-35, 542-565).
+    formulation: "single" (the reference's SingleLayer S), "double" (1/2 I + K)
+    or "combined" (1/2 I + K - i eta S, the sound-soft CFIE).
+    """
+
+    def __init__(self, curve, patch_lims, k, nlevels, Np=10, formulation="combined", eta=None,
+                 delta=None, p=4.0, Ps=5, Pang=5, precision="fp64", nrhs_max=1, host_only=False):
+        L = lib()
+        self.curve = curve
+        self.lims = np.ascontiguousarray(patch_lims, dtype=np.float64)
+        self.k = float(k)
+        self.eta = float(k if eta is None else eta)
+        self.formulation = _FORMS[formulation]
+        self.delta = default_delta(curve, self.lims) if delta is None else float(delta)
+        self._cp = np.ascontiguousarray(curve[1], dtype=np.float64)
+        surf = _Surface(curve[0], _d(self._cp), self._cp.size, _d(self.lims), self.lims.size - 1,
+                        Np, self.delta, float(p))
+        prm = _Params(self.formulation, _PREC[precision], self.k, self.eta, Ps, Pang, nlevels,
+                      nrhs_max, IFGF_PLAN_HOST_ONLY if host_only else 0)
+        h = ctypes.c_void_p()
+        rc = L.ifgf_plan_create(ctypes.byref(surf), ctypes.byref(prm), ctypes.byref(h))
+        if rc != IFGF_OK:
+            raise IFGFError(f"ifgf_plan_create failed ({rc}): {L.ifgf_last_error().decode()}")
+        self._h = h
+        self.N = L.ifgf_plan_size(h)
+        self.Np, self.Ps, self.Pang, self.nlevels = Np, Ps, Pang, nlevels
+        self.npatches = self.lims.size - 1
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().ifgf_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc not in (IFGF_OK,):
+            raise IFGFError(f"{what} failed ({rc}): {lib().ifgf_plan_error(self._h).decode()}")
+
+    # ----- introspection
+    def points(self):
+        x, y, nx, ny, w = (np.empty(self.N) for _ in range(5))
+        self._check(lib().ifgf_plan_points(self._h, _d(x), _d(y), _d(nx), _d(ny), _d(w)), "points")
+        return x, y, nx, ny, w
+
+    def stats(self):
+        s = np.empty(8, dtype=np.int64)
+        self._check(lib().ifgf_plan_stats(self._h, s.ctypes.data_as(_lp)), "stats")
+        return dict(zip(("near", "leaf", "prop", "interp", "cones", "corr", "nlevels", "N"),
+                        s.tolist()))
+
+    def sorting(self):
+        s = np.empty(self.N, dtype=np.int64)
+        self._check(lib().ifgf_plan_sorting(self._h, s.ctypes.data_as(_lp)), "sorting")
+        return s
+
+    def level_cones(self, level):
+        n = lib().ifgf_plan_level_cones(self._h, level, None, None)
+        if n < 0:
+            raise IFGFError(lib().ifgf_last_error().decode())
+        b, q = np.empty(n, dtype=np.int64), np.empty(n, dtype=np.int64)
+        lib().ifgf_plan_level_cones(self._h, level, b.ctypes.data_as(_lp), q.ctypes.data_as(_lp))
+        return b, q
+
+    def correction(self):
+        n = lib().ifgf_plan_correction(self._h, None, None, None)
+        ptr = np.empty(self.N + 1, dtype=np.int64)
+        patch = np.empty(n, dtype=np.int32)
+        w = np.empty(2 * n * self.Np)
+        lib().ifgf_plan_correction(self._h, ptr.ctypes.data_as(_lp), patch.ctypes.data_as(_ip), _d(w))
+        return ptr, patch, w.view(np.complex128).reshape(n, self.Np)
+
+    # ----- the forward map
+    def compute_Ax_acc(self, x):
+        """y = A x on the GPU for host vectors; x may be (N,) or (N, nrhs) (column major)."""
+        x = np.asarray(x, dtype=np.complex128)
+        nrhs = 1 if x.ndim == 1 else x.shape[1]
+        xf = np.asfortranarray(x.reshape(self.N, nrhs))
+        y = np.empty_like(xf, order="F")
+        self._check(lib().ifgf_apply(self._h, xf.ctypes.data, y.ctypes.data, nrhs, self.N, 0, None),
+                    "ifgf_apply")
+        return y.reshape(x.shape)
+
+    __call__ = compute_Ax_acc
+
+    def apply_device(self, x_ptr, y_ptr, nrhs=1, ld=None, stream=None):
+        """Asynchronous apply on device pointers (e.g. torch tensors' data_ptr())."""
+        self._check(lib().ifgf_apply(self._h, ctypes.c_void_p(x_ptr), ctypes.c_void_p(y_ptr), nrhs,
+                                     self.N if ld is None else ld, IFGF_PTR_DEVICE,
+                                     ctypes.c_void_p(stream) if stream else None), "ifgf_apply")
+
+    def compute_Ax_unacc(self, x, targets):
+        """Unaccelerated operator (direct sum + correction) at a target subset."""
+        x = _c(x)
+        t = np.ascontiguousarray(targets, dtype=np.int64)
+        out = np.empty(t.size, dtype=np.complex128)
+        self._check(lib().ifgf_apply_direct(self._h, _d(x.view(np.float64)), t.ctypes.data_as(_lp),
+                                            t.size, _d(out.view(np.float64))), "ifgf_apply_direct")
+        return out
+
+    def propagate_unacc(self, tx, ty, density):
+        """Field of the layer potential at off-surface points (ForwardMap.tpp:702-727)."""
+        tx = np.ascontiguousarray(tx, dtype=np.float64)
+        ty = np.ascontiguousarray(ty, dtype=np.float64)
+        d = _c(density)
+        out = np.empty(tx.size, dtype=np.complex128)
+        self._check(lib().ifgf_propagate(self._h, _d(tx), _d(ty), tx.size, _d(d.view(np.float64)),
+                                         _d(out.view(np.float64))), "ifgf_propagate")
+        return out
+
+    def solve(self, rhs, restart=50, tol=1e-8, maxit=1000, x0=None):
+        """Device GMRES (ForwardMap::solve, ForwardMap.tpp:658-697).
+
+        Returns (x, iterations, true relative residual, residual-estimate history)."""
+        b = _c(rhs)
+        x = np.zeros_like(b) if x0 is None else _c(x0).copy()
+        it, rr = ctypes.c_int(), ctypes.c_double()
+        hist = np.zeros(maxit)
+        rc = lib().ifgf_gmres(self._h, _d(b.view(np.float64)), _d(x.view(np.float64)), restart, tol,
+                              maxit, ctypes.byref(it), ctypes.byref(rr), _d(hist), hist.size)
+        if rc not in (IFGF_OK, IFGF_ERR_NOT_CONVERGED):
+            self._check(rc, "ifgf_gmres")
+        return x, it.value, rr.value, hist[: it.value]
+
+    def profile(self, enable=True):
+        self._check(lib().ifgf_plan_profile(self._h, 1 if enable else 0), "profile")
+
+    def stage_times(self):
+        t = np.zeros(6)
+        c = ctypes.c_longlong()
+        self._check(lib().ifgf_plan_stage_times(self._h, _d(t), ctypes.byref(c)), "stage_times")
+        return dict(zip(("gather", "near", "leaf", "propagation", "interpolation", "finalize"),
+                        t.tolist())), c.value
+
+    def linear_operator(self):
+        """The LinearOperator that ForwardMap::solve builds (ForwardMap.tpp:673-682)."""
+        def mapping(x, out):
+            out[:] = self.compute_Ax_acc(x)
+        return LinearOperator(self.N, self.N, mapping)
+
+
+class LinearOperator:
+    """LinearOperator(rows, cols, mapping) (ForwardMap/LinearOperator.hpp:27-60).
+
+    ``op * x`` allocates the output and calls mapping(x, out) (LinearOperator.cpp:14-23)."""
+
+    def __init__(self, rows, cols, mapping):
+        self._rows, self._cols, self.mapping = int(rows), int(cols), mapping
+
+    def rows(self):
+        return self._rows
+
+    def cols(self):
+        return self._cols
+
+    def __mul__(self, x):
+        out = np.empty(self._rows, dtype=np.complex128)
+        self.mapping(np.asarray(x, dtype=np.complex128), out)
+        return out
+
+    __matmul__ = __mul__
+
+
+def gmres(op, rhs, restart=50, tol=1e-8, maxit=300, x0=None):
+    """Householder GMRES on any LinearOperator (Eigen::opGMRES, opGMRES.hpp:68-269).
+
+    Returns (x, iterations, estimated relative residual, converged)."""
+    n = op.rows()
+    b = _c(rhs)
+    x = np.zeros(n, dtype=np.complex128) if x0 is None else _c(x0).copy()
+
+    def cb(ctx, xp, yp):
+        xin = np.ctypeslib.as_array(xp, shape=(2 * n,)).view(np.complex128)
+        yout = np.ctypeslib.as_array(yp, shape=(2 * n,)).view(np.complex128)
+        yout[:] = op * xin.copy()
+
+    fn = APPLY_FN(cb)
+    it, err = ctypes.c_int(), ctypes.c_double()
+    rc = lib().ifgf_gmres_host(fn, None, n, _d(b.view(np.float64)), _d(x.view(np.float64)), restart,
+                               tol, maxit, ctypes.byref(it), ctypes.byref(err))
+    if rc not in (IFGF_OK, IFGF_ERR_NOT_CONVERGED):
+        raise IFGFError(lib().ifgf_last_error().decode())
+    return x, it.value, err.value, rc == IFGF_OK
+
+
+def hankel01(x):
+    """H0^(1)(x), H1^(1)(x) evaluated by the GPU kernels' FP64 routine."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    h0 = np.empty(x.size, dtype=np.complex128)
+    h1 = np.empty(x.size, dtype=np.complex128)
+    rc = lib().ifgf_hankel01(_d(x), x.size, _d(h0.view(np.float64)), _d(h1.view(np.float64)))
+    if rc != IFGF_OK:
+        raise IFGFError(lib().ifgf_last_error().decode())
+    return h0, h1

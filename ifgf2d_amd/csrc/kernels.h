@@ -1,0 +1,329 @@
+// gfx950 kernels of the IFGF forward map (one apply = the six launches below).
+//
+//   k_gather       rho_s[i] = x[perm[i]] * w[perm[i]]          (SortDensity + compute_intensities)
+//   k_near<F>      near field over leaf neighbours             (BoxTree.h:614-683)
+//   k_leaf<F,P>    leaf cone values + 2D Chebyshev transform   (BoxTree.h:685-762, Interpolator.h:81-120)
+//   k_prop<P>      child -> parent cone propagation            (BoxTree.h:767-868)
+//   k_interp<P>    cone interpolation at every target          (BoxTree.h:876-937)
+//   k_finalize     y = ident x + correction(x) + unsort(sol_s) (ForwardMap.tpp:481-515,609-655)
+//
+// Data layout in HBM: points in Morton order as separate x / y / nx / ny FP64
+// arrays (coalesced per target thread); cone coefficients of all levels in one
+// array [global cone][P] complex, P = Ps*Pang with index itheta*Ps + is
+// (Interpolator.h:46-49); per-cone geometry in a 64-byte ConeGeo record.
+// Every discrete index (which box, which cone) comes from host tables.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "green.h"
+#include "tables.h"
+
+namespace ifgf {
+
+// Functions::Atan2 (PolarCoordinates.h:8-27): theta in [0, 2 pi).
+__device__ inline double atan2_02pi(double y, double x) {
+  if (x > 0 && y >= 0) return atan(y / x);
+  if (x > 0 && y < 0) return atan(y / x) + 2.0 * M_PI;
+  if (x < 0 && y > 0) return atan(y / x) + M_PI;
+  if (x < 0 && y == 0) return M_PI;
+  if (x < 0 && y < 0) return atan(y / x) + M_PI;
+  if (x == 0 && y > 0) return M_PI_2;
+  if (x == 0 && y < 0) return 3.0 * M_PI_2;
+  return 0.0;
+}
+
+// Level::Cheb2Cart + Cheb2Radius (Level.h:347-375, 406-411).
+__device__ inline void cone_node(const ConeGeo& g, double s, double t, double& x, double& y,
+                                 double& r) {
+  r = g.h / ((s + 1.0) / 2.0 * g.dr + g.rmin);
+  const double th = (t + 1.0) / 2.0 * g.da + g.th0;
+  double sn, cs;
+  sincos(th, &sn, &cs);
+  x = r * cs + g.cx;
+  y = r * sn + g.cy;
+}
+
+// Level::Cart2Cheb with a given cone + Cheb2Radius (Level.h:424-437, 406-411).
+__device__ inline void cone_local(const ConeGeo& g, double x, double y, double& s, double& t,
+                                  double& radius) {
+  const double dx = x - g.cx, dy = y - g.cy;
+  const double r = sqrt(dx * dx + dy * dy);
+  const double th = atan2_02pi(dy, dx);
+  s = (g.h / r - g.rmin) / g.dr * 2.0 - 1.0;
+  t = (th - g.th0) / g.da * 2.0 - 1.0;
+  radius = g.h / ((s + 1.0) / 2.0 * g.dr + g.rmin);
+}
+
+// Interpolator::Interpolate (Interpolator.h:55-77).
+template <int Ps, int Pang, typename CT>
+__device__ inline zd cone_eval(const CT* __restrict__ c, double s, double t) {
+  double ts[Ps], ta[Pang];
+  ts[0] = 1.0;
+  if (Ps > 1) ts[1] = s;
+#pragma unroll
+  for (int i = 2; i < Ps; i++) ts[i] = 2.0 * s * ts[i - 1] - ts[i - 2];
+  ta[0] = 1.0;
+  if (Pang > 1) ta[1] = t;
+#pragma unroll
+  for (int i = 2; i < Pang; i++) ta[i] = 2.0 * t * ta[i - 1] - ta[i - 2];
+  zd acc{0.0, 0.0};
+#pragma unroll
+  for (int a = 0; a < Pang; a++) {
+    zd row{0.0, 0.0};
+#pragma unroll
+    for (int m = 0; m < Ps; m++) {
+      const CT v = c[a * Ps + m];
+      row.x = fma(ts[m], (double)v.x, row.x);
+      row.y = fma(ts[m], (double)v.y, row.y);
+    }
+    acc.x = fma(ta[a], row.x, acc.x);
+    acc.y = fma(ta[a], row.y, acc.y);
+  }
+  return acc;
+}
+
+// Separable 2D Chebyshev transform of the P values of one cone held in LDS
+// (Interpolator::GenerateInterpolant, Interpolator.h:81-120). Thread j of the
+// cone computes output j; buf and tmp are this cone's P-slot LDS windows.
+template <int Ps, int Pang>
+__device__ inline zd cone_transform(const zd* buf, zd* tmp, const ChebTables& T, int j, bool active) {
+  const int n = j / Ps, m = j % Ps;
+  // stage 1: tmp[n][k] = sum_l f[l][k] Ta[n][l]
+  if (active) {
+    zd g{0.0, 0.0};
+#pragma unroll
+    for (int l = 0; l < Pang; l++) {
+      const zd f = buf[l * Ps + m];
+      g.x = fma(f.x, T.Ta[n * 16 + l], g.x);
+      g.y = fma(f.y, T.Ta[n * 16 + l], g.y);
+    }
+    tmp[j] = g;
+  }
+  __syncthreads();
+  // stage 2: C[n][m] = sum_k tmp[n][k] Ts[m][k]
+  zd c{0.0, 0.0};
+  if (active) {
+#pragma unroll
+    for (int k = 0; k < Ps; k++) {
+      const zd v = tmp[n * Ps + k];
+      c.x = fma(v.x, T.Ts[m * 16 + k], c.x);
+      c.y = fma(v.y, T.Ts[m * 16 + k], c.y);
+    }
+  }
+  return c;
+}
+
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_gather(int N, const zd* __restrict__ x,
+                                                const double* __restrict__ w,
+                                                const int* __restrict__ perm, zd* __restrict__ rho) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const int o = perm[i];
+  rho[i] = cscale(x[o], w[o]);
+}
+
+template <int F>
+__global__ void __launch_bounds__(256)
+    k_near(int N, const double* __restrict__ px, const double* __restrict__ py,
+           const double* __restrict__ pnx, const double* __restrict__ pny,
+           const int* __restrict__ patch, const int4* __restrict__ excl,
+           const int* __restrict__ leaf_of_point, const int* __restrict__ leaf_begin,
+           const int* __restrict__ nb_ptr, const int* __restrict__ nb_list,
+           const zd* __restrict__ rho, zd* __restrict__ sol, double k, double eta) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const double tx = px[i], ty = py[i];
+  const int4 e = excl[i];
+  const int b = leaf_of_point[i];
+  zd acc{0.0, 0.0};
+  for (int n = nb_ptr[b]; n < nb_ptr[b + 1]; n++) {
+    const int nb = nb_list[n];
+    const int s1 = leaf_begin[nb + 1];
+    for (int s = leaf_begin[nb]; s < s1; s++) {
+      const int ps = patch[s];
+      if (ps == e.x || ps == e.y || ps == e.z || ps == e.w) continue;
+      const zd g = green<F>(tx, ty, px[s], py[s], pnx[s], pny[s], k, eta);
+      cfma(acc, g, rho[s]);
+    }
+  }
+  sol[i] = acc;
+}
+
+template <int F, int Ps, int Pang>
+__global__ void __launch_bounds__(256)
+    k_leaf(int ncone, int cone_off, const ConeGeo* __restrict__ cg, const int* __restrict__ cone_box,
+           const int* __restrict__ leaf_begin, const double* __restrict__ px,
+           const double* __restrict__ py, const double* __restrict__ pnx,
+           const double* __restrict__ pny, const zd* __restrict__ rho, zd* __restrict__ coef,
+           const ChebTables* __restrict__ tabs, double k, double eta) {
+  constexpr int P = Ps * Pang;
+  constexpr int CPB = 256 / P;
+  __shared__ zd buf[CPB * P];
+  __shared__ zd tmp[CPB * P];
+  __shared__ ChebTables T;
+  for (int q = threadIdx.x; q < (int)(sizeof(ChebTables) / sizeof(double)); q += blockDim.x)
+    ((double*)&T)[q] = ((const double*)tabs)[q];
+  __syncthreads();
+  const int cl = threadIdx.x / P, j = threadIdx.x % P;
+  const int c = blockIdx.x * CPB + cl;
+  const bool active = cl < CPB && c < ncone;
+  const int g = cone_off + c;
+  if (active) {
+    const ConeGeo G = cg[g];
+    double nx, ny, r;
+    cone_node(G, T.xs[j % Ps], T.xa[j / Ps], nx, ny, r);
+    const int b = cone_box[g];
+    zd acc{0.0, 0.0};
+    for (int s = leaf_begin[b]; s < leaf_begin[b + 1]; s++) {
+      const zd kv = green<F>(nx, ny, px[s], py[s], pnx[s], pny[s], k, eta);
+      cfma(acc, kv, rho[s]);
+    }
+    buf[threadIdx.x] = cmul(acc, inv_centre_factor(r, k));
+  }
+  __syncthreads();
+  const int base = cl * P;
+  const zd out = cone_transform<Ps, Pang>(buf + (active ? base : 0), tmp + (active ? base : 0), T,
+                                          active ? j : 0, active);
+  if (active) coef[(size_t)g * P + j] = out;
+}
+
+// Propagation into the parent cones of level l-1 from the child cones of
+// level l. proto: [parent nonrel slot][4][P] child nonrel slot; dmap: level-l
+// dense (box, nonrel slot) -> local relevant cone.
+template <int Ps, int Pang>
+__global__ void __launch_bounds__(256)
+    k_prop(int npc, int pcone_off, int pbox_off, int pnrad, int ccone_off, int cnslot,
+           const ConeGeo* __restrict__ cg, const int* __restrict__ cone_box,
+           const int* __restrict__ cone_q, const int* __restrict__ box_child,
+           const int* __restrict__ proto, const int* __restrict__ dmap, zd* __restrict__ coef,
+           const ChebTables* __restrict__ tabs, double k) {
+  constexpr int P = Ps * Pang;
+  constexpr int CPB = 256 / P;
+  __shared__ zd buf[CPB * P];
+  __shared__ zd tmp[CPB * P];
+  __shared__ ChebTables T;
+  for (int q = threadIdx.x; q < (int)(sizeof(ChebTables) / sizeof(double)); q += blockDim.x)
+    ((double*)&T)[q] = ((const double*)tabs)[q];
+  __syncthreads();
+  const int cl = threadIdx.x / P, j = threadIdx.x % P;
+  const int c = blockIdx.x * CPB + cl;
+  const bool active = cl < CPB && c < npc;
+  const int g = pcone_off + c;
+  if (active) {
+    const ConeGeo G = cg[g];
+    double nx, ny, r;
+    cone_node(G, T.xs[j % Ps], T.xa[j / Ps], nx, ny, r);
+    const int pb = cone_box[g];
+    const int q = cone_q[g];
+    zd acc{0.0, 0.0};
+#pragma unroll
+    for (int cp = 0; cp < 4; cp++) {
+      const int cb = box_child[(size_t)(pbox_off + pb) * 4 + cp];
+      if (cb < 0) continue;
+      const int cq = proto[((size_t)q * 4 + cp) * P + j];
+      const int cc = ccone_off + dmap[(size_t)cb * cnslot + cq];
+      const ConeGeo C = cg[cc];
+      double s, t, rad;
+      cone_local(C, nx, ny, s, t, rad);
+      const zd v = cone_eval<Ps, Pang>(coef + (size_t)cc * P, s, t);
+      cfma(acc, v, centre_factor(rad, k));
+    }
+    buf[threadIdx.x] = cmul(acc, inv_centre_factor(r, k));
+  }
+  (void)pnrad;
+  __syncthreads();
+  const int base = cl * P;
+  const zd out = cone_transform<Ps, Pang>(buf + (active ? base : 0), tmp + (active ? base : 0), T,
+                                          active ? j : 0, active);
+  if (active) coef[(size_t)g * P + j] = out;
+}
+
+template <int Ps, int Pang>
+__global__ void __launch_bounds__(256)
+    k_interp(int N, const double* __restrict__ px, const double* __restrict__ py,
+             const long long* __restrict__ it_ptr, const int* __restrict__ it_cone,
+             const ConeGeo* __restrict__ cg, const zd* __restrict__ coef, zd* __restrict__ sol,
+             double k) {
+  constexpr int P = Ps * Pang;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const double x = px[i], y = py[i];
+  zd acc{0.0, 0.0};
+  const long long e1 = it_ptr[i + 1];
+  for (long long e = it_ptr[i]; e < e1; e++) {
+    const int g = it_cone[e];
+    const ConeGeo C = cg[g];
+    double s, t, rad;
+    cone_local(C, x, y, s, t, rad);
+    const zd v = cone_eval<Ps, Pang>(coef + (size_t)g * P, s, t);
+    cfma(acc, v, centre_factor(rad, k));
+  }
+  sol[i] = cadd(sol[i], acc);
+}
+
+__global__ void __launch_bounds__(256)
+    k_finalize(int N, int Np, double ident, const zd* __restrict__ x,
+               const long long* __restrict__ corr_ptr, const int* __restrict__ corr_patch,
+               const zd* __restrict__ corr_w, const int* __restrict__ inv,
+               const zd* __restrict__ sol, zd* __restrict__ y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  zd acc = cscale(x[i], ident);
+  for (long long e = corr_ptr[i]; e < corr_ptr[i + 1]; e++) {
+    const zd* w = corr_w + e * Np;
+    const zd* xs = x + (size_t)corr_patch[e] * Np;
+    for (int q = 0; q < Np; q++) cfma(acc, w[q], xs[q]);
+  }
+  y[i] = cadd(acc, sol[inv[i]]);
+}
+
+// Direct sum at a target subset, excluding the same pairs the correction
+// replaces (compute_non_singular_interactions_unacc, ForwardMap.tpp:438-478);
+// excl_t holds kMaxExcl excluded patches per target.
+template <int F>
+__global__ void __launch_bounds__(256)
+    k_direct(int nt, const int* __restrict__ targets, const int* __restrict__ excl_t, int N, int Np,
+             const double* __restrict__ ox, const double* __restrict__ oy,
+             const double* __restrict__ onx, const double* __restrict__ ony,
+             const zd* __restrict__ rho_o, zd* __restrict__ out, double k, double eta) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nt) return;
+  const int i = targets[t];
+  const int* e = excl_t + (size_t)t * 4;
+  const double tx = ox[i], ty = oy[i];
+  zd acc{0.0, 0.0};
+  for (int s = 0; s < N; s++) {
+    const int ps = s / Np;
+    if (ps == e[0] || ps == e[1] || ps == e[2] || ps == e[3]) continue;
+    cfma(acc, green<F>(tx, ty, ox[s], oy[s], onx[s], ony[s], k, eta), rho_o[s]);
+  }
+  out[t] = acc;
+}
+
+// Off-surface field (propagate_unacc, ForwardMap.tpp:702-727).
+template <int F>
+__global__ void __launch_bounds__(256)
+    k_propagate(int nt, const double* __restrict__ tx, const double* __restrict__ ty, int N,
+                const double* __restrict__ ox, const double* __restrict__ oy,
+                const double* __restrict__ onx, const double* __restrict__ ony,
+                const zd* __restrict__ rho_o, zd* __restrict__ out, double k, double eta) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nt) return;
+  zd acc{0.0, 0.0};
+  for (int s = 0; s < N; s++) cfma(acc, green<F>(tx[t], ty[t], ox[s], oy[s], onx[s], ony[s], k, eta), rho_o[s]);
+  out[t] = acc;
+}
+
+__global__ void __launch_bounds__(256)
+    k_hankel(long long n, const double* __restrict__ x, zd* __restrict__ h0, zd* __restrict__ h1) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  zd a, b;
+  hankel01(x[i], a, b);
+  h0[i] = a;
+  h1[i] = b;
+}
+
+}  // namespace ifgf

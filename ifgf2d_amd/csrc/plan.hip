@@ -1,0 +1,471 @@
+// C ABI of the MI355X IFGF forward map (include/ifgf2d.h): plan creation,
+// device tables, the apply sequence and the auxiliary entry points.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/ifgf2d.h"
+#include "kernels.h"
+#include "plan_internal.h"
+#include "solver.h"
+
+using namespace ifgf;
+
+static thread_local std::string g_last_error;
+
+static int fail(ifgf_plan* p, int code, const std::string& msg) {
+  g_last_error = msg;
+  if (p) p->err = msg;
+  return code;
+}
+int ifgf::plan_fail(ifgf_plan* p, int code, const std::string& msg) { return fail(p, code, msg); }
+
+static void build_device_side(ifgf_plan* P) {
+  const Tree& t = P->tree;
+  const long long N = t.N;
+  const int Pn = t.Ps * t.Pang;
+  HIP_TRY(hipStreamCreate(&P->stream));
+  for (auto& e : P->ev) HIP_TRY(hipEventCreate(&e));
+  P->px.upload(t.sx);
+  P->py.upload(t.sy);
+  P->pnx.upload(t.snx);
+  P->pny.upload(t.sny);
+  P->w.upload(P->geo.w);
+  P->patch.upload(P->patch_sorted);
+  P->perm.upload(t.perm);
+  P->inv.upload(t.inv);
+  P->leaf_of_point.upload(t.leaf_of_point);
+  P->leaf_begin.upload(t.leaf_begin);
+  P->nb_ptr.upload(t.nb_ptr);
+  P->nb_list.upload(t.nb_list);
+  std::vector<int4> ex(N);
+  for (long long i = 0; i < N; i++)
+    ex[i] = make_int4(P->excl[i * 4], P->excl[i * 4 + 1], P->excl[i * 4 + 2], P->excl[i * 4 + 3]);
+  P->excl_d.upload(ex);
+  P->cg.upload(P->cone_geo);
+  P->cone_box.upload(t.cone_box);
+  P->cone_qd.upload(P->cone_q);
+  P->box_child.upload(t.box_child);
+  P->dense_map.upload(t.dense_map);
+  P->proto.upload(t.proto);
+  P->it_ptr.upload(t.it_ptr);
+  P->it_cone.upload(t.it_cone);
+  P->corr_ptr.upload(P->corr.ptr);
+  P->corr_patch.upload(P->corr.patch);
+  P->corr_w.upload(P->corr.w);
+  P->tabs.upload(std::vector<ChebTables>{P->cheb});
+  const int R = std::max(1, P->prm.nrhs_max);
+  P->rho.alloc((size_t)N * R);
+  P->sol.alloc((size_t)N * R);
+  P->coef.alloc((size_t)t.n_cones * Pn * R);
+  P->xbuf.alloc((size_t)N * R);
+  P->ybuf.alloc((size_t)N * R);
+  P->on_device = true;
+}
+
+extern "C" {
+
+const char* ifgf_last_error(void) { return g_last_error.c_str(); }
+const char* ifgf_plan_error(const ifgf_plan* p) { return p ? p->err.c_str() : ""; }
+
+int ifgf_plan_create(const ifgf_surface* s, const ifgf_params* prm, ifgf_plan** out) {
+  if (!s || !prm || !out) return fail(nullptr, IFGF_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (prm->nlevels < 3) return fail(nullptr, IFGF_ERR_LEVELS, "IFGF accelerator requires at least 3 levels.");
+  if (prm->Ps < 2 || prm->Ps > 16 || prm->Pang < 2 || prm->Pang > 16)
+    return fail(nullptr, IFGF_ERR_ARG, "Ps and Pang must lie in [2, 16]");
+  if (prm->formulation < 0 || prm->formulation > 2) return fail(nullptr, IFGF_ERR_ARG, "unknown formulation");
+  if (prm->precision != IFGF_FP64) return fail(nullptr, IFGF_ERR_ARG, "unsupported precision mode");
+  if (!s->patch_lims || s->npatches < 3 || !(s->delta > 0) || !(s->rp_p >= 2))
+    return fail(nullptr, IFGF_ERR_ARG, "invalid surface description");
+  std::unique_ptr<ifgf_plan> P(new ifgf_plan);
+  P->prm = *prm;
+  if (P->prm.nrhs_max < 1) P->prm.nrhs_max = 1;
+  std::string err;
+  try {
+    if (!make_curve(s->curve_type, s->curve_params, s->ncurve_params, P->curve, err))
+      return fail(nullptr, IFGF_ERR_ARG, err);
+    if (!build_geometry(P->curve, s->patch_lims, s->npatches, s->Np, s->delta, P->geo, err))
+      return fail(nullptr, IFGF_ERR_GEOMETRY, err);
+    if (!build_correction(P->geo, P->curve, prm->formulation, prm->k, prm->eta, s->rp_p, P->corr, err))
+      return fail(nullptr, IFGF_ERR_GEOMETRY, err);
+    if (!build_tree(P->geo.x.data(), P->geo.y.data(), P->geo.nx.data(), P->geo.ny.data(), P->geo.N,
+                    prm->nlevels, prm->k, prm->Ps, prm->Pang, P->tree, err))
+      return fail(nullptr, IFGF_ERR_GEOMETRY, err);
+    if (!build_exclusions(P->geo, P->tree, P->patch_sorted, P->excl, P->n_near, err))
+      return fail(nullptr, IFGF_ERR_GEOMETRY, err);
+    // per-cone geometry records
+    const Tree& t = P->tree;
+    P->cone_geo.resize(t.n_cones);
+    P->cone_q.resize(t.n_cones);
+    for (long long g = 0; g < t.n_cones; g++) {
+      const LevelInfo& L = t.lv[t.cone_level[g]];
+      const int b = t.cone_box[g], posr = t.cone_posr[g], post = t.cone_post[g];
+      ConeGeo& G = P->cone_geo[g];
+      G.cx = t.box_cx[L.box_off + b];
+      G.cy = t.box_cy[L.box_off + b];
+      G.h = L.h;
+      G.rmin = L.h / t.radial[L.radial_off + posr + 1];
+      G.dr = L.h / t.radial[L.radial_off + posr] - G.rmin;
+      G.da = M_PI / L.nce;
+      G.th0 = G.da * post;
+      G.pad = 0;
+      P->cone_q[g] = post * L.nrad + posr;
+    }
+    // Chebyshev tables (Chebyshev.h:10-68) with the transform's alpha factors
+    const int Ps = prm->Ps, Pa = prm->Pang;
+    auto pts = [](int n, double* x) {
+      for (int i = 0; i < n; i++) x[n - 1 - i] = std::cos(M_PI * (2.0 * i + 1.0) / (2.0 * n));
+    };
+    pts(Ps, P->cheb.xs);
+    pts(Pa, P->cheb.xa);
+    auto tn = [](int n, const double* x, double* T, double scale) {
+      std::vector<double> t((size_t)n * n);
+      for (int i = 0; i < n; i++) {
+        t[i] = 1.0;
+        t[n + i] = x[i];
+      }
+      for (int m = 2; m < n; m++)
+        for (int i = 0; i < n; i++) t[m * n + i] = 2.0 * x[i] * t[(m - 1) * n + i] - t[(m - 2) * n + i];
+      for (int m = 0; m < n; m++)
+        for (int i = 0; i < n; i++) T[m * 16 + i] = t[m * n + i] * (m == 0 ? 1.0 : 2.0) * scale;
+    };
+    tn(Ps, P->cheb.xs, P->cheb.Ts, 1.0);
+    tn(Pa, P->cheb.xa, P->cheb.Ta, 1.0 / (Ps * Pa));
+    if (!(prm->flags & IFGF_PLAN_HOST_ONLY)) build_device_side(P.get());
+  } catch (std::exception& e) {
+    return fail(nullptr, IFGF_ERR_HIP, e.what());
+  }
+  *out = P.release();
+  return IFGF_OK;
+}
+
+void ifgf_plan_destroy(ifgf_plan* p) { delete p; }
+
+long long ifgf_plan_size(const ifgf_plan* p) { return p ? p->geo.N : -1; }
+
+int ifgf_plan_points(const ifgf_plan* p, double* x, double* y, double* nx, double* ny, double* w) {
+  if (!p) return fail(nullptr, IFGF_ERR_ARG, "null plan");
+  const auto& g = p->geo;
+  if (x) std::copy(g.x.begin(), g.x.end(), x);
+  if (y) std::copy(g.y.begin(), g.y.end(), y);
+  if (nx) std::copy(g.nx.begin(), g.nx.end(), nx);
+  if (ny) std::copy(g.ny.begin(), g.ny.end(), ny);
+  if (w) std::copy(g.w.begin(), g.w.end(), w);
+  return IFGF_OK;
+}
+
+int ifgf_plan_stats(const ifgf_plan* p, long long out[8]) {
+  if (!p || !out) return fail(nullptr, IFGF_ERR_ARG, "null argument");
+  out[0] = p->n_near;
+  out[1] = p->tree.n_leaf;
+  out[2] = p->tree.n_prop;
+  out[3] = p->tree.n_interp;
+  out[4] = p->tree.n_cones;
+  out[5] = (long long)p->corr.patch.size();
+  out[6] = p->tree.nlev;
+  out[7] = p->geo.N;
+  return IFGF_OK;
+}
+
+int ifgf_plan_sorting(const ifgf_plan* p, long long* sorting) {
+  if (!p || !sorting) return fail(nullptr, IFGF_ERR_ARG, "null argument");
+  for (long long i = 0; i < p->tree.N; i++) sorting[i] = p->tree.perm[i];
+  return IFGF_OK;
+}
+
+long long ifgf_plan_level_cones(const ifgf_plan* p, int level, long long* box, long long* nonrel) {
+  if (!p || level < 2 || level > p->tree.D) {
+    fail(nullptr, IFGF_ERR_ARG, "level out of range");
+    return -1;
+  }
+  const LevelInfo& L = p->tree.lv[level];
+  for (int c = 0; c < L.ncone; c++) {
+    const int g = L.cone_off + c;
+    if (box) box[c] = p->tree.box_morton[L.box_off + p->tree.cone_box[g]];
+    if (nonrel) nonrel[c] = (long long)p->tree.cone_post[g] * L.nrad + p->tree.cone_posr[g];
+  }
+  return L.ncone;
+}
+
+long long ifgf_plan_correction(const ifgf_plan* p, long long* ptr, int* patch, double* w) {
+  if (!p) {
+    fail(nullptr, IFGF_ERR_ARG, "null plan");
+    return -1;
+  }
+  if (ptr) std::copy(p->corr.ptr.begin(), p->corr.ptr.end(), ptr);
+  if (patch) std::copy(p->corr.patch.begin(), p->corr.patch.end(), patch);
+  if (w) std::memcpy(w, p->corr.w.data(), p->corr.w.size() * sizeof(zd));
+  return (long long)p->corr.patch.size();
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ apply ---
+template <int F, int Ps, int Pang>
+static void launch_ifgf(ifgf_plan* P, const zd* x, zd* y, hipStream_t st) {
+  constexpr int Pn = Ps * Pang;
+  constexpr int CPB = 256 / Pn;
+  const Tree& t = P->tree;
+  const int N = (int)t.N, D = t.D;
+  const double k = P->prm.k, eta = P->prm.eta;
+  const int nb = (N + 255) / 256;
+  const bool prof = P->profile;
+  if (prof) HIP_TRY(hipEventRecord(P->ev[0], st));
+  hipLaunchKernelGGL(k_gather, dim3(nb), dim3(256), 0, st, N, x, P->w.p, P->perm.p, P->rho.p);
+  if (prof) HIP_TRY(hipEventRecord(P->ev[1], st));
+  hipLaunchKernelGGL((k_near<F>), dim3(nb), dim3(256), 0, st, N, P->px.p, P->py.p, P->pnx.p, P->pny.p,
+                     P->patch.p, P->excl_d.p, P->leaf_of_point.p, P->leaf_begin.p, P->nb_ptr.p,
+                     P->nb_list.p, P->rho.p, P->sol.p, k, eta);
+  if (prof) HIP_TRY(hipEventRecord(P->ev[2], st));
+  const LevelInfo& LD = t.lv[D];
+  hipLaunchKernelGGL((k_leaf<F, Ps, Pang>), dim3((LD.ncone + CPB - 1) / CPB), dim3(256), 0, st,
+                     LD.ncone, LD.cone_off, P->cg.p, P->cone_box.p, P->leaf_begin.p, P->px.p, P->py.p,
+                     P->pnx.p, P->pny.p, P->rho.p, P->coef.p, P->tabs.p, k, eta);
+  if (prof) HIP_TRY(hipEventRecord(P->ev[3], st));
+  for (int l = D; l >= 3; l--) {
+    const LevelInfo& L = t.lv[l];
+    const LevelInfo& Pl = t.lv[l - 1];
+    hipLaunchKernelGGL((k_prop<Ps, Pang>), dim3((Pl.ncone + CPB - 1) / CPB), dim3(256), 0, st, Pl.ncone,
+                       Pl.cone_off, Pl.box_off, Pl.nrad, L.cone_off, L.nslot, P->cg.p, P->cone_box.p,
+                       P->cone_qd.p, P->box_child.p, P->proto.p + L.proto_off, P->dense_map.p + L.map_off,
+                       P->coef.p, P->tabs.p, k);
+  }
+  if (prof) HIP_TRY(hipEventRecord(P->ev[4], st));
+  hipLaunchKernelGGL((k_interp<Ps, Pang>), dim3(nb), dim3(256), 0, st, N, P->px.p, P->py.p, P->it_ptr.p,
+                     P->it_cone.p, P->cg.p, P->coef.p, P->sol.p, k);
+  if (prof) HIP_TRY(hipEventRecord(P->ev[5], st));
+  const double ident = (F == kSingleLayer) ? 0.0 : 0.5;
+  hipLaunchKernelGGL(k_finalize, dim3(nb), dim3(256), 0, st, N, P->geo.Np, ident, x, P->corr_ptr.p,
+                     P->corr_patch.p, P->corr_w.p, P->inv.p, P->sol.p, y);
+  if (prof) HIP_TRY(hipEventRecord(P->ev[6], st));
+  HIP_TRY(hipGetLastError());
+}
+
+template <int F>
+static bool dispatch_p(ifgf_plan* P, const zd* x, zd* y, hipStream_t st) {
+  const int a = P->prm.Ps, b = P->prm.Pang;
+#define IFGF_CASE(A, B)                       \
+  if (a == A && b == B) {                     \
+    launch_ifgf<F, A, B>(P, x, y, st);        \
+    return true;                              \
+  }
+  IFGF_CASE(5, 5)
+  IFGF_CASE(3, 5)
+  IFGF_CASE(5, 7)
+  IFGF_CASE(7, 7)
+  IFGF_CASE(9, 9)
+  IFGF_CASE(11, 11)
+#undef IFGF_CASE
+  return false;
+}
+
+static bool dispatch(ifgf_plan* P, const zd* x, zd* y, hipStream_t st) {
+  switch (P->prm.formulation) {
+    case kSingleLayer: return dispatch_p<kSingleLayer>(P, x, y, st);
+    case kDoubleLayer: return dispatch_p<kDoubleLayer>(P, x, y, st);
+    default: return dispatch_p<kCombined>(P, x, y, st);
+  }
+}
+
+static void collect_profile(ifgf_plan* P) {
+  HIP_TRY(hipEventSynchronize(P->ev[6]));
+  for (int s = 0; s < 6; s++) {
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, P->ev[s], P->ev[s + 1]));
+    P->stage_ms[s] += ms;
+  }
+  P->prof_count++;
+}
+
+// one device-resident application on stream st (used by ifgf_apply and GMRES)
+int ifgf::apply_device(ifgf_plan* P, const zd* x, zd* y, hipStream_t st) {
+  if (!dispatch(P, x, y, st)) return fail(P, IFGF_ERR_ARG, "no kernel instantiation for this (Ps, Pang)");
+  if (P->profile) collect_profile(P);
+  return IFGF_OK;
+}
+
+extern "C" {
+
+int ifgf_apply(ifgf_plan* P, const double* x, double* y, int nrhs, long long ld, int flags, void* stream) {
+  if (!P || !x || !y) return fail(P, IFGF_ERR_ARG, "null argument");
+  if (!P->on_device) return fail(P, IFGF_ERR_NO_DEVICE, "plan was built host-only");
+  const long long N = P->geo.N;
+  if (nrhs < 1 || ld < N) return fail(P, IFGF_ERR_ARG, "nrhs < 1 or ld < n");
+  try {
+    hipStream_t st = stream ? (hipStream_t)stream : P->stream;
+    for (int r = 0; r < nrhs; r++) {
+      const zd* xr = (const zd*)x + (size_t)r * ld;
+      zd* yr = (zd*)y + (size_t)r * ld;
+      if (flags & IFGF_PTR_DEVICE) {
+        int rc = apply_device(P, xr, yr, st);
+        if (rc) return rc;
+      } else {
+        HIP_TRY(hipMemcpyAsync(P->xbuf.p, xr, N * sizeof(zd), hipMemcpyHostToDevice, st));
+        int rc = apply_device(P, P->xbuf.p, P->ybuf.p, st);
+        if (rc) return rc;
+        HIP_TRY(hipMemcpyAsync(yr, P->ybuf.p, N * sizeof(zd), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+      }
+    }
+  } catch (std::exception& e) {
+    return fail(P, IFGF_ERR_HIP, e.what());
+  }
+  return IFGF_OK;
+}
+
+int ifgf_apply_direct(ifgf_plan* P, const double* x, const long long* targets, long long nt, double* out) {
+  if (!P || !x || !targets || !out) return fail(P, IFGF_ERR_ARG, "null argument");
+  if (!P->on_device) return fail(P, IFGF_ERR_NO_DEVICE, "plan was built host-only");
+  const long long N = P->geo.N;
+  const int Np = P->geo.Np;
+  try {
+    std::vector<int> tg(nt), ex((size_t)nt * 4);
+    for (long long t = 0; t < nt; t++) {
+      if (targets[t] < 0 || targets[t] >= N) return fail(P, IFGF_ERR_ARG, "target index out of range");
+      tg[t] = (int)targets[t];
+      const int s = P->tree.inv[tg[t]];
+      for (int q = 0; q < 4; q++) ex[t * 4 + q] = P->excl[(size_t)s * 4 + q];
+    }
+    std::vector<zd> rho(N);
+    const zd* xv = (const zd*)x;
+    for (long long i = 0; i < N; i++) rho[i] = cscale(xv[i], P->geo.w[i]);
+    DevBuf<int> dt, de;
+    DevBuf<double> ox, oy, onx, ony;
+    DevBuf<zd> dr, dout;
+    dt.upload(tg);
+    de.upload(ex);
+    ox.upload(P->geo.x);
+    oy.upload(P->geo.y);
+    onx.upload(P->geo.nx);
+    ony.upload(P->geo.ny);
+    dr.upload(rho);
+    dout.alloc(nt);
+    const int nb = (int)((nt + 255) / 256);
+    const double k = P->prm.k, eta = P->prm.eta;
+    switch (P->prm.formulation) {
+      case kSingleLayer:
+        hipLaunchKernelGGL(k_direct<kSingleLayer>, dim3(nb), dim3(256), 0, P->stream, (int)nt, dt.p, de.p,
+                           (int)N, Np, ox.p, oy.p, onx.p, ony.p, dr.p, dout.p, k, eta);
+        break;
+      case kDoubleLayer:
+        hipLaunchKernelGGL(k_direct<kDoubleLayer>, dim3(nb), dim3(256), 0, P->stream, (int)nt, dt.p, de.p,
+                           (int)N, Np, ox.p, oy.p, onx.p, ony.p, dr.p, dout.p, k, eta);
+        break;
+      default:
+        hipLaunchKernelGGL(k_direct<kCombined>, dim3(nb), dim3(256), 0, P->stream, (int)nt, dt.p, de.p,
+                           (int)N, Np, ox.p, oy.p, onx.p, ony.p, dr.p, dout.p, k, eta);
+    }
+    HIP_TRY(hipGetLastError());
+    std::vector<zd> res(nt);
+    HIP_TRY(hipMemcpyAsync(res.data(), dout.p, nt * sizeof(zd), hipMemcpyDeviceToHost, P->stream));
+    HIP_TRY(hipStreamSynchronize(P->stream));
+    const double ident = P->prm.formulation == kSingleLayer ? 0.0 : 0.5;
+    zd* o = (zd*)out;
+    for (long long t = 0; t < nt; t++) {
+      const long long i = tg[t];
+      zd acc = cscale(xv[i], ident);
+      for (long long e = P->corr.ptr[i]; e < P->corr.ptr[i + 1]; e++)
+        for (int q = 0; q < Np; q++)
+          cfma(acc, P->corr.w[(size_t)e * Np + q], xv[(size_t)P->corr.patch[e] * Np + q]);
+      o[t] = cadd(acc, res[t]);
+    }
+  } catch (std::exception& e) {
+    return fail(P, IFGF_ERR_HIP, e.what());
+  }
+  return IFGF_OK;
+}
+
+int ifgf_propagate(ifgf_plan* P, const double* tx, const double* ty, long long nt, const double* dens,
+                   double* out) {
+  if (!P || !tx || !ty || !dens || !out) return fail(P, IFGF_ERR_ARG, "null argument");
+  if (!P->on_device) return fail(P, IFGF_ERR_NO_DEVICE, "plan was built host-only");
+  const long long N = P->geo.N;
+  try {
+    std::vector<zd> rho(N);
+    const zd* dv = (const zd*)dens;
+    for (long long i = 0; i < N; i++) rho[i] = cscale(dv[i], P->geo.w[i]);
+    DevBuf<double> dx, dy, ox, oy, onx, ony;
+    DevBuf<zd> dr, dout;
+    dx.upload(std::vector<double>(tx, tx + nt));
+    dy.upload(std::vector<double>(ty, ty + nt));
+    ox.upload(P->geo.x);
+    oy.upload(P->geo.y);
+    onx.upload(P->geo.nx);
+    ony.upload(P->geo.ny);
+    dr.upload(rho);
+    dout.alloc(nt);
+    const int nb = (int)((nt + 255) / 256);
+    const double k = P->prm.k, eta = P->prm.eta;
+    switch (P->prm.formulation) {
+      case kSingleLayer:
+        hipLaunchKernelGGL(k_propagate<kSingleLayer>, dim3(nb), dim3(256), 0, P->stream, (int)nt, dx.p, dy.p,
+                           (int)N, ox.p, oy.p, onx.p, ony.p, dr.p, dout.p, k, eta);
+        break;
+      case kDoubleLayer:
+        hipLaunchKernelGGL(k_propagate<kDoubleLayer>, dim3(nb), dim3(256), 0, P->stream, (int)nt, dx.p, dy.p,
+                           (int)N, ox.p, oy.p, onx.p, ony.p, dr.p, dout.p, k, eta);
+        break;
+      default:
+        hipLaunchKernelGGL(k_propagate<kCombined>, dim3(nb), dim3(256), 0, P->stream, (int)nt, dx.p, dy.p,
+                           (int)N, ox.p, oy.p, onx.p, ony.p, dr.p, dout.p, k, eta);
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(out, dout.p, nt * sizeof(zd), hipMemcpyDeviceToHost, P->stream));
+    HIP_TRY(hipStreamSynchronize(P->stream));
+  } catch (std::exception& e) {
+    return fail(P, IFGF_ERR_HIP, e.what());
+  }
+  return IFGF_OK;
+}
+
+int ifgf_hankel01(const double* x, long long n, double* h0, double* h1) {
+  if (!x || !h0 || !h1 || n < 0) return fail(nullptr, IFGF_ERR_ARG, "null argument");
+  try {
+    DevBuf<double> dx;
+    DevBuf<zd> d0, d1;
+    dx.upload(std::vector<double>(x, x + n));
+    d0.alloc(n);
+    d1.alloc(n);
+    hipLaunchKernelGGL(k_hankel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, n, dx.p, d0.p, d1.p);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpy(h0, d0.p, n * sizeof(zd), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(h1, d1.p, n * sizeof(zd), hipMemcpyDeviceToHost));
+  } catch (std::exception& e) {
+    return fail(nullptr, IFGF_ERR_HIP, e.what());
+  }
+  return IFGF_OK;
+}
+
+int ifgf_plan_profile(ifgf_plan* P, int enable) {
+  if (!P) return fail(nullptr, IFGF_ERR_ARG, "null plan");
+  P->profile = enable != 0;
+  for (double& v : P->stage_ms) v = 0.0;
+  P->prof_count = 0;
+  return IFGF_OK;
+}
+
+int ifgf_plan_stage_times(const ifgf_plan* P, double times_ms[6], long long* count) {
+  if (!P || !times_ms) return fail(nullptr, IFGF_ERR_ARG, "null argument");
+  for (int s = 0; s < 6; s++) times_ms[s] = P->stage_ms[s];
+  if (count) *count = P->prof_count;
+  return IFGF_OK;
+}
+
+int ifgf_gmres_host(ifgf_apply_fn apply, void* ctx, long long n, const double* rhs, double* x, int restart,
+                    double tol, int maxit, int* iters, double* err) {
+  if (!apply || !rhs || !x || n < 1 || restart < 1 || !iters || !err)
+    return fail(nullptr, IFGF_ERR_ARG, "invalid argument");
+  const bool ok = householder_gmres(
+      [&](const std::complex<double>* a, std::complex<double>* b) { apply(ctx, (const double*)a, (double*)b); },
+      n, (const std::complex<double>*)rhs, (std::complex<double>*)x, restart, tol, maxit, *iters, *err);
+  return ok ? IFGF_OK : IFGF_ERR_NOT_CONVERGED;
+}
+
+}  // extern "C"

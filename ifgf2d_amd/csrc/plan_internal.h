@@ -1,0 +1,96 @@
+// Internal definition of the plan object shared by plan.hip and gmres.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/ifgf2d.h"
+#include "tables.h"
+#include "zmath.h"
+#include "setup.h"
+
+namespace ifgf {
+
+#define HIP_TRY(expr)                                                              \
+  do {                                                                             \
+    hipError_t _e = (expr);                                                        \
+    if (_e != hipSuccess) {                                                        \
+      throw std::runtime_error(std::string("HIP error: ") + hipGetErrorString(_e) + \
+                               " at " #expr);                                     \
+    }                                                                              \
+  } while (0)
+
+
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  void alloc(size_t cnt) {
+    free();
+    n = cnt;
+    if (cnt) HIP_TRY(hipMalloc((void**)&p, cnt * sizeof(T)));
+  }
+  void upload(const std::vector<T>& v) {
+    alloc(v.size());
+    if (!v.empty()) HIP_TRY(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  }
+  void free() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  ~DevBuf() { free(); }
+};
+
+
+}  // namespace ifgf
+
+struct ifgf_plan {
+  using zd = ifgf::zd;
+  std::string err;
+  ifgf_params prm{};
+  ifgf::Curve curve;
+  ifgf::Geometry geo;
+  ifgf::Correction corr;
+  ifgf::Tree tree;
+  std::vector<int> patch_sorted, excl;
+  long long n_near = 0;
+  std::vector<ifgf::ConeGeo> cone_geo;
+  std::vector<int> cone_q;
+  ifgf::ChebTables cheb{};
+  bool on_device = false;
+  hipStream_t stream = nullptr;
+  // device tables
+  ifgf::DevBuf<double> px, py, pnx, pny, w;
+  ifgf::DevBuf<int> patch, perm, inv, leaf_of_point, leaf_begin, nb_ptr, nb_list;
+  ifgf::DevBuf<int4> excl_d;
+  ifgf::DevBuf<ifgf::ConeGeo> cg;
+  ifgf::DevBuf<int> cone_box, cone_qd, box_child, dense_map, proto, it_cone, corr_patch;
+  ifgf::DevBuf<long long> it_ptr, corr_ptr;
+  ifgf::DevBuf<zd> corr_w;
+  ifgf::DevBuf<ifgf::ChebTables> tabs;
+  // workspaces
+  ifgf::DevBuf<zd> rho, sol, coef, xbuf, ybuf;
+  // profiling
+  bool profile = false;
+  hipEvent_t ev[7] = {};
+  double stage_ms[6] = {0, 0, 0, 0, 0, 0};
+  long long prof_count = 0;
+  ~ifgf_plan() {
+    for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace ifgf {
+
+
+// one device-resident application on stream st (plan.hip)
+int apply_device(ifgf_plan* P, const zd* x, zd* y, hipStream_t st);
+int plan_fail(ifgf_plan* p, int code, const std::string& msg);
+
+}  // namespace ifgf

@@ -1,0 +1,812 @@
+// Host setup of the IFGF forward map (geometry, RP correction, tree tables).
+// See setup.h for the data layout. Reference citations are file:line in
+// manuelarturosantana/IFGF2D.
+#include "setup.h"
+
+#include <omp.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <numeric>
+#include <unordered_map>
+
+#include "green.h"
+
+namespace ifgf {
+
+// ---------------------------------------------------------------- curves ---
+double Curve::x(double t) const {
+  if (type == 0) return R * std::cos(t) + cx;
+  if (type == 1) return A * std::cos(t) + B * std::cos(2.0 * t) + C;
+  double r = 1.0;
+  for (size_t m = 0; m < am.size(); m++) r += am[m] * std::cos(m * t) + bm[m] * std::sin(m * t);
+  return cx + R * r * std::cos(t);
+}
+double Curve::y(double t) const {
+  if (type == 0) return R * std::sin(t) + cy;
+  if (type == 1) return D * std::sin(t);
+  double r = 1.0;
+  for (size_t m = 0; m < am.size(); m++) r += am[m] * std::cos(m * t) + bm[m] * std::sin(m * t);
+  return cy + R * r * std::sin(t);
+}
+static void star_rrp(const Curve& c, double t, double& r, double& rp) {
+  double a = 1.0, d = 0.0;
+  for (size_t m = 0; m < c.am.size(); m++) {
+    a += c.am[m] * std::cos(m * t) + c.bm[m] * std::sin(m * t);
+    d += -c.am[m] * m * std::sin(m * t) + c.bm[m] * m * std::cos(m * t);
+  }
+  r = c.R * a;
+  rp = c.R * d;
+}
+double Curve::xp(double t) const {
+  if (type == 0) return -R * std::sin(t);
+  if (type == 1) return -A * std::sin(t) - 2.0 * B * std::sin(2.0 * t);
+  double r, rp;
+  star_rrp(*this, t, r, rp);
+  return rp * std::cos(t) - r * std::sin(t);
+}
+double Curve::yp(double t) const {
+  if (type == 0) return R * std::cos(t);
+  if (type == 1) return D * std::cos(t);
+  double r, rp;
+  star_rrp(*this, t, r, rp);
+  return rp * std::sin(t) + r * std::cos(t);
+}
+
+void Curve::diff(double a, double d, double& dx, double& dy) const {
+  const double mid = a - 0.5 * d;
+  const double s1 = std::sin(0.5 * d);
+  if (type == 0) {
+    dx = -2.0 * R * std::sin(mid) * s1;
+    dy = 2.0 * R * std::cos(mid) * s1;
+    return;
+  }
+  if (type == 1) {
+    dx = -2.0 * A * std::sin(mid) * s1 - 2.0 * B * std::sin(2.0 * mid) * std::sin(d);
+    dy = 2.0 * D * std::cos(mid) * s1;
+    return;
+  }
+  // star: x = cx + r(t) cos t, r(a) - r(a - d) by the same identities
+  double ra = 1.0, drr = 0.0;
+  for (size_t m = 0; m < am.size(); m++) {
+    ra += am[m] * std::cos(m * a) + bm[m] * std::sin(m * a);
+    const double sm = std::sin(0.5 * m * d);
+    drr += -2.0 * am[m] * std::sin(m * mid) * sm + 2.0 * bm[m] * std::cos(m * mid) * sm;
+  }
+  ra *= R;
+  drr *= R;
+  const double b = a - d;
+  dx = ra * (-2.0 * std::sin(mid) * s1) + drr * std::cos(b);
+  dy = ra * (2.0 * std::cos(mid) * s1) + drr * std::sin(b);
+}
+
+bool make_curve(int type, const double* prm, int nprm, Curve& c, std::string& err) {
+  c = Curve();
+  c.type = type;
+  if (type == 0 && nprm >= 3) {
+    c.cx = prm[0];
+    c.cy = prm[1];
+    c.R = prm[2];
+    return true;
+  }
+  if (type == 1 && nprm >= 4) {
+    c.A = prm[0];
+    c.B = prm[1];
+    c.C = prm[2];
+    c.D = prm[3];
+    return true;
+  }
+  if (type == 2 && nprm >= 4) {
+    c.cx = prm[0];
+    c.cy = prm[1];
+    c.R = prm[2];
+    const int nm = (int)prm[3];
+    if (nm < 0 || nprm < 4 + 2 * nm) {
+      err = "star curve: parameter list shorter than 4 + 2*nmodes";
+      return false;
+    }
+    c.am.assign(prm + 4, prm + 4 + nm);
+    c.bm.assign(prm + 4 + nm, prm + 4 + 2 * nm);
+    return true;
+  }
+  err = "unknown curve type or too few parameters";
+  return false;
+}
+
+// ------------------------------------------------------------ quadrature ---
+// Fejer's first rule, utils/Quadrature.hpp:25-43.
+static void fejer(int n, std::vector<double>& x, std::vector<double>& w) {
+  x.resize(n);
+  w.resize(n);
+  for (int i = 0; i < n; i++) {
+    x[i] = std::cos(M_PI * (2.0 * i + 1.0) / (2.0 * n));
+    double s = 0.0;
+    for (int j = 1; j <= n / 2; j++) s += std::cos(j * M_PI * (2.0 * i + 1.0) / n) / (4.0 * j * j - 1.0);
+    w[i] = (2.0 / n) * (1.0 - 2.0 * s);
+  }
+}
+
+// RP change of variables w(t) and w'(t), utils/Quadrature.hpp:73-102.
+static double vfun(double t, double p) {
+  const double u = t / M_PI - 1.0;
+  return (0.5 - 1.0 / p) * u * u * u + u / p + 0.5;
+}
+static double vpfun(double t, double p) {
+  const double u = t / M_PI - 1.0;
+  return (3.0 / M_PI) * (0.5 - 1.0 / p) * u * u + 1.0 / (p * M_PI);
+}
+static double wcov(double t, double p) {
+  const double a = std::pow(vfun(t, p), p), b = std::pow(vfun(2 * M_PI - t, p), p);
+  return 2.0 * M_PI * a / (a + b);
+}
+static double wpcov(double t, double p) {
+  const double v1 = vfun(t, p), v2 = vfun(2 * M_PI - t, p);
+  const double a = std::pow(v1, p), b = std::pow(v2, p);
+  return 2.0 * M_PI * p * vpfun(t, p) * std::pow(v1, p - 1.0) * std::pow(v2, p - 1.0) / ((a + b) * (a + b));
+}
+static int sgnd(double v) { return (0.0 < v) - (v < 0.0); }
+
+// xi_cov_{center,left,right} and derivatives, utils/Quadrature.hpp:118-147.
+// off = xi - xi_s, the offset from the singular location xi_s (alpha, -1 or
+// +1), returned without cancellation.
+static void rp_map(int loc, double t, double alpha, double p, double& xi, double& dxi, double& off) {
+  if (loc == 0) {
+    off = ((sgnd(t) - alpha) / M_PI) * wcov(M_PI * std::abs(t), p);
+    xi = alpha + off;
+    dxi = (1.0 - alpha * sgnd(t)) * wpcov(M_PI * std::abs(t), p);
+  } else if (loc == -1) {
+    const double u = (t + 1.0) / 2.0;
+    off = (2.0 / M_PI) * wcov(M_PI * std::abs(u), p);
+    xi = -1.0 + off;
+    dxi = wpcov(M_PI * std::abs(u), p) * sgnd(u);
+  } else {
+    const double u = (t - 1.0) / 2.0;
+    off = -((2.0 / M_PI) * wcov(M_PI * std::abs(u), p));
+    xi = 1.0 + off;
+    dxi = -wpcov(M_PI * std::abs(u), p) * sgnd(u);
+  }
+}
+
+// -------------------------------------------------------------- geometry ---
+bool build_geometry(const Curve& c, const double* lims, int npatches, int Np, double delta,
+                    Geometry& g, std::string& err) {
+  if (npatches < 3 || Np < 2 || Np > 64) {
+    err = "need at least 3 patches and 2 <= Np <= 64";
+    return false;
+  }
+  g = Geometry();
+  g.Np = Np;
+  g.npatches = npatches;
+  g.N = (long long)npatches * Np;
+  fejer(Np, g.fejer_nodes, g.fejer_weights);
+  g.t1.resize(npatches);
+  g.t2.resize(npatches);
+  for (auto* v : {&g.x, &g.y, &g.nx, &g.ny, &g.w, &g.t}) v->resize(g.N);
+  // ForwardMap.tpp:75-157: node t = ab2cd(fejer_j, -1, 1, t1, t2), normal (y', -x'),
+  // intensity factor (t2 - t1)/2 * w_j * |x'(t)| (compute_intensities, :425-435).
+  for (int p = 0; p < npatches; p++) {
+    const double a = lims[p], b = lims[p + 1];
+    if (!(b > a)) {
+      err = "patch limits must be increasing";
+      return false;
+    }
+    g.t1[p] = a;
+    g.t2[p] = b;
+    for (int j = 0; j < Np; j++) {
+      const long long i = (long long)p * Np + j;
+      const double t = ((b - a) / 2.0) * (g.fejer_nodes[j] + 1.0) + a;
+      const double n0 = c.yp(t), n1 = -c.xp(t);
+      const double nn = std::sqrt(n0 * n0 + n1 * n1);
+      g.t[i] = t;
+      g.x[i] = c.x(t);
+      g.y[i] = c.y(t);
+      g.nx[i] = n0 / nn;
+      g.ny[i] = n1 / nn;
+      g.w[i] = ((b - a) / 2.0) * g.fejer_weights[j] * nn;
+    }
+  }
+  // Near-singular points: points of the two adjacent patches closer than delta
+  // to the patch (determine_patch_near_singular_points, ForwardMap.tpp:224-285,
+  // with the neighbour estimate of :164-218 for one closed curve). The
+  // closest parameter is t1 if |x(t1) - x_q| < delta, else t2 if that holds,
+  // else the interior minimiser of the distance (golden-section search).
+  g.ns_idx.assign(npatches, {});
+  g.ns_t.assign(npatches, {});
+#pragma omp parallel for schedule(static)
+  for (int p = 0; p < npatches; p++) {
+    const double a = g.t1[p], b = g.t2[p];
+    const int est[2] = {(p - 1 + npatches) % npatches, (p + 1) % npatches};
+    for (int e = 0; e < 2; e++) {
+      for (int j = 0; j < Np; j++) {
+        const long long q = (long long)est[e] * Np + j;
+        const double xq = g.x[q], yq = g.y[q];
+        auto dist = [&](double t) {
+          const double dx = c.x(t) - xq, dy = c.y(t) - yq;
+          return std::sqrt(std::abs(dx * dx + dy * dy));
+        };
+        double ts;
+        if (dist(a) < delta) {
+          ts = a;
+        } else if (dist(b) < delta) {
+          ts = b;
+        } else {
+          const double gr = 0.5 * (std::sqrt(5.0) - 1.0);
+          double lo = a, hi = b, c1 = hi - gr * (hi - lo), c2 = lo + gr * (hi - lo);
+          double f1 = dist(c1), f2 = dist(c2);
+          for (int it = 0; it < 200 && (hi - lo) > 1e-12 * (1.0 + std::abs(lo)); it++) {
+            if (f1 < f2) {
+              hi = c2;
+              c2 = c1;
+              f2 = f1;
+              c1 = hi - gr * (hi - lo);
+              f1 = dist(c1);
+            } else {
+              lo = c1;
+              c1 = c2;
+              f1 = f2;
+              c2 = lo + gr * (hi - lo);
+              f2 = dist(c2);
+            }
+          }
+          ts = 0.5 * (lo + hi);
+          if (!(dist(ts) < delta)) continue;
+        }
+        g.ns_idx[p].push_back(q);
+        g.ns_t[p].push_back(ts);
+      }
+    }
+  }
+  return true;
+}
+
+// ------------------------------------------------------------ correction ---
+// One column of the precomputation matrix: Np Chebyshev moments of the kernel
+// against the patch, integrated with the 40-node RP rule around tsing
+// (single_patch_point_compute_precomputations, ForwardMap.tpp:287-389).
+// The target is the curve point at parameter ttgt; the separation x - y is
+// formed from parameter differences (Curve::diff), so the double-layer
+// numerator (x - y).n_y keeps full relative accuracy as y -> x.
+static void rp_column(const Curve& c, int F, double k, double eta, double p, double t1, double t2,
+                      double tsing, double ttgt, int Np, const std::vector<double>& fn,
+                      const std::vector<double>& fw, zd* out) {
+  const double alpha = ((1.0 - -1.0) / (t2 - t1)) * (tsing - t1) + -1.0;
+  int loc = 0;
+  if (std::abs(-1.0 - alpha) < 1e-5) loc = -1;
+  else if (std::abs(1.0 - alpha) < 1e-5) loc = 1;
+  for (int n = 0; n < Np; n++) out[n] = zd{0.0, 0.0};
+  const int nq = (int)fn.size();
+  // parameter of the singular location the change of variables is centred on
+  const double ts = loc == 0 ? tsing : (loc < 0 ? t1 : t2);
+  double tt = ttgt - ts;  // target offset from it (wrapped to (-pi, pi])
+  if (tt > M_PI) tt -= 2.0 * M_PI;
+  if (tt < -M_PI) tt += 2.0 * M_PI;
+  for (int i = 0; i < nq; i++) {
+    double xi, dxi, off;
+    rp_map(loc, fn[i], alpha, p, xi, dxi, off);
+    const double g = ((t2 - t1) / 2.0) * (xi + 1.0) + t1;
+    const double n0 = c.yp(g), n1 = -c.xp(g);
+    const double cj = std::sqrt(n0 * n0 + n1 * n1);
+    double dx, dy;
+    c.diff(ttgt, tt - ((t2 - t1) / 2.0) * off, dx, dy);
+    const zd gr = green_d_rt(F, dx, dy, n0 / cj, n1 / cj, k, eta);
+    const double s = ((t2 - t1) / 2.0) * cj * dxi * fw[i];
+    const zd v{gr.x * s, gr.y * s};
+    // Chebyshev moments T_n(xi), n < Np (ForwardMap.tpp:373-385)
+    double tm = 1.0, tc = xi;
+    out[0] = cadd(out[0], v);
+    out[1] = cadd(out[1], cscale(v, tc));
+    for (int n = 2; n < Np; n++) {
+      const double tn = 2.0 * xi * tc - tm;
+      out[n] = cadd(out[n], cscale(v, tn));
+      tm = tc;
+      tc = tn;
+    }
+  }
+}
+
+bool build_correction(const Geometry& g, const Curve& c, int F, double k, double eta, double p,
+                      Correction& out, std::string& err) {
+  const int Np = g.Np;
+  std::vector<double> fn, fw;
+  fejer(40, fn, fw);  // num_ns = 40 (ForwardMap.hpp:70)
+  // Chebyshev-coefficient map of Cheb1D::interp_1d<Np> (Chebyshev1d.hpp:134-163):
+  // c_n = (alpha_n / Np) sum_k rho_k T_n(x_k), x_k the Fejer nodes, alpha_0 = 1 else 2.
+  std::vector<double> T(Np * Np);
+  for (int kk = 0; kk < Np; kk++) {
+    const double x = g.fejer_nodes[kk];
+    T[0 * Np + kk] = 1.0;
+    T[1 * Np + kk] = x;
+    for (int n = 2; n < Np; n++) T[n * Np + kk] = 2.0 * x * T[(n - 1) * Np + kk] - T[(n - 2) * Np + kk];
+  }
+  // folded rows W_p[col][k] = sum_n (alpha_n/Np) T_n(x_k) M_p[n][col]
+  std::vector<std::vector<zd>> Wp(g.npatches);
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int pi = 0; pi < g.npatches; pi++) {
+    const int ncol = Np + (int)g.ns_idx[pi].size();
+    std::vector<zd> M((size_t)ncol * Np);
+    for (int j = 0; j < Np; j++) {
+      const long long i = (long long)pi * Np + j;
+      rp_column(c, F, k, eta, p, g.t1[pi], g.t2[pi], g.t[i], g.t[i], Np, fn, fw, &M[(size_t)j * Np]);
+    }
+    for (size_t j = 0; j < g.ns_idx[pi].size(); j++) {
+      const long long q = g.ns_idx[pi][j];
+      rp_column(c, F, k, eta, p, g.t1[pi], g.t2[pi], g.ns_t[pi][j], g.t[q], Np, fn, fw,
+                &M[(size_t)(Np + j) * Np]);
+    }
+    std::vector<zd>& W = Wp[pi];
+    W.assign((size_t)ncol * Np, zd{0.0, 0.0});
+    for (int col = 0; col < ncol; col++)
+      for (int kk = 0; kk < Np; kk++) {
+        zd s{0.0, 0.0};
+        for (int n = 0; n < Np; n++) {
+          const double a = (n == 0 ? 1.0 : 2.0) / Np * T[n * Np + kk];
+          s = cadd(s, cscale(M[(size_t)col * Np + n], a));
+        }
+        W[(size_t)col * Np + kk] = s;
+      }
+  }
+  // target-major CSR: own patch first, then near-singular patches in patch order
+  std::vector<std::vector<std::pair<int, int>>> rows(g.N);  // (patch, column)
+  for (long long i = 0; i < g.N; i++) rows[i].push_back({(int)(i / Np), (int)(i % Np)});
+  for (int pi = 0; pi < g.npatches; pi++)
+    for (size_t j = 0; j < g.ns_idx[pi].size(); j++) rows[g.ns_idx[pi][j]].push_back({pi, Np + (int)j});
+  out.ptr.assign(g.N + 1, 0);
+  for (long long i = 0; i < g.N; i++) out.ptr[i + 1] = out.ptr[i] + (long long)rows[i].size();
+  out.patch.resize(out.ptr[g.N]);
+  out.w.resize((size_t)out.ptr[g.N] * Np);
+  for (long long i = 0; i < g.N; i++)
+    for (size_t e = 0; e < rows[i].size(); e++) {
+      const long long ee = out.ptr[i] + (long long)e;
+      out.patch[ee] = rows[i][e].first;
+      const auto& W = Wp[rows[i][e].first];
+      std::copy(W.begin() + (size_t)rows[i][e].second * Np, W.begin() + (size_t)(rows[i][e].second + 1) * Np,
+                out.w.begin() + (size_t)ee * Np);
+    }
+  (void)err;
+  return true;
+}
+
+// ------------------------------------------------------------------ tree ---
+// Morton interleave, IFGF_Source/Level.h:172-237.
+static uint64_t split_by2(uint64_t v, int level) {
+  uint64_t x = v & ((uint64_t(1) << (level + 1)) - 1);
+  x = (x | (x << 16)) & 0xFFFF0000FFFFull;
+  x = (x | (x << 8)) & 0xFF00FF00FF00FFull;
+  x = (x | (x << 4)) & 0xF0F0F0F0F0F0F0Full;
+  x = (x | (x << 2)) & 0x3333333333333333ull;
+  x = (x | (x << 1)) & 0x5555555555555555ull;
+  return x;
+}
+static uint64_t merge_from2(uint64_t x) {
+  x &= 0x5555555555555555ull;
+  x = (x ^ (x >> 1)) & 0x3333333333333333ull;
+  x = (x ^ (x >> 2)) & 0xF0F0F0F0F0F0F0Full;
+  x = (x ^ (x >> 4)) & 0xFF00FF00FF00FFull;
+  x = (x ^ (x >> 8)) & 0xFFFF0000FFFFull;
+  x = (x ^ (x >> 16)) & 0xffffffffull;
+  return x;
+}
+static long long morton(long long px, long long py, int level) {
+  return (long long)(split_by2((uint64_t)px, level) | (split_by2((uint64_t)py, level) << 1));
+}
+// Functions::Atan2, IFGF_Source/PolarCoordinates.h:8-27 (theta in [0, 2 pi)).
+static double atan2_02pi(double y, double x) {
+  if (x > 0 && y >= 0) return std::atan(y / x);
+  if (x > 0 && y < 0) return std::atan(y / x) + 2.0 * M_PI;
+  if (x < 0 && y > 0) return std::atan(y / x) + M_PI;
+  if (x < 0 && y == 0) return M_PI;
+  if (x < 0 && y < 0) return std::atan(y / x) + M_PI;
+  if (x == 0 && y > 0) return M_PI_2;
+  if (x == 0 && y < 0) return 3.0 * M_PI_2;
+  return 0.0;
+}
+
+namespace {
+struct LevelGeom {
+  double min_x, min_y, bs, h;
+  long long nce, nrad, nbps;
+  std::vector<double> radial;
+  // InitializeConeSegments<Helmholtz>, Level.h:88-142
+  LevelGeom(int level, double mx, double my, double bs_, double k) : min_x(mx), min_y(my), bs(bs_) {
+    nbps = 1ll << level;
+    const double minsize = (3.0 / 2.0 * bs) * (1.0 - 1e-10);
+    const double maxsize = (std::sqrt(2.0) * bs * nbps - std::sqrt(2.0) / 2.0 * bs) * (1.0 + 1e-10);
+    long long scal = 1;
+    if (level > 1) scal = std::max<long long>(1, (long long)std::ceil(k * bs * M_1_PI * 1.2));
+    nce = scal * 2;
+    const long long nrad_on = scal;
+    h = std::sqrt(2.0) / 2.0 * bs;
+    const double eta = std::sqrt(2.0) / 3.0 - 1e-5;
+    const double tmpr = (maxsize - minsize + 1e-10);
+    const double tmps = std::min(eta / nrad_on, h * (1.0 / minsize - 1.0 / (minsize + tmpr)));
+    const long long nact = (long long)std::ceil(h * (1.0 / minsize - 1.0 / maxsize) / tmps);
+    radial.assign(nact + 1, 0.0);
+    radial[0] = minsize;
+    const double fs = (h * (1.0 / minsize - 1.0 / maxsize) / nact);
+    for (long long i = nact - 1; i >= 0; i--) radial[nact - i] = h * maxsize / (h + maxsize * i * fs);
+    nrad = nact;
+  }
+  long long point2morton(double x, double y, int level, bool& ok) const {
+    const long long px = (long long)((x - min_x) / bs), py = (long long)((y - min_y) / bs);
+    ok = !(px < 0 || py < 0 || px >= nbps || py >= nbps);
+    return ok ? morton(px, py, level) : -1;
+  }
+  void center(long long m, double& cx, double& cy) const {  // GetBoxCenter, Level.h:291-298
+    const long long px = (long long)merge_from2((uint64_t)m), py = (long long)merge_from2((uint64_t)m >> 1);
+    cx = min_x + px * bs + 0.5 * bs;
+    cy = min_y + py * bs + 0.5 * bs;
+  }
+  // Cart2Nonrelcone, Level.h:310-323 (returns -1 if outside the cone grid)
+  long long nonrel(long long m, double x, double y) const {
+    double cx, cy;
+    center(m, cx, cy);
+    x -= cx;
+    y -= cy;
+    const double r = std::sqrt(x * x + y * y);
+    if (r == 0.0) return -1;
+    const double th = atan2_02pi(y, x);
+    const long long posr = (long long)(std::upper_bound(radial.begin(), radial.end(), r) - radial.begin()) - 1;
+    const long long post = (long long)(th * nce / M_PI);
+    if (posr < 0 || posr >= nrad || post < 0 || post > 2 * nce) return -1;
+    return post * nrad + posr;
+  }
+  // Cheb2Cart, Level.h:347-375
+  void cheb2cart(long long m, long long q, double s, double t, double& x, double& y) const {
+    const long long posr = q % nrad, post = q / nrad;
+    const double mn = radial[posr];
+    const double rmin = h / radial[posr + 1];
+    const double dr = h / mn - rmin;
+    const double r = h / ((s + 1.0) / 2.0 * dr + rmin);
+    const double da = M_PI / nce;
+    const double th = (t + 1.0) / 2.0 * da + da * post;
+    double cx, cy;
+    center(m, cx, cy);
+    x = r * std::cos(th) + cx;
+    y = r * std::sin(th) + cy;
+  }
+};
+}  // namespace
+
+bool build_tree(const double* x, const double* y, const double* nx, const double* ny, long long N,
+                int nlev, double k, int Ps, int Pang, Tree& t, std::string& err) {
+  if (nlev < 3) {
+    err = "IFGF accelerator requires at least 3 levels.";
+    return false;
+  }
+  if (N < 1 || N > (1ll << 30)) {
+    err = "point count out of range";
+    return false;
+  }
+  t = Tree();
+  t.nlev = nlev;
+  t.D = nlev - 1;
+  t.N = N;
+  t.k = k;
+  t.Ps = Ps;
+  t.Pang = Pang;
+  const int D = t.D, P = Ps * Pang;
+  // Bounding box +- 1e-10 and the level-1 box size (BoxTree.h:179-231).
+  double mn0 = 1e308, mn1 = 1e308, mx0 = -1e308, mx1 = -1e308;
+  for (long long i = 0; i < N; i++) {
+    mn0 = std::min(mn0, x[i]);
+    mn1 = std::min(mn1, y[i]);
+    mx0 = std::max(mx0, x[i]);
+    mx1 = std::max(mx1, y[i]);
+  }
+  mx0 += 1e-10;
+  mx1 += 1e-10;
+  mn0 -= 1e-10;
+  mn1 -= 1e-10;
+  const double bs0 = std::max(mx0 - mn0, mx1 - mn1);
+  std::vector<LevelGeom> G;
+  G.reserve(D + 1);
+  for (int l = 0; l <= D; l++) G.emplace_back(std::max(l, 1), mn0, mn1, bs0 / (1ll << l), k);
+  // SortBox (BoxTree.h:73-106): stable sort by leaf Morton code.
+  std::vector<long long> code(N);
+  for (long long i = 0; i < N; i++) {
+    bool ok;
+    code[i] = G[D].point2morton(x[i], y[i], D, ok);
+    if (!ok) {
+      err = "The 2D box index cannot be < 0 or larger than the number of boxes";
+      return false;
+    }
+  }
+  t.perm.resize(N);
+  std::iota(t.perm.begin(), t.perm.end(), 0);
+  std::stable_sort(t.perm.begin(), t.perm.end(), [&](int a, int b) { return code[a] < code[b]; });
+  t.inv.resize(N);
+  t.sx.resize(N);
+  t.sy.resize(N);
+  t.snx.resize(N);
+  t.sny.resize(N);
+  for (long long i = 0; i < N; i++) {
+    const int o = t.perm[i];
+    t.inv[o] = (int)i;
+    t.sx[i] = x[o];
+    t.sy[i] = y[o];
+    t.snx[i] = nx[o];
+    t.sny[i] = ny[o];
+  }
+  // Relevant boxes per level (BoxTree.h:111-174): leaf boxes from the sorted
+  // codes, parents = codes / 4, each level ascending.
+  std::vector<std::vector<long long>> boxes(D + 1);
+  t.leaf_of_point.resize(N);
+  t.leaf_begin.clear();
+  for (long long i = 0; i < N; i++) {
+    const long long c = code[t.perm[i]];
+    if (boxes[D].empty() || boxes[D].back() != c) {
+      boxes[D].push_back(c);
+      t.leaf_begin.push_back((int)i);
+    }
+    t.leaf_of_point[i] = (int)boxes[D].size() - 1;
+  }
+  t.leaf_begin.push_back((int)N);
+  for (int l = D - 1; l >= 2; l--)
+    for (long long c : boxes[l + 1])
+      if (boxes[l].empty() || boxes[l].back() != c / 4) boxes[l].push_back(c / 4);
+  std::vector<std::unordered_map<long long, int>> box_index(D + 1);
+  t.lv.assign(D + 1, LevelInfo());
+  int box_off = 0;
+  for (int l = 2; l <= D; l++) {
+    for (size_t b = 0; b < boxes[l].size(); b++) box_index[l][boxes[l][b]] = (int)b;
+    LevelInfo& L = t.lv[l];
+    L.level = l;
+    L.min_x = mn0;
+    L.min_y = mn1;
+    L.bs = G[l].bs;
+    L.h = G[l].h;
+    L.nce = (int)G[l].nce;
+    L.nrad = (int)G[l].nrad;
+    L.radial_off = (int)t.radial.size();
+    t.radial.insert(t.radial.end(), G[l].radial.begin(), G[l].radial.end());
+    L.nbox = (int)boxes[l].size();
+    L.box_off = box_off;
+    box_off += L.nbox;
+    L.nslot = (int)((2 * G[l].nce + 1) * G[l].nrad);
+    for (long long m : boxes[l]) {
+      double cx, cy;
+      G[l].center(m, cx, cy);
+      t.box_morton.push_back(m);
+      t.box_cx.push_back(cx);
+      t.box_cy.push_back(cy);
+    }
+  }
+  // children slots (GetMortonRelChildren, BoxTree.h:279-297)
+  t.box_child.assign((size_t)box_off * 4, -1);
+  for (int l = 2; l < D; l++)
+    for (int b = 0; b < t.lv[l].nbox; b++)
+      for (int cp = 0; cp < 4; cp++) {
+        auto it = box_index[l + 1].find(boxes[l][b] * 4 + cp);
+        if (it != box_index[l + 1].end()) t.box_child[(size_t)(t.lv[l].box_off + b) * 4 + cp] = it->second;
+      }
+  // leaf neighbours (GetNeighbours, Level.h:378-404), ascending Morton
+  t.nb_ptr.assign(1, 0);
+  for (size_t b = 0; b < boxes[D].size(); b++) {
+    const long long bx = (long long)merge_from2((uint64_t)boxes[D][b]);
+    const long long by = (long long)merge_from2((uint64_t)boxes[D][b] >> 1);
+    std::vector<std::pair<long long, int>> nb;
+    for (long long j = std::max(0ll, by - 1); j <= std::min(G[D].nbps - 1, by + 1); j++)
+      for (long long i = std::max(0ll, bx - 1); i <= std::min(G[D].nbps - 1, bx + 1); i++) {
+        auto it = box_index[D].find(morton(i, j, D));
+        if (it != box_index[D].end()) nb.push_back({it->first, it->second});
+      }
+    std::sort(nb.begin(), nb.end());
+    for (auto& e : nb) t.nb_list.push_back(e.second);
+    t.nb_ptr.push_back((int)t.nb_list.size());
+  }
+  // cousin lists per box and level (GetCousins, Level.h:261-289), ascending Morton
+  std::vector<std::vector<std::vector<int>>> cz(D + 1);
+  for (int l = 2; l <= D; l++) {
+    cz[l].resize(boxes[l].size());
+    for (size_t b = 0; b < boxes[l].size(); b++) {
+      const long long bx = (long long)merge_from2((uint64_t)boxes[l][b]);
+      const long long by = (long long)merge_from2((uint64_t)boxes[l][b] >> 1);
+      const long long xlo = std::max<long long>((bx / 2) * 2 - 2, 0), xup = std::min<long long>(G[l].nbps - 1, (bx / 2) * 2 + 3);
+      const long long ylo = std::max<long long>((by / 2) * 2 - 2, 0), yup = std::min<long long>(G[l].nbps - 1, (by / 2) * 2 + 3);
+      std::vector<std::pair<long long, int>> c;
+      for (long long j = ylo; j <= yup; j++)
+        for (long long i = xlo; i <= xup; i++) {
+          if (std::abs(i - bx) <= 1 && std::abs(j - by) <= 1) continue;
+          auto it = box_index[l].find(morton(i, j, l));
+          if (it != box_index[l].end()) c.push_back({it->first, it->second});
+        }
+      std::sort(c.begin(), c.end());
+      for (auto& e : c) cz[l][b].push_back(e.second);
+    }
+  }
+  // per sorted point: box at each level (Point2Morton at that level)
+  std::vector<int> pbox((size_t)N * (D + 1), -1);
+  bool bad = false;
+#pragma omp parallel for schedule(static) reduction(|| : bad)
+  for (long long i = 0; i < N; i++)
+    for (int l = 2; l <= D; l++) {
+      bool ok;
+      const long long m = G[l].point2morton(t.sx[i], t.sy[i], l, ok);
+      auto it = ok ? box_index[l].find(m) : box_index[l].end();
+      if (it == box_index[l].end()) {
+        bad = true;
+        continue;
+      }
+      pbox[(size_t)i * (D + 1) + l] = it->second;
+    }
+  if (bad) {
+    err = "point outside the relevant boxes of a level";
+    return false;
+  }
+  // interpolation entries: per target, levels D..2, cousins ascending
+  // (Interpolation, BoxTree.h:876-937); first the non-relevant cone ids.
+  t.it_ptr.assign(N + 1, 0);
+  for (long long i = 0; i < N; i++) {
+    long long c = 0;
+    for (int l = D; l >= 2; l--) c += (long long)cz[l][pbox[(size_t)i * (D + 1) + l]].size();
+    t.it_ptr[i + 1] = t.it_ptr[i] + c;
+  }
+  const long long nent = t.it_ptr[N];
+  std::vector<int> ent_q(nent);
+#pragma omp parallel for schedule(dynamic, 256) reduction(|| : bad)
+  for (long long i = 0; i < N; i++) {
+    long long e = t.it_ptr[i];
+    for (int l = D; l >= 2; l--)
+      for (int c : cz[l][pbox[(size_t)i * (D + 1) + l]]) {
+        const long long q = G[l].nonrel(boxes[l][c], t.sx[i], t.sy[i]);
+        if (q < 0) bad = true;
+        ent_q[e++] = (int)q;
+      }
+  }
+  if (bad) {
+    err = "target outside the cone grid of a cousin box";
+    return false;
+  }
+  // mark relevant cones, level by level top-down, then number them (box
+  // ascending, non-relevant id ascending) -- InitializeRelevantConeSegments,
+  // BoxTree.h:553-583 with the merge of :430-459 and flattening of :461-514.
+  long long map_total = 0;
+  for (int l = 2; l <= D; l++) {
+    t.lv[l].map_off = map_total;
+    map_total += (long long)t.lv[l].nbox * t.lv[l].nslot;
+  }
+  t.dense_map.assign(map_total, -1);
+  std::vector<char> mark(map_total, 0);
+  for (long long i = 0; i < N; i++) {
+    long long e = t.it_ptr[i];
+    for (int l = D; l >= 2; l--)
+      for (int c : cz[l][pbox[(size_t)i * (D + 1) + l]]) mark[t.lv[l].map_off + (long long)c * t.lv[l].nslot + ent_q[e++]] = 1;
+  }
+  // interpolation points (Interpolator.h:37-41): s = xs[j % Ps], theta = xang[j / Ps]
+  std::vector<double> xs(Ps), xa(Pang);
+  for (int i = 0; i < Ps; i++) xs[Ps - 1 - i] = std::cos(M_PI * (2.0 * i + 1.0) / (2.0 * Ps));
+  for (int i = 0; i < Pang; i++) xa[Pang - 1 - i] = std::cos(M_PI * (2.0 * i + 1.0) / (2.0 * Pang));
+  int cone_off = 0;
+  for (int l = 2; l <= D; l++) {
+    LevelInfo& L = t.lv[l];
+    if (l > 2) {
+      // cones needed by the parents (GetRelevantConeSegmentsDueToParents,
+      // BoxTree.h:299-428): protobox[q][cp] is filled from the first parent
+      // cone with non-relevant id q that has a child at position cp.
+      const LevelInfo& Pl = t.lv[l - 1];
+      L.proto_off = (long long)t.proto.size();
+      t.proto.resize(t.proto.size() + (size_t)Pl.nslot * 4 * P, -1);
+      int* proto = &t.proto[L.proto_off];
+      for (int g = Pl.cone_off; g < Pl.cone_off + Pl.ncone; g++) {
+        const int pb = t.cone_box[g];
+        const long long q = (long long)t.cone_post[g] * Pl.nrad + t.cone_posr[g];
+        for (int cp = 0; cp < 4; cp++) {
+          const int cb = t.box_child[(size_t)(Pl.box_off + pb) * 4 + cp];
+          if (cb < 0) continue;
+          int* slot = proto + ((size_t)q * 4 + cp) * P;
+          if (slot[0] < 0) {
+            for (int j = 0; j < P; j++) {
+              double px, py;
+              G[l - 1].cheb2cart(boxes[l - 1][pb], q, xs[j % Ps], xa[j / Ps], px, py);
+              const long long cq = G[l].nonrel(boxes[l][cb], px, py);
+              if (cq < 0) {
+                err = "parent interpolation node outside the child cone grid";
+                return false;
+              }
+              slot[j] = (int)cq;
+            }
+          }
+          for (int j = 0; j < P; j++) mark[L.map_off + (long long)cb * L.nslot + slot[j]] = 1;
+          t.n_prop += P;
+        }
+      }
+    }
+    L.cone_off = cone_off;
+    int nc = 0;
+    for (int b = 0; b < L.nbox; b++)
+      for (int q = 0; q < L.nslot; q++) {
+        const long long a = L.map_off + (long long)b * L.nslot + q;
+        if (!mark[a]) continue;
+        t.dense_map[a] = nc++;
+        t.cone_box.push_back(b);
+        t.cone_posr.push_back(q % L.nrad);
+        t.cone_post.push_back(q / L.nrad);
+        t.cone_level.push_back((unsigned char)l);
+      }
+    if (nc == 0) {
+      err = "The number of cone segments on any level cannot be 0";
+      return false;
+    }
+    L.ncone = nc;
+    cone_off += nc;
+  }
+  t.n_cones = cone_off;
+  for (int g = t.lv[D].cone_off; g < t.lv[D].cone_off + t.lv[D].ncone; g++)
+    t.n_leaf += (long long)P * (t.leaf_begin[t.cone_box[g] + 1] - t.leaf_begin[t.cone_box[g]]);
+  // final interpolation entries as global cone ids
+  t.it_cone.resize(nent);
+#pragma omp parallel for schedule(static)
+  for (long long i = 0; i < N; i++) {
+    long long e = t.it_ptr[i];
+    for (int l = D; l >= 2; l--)
+      for (int c : cz[l][pbox[(size_t)i * (D + 1) + l]]) {
+        t.it_cone[e] = t.lv[l].cone_off + t.dense_map[t.lv[l].map_off + (long long)c * t.lv[l].nslot + ent_q[e]];
+        e++;
+      }
+  }
+  t.n_interp = nent;
+  return true;
+}
+
+bool build_exclusions(const Geometry& g, const Tree& t, std::vector<int>& patch_of_sorted,
+                      std::vector<int>& excl, long long& n_near, std::string& err) {
+  const long long N = g.N;
+  const int Np = g.Np;
+  std::vector<std::vector<int>> ex(N);
+  for (long long i = 0; i < N; i++) ex[i].push_back((int)(i / Np));
+  for (int p = 0; p < g.npatches; p++)
+    for (long long q : g.ns_idx[p])
+      if (std::find(ex[q].begin(), ex[q].end(), p) == ex[q].end()) ex[q].push_back(p);
+  patch_of_sorted.resize(N);
+  excl.assign((size_t)N * kMaxExcl, -1);
+  for (long long i = 0; i < N; i++) {
+    const int s = t.inv[i];
+    patch_of_sorted[s] = (int)(i / Np);
+    if ((int)ex[i].size() > kMaxExcl) {
+      err = "a point is near-singular to more patches than the near-field exclusion list holds";
+      return false;
+    }
+    for (size_t e = 0; e < ex[i].size(); e++) excl[(size_t)s * kMaxExcl + e] = ex[i][e];
+  }
+  // every excluded pair must be a leaf-level neighbour pair
+  // (CheckIfSingandNearSingInNeighborhood, BoxTree.h:1079-1161)
+  const int D = t.D;
+  auto leaf_xy = [&](long long sorted, long long& bx, long long& by) {
+    const long long m = t.box_morton[t.lv[D].box_off + t.leaf_of_point[sorted]];
+    bx = (long long)merge_from2((uint64_t)m);
+    by = (long long)merge_from2((uint64_t)m >> 1);
+  };
+  for (long long i = 0; i < N; i++) {
+    long long ax, ay;
+    leaf_xy(t.inv[i], ax, ay);
+    for (int p : ex[i])
+      for (long long j = (long long)p * Np; j < (long long)(p + 1) * Np; j++) {
+        long long bx, by;
+        leaf_xy(t.inv[j], bx, by);
+        if (std::abs(ax - bx) > 1 || std::abs(ay - by) > 1) {
+          err = "Singular or near singular point not in the leaf neighbourhood (use fewer levels)";
+          return false;
+        }
+      }
+  }
+  // near-field pair count (SingularInteractions, BoxTree.h:614-683)
+  long long cnt = 0;
+#pragma omp parallel for schedule(static) reduction(+ : cnt)
+  for (long long i = 0; i < N; i++) {
+    const int b = t.leaf_of_point[i];
+    const int* e = &excl[(size_t)i * kMaxExcl];
+    for (int n = t.nb_ptr[b]; n < t.nb_ptr[b + 1]; n++) {
+      const int nb = t.nb_list[n];
+      for (int s = t.leaf_begin[nb]; s < t.leaf_begin[nb + 1]; s++) {
+        const int ps = patch_of_sorted[s];
+        if (ps == e[0] || ps == e[1] || ps == e[2] || ps == e[3]) continue;
+        cnt++;
+      }
+    }
+  }
+  n_near = cnt;
+  return true;
+}
+
+}  // namespace ifgf

@@ -1,0 +1,105 @@
+// Host-side setup of the IFGF forward map: geometry, near-singular detection,
+// the rectangular-polar (RP) singular/near-singular correction, and the
+// flattened Morton tree / cone tables that the gfx950 kernels read.
+//
+// Every discrete decision (Morton box of a point, relevant boxes and cones,
+// the cone of every (target, cousin) pair, the child cone of every parent
+// interpolation node) is made here, once per geometry, with the reference's
+// formulas; the kernels only do arithmetic on the resulting index tables.
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "zmath.h"
+
+namespace ifgf {
+
+// Closed parametrised curve, t in [0, 2 pi) (Geometry/Curve.hpp:21-146).
+//   type 0 circle (cx, cy, R); 1 kite x = A cos t + B cos 2t + C, y = D sin t;
+//   2 star (cx, cy, R, nmodes, a[0..nmodes), b[0..nmodes)),
+//     r(t) = R (1 + sum_m a_m cos mt + b_m sin mt).
+struct Curve {
+  int type = 0;
+  double cx = 0, cy = 0, R = 1;
+  double A = 1, B = 0.65, C = -0.65, D = 1.5;
+  std::vector<double> am, bm;
+  double x(double t) const;
+  double y(double t) const;
+  double xp(double t) const;
+  double yp(double t) const;
+  // x(a) - x(a - d), y(a) - y(a - d) from trigonometric difference identities,
+  // accurate to rounding relative to the (small) difference itself.
+  void diff(double a, double d, double& dx, double& dy) const;
+};
+bool make_curve(int type, const double* prm, int nprm, Curve& c, std::string& err);
+
+// Nystrom discretisation in patch order (ForwardMap.tpp:37-222).
+struct Geometry {
+  int Np = 0, npatches = 0;
+  long long N = 0;
+  std::vector<double> t1, t2;                   // per patch
+  std::vector<double> x, y, nx, ny, w, t;       // per point; w = intensity factor
+  std::vector<double> fejer_nodes, fejer_weights;
+  // per patch: near-singular point indices and their closest parameter
+  std::vector<std::vector<long long>> ns_idx;
+  std::vector<std::vector<double>> ns_t;
+};
+bool build_geometry(const Curve& c, const double* lims, int npatches, int Np, double delta,
+                    Geometry& g, std::string& err);
+
+// Folded correction W_p = M_p^T T_Np in target-major CSR: for point i (patch
+// order), entries e in [ptr[i], ptr[i+1]) add sum_k w[e*Np+k] * rho[patch[e]*Np+k].
+struct Correction {
+  std::vector<long long> ptr;
+  std::vector<int> patch;
+  std::vector<zd> w;
+};
+bool build_correction(const Geometry& g, const Curve& c, int F, double k, double eta, double p,
+                      Correction& out, std::string& err);
+
+struct LevelInfo {
+  int level = 0;
+  double min_x = 0, min_y = 0, bs = 0, h = 0;
+  int nce = 0, nrad = 0, radial_off = 0;
+  int nbox = 0, box_off = 0;
+  int ncone = 0, cone_off = 0;
+  int nslot = 0;          // non-relevant cone slots per box: (2 nce + 1) * nrad
+  long long map_off = 0;  // dense (box, nonrel) -> local relevant cone
+  long long proto_off = -1;  // protobox of this (child) level, indexed by parent nonrel
+};
+
+struct Tree {
+  int nlev = 0, D = 0, Ps = 0, Pang = 0;
+  long long N = 0;
+  double k = 0;
+  std::vector<int> perm, inv;                  // sorted -> original, original -> sorted
+  std::vector<double> sx, sy, snx, sny;        // Morton-sorted points
+  std::vector<LevelInfo> lv;                   // indexed by level, valid for 2..D
+  std::vector<double> radial;                  // all levels' radial interval ends
+  std::vector<long long> box_morton;           // global box id (levels 2..D)
+  std::vector<double> box_cx, box_cy;
+  std::vector<int> box_child;                  // 4 per global box: local child index or -1
+  std::vector<int> leaf_begin;                 // leaf boxes: nleaf + 1 point offsets
+  std::vector<int> leaf_of_point;              // sorted point -> leaf box (local)
+  std::vector<int> nb_ptr, nb_list;            // leaf neighbours (local leaf ids)
+  std::vector<int> cone_box, cone_posr, cone_post;   // global cone id -> local box, posr, postheta
+  std::vector<unsigned char> cone_level;
+  std::vector<int> dense_map;                  // see LevelInfo::map_off
+  std::vector<int> proto;                      // see LevelInfo::proto_off, [q][4][P]
+  std::vector<long long> it_ptr;               // interpolation entries per sorted target
+  std::vector<int> it_cone;                    // global cone id per entry
+  long long n_leaf = 0, n_prop = 0, n_interp = 0, n_cones = 0;
+};
+bool build_tree(const double* x, const double* y, const double* nx, const double* ny, long long N,
+                int nlev, double k, int Ps, int Pang, Tree& t, std::string& err);
+
+// Near-field exclusion lists (init_sort_sing_point, ForwardMap.tpp:567-607) in
+// sorted order: own patch + patches the point is near-singular to, padded
+// with -1, kMaxExcl per point; patch id per sorted point. Also checks that all
+// excluded pairs are leaf-level neighbours (BoxTree.h:1079-1161).
+constexpr int kMaxExcl = 4;
+bool build_exclusions(const Geometry& g, const Tree& t, std::vector<int>& patch_of_sorted,
+                      std::vector<int>& excl, long long& n_near, std::string& err);
+
+}  // namespace ifgf

@@ -1,0 +1,24 @@
+// Per-cone geometry records and Chebyshev tables shared by the host plan code
+// and the gfx950 kernels.
+#pragma once
+
+namespace ifgf {
+
+struct ConeGeo {
+  double cx, cy;  // box centre (Level::GetBoxCenter)
+  double h;       // Level::scaling_r2s_
+  double rmin;    // h / radial[posr + 1]
+  double dr;      // h / radial[posr] - rmin
+  double da;      // pi / nconeselevation_
+  double th0;     // da * postheta
+  double pad;
+};
+
+// Chebyshev tables of the cone interpolant (Chebyshev.h:10-68), up to 16x16.
+struct ChebTables {
+  double xs[16], xa[16];    // ascending Chebyshev points in s and theta
+  double Ts[16 * 16];       // Ts[m*16 + k] = T_m(xs[k]) * alpha_m
+  double Ta[16 * 16];       // Ta[n*16 + l] = T_n(xa[l]) * alpha_n / (Ps*Pang)
+};
+
+}  // namespace ifgf

@@ -1,0 +1,35 @@
+// Minimal complex arithmetic shared by host setup code and gfx950 kernels.
+// Complex numbers are stored interleaved (re, im), the layout of
+// std::complex<double> / Eigen::VectorXcd that the reference passes around.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#define IFGF_HD __host__ __device__
+
+namespace ifgf {
+
+struct zd {
+  double x, y;
+};
+
+struct zf {
+  float x, y;
+};
+
+IFGF_HD inline zd cadd(zd a, zd b) { return zd{a.x + b.x, a.y + b.y}; }
+IFGF_HD inline zd csub(zd a, zd b) { return zd{a.x - b.x, a.y - b.y}; }
+IFGF_HD inline zd cmul(zd a, zd b) { return zd{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+IFGF_HD inline zd cscale(zd a, double s) { return zd{a.x * s, a.y * s}; }
+// a += b * c
+IFGF_HD inline void cfma(zd& a, zd b, zd c) {
+  a.x = fma(b.x, c.x, fma(-b.y, c.y, a.x));
+  a.y = fma(b.x, c.y, fma(b.y, c.x, a.y));
+}
+// a / b
+IFGF_HD inline zd cdiv(zd a, zd b) {
+  const double d = 1.0 / (b.x * b.x + b.y * b.y);
+  return zd{(a.x * b.x + a.y * b.y) * d, (a.y * b.x - a.x * b.y) * d};
+}
+
+}  // namespace ifgf

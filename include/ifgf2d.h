@@ -1,0 +1,179 @@
+/*
+ * ifgf2d.h -- C ABI of the MI355X IFGF forward map (libifgf2d.so).
+ *
+ * The drop-in seam of manuelarturosantana/IFGF2D is the operator the GMRES
+ * driver applies once per iteration:
+ *   LinearOperator(rows, cols, mapping)          ForwardMap/LinearOperator.hpp:27-60
+ *   mapping = [&](x, out){ compute_Ax_acc(x, out, boxes); }   ForwardMap.tpp:673-680
+ *   out = mat * v                                 ForwardMap/opGMRES.hpp:102,152,250
+ * Every entry point below replaces one reference interface (cited per
+ * function). Plain pointers and sizes only; complex vectors are interleaved
+ * (re, im) doubles, the std::complex<double> / Eigen::VectorXcd layout, in the
+ * reference's patch order (patch-major, Np nodes per patch). No exceptions
+ * cross this boundary: every function returns a status code (IFGF_OK = 0) and
+ * ifgf_last_error() / ifgf_plan_error() return the message.
+ */
+#ifndef IFGF2D_H
+#define IFGF2D_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define IFGF_OK 0
+#define IFGF_ERR_ARG 1          /* invalid argument */
+#define IFGF_ERR_GEOMETRY 2     /* degenerate geometry / tree invariant violated */
+#define IFGF_ERR_LEVELS 3       /* nlevels < 3 (BoxTree.h:1024-1025) */
+#define IFGF_ERR_HIP 4          /* HIP runtime error */
+#define IFGF_ERR_NO_DEVICE 5    /* plan was built host-only, or no GPU */
+#define IFGF_ERR_NOT_CONVERGED 6 /* GMRES reached maxit (Eigen::NoConvergence) */
+
+/* formulations (GreenFunctions.hpp:15-20) */
+#define IFGF_SINGLE_LAYER 0
+#define IFGF_DOUBLE_LAYER 1
+#define IFGF_COMBINED 2
+
+/* precision modes */
+#define IFGF_FP64 0   /* everything in FP64 */
+#define IFGF_MIXED 1  /* FP32 cone coefficients, propagation and interpolation;
+                         FP64 Hankel, centre factors, near field and correction */
+
+/* curve types (Geometry/Curve.hpp:106-146; star has no reference class) */
+#define IFGF_CURVE_CIRCLE 0 /* params: cx, cy, R */
+#define IFGF_CURVE_KITE 1   /* params: A, B, C, D: x = A cos t + B cos 2t + C, y = D sin t */
+#define IFGF_CURVE_STAR 2   /* params: cx, cy, R, nmodes, a[nmodes], b[nmodes] */
+
+/* plan flags */
+#define IFGF_PLAN_HOST_ONLY 1 /* build every host table, allocate nothing on the GPU */
+
+/* apply flags */
+#define IFGF_PTR_DEVICE 1     /* x / y are device pointers; the call is asynchronous on stream */
+
+typedef struct ifgf_plan ifgf_plan;
+
+/* Discretised closed curve: equal-weight Fejer first-rule nodes on each patch
+ * [patch_lims[p], patch_lims[p+1]] (ForwardMap ctor, ForwardMap.tpp:15-222). */
+typedef struct {
+  int curve_type;
+  const double* curve_params;
+  int ncurve_params;
+  const double* patch_lims; /* npatches + 1 increasing parameters spanning [0, 2 pi] */
+  int npatches;
+  int Np;        /* nodes per patch (template Np of ForwardMap) */
+  double delta;  /* near-singular distance (ForwardMap::delta_) */
+  double rp_p;   /* rectangular-polar change-of-variables strength (ForwardMap::p_) */
+} ifgf_surface;
+
+typedef struct {
+  int formulation;
+  int precision;
+  double k;       /* wavenumber */
+  double eta;     /* combined-field coupling */
+  int Ps, Pang;   /* Chebyshev nodes per cone in s and theta (BoxTree<Ps,Pang>) */
+  int nlevels;    /* tree levels, >= 3 */
+  int nrhs_max;   /* largest nrhs passed to ifgf_apply */
+  int flags;      /* IFGF_PLAN_HOST_ONLY */
+} ifgf_params;
+
+/* Replaces ForwardMap::ForwardMap (ForwardMap.tpp:15-35) followed by
+ * ForwardMap::precomps_and_setup (ForwardMap.tpp:542-565): geometry, RP
+ * precomputations, BoxTree::Reinitialize, the neighbourhood check and
+ * init_sort_sing_point. Errors: IFGF_ERR_LEVELS, IFGF_ERR_GEOMETRY (the
+ * std::logic_error cases of Level.h:153-157 and BoxTree.h:321,571,1117,1152),
+ * IFGF_ERR_ARG, IFGF_ERR_HIP. */
+int ifgf_plan_create(const ifgf_surface* surface, const ifgf_params* params, ifgf_plan** plan);
+
+/* Releases the plan (BoxTree / ForwardMap destructors). */
+void ifgf_plan_destroy(ifgf_plan* plan);
+
+/* Message of the last failing call on this thread / on this plan. */
+const char* ifgf_last_error(void);
+const char* ifgf_plan_error(const ifgf_plan* plan);
+
+/* LinearOperator::rows() / cols() (LinearOperator.hpp:48-49). */
+long long ifgf_plan_size(const ifgf_plan* plan);
+
+/* ForwardMap::xs_, ys_, nxs_, nys_ (ForwardMap.hpp:72-75) and the intensity
+ * factor of compute_intensities (ForwardMap.tpp:425-435); any may be NULL. */
+int ifgf_plan_points(const ifgf_plan* plan, double* x, double* y, double* nx, double* ny, double* w);
+
+/* Work counts of one application: [0] near-field pairs, [1] leaf kernel
+ * evaluations, [2] propagation interpolations, [3] target interpolations,
+ * [4] relevant cones (all levels), [5] correction entries, [6] nlevels,
+ * [7] N. */
+int ifgf_plan_stats(const ifgf_plan* plan, long long out[8]);
+
+/* BoxTree::getSorting() (BoxTree.h:1163-1165): Morton position -> patch-order index. */
+int ifgf_plan_sorting(const ifgf_plan* plan, long long* sorting);
+
+/* Level::relconesegment2cocenteredmortonboxid_ / relconesegment2nonrelconesegment_
+ * (Level.h:41-43) of one level (2..nlevels-1), in the plan's canonical order
+ * (box Morton ascending, non-relevant cone ascending). Returns the count, or
+ * -1 on error; arrays may be NULL to query the count. */
+long long ifgf_plan_level_cones(const ifgf_plan* plan, int level, long long* box_morton,
+                                long long* nonrel);
+
+/* Folded correction rows (compute_sing_near_sing_interactions,
+ * ForwardMap.tpp:481-515, with Cheb1D::interp_1d folded in): ptr has N + 1
+ * entries; entry e adds sum_k w[e][k] x[patch[e]*Np + k] to target i. Any
+ * array may be NULL; returns the entry count, or -1 on error. */
+long long ifgf_plan_correction(const ifgf_plan* plan, long long* ptr, int* patch, double* w);
+
+/* The forward map: replaces ForwardMap::compute_Ax_acc(const VectorXcd&, VectorXcd&,
+ * BoxTree&) (ForwardMap.tpp:648-655), i.e. the mapping LinearOperator::operator*
+ * calls (LinearOperator.cpp:14-23). y = (1/2) x [double layer / combined only]
+ * + correction(x) + IFGF(intensities(x)). x, y: nrhs columns of n complex
+ * values, column stride ld (>= n). Host pointers: synchronous. With
+ * IFGF_PTR_DEVICE: device pointers, asynchronous on stream (hipStream_t, NULL
+ * = the plan's stream). Applies on one plan are serialised; the plan is not
+ * reentrant (like BoxTree's mutable buffers, BoxTree.h:48-51). */
+int ifgf_apply(ifgf_plan* plan, const double* x, double* y, int nrhs, long long ld, int flags,
+               void* stream);
+
+/* Unaccelerated operator on a target subset: replaces compute_Ax_unacc
+ * (ForwardMap.tpp:438-540) evaluated at targets[0..nt). Host pointers. */
+int ifgf_apply_direct(ifgf_plan* plan, const double* x, const long long* targets, long long nt,
+                      double* out);
+
+/* Field at off-surface points: replaces ForwardMap::propagate_unacc
+ * (ForwardMap.tpp:702-727) (direct sum on the GPU). Host pointers. */
+int ifgf_propagate(ifgf_plan* plan, const double* tx, const double* ty, long long nt,
+                   const double* density, double* out);
+
+/* Restarted GMRES on the GPU with the plan's forward map: replaces
+ * ForwardMap::solve's Eigen::opGMRES<LinearOperator> (ForwardMap.tpp:658-697,
+ * opGMRES.hpp:68-269). Arnoldi by classical Gram-Schmidt with one
+ * re-orthogonalisation (FP64). Stops when the recursive residual estimate
+ * |w(k)| / ||r0|| < tol (opGMRES.hpp:207) or after maxit iterations.
+ * rhs / x are host pointers of n complex values; x holds the initial guess.
+ * relres returns the true FP64 residual ||b - A x|| / ||b|| at exit. hist (may
+ * be NULL) receives up to hist_cap residual estimates. Returns IFGF_OK,
+ * IFGF_ERR_NOT_CONVERGED or an error. */
+int ifgf_gmres(ifgf_plan* plan, const double* rhs, double* x, int restart, double tol, int maxit,
+               int* iters, double* relres, double* hist, int hist_cap);
+
+/* Generic host GMRES on any operator: the reference's Householder GMRES
+ * (opGMRES.hpp:68-269) with the mapping of LinearOperator.hpp:54 as a C
+ * callback. Returns IFGF_OK / IFGF_ERR_NOT_CONVERGED; err is the estimate. */
+typedef void (*ifgf_apply_fn)(void* ctx, const double* x, double* y);
+int ifgf_gmres_host(ifgf_apply_fn apply, void* ctx, long long n, const double* rhs, double* x,
+                    int restart, double tol, int maxit, int* iters, double* err);
+
+/* H0^(1)(x) and H1^(1)(x) on the GPU for x[0..n) (replaces
+ * sp_bessel::hankelH1(0|1, x), besselFunctions.h:312-341). Host pointers;
+ * h0 / h1 receive n interleaved complex values. */
+int ifgf_hankel01(const double* x, long long n, double* h0, double* h1);
+
+/* Per-stage GPU timing of applications (hipEvents on the plan's stream).
+ * enable != 0 starts recording; times_ms receives the accumulated time of
+ * [0] intensities+sort, [1] near field, [2] leaf cones, [3] propagation,
+ * [4] interpolation, [5] correction+unsort, and counts[0] the number of
+ * applications recorded. */
+int ifgf_plan_profile(ifgf_plan* plan, int enable);
+int ifgf_plan_stage_times(const ifgf_plan* plan, double times_ms[6], long long* count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* IFGF2D_H */
