@@ -1,0 +1,158 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X IFGF forward map.
+
+Workload: sound-soft kite x = cos t + 0.65 cos 2t - 0.65, y = 1.5 sin t at
+k = 200, 6,400 equal-parameter patches x 10 Fejer nodes (N = 64,000),
+combined-field operator (eta = k), 5 x 5 Chebyshev nodes per cone, 10 tree
+levels, all FP64. One step = one forward-map application (the operator GMRES
+applies once per iteration) on a seeded complex density already resident in
+HBM. After the timed steps, one GMRES(50) solve to 1e-8 for the plane wave
+d = (cos pi/4, sin pi/4) gives the time to solve.
+
+Prints ONE JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 (vector and matrix)
+PEAK_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def kernel_flops_per_pair(formulation):
+    # kernel generation (SLP 40, DLP/CFIE 60 flops: H0 [+ H1] + r and the
+    # normal product) + one complex multiply-add (8 flops)
+    return (40 if formulation == 0 else 60) + 8
+
+
+def cpu_baseline(kw, x):
+    """The oracle (single-threaded CPU restatement, Amos Hankel) on one
+    forward-map application of the same workload; oracle setup excluded."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    t0 = time.time()
+    om = O.ForwardMap(kw["curve"][0], kw["curve"][1], kw["patch_lims"], formulation=O.COMBINED,
+                      k=kw["k"], eta=kw["k"], delta=kw["delta"], p=kw["p"], nlevels=kw["nlevels"])
+    log(f"cpu baseline: oracle setup {time.time() - t0:.1f} s")
+    t0 = time.time()
+    om.apply(x)
+    dt = time.time() - t0
+    log(f"cpu baseline: one oracle application {dt:.2f} s")
+    return {"value": 1.0 / dt, "unit": "matvecs/s", "cores": 1, "kind": "port",
+            "sample": "1 forward-map application (combined field, 5x5) of the 64,000-point kite at "
+                      "k=200 by the single-threaded oracle with the reference's Amos Hankel; "
+                      "oracle setup excluded"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-solve", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the forward map is "
+              "not sharded across GPUs in this build", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import ifgf2d_amd as I
+    from ifgf2d_amd import configs
+
+    kw, d = configs.kite_k200()
+    t0 = time.time()
+    fm = I.ForwardMap(**kw)
+    t_setup = time.time() - t0
+    kw["delta"] = fm.delta
+    st = fm.stats()
+    log(f"plan: N={fm.N} setup {t_setup:.2f} s stats {st}")
+
+    rng = np.random.default_rng(0)
+    x_host = rng.standard_normal(fm.N) + 1j * rng.standard_normal(fm.N)
+    dev = torch.device("cuda:0")
+    xs = torch.from_numpy(x_host).to(dev)
+    ys = torch.empty_like(xs)
+    torch.cuda.synchronize()
+
+    for _ in range(args.warmup):
+        fm.apply_device(xs.data_ptr(), ys.data_ptr())
+    torch.cuda.synchronize()
+
+    fm.profile(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        fm.apply_device(xs.data_ptr(), ys.data_ptr())
+    torch.cuda.synchronize()
+    t_run = time.perf_counter() - t0
+    stages, count = fm.stage_times()
+    fm.profile(False)
+    ms_per_step = 1e3 * t_run / args.steps
+    value = args.steps / t_run
+    stage_avg = {k_: v / max(count, 1) for k_, v in stages.items()}
+    log(f"{args.steps} applications in {t_run:.3f} s -> {value:.1f} matvecs/s; stages (ms/apply) "
+        + ", ".join(f"{k_} {v:.4f}" for k_, v in stage_avg.items()))
+
+    # dominant kernel and its roofline (algorithmic FP64 flops per launch)
+    per_pair = kernel_flops_per_pair(fm.formulation)
+    work = {"near": st["near"] * per_pair, "leaf": st["leaf"] * per_pair}
+    dom = max(("near", "leaf"), key=lambda s: stage_avg[s])
+    achieved = work[dom] / (stage_avg[dom] * 1e-3) / 1e12
+    roofline = {"bound": "mfma", "kernel": "k_near" if dom == "near" else "k_leaf",
+                "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None,
+                "flops_per_launch": work[dom], "avg_launch_ms": stage_avg[dom]}
+    traffic_file = os.path.join(ROOT, "profiles", f"traffic_{roofline['kernel']}_kite_k200.json")
+    if os.path.exists(traffic_file):
+        roofline["traffic"] = json.load(open(traffic_file)).get("bytes_per_launch")
+
+    solve = None
+    if not args.no_solve:
+        xg, yg, *_ = fm.points()
+        rhs = -configs.plane_wave(xg, yg, kw["k"], d)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        dens, it, relres, hist = fm.solve(rhs, restart=50, tol=1e-8, maxit=2000)
+        t_solve = time.perf_counter() - t0
+        solve = {"time_to_solve_s": round(t_solve, 4), "gmres_iterations": it,
+                 "true_relres": relres, "restart": 50, "tol": 1e-8}
+        log(f"GMRES: {it} iterations, {t_solve:.3f} s, true relres {relres:.2e}")
+
+    cpu = None if args.no_cpu_baseline else cpu_baseline(kw, x_host)
+
+    out = {
+        "metric": "time-to-solve (rel. residual 1e-8) + IFGF matvecs/s vs plain FP64 on GPU",
+        "value": round(value, 3), "unit": "matvecs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_per_step, 5), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "kite k=200, 6400 patches x 10 Fejer nodes (N=64000), combined "
+                               "field eta=k, 5x5 cone nodes, 10 levels, FP64",
+                   "N": fm.N, "nrhs": 1, "nlevels": kw["nlevels"], "Ps": 5, "Pang": 5,
+                   "parallelism": "single GPU"},
+        "stages_ms": {k_: round(v, 5) for k_, v in stage_avg.items()},
+        "work": {k_: st[k_] for k_ in ("near", "leaf", "prop", "interp", "cones", "corr")},
+        "plan_setup_s": round(t_setup, 3),
+        "solve": solve,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["gpu_over_cpu"] = round(value / cpu["value"], 1)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

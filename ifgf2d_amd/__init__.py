@@ -147,6 +147,8 @@ def _curve_eval(curve, t):
         return p[0] + p[2] * np.cos(t), p[1] + p[2] * np.sin(t)
     if typ == 1:
         return p[0] * np.cos(t) + p[1] * np.cos(2 * t) + p[2], p[3] * np.sin(t)
+    if typ != 2 or len(p) < 4:
+        raise IFGFError("unknown curve type or too few curve parameters")
     n = int(p[3])
     r = np.ones_like(t)
     for m in range(n):

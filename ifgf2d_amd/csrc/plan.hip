@@ -31,7 +31,6 @@ static void build_device_side(ifgf_plan* P) {
   const long long N = t.N;
   const int Pn = t.Ps * t.Pang;
   HIP_TRY(hipStreamCreate(&P->stream));
-  for (auto& e : P->ev) HIP_TRY(hipEventCreate(&e));
   P->px.upload(t.sx);
   P->py.upload(t.sy);
   P->pnx.upload(t.snx);
@@ -217,18 +216,28 @@ static void launch_ifgf(ifgf_plan* P, const zd* x, zd* y, hipStream_t st) {
   const double k = P->prm.k, eta = P->prm.eta;
   const int nb = (N + 255) / 256;
   const bool prof = P->profile;
-  if (prof) HIP_TRY(hipEventRecord(P->ev[0], st));
+  hipEvent_t* ev = nullptr;
+  if (prof) {
+    if (P->ev_used + 7 > P->ev_pool.size()) {
+      const size_t old = P->ev_pool.size();
+      P->ev_pool.resize(old + 7 * 64, nullptr);
+      for (size_t q = old; q < P->ev_pool.size(); q++) HIP_TRY(hipEventCreate(&P->ev_pool[q]));
+    }
+    ev = &P->ev_pool[P->ev_used];
+    P->ev_used += 7;
+    HIP_TRY(hipEventRecord(ev[0], st));
+  }
   hipLaunchKernelGGL(k_gather, dim3(nb), dim3(256), 0, st, N, x, P->w.p, P->perm.p, P->rho.p);
-  if (prof) HIP_TRY(hipEventRecord(P->ev[1], st));
+  if (prof) HIP_TRY(hipEventRecord(ev[1], st));
   hipLaunchKernelGGL((k_near<F>), dim3(nb), dim3(256), 0, st, N, P->px.p, P->py.p, P->pnx.p, P->pny.p,
                      P->patch.p, P->excl_d.p, P->leaf_of_point.p, P->leaf_begin.p, P->nb_ptr.p,
                      P->nb_list.p, P->rho.p, P->sol.p, k, eta);
-  if (prof) HIP_TRY(hipEventRecord(P->ev[2], st));
+  if (prof) HIP_TRY(hipEventRecord(ev[2], st));
   const LevelInfo& LD = t.lv[D];
   hipLaunchKernelGGL((k_leaf<F, Ps, Pang>), dim3((LD.ncone + CPB - 1) / CPB), dim3(256), 0, st,
                      LD.ncone, LD.cone_off, P->cg.p, P->cone_box.p, P->leaf_begin.p, P->px.p, P->py.p,
                      P->pnx.p, P->pny.p, P->rho.p, P->coef.p, P->tabs.p, k, eta);
-  if (prof) HIP_TRY(hipEventRecord(P->ev[3], st));
+  if (prof) HIP_TRY(hipEventRecord(ev[3], st));
   for (int l = D; l >= 3; l--) {
     const LevelInfo& L = t.lv[l];
     const LevelInfo& Pl = t.lv[l - 1];
@@ -237,14 +246,14 @@ static void launch_ifgf(ifgf_plan* P, const zd* x, zd* y, hipStream_t st) {
                        P->cone_qd.p, P->box_child.p, P->proto.p + L.proto_off, P->dense_map.p + L.map_off,
                        P->coef.p, P->tabs.p, k);
   }
-  if (prof) HIP_TRY(hipEventRecord(P->ev[4], st));
+  if (prof) HIP_TRY(hipEventRecord(ev[4], st));
   hipLaunchKernelGGL((k_interp<Ps, Pang>), dim3(nb), dim3(256), 0, st, N, P->px.p, P->py.p, P->it_ptr.p,
                      P->it_cone.p, P->cg.p, P->coef.p, P->sol.p, k);
-  if (prof) HIP_TRY(hipEventRecord(P->ev[5], st));
+  if (prof) HIP_TRY(hipEventRecord(ev[5], st));
   const double ident = (F == kSingleLayer) ? 0.0 : 0.5;
   hipLaunchKernelGGL(k_finalize, dim3(nb), dim3(256), 0, st, N, P->geo.Np, ident, x, P->corr_ptr.p,
                      P->corr_patch.p, P->corr_w.p, P->inv.p, P->sol.p, y);
-  if (prof) HIP_TRY(hipEventRecord(P->ev[6], st));
+  if (prof) HIP_TRY(hipEventRecord(ev[6], st));
   HIP_TRY(hipGetLastError());
 }
 
@@ -274,20 +283,26 @@ static bool dispatch(ifgf_plan* P, const zd* x, zd* y, hipStream_t st) {
   }
 }
 
+// accumulate the recorded applications' stage times and recycle their events
 static void collect_profile(ifgf_plan* P) {
-  HIP_TRY(hipEventSynchronize(P->ev[6]));
-  for (int s = 0; s < 6; s++) {
-    float ms = 0.f;
-    HIP_TRY(hipEventElapsedTime(&ms, P->ev[s], P->ev[s + 1]));
-    P->stage_ms[s] += ms;
+  for (size_t a = 0; a + 7 <= P->ev_used; a += 7) {
+    hipEvent_t* ev = &P->ev_pool[a];
+    HIP_TRY(hipEventSynchronize(ev[6]));
+    for (int s = 0; s < 6; s++) {
+      float ms = 0.f;
+      HIP_TRY(hipEventElapsedTime(&ms, ev[s], ev[s + 1]));
+      P->stage_ms[s] += ms;
+    }
+    P->prof_count++;
   }
-  P->prof_count++;
+  P->ev_used = 0;
 }
 
 // one device-resident application on stream st (used by ifgf_apply and GMRES)
 int ifgf::apply_device(ifgf_plan* P, const zd* x, zd* y, hipStream_t st) {
   if (!dispatch(P, x, y, st)) return fail(P, IFGF_ERR_ARG, "no kernel instantiation for this (Ps, Pang)");
-  if (P->profile) collect_profile(P);
+  // bound the number of outstanding event sets
+  if (P->profile && P->ev_used >= 7 * 4096) collect_profile(P);
   return IFGF_OK;
 }
 
@@ -446,13 +461,19 @@ int ifgf_hankel01(const double* x, long long n, double* h0, double* h1) {
 int ifgf_plan_profile(ifgf_plan* P, int enable) {
   if (!P) return fail(nullptr, IFGF_ERR_ARG, "null plan");
   P->profile = enable != 0;
+  P->ev_used = 0;
   for (double& v : P->stage_ms) v = 0.0;
   P->prof_count = 0;
   return IFGF_OK;
 }
 
-int ifgf_plan_stage_times(const ifgf_plan* P, double times_ms[6], long long* count) {
+int ifgf_plan_stage_times(ifgf_plan* P, double times_ms[6], long long* count) {
   if (!P || !times_ms) return fail(nullptr, IFGF_ERR_ARG, "null argument");
+  try {
+    collect_profile(P);
+  } catch (std::exception& e) {
+    return fail(P, IFGF_ERR_HIP, e.what());
+  }
   for (int s = 0; s < 6; s++) times_ms[s] = P->stage_ms[s];
   if (count) *count = P->prof_count;
   return IFGF_OK;

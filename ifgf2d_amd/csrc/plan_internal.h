@@ -75,12 +75,15 @@ struct ifgf_plan {
   // workspaces
   ifgf::DevBuf<zd> rho, sol, coef, xbuf, ybuf;
   // profiling
+  // profiling: 7 events per recorded application, read back (and summed) on
+  // demand so recording adds no host synchronisation to the applies
   bool profile = false;
-  hipEvent_t ev[7] = {};
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
   double stage_ms[6] = {0, 0, 0, 0, 0, 0};
   long long prof_count = 0;
   ~ifgf_plan() {
-    for (auto& e : ev)
+    for (auto& e : ev_pool)
       if (e) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
   }

@@ -165,13 +165,16 @@ int ifgf_gmres_host(ifgf_apply_fn apply, void* ctx, long long n, const double* r
  * h0 / h1 receive n interleaved complex values. */
 int ifgf_hankel01(const double* x, long long n, double* h0, double* h1);
 
-/* Per-stage GPU timing of applications (hipEvents on the plan's stream).
- * enable != 0 starts recording; times_ms receives the accumulated time of
- * [0] intensities+sort, [1] near field, [2] leaf cones, [3] propagation,
- * [4] interpolation, [5] correction+unsort, and counts[0] the number of
- * applications recorded. */
+/* Per-stage GPU timing of applications: hipEvents recorded on the stream each
+ * application's kernels run on, without host synchronisation (the hipEvent
+ * timers that replace the reference's std::chrono prints,
+ * ForwardMap.tpp:545-563). enable != 0 clears and starts recording;
+ * ifgf_plan_stage_times waits for the recorded applications and returns the
+ * accumulated time of [0] intensities+sort, [1] near field, [2] leaf cones,
+ * [3] propagation, [4] interpolation, [5] correction+unsort, and in *count
+ * the number of applications recorded. */
 int ifgf_plan_profile(ifgf_plan* plan, int enable);
-int ifgf_plan_stage_times(const ifgf_plan* plan, double times_ms[6], long long* count);
+int ifgf_plan_stage_times(ifgf_plan* plan, double times_ms[6], long long* count);
 
 #ifdef __cplusplus
 }

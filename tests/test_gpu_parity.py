@@ -1,0 +1,168 @@
+"""GPU parity: the HIP forward map (through the C ABI) against the CPU oracle.
+
+Bar: the plain-FP64 GPU forward map matches the oracle's restatement of
+ForwardMap::compute_Ax_acc to <= 1e-12 relative (2-norm over the vector) on
+the same seeded inputs; the device Hankel evaluator matches mpmath to 2e-15.
+At the full 64k-point kite size, size-independent properties are checked
+(linearity, bitwise determinism, IFGF accuracy against direct summation).
+"""
+import numpy as np
+import pytest
+
+import ifgf2d_amd as I
+import oracle as O
+from ifgf2d_amd import configs
+
+pytestmark = pytest.mark.gpu
+
+PARITY_TOL = 1e-12
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def density(n, seed=0):
+    rng = np.random.default_rng(seed)
+    return rng.standard_normal(n) + 1j * rng.standard_normal(n)
+
+
+def test_device_hankel_matches_mpmath(hankel_golden):
+    x = hankel_golden[:, 0]
+    h0, h1 = I.hankel01(x)
+    e0 = np.abs(h0 - (hankel_golden[:, 1] + 1j * hankel_golden[:, 2])) / np.abs(h0)
+    e1 = np.abs(h1 - (hankel_golden[:, 3] + 1j * hankel_golden[:, 4])) / np.abs(h1)
+    print("device Hankel max rel err", e0.max(), e1.max())
+    assert e0.max() < 2e-15 and e1.max() < 2e-15
+
+
+CASES = [
+    # name, curve, npatches, k, nlevels, formulation, Ps, Pang
+    ("circle-cfie", I.circle(), 40, 10.0, 3, I.COMBINED, 5, 5),
+    ("circle-slp", I.circle(), 40, 10.0, 3, I.SINGLE_LAYER, 5, 5),
+    ("circle-dlp", I.circle(), 40, 10.0, 3, I.DOUBLE_LAYER, 5, 5),
+    ("circle-cfie-9x9", I.circle(), 40, 10.0, 3, I.COMBINED, 9, 9),
+    ("kite-cfie", I.kite(), 400, 40.0, 6, I.COMBINED, 5, 5),
+    ("kite-slp-3x5", I.kite(), 400, 40.0, 6, I.SINGLE_LAYER, 3, 5),
+    ("star-cfie", I.star(a=[0, 0, 0, 0, 0, 0.3]), 800, 60.0, 7, I.COMBINED, 5, 5),
+    ("star-slp-5x7", I.star(a=[0, 0, 0, 0, 0, 0.3]), 800, 60.0, 7, I.SINGLE_LAYER, 5, 7),
+    ("star-cfie-7x7", I.star(a=[0, 0, 0, 0, 0, 0.3]), 800, 60.0, 7, I.COMBINED, 7, 7),
+]
+
+
+def build_pair(curve, npatch, k, nlev, F, Ps, Pang):
+    lims = I.equal_parameter_lims(npatch)
+    delta = I.default_delta(curve, lims)
+    fm = I.ForwardMap(curve, lims, k, nlev, formulation=F, delta=delta, p=6.0, Ps=Ps, Pang=Pang,
+                      nrhs_max=3)
+    om = O.ForwardMap(curve[0], curve[1], lims, formulation=F, k=k, eta=k, delta=delta, p=6.0,
+                      Ps=Ps, Pang=Pang, nlevels=nlev)
+    return fm, om
+
+
+@pytest.mark.parametrize("name,curve,npatch,k,nlev,F,Ps,Pang", CASES, ids=[c[0] for c in CASES])
+def test_forward_map_parity(name, curve, npatch, k, nlev, F, Ps, Pang):
+    print(f"case {name}: N={npatch * 10} k={k} nlev={nlev} F={F} P={Ps}x{Pang}", flush=True)
+    fm, om = build_pair(curve, npatch, k, nlev, F, Ps, Pang)
+    for seed in (0, 1):
+        x = density(fm.N, seed)
+        yg = fm.compute_Ax_acc(x)
+        yo = om.apply(x)
+        e = rel(yg, yo)
+        print(f"  seed {seed}: rel diff {e:.3e}  max abs {np.max(np.abs(yg - yo)):.3e}", flush=True)
+        assert e <= PARITY_TOL
+
+
+def test_apply_is_deterministic_and_linear():
+    fm, _ = build_pair(I.kite(), 400, 40.0, 6, I.COMBINED, 5, 5)
+    x1, x2 = density(fm.N, 3), density(fm.N, 4)
+    a, b = 0.3 - 1.1j, -2.0 + 0.5j
+    y1 = fm.compute_Ax_acc(x1)
+    assert np.array_equal(y1, fm.compute_Ax_acc(x1))
+    y2 = fm.compute_Ax_acc(x2)
+    assert rel(fm.compute_Ax_acc(a * x1 + b * x2), a * y1 + b * y2) < 1e-13
+
+
+def test_multiple_right_hand_sides():
+    fm, _ = build_pair(I.star(a=[0, 0, 0, 0, 0, 0.3]), 800, 60.0, 7, I.COMBINED, 5, 5)
+    X = np.stack([density(fm.N, s) for s in (5, 6, 7)], axis=1)
+    Y = fm.compute_Ax_acc(X)
+    for j in range(3):
+        assert np.array_equal(Y[:, j], fm.compute_Ax_acc(X[:, j]))
+
+
+def test_device_pointer_apply_with_torch_stream():
+    torch = pytest.importorskip("torch")
+    fm, _ = build_pair(I.kite(), 400, 40.0, 6, I.COMBINED, 5, 5)
+    x = density(fm.N, 8)
+    xt = torch.from_numpy(x).to("cuda:0")
+    yt = torch.empty_like(xt)
+    s = torch.cuda.current_stream()
+    fm.apply_device(xt.data_ptr(), yt.data_ptr(), stream=s.cuda_stream)
+    torch.cuda.synchronize()
+    assert np.array_equal(yt.cpu().numpy(), fm.compute_Ax_acc(x))
+
+
+def test_unaccelerated_subset_and_propagation_match_oracle():
+    fm, om = build_pair(I.kite(), 400, 40.0, 6, I.COMBINED, 5, 5)
+    x = density(fm.N, 9)
+    tg = np.arange(0, fm.N, 13)
+    yd = fm.compute_Ax_unacc(x, tg)
+    yo = om.apply(x, accelerated=False)[tg]
+    assert rel(yd, yo) <= PARITY_TOL
+    th = np.linspace(0, 2 * np.pi, 32, endpoint=False)
+    tx, ty = 3 * np.cos(th), 3 * np.sin(th)
+    assert rel(fm.propagate_unacc(tx, ty, x), om.propagate(tx, ty, x)) <= PARITY_TOL
+
+
+def mie(r, th, k, d_angle=0.0):
+    import scipy.special as sp
+    M = int(k + 40)
+    m = np.arange(-M, M + 1)
+    c = -(1j ** m) * sp.jv(m, k) / sp.hankel1(m, k)
+    return np.sum(c[None, :] * sp.hankel1(m[None, :], k * r)
+                  * np.exp(1j * m[None, :] * (th[:, None] - d_angle)), axis=1)
+
+
+def test_device_gmres_circle_mie():
+    """Sound-soft unit circle at k = 10 (40 patches x 10 nodes), plane wave
+    d = (1, 0): device GMRES(50) to 1e-8, then the scattered field at 64 points
+    on r = 3 against the Mie series."""
+    kw, d = configs.circle_k10(Ps=9, Pang=9)
+    fm = I.ForwardMap(**kw)
+    x, y, *_ = fm.points()
+    rhs = -configs.plane_wave(x, y, kw["k"], d)
+    dens, it, relres, hist = fm.solve(rhs, restart=50, tol=1e-8)
+    print("GMRES iterations", it, "true relres", relres)
+    assert relres < 5e-8 and hist[-1] < 1e-8
+    th = 2 * np.pi * np.arange(64) / 64
+    us = fm.propagate_unacc(3 * np.cos(th), 3 * np.sin(th), dens)
+    err = rel(us, mie(3.0, th, kw["k"]))
+    print("Mie error", err)
+    assert err < 1e-5
+    # the oracle's Householder GMRES on the same discrete system
+    lims = kw["patch_lims"]
+    om = O.ForwardMap(O.CIRCLE, [0.0, 0.0, 1.0], lims, formulation=O.COMBINED, k=10.0, eta=10.0,
+                      delta=fm.delta, p=6.0, Ps=9, Pang=9, nlevels=3)
+    do, ito, erro, _ = om.gmres(rhs, restart=50, tol=1e-8)
+    assert abs(ito - it) <= 2
+    assert rel(dens, do) < 1e-7
+
+
+def test_full_size_kite_properties():
+    """Kite at k = 200, 6400 patches (N = 64,000), nlevels 10: determinism,
+    linearity, and IFGF accuracy against the GPU direct sum on 2,000 targets."""
+    kw, _ = configs.kite_k200()
+    fm = I.ForwardMap(**kw)
+    print("kite stats", fm.stats(), flush=True)
+    x1 = density(fm.N, 11)
+    y1 = fm.compute_Ax_acc(x1)
+    assert np.array_equal(y1, fm.compute_Ax_acc(x1))
+    x2 = density(fm.N, 12)
+    y2 = fm.compute_Ax_acc(x2)
+    assert rel(fm.compute_Ax_acc(x1 - 2j * x2), y1 - 2j * y2) < 1e-13
+    tg = np.arange(2000)
+    yd = fm.compute_Ax_unacc(x1, tg)
+    err = rel(y1[tg], yd)
+    print("IFGF 5x5 vs direct on 2000 targets", err)
+    assert 1e-6 < err < 2e-3
