@@ -21,69 +21,82 @@ namespace ifgf {
 constexpr double kTwoOverPi = 0.63661977236758134308;   // 2/pi
 constexpr double kBesselSplit = 8.0;
 
-// Clenshaw summation of sum_{k<N} c_k T_k(t).
-template <int N>
-IFGF_HD inline double cheb_sum(const double (&c)[N], double t) {
-  double b1 = 0.0, b2 = 0.0;
-  const double t2 = 2.0 * t;
+// W simultaneous Clenshaw sums out[q] = sum_{k<N} c[k][q] T_k(t). The W
+// recurrences are independent (instruction-level parallelism), and the loop
+// is kept rolled so the coefficients are read from the constant table each
+// step instead of occupying registers.
+template <int N, int W>
+IFGF_HD inline void cheb_sums(const double (&c)[N][W], double t, double (&out)[W]) {
+  double b1[W], b2[W];
 #pragma unroll
+  for (int q = 0; q < W; q++) b1[q] = b2[q] = 0.0;
+  const double t2 = 2.0 * t;
+#pragma unroll 1
   for (int k = N - 1; k >= 1; --k) {
-    const double b0 = fma(t2, b1, c[k] - b2);
-    b2 = b1;
-    b1 = b0;
+#pragma unroll
+    for (int q = 0; q < W; q++) {
+      const double b0 = fma(t2, b1[q], c[k][q] - b2[q]);
+      b2[q] = b1[q];
+      b1[q] = b0;
+    }
   }
-  return fma(t, b1, c[0] - b2);
+#pragma unroll
+  for (int q = 0; q < W; q++) out[q] = fma(t, b1[q], c[0][q] - b2[q]);
 }
+
+// x > 8 branch of hankel0, out of line for the same reason as hankel01_large.
+__attribute__((noinline)) IFGF_HD inline zd hankel0_large(double x);
 
 // H0^(1)(x) only (single-layer kernel). x > 0.
 IFGF_HD inline zd hankel0(double x) {
+  double v[2];
   if (x <= kBesselSplit) {
-    const double t = fma(x * x, 1.0 / 32.0, -1.0);
-    const double j0 = cheb_sum(kJ0Small, t);
-    const double r0 = cheb_sum(kR0Small, t);
-    const double y0 = fma(kTwoOverPi * log(0.5 * x), j0, r0);
-    return zd{j0, y0};
+    cheb_sums(kSmall2, fma(x * x, 1.0 / 32.0, -1.0), v);  // J0, R0
+    return zd{v[0], fma(kTwoOverPi * log(0.5 * x), v[0], v[1])};
   }
+  return hankel0_large(x);
+}
+
+__attribute__((noinline)) IFGF_HD inline zd hankel0_large(double x) {
+  double v[2];
   const double ix = 1.0 / x;
-  const double z = 64.0 * ix * ix;
-  const double t = fma(2.0, z, -1.0);
-  const double p0 = cheb_sum(kP0Large, t);
-  const double q0 = 8.0 * ix * cheb_sum(kQ0Large, t);
+  cheb_sums(kLarge2, fma(128.0 * ix, ix, -1.0), v);  // P0, Qt0 at t = 2 (8/x)^2 - 1
   double s, c;
   sincos(x, &s, &c);
   // exp(i(x - pi/4)) = ((c + s) + i(s - c)) / sqrt(2)
   const double amp = sqrt(kTwoOverPi * ix) * 0.70710678118654752440;
-  const zd ph{(c + s) * amp, (s - c) * amp};
-  return cmul(zd{p0, q0}, ph);
+  return cmul(zd{v[0], 8.0 * ix * v[1]}, zd{(c + s) * amp, (s - c) * amp});
 }
+
+// x > 8 branch of hankel01, kept out of line: the kernels' hot arguments lie
+// below the split, and inlining the sincos path raises their register use.
+__attribute__((noinline)) IFGF_HD inline void hankel01_large(double x, zd& h0, zd& h1);
 
 // H0^(1)(x) and H1^(1)(x) together (double-layer / combined-field kernels). x > 0.
 IFGF_HD inline void hankel01(double x, zd& h0, zd& h1) {
+  double v[4];
   if (x <= kBesselSplit) {
-    const double t = fma(x * x, 1.0 / 32.0, -1.0);
-    const double j0 = cheb_sum(kJ0Small, t);
-    const double r0 = cheb_sum(kR0Small, t);
-    const double j1 = x * cheb_sum(kJ1Small, t);
-    const double r1 = x * cheb_sum(kR1Small, t);
+    cheb_sums(kSmall4, fma(x * x, 1.0 / 32.0, -1.0), v);  // J0, J1/x, R0, R1/x
     const double l = kTwoOverPi * log(0.5 * x);
-    h0 = zd{j0, fma(l, j0, r0)};
-    h1 = zd{j1, fma(l, j1, r1) - kTwoOverPi / x};
+    const double j1 = x * v[1];
+    h0 = zd{v[0], fma(l, v[0], v[2])};
+    h1 = zd{j1, fma(l, j1, x * v[3]) - kTwoOverPi / x};
     return;
   }
+  hankel01_large(x, h0, h1);
+}
+
+__attribute__((noinline)) IFGF_HD inline void hankel01_large(double x, zd& h0, zd& h1) {
+  double v[4];
   const double ix = 1.0 / x;
-  const double z = 64.0 * ix * ix;
-  const double t = fma(2.0, z, -1.0);
-  const double p0 = cheb_sum(kP0Large, t);
-  const double q0 = 8.0 * ix * cheb_sum(kQ0Large, t);
-  const double p1 = cheb_sum(kP1Large, t);
-  const double q1 = 8.0 * ix * cheb_sum(kQ1Large, t);
+  cheb_sums(kLarge4, fma(128.0 * ix, ix, -1.0), v);  // P0, Qt0, P1, Qt1
   double s, c;
   sincos(x, &s, &c);
   const double amp = sqrt(kTwoOverPi * ix) * 0.70710678118654752440;
   const zd e0{(c + s) * amp, (s - c) * amp};   // amp * exp(i(x - pi/4))
   const zd e1{e0.y, -e0.x};                    // amp * exp(i(x - 3pi/4))
-  h0 = cmul(zd{p0, q0}, e0);
-  h1 = cmul(zd{p1, q1}, e1);
+  h0 = cmul(zd{v[0], 8.0 * ix * v[1]}, e0);
+  h1 = cmul(zd{v[2], 8.0 * ix * v[3]}, e1);
 }
 
 }  // namespace ifgf

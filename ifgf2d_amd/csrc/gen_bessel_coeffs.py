@@ -90,6 +90,17 @@ def emit(name, coeffs):
     return "\n".join(lines)
 
 
+def emit_interleaved(name, cols):
+    n = max(len(c) for c in cols)
+    lines = [f"static constexpr int {name}_N = {n};",
+             f"IFGF_BESSEL_TABLE double {name}[{n}][{len(cols)}] = {{"]
+    for k in range(n):
+        vals = [repr(float(c[k])) if k < len(c) else "0.0" for c in cols]
+        lines.append("    {" + ", ".join(vals) + "},")
+    lines.append("};")
+    return "\n".join(lines)
+
+
 def main():
     tables = [
         ("kJ0Small", cheb_coeffs(j0_small)),
@@ -108,6 +119,13 @@ def main():
     out.write("#pragma once\n#ifndef IFGF_BESSEL_TABLE\n#define IFGF_BESSEL_TABLE static constexpr\n#endif\n\n")
     for name, coeffs in tables:
         out.write(emit(name, coeffs) + "\n\n")
+    t = dict(tables)
+    # interleaved tables for the multi-chain Clenshaw evaluation:
+    # small x: (J0, R0) and (J0, J1/x, R0, R1/x); large x: (P0, Qt0) and (P0, Qt0, P1, Qt1)
+    out.write(emit_interleaved("kSmall2", [t["kJ0Small"], t["kR0Small"]]) + "\n\n")
+    out.write(emit_interleaved("kSmall4", [t["kJ0Small"], t["kJ1Small"], t["kR0Small"], t["kR1Small"]]) + "\n\n")
+    out.write(emit_interleaved("kLarge2", [t["kP0Large"], t["kQ0Large"]]) + "\n\n")
+    out.write(emit_interleaved("kLarge4", [t["kP0Large"], t["kQ0Large"], t["kP1Large"], t["kQ1Large"]]) + "\n\n")
 
 
 if __name__ == "__main__":

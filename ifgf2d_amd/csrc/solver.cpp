@@ -10,7 +10,10 @@ namespace ifgf {
 namespace {
 
 // Reflector H = I - tau [1; v][1; v]^H mapping a (length m) to beta e1, with
-// beta = -sign(Re a0) ||a|| (Eigen's makeHouseholder convention).
+// beta = -sign(Re a0) ||a|| (Eigen's makeHouseholder convention). H is unitary
+// but not Hermitian for complex a0, so the Krylov vectors and the solution are
+// built with H^H = H^-1 (opGMRES.hpp:147-149 and :224-229 apply H as written,
+// which for complex data is a consistent but not residual-minimising iteration).
 struct Reflector {
   std::vector<cplx> v;
   cplx tau{0.0, 0.0};
@@ -34,15 +37,17 @@ double make_reflector(const cplx* a, long long m, Reflector& r) {
   return beta;
 }
 
-void reflect(cplx* a, long long m, const Reflector& r) {
+// a <- H a, or a <- H^H a (= H^-1 a) when adjoint is set.
+void reflect(cplx* a, long long m, const Reflector& r, bool adjoint = false) {
+  const cplx tau = adjoint ? std::conj(r.tau) : r.tau;
   if (m == 1) {
-    a[0] *= (1.0 - r.tau);
+    a[0] *= (1.0 - tau);
     return;
   }
-  if (r.tau == cplx(0.0)) return;
+  if (tau == cplx(0.0)) return;
   cplx s = a[0];
   for (long long i = 1; i < m; i++) s += std::conj(r.v[i - 1]) * a[i];
-  const cplx ts = r.tau * s;
+  const cplx ts = tau * s;
   a[0] -= ts;
   for (long long i = 1; i < m; i++) a[i] -= r.v[i - 1] * ts;
 }
@@ -127,7 +132,7 @@ bool householder_gmres(const ApplyFn& A, long long n, const cplx* rhs, cplx* x, 
     // v = P_1 ... P_k e_k
     std::fill(v.begin(), v.end(), cplx(0.0));
     v[k - 1] = 1.0;
-    for (int i = k - 1; i >= 0; --i) reflect(&v[i], n - i, H[i]);
+    for (int i = k - 1; i >= 0; --i) reflect(&v[i], n - i, H[i], true);
     A(v.data(), t.data());
     v.swap(t);
     for (int i = 0; i < k; ++i) reflect(&v[i], n - i, H[i]);
@@ -156,7 +161,7 @@ bool householder_gmres(const ApplyFn& A, long long n, const cplx* rhs, cplx* x, 
       std::fill(dx.begin(), dx.end(), cplx(0.0));
       for (int i = k - 1; i >= 0; --i) {
         dx[i] += y[i];
-        reflect(&dx[i], n - i, H[i]);
+        reflect(&dx[i], n - i, H[i], true);
       }
       for (long long i = 0; i < n; i++) x[i] += dx[i];
       if (stop) return k == n || err < tol;

@@ -1183,6 +1183,10 @@ struct ForwardMap {
 // Householder GMRES, ForwardMap/opGMRES.hpp:68-269 (Walker 1988), identity
 // preconditioner; Householder reflectors and Givens rotations in the
 // convention of Eigen's makeHouseholder / JacobiRotation::makeGivens.
+// makeHouseholder builds a unitary, non-Hermitian H with H r = beta e1 for
+// complex r, so the Krylov vectors (opGMRES.hpp:147-149) and the solution
+// update (:224-229) are formed with H^H = H^-1; applying H there as written
+// gives a consistent but not residual-minimising iteration for complex data.
 struct House {
   std::vector<cd> ess;  // essential part (length m - i - 1)
   cd tau = 0.0;
@@ -1205,17 +1209,19 @@ static void make_householder(const cd* v, int n, House& h, double& beta) {
     h.tau = std::conj((beta - c0) / beta);
   }
 }
-// applyHouseholderOnTheLeft on v (length n)
-static void apply_householder(cd* v, int n, const House& h) {
+// applyHouseholderOnTheLeft on v (length n): v <- H v, H = I - tau [1;ess][1;ess]^H
+// (adjoint = true applies H^H = H^-1, i.e. conj(tau)).
+static void apply_householder(cd* v, int n, const House& h, bool adjoint = false) {
+  const cd tau = adjoint ? std::conj(h.tau) : h.tau;
   if (n == 1) {
-    v[0] *= (1.0 - h.tau);
+    v[0] *= (1.0 - tau);
     return;
   }
-  if (h.tau == 0.0) return;
+  if (tau == 0.0) return;
   cd t = v[0];
   for (int i = 1; i < n; i++) t += std::conj(h.ess[i - 1]) * v[i];
-  v[0] -= h.tau * t;
-  for (int i = 1; i < n; i++) v[i] -= h.tau * h.ess[i - 1] * t;
+  v[0] -= tau * t;
+  for (int i = 1; i < n; i++) v[i] -= tau * h.ess[i - 1] * t;
 }
 struct Givens {
   cd c = 1.0, s = 0.0;
@@ -1294,7 +1300,7 @@ bool gmres(const std::function<void(const std::vector<cd>&, std::vector<cd>&)>& 
     ++iters;
     std::fill(v.begin(), v.end(), cd(0.0));
     v[k - 1] = 1.0;
-    for (int i = k - 1; i >= 0; --i) apply_householder(&v[i], m - i, H[i]);
+    for (int i = k - 1; i >= 0; --i) apply_householder(&v[i], m - i, H[i], true);
     A(v, t);
     v = t;
     for (int i = 0; i < k; ++i) apply_householder(&v[i], m - i, H[i]);
@@ -1323,7 +1329,7 @@ bool gmres(const std::function<void(const std::vector<cd>&, std::vector<cd>&)>& 
       std::fill(xn.begin(), xn.end(), cd(0.0));
       for (int i = k - 1; i >= 0; --i) {
         xn[i] += y[i];
-        apply_householder(&xn[i], m - i, H[i]);
+        apply_householder(&xn[i], m - i, H[i], true);
       }
       for (int i = 0; i < m; i++) x[i] += xn[i];
       if (stop) return true;

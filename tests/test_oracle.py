@@ -175,3 +175,35 @@ def test_mie_series_solve():
     th = 2 * np.pi * np.arange(64) / 64
     us = fm.propagate(3 * np.cos(th), 3 * np.sin(th), dens)
     assert rel(us, mie(3.0, th, 10.0)) < 1e-5
+
+
+def test_gmres_is_residual_minimising():
+    """The Householder GMRES residual after k steps equals the minimum over the
+    Krylov space (computed by Arnoldi + least squares), for complex data."""
+    rng = np.random.default_rng(5)
+    n = 120
+    A = np.eye(n) + 0.9 * (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) / np.sqrt(n)
+    b = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    V = np.zeros((n, 41), complex)
+    H = np.zeros((41, 40), complex)
+    beta = np.linalg.norm(b)
+    V[:, 0] = b / beta
+    best = []
+    for j in range(40):
+        w = A @ V[:, j]
+        for _ in range(2):
+            for i in range(j + 1):
+                c = np.vdot(V[:, i], w)
+                H[i, j] += c
+                w = w - c * V[:, i]
+        H[j + 1, j] = np.linalg.norm(w)
+        V[:, j + 1] = w / H[j + 1, j]
+        e = np.zeros(j + 2, complex)
+        e[0] = beta
+        y = np.linalg.lstsq(H[: j + 2, : j + 1], e, rcond=None)[0]
+        best.append(np.linalg.norm(e - H[: j + 2, : j + 1] @ y) / beta)
+    for k in (1, 2, 5, 10, 40):
+        x, it, err, rc = O.gmres_dense(A, b, restart=40, tol=1e-14, maxit=k)
+        true = np.linalg.norm(b - A @ x) / np.linalg.norm(b)
+        assert abs(true - best[k - 1]) < 1e-10 * max(1.0, best[k - 1])
+        assert abs(err - true) < 1e-10
