@@ -70,25 +70,27 @@ __attribute__((noinline)) IFGF_HD inline zd hankel0_large(double x) {
 
 // x > 8 branch of hankel01, kept out of line: the kernels' hot arguments lie
 // below the split, and inlining the sincos path raises their register use.
-__attribute__((noinline)) IFGF_HD inline void hankel01_large(double x, zd& h0, zd& h1);
+__attribute__((noinline)) IFGF_HD inline void hankel01_large(double x, double ix, zd& h0, zd& h1);
 
-// H0^(1)(x) and H1^(1)(x) together (double-layer / combined-field kernels). x > 0.
-IFGF_HD inline void hankel01(double x, zd& h0, zd& h1) {
+// H0^(1)(x) and H1^(1)(x) together (double-layer / combined-field kernels).
+// x > 0, ix = 1/x (callers usually have it from a reciprocal square root).
+IFGF_HD inline void hankel01(double x, double ix, zd& h0, zd& h1) {
   double v[4];
   if (x <= kBesselSplit) {
     cheb_sums(kSmall4, fma(x * x, 1.0 / 32.0, -1.0), v);  // J0, J1/x, R0, R1/x
     const double l = kTwoOverPi * log(0.5 * x);
     const double j1 = x * v[1];
     h0 = zd{v[0], fma(l, v[0], v[2])};
-    h1 = zd{j1, fma(l, j1, x * v[3]) - kTwoOverPi / x};
+    h1 = zd{j1, fma(l, j1, x * v[3]) - kTwoOverPi * ix};
     return;
   }
-  hankel01_large(x, h0, h1);
+  hankel01_large(x, ix, h0, h1);
 }
 
-__attribute__((noinline)) IFGF_HD inline void hankel01_large(double x, zd& h0, zd& h1) {
+IFGF_HD inline void hankel01(double x, zd& h0, zd& h1) { hankel01(x, 1.0 / x, h0, h1); }
+
+__attribute__((noinline)) IFGF_HD inline void hankel01_large(double x, double ix, zd& h0, zd& h1) {
   double v[4];
-  const double ix = 1.0 / x;
   cheb_sums(kLarge4, fma(128.0 * ix, ix, -1.0), v);  // P0, Qt0, P1, Qt1
   double s, c;
   sincos(x, &s, &c);

@@ -18,18 +18,21 @@ namespace ifgf {
 
 enum Formulation : int { kSingleLayer = 0, kDoubleLayer = 1, kCombined = 2 };
 
-// Kernel from the separation d = x - y and the unit source normal n.
+// Kernel from the separation d = x - y and the unit source normal n. One
+// reciprocal square root gives r, 1/r and 1/(k r).
 template <int F>
 IFGF_HD inline zd green_d(double dx, double dy, double nx, double ny, double k, double eta) {
-  const double r = sqrt(dx * dx + dy * dy);
-  if (r < 1e-14) return zd{0.0, 0.0};  // GreenFunctions.hpp:97-100
+  const double r2 = dx * dx + dy * dy;
+  if (r2 < 1e-28) return zd{0.0, 0.0};  // r < 1e-14, GreenFunctions.hpp:97-100
+  const double ri = rsqrt_hd(r2);
+  const double r = r2 * ri;
   if (F == kSingleLayer) {
     const zd h0 = hankel0(k * r);
     return zd{-0.25 * h0.y, 0.25 * h0.x};
   } else {
     zd h0, h1;
-    hankel01(k * r, h0, h1);
-    const double c = 0.25 * k * (dx * nx + dy * ny) / r;
+    hankel01(k * r, ri / k, h0, h1);
+    const double c = 0.25 * k * (dx * nx + dy * ny) * ri;
     zd out{-c * h1.y, c * h1.x};  // i * c * H1
     if (F == kCombined) {
       out.x = fma(0.25 * eta, h0.x, out.x);  // -i eta (i/4) H0 = (eta/4) H0

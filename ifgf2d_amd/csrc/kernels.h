@@ -124,7 +124,19 @@ __global__ void __launch_bounds__(256) k_gather(int N, const zd* __restrict__ x,
   rho[i] = cscale(x[o], w[o]);
 }
 
-template <int F>
+// Sum of S lanes' partial results (lanes i*S .. i*S+S-1 of a wave), fixed order.
+template <int S>
+__device__ inline zd lane_group_sum(zd v) {
+#pragma unroll
+  for (int o = S / 2; o > 0; o >>= 1) {
+    v.x += __shfl_xor(v.x, o, S);
+    v.y += __shfl_xor(v.y, o, S);
+  }
+  return v;
+}
+
+// S lanes per target: lane `sl` takes every S-th source of the neighbour boxes.
+template <int F, int S>
 __global__ void __launch_bounds__(256)
     k_near(int N, const double* __restrict__ px, const double* __restrict__ py,
            const double* __restrict__ pnx, const double* __restrict__ pny,
@@ -132,26 +144,31 @@ __global__ void __launch_bounds__(256)
            const int* __restrict__ leaf_of_point, const int* __restrict__ leaf_begin,
            const int* __restrict__ nb_ptr, const int* __restrict__ nb_list,
            const zd* __restrict__ rho, zd* __restrict__ sol, double k, double eta) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
-  const double tx = px[i], ty = py[i];
-  const int4 e = excl[i];
-  const int b = leaf_of_point[i];
+  const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = gt / S, sl = gt % S;
   zd acc{0.0, 0.0};
-  for (int n = nb_ptr[b]; n < nb_ptr[b + 1]; n++) {
-    const int nb = nb_list[n];
-    const int s1 = leaf_begin[nb + 1];
-    for (int s = leaf_begin[nb]; s < s1; s++) {
-      const int ps = patch[s];
-      if (ps == e.x || ps == e.y || ps == e.z || ps == e.w) continue;
-      const zd g = green<F>(tx, ty, px[s], py[s], pnx[s], pny[s], k, eta);
-      cfma(acc, g, rho[s]);
+  if (i < N) {
+    const double tx = px[i], ty = py[i];
+    const int4 e = excl[i];
+    const int b = leaf_of_point[i];
+    for (int n = nb_ptr[b]; n < nb_ptr[b + 1]; n++) {
+      const int nb = nb_list[n];
+      const int s1 = leaf_begin[nb + 1];
+      for (int s = leaf_begin[nb] + sl; s < s1; s += S) {
+        const int ps = patch[s];
+        if (ps == e.x || ps == e.y || ps == e.z || ps == e.w) continue;
+        const zd g = green_d<F>(tx - px[s], ty - py[s], pnx[s], pny[s], k, eta);
+        cfma(acc, g, rho[s]);
+      }
     }
   }
-  sol[i] = acc;
+  acc = lane_group_sum<S>(acc);
+  if (i < N && sl == 0) sol[i] = acc;
 }
 
-template <int F, int Ps, int Pang>
+// Threads (cone, node, slice): S lanes per cone node share the box's sources;
+// CPB cones per 256-thread block.
+template <int F, int Ps, int Pang, int S>
 __global__ void __launch_bounds__(256)
     k_leaf(int ncone, int cone_off, const ConeGeo* __restrict__ cg, const int* __restrict__ cone_box,
            const int* __restrict__ leaf_begin, const double* __restrict__ px,
@@ -159,34 +176,41 @@ __global__ void __launch_bounds__(256)
            const double* __restrict__ pny, const zd* __restrict__ rho, zd* __restrict__ coef,
            const ChebTables* __restrict__ tabs, double k, double eta) {
   constexpr int P = Ps * Pang;
-  constexpr int CPB = 256 / P;
+  constexpr int CPB = 256 / (P * S);
   __shared__ zd buf[CPB * P];
   __shared__ zd tmp[CPB * P];
   __shared__ ChebTables T;
   for (int q = threadIdx.x; q < (int)(sizeof(ChebTables) / sizeof(double)); q += blockDim.x)
     ((double*)&T)[q] = ((const double*)tabs)[q];
   __syncthreads();
-  const int cl = threadIdx.x / P, j = threadIdx.x % P;
+  const int cl = threadIdx.x / (P * S);
+  const int rem = threadIdx.x % (P * S);
+  const int j = rem / S, sl = rem % S;
   const int c = blockIdx.x * CPB + cl;
   const bool active = cl < CPB && c < ncone;
   const int g = cone_off + c;
+  zd acc{0.0, 0.0};
+  double r = 1.0;
   if (active) {
     const ConeGeo G = cg[g];
-    double nx, ny, r;
+    double nx, ny;
     cone_node(G, T.xs[j % Ps], T.xa[j / Ps], nx, ny, r);
     const int b = cone_box[g];
-    zd acc{0.0, 0.0};
-    for (int s = leaf_begin[b]; s < leaf_begin[b + 1]; s++) {
-      const zd kv = green<F>(nx, ny, px[s], py[s], pnx[s], pny[s], k, eta);
+    const int s1 = leaf_begin[b + 1];
+    for (int s = leaf_begin[b] + sl; s < s1; s += S) {
+      const zd kv = green_d<F>(nx - px[s], ny - py[s], pnx[s], pny[s], k, eta);
       cfma(acc, kv, rho[s]);
     }
-    buf[threadIdx.x] = cmul(acc, inv_centre_factor(r, k));
   }
+  acc = lane_group_sum<S>(acc);
+  if (active && sl == 0) buf[cl * P + j] = cmul(acc, inv_centre_factor(r, k));
   __syncthreads();
-  const int base = cl * P;
-  const zd out = cone_transform<Ps, Pang>(buf + (active ? base : 0), tmp + (active ? base : 0), T,
-                                          active ? j : 0, active);
-  if (active) coef[(size_t)g * P + j] = out;
+  // transform: the first CPB*P threads take (cone, node) = (t / P, t % P)
+  const int tc = threadIdx.x / P, tj = threadIdx.x % P;
+  const bool tact = tc < CPB && blockIdx.x * CPB + tc < ncone;
+  const zd out = cone_transform<Ps, Pang>(buf + (tact ? tc * P : 0), tmp + (tact ? tc * P : 0), T,
+                                          tact ? tj : 0, tact);
+  if (tact) coef[(size_t)(cone_off + blockIdx.x * CPB + tc) * P + tj] = out;
 }
 
 // Propagation into the parent cones of level l-1 from the child cones of
@@ -240,27 +264,31 @@ __global__ void __launch_bounds__(256)
   if (active) coef[(size_t)g * P + j] = out;
 }
 
-template <int Ps, int Pang>
+template <int Ps, int Pang, int S>
 __global__ void __launch_bounds__(256)
     k_interp(int N, const double* __restrict__ px, const double* __restrict__ py,
              const long long* __restrict__ it_ptr, const int* __restrict__ it_cone,
              const ConeGeo* __restrict__ cg, const zd* __restrict__ coef, zd* __restrict__ sol,
              double k) {
   constexpr int P = Ps * Pang;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= N) return;
-  const double x = px[i], y = py[i];
+  const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = gt / S, sl = gt % S;
   zd acc{0.0, 0.0};
-  const long long e1 = it_ptr[i + 1];
-  for (long long e = it_ptr[i]; e < e1; e++) {
-    const int g = it_cone[e];
-    const ConeGeo C = cg[g];
-    double s, t, rad;
-    cone_local(C, x, y, s, t, rad);
-    const zd v = cone_eval<Ps, Pang>(coef + (size_t)g * P, s, t);
-    cfma(acc, v, centre_factor(rad, k));
+  if (i < N) {
+    const double x = px[i], y = py[i];
+    const long long e1 = it_ptr[i + 1];
+#pragma unroll 1
+    for (long long e = it_ptr[i] + sl; e < e1; e += S) {
+      const int g = it_cone[e];
+      const ConeGeo C = cg[g];
+      double s, t, rad;
+      cone_local(C, x, y, s, t, rad);
+      const zd v = cone_eval<Ps, Pang>(coef + (size_t)g * P, s, t);
+      cfma(acc, v, centre_factor(rad, k));
+    }
   }
-  sol[i] = cadd(sol[i], acc);
+  acc = lane_group_sum<S>(acc);
+  if (i < N && sl == 0) sol[i] = cadd(sol[i], acc);
 }
 
 __global__ void __launch_bounds__(256)

@@ -207,10 +207,19 @@ long long ifgf_plan_correction(const ifgf_plan* p, long long* ptr, int* patch, d
 }  // extern "C"
 
 // ------------------------------------------------------------------ apply ---
+// lanes that share one target (near field, interpolation) or one leaf cone node
+constexpr int kNearLanes = 8;
+constexpr int kInterpLanes = 4;
+template <int P>
+struct LeafLanes {
+  static constexpr int value = (P * 2 <= 256) ? 2 : 1;
+};
+
 template <int F, int Ps, int Pang>
 static void launch_ifgf(ifgf_plan* P, const zd* x, zd* y, hipStream_t st) {
   constexpr int Pn = Ps * Pang;
   constexpr int CPB = 256 / Pn;
+  constexpr int kLeafLanes = LeafLanes<Pn>::value;
   const Tree& t = P->tree;
   const int N = (int)t.N, D = t.D;
   const double k = P->prm.k, eta = P->prm.eta;
@@ -229,12 +238,13 @@ static void launch_ifgf(ifgf_plan* P, const zd* x, zd* y, hipStream_t st) {
   }
   hipLaunchKernelGGL(k_gather, dim3(nb), dim3(256), 0, st, N, x, P->w.p, P->perm.p, P->rho.p);
   if (prof) HIP_TRY(hipEventRecord(ev[1], st));
-  hipLaunchKernelGGL((k_near<F>), dim3(nb), dim3(256), 0, st, N, P->px.p, P->py.p, P->pnx.p, P->pny.p,
+  hipLaunchKernelGGL((k_near<F, kNearLanes>), dim3((N + 256 / kNearLanes - 1) / (256 / kNearLanes)), dim3(256), 0, st, N, P->px.p, P->py.p, P->pnx.p, P->pny.p,
                      P->patch.p, P->excl_d.p, P->leaf_of_point.p, P->leaf_begin.p, P->nb_ptr.p,
                      P->nb_list.p, P->rho.p, P->sol.p, k, eta);
   if (prof) HIP_TRY(hipEventRecord(ev[2], st));
   const LevelInfo& LD = t.lv[D];
-  hipLaunchKernelGGL((k_leaf<F, Ps, Pang>), dim3((LD.ncone + CPB - 1) / CPB), dim3(256), 0, st,
+  constexpr int LCPB = 256 / (Pn * kLeafLanes);
+  hipLaunchKernelGGL((k_leaf<F, Ps, Pang, kLeafLanes>), dim3((LD.ncone + LCPB - 1) / LCPB), dim3(256), 0, st,
                      LD.ncone, LD.cone_off, P->cg.p, P->cone_box.p, P->leaf_begin.p, P->px.p, P->py.p,
                      P->pnx.p, P->pny.p, P->rho.p, P->coef.p, P->tabs.p, k, eta);
   if (prof) HIP_TRY(hipEventRecord(ev[3], st));
@@ -247,7 +257,7 @@ static void launch_ifgf(ifgf_plan* P, const zd* x, zd* y, hipStream_t st) {
                        P->coef.p, P->tabs.p, k);
   }
   if (prof) HIP_TRY(hipEventRecord(ev[4], st));
-  hipLaunchKernelGGL((k_interp<Ps, Pang>), dim3(nb), dim3(256), 0, st, N, P->px.p, P->py.p, P->it_ptr.p,
+  hipLaunchKernelGGL((k_interp<Ps, Pang, kInterpLanes>), dim3((N + 256 / kInterpLanes - 1) / (256 / kInterpLanes)), dim3(256), 0, st, N, P->px.p, P->py.p, P->it_ptr.p,
                      P->it_cone.p, P->cg.p, P->coef.p, P->sol.p, k);
   if (prof) HIP_TRY(hipEventRecord(ev[5], st));
   const double ident = (F == kSingleLayer) ? 0.0 : 0.5;

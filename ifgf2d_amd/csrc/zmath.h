@@ -17,6 +17,15 @@ struct zf {
   float x, y;
 };
 
+// 1/sqrt(x): the device's reciprocal square root; on the host 1/sqrt.
+IFGF_HD inline double rsqrt_hd(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return rsqrt(x);
+#else
+  return 1.0 / sqrt(x);
+#endif
+}
+
 IFGF_HD inline zd cadd(zd a, zd b) { return zd{a.x + b.x, a.y + b.y}; }
 IFGF_HD inline zd csub(zd a, zd b) { return zd{a.x - b.x, a.y - b.y}; }
 IFGF_HD inline zd cmul(zd a, zd b) { return zd{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
