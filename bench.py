@@ -6,7 +6,10 @@ k = 200, 6,400 equal-parameter patches x 10 Fejer nodes (N = 64,000),
 combined-field operator (eta = k), 5 x 5 Chebyshev nodes per cone, 10 tree
 levels, all FP64. One step = one forward-map application (the operator GMRES
 applies once per iteration) on a seeded complex density already resident in
-HBM. After the timed steps, one GMRES(50) solve to 1e-8 for the plane wave
+HBM. The near-field kernel values and the folded leaf operator depend only on
+the geometry and are evaluated once at plan creation (counted in
+plan_setup_s); the on-the-fly variant that re-evaluates them in every
+application is timed too (on_the_fly_matvecs_per_s). After the timed steps, one GMRES(50) solve to 1e-8 for the plane wave
 d = (cos pi/4, sin pi/4) gives the time to solve.
 
 Prints ONE JSON line on stdout; progress goes to stderr.
@@ -28,10 +31,24 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def kernel_flops_per_pair(formulation):
-    # kernel generation (SLP 40, DLP/CFIE 60 flops: H0 [+ H1] + r and the
-    # normal product) + one complex multiply-add (8 flops)
-    return (40 if formulation == 0 else 60) + 8
+def stage_work(st, formulation, P, Ps, Pang, stored):
+    """Algorithmic work of each stage of one application: (bound, amount, unit).
+
+    Stored operators stream 16 bytes per near pair / leaf evaluation from HBM;
+    kernel generation costs 40 (single layer) or 60 (double layer / combined)
+    FLOP per pair plus 8 per complex multiply-add; a cone interpolation costs
+    4P FLOP plus 20 for its centre factor; a cone transform 4P(Ps+Pang)."""
+    gen = 40 if formulation == 0 else 60
+    w = {}
+    if stored:
+        w["near"] = ("hbm", 16.0 * st["near_stored"], "B")
+        w["leaf"] = ("hbm", 16.0 * st["leaf"], "B")
+    else:
+        w["near"] = ("mfma", (gen + 8.0) * st["near"], "FLOP")
+        w["leaf"] = ("mfma", (gen + 8.0) * st["leaf"] + 4.0 * P * (Ps + Pang) * st["leaf_cones"], "FLOP")
+    w["propagation"] = ("mfma", (4.0 * P + 20) * st["prop"] + 4.0 * P * (Ps + Pang) * st["prop_cones"], "FLOP")
+    w["interpolation"] = ("mfma", (4.0 * P + 20) * st["interp"], "FLOP")
+    return w
 
 
 def cpu_baseline(kw, x):
@@ -60,6 +77,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--no-on-the-fly", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the forward map is "
@@ -106,15 +124,39 @@ def main():
     log(f"{args.steps} applications in {t_run:.3f} s -> {value:.1f} matvecs/s; stages (ms/apply) "
         + ", ".join(f"{k_} {v:.4f}" for k_, v in stage_avg.items()))
 
-    # dominant kernel and its roofline (algorithmic FP64 flops per launch)
-    per_pair = kernel_flops_per_pair(fm.formulation)
-    work = {"near": st["near"] * per_pair, "leaf": st["leaf"] * per_pair}
-    dom = max(("near", "leaf"), key=lambda s: stage_avg[s])
-    achieved = work[dom] / (stage_avg[dom] * 1e-3) / 1e12
-    roofline = {"bound": "mfma", "kernel": "k_near" if dom == "near" else "k_leaf",
-                "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None,
-                "flops_per_launch": work[dom], "avg_launch_ms": stage_avg[dom]}
+    # dominant stage and its roofline (algorithmic bytes or FLOP per application)
+    P = 25
+    ext = fm.extra_stats()
+    work = stage_work(dict(st, **ext), fm.formulation, P, 5, 5, ext["stored"])
+    dom = max(work, key=lambda s: stage_avg[s])
+    bound, amount, unit = work[dom]
+    secs = stage_avg[dom] * 1e-3
+    if bound == "hbm":
+        achieved, peak, runit = amount / secs / 1e9, PEAK_HBM_GBS, "GB/s"
+    else:
+        achieved, peak, runit = amount / secs / 1e12, PEAK_FP64_TFLOPS, "TFLOP/s"
+    kname = {"near": "k_near_apply" if ext["stored"] else "k_near",
+             "leaf": "k_leaf_apply" if ext["stored"] else "k_leaf",
+             "propagation": "k_prop", "interpolation": "k_interp"}[dom]
+    roofline = {"bound": bound, "kernel": kname, "achieved": round(achieved, 3), "peak": peak,
+                "unit": runit, "frac": round(achieved / peak, 4), "traffic": None,
+                "algorithmic_per_launch": amount, "algorithmic_unit": unit,
+                "avg_launch_ms": stage_avg[dom]}
+    # the on-the-fly variant of the same operator (kernels re-evaluated per application)
+    otf = None
+    if not args.no_on_the_fly:
+        fo = I.ForwardMap(**dict(kw, on_the_fly=True))
+        for _ in range(3):
+            fo.apply_device(xs.data_ptr(), ys.data_ptr())
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        nrep = max(1, args.steps // 5)
+        for _ in range(nrep):
+            fo.apply_device(xs.data_ptr(), ys.data_ptr())
+        torch.cuda.synchronize()
+        otf = round(nrep / (time.perf_counter() - t0), 3)
+        log(f"on-the-fly kernels: {otf} matvecs/s")
+        del fo
     traffic_file = os.path.join(ROOT, "profiles", f"traffic_{roofline['kernel']}_kite_k200.json")
     if os.path.exists(traffic_file):
         roofline["traffic"] = json.load(open(traffic_file)).get("bytes_per_launch")
@@ -145,6 +187,8 @@ def main():
         "stages_ms": {k_: round(v, 5) for k_, v in stage_avg.items()},
         "work": {k_: st[k_] for k_ in ("near", "leaf", "prop", "interp", "cones", "corr")},
         "plan_setup_s": round(t_setup, 3),
+        "stored_operator_bytes": ext["stored_bytes"],
+        "on_the_fly_matvecs_per_s": otf,
         "solve": solve,
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -37,7 +37,7 @@ _FORMS = {"single": SINGLE_LAYER, "double": DOUBLE_LAYER, "combined": COMBINED,
 _PREC = {"fp64": FP64, "mixed": MIXED, FP64: FP64, MIXED: MIXED}
 
 IFGF_OK, IFGF_ERR_NOT_CONVERGED = 0, 6
-IFGF_PLAN_HOST_ONLY, IFGF_PTR_DEVICE = 1, 1
+IFGF_PLAN_HOST_ONLY, IFGF_PLAN_ON_THE_FLY, IFGF_PTR_DEVICE = 1, 2, 1
 
 
 class IFGFError(RuntimeError):
@@ -170,10 +170,13 @@ class ForwardMap:
     by precomps_and_setup(k, nlevels) (ForwardMap.tpp:15-35, 542-565).
     formulation: "single" (the reference's SingleLayer S), "double" (1/2 I + K)
     or "combined" (1/2 I + K - i eta S, the sound-soft CFIE).
+    on_the_fly: re-evaluate the near-field and leaf kernels in every application
+    instead of storing them once at plan creation (less memory, slower).
     """
 
     def __init__(self, curve, patch_lims, k, nlevels, Np=10, formulation="combined", eta=None,
-                 delta=None, p=4.0, Ps=5, Pang=5, precision="fp64", nrhs_max=1, host_only=False):
+                 delta=None, p=4.0, Ps=5, Pang=5, precision="fp64", nrhs_max=1, host_only=False,
+                 on_the_fly=False):
         L = lib()
         self.curve = curve
         self.lims = np.ascontiguousarray(patch_lims, dtype=np.float64)
@@ -184,8 +187,9 @@ class ForwardMap:
         self._cp = np.ascontiguousarray(curve[1], dtype=np.float64)
         surf = _Surface(curve[0], _d(self._cp), self._cp.size, _d(self.lims), self.lims.size - 1,
                         Np, self.delta, float(p))
+        flags = (IFGF_PLAN_HOST_ONLY if host_only else 0) | (IFGF_PLAN_ON_THE_FLY if on_the_fly else 0)
         prm = _Params(self.formulation, _PREC[precision], self.k, self.eta, Ps, Pang, nlevels,
-                      nrhs_max, IFGF_PLAN_HOST_ONLY if host_only else 0)
+                      nrhs_max, flags)
         h = ctypes.c_void_p()
         rc = L.ifgf_plan_create(ctypes.byref(surf), ctypes.byref(prm), ctypes.byref(h))
         if rc != IFGF_OK:
@@ -216,11 +220,20 @@ class ForwardMap:
         self._check(lib().ifgf_plan_points(self._h, _d(x), _d(y), _d(nx), _d(ny), _d(w)), "points")
         return x, y, nx, ny, w
 
-    def stats(self):
-        s = np.empty(8, dtype=np.int64)
+    def _stats(self):
+        s = np.empty(12, dtype=np.int64)
         self._check(lib().ifgf_plan_stats(self._h, s.ctypes.data_as(_lp)), "stats")
+        return s.tolist()
+
+    def stats(self):
+        """Work counts of one application (near pairs, leaf evaluations, ...)."""
         return dict(zip(("near", "leaf", "prop", "interp", "cones", "corr", "nlevels", "N"),
-                        s.tolist()))
+                        self._stats()[:8]))
+
+    def extra_stats(self):
+        s = self._stats()
+        return {"leaf_cones": s[8], "prop_cones": s[4] - s[8], "stored": bool(s[9]),
+                "near_stored": s[10], "stored_bytes": s[11]}
 
     def sorting(self):
         s = np.empty(self.N, dtype=np.int64)

@@ -213,6 +213,154 @@ __global__ void __launch_bounds__(256)
   if (tact) coef[(size_t)(cone_off + blockIdx.x * CPB + tc) * P + tj] = out;
 }
 
+// ---------------------------------------------------------------------------
+// Stored-operator path. The near-field kernel values and the leaf operator
+// (kernel -> division by the centre factor -> 2D Chebyshev transform) depend
+// only on the geometry; they are evaluated once per plan and each application
+// then streams them from HBM.
+//
+// Near field: per target i a contiguous segment at nf_off[i] holding one
+// value per (neighbour box ascending, source ascending) pair, 0 for excluded
+// pairs (same loop order as k_near).
+template <int F, int S>
+__global__ void __launch_bounds__(256)
+    k_near_build(int N, const double* __restrict__ px, const double* __restrict__ py,
+                 const double* __restrict__ pnx, const double* __restrict__ pny,
+                 const int* __restrict__ patch, const int4* __restrict__ excl,
+                 const int* __restrict__ leaf_of_point, const int* __restrict__ leaf_begin,
+                 const int* __restrict__ nb_ptr, const int* __restrict__ nb_list,
+                 const long long* __restrict__ nf_off, zd* __restrict__ nf_val, double k, double eta) {
+  const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = gt / S, sl = gt % S;
+  if (i >= N) return;
+  const double tx = px[i], ty = py[i];
+  const int4 e = excl[i];
+  const int b = leaf_of_point[i];
+  zd* out = nf_val + nf_off[i];
+  int base = 0;
+  for (int n = nb_ptr[b]; n < nb_ptr[b + 1]; n++) {
+    const int nb = nb_list[n];
+    const int s0 = leaf_begin[nb], s1 = leaf_begin[nb + 1];
+    for (int s = s0 + sl; s < s1; s += S) {
+      const int ps = patch[s];
+      zd g{0.0, 0.0};
+      if (!(ps == e.x || ps == e.y || ps == e.z || ps == e.w))
+        g = green_d<F>(tx - px[s], ty - py[s], pnx[s], pny[s], k, eta);
+      out[base + (s - s0)] = g;
+    }
+    base += s1 - s0;
+  }
+}
+
+template <int S>
+__global__ void __launch_bounds__(256)
+    k_near_apply(int N, const int* __restrict__ leaf_of_point, const int* __restrict__ leaf_begin,
+                 const int* __restrict__ nb_ptr, const int* __restrict__ nb_list,
+                 const long long* __restrict__ nf_off, const zd* __restrict__ nf_val,
+                 const zd* __restrict__ rho, zd* __restrict__ sol) {
+  const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = gt / S, sl = gt % S;
+  zd acc{0.0, 0.0};
+  if (i < N) {
+    const int b = leaf_of_point[i];
+    const zd* v = nf_val + nf_off[i];
+    int base = 0;
+    for (int n = nb_ptr[b]; n < nb_ptr[b + 1]; n++) {
+      const int nb = nb_list[n];
+      const int s0 = leaf_begin[nb], s1 = leaf_begin[nb + 1];
+      for (int s = s0 + sl; s < s1; s += S) cfma(acc, v[base + (s - s0)], rho[s]);
+      base += s1 - s0;
+    }
+  }
+  acc = lane_group_sum<S>(acc);
+  if (i < N && sl == 0) sol[i] = acc;
+}
+
+// Leaf operator L_c (P x n_src per leaf cone c), stored source-major at
+// lf_off[c]: lf[src * P + m]. Step 1: kernel values divided by the node's
+// centre factor, one thread per (cone, node, source).
+template <int F, int Ps, int Pang>
+__global__ void __launch_bounds__(256)
+    k_leaf_build_values(int ncone, int cone_off, const ConeGeo* __restrict__ cg,
+                        const int* __restrict__ cone_box, const int* __restrict__ leaf_begin,
+                        const long long* __restrict__ lf_off, const long long* __restrict__ item_off,
+                        const double* __restrict__ px, const double* __restrict__ py,
+                        const double* __restrict__ pnx, const double* __restrict__ pny,
+                        const ChebTables* __restrict__ T, zd* __restrict__ tmp, long long nitems,
+                        double k, double eta) {
+  constexpr int P = Ps * Pang;
+  const long long it = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (it >= nitems) return;
+  // item_off[c] = first item of cone c (items = P * n_src per cone)
+  int lo = 0, hi = ncone - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (item_off[mid] <= it) lo = mid;
+    else hi = mid - 1;
+  }
+  const int c = lo;
+  const int g = cone_off + c;
+  const int b = cone_box[g];
+  const int s0 = leaf_begin[b];
+  const long long local = it - item_off[c];
+  const int j = (int)(local % P);
+  const int src = (int)(local / P);
+  const ConeGeo G = cg[g];
+  double nx, ny, r;
+  cone_node(G, T->xs[j % Ps], T->xa[j / Ps], nx, ny, r);
+  const int s = s0 + src;
+  const zd kv = green_d<F>(nx - px[s], ny - py[s], pnx[s], pny[s], k, eta);
+  tmp[lf_off[c] + (long long)src * P + j] = cmul(kv, inv_centre_factor(r, k));
+}
+
+// Step 2: the 2D Chebyshev transform of every (cone, source) column.
+template <int Ps, int Pang>
+__global__ void __launch_bounds__(256)
+    k_leaf_build_transform(long long ncol, const zd* __restrict__ tmp, const ChebTables* __restrict__ T,
+                           zd* __restrict__ lf) {
+  constexpr int P = Ps * Pang;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ncol * P) return;
+  const long long col = t / P;
+  const int m = (int)(t % P);
+  const int n = m / Ps, ms = m % Ps;
+  const zd* in = tmp + col * P;
+  zd acc{0.0, 0.0};
+  for (int l = 0; l < Pang; l++) {
+    zd row{0.0, 0.0};
+    for (int kk = 0; kk < Ps; kk++) {
+      const zd v = in[l * Ps + kk];
+      row.x = fma(v.x, T->Ts[ms * 16 + kk], row.x);
+      row.y = fma(v.y, T->Ts[ms * 16 + kk], row.y);
+    }
+    acc.x = fma(row.x, T->Ta[n * 16 + l], acc.x);
+    acc.y = fma(row.y, T->Ta[n * 16 + l], acc.y);
+  }
+  lf[col * P + m] = acc;
+}
+
+// coef[leaf cone][m] = sum_src lf[src][m] rho[src]: threads (cone, m, slice).
+template <int P, int S>
+__global__ void __launch_bounds__(256)
+    k_leaf_apply(int ncone, int cone_off, const int* __restrict__ cone_box,
+                 const int* __restrict__ leaf_begin, const long long* __restrict__ lf_off,
+                 const zd* __restrict__ lf, const zd* __restrict__ rho, zd* __restrict__ coef) {
+  const long long gt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long item = gt / S;
+  const int sl = (int)(gt % S);
+  const int c = (int)(item / P), m = (int)(item % P);
+  zd acc{0.0, 0.0};
+  if (c < ncone) {
+    const int g = cone_off + c;
+    const int b = cone_box[g];
+    const int s0 = leaf_begin[b], ns = leaf_begin[b + 1] - s0;
+    const zd* L = lf + lf_off[c] + m;
+    for (int src = sl; src < ns; src += S) cfma(acc, L[(long long)src * P], rho[s0 + src]);
+  }
+  acc = lane_group_sum<S>(acc);
+  if (c < ncone && sl == 0) coef[(size_t)(cone_off + c) * P + m] = acc;
+}
+
 // Propagation into the parent cones of level l-1 from the child cones of
 // level l. proto: [parent nonrel slot][4][P] child nonrel slot; dmap: level-l
 // dense (box, nonrel slot) -> local relevant cone.

@@ -26,6 +26,8 @@ static int fail(ifgf_plan* p, int code, const std::string& msg) {
 }
 int ifgf::plan_fail(ifgf_plan* p, int code, const std::string& msg) { return fail(p, code, msg); }
 
+static bool build_stored_any(ifgf_plan* P);
+
 static void build_device_side(ifgf_plan* P) {
   const Tree& t = P->tree;
   const long long N = t.N;
@@ -137,7 +139,12 @@ int ifgf_plan_create(const ifgf_surface* s, const ifgf_params* prm, ifgf_plan** 
     };
     tn(Ps, P->cheb.xs, P->cheb.Ts, 1.0);
     tn(Pa, P->cheb.xa, P->cheb.Ta, 1.0 / (Ps * Pa));
-    if (!(prm->flags & IFGF_PLAN_HOST_ONLY)) build_device_side(P.get());
+    if (!(prm->flags & IFGF_PLAN_HOST_ONLY)) {
+      build_device_side(P.get());
+      if (!(prm->flags & IFGF_PLAN_ON_THE_FLY)) {
+        if (!build_stored_any(P.get())) return fail(nullptr, IFGF_ERR_ARG, "no kernel instantiation for this (Ps, Pang)");
+      }
+    }
   } catch (std::exception& e) {
     return fail(nullptr, IFGF_ERR_HIP, e.what());
   }
@@ -160,7 +167,7 @@ int ifgf_plan_points(const ifgf_plan* p, double* x, double* y, double* nx, doubl
   return IFGF_OK;
 }
 
-int ifgf_plan_stats(const ifgf_plan* p, long long out[8]) {
+int ifgf_plan_stats(const ifgf_plan* p, long long out[12]) {
   if (!p || !out) return fail(nullptr, IFGF_ERR_ARG, "null argument");
   out[0] = p->n_near;
   out[1] = p->tree.n_leaf;
@@ -170,6 +177,10 @@ int ifgf_plan_stats(const ifgf_plan* p, long long out[8]) {
   out[5] = (long long)p->corr.patch.size();
   out[6] = p->tree.nlev;
   out[7] = p->geo.N;
+  out[8] = p->tree.lv[p->tree.D].ncone;
+  out[9] = p->stored ? 1 : 0;
+  out[10] = p->stored ? (long long)p->nf_val.n : 0;
+  out[11] = p->stored_bytes;
   return IFGF_OK;
 }
 
@@ -210,6 +221,7 @@ long long ifgf_plan_correction(const ifgf_plan* p, long long* ptr, int* patch, d
 // lanes that share one target (near field, interpolation) or one leaf cone node
 constexpr int kNearLanes = 8;
 constexpr int kInterpLanes = 4;
+constexpr int kLeafApplyLanes = 4;
 template <int P>
 struct LeafLanes {
   static constexpr int value = (P * 2 <= 256) ? 2 : 1;
@@ -238,15 +250,27 @@ static void launch_ifgf(ifgf_plan* P, const zd* x, zd* y, hipStream_t st) {
   }
   hipLaunchKernelGGL(k_gather, dim3(nb), dim3(256), 0, st, N, x, P->w.p, P->perm.p, P->rho.p);
   if (prof) HIP_TRY(hipEventRecord(ev[1], st));
-  hipLaunchKernelGGL((k_near<F, kNearLanes>), dim3((N + 256 / kNearLanes - 1) / (256 / kNearLanes)), dim3(256), 0, st, N, P->px.p, P->py.p, P->pnx.p, P->pny.p,
-                     P->patch.p, P->excl_d.p, P->leaf_of_point.p, P->leaf_begin.p, P->nb_ptr.p,
-                     P->nb_list.p, P->rho.p, P->sol.p, k, eta);
+  if (P->stored)
+    hipLaunchKernelGGL((k_near_apply<kNearLanes>), dim3((N + 256 / kNearLanes - 1) / (256 / kNearLanes)),
+                       dim3(256), 0, st, N, P->leaf_of_point.p, P->leaf_begin.p, P->nb_ptr.p, P->nb_list.p,
+                       P->nf_off.p, P->nf_val.p, P->rho.p, P->sol.p);
+  else
+    hipLaunchKernelGGL((k_near<F, kNearLanes>), dim3((N + 256 / kNearLanes - 1) / (256 / kNearLanes)), dim3(256), 0, st, N, P->px.p, P->py.p, P->pnx.p, P->pny.p,
+                       P->patch.p, P->excl_d.p, P->leaf_of_point.p, P->leaf_begin.p, P->nb_ptr.p,
+                       P->nb_list.p, P->rho.p, P->sol.p, k, eta);
   if (prof) HIP_TRY(hipEventRecord(ev[2], st));
   const LevelInfo& LD = t.lv[D];
   constexpr int LCPB = 256 / (Pn * kLeafLanes);
-  hipLaunchKernelGGL((k_leaf<F, Ps, Pang, kLeafLanes>), dim3((LD.ncone + LCPB - 1) / LCPB), dim3(256), 0, st,
-                     LD.ncone, LD.cone_off, P->cg.p, P->cone_box.p, P->leaf_begin.p, P->px.p, P->py.p,
-                     P->pnx.p, P->pny.p, P->rho.p, P->coef.p, P->tabs.p, k, eta);
+  if (P->stored) {
+    const long long nthreads = (long long)LD.ncone * Pn * kLeafApplyLanes;
+    hipLaunchKernelGGL((k_leaf_apply<Pn, kLeafApplyLanes>), dim3((unsigned)((nthreads + 255) / 256)), dim3(256), 0,
+                       st, LD.ncone, LD.cone_off, P->cone_box.p, P->leaf_begin.p, P->lf_off.p, P->lf_val.p,
+                       P->rho.p, P->coef.p);
+  } else {
+    hipLaunchKernelGGL((k_leaf<F, Ps, Pang, kLeafLanes>), dim3((LD.ncone + LCPB - 1) / LCPB), dim3(256), 0, st,
+                       LD.ncone, LD.cone_off, P->cg.p, P->cone_box.p, P->leaf_begin.p, P->px.p, P->py.p,
+                       P->pnx.p, P->pny.p, P->rho.p, P->coef.p, P->tabs.p, k, eta);
+  }
   if (prof) HIP_TRY(hipEventRecord(ev[3], st));
   for (int l = D; l >= 3; l--) {
     const LevelInfo& L = t.lv[l];
@@ -265,6 +289,81 @@ static void launch_ifgf(ifgf_plan* P, const zd* x, zd* y, hipStream_t st) {
                      P->corr_patch.p, P->corr_w.p, P->inv.p, P->sol.p, y);
   if (prof) HIP_TRY(hipEventRecord(ev[6], st));
   HIP_TRY(hipGetLastError());
+}
+
+// Evaluates the stored operators once (plan creation).
+template <int F, int Ps, int Pang>
+static void build_stored(ifgf_plan* P) {
+  constexpr int Pn = Ps * Pang;
+  const Tree& t = P->tree;
+  const int N = (int)t.N, D = t.D;
+  const double k = P->prm.k, eta = P->prm.eta;
+  hipStream_t st = P->stream;
+  // near field: segment length of target i = points of its leaf neighbours
+  std::vector<long long> nf_off(N + 1, 0);
+  for (int i = 0; i < N; i++) {
+    const int b = t.leaf_of_point[i];
+    long long c = 0;
+    for (int n = t.nb_ptr[b]; n < t.nb_ptr[b + 1]; n++)
+      c += t.leaf_begin[t.nb_list[n] + 1] - t.leaf_begin[t.nb_list[n]];
+    nf_off[i + 1] = nf_off[i] + c;
+  }
+  P->nf_off.upload(nf_off);
+  P->nf_val.alloc(nf_off[N]);
+  hipLaunchKernelGGL((k_near_build<F, kNearLanes>), dim3((N + 256 / kNearLanes - 1) / (256 / kNearLanes)),
+                     dim3(256), 0, st, N, P->px.p, P->py.p, P->pnx.p, P->pny.p, P->patch.p, P->excl_d.p,
+                     P->leaf_of_point.p, P->leaf_begin.p, P->nb_ptr.p, P->nb_list.p, P->nf_off.p,
+                     P->nf_val.p, k, eta);
+  HIP_TRY(hipGetLastError());
+  // leaf operator: P x n_src per leaf cone
+  const LevelInfo& LD = t.lv[D];
+  std::vector<long long> lf_off(LD.ncone + 1, 0);
+  for (int c = 0; c < LD.ncone; c++) {
+    const int b = t.cone_box[LD.cone_off + c];
+    lf_off[c + 1] = lf_off[c] + (long long)Pn * (t.leaf_begin[b + 1] - t.leaf_begin[b]);
+  }
+  P->lf_off.upload(lf_off);
+  P->lf_val.alloc(lf_off[LD.ncone]);
+  {
+    DevBuf<zd> tmp;
+    tmp.alloc(lf_off[LD.ncone]);
+    const long long nitems = lf_off[LD.ncone];
+    hipLaunchKernelGGL((k_leaf_build_values<F, Ps, Pang>), dim3((unsigned)((nitems + 255) / 256)), dim3(256), 0,
+                       st, LD.ncone, LD.cone_off, P->cg.p, P->cone_box.p, P->leaf_begin.p, P->lf_off.p,
+                       P->lf_off.p, P->px.p, P->py.p, P->pnx.p, P->pny.p, P->tabs.p, tmp.p, nitems, k, eta);
+    hipLaunchKernelGGL((k_leaf_build_transform<Ps, Pang>), dim3((unsigned)((nitems + 255) / 256)), dim3(256), 0,
+                       st, nitems / Pn, tmp.p, P->tabs.p, P->lf_val.p);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(st));
+  }
+  P->stored_bytes = (long long)(nf_off[N] + lf_off[LD.ncone]) * (long long)sizeof(zd);
+  P->stored = true;
+}
+
+template <int F>
+static bool dispatch_build(ifgf_plan* P) {
+  const int a = P->prm.Ps, b = P->prm.Pang;
+#define IFGF_CASE(A, B)            \
+  if (a == A && b == B) {          \
+    build_stored<F, A, B>(P);      \
+    return true;                   \
+  }
+  IFGF_CASE(5, 5)
+  IFGF_CASE(3, 5)
+  IFGF_CASE(5, 7)
+  IFGF_CASE(7, 7)
+  IFGF_CASE(9, 9)
+  IFGF_CASE(11, 11)
+#undef IFGF_CASE
+  return false;
+}
+
+static bool build_stored_any(ifgf_plan* P) {
+  switch (P->prm.formulation) {
+    case kSingleLayer: return dispatch_build<kSingleLayer>(P);
+    case kDoubleLayer: return dispatch_build<kDoubleLayer>(P);
+    default: return dispatch_build<kCombined>(P);
+  }
 }
 
 template <int F>

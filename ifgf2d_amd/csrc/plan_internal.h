@@ -72,6 +72,11 @@ struct ifgf_plan {
   ifgf::DevBuf<long long> it_ptr, corr_ptr;
   ifgf::DevBuf<zd> corr_w;
   ifgf::DevBuf<ifgf::ChebTables> tabs;
+  // stored operators (near-field values, folded leaf operator)
+  bool stored = false;
+  ifgf::DevBuf<long long> nf_off, lf_off;
+  ifgf::DevBuf<zd> nf_val, lf_val;
+  long long stored_bytes = 0;
   // workspaces
   ifgf::DevBuf<zd> rho, sol, coef, xbuf, ybuf;
   // profiling

@@ -45,7 +45,12 @@ extern "C" {
 #define IFGF_CURVE_STAR 2   /* params: cx, cy, R, nmodes, a[nmodes], b[nmodes] */
 
 /* plan flags */
-#define IFGF_PLAN_HOST_ONLY 1 /* build every host table, allocate nothing on the GPU */
+#define IFGF_PLAN_HOST_ONLY 1  /* build every host table, allocate nothing on the GPU */
+#define IFGF_PLAN_ON_THE_FLY 2 /* re-evaluate the near-field and leaf kernels in every
+                                  application instead of storing them (default: the
+                                  geometry-only near-field values and the folded leaf
+                                  operator are evaluated once at plan creation and
+                                  streamed from HBM by every application) */
 
 /* apply flags */
 #define IFGF_PTR_DEVICE 1     /* x / y are device pointers; the call is asynchronous on stream */
@@ -73,7 +78,7 @@ typedef struct {
   int Ps, Pang;   /* Chebyshev nodes per cone in s and theta (BoxTree<Ps,Pang>) */
   int nlevels;    /* tree levels, >= 3 */
   int nrhs_max;   /* largest nrhs passed to ifgf_apply */
-  int flags;      /* IFGF_PLAN_HOST_ONLY */
+  int flags;      /* IFGF_PLAN_HOST_ONLY | IFGF_PLAN_ON_THE_FLY */
 } ifgf_params;
 
 /* Replaces ForwardMap::ForwardMap (ForwardMap.tpp:15-35) followed by
@@ -101,8 +106,10 @@ int ifgf_plan_points(const ifgf_plan* plan, double* x, double* y, double* nx, do
 /* Work counts of one application: [0] near-field pairs, [1] leaf kernel
  * evaluations, [2] propagation interpolations, [3] target interpolations,
  * [4] relevant cones (all levels), [5] correction entries, [6] nlevels,
- * [7] N. */
-int ifgf_plan_stats(const ifgf_plan* plan, long long out[8]);
+ * [7] N, [8] leaf-level cones, [9] 1 if the near-field values and leaf
+ * operator are stored, [10] stored near-field entries (excluded pairs
+ * included as zeros), [11] bytes of the stored operators. */
+int ifgf_plan_stats(const ifgf_plan* plan, long long out[12]);
 
 /* BoxTree::getSorting() (BoxTree.h:1163-1165): Morton position -> patch-order index. */
 int ifgf_plan_sorting(const ifgf_plan* plan, long long* sorting);

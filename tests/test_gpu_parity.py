@@ -50,11 +50,11 @@ CASES = [
 ]
 
 
-def build_pair(curve, npatch, k, nlev, F, Ps, Pang):
+def build_pair(curve, npatch, k, nlev, F, Ps, Pang, on_the_fly=False):
     lims = I.equal_parameter_lims(npatch)
     delta = I.default_delta(curve, lims)
     fm = I.ForwardMap(curve, lims, k, nlev, formulation=F, delta=delta, p=6.0, Ps=Ps, Pang=Pang,
-                      nrhs_max=3)
+                      nrhs_max=3, on_the_fly=on_the_fly)
     om = O.ForwardMap(curve[0], curve[1], lims, formulation=F, k=k, eta=k, delta=delta, p=6.0,
                       Ps=Ps, Pang=Pang, nlevels=nlev)
     return fm, om
@@ -71,6 +71,19 @@ def test_forward_map_parity(name, curve, npatch, k, nlev, F, Ps, Pang):
         e = rel(yg, yo)
         print(f"  seed {seed}: rel diff {e:.3e}  max abs {np.max(np.abs(yg - yo)):.3e}", flush=True)
         assert e <= PARITY_TOL
+
+
+@pytest.mark.parametrize("name,curve,npatch,k,nlev,F,Ps,Pang", [CASES[0], CASES[4], CASES[6], CASES[7]],
+                         ids=[CASES[i][0] for i in (0, 4, 6, 7)])
+def test_forward_map_parity_on_the_fly(name, curve, npatch, k, nlev, F, Ps, Pang):
+    """The same parity bar with the near-field and leaf kernels re-evaluated in
+    every application (IFGF_PLAN_ON_THE_FLY)."""
+    print(f"case {name} on the fly", flush=True)
+    fm, om = build_pair(curve, npatch, k, nlev, F, Ps, Pang, on_the_fly=True)
+    x = density(fm.N, 2)
+    e = rel(fm.compute_Ax_acc(x), om.apply(x))
+    print(f"  rel diff {e:.3e}", flush=True)
+    assert e <= PARITY_TOL
 
 
 def test_apply_is_deterministic_and_linear():
