@@ -78,7 +78,18 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-on-the-fly", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run 20 device copies of 256 MiB (counter calibration) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        a = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda:0")
+        b_ = torch.empty_like(a)
+        for _ in range(20):
+            b_.copy_(a)
+        torch.cuda.synchronize()
+        log("copy calibration: 20 copies of 268435456 bytes")
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the forward map is "
               "not sharded across GPUs in this build", file=sys.stderr)

@@ -21,6 +21,19 @@
 
 namespace ifgf {
 
+// Workgroups are dispatched round-robin over the 8 XCDs (block b runs on XCD
+// b % 8), each with its own L2. Remapping b to a logical block id so that every
+// XCD works on one contiguous range of the Morton-ordered work keeps the cone
+// coefficients and densities a range touches in that XCD's L2. Bijective on
+// [0, gridDim.x).
+__device__ inline int xcd_block() {
+  constexpr int X = 8;
+  const int nb = gridDim.x, b = blockIdx.x;
+  const int x = b % X, i = b / X;
+  const int base = nb / X, rem = nb % X;
+  return x * base + (x < rem ? x : rem) + i;
+}
+
 // Functions::Atan2 (PolarCoordinates.h:8-27): theta in [0, 2 pi).
 __device__ inline double atan2_02pi(double y, double x) {
   if (x > 0 && y >= 0) return atan(y / x);
@@ -144,7 +157,8 @@ __global__ void __launch_bounds__(256)
            const int* __restrict__ leaf_of_point, const int* __restrict__ leaf_begin,
            const int* __restrict__ nb_ptr, const int* __restrict__ nb_list,
            const zd* __restrict__ rho, zd* __restrict__ sol, double k, double eta) {
-  const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lb = xcd_block();
+  const int gt = lb * blockDim.x + threadIdx.x;
   const int i = gt / S, sl = gt % S;
   zd acc{0.0, 0.0};
   if (i < N) {
@@ -175,6 +189,7 @@ __global__ void __launch_bounds__(256)
            const double* __restrict__ py, const double* __restrict__ pnx,
            const double* __restrict__ pny, const zd* __restrict__ rho, zd* __restrict__ coef,
            const ChebTables* __restrict__ tabs, double k, double eta) {
+  const int lb = xcd_block();
   constexpr int P = Ps * Pang;
   constexpr int CPB = 256 / (P * S);
   __shared__ zd buf[CPB * P];
@@ -186,7 +201,7 @@ __global__ void __launch_bounds__(256)
   const int cl = threadIdx.x / (P * S);
   const int rem = threadIdx.x % (P * S);
   const int j = rem / S, sl = rem % S;
-  const int c = blockIdx.x * CPB + cl;
+  const int c = lb * CPB + cl;
   const bool active = cl < CPB && c < ncone;
   const int g = cone_off + c;
   zd acc{0.0, 0.0};
@@ -207,10 +222,10 @@ __global__ void __launch_bounds__(256)
   __syncthreads();
   // transform: the first CPB*P threads take (cone, node) = (t / P, t % P)
   const int tc = threadIdx.x / P, tj = threadIdx.x % P;
-  const bool tact = tc < CPB && blockIdx.x * CPB + tc < ncone;
+  const bool tact = tc < CPB && lb * CPB + tc < ncone;
   const zd out = cone_transform<Ps, Pang>(buf + (tact ? tc * P : 0), tmp + (tact ? tc * P : 0), T,
                                           tact ? tj : 0, tact);
-  if (tact) coef[(size_t)(cone_off + blockIdx.x * CPB + tc) * P + tj] = out;
+  if (tact) coef[(size_t)(cone_off + lb * CPB + tc) * P + tj] = out;
 }
 
 // ---------------------------------------------------------------------------
@@ -258,7 +273,8 @@ __global__ void __launch_bounds__(256)
                  const int* __restrict__ nb_ptr, const int* __restrict__ nb_list,
                  const long long* __restrict__ nf_off, const zd* __restrict__ nf_val,
                  const zd* __restrict__ rho, zd* __restrict__ sol) {
-  const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lb = xcd_block();
+  const int gt = lb * blockDim.x + threadIdx.x;
   const int i = gt / S, sl = gt % S;
   zd acc{0.0, 0.0};
   if (i < N) {
@@ -345,7 +361,8 @@ __global__ void __launch_bounds__(256)
     k_leaf_apply(int ncone, int cone_off, const int* __restrict__ cone_box,
                  const int* __restrict__ leaf_begin, const long long* __restrict__ lf_off,
                  const zd* __restrict__ lf, const zd* __restrict__ rho, zd* __restrict__ coef) {
-  const long long gt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lb = xcd_block();
+  const long long gt = (long long)lb * blockDim.x + threadIdx.x;
   const long long item = gt / S;
   const int sl = (int)(gt % S);
   const int c = (int)(item / P), m = (int)(item % P);
@@ -361,6 +378,119 @@ __global__ void __launch_bounds__(256)
   if (c < ncone && sl == 0) coef[(size_t)(cone_off + c) * P + m] = acc;
 }
 
+// Stored interpolation and propagation geometry: (s, theta, centre factor)
+// of every (target, cousin cone) and every (parent node, child cone) pair,
+// computed once with the same device functions the on-the-fly kernels use.
+template <int S>
+__global__ void __launch_bounds__(256)
+    k_interp_build(int N, const double* __restrict__ px, const double* __restrict__ py,
+                   const long long* __restrict__ it_ptr, const int* __restrict__ it_cone,
+                   const ConeGeo* __restrict__ cg, Geo4* __restrict__ it_geo, double k) {
+  const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = gt / S, sl = gt % S;
+  if (i >= N) return;
+  const double x = px[i], y = py[i];
+  for (long long e = it_ptr[i] + sl; e < it_ptr[i + 1]; e += S) {
+    double s, t, rad;
+    cone_local(cg[it_cone[e]], x, y, s, t, rad);
+    const zd f = centre_factor(rad, k);
+    it_geo[e] = Geo4{s, t, f.x, f.y};
+  }
+}
+
+template <int Ps, int Pang, int S>
+__global__ void __launch_bounds__(256)
+    k_interp_stored(int N, const long long* __restrict__ it_ptr, const int* __restrict__ it_cone,
+                    const Geo4* __restrict__ it_geo, const zd* __restrict__ coef, zd* __restrict__ sol) {
+  const int lb = xcd_block();
+  constexpr int P = Ps * Pang;
+  const int gt = lb * blockDim.x + threadIdx.x;
+  const int i = gt / S, sl = gt % S;
+  zd acc{0.0, 0.0};
+  if (i < N) {
+    const long long e1 = it_ptr[i + 1];
+    for (long long e = it_ptr[i] + sl; e < e1; e += S) {
+      const Geo4 G = it_geo[e];
+      const zd v = cone_eval<Ps, Pang>(coef + (size_t)it_cone[e] * P, G.s, G.t);
+      cfma(acc, v, zd{G.fre, G.fim});
+    }
+  }
+  acc = lane_group_sum<S>(acc);
+  if (i < N && sl == 0) sol[i] = cadd(sol[i], acc);
+}
+
+// Parent cones of level l-1 (global ids pcone_off..+npc): for node j and the
+// k-th relevant child, entry pr_ptr[g] + k*P + j holds the child cone id and
+// the node's (s, theta, factor) in it; pn_ifac[g*P + j] = 1/factor of the node.
+template <int Ps, int Pang>
+__global__ void __launch_bounds__(256)
+    k_prop_build(int npc, int pcone_off, int pbox_off, int ccone_off, int cnslot,
+                 const ConeGeo* __restrict__ cg, const int* __restrict__ cone_box,
+                 const int* __restrict__ cone_q, const int* __restrict__ box_child,
+                 const int* __restrict__ proto, const int* __restrict__ dmap,
+                 const ChebTables* __restrict__ T, const long long* __restrict__ pr_ptr,
+                 int* __restrict__ pr_cone, Geo4* __restrict__ pr_geo, zd* __restrict__ pn_ifac,
+                 double k) {
+  constexpr int P = Ps * Pang;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= npc * P) return;
+  const int c = t / P, j = t % P;
+  const int g = pcone_off + c;
+  double nx, ny, r;
+  cone_node(cg[g], T->xs[j % Ps], T->xa[j / Ps], nx, ny, r);
+  pn_ifac[(size_t)g * P + j] = inv_centre_factor(r, k);
+  const int pb = cone_box[g], q = cone_q[g];
+  long long e = pr_ptr[g] + j;
+  for (int cp = 0; cp < 4; cp++) {
+    const int cb = box_child[(size_t)(pbox_off + pb) * 4 + cp];
+    if (cb < 0) continue;
+    const int cq = proto[((size_t)q * 4 + cp) * P + j];
+    const int cc = ccone_off + dmap[(size_t)cb * cnslot + cq];
+    double s, tt, rad;
+    cone_local(cg[cc], nx, ny, s, tt, rad);
+    const zd f = centre_factor(rad, k);
+    pr_cone[e] = cc;
+    pr_geo[e] = Geo4{s, tt, f.x, f.y};
+    e += P;
+  }
+}
+
+template <int Ps, int Pang>
+__global__ void __launch_bounds__(256)
+    k_prop_stored(int npc, int pcone_off, const long long* __restrict__ pr_ptr,
+                  const int* __restrict__ pr_cone, const Geo4* __restrict__ pr_geo,
+                  const zd* __restrict__ pn_ifac, zd* __restrict__ coef,
+                  const ChebTables* __restrict__ tabs) {
+  const int lb = xcd_block();
+  constexpr int P = Ps * Pang;
+  constexpr int CPB = 256 / P;
+  __shared__ zd buf[CPB * P];
+  __shared__ zd tmp[CPB * P];
+  __shared__ ChebTables T;
+  for (int q = threadIdx.x; q < (int)(sizeof(ChebTables) / sizeof(double)); q += blockDim.x)
+    ((double*)&T)[q] = ((const double*)tabs)[q];
+  __syncthreads();
+  const int cl = threadIdx.x / P, j = threadIdx.x % P;
+  const int c = lb * CPB + cl;
+  const bool active = cl < CPB && c < npc;
+  const int g = pcone_off + c;
+  if (active) {
+    zd acc{0.0, 0.0};
+    const long long e1 = pr_ptr[g + 1];
+    for (long long e = pr_ptr[g] + j; e < e1; e += P) {
+      const Geo4 G = pr_geo[e];
+      const zd v = cone_eval<Ps, Pang>(coef + (size_t)pr_cone[e] * P, G.s, G.t);
+      cfma(acc, v, zd{G.fre, G.fim});
+    }
+    buf[threadIdx.x] = cmul(acc, pn_ifac[(size_t)g * P + j]);
+  }
+  __syncthreads();
+  const int base = cl * P;
+  const zd out = cone_transform<Ps, Pang>(buf + (active ? base : 0), tmp + (active ? base : 0), T,
+                                          active ? j : 0, active);
+  if (active) coef[(size_t)g * P + j] = out;
+}
+
 // Propagation into the parent cones of level l-1 from the child cones of
 // level l. proto: [parent nonrel slot][4][P] child nonrel slot; dmap: level-l
 // dense (box, nonrel slot) -> local relevant cone.
@@ -371,6 +501,7 @@ __global__ void __launch_bounds__(256)
            const int* __restrict__ cone_q, const int* __restrict__ box_child,
            const int* __restrict__ proto, const int* __restrict__ dmap, zd* __restrict__ coef,
            const ChebTables* __restrict__ tabs, double k) {
+  const int lb = xcd_block();
   constexpr int P = Ps * Pang;
   constexpr int CPB = 256 / P;
   __shared__ zd buf[CPB * P];
@@ -380,7 +511,7 @@ __global__ void __launch_bounds__(256)
     ((double*)&T)[q] = ((const double*)tabs)[q];
   __syncthreads();
   const int cl = threadIdx.x / P, j = threadIdx.x % P;
-  const int c = blockIdx.x * CPB + cl;
+  const int c = lb * CPB + cl;
   const bool active = cl < CPB && c < npc;
   const int g = pcone_off + c;
   if (active) {
@@ -418,8 +549,9 @@ __global__ void __launch_bounds__(256)
              const long long* __restrict__ it_ptr, const int* __restrict__ it_cone,
              const ConeGeo* __restrict__ cg, const zd* __restrict__ coef, zd* __restrict__ sol,
              double k) {
+  const int lb = xcd_block();
   constexpr int P = Ps * Pang;
-  const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  const int gt = lb * blockDim.x + threadIdx.x;
   const int i = gt / S, sl = gt % S;
   zd acc{0.0, 0.0};
   if (i < N) {
@@ -444,7 +576,8 @@ __global__ void __launch_bounds__(256)
                const long long* __restrict__ corr_ptr, const int* __restrict__ corr_patch,
                const zd* __restrict__ corr_w, const int* __restrict__ inv,
                const zd* __restrict__ sol, zd* __restrict__ y) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lb = xcd_block();
+  const int i = lb * blockDim.x + threadIdx.x;
   if (i >= N) return;
   zd acc = cscale(x[i], ident);
   for (long long e = corr_ptr[i]; e < corr_ptr[i + 1]; e++) {

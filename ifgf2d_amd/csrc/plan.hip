@@ -275,12 +275,21 @@ static void launch_ifgf(ifgf_plan* P, const zd* x, zd* y, hipStream_t st) {
   for (int l = D; l >= 3; l--) {
     const LevelInfo& L = t.lv[l];
     const LevelInfo& Pl = t.lv[l - 1];
+    if (P->stored) {
+      hipLaunchKernelGGL((k_prop_stored<Ps, Pang>), dim3((Pl.ncone + CPB - 1) / CPB), dim3(256), 0, st, Pl.ncone,
+                         Pl.cone_off, P->pr_ptr.p, P->pr_cone.p, P->pr_geo.p, P->pn_ifac.p, P->coef.p, P->tabs.p);
+      continue;
+    }
     hipLaunchKernelGGL((k_prop<Ps, Pang>), dim3((Pl.ncone + CPB - 1) / CPB), dim3(256), 0, st, Pl.ncone,
                        Pl.cone_off, Pl.box_off, Pl.nrad, L.cone_off, L.nslot, P->cg.p, P->cone_box.p,
                        P->cone_qd.p, P->box_child.p, P->proto.p + L.proto_off, P->dense_map.p + L.map_off,
                        P->coef.p, P->tabs.p, k);
   }
   if (prof) HIP_TRY(hipEventRecord(ev[4], st));
+  if (P->stored)
+    hipLaunchKernelGGL((k_interp_stored<Ps, Pang, kInterpLanes>), dim3((N + 256 / kInterpLanes - 1) / (256 / kInterpLanes)),
+                       dim3(256), 0, st, N, P->it_ptr.p, P->it_cone.p, P->it_geo.p, P->coef.p, P->sol.p);
+  else
   hipLaunchKernelGGL((k_interp<Ps, Pang, kInterpLanes>), dim3((N + 256 / kInterpLanes - 1) / (256 / kInterpLanes)), dim3(256), 0, st, N, P->px.p, P->py.p, P->it_ptr.p,
                      P->it_cone.p, P->cg.p, P->coef.p, P->sol.p, k);
   if (prof) HIP_TRY(hipEventRecord(ev[5], st));
@@ -336,7 +345,39 @@ static void build_stored(ifgf_plan* P) {
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(st));
   }
-  P->stored_bytes = (long long)(nf_off[N] + lf_off[LD.ncone]) * (long long)sizeof(zd);
+  // interpolation geometry per (target, cousin cone)
+  P->it_geo.alloc(t.it_ptr[N]);
+  hipLaunchKernelGGL((k_interp_build<kInterpLanes>), dim3((N + 256 / kInterpLanes - 1) / (256 / kInterpLanes)),
+                     dim3(256), 0, st, N, P->px.p, P->py.p, P->it_ptr.p, P->it_cone.p, P->cg.p, P->it_geo.p, k);
+  HIP_TRY(hipGetLastError());
+  // propagation geometry per (parent node, child cone)
+  std::vector<long long> pr_ptr(t.n_cones + 1, 0);
+  for (long long g = 0; g < t.n_cones; g++) {
+    long long c = 0;
+    const int l = t.cone_level[g];
+    if (l < D) {
+      const int pb = t.cone_box[g];
+      for (int cp = 0; cp < 4; cp++) c += t.box_child[(size_t)(t.lv[l].box_off + pb) * 4 + cp] >= 0 ? Pn : 0;
+    }
+    pr_ptr[g + 1] = pr_ptr[g] + c;
+  }
+  P->pr_ptr.upload(pr_ptr);
+  P->pr_cone.alloc(pr_ptr[t.n_cones]);
+  P->pr_geo.alloc(pr_ptr[t.n_cones]);
+  P->pn_ifac.alloc((size_t)t.n_cones * Pn);
+  for (int l = D; l >= 3; l--) {
+    const LevelInfo& L = t.lv[l];
+    const LevelInfo& Pl = t.lv[l - 1];
+    hipLaunchKernelGGL((k_prop_build<Ps, Pang>), dim3((Pl.ncone * Pn + 255) / 256), dim3(256), 0, st, Pl.ncone,
+                       Pl.cone_off, Pl.box_off, L.cone_off, L.nslot, P->cg.p, P->cone_box.p, P->cone_qd.p,
+                       P->box_child.p, P->proto.p + L.proto_off, P->dense_map.p + L.map_off, P->tabs.p,
+                       P->pr_ptr.p, P->pr_cone.p, P->pr_geo.p, P->pn_ifac.p, k);
+  }
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(st));
+  P->stored_bytes = (long long)(nf_off[N] + lf_off[LD.ncone] + (long long)t.n_cones * Pn) * (long long)sizeof(zd) +
+                    (long long)(t.it_ptr[N] + pr_ptr[t.n_cones]) * (long long)sizeof(Geo4) +
+                    pr_ptr[t.n_cones] * (long long)sizeof(int);
   P->stored = true;
 }
 

@@ -74,8 +74,10 @@ struct ifgf_plan {
   ifgf::DevBuf<ifgf::ChebTables> tabs;
   // stored operators (near-field values, folded leaf operator)
   bool stored = false;
-  ifgf::DevBuf<long long> nf_off, lf_off;
-  ifgf::DevBuf<zd> nf_val, lf_val;
+  ifgf::DevBuf<long long> nf_off, lf_off, pr_ptr;
+  ifgf::DevBuf<zd> nf_val, lf_val, pn_ifac;
+  ifgf::DevBuf<ifgf::Geo4> it_geo, pr_geo;
+  ifgf::DevBuf<int> pr_cone;
   long long stored_bytes = 0;
   // workspaces
   ifgf::DevBuf<zd> rho, sol, coef, xbuf, ybuf;

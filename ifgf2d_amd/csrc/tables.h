@@ -21,4 +21,11 @@ struct ChebTables {
   double Ta[16 * 16];       // Ta[n*16 + l] = T_n(xa[l]) * alpha_n / (Ps*Pang)
 };
 
+// Local Chebyshev coordinates (s, theta) of a point in a cone and the centre
+// factor exp(i k r)/sqrt(r) at its distance r from the cone's box centre.
+struct Geo4 {
+  double s, t;
+  double fre, fim;
+};
+
 }  // namespace ifgf
