@@ -37,7 +37,7 @@ _FORMS = {"single": SINGLE_LAYER, "double": DOUBLE_LAYER, "combined": COMBINED,
 _PREC = {"fp64": FP64, "mixed": MIXED, FP64: FP64, MIXED: MIXED}
 
 IFGF_OK, IFGF_ERR_NOT_CONVERGED = 0, 6
-IFGF_PLAN_HOST_ONLY, IFGF_PLAN_ON_THE_FLY, IFGF_PTR_DEVICE = 1, 2, 1
+IFGF_PLAN_HOST_ONLY, IFGF_PLAN_ON_THE_FLY, IFGF_PLAN_AUTOTUNE, IFGF_PTR_DEVICE = 1, 2, 4, 1
 
 
 class IFGFError(RuntimeError):
@@ -172,11 +172,12 @@ class ForwardMap:
     or "combined" (1/2 I + K - i eta S, the sound-soft CFIE).
     on_the_fly: re-evaluate the near-field and leaf kernels in every application
     instead of storing them once at plan creation (less memory, slower).
+    autotune: time the candidate lane counts of the stored-path kernels once.
     """
 
     def __init__(self, curve, patch_lims, k, nlevels, Np=10, formulation="combined", eta=None,
                  delta=None, p=4.0, Ps=5, Pang=5, precision="fp64", nrhs_max=1, host_only=False,
-                 on_the_fly=False):
+                 on_the_fly=False, autotune=False):
         L = lib()
         self.curve = curve
         self.lims = np.ascontiguousarray(patch_lims, dtype=np.float64)
@@ -187,7 +188,8 @@ class ForwardMap:
         self._cp = np.ascontiguousarray(curve[1], dtype=np.float64)
         surf = _Surface(curve[0], _d(self._cp), self._cp.size, _d(self.lims), self.lims.size - 1,
                         Np, self.delta, float(p))
-        flags = (IFGF_PLAN_HOST_ONLY if host_only else 0) | (IFGF_PLAN_ON_THE_FLY if on_the_fly else 0)
+        flags = ((IFGF_PLAN_HOST_ONLY if host_only else 0) | (IFGF_PLAN_ON_THE_FLY if on_the_fly else 0)
+                 | (IFGF_PLAN_AUTOTUNE if autotune else 0))
         prm = _Params(self.formulation, _PREC[precision], self.k, self.eta, Ps, Pang, nlevels,
                       nrhs_max, flags)
         h = ctypes.c_void_p()
@@ -221,7 +223,7 @@ class ForwardMap:
         return x, y, nx, ny, w
 
     def _stats(self):
-        s = np.empty(12, dtype=np.int64)
+        s = np.empty(16, dtype=np.int64)
         self._check(lib().ifgf_plan_stats(self._h, s.ctypes.data_as(_lp)), "stats")
         return s.tolist()
 
@@ -233,7 +235,8 @@ class ForwardMap:
     def extra_stats(self):
         s = self._stats()
         return {"leaf_cones": s[8], "prop_cones": s[4] - s[8], "stored": bool(s[9]),
-                "near_stored": s[10], "stored_bytes": s[11]}
+                "near_stored": s[10], "stored_bytes": s[11],
+                "lanes": {"near": s[12], "leaf": s[13], "prop": s[14], "finalize": s[15]}}
 
     def sorting(self):
         s = np.empty(self.N, dtype=np.int64)

@@ -267,29 +267,55 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+
+// Stored near field for targets [n0, n1). The values of target i are the
+// flattened neighbour boxes' sources (in neighbour order), so flat offset o
+// maps to source o + shift[q] for the segment q with end[q-1] <= o < end[q];
+// the kMaxNb (end, shift) pairs of the target's leaf box come from nb_seg.
+// S lanes per target; lane sl takes every S-th offset, and the loop body has
+// no load-to-load dependency, so the unrolled iterations' loads are in flight
+// together (the values stream from HBM).
+constexpr int kMaxNb = 9;
+constexpr int kNbSegInts = 20;  // 9 (end, shift) pairs padded to 5 int4
+
+struct NearArgs {
+  const int* leaf_of_point;
+  const int4* nb_seg;
+  const long long* nf_off;
+  const zd* nf_val;
+  const zd* rho;
+  zd* sol;
+};
+
 template <int S>
-__global__ void __launch_bounds__(256)
-    k_near_apply(int N, const int* __restrict__ leaf_of_point, const int* __restrict__ leaf_begin,
-                 const int* __restrict__ nb_ptr, const int* __restrict__ nb_list,
-                 const long long* __restrict__ nf_off, const zd* __restrict__ nf_val,
-                 const zd* __restrict__ rho, zd* __restrict__ sol) {
-  const int lb = xcd_block();
+__device__ inline void near_block(const NearArgs& a, int lb, int n0, int n1) {
   const int gt = lb * blockDim.x + threadIdx.x;
-  const int i = gt / S, sl = gt % S;
+  const int i = n0 + gt / S, sl = gt % S;
   zd acc{0.0, 0.0};
-  if (i < N) {
-    const int b = leaf_of_point[i];
-    const zd* v = nf_val + nf_off[i];
-    int base = 0;
-    for (int n = nb_ptr[b]; n < nb_ptr[b + 1]; n++) {
-      const int nb = nb_list[n];
-      const int s0 = leaf_begin[nb], s1 = leaf_begin[nb + 1];
-      for (int s = s0 + sl; s < s1; s += S) cfma(acc, v[base + (s - s0)], rho[s]);
-      base += s1 - s0;
+  if (i < n1) {
+    const long long o0 = a.nf_off[i];
+    const int len = (int)(a.nf_off[i + 1] - o0);
+    const int4* sg = a.nb_seg + (size_t)a.leaf_of_point[i] * (kNbSegInts / 4);
+    const int4 q0 = sg[0], q1 = sg[1], q2 = sg[2], q3 = sg[3], q4 = sg[4];
+    const int end[kMaxNb - 1] = {q0.x, q0.z, q1.x, q1.z, q2.x, q2.z, q3.x, q3.z};
+    const int sh[kMaxNb] = {q0.y, q0.w, q1.y, q1.w, q2.y, q2.w, q3.y, q3.w, q4.y};
+    const zd* __restrict__ v = a.nf_val + o0;
+    const zd* __restrict__ rho = a.rho;
+#pragma unroll 4
+    for (int o = sl; o < len; o += S) {
+      int src = o + sh[0];
+#pragma unroll
+      for (int q = 1; q < kMaxNb; q++) src = (o >= end[q - 1]) ? o + sh[q] : src;
+      cfma(acc, v[o], rho[src]);
     }
   }
   acc = lane_group_sum<S>(acc);
-  if (i < N && sl == 0) sol[i] = acc;
+  if (i < n1 && sl == 0) a.sol[i] = acc;
+}
+
+template <int S>
+__global__ void __launch_bounds__(256) k_near_apply(int N, NearArgs a) {
+  near_block<S>(a, xcd_block(), 0, N);
 }
 
 // Leaf operator L_c (P x n_src per leaf cone c), stored source-major at
@@ -372,6 +398,7 @@ __global__ void __launch_bounds__(256)
     const int b = cone_box[g];
     const int s0 = leaf_begin[b], ns = leaf_begin[b + 1] - s0;
     const zd* L = lf + lf_off[c] + m;
+#pragma unroll 4
     for (int src = sl; src < ns; src += S) cfma(acc, L[(long long)src * P], rho[s0 + src]);
   }
   acc = lane_group_sum<S>(acc);
@@ -409,6 +436,7 @@ __global__ void __launch_bounds__(256)
   zd acc{0.0, 0.0};
   if (i < N) {
     const long long e1 = it_ptr[i + 1];
+#pragma unroll 2
     for (long long e = it_ptr[i] + sl; e < e1; e += S) {
       const Geo4 G = it_geo[e];
       const zd v = cone_eval<Ps, Pang>(coef + (size_t)it_cone[e] * P, G.s, G.t);
@@ -417,6 +445,64 @@ __global__ void __launch_bounds__(256)
   }
   acc = lane_group_sum<S>(acc);
   if (i < N && sl == 0) sol[i] = cadd(sol[i], acc);
+}
+
+// Interpolation with the task's cones staged in LDS (cs, ncone * P values):
+// block per task, S lanes per target over its entries at the task's level;
+// writes the level's partial sum part[li * N + i] (summed over levels in
+// fixed order by k_finalize).
+struct InterpArgs {
+  const ITask* tasks;
+  const int* task_cones;
+  const long long* it_lvl;
+  const unsigned short* ent_slot;
+  const Geo4* it_geo;
+  const zd* coef;
+  zd* part;
+  int N, D;
+};
+
+template <int Ps, int Pang, int S>
+__device__ inline void interp_block(const InterpArgs& a, int q, zd* cs) {
+  constexpr int P = Ps * Pang;
+  const ITask task = a.tasks[q];
+  const int il = threadIdx.x / S, sl = threadIdx.x % S;
+  const int i = task.t0 + il;
+  const bool act = il < task.nt;
+  // entry range and first entry: loaded before the barrier so that their
+  // latency overlaps the staging of the task's cones
+  long long e = 0, e1 = 0;
+  if (act) {
+    e = a.it_lvl[(size_t)i * a.D + task.li] + sl;
+    e1 = a.it_lvl[(size_t)i * a.D + task.li + 1];
+  }
+  Geo4 G{0.0, 0.0, 0.0, 0.0};
+  int slot = 0;
+  if (e < e1) {
+    G = a.it_geo[e];
+    slot = a.ent_slot[e];
+  }
+  for (int r = threadIdx.x; r < task.ncone * P; r += blockDim.x)
+    cs[r] = a.coef[(size_t)a.task_cones[task.cone_off + r / P] * P + r % P];
+  __syncthreads();
+  zd acc{0.0, 0.0};
+  while (e < e1) {
+    const zd v = cone_eval<Ps, Pang>(cs + slot * P, G.s, G.t);
+    cfma(acc, v, zd{G.fre, G.fim});
+    e += S;
+    if (e < e1) {
+      G = a.it_geo[e];
+      slot = a.ent_slot[e];
+    }
+  }
+  acc = lane_group_sum<S>(acc);
+  if (act && sl == 0) a.part[(size_t)task.li * a.N + i] = acc;
+}
+
+template <int Ps, int Pang, int S>
+__global__ void __launch_bounds__(256) k_interp_lds(InterpArgs a) {
+  extern __shared__ zd cs[];
+  interp_block<Ps, Pang, S>(a, blockIdx.x, cs);
 }
 
 // Parent cones of level l-1 (global ids pcone_off..+npc): for node j and the
@@ -455,40 +541,82 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-template <int Ps, int Pang>
-__global__ void __launch_bounds__(256)
-    k_prop_stored(int npc, int pcone_off, const long long* __restrict__ pr_ptr,
-                  const int* __restrict__ pr_cone, const Geo4* __restrict__ pr_geo,
-                  const zd* __restrict__ pn_ifac, zd* __restrict__ coef,
-                  const ChebTables* __restrict__ tabs) {
-  const int lb = xcd_block();
+
+// Propagation into parent cones pcone_off..+npc of level l-1 with stored
+// geometry. Threads (parent cone, node, child lane): KC lanes per node share
+// its (up to 4) children, reduced in fixed order; CPB parent cones per
+// 256-thread block (KC * P <= 256). The work is latency-bound (a few hundred
+// blocks per level), so the table copy into LDS is only waited for at the one
+// barrier before the transform, and the entries of all children are loaded
+// before any is used. LDS: ChebTables, then 2 * CPB * P values.
+struct PropArgs {
+  const long long* pr_ptr;
+  const int* pr_cone;
+  const Geo4* pr_geo;
+  const zd* pn_ifac;
+  const ChebTables* tabs;
+  zd* coef;
+  int npc, pcone_off;
+};
+
+template <int Ps, int Pang, int KC>
+constexpr size_t prop_lds_bytes() {
+  return sizeof(ChebTables) + 2 * sizeof(zd) * (256 / (KC * Ps * Pang)) * Ps * Pang;
+}
+
+template <int Ps, int Pang, int KC>
+__device__ inline void prop_block(const PropArgs& a, int lb, zd* lds) {
   constexpr int P = Ps * Pang;
-  constexpr int CPB = 256 / P;
-  __shared__ zd buf[CPB * P];
-  __shared__ zd tmp[CPB * P];
-  __shared__ ChebTables T;
+  constexpr int CPB = 256 / (KC * P);
+  constexpr int NQ = (4 + KC - 1) / KC;  // children per lane
+  ChebTables& T = *reinterpret_cast<ChebTables*>(lds);
+  zd* buf = lds + sizeof(ChebTables) / sizeof(zd);
+  zd* tmp = buf + CPB * P;
   for (int q = threadIdx.x; q < (int)(sizeof(ChebTables) / sizeof(double)); q += blockDim.x)
-    ((double*)&T)[q] = ((const double*)tabs)[q];
-  __syncthreads();
-  const int cl = threadIdx.x / P, j = threadIdx.x % P;
+    ((double*)&T)[q] = ((const double*)a.tabs)[q];
+  const int cl = threadIdx.x / (KC * P), rem = threadIdx.x % (KC * P);
+  const int j = rem / KC, kc = rem % KC;
   const int c = lb * CPB + cl;
-  const bool active = cl < CPB && c < npc;
-  const int g = pcone_off + c;
+  const bool active = cl < CPB && c < a.npc;
+  const int g = a.pcone_off + c;
+  zd acc{0.0, 0.0}, ifac{0.0, 0.0};
   if (active) {
-    zd acc{0.0, 0.0};
-    const long long e1 = pr_ptr[g + 1];
-    for (long long e = pr_ptr[g] + j; e < e1; e += P) {
-      const Geo4 G = pr_geo[e];
-      const zd v = cone_eval<Ps, Pang>(coef + (size_t)pr_cone[e] * P, G.s, G.t);
-      cfma(acc, v, zd{G.fre, G.fim});
+    if (kc == 0) ifac = a.pn_ifac[(size_t)g * P + j];
+    const long long e0 = a.pr_ptr[g];
+    const int nch = (int)((a.pr_ptr[g + 1] - e0) / P);
+    int cid[NQ];
+    Geo4 G[NQ];
+#pragma unroll
+    for (int u = 0; u < NQ; u++) {
+      const int q = kc + u * KC;
+      if (q < nch) {
+        const long long e = e0 + (long long)q * P + j;
+        cid[u] = a.pr_cone[e];
+        G[u] = a.pr_geo[e];
+      }
     }
-    buf[threadIdx.x] = cmul(acc, pn_ifac[(size_t)g * P + j]);
+#pragma unroll
+    for (int u = 0; u < NQ; u++) {
+      if (kc + u * KC < nch) {
+        const zd v = cone_eval<Ps, Pang>(a.coef + (size_t)cid[u] * P, G[u].s, G[u].t);
+        cfma(acc, v, zd{G[u].fre, G[u].fim});
+      }
+    }
   }
+  acc = lane_group_sum<KC>(acc);
+  if (active && kc == 0) buf[cl * P + j] = cmul(acc, ifac);
   __syncthreads();
-  const int base = cl * P;
-  const zd out = cone_transform<Ps, Pang>(buf + (active ? base : 0), tmp + (active ? base : 0), T,
-                                          active ? j : 0, active);
-  if (active) coef[(size_t)g * P + j] = out;
+  const int tc = threadIdx.x / P, tj = threadIdx.x % P;
+  const bool tact = tc < CPB && lb * CPB + tc < a.npc;
+  const zd out = cone_transform<Ps, Pang>(buf + (tact ? tc * P : 0), tmp + (tact ? tc * P : 0), T,
+                                          tact ? tj : 0, tact);
+  if (tact) a.coef[(size_t)(a.pcone_off + lb * CPB + tc) * P + tj] = out;
+}
+
+template <int Ps, int Pang, int KC>
+__global__ void __launch_bounds__(256) k_prop_stored(PropArgs a) {
+  __shared__ __align__(16) unsigned char lds[prop_lds_bytes<Ps, Pang, KC>()];
+  prop_block<Ps, Pang, KC>(a, xcd_block(), reinterpret_cast<zd*>(lds));
 }
 
 // Propagation into the parent cones of level l-1 from the child cones of
@@ -571,21 +699,34 @@ __global__ void __launch_bounds__(256)
   if (i < N && sl == 0) sol[i] = cadd(sol[i], acc);
 }
 
+// y[i] = ident x[i] + sum_e W_e x[patch_e] + far[inv[i]]: S lanes per target
+// split the correction products; lane 0 adds the identity and far-field terms.
+template <int S>
 __global__ void __launch_bounds__(256)
     k_finalize(int N, int Np, double ident, const zd* __restrict__ x,
                const long long* __restrict__ corr_ptr, const int* __restrict__ corr_patch,
                const zd* __restrict__ corr_w, const int* __restrict__ inv,
-               const zd* __restrict__ sol, zd* __restrict__ y) {
+               const zd* __restrict__ sol, const zd* __restrict__ part, int nparts, zd* __restrict__ y) {
   const int lb = xcd_block();
-  const int i = lb * blockDim.x + threadIdx.x;
-  if (i >= N) return;
-  zd acc = cscale(x[i], ident);
-  for (long long e = corr_ptr[i]; e < corr_ptr[i + 1]; e++) {
-    const zd* w = corr_w + e * Np;
-    const zd* xs = x + (size_t)corr_patch[e] * Np;
-    for (int q = 0; q < Np; q++) cfma(acc, w[q], xs[q]);
+  const int gt = lb * blockDim.x + threadIdx.x;
+  const int i = gt / S, sl = gt % S;
+  zd acc{0.0, 0.0};
+  if (i < N) {
+    const long long e0 = corr_ptr[i], e1 = corr_ptr[i + 1];
+    const long long nitem = (e1 - e0) * Np;
+    for (long long it = sl; it < nitem; it += S) {
+      const long long e = e0 + it / Np;
+      const int q = (int)(it % Np);
+      cfma(acc, corr_w[e * Np + q], x[(size_t)corr_patch[e] * Np + q]);
+    }
   }
-  y[i] = cadd(acc, sol[inv[i]]);
+  acc = lane_group_sum<S>(acc);
+  if (i < N && sl == 0) {
+    const int si = inv[i];
+    zd far = sol[si];
+    for (int l = 0; l < nparts; l++) far = cadd(far, part[(size_t)l * N + si]);
+    y[i] = cadd(cadd(cscale(x[i], ident), acc), far);
+  }
 }
 
 // Direct sum at a target subset, excluding the same pairs the correction

@@ -27,6 +27,7 @@ static int fail(ifgf_plan* p, int code, const std::string& msg) {
 int ifgf::plan_fail(ifgf_plan* p, int code, const std::string& msg) { return fail(p, code, msg); }
 
 static bool build_stored_any(ifgf_plan* P);
+static void autotune(ifgf_plan* P);
 
 static void build_device_side(ifgf_plan* P) {
   const Tree& t = P->tree;
@@ -167,7 +168,7 @@ int ifgf_plan_points(const ifgf_plan* p, double* x, double* y, double* nx, doubl
   return IFGF_OK;
 }
 
-int ifgf_plan_stats(const ifgf_plan* p, long long out[12]) {
+int ifgf_plan_stats(const ifgf_plan* p, long long out[16]) {
   if (!p || !out) return fail(nullptr, IFGF_ERR_ARG, "null argument");
   out[0] = p->n_near;
   out[1] = p->tree.n_leaf;
@@ -181,6 +182,10 @@ int ifgf_plan_stats(const ifgf_plan* p, long long out[12]) {
   out[9] = p->stored ? 1 : 0;
   out[10] = p->stored ? (long long)p->nf_val.n : 0;
   out[11] = p->stored_bytes;
+  out[12] = p->near_lanes;
+  out[13] = p->leaf_lanes;
+  out[14] = p->prop_lanes;
+  out[15] = p->final_lanes;
   return IFGF_OK;
 }
 
@@ -222,11 +227,71 @@ long long ifgf_plan_correction(const ifgf_plan* p, long long* ptr, int* patch, d
 constexpr int kNearLanes = 8;
 constexpr int kInterpLanes = 4;
 constexpr int kLeafApplyLanes = 4;
+constexpr int kFinalLanes = 8;
 template <int P>
 struct LeafLanes {
   static constexpr int value = (P * 2 <= 256) ? 2 : 1;
 };
 
+// Per-stage timing events of one application: [2s] / [2s+1] bracket stage s
+// (gather, near, leaf, propagation, interpolation, finalize) on the stream the
+// stage runs on.
+static int events_per_apply(const ifgf_plan*) { return 12; }
+
+static hipEvent_t* take_events(ifgf_plan* P) {
+  const size_t per = events_per_apply(P);
+  if (P->ev_used + per > P->ev_pool.size()) {
+    const size_t old = P->ev_pool.size();
+    P->ev_pool.resize(old + per * 64, nullptr);
+    for (size_t q = old; q < P->ev_pool.size(); q++) HIP_TRY(hipEventCreate(&P->ev_pool[q]));
+  }
+  hipEvent_t* ev = &P->ev_pool[P->ev_used];
+  P->ev_used += per;
+  return ev;
+}
+
+template <int Ps, int Pang, int KC>
+static void launch_prop_stored(ifgf_plan* P, int npc, int pcone_off, hipStream_t st) {
+  constexpr int CPB = 256 / (KC * Ps * Pang);
+  const PropArgs a{P->pr_ptr.p, P->pr_cone.p, P->pr_geo.p, P->pn_ifac.p, P->tabs.p, P->coef.p, npc, pcone_off};
+  hipLaunchKernelGGL((k_prop_stored<Ps, Pang, KC>), dim3((npc + CPB - 1) / CPB), dim3(256), 0, st, a);
+}
+
+static NearArgs near_args(ifgf_plan* P) {
+  return NearArgs{P->leaf_of_point.p, P->nb_seg.p, P->nf_off.p, P->nf_val.p, P->rho.p, P->sol.p};
+}
+
+template <int S>
+static void launch_near_stored(ifgf_plan* P, hipStream_t st) {
+  const int N = (int)P->tree.N;
+  hipLaunchKernelGGL((k_near_apply<S>), dim3((N + 256 / S - 1) / (256 / S)), dim3(256), 0, st, N, near_args(P));
+}
+
+static InterpArgs interp_args(ifgf_plan* P) {
+  return InterpArgs{P->itasks.p, P->itask_cones.p, P->it_lvl.p, P->ent_slot.p, P->it_geo.p,
+                    P->coef.p,   P->part.p,        (int)P->tree.N, P->tree.D};
+}
+
+// Dispatch on the autotuned propagation lanes (KC * Ps * Pang <= 256).
+template <int Ps, int Pang, typename Fn1, typename Fn2, typename Fn4>
+static void with_prop_lanes(int lanes, Fn1 f1, Fn2 f2, Fn4 f4) {
+  constexpr int Pn = Ps * Pang;
+  if constexpr (4 * Pn <= 256) {
+    if (lanes == 4) return f4();
+  }
+  if constexpr (2 * Pn <= 256) {
+    if (lanes >= 2) return f2();
+  }
+  f1();
+}
+
+// One application y = A x on stream st: gather -> near field -> leaf cones ->
+// propagation chain -> interpolation -> correction/identity/unsort. (Two other
+// schedules were measured slower on the 64k kite and dropped: the near field
+// and interpolation on a second stream beside the propagation chain, and one
+// launch per level co-scheduling propagation, that level's interpolation and
+// a slice of the near field. Neither overlapped: the kernels' times added up
+// under contention.)
 template <int F, int Ps, int Pang>
 static void launch_ifgf(ifgf_plan* P, const zd* x, zd* y, hipStream_t st) {
   constexpr int Pn = Ps * Pang;
@@ -237,70 +302,127 @@ static void launch_ifgf(ifgf_plan* P, const zd* x, zd* y, hipStream_t st) {
   const double k = P->prm.k, eta = P->prm.eta;
   const int nb = (N + 255) / 256;
   const bool prof = P->profile;
-  hipEvent_t* ev = nullptr;
-  if (prof) {
-    if (P->ev_used + 7 > P->ev_pool.size()) {
-      const size_t old = P->ev_pool.size();
-      P->ev_pool.resize(old + 7 * 64, nullptr);
-      for (size_t q = old; q < P->ev_pool.size(); q++) HIP_TRY(hipEventCreate(&P->ev_pool[q]));
-    }
-    ev = &P->ev_pool[P->ev_used];
-    P->ev_used += 7;
-    HIP_TRY(hipEventRecord(ev[0], st));
-  }
+  hipEvent_t* ev = prof ? take_events(P) : nullptr;
+  auto mark = [&](int i) {
+    if (prof) HIP_TRY(hipEventRecord(ev[i], st));
+  };
+
+  mark(0);
   hipLaunchKernelGGL(k_gather, dim3(nb), dim3(256), 0, st, N, x, P->w.p, P->perm.p, P->rho.p);
-  if (prof) HIP_TRY(hipEventRecord(ev[1], st));
-  if (P->stored)
-    hipLaunchKernelGGL((k_near_apply<kNearLanes>), dim3((N + 256 / kNearLanes - 1) / (256 / kNearLanes)),
-                       dim3(256), 0, st, N, P->leaf_of_point.p, P->leaf_begin.p, P->nb_ptr.p, P->nb_list.p,
-                       P->nf_off.p, P->nf_val.p, P->rho.p, P->sol.p);
-  else
-    hipLaunchKernelGGL((k_near<F, kNearLanes>), dim3((N + 256 / kNearLanes - 1) / (256 / kNearLanes)), dim3(256), 0, st, N, P->px.p, P->py.p, P->pnx.p, P->pny.p,
-                       P->patch.p, P->excl_d.p, P->leaf_of_point.p, P->leaf_begin.p, P->nb_ptr.p,
-                       P->nb_list.p, P->rho.p, P->sol.p, k, eta);
-  if (prof) HIP_TRY(hipEventRecord(ev[2], st));
-  const LevelInfo& LD = t.lv[D];
-  constexpr int LCPB = 256 / (Pn * kLeafLanes);
+  mark(1);
+
+  // near field
+  mark(2);
   if (P->stored) {
-    const long long nthreads = (long long)LD.ncone * Pn * kLeafApplyLanes;
-    hipLaunchKernelGGL((k_leaf_apply<Pn, kLeafApplyLanes>), dim3((unsigned)((nthreads + 255) / 256)), dim3(256), 0,
-                       st, LD.ncone, LD.cone_off, P->cone_box.p, P->leaf_begin.p, P->lf_off.p, P->lf_val.p,
-                       P->rho.p, P->coef.p);
+    if (P->near_lanes == 4) launch_near_stored<4>(P, st);
+    else if (P->near_lanes == 16) launch_near_stored<16>(P, st);
+    else launch_near_stored<8>(P, st);
   } else {
+    hipLaunchKernelGGL((k_near<F, kNearLanes>), dim3((N + 256 / kNearLanes - 1) / (256 / kNearLanes)), dim3(256), 0,
+                       st, N, P->px.p, P->py.p, P->pnx.p, P->pny.p, P->patch.p, P->excl_d.p, P->leaf_of_point.p,
+                       P->leaf_begin.p, P->nb_ptr.p, P->nb_list.p, P->rho.p, P->sol.p, k, eta);
+  }
+  mark(3);
+
+  // leaf cones
+  mark(4);
+  const LevelInfo& LD = t.lv[D];
+  if (P->stored) {
+#define IFGF_LEAF(S)                                                                                           \
+  hipLaunchKernelGGL((k_leaf_apply<Pn, S>), dim3((unsigned)(((long long)LD.ncone * Pn * S + 255) / 256)), dim3(256), \
+                     0, st, LD.ncone, LD.cone_off, P->cone_box.p, P->leaf_begin.p, P->lf_off.p, P->lf_val.p,     \
+                     P->rho.p, P->coef.p)
+    if (P->leaf_lanes == 2) IFGF_LEAF(2);
+    else if (P->leaf_lanes == 8) IFGF_LEAF(8);
+    else IFGF_LEAF(4);
+#undef IFGF_LEAF
+  } else {
+    constexpr int LCPB = 256 / (Pn * kLeafLanes);
     hipLaunchKernelGGL((k_leaf<F, Ps, Pang, kLeafLanes>), dim3((LD.ncone + LCPB - 1) / LCPB), dim3(256), 0, st,
                        LD.ncone, LD.cone_off, P->cg.p, P->cone_box.p, P->leaf_begin.p, P->px.p, P->py.p,
                        P->pnx.p, P->pny.p, P->rho.p, P->coef.p, P->tabs.p, k, eta);
   }
-  if (prof) HIP_TRY(hipEventRecord(ev[3], st));
-  for (int l = D; l >= 3; l--) {
-    const LevelInfo& L = t.lv[l];
-    const LevelInfo& Pl = t.lv[l - 1];
-    if (P->stored) {
-      hipLaunchKernelGGL((k_prop_stored<Ps, Pang>), dim3((Pl.ncone + CPB - 1) / CPB), dim3(256), 0, st, Pl.ncone,
-                         Pl.cone_off, P->pr_ptr.p, P->pr_cone.p, P->pr_geo.p, P->pn_ifac.p, P->coef.p, P->tabs.p);
-      continue;
+  mark(5);
+
+  // propagation chain
+  mark(6);
+  {
+    for (int l = D; l >= 3; l--) {
+      const LevelInfo& L = t.lv[l];
+      const LevelInfo& Pl = t.lv[l - 1];
+      if (P->stored) {
+        with_prop_lanes<Ps, Pang>(
+            P->prop_lanes, [&] { launch_prop_stored<Ps, Pang, 1>(P, Pl.ncone, (int)Pl.cone_off, st); },
+            [&] {
+              if constexpr (2 * Pn <= 256) launch_prop_stored<Ps, Pang, 2>(P, Pl.ncone, (int)Pl.cone_off, st);
+            },
+            [&] {
+              if constexpr (4 * Pn <= 256) launch_prop_stored<Ps, Pang, 4>(P, Pl.ncone, (int)Pl.cone_off, st);
+            });
+      } else {
+        hipLaunchKernelGGL((k_prop<Ps, Pang>), dim3((Pl.ncone + CPB - 1) / CPB), dim3(256), 0, st, Pl.ncone,
+                           Pl.cone_off, Pl.box_off, Pl.nrad, L.cone_off, L.nslot, P->cg.p, P->cone_box.p,
+                           P->cone_qd.p, P->box_child.p, P->proto.p + L.proto_off, P->dense_map.p + L.map_off,
+                           P->coef.p, P->tabs.p, k);
+      }
     }
-    hipLaunchKernelGGL((k_prop<Ps, Pang>), dim3((Pl.ncone + CPB - 1) / CPB), dim3(256), 0, st, Pl.ncone,
-                       Pl.cone_off, Pl.box_off, Pl.nrad, L.cone_off, L.nslot, P->cg.p, P->cone_box.p,
-                       P->cone_qd.p, P->box_child.p, P->proto.p + L.proto_off, P->dense_map.p + L.map_off,
-                       P->coef.p, P->tabs.p, k);
   }
-  if (prof) HIP_TRY(hipEventRecord(ev[4], st));
-  if (P->stored)
-    hipLaunchKernelGGL((k_interp_stored<Ps, Pang, kInterpLanes>), dim3((N + 256 / kInterpLanes - 1) / (256 / kInterpLanes)),
-                       dim3(256), 0, st, N, P->it_ptr.p, P->it_cone.p, P->it_geo.p, P->coef.p, P->sol.p);
-  else
-  hipLaunchKernelGGL((k_interp<Ps, Pang, kInterpLanes>), dim3((N + 256 / kInterpLanes - 1) / (256 / kInterpLanes)), dim3(256), 0, st, N, P->px.p, P->py.p, P->it_ptr.p,
-                     P->it_cone.p, P->cg.p, P->coef.p, P->sol.p, k);
-  if (prof) HIP_TRY(hipEventRecord(ev[5], st));
+  mark(7);
+
+  // interpolation: stored path, one launch over the LDS tasks of all levels
+  // (each task writes its level's partial sums); on the fly, one pass over
+  // all points
+  mark(8);
+  if (P->stored) {
+    if (P->nitasks > 0)
+      hipLaunchKernelGGL((k_interp_lds<Ps, Pang, kInterpLanes>), dim3(P->nitasks), dim3(256), P->itask_lds, st,
+                         interp_args(P));
+  } else {
+    hipLaunchKernelGGL((k_interp<Ps, Pang, kInterpLanes>), dim3((N + 256 / kInterpLanes - 1) / (256 / kInterpLanes)),
+                       dim3(256), 0, st, N, P->px.p, P->py.p, P->it_ptr.p, P->it_cone.p, P->cg.p, P->coef.p,
+                       P->sol.p, k);
+  }
+  mark(9);
+
+  // correction, identity and unsort
+  mark(10);
   const double ident = (F == kSingleLayer) ? 0.0 : 0.5;
-  hipLaunchKernelGGL(k_finalize, dim3(nb), dim3(256), 0, st, N, P->geo.Np, ident, x, P->corr_ptr.p,
-                     P->corr_patch.p, P->corr_w.p, P->inv.p, P->sol.p, y);
-  if (prof) HIP_TRY(hipEventRecord(ev[6], st));
+#define IFGF_FIN(S)                                                                                            \
+  hipLaunchKernelGGL(k_finalize<S>, dim3((N + 256 / S - 1) / (256 / S)), dim3(256), 0, st, N, P->geo.Np, ident, x, \
+                     P->corr_ptr.p, P->corr_patch.p, P->corr_w.p, P->inv.p, P->sol.p, P->part.p,                  \
+                     P->stored ? D - 1 : 0, y)
+  if (P->final_lanes == 4) IFGF_FIN(4);
+  else if (P->final_lanes == 16) IFGF_FIN(16);
+  else IFGF_FIN(8);
+#undef IFGF_FIN
+  mark(11);
   HIP_TRY(hipGetLastError());
 }
 
 // Evaluates the stored operators once (plan creation).
+// Groups the targets of each box and level into LDS interpolation tasks of at
+// most 256 / kInterpLanes targets whose distinct cousin cones (at most
+// max_cones) are staged in LDS; entry slots index those staged cones.
+static void build_itasks(ifgf_plan* P, int max_cones) {
+  const Tree& t = P->tree;
+  const int Pn = t.Ps * t.Pang;
+  std::vector<InterpTask> tasks;
+  std::vector<int> tcones;
+  std::vector<unsigned short> slots;
+  std::string err;
+  if (!build_interp_tasks(t, 256 / kInterpLanes, max_cones, tasks, tcones, slots, err))
+    throw std::runtime_error(err);
+  std::vector<ITask> dt(tasks.size());
+  for (size_t q = 0; q < tasks.size(); q++)
+    dt[q] = ITask{tasks[q].li, tasks[q].t0, tasks[q].nt, tasks[q].cone_off, tasks[q].ncone, 0};
+  P->itasks.upload(dt);
+  P->itask_cones.upload(tcones);
+  P->ent_slot.upload(slots);
+  P->nitasks = (int)dt.size();
+  P->itask_max_cones = max_cones;
+  P->itask_lds = (size_t)max_cones * Pn * sizeof(zd);
+}
+
 template <int F, int Ps, int Pang>
 static void build_stored(ifgf_plan* P) {
   constexpr int Pn = Ps * Pang;
@@ -318,6 +440,24 @@ static void build_stored(ifgf_plan* P) {
     nf_off[i + 1] = nf_off[i] + c;
   }
   P->nf_off.upload(nf_off);
+  // per leaf box, its neighbour boxes as (exclusive end in the flattened
+  // segment, source index minus flat offset), padded to kMaxNb segments
+  P->nleaf = (int)t.leaf_begin.size() - 1;
+  std::vector<int> seg((size_t)P->nleaf * kNbSegInts, 0);
+  for (int b = 0; b < P->nleaf; b++) {
+    const int nn = t.nb_ptr[b + 1] - t.nb_ptr[b];
+    if (nn > kMaxNb) throw std::runtime_error("a leaf box has more than 9 neighbour boxes");
+    int* sg = &seg[(size_t)b * kNbSegInts];
+    int cum = 0, q = 0;
+    for (int n = t.nb_ptr[b]; n < t.nb_ptr[b + 1]; n++, q++) {
+      const int s0 = t.leaf_begin[t.nb_list[n]], len = t.leaf_begin[t.nb_list[n] + 1] - s0;
+      sg[2 * q + 1] = s0 - cum;
+      cum += len;
+      sg[2 * q] = cum;
+    }
+    for (; q < kMaxNb; q++) sg[2 * q] = cum;
+  }
+  P->nb_seg.upload(std::vector<int4>((const int4*)seg.data(), (const int4*)seg.data() + (size_t)P->nleaf * kNbSegInts / 4));
   P->nf_val.alloc(nf_off[N]);
   hipLaunchKernelGGL((k_near_build<F, kNearLanes>), dim3((N + 256 / kNearLanes - 1) / (256 / kNearLanes)),
                      dim3(256), 0, st, N, P->px.p, P->py.p, P->pnx.p, P->pny.p, P->patch.p, P->excl_d.p,
@@ -345,6 +485,10 @@ static void build_stored(ifgf_plan* P) {
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(st));
   }
+  // interpolation tasks with LDS-staged cones
+  P->it_lvl.upload(t.it_lvl);
+  P->part.alloc((size_t)(D - 1) * N);
+  build_itasks(P, std::max<int>(27, (int)(32768 / (Pn * sizeof(zd)))));
   // interpolation geometry per (target, cousin cone)
   P->it_geo.alloc(t.it_ptr[N]);
   hipLaunchKernelGGL((k_interp_build<kInterpLanes>), dim3((N + 256 / kInterpLanes - 1) / (256 / kInterpLanes)),
@@ -434,13 +578,84 @@ static bool dispatch(ifgf_plan* P, const zd* x, zd* y, hipStream_t st) {
 }
 
 // accumulate the recorded applications' stage times and recycle their events
+// Times whole applications on the plan's stream for each candidate lane count
+// of each stored-path stage (the others held fixed) and keeps the fastest.
+static void autotune(ifgf_plan* P) {
+  hipStream_t st = P->stream;
+  const long long N = P->geo.N;
+  std::vector<zd> xh(N);
+  for (long long i = 0; i < N; i++) xh[i] = zd{std::cos(0.37 * i), std::sin(0.11 * i)};
+  HIP_TRY(hipMemcpy(P->xbuf.p, xh.data(), N * sizeof(zd), hipMemcpyHostToDevice));
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  auto time_apply = [&]() {
+    for (int w = 0; w < 2; w++) dispatch(P, P->xbuf.p, P->ybuf.p, st);
+    HIP_TRY(hipEventRecord(e0, st));
+    for (int r = 0; r < 5; r++) dispatch(P, P->xbuf.p, P->ybuf.p, st);
+    HIP_TRY(hipEventRecord(e1, st));
+    HIP_TRY(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+    return ms;
+  };
+  struct Knob {
+    int* v;
+    std::vector<int> cand;
+  };
+  const int Pn = P->prm.Ps * P->prm.Pang;
+  std::vector<int> prop_c{1};
+  if (2 * Pn <= 256) prop_c.push_back(2);
+  if (4 * Pn <= 256) prop_c.push_back(4);
+  Knob knobs[] = {{&P->near_lanes, {4, 8, 16}},
+                  {&P->leaf_lanes, {2, 4, 8}},
+                  {&P->prop_lanes, prop_c},
+                  {&P->final_lanes, {4, 8, 16}}};
+  // LDS per interpolation task (cones staged per task): smaller tasks stage
+  // some cones more than once but let more blocks share a CU
+  if (P->stored) {
+    std::vector<int> cand;
+    for (int kb : {8192, 16384, 32768}) {
+      const int c = std::max<int>(27, (int)(kb / (Pn * sizeof(zd))));
+      if (std::find(cand.begin(), cand.end(), c) == cand.end()) cand.push_back(c);
+    }
+    int best = P->itask_max_cones;
+    float tbest = 1e30f;
+    for (int c : cand) {
+      build_itasks(P, c);
+      const float t = time_apply();
+      if (t < tbest) {
+        tbest = t;
+        best = c;
+      }
+    }
+    if (best != P->itask_max_cones) build_itasks(P, best);
+  }
+  for (auto& kb : knobs) {
+    int best = *kb.v;
+    float tbest = 1e30f;
+    for (int c : kb.cand) {
+      *kb.v = c;
+      const float t = time_apply();
+      if (t < tbest) {
+        tbest = t;
+        best = c;
+      }
+    }
+    *kb.v = best;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+}
+
 static void collect_profile(ifgf_plan* P) {
-  for (size_t a = 0; a + 7 <= P->ev_used; a += 7) {
+  const size_t per = events_per_apply(P);
+  for (size_t a = 0; a + per <= P->ev_used; a += per) {
     hipEvent_t* ev = &P->ev_pool[a];
-    HIP_TRY(hipEventSynchronize(ev[6]));
+    HIP_TRY(hipEventSynchronize(ev[11]));
     for (int s = 0; s < 6; s++) {
       float ms = 0.f;
-      HIP_TRY(hipEventElapsedTime(&ms, ev[s], ev[s + 1]));
+      HIP_TRY(hipEventElapsedTime(&ms, ev[2 * s], ev[2 * s + 1]));
       P->stage_ms[s] += ms;
     }
     P->prof_count++;
@@ -452,7 +667,7 @@ static void collect_profile(ifgf_plan* P) {
 int ifgf::apply_device(ifgf_plan* P, const zd* x, zd* y, hipStream_t st) {
   if (!dispatch(P, x, y, st)) return fail(P, IFGF_ERR_ARG, "no kernel instantiation for this (Ps, Pang)");
   // bound the number of outstanding event sets
-  if (P->profile && P->ev_used >= 7 * 4096) collect_profile(P);
+  if (P->profile && P->ev_used >= (size_t)events_per_apply(P) * 4096) collect_profile(P);
   return IFGF_OK;
 }
 

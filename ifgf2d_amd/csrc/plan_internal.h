@@ -78,6 +78,19 @@ struct ifgf_plan {
   ifgf::DevBuf<zd> nf_val, lf_val, pn_ifac;
   ifgf::DevBuf<ifgf::Geo4> it_geo, pr_geo;
   ifgf::DevBuf<int> pr_cone;
+  // LDS interpolation tasks
+  ifgf::DevBuf<ifgf::ITask> itasks;
+  ifgf::DevBuf<int> itask_cones;
+  ifgf::DevBuf<unsigned short> ent_slot;
+  ifgf::DevBuf<long long> it_lvl;
+  ifgf::DevBuf<zd> part;
+  int nitasks = 0;
+  ifgf::DevBuf<int4> nb_seg;  // per leaf box: kMaxNb (end, source shift) pairs
+  int nleaf = 0;
+  // lanes per work item of the stored-path kernels (chosen by the autotuner)
+  int near_lanes = 8, leaf_lanes = 4, prop_lanes = 1, final_lanes = 8;
+  size_t itask_lds = 0;
+  int itask_max_cones = 0;
   long long stored_bytes = 0;
   // workspaces
   ifgf::DevBuf<zd> rho, sol, coef, xbuf, ybuf;

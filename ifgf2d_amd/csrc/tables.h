@@ -21,6 +21,11 @@ struct ChebTables {
   double Ta[16 * 16];       // Ta[n*16 + l] = T_n(xa[l]) * alpha_n / (Ps*Pang)
 };
 
+// One LDS interpolation task (mirrors setup.h's InterpTask).
+struct ITask {
+  int li, t0, nt, cone_off, ncone, pad;
+};
+
 // Local Chebyshev coordinates (s, theta) of a point in a cone and the centre
 // factor exp(i k r)/sqrt(r) at its distance r from the cone's box centre.
 struct Geo4 {

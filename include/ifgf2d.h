@@ -51,6 +51,8 @@ extern "C" {
                                   geometry-only near-field values and the folded leaf
                                   operator are evaluated once at plan creation and
                                   streamed from HBM by every application) */
+#define IFGF_PLAN_AUTOTUNE 4   /* time the candidate lane counts of the stored-path
+                                  kernels on this plan and keep the fastest */
 
 /* apply flags */
 #define IFGF_PTR_DEVICE 1     /* x / y are device pointers; the call is asynchronous on stream */
@@ -78,7 +80,7 @@ typedef struct {
   int Ps, Pang;   /* Chebyshev nodes per cone in s and theta (BoxTree<Ps,Pang>) */
   int nlevels;    /* tree levels, >= 3 */
   int nrhs_max;   /* largest nrhs passed to ifgf_apply */
-  int flags;      /* IFGF_PLAN_HOST_ONLY | IFGF_PLAN_ON_THE_FLY */
+  int flags;      /* IFGF_PLAN_HOST_ONLY | IFGF_PLAN_ON_THE_FLY | IFGF_PLAN_AUTOTUNE */
 } ifgf_params;
 
 /* Replaces ForwardMap::ForwardMap (ForwardMap.tpp:15-35) followed by
@@ -108,8 +110,9 @@ int ifgf_plan_points(const ifgf_plan* plan, double* x, double* y, double* nx, do
  * [4] relevant cones (all levels), [5] correction entries, [6] nlevels,
  * [7] N, [8] leaf-level cones, [9] 1 if the near-field values and leaf
  * operator are stored, [10] stored near-field entries (excluded pairs
- * included as zeros), [11] bytes of the stored operators. */
-int ifgf_plan_stats(const ifgf_plan* plan, long long out[12]);
+ * included as zeros), [11] bytes of the stored operators, [12..15] lanes per
+ * work item of the near-field, leaf, propagation and correction kernels. */
+int ifgf_plan_stats(const ifgf_plan* plan, long long out[16]);
 
 /* BoxTree::getSorting() (BoxTree.h:1163-1165): Morton position -> patch-order index. */
 int ifgf_plan_sorting(const ifgf_plan* plan, long long* sorting);
@@ -179,7 +182,10 @@ int ifgf_hankel01(const double* x, long long n, double* h0, double* h1);
  * ifgf_plan_stage_times waits for the recorded applications and returns the
  * accumulated time of [0] intensities+sort, [1] near field, [2] leaf cones,
  * [3] propagation, [4] interpolation, [5] correction+unsort, and in *count
- * the number of applications recorded. */
+ * the number of applications recorded. Each stage time is the span of its
+ * kernels on the application's stream. Recording the events costs time of
+ * its own (several microseconds per stage), so applications timed for
+ * throughput should run with profiling off. */
 int ifgf_plan_profile(ifgf_plan* plan, int enable);
 int ifgf_plan_stage_times(ifgf_plan* plan, double times_ms[6], long long* count);
 
