@@ -34,6 +34,9 @@ __device__ inline int xcd_block() {
   return x * base + (x < rem ? x : rem) + i;
 }
 
+// Two doubles as one 16-byte vector load (a complex value).
+typedef double dbl2 __attribute__((ext_vector_type(2)));
+
 // Functions::Atan2 (PolarCoordinates.h:8-27): theta in [0, 2 pi).
 __device__ inline double atan2_02pi(double y, double x) {
   if (x > 0 && y >= 0) return atan(y / x);
@@ -234,9 +237,26 @@ __global__ void __launch_bounds__(256)
 // only on the geometry; they are evaluated once per plan and each application
 // then streams them from HBM.
 //
-// Near field: per target i a contiguous segment at nf_off[i] holding one
-// value per (neighbour box ascending, source ascending) pair, 0 for excluded
-// pairs (same loop order as k_near).
+// Near field: target i holds one value per (neighbour box in order, source
+// ascending) pair at flat offset o (0 for excluded pairs, same loop order as
+// k_near). Storage is grouped for coalescing: the kNearGroup consecutive
+// targets of group g = i / kNearGroup are interleaved in chunks of kNearChunk
+// offsets, value (i, o) at
+//   nf_goff[g] + (o / kNearChunk) * kNearGroup * kNearChunk
+//              + (i % kNearGroup) * kNearChunk + o % kNearChunk,
+// each target padded with zeros to the group's longest segment (rounded up to
+// a chunk). A wave of kNearGroup targets x kNearChunk lanes then reads one
+// contiguous kilobyte per load instruction.
+constexpr int kNearGroup = 8;
+constexpr int kNearChunk = 8;
+constexpr int kMaxNb = 9;
+constexpr int kNbSegInts = 20;  // 9 (end, shift) pairs padded to 5 int4
+
+__device__ inline long long near_index(long long goff, int i, int o) {
+  return goff + (long long)(o / kNearChunk) * (kNearGroup * kNearChunk) + (i % kNearGroup) * kNearChunk +
+         o % kNearChunk;
+}
+
 template <int F, int S>
 __global__ void __launch_bounds__(256)
     k_near_build(int N, const double* __restrict__ px, const double* __restrict__ py,
@@ -244,14 +264,14 @@ __global__ void __launch_bounds__(256)
                  const int* __restrict__ patch, const int4* __restrict__ excl,
                  const int* __restrict__ leaf_of_point, const int* __restrict__ leaf_begin,
                  const int* __restrict__ nb_ptr, const int* __restrict__ nb_list,
-                 const long long* __restrict__ nf_off, zd* __restrict__ nf_val, double k, double eta) {
+                 const long long* __restrict__ nf_goff, zd* __restrict__ nf_val, double k, double eta) {
   const int gt = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = gt / S, sl = gt % S;
   if (i >= N) return;
   const double tx = px[i], ty = py[i];
   const int4 e = excl[i];
   const int b = leaf_of_point[i];
-  zd* out = nf_val + nf_off[i];
+  const long long goff = nf_goff[i / kNearGroup];
   int base = 0;
   for (int n = nb_ptr[b]; n < nb_ptr[b + 1]; n++) {
     const int nb = nb_list[n];
@@ -261,66 +281,78 @@ __global__ void __launch_bounds__(256)
       zd g{0.0, 0.0};
       if (!(ps == e.x || ps == e.y || ps == e.z || ps == e.w))
         g = green_d<F>(tx - px[s], ty - py[s], pnx[s], pny[s], k, eta);
-      out[base + (s - s0)] = g;
+      nf_val[near_index(goff, i, base + (s - s0))] = g;
     }
     base += s1 - s0;
   }
 }
 
-
-// Stored near field for targets [n0, n1). The values of target i are the
-// flattened neighbour boxes' sources (in neighbour order), so flat offset o
-// maps to source o + shift[q] for the segment q with end[q-1] <= o < end[q];
-// the kMaxNb (end, shift) pairs of the target's leaf box come from nb_seg.
-// S lanes per target; lane sl takes every S-th offset, and the loop body has
-// no load-to-load dependency, so the unrolled iterations' loads are in flight
-// together (the values stream from HBM).
-constexpr int kMaxNb = 9;
-constexpr int kNbSegInts = 20;  // 9 (end, shift) pairs padded to 5 int4
-
+// Stored near field for targets [n0, n1), n0 a multiple of kNearGroup:
+// kNearChunk lanes per target, lane sl takes the offsets o = sl (mod
+// kNearChunk). Flat offset o maps to source o + shift[q] for the segment q
+// with end[q-1] <= o < end[q]; the kMaxNb (end, shift) pairs of target i
+// (those of its leaf box) are nf_seg[i] (the last end is the segment length).
+// All per-target loads are issued together before the loop, and the loop body
+// has no load-to-load dependency, so the unrolled iterations' loads are in
+// flight together; the values are read once per application and bypass the
+// caches (non-temporal loads).
 struct NearArgs {
-  const int* leaf_of_point;
-  const int4* nb_seg;
-  const long long* nf_off;
+  const int4* nf_seg;
+  const long long* nf_goff;
   const zd* nf_val;
   const zd* rho;
   zd* sol;
 };
 
-template <int S>
 __device__ inline void near_block(const NearArgs& a, int lb, int n0, int n1) {
+  constexpr int S = kNearChunk;
   const int gt = lb * blockDim.x + threadIdx.x;
   const int i = n0 + gt / S, sl = gt % S;
   zd acc{0.0, 0.0};
   if (i < n1) {
-    const long long o0 = a.nf_off[i];
-    const int len = (int)(a.nf_off[i + 1] - o0);
-    const int4* sg = a.nb_seg + (size_t)a.leaf_of_point[i] * (kNbSegInts / 4);
+    const int4* sg = a.nf_seg + (size_t)i * (kNbSegInts / 4);
+    const long long goff = a.nf_goff[i / kNearGroup];
     const int4 q0 = sg[0], q1 = sg[1], q2 = sg[2], q3 = sg[3], q4 = sg[4];
+    // consume every loaded value once here, so the loads are issued together
+    // instead of being sunk below the loop-entry test
+    asm volatile("" ::"v"(q0.x ^ q0.y ^ q0.z ^ q0.w ^ q1.x ^ q1.y ^ q1.z ^ q1.w ^ q2.x ^ q2.y ^ q2.z ^ q2.w ^
+                          q3.x ^ q3.y ^ q3.z ^ q3.w ^ q4.x ^ q4.y ^ (int)goff));
     const int end[kMaxNb - 1] = {q0.x, q0.z, q1.x, q1.z, q2.x, q2.z, q3.x, q3.z};
     const int sh[kMaxNb] = {q0.y, q0.w, q1.y, q1.w, q2.y, q2.w, q3.y, q3.w, q4.y};
-    const zd* __restrict__ v = a.nf_val + o0;
+    const int len = q4.x;
+    const dbl2* __restrict__ v = reinterpret_cast<const dbl2*>(a.nf_val) + near_index(goff, i, sl);
     const zd* __restrict__ rho = a.rho;
 #pragma unroll 4
-    for (int o = sl; o < len; o += S) {
+    for (int o = sl; o < len; o += S, v += kNearGroup * kNearChunk) {
       int src = o + sh[0];
 #pragma unroll
       for (int q = 1; q < kMaxNb; q++) src = (o >= end[q - 1]) ? o + sh[q] : src;
-      cfma(acc, v[o], rho[src]);
+      const dbl2 w = *v;
+      cfma(acc, zd{w.x, w.y}, rho[src]);
     }
   }
   acc = lane_group_sum<S>(acc);
   if (i < n1 && sl == 0) a.sol[i] = acc;
 }
 
-template <int S>
-__global__ void __launch_bounds__(256) k_near_apply(int N, NearArgs a) {
-  near_block<S>(a, xcd_block(), 0, N);
+__global__ void __launch_bounds__(256) k_near_apply(int N, NearArgs a) { near_block(a, xcd_block(), 0, N); }
+
+// Leaf operator L_c (P x n_src per leaf cone c). Storage: the sources of a
+// cone are padded to a multiple of kLeafChunk, and the padded columns
+// (col = lf_off[c] / P + src, a multiple of kLeafChunk at every cone's start)
+// are tiled kLeafChunk at a time: value (col, m) at
+//   (col / kLeafChunk) * kLeafChunk * P + m * kLeafChunk + col % kLeafChunk,
+// so the lanes (m, src mod kLeafChunk) of the apply read consecutive values.
+// Step 1: kernel values divided by the node's centre factor, one thread per
+// (cone, source, node) of the padded columns (0 in the padding), written
+// column-major to tmp[col * P + j].
+constexpr int kLeafChunk = 4;
+
+template <int P>
+__device__ inline long long leaf_index(long long col, int m) {
+  return (col / kLeafChunk) * (kLeafChunk * P) + m * kLeafChunk + col % kLeafChunk;
 }
 
-// Leaf operator L_c (P x n_src per leaf cone c), stored source-major at
-// lf_off[c]: lf[src * P + m]. Step 1: kernel values divided by the node's
-// centre factor, one thread per (cone, node, source).
 template <int F, int Ps, int Pang>
 __global__ void __launch_bounds__(256)
     k_leaf_build_values(int ncone, int cone_off, const ConeGeo* __restrict__ cg,
@@ -347,6 +379,10 @@ __global__ void __launch_bounds__(256)
   const long long local = it - item_off[c];
   const int j = (int)(local % P);
   const int src = (int)(local / P);
+  if (src >= leaf_begin[b + 1] - s0) {
+    tmp[lf_off[c] + (long long)src * P + j] = zd{0.0, 0.0};
+    return;
+  }
   const ConeGeo G = cg[g];
   double nx, ny, r;
   cone_node(G, T->xs[j % Ps], T->xa[j / Ps], nx, ny, r);
@@ -378,15 +414,16 @@ __global__ void __launch_bounds__(256)
     acc.x = fma(row.x, T->Ta[n * 16 + l], acc.x);
     acc.y = fma(row.y, T->Ta[n * 16 + l], acc.y);
   }
-  lf[col * P + m] = acc;
+  lf[leaf_index<P>(col, m)] = acc;
 }
 
-// coef[leaf cone][m] = sum_src lf[src][m] rho[src]: threads (cone, m, slice).
-template <int P, int S>
+// coef[leaf cone][m] = sum_src L[src][m] rho[s0 + src]: threads (cone, m,
+// slice), kLeafChunk slices; lf_src[c] = (s0, n_src) of the cone's box.
+template <int P>
 __global__ void __launch_bounds__(256)
-    k_leaf_apply(int ncone, int cone_off, const int* __restrict__ cone_box,
-                 const int* __restrict__ leaf_begin, const long long* __restrict__ lf_off,
+    k_leaf_apply(int ncone, int cone_off, const int2* __restrict__ lf_src, const long long* __restrict__ lf_off,
                  const zd* __restrict__ lf, const zd* __restrict__ rho, zd* __restrict__ coef) {
+  constexpr int S = kLeafChunk;
   const int lb = xcd_block();
   const long long gt = (long long)lb * blockDim.x + threadIdx.x;
   const long long item = gt / S;
@@ -394,12 +431,16 @@ __global__ void __launch_bounds__(256)
   const int c = (int)(item / P), m = (int)(item % P);
   zd acc{0.0, 0.0};
   if (c < ncone) {
-    const int g = cone_off + c;
-    const int b = cone_box[g];
-    const int s0 = leaf_begin[b], ns = leaf_begin[b + 1] - s0;
-    const zd* L = lf + lf_off[c] + m;
+    const int2 sr = lf_src[c];
+    const long long off = lf_off[c];
+    asm volatile("" ::"v"(sr.x ^ sr.y ^ (int)off));  // issue both loads before the loop test
+    const dbl2* __restrict__ L = reinterpret_cast<const dbl2*>(lf) + off + m * S + sl;
+    const zd* __restrict__ r = rho + sr.x;
 #pragma unroll 4
-    for (int src = sl; src < ns; src += S) cfma(acc, L[(long long)src * P], rho[s0 + src]);
+    for (int src = sl; src < sr.y; src += S, L += S * P) {
+      const dbl2 w = *L;
+      cfma(acc, zd{w.x, w.y}, r[src]);
+    }
   }
   acc = lane_group_sum<S>(acc);
   if (c < ncone && sl == 0) coef[(size_t)(cone_off + c) * P + m] = acc;
@@ -499,10 +540,14 @@ __device__ inline void interp_block(const InterpArgs& a, int q, zd* cs) {
   if (act && sl == 0) a.part[(size_t)task.li * a.N + i] = acc;
 }
 
+// Tasks are ordered by their first target (Morton order) and then level, so
+// the XCD remap gives every XCD one region of the curve with all its levels:
+// the XCDs get similar work, and neighbouring tasks of a level, which share
+// most of their cones, meet in one XCD's L2.
 template <int Ps, int Pang, int S>
 __global__ void __launch_bounds__(256) k_interp_lds(InterpArgs a) {
   extern __shared__ zd cs[];
-  interp_block<Ps, Pang, S>(a, blockIdx.x, cs);
+  interp_block<Ps, Pang, S>(a, xcd_block(), cs);
 }
 
 // Parent cones of level l-1 (global ids pcone_off..+npc): for node j and the

@@ -180,7 +180,7 @@ int ifgf_plan_stats(const ifgf_plan* p, long long out[16]) {
   out[7] = p->geo.N;
   out[8] = p->tree.lv[p->tree.D].ncone;
   out[9] = p->stored ? 1 : 0;
-  out[10] = p->stored ? (long long)p->nf_val.n : 0;
+  out[10] = p->stored ? p->near_entries : 0;
   out[11] = p->stored_bytes;
   out[12] = p->near_lanes;
   out[13] = p->leaf_lanes;
@@ -226,7 +226,6 @@ long long ifgf_plan_correction(const ifgf_plan* p, long long* ptr, int* patch, d
 // lanes that share one target (near field, interpolation) or one leaf cone node
 constexpr int kNearLanes = 8;
 constexpr int kInterpLanes = 4;
-constexpr int kLeafApplyLanes = 4;
 constexpr int kFinalLanes = 8;
 template <int P>
 struct LeafLanes {
@@ -258,13 +257,13 @@ static void launch_prop_stored(ifgf_plan* P, int npc, int pcone_off, hipStream_t
 }
 
 static NearArgs near_args(ifgf_plan* P) {
-  return NearArgs{P->leaf_of_point.p, P->nb_seg.p, P->nf_off.p, P->nf_val.p, P->rho.p, P->sol.p};
+  return NearArgs{P->nf_seg.p, P->nf_goff.p, P->nf_val.p, P->rho.p, P->sol.p};
 }
 
-template <int S>
 static void launch_near_stored(ifgf_plan* P, hipStream_t st) {
   const int N = (int)P->tree.N;
-  hipLaunchKernelGGL((k_near_apply<S>), dim3((N + 256 / S - 1) / (256 / S)), dim3(256), 0, st, N, near_args(P));
+  constexpr int TPB = 256 / kNearChunk;
+  hipLaunchKernelGGL(k_near_apply, dim3((N + TPB - 1) / TPB), dim3(256), 0, st, N, near_args(P));
 }
 
 static InterpArgs interp_args(ifgf_plan* P) {
@@ -314,9 +313,7 @@ static void launch_ifgf(ifgf_plan* P, const zd* x, zd* y, hipStream_t st) {
   // near field
   mark(2);
   if (P->stored) {
-    if (P->near_lanes == 4) launch_near_stored<4>(P, st);
-    else if (P->near_lanes == 16) launch_near_stored<16>(P, st);
-    else launch_near_stored<8>(P, st);
+    launch_near_stored(P, st);
   } else {
     hipLaunchKernelGGL((k_near<F, kNearLanes>), dim3((N + 256 / kNearLanes - 1) / (256 / kNearLanes)), dim3(256), 0,
                        st, N, P->px.p, P->py.p, P->pnx.p, P->pny.p, P->patch.p, P->excl_d.p, P->leaf_of_point.p,
@@ -328,14 +325,9 @@ static void launch_ifgf(ifgf_plan* P, const zd* x, zd* y, hipStream_t st) {
   mark(4);
   const LevelInfo& LD = t.lv[D];
   if (P->stored) {
-#define IFGF_LEAF(S)                                                                                           \
-  hipLaunchKernelGGL((k_leaf_apply<Pn, S>), dim3((unsigned)(((long long)LD.ncone * Pn * S + 255) / 256)), dim3(256), \
-                     0, st, LD.ncone, LD.cone_off, P->cone_box.p, P->leaf_begin.p, P->lf_off.p, P->lf_val.p,     \
-                     P->rho.p, P->coef.p)
-    if (P->leaf_lanes == 2) IFGF_LEAF(2);
-    else if (P->leaf_lanes == 8) IFGF_LEAF(8);
-    else IFGF_LEAF(4);
-#undef IFGF_LEAF
+    hipLaunchKernelGGL((k_leaf_apply<Pn>), dim3((unsigned)(((long long)LD.ncone * Pn * kLeafChunk + 255) / 256)),
+                       dim3(256), 0, st, LD.ncone, LD.cone_off, P->lf_src.p, P->lf_off.p, P->lf_val.p, P->rho.p,
+                       P->coef.p);
   } else {
     constexpr int LCPB = 256 / (Pn * kLeafLanes);
     hipLaunchKernelGGL((k_leaf<F, Ps, Pang, kLeafLanes>), dim3((LD.ncone + LCPB - 1) / LCPB), dim3(256), 0, st,
@@ -412,9 +404,17 @@ static void build_itasks(ifgf_plan* P, int max_cones) {
   std::string err;
   if (!build_interp_tasks(t, 256 / kInterpLanes, max_cones, tasks, tcones, slots, err))
     throw std::runtime_error(err);
+  // launch order: by first target, then level (stable)
+  std::vector<size_t> order(tasks.size());
+  for (size_t q = 0; q < order.size(); q++) order[q] = q;
+  std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+    return tasks[a].t0 != tasks[b].t0 ? tasks[a].t0 < tasks[b].t0 : tasks[a].li < tasks[b].li;
+  });
   std::vector<ITask> dt(tasks.size());
-  for (size_t q = 0; q < tasks.size(); q++)
-    dt[q] = ITask{tasks[q].li, tasks[q].t0, tasks[q].nt, tasks[q].cone_off, tasks[q].ncone, 0};
+  for (size_t q = 0; q < tasks.size(); q++) {
+    const InterpTask& s = tasks[order[q]];
+    dt[q] = ITask{s.li, s.t0, s.nt, s.cone_off, s.ncone, 0};
+  }
   P->itasks.upload(dt);
   P->itask_cones.upload(tcones);
   P->ent_slot.upload(slots);
@@ -430,16 +430,28 @@ static void build_stored(ifgf_plan* P) {
   const int N = (int)t.N, D = t.D;
   const double k = P->prm.k, eta = P->prm.eta;
   hipStream_t st = P->stream;
-  // near field: segment length of target i = points of its leaf neighbours
-  std::vector<long long> nf_off(N + 1, 0);
+  // near field: segment length of target i = points of its leaf neighbours;
+  // groups of kNearGroup targets padded to their longest segment
+  std::vector<int> nf_len(N);
+  long long near_entries = 0;
   for (int i = 0; i < N; i++) {
     const int b = t.leaf_of_point[i];
-    long long c = 0;
+    int c = 0;
     for (int n = t.nb_ptr[b]; n < t.nb_ptr[b + 1]; n++)
       c += t.leaf_begin[t.nb_list[n] + 1] - t.leaf_begin[t.nb_list[n]];
-    nf_off[i + 1] = nf_off[i] + c;
+    nf_len[i] = c;
+    near_entries += c;
   }
-  P->nf_off.upload(nf_off);
+  const int ngroups = (N + kNearGroup - 1) / kNearGroup;
+  std::vector<long long> nf_goff(ngroups + 1, 0);
+  for (int g = 0; g < ngroups; g++) {
+    int mx = 0;
+    for (int i = g * kNearGroup; i < std::min(N, (g + 1) * kNearGroup); i++) mx = std::max(mx, nf_len[i]);
+    const long long chunks = (mx + kNearChunk - 1) / kNearChunk;
+    nf_goff[g + 1] = nf_goff[g] + chunks * kNearGroup * kNearChunk;
+  }
+  P->nf_goff.upload(nf_goff);
+  P->near_entries = near_entries;
   // per leaf box, its neighbour boxes as (exclusive end in the flattened
   // segment, source index minus flat offset), padded to kMaxNb segments
   P->nleaf = (int)t.leaf_begin.size() - 1;
@@ -457,21 +469,33 @@ static void build_stored(ifgf_plan* P) {
     }
     for (; q < kMaxNb; q++) sg[2 * q] = cum;
   }
-  P->nb_seg.upload(std::vector<int4>((const int4*)seg.data(), (const int4*)seg.data() + (size_t)P->nleaf * kNbSegInts / 4));
-  P->nf_val.alloc(nf_off[N]);
+  {
+    std::vector<int4> per_target((size_t)N * (kNbSegInts / 4));
+    for (int i = 0; i < N; i++)
+      std::memcpy(&per_target[(size_t)i * (kNbSegInts / 4)], &seg[(size_t)t.leaf_of_point[i] * kNbSegInts],
+                  kNbSegInts * sizeof(int));
+    P->nf_seg.upload(per_target);
+  }
+  P->nf_val.alloc(nf_goff[ngroups]);
+  HIP_TRY(hipMemsetAsync(P->nf_val.p, 0, nf_goff[ngroups] * sizeof(zd), st));
   hipLaunchKernelGGL((k_near_build<F, kNearLanes>), dim3((N + 256 / kNearLanes - 1) / (256 / kNearLanes)),
                      dim3(256), 0, st, N, P->px.p, P->py.p, P->pnx.p, P->pny.p, P->patch.p, P->excl_d.p,
-                     P->leaf_of_point.p, P->leaf_begin.p, P->nb_ptr.p, P->nb_list.p, P->nf_off.p,
+                     P->leaf_of_point.p, P->leaf_begin.p, P->nb_ptr.p, P->nb_list.p, P->nf_goff.p,
                      P->nf_val.p, k, eta);
   HIP_TRY(hipGetLastError());
   // leaf operator: P x n_src per leaf cone
   const LevelInfo& LD = t.lv[D];
+  // (sources padded to a multiple of kLeafChunk; lf_src = (first source, count))
   std::vector<long long> lf_off(LD.ncone + 1, 0);
+  std::vector<int2> lf_src(LD.ncone);
   for (int c = 0; c < LD.ncone; c++) {
     const int b = t.cone_box[LD.cone_off + c];
-    lf_off[c + 1] = lf_off[c] + (long long)Pn * (t.leaf_begin[b + 1] - t.leaf_begin[b]);
+    const int ns = t.leaf_begin[b + 1] - t.leaf_begin[b];
+    lf_src[c] = make_int2(t.leaf_begin[b], ns);
+    lf_off[c + 1] = lf_off[c] + (long long)Pn * ((ns + kLeafChunk - 1) / kLeafChunk * kLeafChunk);
   }
   P->lf_off.upload(lf_off);
+  P->lf_src.upload(lf_src);
   P->lf_val.alloc(lf_off[LD.ncone]);
   {
     DevBuf<zd> tmp;
@@ -519,7 +543,7 @@ static void build_stored(ifgf_plan* P) {
   }
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(st));
-  P->stored_bytes = (long long)(nf_off[N] + lf_off[LD.ncone] + (long long)t.n_cones * Pn) * (long long)sizeof(zd) +
+  P->stored_bytes = (long long)(nf_goff[ngroups] + lf_off[LD.ncone] + (long long)t.n_cones * Pn) * (long long)sizeof(zd) +
                     (long long)(t.it_ptr[N] + pr_ptr[t.n_cones]) * (long long)sizeof(Geo4) +
                     pr_ptr[t.n_cones] * (long long)sizeof(int);
   P->stored = true;
@@ -607,9 +631,7 @@ static void autotune(ifgf_plan* P) {
   std::vector<int> prop_c{1};
   if (2 * Pn <= 256) prop_c.push_back(2);
   if (4 * Pn <= 256) prop_c.push_back(4);
-  Knob knobs[] = {{&P->near_lanes, {4, 8, 16}},
-                  {&P->leaf_lanes, {2, 4, 8}},
-                  {&P->prop_lanes, prop_c},
+  Knob knobs[] = {{&P->prop_lanes, prop_c},
                   {&P->final_lanes, {4, 8, 16}}};
   // LDS per interpolation task (cones staged per task): smaller tasks stage
   // some cones more than once but let more blocks share a CU

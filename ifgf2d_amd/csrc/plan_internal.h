@@ -74,7 +74,8 @@ struct ifgf_plan {
   ifgf::DevBuf<ifgf::ChebTables> tabs;
   // stored operators (near-field values, folded leaf operator)
   bool stored = false;
-  ifgf::DevBuf<long long> nf_off, lf_off, pr_ptr;
+  ifgf::DevBuf<long long> nf_goff, lf_off, pr_ptr;
+  long long near_entries = 0;  // stored near-field values without the group padding
   ifgf::DevBuf<zd> nf_val, lf_val, pn_ifac;
   ifgf::DevBuf<ifgf::Geo4> it_geo, pr_geo;
   ifgf::DevBuf<int> pr_cone;
@@ -85,10 +86,11 @@ struct ifgf_plan {
   ifgf::DevBuf<long long> it_lvl;
   ifgf::DevBuf<zd> part;
   int nitasks = 0;
-  ifgf::DevBuf<int4> nb_seg;  // per leaf box: kMaxNb (end, source shift) pairs
+  ifgf::DevBuf<int2> lf_src;  // per leaf cone: (first source, source count)
+  ifgf::DevBuf<int4> nf_seg;  // per target: its leaf box's kMaxNb (end, source shift) pairs
   int nleaf = 0;
   // lanes per work item of the stored-path kernels (chosen by the autotuner)
-  int near_lanes = 8, leaf_lanes = 4, prop_lanes = 1, final_lanes = 8;
+  int near_lanes = 8 /* kNearChunk */, leaf_lanes = 4 /* kLeafChunk */, prop_lanes = 1, final_lanes = 8;
   size_t itask_lds = 0;
   int itask_max_cones = 0;
   long long stored_bytes = 0;
