@@ -147,25 +147,44 @@ def main():
     ext = fm.extra_stats()
     work = stage_work(dict(st, **ext), fm.formulation, P, 5, 5, ext["stored"])
     nlev = kw["nlevels"]
+    stored = ext["stored"]
+    if stored:
+        # one push launch per level carries propagation and interpolation;
+        # it streams 40 B of (s, theta, factor, destination) per evaluation and
+        # 64 B of child slots + 16 B factor per parent node, and writes 16 B per evaluation
+        flop = work["propagation"][1] + work["interpolation"][1]
+        nev = st["prop"] + st["interp"]
+        byts = 56.0 * nev + 80.0 * 25 * (st["cones"] - ext["leaf_cones"])
+        work["propagation"] = ("mixed", (flop, byts), "FLOP+B")
+        work.pop("interpolation")
     dom = max(work, key=lambda s: stage_avg[s])
     bound, amount, unit = work[dom]
     secs = stage_avg[dom] * 1e-3
-    launches = {"near": 1, "leaf": 1, "propagation": nlev - 3, "interpolation": 1}
+    launches = {"near": 1, "leaf": 1, "propagation": (nlev - 2) if stored else (nlev - 3), "interpolation": 1}
+    other = None
+    if bound == "mixed":
+        flop, byts = amount
+        f_flop = flop / secs / 1e12 / PEAK_FP64_TFLOPS
+        f_byte = byts / secs / 1e9 / PEAK_HBM_GBS
+        other = {"flop_per_application": flop, "bytes_per_application": byts,
+                 "fp64_frac": round(f_flop, 4), "hbm_frac": round(f_byte, 4)}
+        bound, amount, unit = ("hbm", byts, "B") if f_byte >= f_flop else ("mfma", flop, "FLOP")
     if bound == "hbm":
         achieved, peak, runit = amount / secs / 1e9, PEAK_HBM_GBS, "GB/s"
     else:
         achieved, peak, runit = amount / secs / 1e12, PEAK_FP64_TFLOPS, "TFLOP/s"
-    stored = ext["stored"]
     kname = {"near": "k_near_apply" if stored else "k_near",
              "leaf": "k_leaf_apply" if stored else "k_leaf",
-             "propagation": "k_prop_stored" if stored else "k_prop",
-             "interpolation": "k_interp_lds" if stored else "k_interp"}[dom]
+             "propagation": "k_level_push" if stored else "k_prop",
+             "interpolation": "k_interp"}[dom]
     roofline = {"bound": bound, "kernel": kname, "achieved": round(achieved, 3), "peak": peak,
                 "unit": runit, "frac": round(achieved / peak, 4), "traffic": None,
                 "algorithmic_per_application": amount, "algorithmic_unit": unit,
                 "launches_per_application": launches[dom],
                 "algorithmic_per_launch": amount / launches[dom],
                 "avg_launch_ms": stage_avg[dom] / launches[dom]}
+    if other:
+        roofline["both_resources"] = other
     # the on-the-fly variant of the same operator (kernels re-evaluated per application)
     otf = None
     if not args.no_on_the_fly:

@@ -236,7 +236,7 @@ class ForwardMap:
         s = self._stats()
         return {"leaf_cones": s[8], "prop_cones": s[4] - s[8], "stored": bool(s[9]),
                 "near_stored": s[10], "stored_bytes": s[11],
-                "lanes": {"near": s[12], "leaf": s[13], "prop": s[14], "finalize": s[15]}}
+                "lanes": {"near": s[12], "leaf": s[13], "push_task_entries": s[14], "finalize": s[15]}}
 
     def sorting(self):
         s = np.empty(self.N, dtype=np.int64)

@@ -466,90 +466,6 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-template <int Ps, int Pang, int S>
-__global__ void __launch_bounds__(256)
-    k_interp_stored(int N, const long long* __restrict__ it_ptr, const int* __restrict__ it_cone,
-                    const Geo4* __restrict__ it_geo, const zd* __restrict__ coef, zd* __restrict__ sol) {
-  const int lb = xcd_block();
-  constexpr int P = Ps * Pang;
-  const int gt = lb * blockDim.x + threadIdx.x;
-  const int i = gt / S, sl = gt % S;
-  zd acc{0.0, 0.0};
-  if (i < N) {
-    const long long e1 = it_ptr[i + 1];
-#pragma unroll 2
-    for (long long e = it_ptr[i] + sl; e < e1; e += S) {
-      const Geo4 G = it_geo[e];
-      const zd v = cone_eval<Ps, Pang>(coef + (size_t)it_cone[e] * P, G.s, G.t);
-      cfma(acc, v, zd{G.fre, G.fim});
-    }
-  }
-  acc = lane_group_sum<S>(acc);
-  if (i < N && sl == 0) sol[i] = cadd(sol[i], acc);
-}
-
-// Interpolation with the task's cones staged in LDS (cs, ncone * P values):
-// block per task, S lanes per target over its entries at the task's level;
-// writes the level's partial sum part[li * N + i] (summed over levels in
-// fixed order by k_finalize).
-struct InterpArgs {
-  const ITask* tasks;
-  const int* task_cones;
-  const long long* it_lvl;
-  const unsigned short* ent_slot;
-  const Geo4* it_geo;
-  const zd* coef;
-  zd* part;
-  int N, D;
-};
-
-template <int Ps, int Pang, int S>
-__device__ inline void interp_block(const InterpArgs& a, int q, zd* cs) {
-  constexpr int P = Ps * Pang;
-  const ITask task = a.tasks[q];
-  const int il = threadIdx.x / S, sl = threadIdx.x % S;
-  const int i = task.t0 + il;
-  const bool act = il < task.nt;
-  // entry range and first entry: loaded before the barrier so that their
-  // latency overlaps the staging of the task's cones
-  long long e = 0, e1 = 0;
-  if (act) {
-    e = a.it_lvl[(size_t)i * a.D + task.li] + sl;
-    e1 = a.it_lvl[(size_t)i * a.D + task.li + 1];
-  }
-  Geo4 G{0.0, 0.0, 0.0, 0.0};
-  int slot = 0;
-  if (e < e1) {
-    G = a.it_geo[e];
-    slot = a.ent_slot[e];
-  }
-  for (int r = threadIdx.x; r < task.ncone * P; r += blockDim.x)
-    cs[r] = a.coef[(size_t)a.task_cones[task.cone_off + r / P] * P + r % P];
-  __syncthreads();
-  zd acc{0.0, 0.0};
-  while (e < e1) {
-    const zd v = cone_eval<Ps, Pang>(cs + slot * P, G.s, G.t);
-    cfma(acc, v, zd{G.fre, G.fim});
-    e += S;
-    if (e < e1) {
-      G = a.it_geo[e];
-      slot = a.ent_slot[e];
-    }
-  }
-  acc = lane_group_sum<S>(acc);
-  if (act && sl == 0) a.part[(size_t)task.li * a.N + i] = acc;
-}
-
-// Tasks are ordered by their first target (Morton order) and then level, so
-// the XCD remap gives every XCD one region of the curve with all its levels:
-// the XCDs get similar work, and neighbouring tasks of a level, which share
-// most of their cones, meet in one XCD's L2.
-template <int Ps, int Pang, int S>
-__global__ void __launch_bounds__(256) k_interp_lds(InterpArgs a) {
-  extern __shared__ zd cs[];
-  interp_block<Ps, Pang, S>(a, xcd_block(), cs);
-}
-
 // Parent cones of level l-1 (global ids pcone_off..+npc): for node j and the
 // k-th relevant child, entry pr_ptr[g] + k*P + j holds the child cone id and
 // the node's (s, theta, factor) in it; pn_ifac[g*P + j] = 1/factor of the node.
@@ -587,81 +503,101 @@ __global__ void __launch_bounds__(256)
 }
 
 
-// Propagation into parent cones pcone_off..+npc of level l-1 with stored
-// geometry. Threads (parent cone, node, child lane): KC lanes per node share
-// its (up to 4) children, reduced in fixed order; CPB parent cones per
-// 256-thread block (KC * P <= 256). The work is latency-bound (a few hundred
-// blocks per level), so the table copy into LDS is only waited for at the one
-// barrier before the transform, and the entries of all children are loaded
-// before any is used. LDS: ChebTables, then 2 * CPB * P values.
-struct PropArgs {
-  const long long* pr_ptr;
-  const int* pr_cone;
-  const Geo4* pr_geo;
-  const zd* pn_ifac;
+// ---------------------------------------------------------------------------
+// Stored path, one launch per tree level l = D..2 ("push" form of
+// BoxTree::Propagation and BoxTree::Interpolation, BoxTree.h:767-937).
+//
+// Each block owns a task: up to 256 / P cones of level l (or a slice of one
+// cone's entries) and a contiguous range of that level's push entries. The
+// block first forms the cones' coefficients in LDS:
+//   l = D: the leaf operator's output coef[g][m] (already transformed);
+//   l < D: node value v_j = (p_0 + p_1 + p_2 + p_3) / factor_j from the four
+//          child slots p_k = pbuf[(g P + j) 4 + k] that the level below wrote
+//          (empty slots stay 0), then the separable 2D transform.
+// It then evaluates those interpolants at every push entry (cone, s, theta,
+// factor, destination) of the task and writes value * factor to
+// pbuf[destination]: either the k-th child slot of a parent node at level
+// l-1 (propagation) or the slot of one (target, cousin cone) interpolation
+// entry (summed per target in fixed order by k_finalize). Every destination
+// slot has exactly one writer, so no atomics and a fixed summation order.
+// A level costs one dependent round trip (the four child slots) before the
+// LDS work, instead of the pull form's entry range -> child entries -> child
+// coefficients chain.
+struct PushArgs {
+  const PushTask* tasks;
+  const Geo4* geo;       // per push entry: (s, theta, centre factor)
+  const int2* dst;       // per push entry: (destination slot in pbuf, global cone id)
+  const zd* coef;        // leaf coefficients (level D)
+  const zd* pn_ifac;     // 1 / centre factor per (cone, node), levels < D
   const ChebTables* tabs;
-  zd* coef;
-  int npc, pcone_off;
+  zd* pbuf;              // child slots of every (cone, node), then interpolation entries
+  int leaf;              // 1 at level D
 };
 
-template <int Ps, int Pang, int KC>
-constexpr size_t prop_lds_bytes() {
-  return sizeof(ChebTables) + 2 * sizeof(zd) * (256 / (KC * Ps * Pang)) * Ps * Pang;
-}
-
-template <int Ps, int Pang, int KC>
-__device__ inline void prop_block(const PropArgs& a, int lb, zd* lds) {
+// EPT entries per thread: a task holds at most 256 * EPT entries, and every
+// thread loads all of its entries together with the cones' inputs before the
+// first barrier, so a level costs one memory round trip before the LDS work.
+template <int Ps, int Pang, int EPT>
+__global__ void __launch_bounds__(256) k_level_push(PushArgs a) {
   constexpr int P = Ps * Pang;
-  constexpr int CPB = 256 / (KC * P);
-  constexpr int NQ = (4 + KC - 1) / KC;  // children per lane
-  ChebTables& T = *reinterpret_cast<ChebTables*>(lds);
-  zd* buf = lds + sizeof(ChebTables) / sizeof(zd);
-  zd* tmp = buf + CPB * P;
-  for (int q = threadIdx.x; q < (int)(sizeof(ChebTables) / sizeof(double)); q += blockDim.x)
-    ((double*)&T)[q] = ((const double*)a.tabs)[q];
-  const int cl = threadIdx.x / (KC * P), rem = threadIdx.x % (KC * P);
-  const int j = rem / KC, kc = rem % KC;
-  const int c = lb * CPB + cl;
-  const bool active = cl < CPB && c < a.npc;
-  const int g = a.pcone_off + c;
-  zd acc{0.0, 0.0}, ifac{0.0, 0.0};
-  if (active) {
-    if (kc == 0) ifac = a.pn_ifac[(size_t)g * P + j];
-    const long long e0 = a.pr_ptr[g];
-    const int nch = (int)((a.pr_ptr[g + 1] - e0) / P);
-    int cid[NQ];
-    Geo4 G[NQ];
+  constexpr int CMAX = 256 / P;
+  __shared__ ChebTables T;
+  __shared__ zd buf[CMAX * P];
+  __shared__ zd tmp[CMAX * P];
+  const PushTask task = a.tasks[xcd_block()];
+  const int tid = threadIdx.x;
+  if (!a.leaf)
+    for (int q = tid; q < (int)(sizeof(ChebTables) / sizeof(double)); q += blockDim.x)
+      ((double*)&T)[q] = ((const double*)a.tabs)[q];
+  const int cl = tid / P, j = tid % P;
+  const bool act = cl < task.nc;
+  const long long g = (long long)task.c0 + cl;
+  Geo4 G[EPT];
+  int2 d[EPT];
 #pragma unroll
-    for (int u = 0; u < NQ; u++) {
-      const int q = kc + u * KC;
-      if (q < nch) {
-        const long long e = e0 + (long long)q * P + j;
-        cid[u] = a.pr_cone[e];
-        G[u] = a.pr_geo[e];
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < NQ; u++) {
-      if (kc + u * KC < nch) {
-        const zd v = cone_eval<Ps, Pang>(a.coef + (size_t)cid[u] * P, G[u].s, G[u].t);
-        cfma(acc, v, zd{G[u].fre, G[u].fim});
-      }
+  for (int u = 0; u < EPT; u++) {
+    const int e = tid + u * 256;
+    if (e < task.ne) {
+      G[u] = a.geo[task.e0 + e];
+      d[u] = a.dst[task.e0 + e];
     }
   }
-  acc = lane_group_sum<KC>(acc);
-  if (active && kc == 0) buf[cl * P + j] = cmul(acc, ifac);
+  zd v{0.0, 0.0};
+  if (act) {
+    if (a.leaf) {
+      v = a.coef[g * P + j];
+    } else {
+      const zd* s = a.pbuf + (g * P + j) * 4;
+      const zd s0 = s[0], s1 = s[1], s2 = s[2], s3 = s[3];
+      v = cmul(cadd(cadd(cadd(s0, s1), s2), s3), a.pn_ifac[g * P + j]);
+    }
+    buf[tid] = v;
+  }
   __syncthreads();
-  const int tc = threadIdx.x / P, tj = threadIdx.x % P;
-  const bool tact = tc < CPB && lb * CPB + tc < a.npc;
-  const zd out = cone_transform<Ps, Pang>(buf + (tact ? tc * P : 0), tmp + (tact ? tc * P : 0), T,
-                                          tact ? tj : 0, tact);
-  if (tact) a.coef[(size_t)(a.pcone_off + lb * CPB + tc) * P + tj] = out;
+  if (!a.leaf) {
+    const zd c = cone_transform<Ps, Pang>(buf + (act ? cl * P : 0), tmp + (act ? cl * P : 0), T, act ? j : 0, act);
+    // every thread has passed the transform's barrier, so buf is free again
+    if (act) buf[tid] = c;
+    __syncthreads();
+  }
+#pragma unroll
+  for (int u = 0; u < EPT; u++) {
+    if (tid + u * 256 < task.ne) {
+      const zd val = cone_eval<Ps, Pang>(buf + (d[u].y - task.c0) * P, G[u].s, G[u].t);
+      a.pbuf[d[u].x] = cmul(val, zd{G[u].fre, G[u].fim});
+    }
+  }
 }
 
-template <int Ps, int Pang, int KC>
-__global__ void __launch_bounds__(256) k_prop_stored(PropArgs a) {
-  __shared__ __align__(16) unsigned char lds[prop_lds_bytes<Ps, Pang, KC>()];
-  prop_block<Ps, Pang, KC>(a, xcd_block(), reinterpret_cast<zd*>(lds));
+// Build: push entry q takes its geometry from the propagation (src < npr)
+// or the interpolation (src - npr) geometry.
+__global__ void __launch_bounds__(256)
+    k_push_gather(long long n, const long long* __restrict__ src, long long npr, const Geo4* __restrict__ pr_geo,
+                  const Geo4* __restrict__ it_geo, Geo4* __restrict__ out) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const long long s = src[q];
+  out[q] = s < npr ? pr_geo[s] : it_geo[s - npr];
 }
 
 // Propagation into the parent cones of level l-1 from the child cones of
@@ -745,18 +681,23 @@ __global__ void __launch_bounds__(256)
 }
 
 // y[i] = ident x[i] + sum_e W_e x[patch_e] + far[inv[i]]: S lanes per target
-// split the correction products; lane 0 adds the identity and far-field terms.
+// split the correction products and the target's interpolation entries
+// (stored path: ient[it_ptr[si] .. it_ptr[si+1]) in pbuf, one per (level,
+// cousin cone)); lane 0 adds the identity and the near field.
 template <int S>
 __global__ void __launch_bounds__(256)
     k_finalize(int N, int Np, double ident, const zd* __restrict__ x,
                const long long* __restrict__ corr_ptr, const int* __restrict__ corr_patch,
                const zd* __restrict__ corr_w, const int* __restrict__ inv,
-               const zd* __restrict__ sol, const zd* __restrict__ part, int nparts, zd* __restrict__ y) {
+               const zd* __restrict__ sol, const long long* __restrict__ it_ptr, const zd* __restrict__ ient,
+               zd* __restrict__ y) {
   const int lb = xcd_block();
   const int gt = lb * blockDim.x + threadIdx.x;
   const int i = gt / S, sl = gt % S;
-  zd acc{0.0, 0.0};
+  zd acc{0.0, 0.0}, far{0.0, 0.0};
+  int si = 0;
   if (i < N) {
+    si = inv[i];
     const long long e0 = corr_ptr[i], e1 = corr_ptr[i + 1];
     const long long nitem = (e1 - e0) * Np;
     for (long long it = sl; it < nitem; it += S) {
@@ -764,14 +705,14 @@ __global__ void __launch_bounds__(256)
       const int q = (int)(it % Np);
       cfma(acc, corr_w[e * Np + q], x[(size_t)corr_patch[e] * Np + q]);
     }
+    if (ient) {
+      const long long f1 = it_ptr[si + 1];
+      for (long long f = it_ptr[si] + sl; f < f1; f += S) far = cadd(far, ient[f]);
+    }
   }
   acc = lane_group_sum<S>(acc);
-  if (i < N && sl == 0) {
-    const int si = inv[i];
-    zd far = sol[si];
-    for (int l = 0; l < nparts; l++) far = cadd(far, part[(size_t)l * N + si]);
-    y[i] = cadd(cadd(cscale(x[i], ident), acc), far);
-  }
+  far = lane_group_sum<S>(far);
+  if (i < N && sl == 0) y[i] = cadd(cadd(cscale(x[i], ident), acc), cadd(sol[si], far));
 }
 
 // Direct sum at a target subset, excluding the same pairs the correction

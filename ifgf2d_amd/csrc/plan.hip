@@ -184,7 +184,7 @@ int ifgf_plan_stats(const ifgf_plan* p, long long out[16]) {
   out[11] = p->stored_bytes;
   out[12] = p->near_lanes;
   out[13] = p->leaf_lanes;
-  out[14] = p->prop_lanes;
+  out[14] = p->push_emax;
   out[15] = p->final_lanes;
   return IFGF_OK;
 }
@@ -249,13 +249,6 @@ static hipEvent_t* take_events(ifgf_plan* P) {
   return ev;
 }
 
-template <int Ps, int Pang, int KC>
-static void launch_prop_stored(ifgf_plan* P, int npc, int pcone_off, hipStream_t st) {
-  constexpr int CPB = 256 / (KC * Ps * Pang);
-  const PropArgs a{P->pr_ptr.p, P->pr_cone.p, P->pr_geo.p, P->pn_ifac.p, P->tabs.p, P->coef.p, npc, pcone_off};
-  hipLaunchKernelGGL((k_prop_stored<Ps, Pang, KC>), dim3((npc + CPB - 1) / CPB), dim3(256), 0, st, a);
-}
-
 static NearArgs near_args(ifgf_plan* P) {
   return NearArgs{P->nf_seg.p, P->nf_goff.p, P->nf_val.p, P->rho.p, P->sol.p};
 }
@@ -266,31 +259,35 @@ static void launch_near_stored(ifgf_plan* P, hipStream_t st) {
   hipLaunchKernelGGL(k_near_apply, dim3((N + TPB - 1) / TPB), dim3(256), 0, st, N, near_args(P));
 }
 
-static InterpArgs interp_args(ifgf_plan* P) {
-  return InterpArgs{P->itasks.p, P->itask_cones.p, P->it_lvl.p, P->ent_slot.p, P->it_geo.p,
-                    P->coef.p,   P->part.p,        (int)P->tree.N, P->tree.D};
+template <int Ps, int Pang, int EPT>
+static void launch_level_push_ept(ifgf_plan* P, hipStream_t st) {
+  const int D = P->tree.D;
+  PushArgs a{nullptr, P->push_geo.p, P->push_dst.p, P->coef.p, P->pn_ifac.p, P->tabs.p, P->pbuf.p, 0};
+  for (int li = 0; li + 1 < D; li++) {
+    const int t0 = P->push_level_off[li], nt = P->push_level_off[li + 1] - t0;
+    if (nt == 0) continue;
+    a.tasks = P->push_tasks.p + t0;
+    a.leaf = li == 0;
+    hipLaunchKernelGGL((k_level_push<Ps, Pang, EPT>), dim3(nt), dim3(256), 0, st, a);
+  }
 }
 
-// Dispatch on the autotuned propagation lanes (KC * Ps * Pang <= 256).
-template <int Ps, int Pang, typename Fn1, typename Fn2, typename Fn4>
-static void with_prop_lanes(int lanes, Fn1 f1, Fn2 f2, Fn4 f4) {
-  constexpr int Pn = Ps * Pang;
-  if constexpr (4 * Pn <= 256) {
-    if (lanes == 4) return f4();
+// tasks hold at most 256 * EPT entries (push_emax)
+template <int Ps, int Pang>
+static void launch_level_push(ifgf_plan* P, hipStream_t st) {
+  switch (P->push_emax) {
+    case 256: launch_level_push_ept<Ps, Pang, 1>(P, st); break;
+    case 512: launch_level_push_ept<Ps, Pang, 2>(P, st); break;
+    case 1024: launch_level_push_ept<Ps, Pang, 4>(P, st); break;
+    default: launch_level_push_ept<Ps, Pang, 8>(P, st); break;
   }
-  if constexpr (2 * Pn <= 256) {
-    if (lanes >= 2) return f2();
-  }
-  f1();
 }
 
 // One application y = A x on stream st: gather -> near field -> leaf cones ->
-// propagation chain -> interpolation -> correction/identity/unsort. (Two other
-// schedules were measured slower on the 64k kite and dropped: the near field
-// and interpolation on a second stream beside the propagation chain, and one
-// launch per level co-scheduling propagation, that level's interpolation and
-// a slice of the near field. Neither overlapped: the kernels' times added up
-// under contention.)
+// one push launch per level (propagation + interpolation) -> correction,
+// identity and unsort. (Schedules that overlapped whole stages on a second
+// stream or in shared grids were measured slower on the 64k kite: the
+// kernels' times added up under contention.)
 template <int F, int Ps, int Pang>
 static void launch_ifgf(ifgf_plan* P, const zd* x, zd* y, hipStream_t st) {
   constexpr int Pn = Ps * Pang;
@@ -336,44 +333,28 @@ static void launch_ifgf(ifgf_plan* P, const zd* x, zd* y, hipStream_t st) {
   }
   mark(5);
 
-  // propagation chain
+  // propagation and interpolation: stored path, one push launch per level
+  // (both stages in it, timed as the propagation stage); on the fly, the
+  // pull chain and one interpolation pass over all points
   mark(6);
-  {
+  if (P->stored) {
+    launch_level_push<Ps, Pang>(P, st);
+  } else {
     for (int l = D; l >= 3; l--) {
       const LevelInfo& L = t.lv[l];
       const LevelInfo& Pl = t.lv[l - 1];
-      if (P->stored) {
-        with_prop_lanes<Ps, Pang>(
-            P->prop_lanes, [&] { launch_prop_stored<Ps, Pang, 1>(P, Pl.ncone, (int)Pl.cone_off, st); },
-            [&] {
-              if constexpr (2 * Pn <= 256) launch_prop_stored<Ps, Pang, 2>(P, Pl.ncone, (int)Pl.cone_off, st);
-            },
-            [&] {
-              if constexpr (4 * Pn <= 256) launch_prop_stored<Ps, Pang, 4>(P, Pl.ncone, (int)Pl.cone_off, st);
-            });
-      } else {
-        hipLaunchKernelGGL((k_prop<Ps, Pang>), dim3((Pl.ncone + CPB - 1) / CPB), dim3(256), 0, st, Pl.ncone,
-                           Pl.cone_off, Pl.box_off, Pl.nrad, L.cone_off, L.nslot, P->cg.p, P->cone_box.p,
-                           P->cone_qd.p, P->box_child.p, P->proto.p + L.proto_off, P->dense_map.p + L.map_off,
-                           P->coef.p, P->tabs.p, k);
-      }
+      hipLaunchKernelGGL((k_prop<Ps, Pang>), dim3((Pl.ncone + CPB - 1) / CPB), dim3(256), 0, st, Pl.ncone,
+                         Pl.cone_off, Pl.box_off, Pl.nrad, L.cone_off, L.nslot, P->cg.p, P->cone_box.p,
+                         P->cone_qd.p, P->box_child.p, P->proto.p + L.proto_off, P->dense_map.p + L.map_off,
+                         P->coef.p, P->tabs.p, k);
     }
   }
   mark(7);
-
-  // interpolation: stored path, one launch over the LDS tasks of all levels
-  // (each task writes its level's partial sums); on the fly, one pass over
-  // all points
   mark(8);
-  if (P->stored) {
-    if (P->nitasks > 0)
-      hipLaunchKernelGGL((k_interp_lds<Ps, Pang, kInterpLanes>), dim3(P->nitasks), dim3(256), P->itask_lds, st,
-                         interp_args(P));
-  } else {
+  if (!P->stored)
     hipLaunchKernelGGL((k_interp<Ps, Pang, kInterpLanes>), dim3((N + 256 / kInterpLanes - 1) / (256 / kInterpLanes)),
                        dim3(256), 0, st, N, P->px.p, P->py.p, P->it_ptr.p, P->it_cone.p, P->cg.p, P->coef.p,
                        P->sol.p, k);
-  }
   mark(9);
 
   // correction, identity and unsort
@@ -381,8 +362,8 @@ static void launch_ifgf(ifgf_plan* P, const zd* x, zd* y, hipStream_t st) {
   const double ident = (F == kSingleLayer) ? 0.0 : 0.5;
 #define IFGF_FIN(S)                                                                                            \
   hipLaunchKernelGGL(k_finalize<S>, dim3((N + 256 / S - 1) / (256 / S)), dim3(256), 0, st, N, P->geo.Np, ident, x, \
-                     P->corr_ptr.p, P->corr_patch.p, P->corr_w.p, P->inv.p, P->sol.p, P->part.p,                  \
-                     P->stored ? D - 1 : 0, y)
+                     P->corr_ptr.p, P->corr_patch.p, P->corr_w.p, P->inv.p, P->sol.p, P->it_ptr.p,                 \
+                     P->stored ? P->pbuf.p + P->pbuf_ient : nullptr, y)
   if (P->final_lanes == 4) IFGF_FIN(4);
   else if (P->final_lanes == 16) IFGF_FIN(16);
   else IFGF_FIN(8);
@@ -392,35 +373,46 @@ static void launch_ifgf(ifgf_plan* P, const zd* x, zd* y, hipStream_t st) {
 }
 
 // Evaluates the stored operators once (plan creation).
-// Groups the targets of each box and level into LDS interpolation tasks of at
-// most 256 / kInterpLanes targets whose distinct cousin cones (at most
-// max_cones) are staged in LDS; entry slots index those staged cones.
-static void build_itasks(ifgf_plan* P, int max_cones) {
+// Push tasks per level l = D..2 (level index li = D - l): runs of consecutive
+// cones of the level (at most 256 / P cones, at most emax entries); a cone
+// with more than emax entries gets tasks of its own, emax entries each.
+static void build_push_tasks(ifgf_plan* P, int emax) {
+  if (emax != 256 && emax != 512 && emax != 1024 && emax != 2048) throw std::runtime_error("push task size");
   const Tree& t = P->tree;
-  const int Pn = t.Ps * t.Pang;
-  std::vector<InterpTask> tasks;
-  std::vector<int> tcones;
-  std::vector<unsigned short> slots;
-  std::string err;
-  if (!build_interp_tasks(t, 256 / kInterpLanes, max_cones, tasks, tcones, slots, err))
-    throw std::runtime_error(err);
-  // launch order: by first target, then level (stable)
-  std::vector<size_t> order(tasks.size());
-  for (size_t q = 0; q < order.size(); q++) order[q] = q;
-  std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
-    return tasks[a].t0 != tasks[b].t0 ? tasks[a].t0 < tasks[b].t0 : tasks[a].li < tasks[b].li;
-  });
-  std::vector<ITask> dt(tasks.size());
-  for (size_t q = 0; q < tasks.size(); q++) {
-    const InterpTask& s = tasks[order[q]];
-    dt[q] = ITask{s.li, s.t0, s.nt, s.cone_off, s.ncone, 0};
+  const int D = t.D, Pn = t.Ps * t.Pang, cmax = 256 / Pn;
+  const std::vector<long long>& off = P->push_off;
+  std::vector<PushTask> tasks;
+  P->push_level_off.assign(D, 0);
+  for (int li = 0; li + 1 < D; li++) {
+    const int l = D - li;
+    P->push_level_off[li] = (int)tasks.size();
+    PushTask cur{0, 0, 0, 0};
+    auto flush = [&] {
+      if (cur.nc) tasks.push_back(cur);
+      cur = PushTask{0, 0, 0, 0};
+    };
+    const long long c0 = t.lv[l].cone_off, c1 = c0 + t.lv[l].ncone;
+    for (long long c = c0; c < c1; c++) {
+      const long long n = off[c + 1] - off[c];
+      if (n > emax) {
+        flush();
+        for (long long s = 0; s < n; s += emax)
+          tasks.push_back(PushTask{(int)c, 1, (int)(off[c] + s), (int)std::min<long long>(emax, n - s)});
+        continue;
+      }
+      if (cur.nc == cmax || cur.ne + n > emax) flush();
+      if (cur.nc == 0) {
+        cur.c0 = (int)c;
+        cur.e0 = (int)off[c];
+      }
+      cur.nc++;
+      cur.ne += (int)n;
+    }
+    flush();
   }
-  P->itasks.upload(dt);
-  P->itask_cones.upload(tcones);
-  P->ent_slot.upload(slots);
-  P->nitasks = (int)dt.size();
-  P->itask_max_cones = max_cones;
-  P->itask_lds = (size_t)max_cones * Pn * sizeof(zd);
+  P->push_level_off[D - 1] = (int)tasks.size();
+  P->push_tasks.upload(tasks);
+  P->push_emax = emax;
 }
 
 template <int F, int Ps, int Pang>
@@ -509,16 +501,14 @@ static void build_stored(ifgf_plan* P) {
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(st));
   }
-  // interpolation tasks with LDS-staged cones
-  P->it_lvl.upload(t.it_lvl);
-  P->part.alloc((size_t)(D - 1) * N);
-  build_itasks(P, std::max<int>(27, (int)(32768 / (Pn * sizeof(zd)))));
-  // interpolation geometry per (target, cousin cone)
-  P->it_geo.alloc(t.it_ptr[N]);
+  // interpolation geometry per (target, cousin cone), in it_ptr order
+  DevBuf<Geo4> it_geo;
+  it_geo.alloc(t.it_ptr[N]);
   hipLaunchKernelGGL((k_interp_build<kInterpLanes>), dim3((N + 256 / kInterpLanes - 1) / (256 / kInterpLanes)),
-                     dim3(256), 0, st, N, P->px.p, P->py.p, P->it_ptr.p, P->it_cone.p, P->cg.p, P->it_geo.p, k);
+                     dim3(256), 0, st, N, P->px.p, P->py.p, P->it_ptr.p, P->it_cone.p, P->cg.p, it_geo.p, k);
   HIP_TRY(hipGetLastError());
-  // propagation geometry per (parent node, child cone)
+  // propagation geometry per (parent node, relevant child): entry
+  // pr_ptr[g] + k P + j holds node j of parent cone g in its k-th child
   std::vector<long long> pr_ptr(t.n_cones + 1, 0);
   for (long long g = 0; g < t.n_cones; g++) {
     long long c = 0;
@@ -529,9 +519,13 @@ static void build_stored(ifgf_plan* P) {
     }
     pr_ptr[g + 1] = pr_ptr[g] + c;
   }
-  P->pr_ptr.upload(pr_ptr);
-  P->pr_cone.alloc(pr_ptr[t.n_cones]);
-  P->pr_geo.alloc(pr_ptr[t.n_cones]);
+  const long long npr = pr_ptr[t.n_cones], nit = t.it_ptr[N];
+  DevBuf<long long> d_pr_ptr;
+  DevBuf<int> d_pr_cone;
+  DevBuf<Geo4> pr_geo;
+  d_pr_ptr.upload(pr_ptr);
+  d_pr_cone.alloc(npr);
+  pr_geo.alloc(npr);
   P->pn_ifac.alloc((size_t)t.n_cones * Pn);
   for (int l = D; l >= 3; l--) {
     const LevelInfo& L = t.lv[l];
@@ -539,13 +533,56 @@ static void build_stored(ifgf_plan* P) {
     hipLaunchKernelGGL((k_prop_build<Ps, Pang>), dim3((Pl.ncone * Pn + 255) / 256), dim3(256), 0, st, Pl.ncone,
                        Pl.cone_off, Pl.box_off, L.cone_off, L.nslot, P->cg.p, P->cone_box.p, P->cone_qd.p,
                        P->box_child.p, P->proto.p + L.proto_off, P->dense_map.p + L.map_off, P->tabs.p,
-                       P->pr_ptr.p, P->pr_cone.p, P->pr_geo.p, P->pn_ifac.p, k);
+                       d_pr_ptr.p, d_pr_cone.p, pr_geo.p, P->pn_ifac.p, k);
   }
   HIP_TRY(hipGetLastError());
+  std::vector<int> pr_cone(npr);
+  if (npr) HIP_TRY(hipMemcpyAsync(pr_cone.data(), d_pr_cone.p, npr * sizeof(int), hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
-  P->stored_bytes = (long long)(nf_goff[ngroups] + lf_off[LD.ncone] + (long long)t.n_cones * Pn) * (long long)sizeof(zd) +
-                    (long long)(t.it_ptr[N] + pr_ptr[t.n_cones]) * (long long)sizeof(Geo4) +
-                    pr_ptr[t.n_cones] * (long long)sizeof(int);
+  // push entries grouped by the cone that evaluates them: first the parent
+  // nodes that fall in the cone (destination: the node's k-th child slot),
+  // then the (target, cousin) interpolation entries (destination: the entry)
+  const long long ncs = t.n_cones * (long long)Pn * 4;
+  if (ncs + nit > 2147483647LL) throw std::runtime_error("too many propagation slots and interpolation entries");
+  std::vector<long long> off(t.n_cones + 1, 0);
+  for (long long e = 0; e < npr; e++) off[pr_cone[e] + 1]++;
+  for (long long e = 0; e < nit; e++) off[t.it_cone[e] + 1]++;
+  for (long long g = 0; g < t.n_cones; g++) off[g + 1] += off[g];
+  std::vector<long long> pos(off.begin(), off.end() - 1), src(npr + nit);
+  std::vector<int2> dst(npr + nit);
+  for (long long g = 0; g < t.n_cones; g++) {
+    const int nch = (int)((pr_ptr[g + 1] - pr_ptr[g]) / Pn);
+    for (int kc = 0; kc < nch; kc++)
+      for (int j = 0; j < Pn; j++) {
+        const long long e = pr_ptr[g] + (long long)kc * Pn + j;
+        const long long q = pos[pr_cone[e]]++;
+        src[q] = e;
+        dst[q] = make_int2((int)((g * Pn + j) * 4 + kc), pr_cone[e]);
+      }
+  }
+  for (long long e = 0; e < nit; e++) {
+    const long long q = pos[t.it_cone[e]]++;
+    src[q] = npr + e;
+    dst[q] = make_int2((int)(ncs + e), t.it_cone[e]);
+  }
+  P->push_off = off;
+  P->push_dst.upload(dst);
+  P->push_geo.alloc(npr + nit);
+  {
+    DevBuf<long long> d_src;
+    d_src.upload(src);
+    hipLaunchKernelGGL(k_push_gather, dim3((unsigned)((npr + nit + 255) / 256)), dim3(256), 0, st, npr + nit, d_src.p,
+                       npr, pr_geo.p, it_geo.p, P->push_geo.p);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(st));
+  }
+  P->pbuf.alloc(ncs + nit);
+  HIP_TRY(hipMemset(P->pbuf.p, 0, (ncs + nit) * sizeof(zd)));
+  P->pbuf_ient = ncs;
+  build_push_tasks(P, P->push_emax);
+  P->stored_bytes = (long long)(nf_goff[ngroups] + lf_off[LD.ncone] + (long long)t.n_cones * Pn + ncs + nit) *
+                        (long long)sizeof(zd) +
+                    (npr + nit) * (long long)(sizeof(Geo4) + sizeof(int2));
   P->stored = true;
 }
 
@@ -627,31 +664,20 @@ static void autotune(ifgf_plan* P) {
     int* v;
     std::vector<int> cand;
   };
-  const int Pn = P->prm.Ps * P->prm.Pang;
-  std::vector<int> prop_c{1};
-  if (2 * Pn <= 256) prop_c.push_back(2);
-  if (4 * Pn <= 256) prop_c.push_back(4);
-  Knob knobs[] = {{&P->prop_lanes, prop_c},
-                  {&P->final_lanes, {4, 8, 16}}};
-  // LDS per interpolation task (cones staged per task): smaller tasks stage
-  // some cones more than once but let more blocks share a CU
+  Knob knobs[] = {{&P->final_lanes, {4, 8, 16}}};
+  // entries per level-push task: fewer give more, shorter blocks
   if (P->stored) {
-    std::vector<int> cand;
-    for (int kb : {8192, 16384, 32768}) {
-      const int c = std::max<int>(27, (int)(kb / (Pn * sizeof(zd))));
-      if (std::find(cand.begin(), cand.end(), c) == cand.end()) cand.push_back(c);
-    }
-    int best = P->itask_max_cones;
+    int best = P->push_emax;
     float tbest = 1e30f;
-    for (int c : cand) {
-      build_itasks(P, c);
+    for (int c : {256, 512, 1024, 2048}) {
+      build_push_tasks(P, c);
       const float t = time_apply();
       if (t < tbest) {
         tbest = t;
         best = c;
       }
     }
-    if (best != P->itask_max_cones) build_itasks(P, best);
+    if (best != P->push_emax) build_push_tasks(P, best);
   }
   for (auto& kb : knobs) {
     int best = *kb.v;

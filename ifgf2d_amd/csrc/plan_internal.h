@@ -74,25 +74,23 @@ struct ifgf_plan {
   ifgf::DevBuf<ifgf::ChebTables> tabs;
   // stored operators (near-field values, folded leaf operator)
   bool stored = false;
-  ifgf::DevBuf<long long> nf_goff, lf_off, pr_ptr;
+  ifgf::DevBuf<long long> nf_goff, lf_off;
   long long near_entries = 0;  // stored near-field values without the group padding
   ifgf::DevBuf<zd> nf_val, lf_val, pn_ifac;
-  ifgf::DevBuf<ifgf::Geo4> it_geo, pr_geo;
-  ifgf::DevBuf<int> pr_cone;
-  // LDS interpolation tasks
-  ifgf::DevBuf<ifgf::ITask> itasks;
-  ifgf::DevBuf<int> itask_cones;
-  ifgf::DevBuf<unsigned short> ent_slot;
-  ifgf::DevBuf<long long> it_lvl;
-  ifgf::DevBuf<zd> part;
-  int nitasks = 0;
+  // level-push propagation / interpolation
+  ifgf::DevBuf<ifgf::PushTask> push_tasks;
+  ifgf::DevBuf<ifgf::Geo4> push_geo;
+  ifgf::DevBuf<int2> push_dst;     // (destination slot in pbuf, global cone id)
+  ifgf::DevBuf<zd> pbuf;           // child slots [cone][node][4], then interpolation entries
+  long long pbuf_ient = 0;         // offset of the interpolation entries in pbuf
+  std::vector<long long> push_off; // push entries of global cone g: [off[g], off[g+1])
+  std::vector<int> push_level_off; // tasks of level index li: [off[li], off[li+1])
+  int push_emax = 512;             // entries per push task (autotuned)
   ifgf::DevBuf<int2> lf_src;  // per leaf cone: (first source, source count)
   ifgf::DevBuf<int4> nf_seg;  // per target: its leaf box's kMaxNb (end, source shift) pairs
   int nleaf = 0;
-  // lanes per work item of the stored-path kernels (chosen by the autotuner)
-  int near_lanes = 8 /* kNearChunk */, leaf_lanes = 4 /* kLeafChunk */, prop_lanes = 1, final_lanes = 8;
-  size_t itask_lds = 0;
-  int itask_max_cones = 0;
+  // lanes per work item of the stored-path kernels
+  int near_lanes = 8 /* kNearChunk */, leaf_lanes = 4 /* kLeafChunk */, final_lanes = 8;
   long long stored_bytes = 0;
   // workspaces
   ifgf::DevBuf<zd> rho, sol, coef, xbuf, ybuf;

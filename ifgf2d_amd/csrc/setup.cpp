@@ -746,78 +746,10 @@ bool build_tree(const double* x, const double* y, const double* nx, const double
       }
   }
   t.n_interp = nent;
-  // per-target, per-level entry starts
-  t.it_lvl.assign((size_t)N * D, 0);
-  for (long long i = 0; i < N; i++) {
-    long long e = t.it_ptr[i];
-    for (int l = D; l >= 2; l--) {
-      t.it_lvl[(size_t)i * D + (D - l)] = e;
-      e += (long long)cz[l][pbox[(size_t)i * (D + 1) + l]].size();
-    }
-    t.it_lvl[(size_t)i * D + D - 1] = e;
-  }
   t.point_box.swap(pbox);
   return true;
 }
 
-bool build_interp_tasks(const Tree& t, int max_targets, int max_cones, std::vector<InterpTask>& tasks,
-                        std::vector<int>& task_cones, std::vector<unsigned short>& ent_slot,
-                        std::string& err) {
-  const int D = t.D;
-  const long long N = t.N;
-  tasks.clear();
-  task_cones.clear();
-  ent_slot.assign(t.it_cone.size(), 0);
-  std::vector<int> cur;
-  for (int l = D; l >= 2; l--) {
-    const int li = D - l;
-    long long i = 0;
-    while (i < N) {
-      // targets of the box containing i at level l are contiguous in Morton order
-      const int b = t.point_box[(size_t)i * (D + 1) + l];
-      long long iend = i;
-      while (iend < N && t.point_box[(size_t)iend * (D + 1) + l] == b) iend++;
-      InterpTask task{li, (int)i, 0, (int)task_cones.size(), 0, 0};
-      cur.clear();
-      for (long long p = i; p < iend; p++) {
-        const long long e0 = t.it_lvl[(size_t)p * D + li], e1 = t.it_lvl[(size_t)p * D + li + 1];
-        int fresh = 0;
-        for (long long e = e0; e < e1; e++)
-          if (std::find(cur.begin(), cur.end(), t.it_cone[e]) == cur.end() &&
-              std::find(t.it_cone.begin() + e0, t.it_cone.begin() + e, t.it_cone[e]) == t.it_cone.begin() + e)
-            fresh++;
-        if (task.nt > 0 && (task.nt == max_targets || (int)cur.size() + fresh > max_cones)) {
-          task.ncone = (int)cur.size();
-          task_cones.insert(task_cones.end(), cur.begin(), cur.end());
-          tasks.push_back(task);
-          task = InterpTask{li, (int)p, 0, (int)task_cones.size(), 0, 0};
-          cur.clear();
-        }
-        if (e1 - e0 > max_cones) {
-          err = "a target needs more cones at one level than an interpolation task holds";
-          return false;
-        }
-        for (long long e = e0; e < e1; e++) {
-          auto f = std::find(cur.begin(), cur.end(), t.it_cone[e]);
-          int slot;
-          if (f == cur.end()) {
-            slot = (int)cur.size();
-            cur.push_back(t.it_cone[e]);
-          } else {
-            slot = (int)(f - cur.begin());
-          }
-          ent_slot[e] = (unsigned short)slot;
-        }
-        task.nt++;
-      }
-      task.ncone = (int)cur.size();
-      task_cones.insert(task_cones.end(), cur.begin(), cur.end());
-      tasks.push_back(task);
-      i = iend;
-    }
-  }
-  return true;
-}
 
 bool build_exclusions(const Geometry& g, const Tree& t, std::vector<int>& patch_of_sorted,
                       std::vector<int>& excl, long long& n_near, std::string& err) {

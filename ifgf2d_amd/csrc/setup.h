@@ -89,29 +89,11 @@ struct Tree {
   std::vector<int> proto;                      // see LevelInfo::proto_off, [q][4][P]
   std::vector<long long> it_ptr;               // interpolation entries per sorted target
   std::vector<int> it_cone;                    // global cone id per entry
-  // it_lvl[i*D + li]: first entry of target i at level D - li (li = 0..D-2);
-  // it_lvl[i*D + D-1] = it_ptr[i+1]
-  std::vector<long long> it_lvl;
   std::vector<int> point_box;                  // sorted point x level -> local box (N x (D+1))
   long long n_leaf = 0, n_prop = 0, n_interp = 0, n_cones = 0;
 };
 bool build_tree(const double* x, const double* y, const double* nx, const double* ny, long long N,
                 int nlev, double k, int Ps, int Pang, Tree& t, std::string& err);
-
-// Interpolation work split into LDS-sized tasks: each task is up to
-// max_targets Morton-consecutive targets of one box at one level whose level
-// entries use at most max_cones distinct cones; the cones are staged in LDS and
-// every entry addresses its cone by slot.
-struct InterpTask {
-  int li;        // level index D - level
-  int t0, nt;    // targets [t0, t0 + nt) (sorted order)
-  int cone_off;  // into task_cones
-  int ncone;
-  int pad;
-};
-bool build_interp_tasks(const Tree& t, int max_targets, int max_cones, std::vector<InterpTask>& tasks,
-                        std::vector<int>& task_cones, std::vector<unsigned short>& ent_slot,
-                        std::string& err);
 
 // Near-field exclusion lists (init_sort_sing_point, ForwardMap.tpp:567-607) in
 // sorted order: own patch + patches the point is near-singular to, padded

@@ -21,9 +21,11 @@ struct ChebTables {
   double Ta[16 * 16];       // Ta[n*16 + l] = T_n(xa[l]) * alpha_n / (Ps*Pang)
 };
 
-// One LDS interpolation task (mirrors setup.h's InterpTask).
-struct ITask {
-  int li, t0, nt, cone_off, ncone, pad;
+// One level-push task: cones [c0, c0 + nc) of one level (global ids) and
+// their push entries [e0, e0 + ne).
+struct PushTask {
+  int c0, nc;
+  int e0, ne;
 };
 
 // Local Chebyshev coordinates (s, theta) of a point in a cone and the centre

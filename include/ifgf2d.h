@@ -110,8 +110,10 @@ int ifgf_plan_points(const ifgf_plan* plan, double* x, double* y, double* nx, do
  * [4] relevant cones (all levels), [5] correction entries, [6] nlevels,
  * [7] N, [8] leaf-level cones, [9] 1 if the near-field values and leaf
  * operator are stored, [10] stored near-field entries (excluded pairs
- * included as zeros), [11] bytes of the stored operators, [12..15] lanes per
- * work item of the near-field, leaf, propagation and correction kernels. */
+ * included as zeros), [11] bytes of the stored operators, [12] and [13] lanes
+ * per target / leaf-cone node of the near-field and leaf kernels, [14]
+ * evaluations per block of the per-level propagation/interpolation kernel,
+ * [15] lanes per target of the correction kernel. */
 int ifgf_plan_stats(const ifgf_plan* plan, long long out[16]);
 
 /* BoxTree::getSorting() (BoxTree.h:1163-1165): Morton position -> patch-order index. */
