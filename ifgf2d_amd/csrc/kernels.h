@@ -287,55 +287,55 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// Stored near field for targets [n0, n1), n0 a multiple of kNearGroup:
-// kNearChunk lanes per target, lane sl takes the offsets o = sl (mod
-// kNearChunk). Flat offset o maps to source o + shift[q] for the segment q
-// with end[q-1] <= o < end[q]; the kMaxNb (end, shift) pairs of target i
-// (those of its leaf box) are nf_seg[i] (the last end is the segment length).
-// All per-target loads are issued together before the loop, and the loop body
-// has no load-to-load dependency, so the unrolled iterations' loads are in
-// flight together; the values are read once per application and bypass the
-// caches (non-temporal loads).
-struct NearArgs {
-  const int4* nf_seg;
-  const long long* nf_goff;
-  const zd* nf_val;
-  const zd* rho;
-  zd* sol;
-};
+// Stored near field in equal work items: one wave per item, an item being
+// kNearItemChunks consecutive chunks of one target group (the group's last
+// item may hold fewer). Lane (t, sl) of the wave handles target 8g + t and the
+// flat offsets o = 8u + sl of the item's chunks u; flat offset o maps to
+// source o + shift[q] for the segment q with end[q-1] <= o < end[q], the
+// kMaxNb (end, shift) pairs of target i being nf_seg[i] (the last end is the
+// row length). Each item writes one partial sum per target of its group to
+// npart[item * 8 + t]; k_finalize adds a target's partials in item order.
+// Rows vary from a few dozen to several hundred values, and equal items keep
+// every wave's dependent chain short (item -> values and segment table ->
+// densities), so the stream is not held up by the longest rows.
+constexpr int kNearItemChunks = 4;
 
-__device__ inline void near_block(const NearArgs& a, int lb, int n0, int n1) {
-  constexpr int S = kNearChunk;
-  const int gt = lb * blockDim.x + threadIdx.x;
-  const int i = n0 + gt / S, sl = gt % S;
+__global__ void __launch_bounds__(256)
+    k_near_items(int N, int nitems, const NearItem* __restrict__ items, const int4* __restrict__ nf_seg,
+                 const zd* __restrict__ nf_val, const zd* __restrict__ rho, zd* __restrict__ npart) {
+  const int w = xcd_block() * 4 + threadIdx.x / 64;
+  const int lane = threadIdx.x % 64;
+  const int t = lane / kNearChunk, sl = lane % kNearChunk;
   zd acc{0.0, 0.0};
-  if (i < n1) {
-    const int4* sg = a.nf_seg + (size_t)i * (kNbSegInts / 4);
-    const long long goff = a.nf_goff[i / kNearGroup];
-    const int4 q0 = sg[0], q1 = sg[1], q2 = sg[2], q3 = sg[3], q4 = sg[4];
-    // consume every loaded value once here, so the loads are issued together
-    // instead of being sunk below the loop-entry test
-    asm volatile("" ::"v"(q0.x ^ q0.y ^ q0.z ^ q0.w ^ q1.x ^ q1.y ^ q1.z ^ q1.w ^ q2.x ^ q2.y ^ q2.z ^ q2.w ^
-                          q3.x ^ q3.y ^ q3.z ^ q3.w ^ q4.x ^ q4.y ^ (int)goff));
-    const int end[kMaxNb - 1] = {q0.x, q0.z, q1.x, q1.z, q2.x, q2.z, q3.x, q3.z};
-    const int sh[kMaxNb] = {q0.y, q0.w, q1.y, q1.w, q2.y, q2.w, q3.y, q3.w, q4.y};
-    const int len = q4.x;
-    const dbl2* __restrict__ v = reinterpret_cast<const dbl2*>(a.nf_val) + near_index(goff, i, sl);
-    const zd* __restrict__ rho = a.rho;
-#pragma unroll 4
-    for (int o = sl; o < len; o += S, v += kNearGroup * kNearChunk) {
-      int src = o + sh[0];
+  if (w < nitems) {
+    const NearItem it = items[w];
+    const int i = it.g * kNearGroup + t;
+    if (i < N) {
+      const dbl2* __restrict__ v = reinterpret_cast<const dbl2*>(nf_val) + it.off + t * kNearChunk + sl;
+      dbl2 val[kNearItemChunks];
 #pragma unroll
-      for (int q = 1; q < kMaxNb; q++) src = (o >= end[q - 1]) ? o + sh[q] : src;
-      const dbl2 w = *v;
-      cfma(acc, zd{w.x, w.y}, rho[src]);
+      for (int u = 0; u < kNearItemChunks; u++)
+        if (u < it.nc) val[u] = v[u * kNearGroup * kNearChunk];
+      const int4* sg = nf_seg + (size_t)i * (kNbSegInts / 4);
+      const int4 q0 = sg[0], q1 = sg[1], q2 = sg[2], q3 = sg[3], q4 = sg[4];
+      const int end[kMaxNb - 1] = {q0.x, q0.z, q1.x, q1.z, q2.x, q2.z, q3.x, q3.z};
+      const int sh[kMaxNb] = {q0.y, q0.w, q1.y, q1.w, q2.y, q2.w, q3.y, q3.w, q4.y};
+      const int len = q4.x;
+#pragma unroll
+      for (int u = 0; u < kNearItemChunks; u++) {
+        const int o = (it.u0 + u) * kNearChunk + sl;
+        if (u < it.nc && o < len) {
+          int src = o + sh[0];
+#pragma unroll
+          for (int q = 1; q < kMaxNb; q++) src = (o >= end[q - 1]) ? o + sh[q] : src;
+          cfma(acc, zd{val[u].x, val[u].y}, rho[src]);
+        }
+      }
     }
   }
-  acc = lane_group_sum<S>(acc);
-  if (i < n1 && sl == 0) a.sol[i] = acc;
+  acc = lane_group_sum<kNearChunk>(acc);
+  if (w < nitems && sl == 0) npart[(size_t)w * kNearGroup + t] = acc;
 }
-
-__global__ void __launch_bounds__(256) k_near_apply(int N, NearArgs a) { near_block(a, xcd_block(), 0, N); }
 
 // Leaf operator L_c (P x n_src per leaf cone c). Storage: the sources of a
 // cone are padded to a multiple of kLeafChunk, and the padded columns
@@ -680,21 +680,23 @@ __global__ void __launch_bounds__(256)
   if (i < N && sl == 0) sol[i] = cadd(sol[i], acc);
 }
 
-// y[i] = ident x[i] + sum_e W_e x[patch_e] + far[inv[i]]: S lanes per target
-// split the correction products and the target's interpolation entries
-// (stored path: ient[it_ptr[si] .. it_ptr[si+1]) in pbuf, one per (level,
-// cousin cone)); lane 0 adds the identity and the near field.
+// y[i] = ident x[i] + sum_e W_e x[patch_e] + near[si] + far[si], si = inv[i]:
+// S lanes per target split the correction products, the target's near-field
+// item partials (stored path: npart[item * 8 + si % 8] for the items
+// [nio[si / 8], nio[si / 8 + 1]) of its group; on the fly: sol[si]) and its
+// interpolation entries (stored path: ient[it_ptr[si] .. it_ptr[si+1]), one
+// per (level, cousin cone)); lane 0 adds the identity.
 template <int S>
 __global__ void __launch_bounds__(256)
     k_finalize(int N, int Np, double ident, const zd* __restrict__ x,
                const long long* __restrict__ corr_ptr, const int* __restrict__ corr_patch,
-               const zd* __restrict__ corr_w, const int* __restrict__ inv,
-               const zd* __restrict__ sol, const long long* __restrict__ it_ptr, const zd* __restrict__ ient,
-               zd* __restrict__ y) {
+               const zd* __restrict__ corr_w, const int* __restrict__ inv, const zd* __restrict__ sol,
+               const int* __restrict__ nio, const zd* __restrict__ npart,
+               const long long* __restrict__ it_ptr, const zd* __restrict__ ient, zd* __restrict__ y) {
   const int lb = xcd_block();
   const int gt = lb * blockDim.x + threadIdx.x;
   const int i = gt / S, sl = gt % S;
-  zd acc{0.0, 0.0}, far{0.0, 0.0};
+  zd acc{0.0, 0.0}, near{0.0, 0.0}, far{0.0, 0.0};
   int si = 0;
   if (i < N) {
     si = inv[i];
@@ -705,14 +707,22 @@ __global__ void __launch_bounds__(256)
       const int q = (int)(it % Np);
       cfma(acc, corr_w[e * Np + q], x[(size_t)corr_patch[e] * Np + q]);
     }
+    if (npart) {
+      const int g = si / kNearGroup, t = si % kNearGroup;
+      const int w1 = nio[g + 1];
+      for (int w = nio[g] + sl; w < w1; w += S) near = cadd(near, npart[(size_t)w * kNearGroup + t]);
+    } else if (sl == 0) {
+      near = sol[si];
+    }
     if (ient) {
       const long long f1 = it_ptr[si + 1];
       for (long long f = it_ptr[si] + sl; f < f1; f += S) far = cadd(far, ient[f]);
     }
   }
   acc = lane_group_sum<S>(acc);
+  near = lane_group_sum<S>(near);
   far = lane_group_sum<S>(far);
-  if (i < N && sl == 0) y[i] = cadd(cadd(cscale(x[i], ident), acc), cadd(sol[si], far));
+  if (i < N && sl == 0) y[i] = cadd(cadd(cscale(x[i], ident), acc), cadd(near, far));
 }
 
 // Direct sum at a target subset, excluding the same pairs the correction

@@ -249,14 +249,10 @@ static hipEvent_t* take_events(ifgf_plan* P) {
   return ev;
 }
 
-static NearArgs near_args(ifgf_plan* P) {
-  return NearArgs{P->nf_seg.p, P->nf_goff.p, P->nf_val.p, P->rho.p, P->sol.p};
-}
-
 static void launch_near_stored(ifgf_plan* P, hipStream_t st) {
-  const int N = (int)P->tree.N;
-  constexpr int TPB = 256 / kNearChunk;
-  hipLaunchKernelGGL(k_near_apply, dim3((N + TPB - 1) / TPB), dim3(256), 0, st, N, near_args(P));
+  const int N = (int)P->tree.N, M = (int)P->near_items.n;
+  hipLaunchKernelGGL(k_near_items, dim3((M + 3) / 4), dim3(256), 0, st, N, M, P->near_items.p, P->nf_seg.p,
+                     P->nf_val.p, P->rho.p, P->npart.p);
 }
 
 template <int Ps, int Pang, int EPT>
@@ -362,8 +358,8 @@ static void launch_ifgf(ifgf_plan* P, const zd* x, zd* y, hipStream_t st) {
   const double ident = (F == kSingleLayer) ? 0.0 : 0.5;
 #define IFGF_FIN(S)                                                                                            \
   hipLaunchKernelGGL(k_finalize<S>, dim3((N + 256 / S - 1) / (256 / S)), dim3(256), 0, st, N, P->geo.Np, ident, x, \
-                     P->corr_ptr.p, P->corr_patch.p, P->corr_w.p, P->inv.p, P->sol.p, P->it_ptr.p,                 \
-                     P->stored ? P->pbuf.p + P->pbuf_ient : nullptr, y)
+                     P->corr_ptr.p, P->corr_patch.p, P->corr_w.p, P->inv.p, P->sol.p, P->near_item_off.p,           \
+                     P->stored ? P->npart.p : nullptr, P->it_ptr.p, P->stored ? P->pbuf.p + P->pbuf_ient : nullptr, y)
   if (P->final_lanes == 4) IFGF_FIN(4);
   else if (P->final_lanes == 16) IFGF_FIN(16);
   else IFGF_FIN(8);
@@ -444,6 +440,22 @@ static void build_stored(ifgf_plan* P) {
   }
   P->nf_goff.upload(nf_goff);
   P->near_entries = near_entries;
+  {
+    std::vector<NearItem> items;
+    std::vector<int> item_off(ngroups + 1, 0);
+    for (int g = 0; g < ngroups; g++) {
+      item_off[g] = (int)items.size();
+      const int nchunk = (int)((nf_goff[g + 1] - nf_goff[g]) / (kNearGroup * kNearChunk));
+      if (nchunk > 32767) throw std::runtime_error("near-field rows too long");
+      for (int u0 = 0; u0 < nchunk; u0 += kNearItemChunks)
+        items.push_back(NearItem{nf_goff[g] + (long long)u0 * kNearGroup * kNearChunk, g, (short)u0,
+                                 (short)std::min(kNearItemChunks, nchunk - u0)});
+    }
+    item_off[ngroups] = (int)items.size();
+    P->near_items.upload(items);
+    P->near_item_off.upload(item_off);
+    P->npart.alloc(items.size() * kNearGroup);
+  }
   // per leaf box, its neighbour boxes as (exclusive end in the flattened
   // segment, source index minus flat offset), padded to kMaxNb segments
   P->nleaf = (int)t.leaf_begin.size() - 1;

@@ -88,6 +88,9 @@ struct ifgf_plan {
   int push_emax = 512;             // entries per push task (autotuned)
   ifgf::DevBuf<int2> lf_src;  // per leaf cone: (first source, source count)
   ifgf::DevBuf<int4> nf_seg;  // per target: its leaf box's kMaxNb (end, source shift) pairs
+  ifgf::DevBuf<ifgf::NearItem> near_items;  // near-field work items
+  ifgf::DevBuf<int> near_item_off;          // items of target group g: [off[g], off[g+1])
+  ifgf::DevBuf<zd> npart;                   // near-field partial sums [item][8]
   int nleaf = 0;
   // lanes per work item of the stored-path kernels
   int near_lanes = 8 /* kNearChunk */, leaf_lanes = 4 /* kLeafChunk */, final_lanes = 8;

@@ -21,6 +21,14 @@ struct ChebTables {
   double Ta[16 * 16];       // Ta[n*16 + l] = T_n(xa[l]) * alpha_n / (Ps*Pang)
 };
 
+// One near-field work item: kNearItemChunks (or fewer) chunks of one target
+// group, starting at value offset off.
+struct NearItem {
+  long long off;
+  int g;
+  short u0, nc;
+};
+
 // One level-push task: cones [c0, c0 + nc) of one level (global ids) and
 // their push entries [e0, e0 + ne).
 struct PushTask {
