@@ -417,33 +417,42 @@ __global__ void __launch_bounds__(256)
   lf[leaf_index<P>(col, m)] = acc;
 }
 
-// coef[leaf cone][m] = sum_src L[src][m] rho[s0 + src]: threads (cone, m,
-// slice), kLeafChunk slices; lf_src[c] = (s0, n_src) of the cone's box.
+// Leaf operator applied in equal work items: an item is up to
+// kLeafItemTiles tiles (kLeafChunk sources each) of one leaf cone, threads
+// (item, m, slice) with kLeafChunk slices; it writes the partial
+// coefficients lpart[item * P + m], which k_level_push sums over the cone's
+// items in order. Sources per leaf box range from 1 to over a hundred; equal
+// items keep every wave's chain to one item -> values/densities round trip.
+constexpr int kLeafItemTiles = 4;
+
 template <int P>
 __global__ void __launch_bounds__(256)
-    k_leaf_apply(int ncone, int cone_off, const int2* __restrict__ lf_src, const long long* __restrict__ lf_off,
-                 const zd* __restrict__ lf, const zd* __restrict__ rho, zd* __restrict__ coef) {
+    k_leaf_items(int nitems, const LeafItem* __restrict__ items, const zd* __restrict__ lf,
+                 const zd* __restrict__ rho, zd* __restrict__ lpart) {
   constexpr int S = kLeafChunk;
-  const int lb = xcd_block();
-  const long long gt = (long long)lb * blockDim.x + threadIdx.x;
-  const long long item = gt / S;
-  const int sl = (int)(gt % S);
-  const int c = (int)(item / P), m = (int)(item % P);
+  const long long gt = (long long)xcd_block() * blockDim.x + threadIdx.x;
+  const long long item = gt / (P * S);
+  const int r = (int)(gt % (P * S));
+  const int m = r / S, sl = r % S;
   zd acc{0.0, 0.0};
-  if (c < ncone) {
-    const int2 sr = lf_src[c];
-    const long long off = lf_off[c];
-    asm volatile("" ::"v"(sr.x ^ sr.y ^ (int)off));  // issue both loads before the loop test
-    const dbl2* __restrict__ L = reinterpret_cast<const dbl2*>(lf) + off + m * S + sl;
-    const zd* __restrict__ r = rho + sr.x;
-#pragma unroll 4
-    for (int src = sl; src < sr.y; src += S, L += S * P) {
-      const dbl2 w = *L;
-      cfma(acc, zd{w.x, w.y}, r[src]);
+  if (item < nitems) {
+    const LeafItem it = items[item];
+    const dbl2* __restrict__ L = reinterpret_cast<const dbl2*>(lf) + it.off + m * S + sl;
+    dbl2 val[kLeafItemTiles];
+    zd rv[kLeafItemTiles];
+#pragma unroll
+    for (int u = 0; u < kLeafItemTiles; u++) {
+      if (u * S + sl < it.n) {
+        val[u] = L[u * S * P];
+        rv[u] = rho[it.s0 + u * S + sl];
+      }
     }
+#pragma unroll
+    for (int u = 0; u < kLeafItemTiles; u++)
+      if (u * S + sl < it.n) cfma(acc, zd{val[u].x, val[u].y}, rv[u]);
   }
   acc = lane_group_sum<S>(acc);
-  if (c < ncone && sl == 0) coef[(size_t)(cone_off + c) * P + m] = acc;
+  if (item < nitems && sl == 0) lpart[item * P + m] = acc;
 }
 
 // Stored interpolation and propagation geometry: (s, theta, centre factor)
@@ -510,7 +519,8 @@ __global__ void __launch_bounds__(256)
 // Each block owns a task: up to 256 / P cones of level l (or a slice of one
 // cone's entries) and a contiguous range of that level's push entries. The
 // block first forms the cones' coefficients in LDS:
-//   l = D: the leaf operator's output coef[g][m] (already transformed);
+//   l = D: the sum of the leaf operator items' partial coefficients (already
+//          transformed);
 //   l < D: node value v_j = (p_0 + p_1 + p_2 + p_3) / factor_j from the four
 //          child slots p_k = pbuf[(g P + j) 4 + k] that the level below wrote
 //          (empty slots stay 0), then the separable 2D transform.
@@ -527,7 +537,9 @@ struct PushArgs {
   const PushTask* tasks;
   const Geo4* geo;       // per push entry: (s, theta, centre factor)
   const int2* dst;       // per push entry: (destination slot in pbuf, global cone id)
-  const zd* coef;        // leaf coefficients (level D)
+  const int* lio;        // level D: items of leaf cone c: [lio[c], lio[c+1])
+  const zd* lpart;       // level D: leaf item partial coefficients [item][P]
+  long long leaf_cone_off;
   const zd* pn_ifac;     // 1 / centre factor per (cone, node), levels < D
   const ChebTables* tabs;
   zd* pbuf;              // child slots of every (cone, node), then interpolation entries
@@ -565,7 +577,9 @@ __global__ void __launch_bounds__(256) k_level_push(PushArgs a) {
   zd v{0.0, 0.0};
   if (act) {
     if (a.leaf) {
-      v = a.coef[g * P + j];
+      const int c = (int)(g - a.leaf_cone_off);
+      const int w1 = a.lio[c + 1];
+      for (int w = a.lio[c]; w < w1; w++) v = cadd(v, a.lpart[(size_t)w * P + j]);
     } else {
       const zd* s = a.pbuf + (g * P + j) * 4;
       const zd s0 = s[0], s1 = s[1], s2 = s[2], s3 = s[3];

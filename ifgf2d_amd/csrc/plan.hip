@@ -258,7 +258,8 @@ static void launch_near_stored(ifgf_plan* P, hipStream_t st) {
 template <int Ps, int Pang, int EPT>
 static void launch_level_push_ept(ifgf_plan* P, hipStream_t st) {
   const int D = P->tree.D;
-  PushArgs a{nullptr, P->push_geo.p, P->push_dst.p, P->coef.p, P->pn_ifac.p, P->tabs.p, P->pbuf.p, 0};
+  PushArgs a{nullptr,          P->push_geo.p, P->push_dst.p, P->leaf_item_off.p, P->lpart.p,
+             P->tree.lv[D].cone_off, P->pn_ifac.p, P->tabs.p,  P->pbuf.p,          0};
   for (int li = 0; li + 1 < D; li++) {
     const int t0 = P->push_level_off[li], nt = P->push_level_off[li + 1] - t0;
     if (nt == 0) continue;
@@ -318,9 +319,9 @@ static void launch_ifgf(ifgf_plan* P, const zd* x, zd* y, hipStream_t st) {
   mark(4);
   const LevelInfo& LD = t.lv[D];
   if (P->stored) {
-    hipLaunchKernelGGL((k_leaf_apply<Pn>), dim3((unsigned)(((long long)LD.ncone * Pn * kLeafChunk + 255) / 256)),
-                       dim3(256), 0, st, LD.ncone, LD.cone_off, P->lf_src.p, P->lf_off.p, P->lf_val.p, P->rho.p,
-                       P->coef.p);
+    const long long M = P->leaf_items.n;
+    hipLaunchKernelGGL((k_leaf_items<Pn>), dim3((unsigned)((M * Pn * kLeafChunk + 255) / 256)), dim3(256), 0, st,
+                       (int)M, P->leaf_items.p, P->lf_val.p, P->rho.p, P->lpart.p);
   } else {
     constexpr int LCPB = 256 / (Pn * kLeafLanes);
     hipLaunchKernelGGL((k_leaf<F, Ps, Pang, kLeafLanes>), dim3((LD.ncone + LCPB - 1) / LCPB), dim3(256), 0, st,
@@ -499,7 +500,20 @@ static void build_stored(ifgf_plan* P) {
     lf_off[c + 1] = lf_off[c] + (long long)Pn * ((ns + kLeafChunk - 1) / kLeafChunk * kLeafChunk);
   }
   P->lf_off.upload(lf_off);
-  P->lf_src.upload(lf_src);
+  {
+    std::vector<LeafItem> items;
+    std::vector<int> item_off(LD.ncone + 1, 0);
+    for (int c = 0; c < LD.ncone; c++) {
+      item_off[c] = (int)items.size();
+      for (int s = 0; s < lf_src[c].y; s += kLeafItemTiles * kLeafChunk)
+        items.push_back(LeafItem{lf_off[c] + (long long)s * Pn, lf_src[c].x + s,
+                                 std::min(kLeafItemTiles * kLeafChunk, lf_src[c].y - s)});
+    }
+    item_off[LD.ncone] = (int)items.size();
+    P->leaf_items.upload(items);
+    P->leaf_item_off.upload(item_off);
+    P->lpart.alloc(items.size() * Pn);
+  }
   P->lf_val.alloc(lf_off[LD.ncone]);
   {
     DevBuf<zd> tmp;

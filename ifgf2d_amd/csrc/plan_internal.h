@@ -86,7 +86,9 @@ struct ifgf_plan {
   std::vector<long long> push_off; // push entries of global cone g: [off[g], off[g+1])
   std::vector<int> push_level_off; // tasks of level index li: [off[li], off[li+1])
   int push_emax = 512;             // entries per push task (autotuned)
-  ifgf::DevBuf<int2> lf_src;  // per leaf cone: (first source, source count)
+  ifgf::DevBuf<ifgf::LeafItem> leaf_items;  // leaf-operator work items
+  ifgf::DevBuf<int> leaf_item_off;          // items of leaf cone c: [off[c], off[c+1])
+  ifgf::DevBuf<zd> lpart;                   // leaf partial coefficients [item][P]
   ifgf::DevBuf<int4> nf_seg;  // per target: its leaf box's kMaxNb (end, source shift) pairs
   ifgf::DevBuf<ifgf::NearItem> near_items;  // near-field work items
   ifgf::DevBuf<int> near_item_off;          // items of target group g: [off[g], off[g+1])

@@ -29,6 +29,14 @@ struct NearItem {
   short u0, nc;
 };
 
+// One leaf-operator work item: up to kLeafItemTiles tiles of one leaf cone;
+// off = value offset of its first tile, s0 = density index of its first
+// source, n = sources in the item.
+struct LeafItem {
+  long long off;
+  int s0, n;
+};
+
 // One level-push task: cones [c0, c0 + nc) of one level (global ids) and
 // their push entries [e0, e0 + ne).
 struct PushTask {
