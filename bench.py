@@ -173,8 +173,8 @@ def main():
         achieved, peak, runit = amount / secs / 1e9, PEAK_HBM_GBS, "GB/s"
     else:
         achieved, peak, runit = amount / secs / 1e12, PEAK_FP64_TFLOPS, "TFLOP/s"
-    kname = {"near": "k_near_apply" if stored else "k_near",
-             "leaf": "k_leaf_apply" if stored else "k_leaf",
+    kname = {"near": "k_near_items" if stored else "k_near",
+             "leaf": "k_leaf_items" if stored else "k_leaf",
              "propagation": "k_level_push" if stored else "k_prop",
              "interpolation": "k_interp"}[dom]
     roofline = {"bound": bound, "kernel": kname, "achieved": round(achieved, 3), "peak": peak,
