@@ -38,6 +38,7 @@ _PREC = {"fp64": FP64, "mixed": MIXED, FP64: FP64, MIXED: MIXED}
 
 IFGF_OK, IFGF_ERR_NOT_CONVERGED = 0, 6
 IFGF_PLAN_HOST_ONLY, IFGF_PLAN_ON_THE_FLY, IFGF_PLAN_AUTOTUNE, IFGF_PTR_DEVICE = 1, 2, 4, 1
+IFGF_APPLY_FP64 = 2
 
 
 class IFGFError(RuntimeError):
@@ -66,7 +67,7 @@ EXPORTS = [
     "ifgf_plan_create", "ifgf_plan_destroy", "ifgf_last_error", "ifgf_plan_error", "ifgf_plan_size",
     "ifgf_plan_points", "ifgf_plan_stats", "ifgf_plan_sorting", "ifgf_plan_level_cones",
     "ifgf_plan_correction", "ifgf_apply", "ifgf_apply_direct", "ifgf_propagate", "ifgf_gmres",
-    "ifgf_gmres_host", "ifgf_hankel01", "ifgf_plan_profile", "ifgf_plan_stage_times",
+    "ifgf_fgmres", "ifgf_gmres_host", "ifgf_hankel01", "ifgf_plan_profile", "ifgf_plan_stage_times",
 ]
 
 _LIB = None
@@ -101,6 +102,9 @@ def lib():
         L.ifgf_propagate.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_longlong, _dp, _dp]
         L.ifgf_gmres.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int, ctypes.c_double,
                                  ctypes.c_int, _ip, _dp, _dp, ctypes.c_int]
+        L.ifgf_fgmres.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int, ctypes.c_double,
+                                  ctypes.c_int, ctypes.c_int, ctypes.c_double, _ip, _ip, _dp, _dp,
+                                  ctypes.c_int]
         L.ifgf_gmres_host.argtypes = [APPLY_FN, ctypes.c_void_p, ctypes.c_longlong, _dp, _dp,
                                       ctypes.c_int, ctypes.c_double, ctypes.c_int, _ip, _dp]
         L.ifgf_hankel01.argtypes = [_dp, ctypes.c_longlong, _dp, _dp]
@@ -260,22 +264,24 @@ class ForwardMap:
         return ptr, patch, w.view(np.complex128).reshape(n, self.Np)
 
     # ----- the forward map
-    def compute_Ax_acc(self, x):
-        """y = A x on the GPU for host vectors; x may be (N,) or (N, nrhs) (column major)."""
+    def compute_Ax_acc(self, x, fp64=False):
+        """y = A x on the GPU for host vectors; x may be (N,) or (N, nrhs) (column major).
+        fp64: on a mixed-precision plan, apply the all-FP64 operator."""
         x = np.asarray(x, dtype=np.complex128)
         nrhs = 1 if x.ndim == 1 else x.shape[1]
         xf = np.asfortranarray(x.reshape(self.N, nrhs))
         y = np.empty_like(xf, order="F")
-        self._check(lib().ifgf_apply(self._h, xf.ctypes.data, y.ctypes.data, nrhs, self.N, 0, None),
-                    "ifgf_apply")
+        self._check(lib().ifgf_apply(self._h, xf.ctypes.data, y.ctypes.data, nrhs, self.N,
+                                     IFGF_APPLY_FP64 if fp64 else 0, None), "ifgf_apply")
         return y.reshape(x.shape)
 
     __call__ = compute_Ax_acc
 
-    def apply_device(self, x_ptr, y_ptr, nrhs=1, ld=None, stream=None):
+    def apply_device(self, x_ptr, y_ptr, nrhs=1, ld=None, stream=None, fp64=False):
         """Asynchronous apply on device pointers (e.g. torch tensors' data_ptr())."""
+        flags = IFGF_PTR_DEVICE | (IFGF_APPLY_FP64 if fp64 else 0)
         self._check(lib().ifgf_apply(self._h, ctypes.c_void_p(x_ptr), ctypes.c_void_p(y_ptr), nrhs,
-                                     self.N if ld is None else ld, IFGF_PTR_DEVICE,
+                                     self.N if ld is None else ld, flags,
                                      ctypes.c_void_p(stream) if stream else None), "ifgf_apply")
 
     def compute_Ax_unacc(self, x, targets):
@@ -310,6 +316,23 @@ class ForwardMap:
         if rc not in (IFGF_OK, IFGF_ERR_NOT_CONVERGED):
             self._check(rc, "ifgf_gmres")
         return x, it.value, rr.value, hist[: it.value]
+
+    def solve_fgmres(self, rhs, restart=50, tol=1e-8, maxit=200, inner_restart=50, inner_floor=1e-5,
+                     x0=None):
+        """Flexible GMRES with FP64 refinement: FP64 outer Arnoldi with the all-FP64
+        operator, inner GMRES cycles with the plan's default operator (mixed precision
+        on a mixed plan). Returns (x, outer iterations, inner iterations, true relative
+        residual, outer residual-estimate history)."""
+        b = _c(rhs)
+        x = np.zeros_like(b) if x0 is None else _c(x0).copy()
+        it, iit, rr = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        hist = np.zeros(maxit)
+        rc = lib().ifgf_fgmres(self._h, _d(b.view(np.float64)), _d(x.view(np.float64)), restart, tol,
+                               maxit, inner_restart, inner_floor, ctypes.byref(it), ctypes.byref(iit),
+                               ctypes.byref(rr), _d(hist), hist.size)
+        if rc not in (IFGF_OK, IFGF_ERR_NOT_CONVERGED):
+            self._check(rc, "ifgf_fgmres")
+        return x, it.value, iit.value, rr.value, hist[: it.value]
 
     def profile(self, enable=True):
         self._check(lib().ifgf_plan_profile(self._h, 1 if enable else 0), "profile")

@@ -10,6 +10,7 @@
 // |w(k)| / ||b - A x0|| (opGMRES.hpp:207) are updated.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <complex>
 #include <vector>
@@ -131,6 +132,136 @@ void apply_givens(const Givens& g, std::complex<double>& a, std::complex<double>
   a = t;
 }
 
+// Device workspace of one GMRES cycle of restart m on vectors of length n.
+struct Krylov {
+  DevBuf<zd> V, w, tmp, part, h;
+  Krylov(long long n, int m) {
+    V.alloc((size_t)n * (m + 1));
+    w.alloc(n);
+    tmp.alloc(n);
+    part.alloc((size_t)kChunks * (m + 2));
+    h.alloc(2 * (m + 2));  // [0, m+2): projections; [m+2, 2m+4): second CGS pass
+  }
+};
+
+struct Solver {
+  ifgf_plan* P;
+  hipStream_t st;
+  long long n;
+  int nb;
+  Solver(ifgf_plan* p) : P(p), st(p->stream), n(p->geo.N), nb((int)((p->geo.N + 255) / 256)) {}
+
+  void apply(const zd* x, zd* y, bool fp64) {
+    const int rc = apply_device(P, x, y, st, fp64);
+    if (rc) throw std::runtime_error(P->err);
+  }
+  // ||v||^2 into dst[0].x (device)
+  void norm2(Krylov& K, const zd* v, zd* dst) {
+    hipLaunchKernelGGL(k_mdot, dim3(kChunks, 1), dim3(256), 0, st, n, v, n, v, K.part.p);
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, st, K.part.p, dst, 0);
+  }
+  double norm(Krylov& K, const zd* v) {
+    norm2(K, v, K.h.p);
+    std::vector<std::complex<double>> hv;
+    fetch(K.h.p, 1, hv);
+    return std::sqrt(hv[0].real());
+  }
+  void fetch(const zd* src, int cnt, std::vector<std::complex<double>>& out) {
+    out.resize(cnt);
+    HIP_TRY(hipMemcpyAsync(out.data(), src, cnt * sizeof(zd), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+  }
+  // r = b - A x
+  void residual(const zd* b, const zd* x, zd* r, bool fp64) {
+    apply(x, r, fp64);
+    hipLaunchKernelGGL(k_residual, dim3(nb), dim3(256), 0, st, n, b, r);
+  }
+  // Appends column j+1 of the Arnoldi basis: w is orthogonalised against
+  // V[:, 0..j] by CGS2 and normalised into V[:, j+1]; returns the Hessenberg
+  // column h[0..j+1] on the host.
+  void arnoldi_step(Krylov& K, int m, int j, std::vector<std::complex<double>>& hv) {
+    hipLaunchKernelGGL(k_mdot, dim3(kChunks, j + 1), dim3(256), 0, st, n, K.V.p, n, K.w.p, K.part.p);
+    hipLaunchKernelGGL(k_reduce, dim3(j + 1), dim3(256), 0, st, K.part.p, K.h.p, 0);
+    hipLaunchKernelGGL(k_update, dim3(nb), dim3(256), 0, st, n, j + 1, K.V.p, n, K.h.p, 1.0, K.w.p);
+    hipLaunchKernelGGL(k_mdot, dim3(kChunks, j + 1), dim3(256), 0, st, n, K.V.p, n, K.w.p, K.part.p);
+    hipLaunchKernelGGL(k_reduce, dim3(j + 1), dim3(256), 0, st, K.part.p, K.h.p + m + 2, 0);
+    hipLaunchKernelGGL(k_update, dim3(nb), dim3(256), 0, st, n, j + 1, K.V.p, n, K.h.p + m + 2, 1.0, K.w.p);
+    hipLaunchKernelGGL(k_reduce, dim3(j + 1), dim3(256), 0, st, K.part.p, K.h.p, 1);  // h += h2
+    norm2(K, K.w.p, K.h.p + j + 1);
+    hipLaunchKernelGGL(k_normalize, dim3(nb), dim3(256), 0, st, n, K.w.p, K.h.p + j + 1, K.V.p + (size_t)(j + 1) * n);
+    fetch(K.h.p, j + 2, hv);
+    hv[j + 1] = std::sqrt(hv[j + 1].real());
+  }
+  // x += B y for the first kk columns of B (ld n)
+  void update(Krylov& K, const zd* B, int kk, const std::vector<std::complex<double>>& y, zd* x) {
+    HIP_TRY(hipMemcpyAsync(K.h.p, y.data(), kk * sizeof(zd), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_update, dim3(nb), dim3(256), 0, st, n, kk, B, n, K.h.p, -1.0, x);
+  }
+};
+
+// Upper-triangular solve R y = g (R = the rotated Hessenberg, row stride m).
+std::vector<std::complex<double>> back_substitute(const std::vector<std::complex<double>>& H, int m,
+                                                  const std::vector<std::complex<double>>& g, int kk) {
+  std::vector<std::complex<double>> y(kk);
+  for (int i = kk - 1; i >= 0; i--) {
+    std::complex<double> s = g[i];
+    for (int q = i + 1; q < kk; q++) s -= H[(size_t)i * m + q] * y[q];
+    y[i] = s / H[(size_t)i * m + i];
+  }
+  return y;
+}
+
+// Restarted GMRES(m) on device vectors: x (initial guess in, solution out)
+// for A x = b with the plan's operator (fp64: the all-FP64 one). Stops when
+// |g(k)| / ||b - A x0|| < tol (opGMRES.hpp:207) or after maxit iterations.
+// Returns the iteration count; *est the final estimate.
+int gmres_core(Solver& S, Krylov& K, bool fp64, const zd* b, zd* x, int m, double tol, int maxit, double* est_out,
+               double* hist, int hist_cap, bool* conv) {
+  using C = std::complex<double>;
+  const long long n = S.n;
+  S.residual(b, x, K.tmp.p, fp64);
+  const double r0n = S.norm(K, K.tmp.p);
+  double beta = r0n;
+  bool converged = (r0n == 0.0);
+  double est = converged ? 0.0 : 1.0;
+  int it = 0;
+  std::vector<C> H((size_t)(m + 1) * m), g(m + 1), hv;
+  std::vector<Givens> rot(m);
+  while (!converged && it < maxit) {
+    // V[:,0] = r / beta (norm recomputed on the device from K.h[0])
+    S.norm2(K, K.tmp.p, K.h.p);
+    hipLaunchKernelGGL(k_normalize, dim3(S.nb), dim3(256), 0, S.st, n, K.tmp.p, K.h.p, K.V.p);
+    std::fill(g.begin(), g.end(), C(0.0));
+    g[0] = beta;
+    int j = 0;
+    for (; j < m && it < maxit; j++) {
+      it++;
+      S.apply(K.V.p + (size_t)j * n, K.w.p, fp64);
+      S.arnoldi_step(K, m, j, hv);
+      for (int i = 0; i < j; i++) apply_givens(rot[i], hv[i], hv[i + 1]);
+      rot[j] = make_givens(hv[j], hv[j + 1]);
+      apply_givens(rot[j], hv[j], hv[j + 1]);
+      apply_givens(rot[j], g[j], g[j + 1]);
+      for (int i = 0; i <= j; i++) H[(size_t)i * m + j] = hv[i];
+      est = std::abs(g[j + 1]) / r0n;
+      if (hist && it - 1 < hist_cap) hist[it - 1] = est;
+      if (est < tol) {
+        converged = true;
+        j++;
+        break;
+      }
+    }
+    S.update(K, K.V.p, j, back_substitute(H, m, g, j), x);
+    if (!converged && it < maxit) {
+      S.residual(b, x, K.tmp.p, fp64);
+      beta = S.norm(K, K.tmp.p);
+    }
+  }
+  *est_out = est;
+  *conv = converged;
+  return it;
+}
+
 }  // namespace
 
 extern "C" int ifgf_gmres(ifgf_plan* P, const double* rhs, double* x, int restart, double tol, int maxit,
@@ -138,87 +269,98 @@ extern "C" int ifgf_gmres(ifgf_plan* P, const double* rhs, double* x, int restar
   if (!P || !rhs || !x || !iters || !relres || restart < 1 || maxit < 1)
     return plan_fail(P, IFGF_ERR_ARG, "invalid argument");
   if (!P->on_device) return plan_fail(P, IFGF_ERR_NO_DEVICE, "plan was built host-only");
-  using C = std::complex<double>;
   const long long n = P->geo.N;
-  const int m = restart;
   *iters = 0;
   try {
-    hipStream_t st = P->stream;
-    DevBuf<zd> V, b, xd, w, part, h, tmp;
-    V.alloc((size_t)n * (m + 1));
+    Solver S(P);
+    Krylov K(n, restart);
+    DevBuf<zd> b, xd;
     b.alloc(n);
     xd.alloc(n);
-    w.alloc(n);
-    tmp.alloc(n);
-    part.alloc((size_t)kChunks * (m + 2));
-    h.alloc(2 * (m + 2));  // [0, m+2): projections; [m+2, 2m+4): second CGS pass
-    HIP_TRY(hipMemcpyAsync(b.p, rhs, n * sizeof(zd), hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(xd.p, x, n * sizeof(zd), hipMemcpyHostToDevice, st));
-    const int nb = (int)((n + 255) / 256);
-    auto norm2 = [&](const zd* v, zd* dst) {
-      hipLaunchKernelGGL(k_mdot, dim3(kChunks, 1), dim3(256), 0, st, n, v, n, v, part.p);
-      hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, st, part.p, dst, 0);
-    };
-    auto fetch = [&](const zd* src, int cnt, std::vector<C>& out) {
-      out.resize(cnt);
-      HIP_TRY(hipMemcpyAsync(out.data(), src, cnt * sizeof(zd), hipMemcpyDeviceToHost, st));
-      HIP_TRY(hipStreamSynchronize(st));
-    };
-    // ||b||
-    std::vector<C> hv;
-    norm2(b.p, h.p);
-    fetch(h.p, 1, hv);
-    const double bnorm = std::sqrt(hv[0].real());
+    HIP_TRY(hipMemcpyAsync(b.p, rhs, n * sizeof(zd), hipMemcpyHostToDevice, S.st));
+    HIP_TRY(hipMemcpyAsync(xd.p, x, n * sizeof(zd), hipMemcpyHostToDevice, S.st));
+    const double bnorm = S.norm(K, b.p);
     if (bnorm == 0.0) {
-      std::vector<C> z(n, 0.0);
-      std::copy(z.begin(), z.end(), (C*)x);
+      std::fill((std::complex<double>*)x, (std::complex<double>*)x + n, std::complex<double>(0.0));
       *relres = 0.0;
       return IFGF_OK;
     }
-    // r0 = b - A x0  (into V[:,0] before normalisation)
-    auto residual_into = [&](zd* r) {
-      int rc = apply_device(P, xd.p, r, st);
-      if (rc) throw std::runtime_error(P->err);
-      hipLaunchKernelGGL(k_residual, dim3(nb), dim3(256), 0, st, n, b.p, r);
-    };
-    residual_into(tmp.p);
-    norm2(tmp.p, h.p);
-    fetch(h.p, 1, hv);
-    const double r0n = std::sqrt(hv[0].real());
+    double est = 0.0;
+    bool converged = false;
+    *iters = gmres_core(S, K, false, b.p, xd.p, restart, tol, maxit, &est, hist, hist_cap, &converged);
+    // true residual with the all-FP64 operator
+    S.residual(b.p, xd.p, K.tmp.p, true);
+    *relres = S.norm(K, K.tmp.p) / bnorm;
+    HIP_TRY(hipMemcpyAsync(x, xd.p, n * sizeof(zd), hipMemcpyDeviceToHost, S.st));
+    HIP_TRY(hipStreamSynchronize(S.st));
+    return converged ? IFGF_OK : IFGF_ERR_NOT_CONVERGED;
+  } catch (std::exception& e) {
+    return plan_fail(P, IFGF_ERR_HIP, e.what());
+  }
+}
+
+extern "C" int ifgf_fgmres(ifgf_plan* P, const double* rhs, double* x, int restart, double tol, int maxit,
+                           int inner_restart, double inner_floor, int* outer_iters, int* inner_iters,
+                           double* relres, double* hist, int hist_cap) {
+  if (!P || !rhs || !x || !outer_iters || !inner_iters || !relres || restart < 1 || maxit < 1 || inner_restart < 1 ||
+      !(inner_floor > 0.0))
+    return plan_fail(P, IFGF_ERR_ARG, "invalid argument");
+  if (!P->on_device) return plan_fail(P, IFGF_ERR_NO_DEVICE, "plan was built host-only");
+  using C = std::complex<double>;
+  const long long n = P->geo.N;
+  const int m = restart;
+  *outer_iters = 0;
+  *inner_iters = 0;
+  try {
+    Solver S(P);
+    Krylov K(n, m), Ki(n, inner_restart);
+    DevBuf<zd> b, xd, Z;
+    b.alloc(n);
+    xd.alloc(n);
+    Z.alloc((size_t)n * m);
+    HIP_TRY(hipMemcpyAsync(b.p, rhs, n * sizeof(zd), hipMemcpyHostToDevice, S.st));
+    HIP_TRY(hipMemcpyAsync(xd.p, x, n * sizeof(zd), hipMemcpyHostToDevice, S.st));
+    const double bnorm = S.norm(K, b.p);
+    if (bnorm == 0.0) {
+      std::fill((C*)x, (C*)x + n, C(0.0));
+      *relres = 0.0;
+      return IFGF_OK;
+    }
+    S.residual(b.p, xd.p, K.tmp.p, true);
+    const double r0n = S.norm(K, K.tmp.p);
     double beta = r0n;
     bool converged = (r0n == 0.0);
-    double est = r0n / bnorm;
-    int it = 0;
-    std::vector<C> H((size_t)(m + 1) * m), g(m + 1);
+    int it = 0, inner_total = 0;
+    std::vector<C> H((size_t)(m + 1) * m), g(m + 1), hv;
     std::vector<Givens> rot(m);
     while (!converged && it < maxit) {
-      // V[:,0] = r / beta
-      hipLaunchKernelGGL(k_normalize, dim3(nb), dim3(256), 0, st, n, tmp.p, h.p, V.p);
+      S.norm2(K, K.tmp.p, K.h.p);
+      hipLaunchKernelGGL(k_normalize, dim3(S.nb), dim3(256), 0, S.st, n, K.tmp.p, K.h.p, K.V.p);
       std::fill(g.begin(), g.end(), C(0.0));
       g[0] = beta;
       int j = 0;
       for (; j < m && it < maxit; j++) {
         it++;
-        int rc = apply_device(P, V.p + (size_t)j * n, w.p, st);
-        if (rc) return rc;
-        // CGS2: h = V^H w; w -= V h; h2 = V^H w; w -= V h2; h += h2
-        hipLaunchKernelGGL(k_mdot, dim3(kChunks, j + 1), dim3(256), 0, st, n, V.p, n, w.p, part.p);
-        hipLaunchKernelGGL(k_reduce, dim3(j + 1), dim3(256), 0, st, part.p, h.p, 0);
-        hipLaunchKernelGGL(k_update, dim3(nb), dim3(256), 0, st, n, j + 1, V.p, n, h.p, 1.0, w.p);
-        hipLaunchKernelGGL(k_mdot, dim3(kChunks, j + 1), dim3(256), 0, st, n, V.p, n, w.p, part.p);
-        hipLaunchKernelGGL(k_reduce, dim3(j + 1), dim3(256), 0, st, part.p, h.p + m + 2, 0);
-        hipLaunchKernelGGL(k_update, dim3(nb), dim3(256), 0, st, n, j + 1, V.p, n, h.p + m + 2, 1.0, w.p);
-        hipLaunchKernelGGL(k_reduce, dim3(j + 1), dim3(256), 0, st, part.p, h.p, 1);  // h += h2
-        norm2(w.p, h.p + j + 1);
-        hipLaunchKernelGGL(k_normalize, dim3(nb), dim3(256), 0, st, n, w.p, h.p + j + 1, V.p + (size_t)(j + 1) * n);
-        fetch(h.p, j + 2, hv);
-        hv[j + 1] = std::sqrt(hv[j + 1].real());
+        // z_j ~ A_mixed^-1 v_j: the outer residual must still fall by
+        // tol / (current relative residual), so the inner solve needs no more
+        // relative accuracy than that (never below inner_floor)
+        const double cur = std::abs(g[j]) / r0n;
+        const double tau = std::min(0.5, std::max(inner_floor, tol / std::max(cur, 1e-300)));
+        zd* zj = Z.p + (size_t)j * n;
+        HIP_TRY(hipMemsetAsync(zj, 0, n * sizeof(zd), S.st));
+        double iest = 0.0;
+        bool iconv = false;
+        inner_total += gmres_core(S, Ki, false, K.V.p + (size_t)j * n, zj, inner_restart, tau, inner_restart, &iest,
+                                  nullptr, 0, &iconv);
+        // w = A_fp64 z_j, orthogonalised against V (FP64 Arnoldi)
+        S.apply(zj, K.w.p, true);
+        S.arnoldi_step(K, m, j, hv);
         for (int i = 0; i < j; i++) apply_givens(rot[i], hv[i], hv[i + 1]);
         rot[j] = make_givens(hv[j], hv[j + 1]);
         apply_givens(rot[j], hv[j], hv[j + 1]);
         apply_givens(rot[j], g[j], g[j + 1]);
         for (int i = 0; i <= j; i++) H[(size_t)i * m + j] = hv[i];
-        est = std::abs(g[j + 1]) / r0n;
+        const double est = std::abs(g[j + 1]) / r0n;
         if (hist && it - 1 < hist_cap) hist[it - 1] = est;
         if (est < tol) {
           converged = true;
@@ -226,31 +368,19 @@ extern "C" int ifgf_gmres(ifgf_plan* P, const double* rhs, double* x, int restar
           break;
         }
       }
-      // y = R^{-1} g; x += V y
-      const int kk = j;
-      std::vector<C> y(kk);
-      for (int i = kk - 1; i >= 0; i--) {
-        C s = g[i];
-        for (int q = i + 1; q < kk; q++) s -= H[(size_t)i * m + q] * y[q];
-        y[i] = s / H[(size_t)i * m + i];
-      }
-      HIP_TRY(hipMemcpyAsync(h.p, y.data(), kk * sizeof(zd), hipMemcpyHostToDevice, st));
-      hipLaunchKernelGGL(k_update, dim3(nb), dim3(256), 0, st, n, kk, V.p, n, h.p, -1.0, xd.p);
+      // x += Z y (flexible: the update uses the preconditioned directions)
+      S.update(K, Z.p, j, back_substitute(H, m, g, j), xd.p);
       if (!converged && it < maxit) {
-        residual_into(tmp.p);
-        norm2(tmp.p, h.p);
-        fetch(h.p, 1, hv);
-        beta = std::sqrt(hv[0].real());
+        S.residual(b.p, xd.p, K.tmp.p, true);
+        beta = S.norm(K, K.tmp.p);
       }
     }
-    // true residual
-    residual_into(tmp.p);
-    norm2(tmp.p, h.p);
-    fetch(h.p, 1, hv);
-    *relres = std::sqrt(hv[0].real()) / bnorm;
-    *iters = it;
-    HIP_TRY(hipMemcpyAsync(x, xd.p, n * sizeof(zd), hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
+    S.residual(b.p, xd.p, K.tmp.p, true);
+    *relres = S.norm(K, K.tmp.p) / bnorm;
+    *outer_iters = it;
+    *inner_iters = inner_total;
+    HIP_TRY(hipMemcpyAsync(x, xd.p, n * sizeof(zd), hipMemcpyDeviceToHost, S.st));
+    HIP_TRY(hipStreamSynchronize(S.st));
     return converged ? IFGF_OK : IFGF_ERR_NOT_CONVERGED;
   } catch (std::exception& e) {
     return plan_fail(P, IFGF_ERR_HIP, e.what());

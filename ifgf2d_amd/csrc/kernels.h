@@ -34,8 +34,20 @@ __device__ inline int xcd_block() {
   return x * base + (x < rem ? x : rem) + i;
 }
 
-// Two doubles as one 16-byte vector load (a complex value).
+// Two doubles as one 16-byte vector load (a complex value); two floats as one
+// 8-byte load (an FP32 complex value).
 typedef double dbl2 __attribute__((ext_vector_type(2)));
+typedef float flt2 __attribute__((ext_vector_type(2)));
+template <typename Z>
+struct vec2_of;
+template <>
+struct vec2_of<zd> {
+  using type = dbl2;
+};
+template <>
+struct vec2_of<zf> {
+  using type = flt2;
+};
 
 // Functions::Atan2 (PolarCoordinates.h:8-27): theta in [0, 2 pi).
 __device__ inline double atan2_02pi(double y, double x) {
@@ -71,27 +83,29 @@ __device__ inline void cone_local(const ConeGeo& g, double x, double y, double& 
   radius = g.h / ((s + 1.0) / 2.0 * g.dr + g.rmin);
 }
 
-// Interpolator::Interpolate (Interpolator.h:55-77).
-template <int Ps, int Pang, typename CT>
-__device__ inline zd cone_eval(const CT* __restrict__ c, double s, double t) {
-  double ts[Ps], ta[Pang];
-  ts[0] = 1.0;
+// Interpolator::Interpolate (Interpolator.h:55-77), in the precision of the
+// coefficients (FP64 zd or FP32 zf).
+template <int Ps, int Pang, typename Z>
+__device__ inline Z cone_eval(const Z* __restrict__ c, decltype(Z::x) s, decltype(Z::x) t) {
+  using R = decltype(Z::x);
+  R ts[Ps], ta[Pang];
+  ts[0] = 1;
   if (Ps > 1) ts[1] = s;
 #pragma unroll
-  for (int i = 2; i < Ps; i++) ts[i] = 2.0 * s * ts[i - 1] - ts[i - 2];
-  ta[0] = 1.0;
+  for (int i = 2; i < Ps; i++) ts[i] = 2 * s * ts[i - 1] - ts[i - 2];
+  ta[0] = 1;
   if (Pang > 1) ta[1] = t;
 #pragma unroll
-  for (int i = 2; i < Pang; i++) ta[i] = 2.0 * t * ta[i - 1] - ta[i - 2];
-  zd acc{0.0, 0.0};
+  for (int i = 2; i < Pang; i++) ta[i] = 2 * t * ta[i - 1] - ta[i - 2];
+  Z acc{0, 0};
 #pragma unroll
   for (int a = 0; a < Pang; a++) {
-    zd row{0.0, 0.0};
+    Z row{0, 0};
 #pragma unroll
     for (int m = 0; m < Ps; m++) {
-      const CT v = c[a * Ps + m];
-      row.x = fma(ts[m], (double)v.x, row.x);
-      row.y = fma(ts[m], (double)v.y, row.y);
+      const Z v = c[a * Ps + m];
+      row.x = fma(ts[m], v.x, row.x);
+      row.y = fma(ts[m], v.y, row.y);
     }
     acc.x = fma(ta[a], row.x, acc.x);
     acc.y = fma(ta[a], row.y, acc.y);
@@ -102,15 +116,15 @@ __device__ inline zd cone_eval(const CT* __restrict__ c, double s, double t) {
 // Separable 2D Chebyshev transform of the P values of one cone held in LDS
 // (Interpolator::GenerateInterpolant, Interpolator.h:81-120). Thread j of the
 // cone computes output j; buf and tmp are this cone's P-slot LDS windows.
-template <int Ps, int Pang>
-__device__ inline zd cone_transform(const zd* buf, zd* tmp, const ChebTables& T, int j, bool active) {
+template <int Ps, int Pang, typename Z, typename TT>
+__device__ inline Z cone_transform(const Z* buf, Z* tmp, const TT& T, int j, bool active) {
   const int n = j / Ps, m = j % Ps;
   // stage 1: tmp[n][k] = sum_l f[l][k] Ta[n][l]
   if (active) {
-    zd g{0.0, 0.0};
+    Z g{0, 0};
 #pragma unroll
     for (int l = 0; l < Pang; l++) {
-      const zd f = buf[l * Ps + m];
+      const Z f = buf[l * Ps + m];
       g.x = fma(f.x, T.Ta[n * 16 + l], g.x);
       g.y = fma(f.y, T.Ta[n * 16 + l], g.y);
     }
@@ -118,11 +132,11 @@ __device__ inline zd cone_transform(const zd* buf, zd* tmp, const ChebTables& T,
   }
   __syncthreads();
   // stage 2: C[n][m] = sum_k tmp[n][k] Ts[m][k]
-  zd c{0.0, 0.0};
+  Z c{0, 0};
   if (active) {
 #pragma unroll
     for (int k = 0; k < Ps; k++) {
-      const zd v = tmp[n * Ps + k];
+      const Z v = tmp[n * Ps + k];
       c.x = fma(v.x, T.Ts[m * 16 + k], c.x);
       c.y = fma(v.y, T.Ts[m * 16 + k], c.y);
     }
@@ -141,6 +155,12 @@ __global__ void __launch_bounds__(256) k_gather(int N, const zd* __restrict__ x,
 }
 
 // Sum of S lanes' partial results (lanes i*S .. i*S+S-1 of a wave), fixed order.
+template <int S, typename R>
+__device__ inline R lane_sum(R v) {
+#pragma unroll
+  for (int o = S / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, S);
+  return v;
+}
 template <int S>
 __device__ inline zd lane_group_sum(zd v) {
 #pragma unroll
@@ -425,33 +445,35 @@ __global__ void __launch_bounds__(256)
 // items keep every wave's chain to one item -> values/densities round trip.
 constexpr int kLeafItemTiles = 4;
 
-template <int P>
+template <int P, typename Z>
 __global__ void __launch_bounds__(256)
-    k_leaf_items(int nitems, const LeafItem* __restrict__ items, const zd* __restrict__ lf,
-                 const zd* __restrict__ rho, zd* __restrict__ lpart) {
+    k_leaf_items(int nitems, const LeafItem* __restrict__ items, const Z* __restrict__ lf,
+                 const zd* __restrict__ rho, Z* __restrict__ lpart) {
   constexpr int S = kLeafChunk;
+  using V2 = typename vec2_of<Z>::type;
   const long long gt = (long long)xcd_block() * blockDim.x + threadIdx.x;
   const long long item = gt / (P * S);
   const int r = (int)(gt % (P * S));
   const int m = r / S, sl = r % S;
-  zd acc{0.0, 0.0};
+  Z acc{0, 0};
   if (item < nitems) {
     const LeafItem it = items[item];
-    const dbl2* __restrict__ L = reinterpret_cast<const dbl2*>(lf) + it.off + m * S + sl;
-    dbl2 val[kLeafItemTiles];
-    zd rv[kLeafItemTiles];
+    const V2* __restrict__ L = reinterpret_cast<const V2*>(lf) + it.off + m * S + sl;
+    V2 val[kLeafItemTiles];
+    Z rv[kLeafItemTiles];
 #pragma unroll
     for (int u = 0; u < kLeafItemTiles; u++) {
       if (u * S + sl < it.n) {
         val[u] = L[u * S * P];
-        rv[u] = rho[it.s0 + u * S + sl];
+        rv[u] = zcast<Z>(rho[it.s0 + u * S + sl]);
       }
     }
 #pragma unroll
     for (int u = 0; u < kLeafItemTiles; u++)
-      if (u * S + sl < it.n) cfma(acc, zd{val[u].x, val[u].y}, rv[u]);
+      if (u * S + sl < it.n) zfma(acc, Z{val[u].x, val[u].y}, rv[u]);
   }
-  acc = lane_group_sum<S>(acc);
+  acc.x = lane_sum<S>(acc.x);
+  acc.y = lane_sum<S>(acc.y);
   if (item < nitems && sl == 0) lpart[item * P + m] = acc;
 }
 
@@ -533,38 +555,44 @@ __global__ void __launch_bounds__(256)
 // A level costs one dependent round trip (the four child slots) before the
 // LDS work, instead of the pull form's entry range -> child entries -> child
 // coefficients chain.
-struct PushArgs {
+template <typename R>
+struct PushArgsT {
+  using Z = typename complex_of<R>::type;
   const PushTask* tasks;
-  const Geo4* geo;       // per push entry: (s, theta, centre factor)
+  const Geo4T<R>* geo;   // per push entry: (s, theta, centre factor)
   const int2* dst;       // per push entry: (destination slot in pbuf, global cone id)
   const int* lio;        // level D: items of leaf cone c: [lio[c], lio[c+1])
-  const zd* lpart;       // level D: leaf item partial coefficients [item][P]
+  const Z* lpart;        // level D: leaf item partial coefficients [item][P]
   long long leaf_cone_off;
-  const zd* pn_ifac;     // 1 / centre factor per (cone, node), levels < D
-  const ChebTables* tabs;
-  zd* pbuf;              // child slots of every (cone, node), then interpolation entries
+  const Z* pn_ifac;      // 1 / centre factor per (cone, node), levels < D
+  const ChebTablesT<R>* tabs;
+  Z* pbuf;               // child slots of every (cone, node), then interpolation entries
   int leaf;              // 1 at level D
 };
 
 // EPT entries per thread: a task holds at most 256 * EPT entries, and every
 // thread loads all of its entries together with the cones' inputs before the
 // first barrier, so a level costs one memory round trip before the LDS work.
-template <int Ps, int Pang, int EPT>
-__global__ void __launch_bounds__(256) k_level_push(PushArgs a) {
+// R = double on a plain plan; float (FP32 coefficients, transforms and
+// evaluations, FP64-evaluated centre factors rounded to FP32) on a
+// mixed-precision plan.
+template <int Ps, int Pang, int EPT, typename R>
+__global__ void __launch_bounds__(256) k_level_push(PushArgsT<R> a) {
+  using Z = typename complex_of<R>::type;
   constexpr int P = Ps * Pang;
   constexpr int CMAX = 256 / P;
-  __shared__ ChebTables T;
-  __shared__ zd buf[CMAX * P];
-  __shared__ zd tmp[CMAX * P];
+  __shared__ ChebTablesT<R> T;
+  __shared__ Z buf[CMAX * P];
+  __shared__ Z tmp[CMAX * P];
   const PushTask task = a.tasks[xcd_block()];
   const int tid = threadIdx.x;
   if (!a.leaf)
-    for (int q = tid; q < (int)(sizeof(ChebTables) / sizeof(double)); q += blockDim.x)
-      ((double*)&T)[q] = ((const double*)a.tabs)[q];
+    for (int q = tid; q < (int)(sizeof(ChebTablesT<R>) / sizeof(R)); q += blockDim.x)
+      ((R*)&T)[q] = ((const R*)a.tabs)[q];
   const int cl = tid / P, j = tid % P;
   const bool act = cl < task.nc;
   const long long g = (long long)task.c0 + cl;
-  Geo4 G[EPT];
+  Geo4T<R> G[EPT];
   int2 d[EPT];
 #pragma unroll
   for (int u = 0; u < EPT; u++) {
@@ -574,22 +602,22 @@ __global__ void __launch_bounds__(256) k_level_push(PushArgs a) {
       d[u] = a.dst[task.e0 + e];
     }
   }
-  zd v{0.0, 0.0};
+  Z v{0, 0};
   if (act) {
     if (a.leaf) {
       const int c = (int)(g - a.leaf_cone_off);
       const int w1 = a.lio[c + 1];
-      for (int w = a.lio[c]; w < w1; w++) v = cadd(v, a.lpart[(size_t)w * P + j]);
+      for (int w = a.lio[c]; w < w1; w++) v = zadd(v, a.lpart[(size_t)w * P + j]);
     } else {
-      const zd* s = a.pbuf + (g * P + j) * 4;
-      const zd s0 = s[0], s1 = s[1], s2 = s[2], s3 = s[3];
-      v = cmul(cadd(cadd(cadd(s0, s1), s2), s3), a.pn_ifac[g * P + j]);
+      const Z* s = a.pbuf + (g * P + j) * 4;
+      const Z s0 = s[0], s1 = s[1], s2 = s[2], s3 = s[3];
+      v = zmul(zadd(zadd(zadd(s0, s1), s2), s3), a.pn_ifac[g * P + j]);
     }
     buf[tid] = v;
   }
   __syncthreads();
   if (!a.leaf) {
-    const zd c = cone_transform<Ps, Pang>(buf + (act ? cl * P : 0), tmp + (act ? cl * P : 0), T, act ? j : 0, act);
+    const Z c = cone_transform<Ps, Pang>(buf + (act ? cl * P : 0), tmp + (act ? cl * P : 0), T, act ? j : 0, act);
     // every thread has passed the transform's barrier, so buf is free again
     if (act) buf[tid] = c;
     __syncthreads();
@@ -597,10 +625,19 @@ __global__ void __launch_bounds__(256) k_level_push(PushArgs a) {
 #pragma unroll
   for (int u = 0; u < EPT; u++) {
     if (tid + u * 256 < task.ne) {
-      const zd val = cone_eval<Ps, Pang>(buf + (d[u].y - task.c0) * P, G[u].s, G[u].t);
-      a.pbuf[d[u].x] = cmul(val, zd{G[u].fre, G[u].fim});
+      const Z val = cone_eval<Ps, Pang>(buf + (d[u].y - task.c0) * P, G[u].s, G[u].t);
+      a.pbuf[d[u].x] = zmul(val, Z{G[u].fre, G[u].fim});
     }
   }
+}
+
+// FP64 -> FP32 copies of the stored operators of a mixed-precision plan
+// (complex values and (s, theta, factor) records alike: n pairs of doubles).
+__global__ void __launch_bounds__(256) k_round_f32(long long n, const dbl2* __restrict__ in, flt2* __restrict__ out) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const dbl2 v = in[i];
+  out[i] = flt2{(float)v.x, (float)v.y};
 }
 
 // Build: push entry q takes its geometry from the propagation (src < npr)
@@ -700,13 +737,13 @@ __global__ void __launch_bounds__(256)
 // [nio[si / 8], nio[si / 8 + 1]) of its group; on the fly: sol[si]) and its
 // interpolation entries (stored path: ient[it_ptr[si] .. it_ptr[si+1]), one
 // per (level, cousin cone)); lane 0 adds the identity.
-template <int S>
+template <int S, typename ZI>
 __global__ void __launch_bounds__(256)
     k_finalize(int N, int Np, double ident, const zd* __restrict__ x,
                const long long* __restrict__ corr_ptr, const int* __restrict__ corr_patch,
                const zd* __restrict__ corr_w, const int* __restrict__ inv, const zd* __restrict__ sol,
                const int* __restrict__ nio, const zd* __restrict__ npart,
-               const long long* __restrict__ it_ptr, const zd* __restrict__ ient, zd* __restrict__ y) {
+               const long long* __restrict__ it_ptr, const ZI* __restrict__ ient, zd* __restrict__ y) {
   const int lb = xcd_block();
   const int gt = lb * blockDim.x + threadIdx.x;
   const int i = gt / S, sl = gt % S;
@@ -730,7 +767,7 @@ __global__ void __launch_bounds__(256)
     }
     if (ient) {
       const long long f1 = it_ptr[si + 1];
-      for (long long f = it_ptr[si] + sl; f < f1; f += S) far = cadd(far, ient[f]);
+      for (long long f = it_ptr[si] + sl; f < f1; f += S) far = cadd(far, zwiden(ient[f]));
     }
   }
   acc = lane_group_sum<S>(acc);

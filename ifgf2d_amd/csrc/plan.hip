@@ -8,6 +8,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/ifgf2d.h"
@@ -83,7 +84,10 @@ int ifgf_plan_create(const ifgf_surface* s, const ifgf_params* prm, ifgf_plan** 
   if (prm->Ps < 2 || prm->Ps > 16 || prm->Pang < 2 || prm->Pang > 16)
     return fail(nullptr, IFGF_ERR_ARG, "Ps and Pang must lie in [2, 16]");
   if (prm->formulation < 0 || prm->formulation > 2) return fail(nullptr, IFGF_ERR_ARG, "unknown formulation");
-  if (prm->precision != IFGF_FP64) return fail(nullptr, IFGF_ERR_ARG, "unsupported precision mode");
+  if (prm->precision != IFGF_FP64 && prm->precision != IFGF_MIXED)
+    return fail(nullptr, IFGF_ERR_ARG, "unsupported precision mode");
+  if (prm->precision == IFGF_MIXED && (prm->flags & IFGF_PLAN_ON_THE_FLY))
+    return fail(nullptr, IFGF_ERR_ARG, "mixed precision needs the stored operators (no IFGF_PLAN_ON_THE_FLY)");
   if (!s->patch_lims || s->npatches < 3 || !(s->delta > 0) || !(s->rp_p >= 2))
     return fail(nullptr, IFGF_ERR_ARG, "invalid surface description");
   std::unique_ptr<ifgf_plan> P(new ifgf_plan);
@@ -255,28 +259,34 @@ static void launch_near_stored(ifgf_plan* P, hipStream_t st) {
                      P->nf_val.p, P->rho.p, P->npart.p);
 }
 
-template <int Ps, int Pang, int EPT>
+template <int Ps, int Pang, int EPT, typename R>
 static void launch_level_push_ept(ifgf_plan* P, hipStream_t st) {
   const int D = P->tree.D;
-  PushArgs a{nullptr,          P->push_geo.p, P->push_dst.p, P->leaf_item_off.p, P->lpart.p,
-             P->tree.lv[D].cone_off, P->pn_ifac.p, P->tabs.p,  P->pbuf.p,          0};
+  PushArgsT<R> a{};
+  if constexpr (std::is_same<R, float>::value) {
+    a = PushArgsT<R>{nullptr,     P->push_geo32.p, P->push_dst.p, P->leaf_item_off.p, P->lpart32.p,
+                     P->tree.lv[D].cone_off, P->pn_ifac32.p, P->tabs32.p, P->pbuf32.p, 0};
+  } else {
+    a = PushArgsT<R>{nullptr,   P->push_geo.p, P->push_dst.p, P->leaf_item_off.p, P->lpart.p,
+                     P->tree.lv[D].cone_off, P->pn_ifac.p, P->tabs.p, P->pbuf.p, 0};
+  }
   for (int li = 0; li + 1 < D; li++) {
     const int t0 = P->push_level_off[li], nt = P->push_level_off[li + 1] - t0;
     if (nt == 0) continue;
     a.tasks = P->push_tasks.p + t0;
     a.leaf = li == 0;
-    hipLaunchKernelGGL((k_level_push<Ps, Pang, EPT>), dim3(nt), dim3(256), 0, st, a);
+    hipLaunchKernelGGL((k_level_push<Ps, Pang, EPT, R>), dim3(nt), dim3(256), 0, st, a);
   }
 }
 
 // tasks hold at most 256 * EPT entries (push_emax)
-template <int Ps, int Pang>
+template <int Ps, int Pang, typename R>
 static void launch_level_push(ifgf_plan* P, hipStream_t st) {
   switch (P->push_emax) {
-    case 256: launch_level_push_ept<Ps, Pang, 1>(P, st); break;
-    case 512: launch_level_push_ept<Ps, Pang, 2>(P, st); break;
-    case 1024: launch_level_push_ept<Ps, Pang, 4>(P, st); break;
-    default: launch_level_push_ept<Ps, Pang, 8>(P, st); break;
+    case 256: launch_level_push_ept<Ps, Pang, 1, R>(P, st); break;
+    case 512: launch_level_push_ept<Ps, Pang, 2, R>(P, st); break;
+    case 1024: launch_level_push_ept<Ps, Pang, 4, R>(P, st); break;
+    default: launch_level_push_ept<Ps, Pang, 8, R>(P, st); break;
   }
 }
 
@@ -285,8 +295,12 @@ static void launch_level_push(ifgf_plan* P, hipStream_t st) {
 // identity and unsort. (Schedules that overlapped whole stages on a second
 // stream or in shared grids were measured slower on the 64k kite: the
 // kernels' times added up under contention.)
-template <int F, int Ps, int Pang>
+// R = double: the all-FP64 operator; R = float: the mixed-precision operator
+// (FP32 leaf operator, coefficients, propagation and interpolation).
+template <int F, int Ps, int Pang, typename R>
 static void launch_ifgf(ifgf_plan* P, const zd* x, zd* y, hipStream_t st) {
+  using Z = typename complex_of<R>::type;
+  constexpr bool f32 = std::is_same<R, float>::value;
   constexpr int Pn = Ps * Pang;
   constexpr int CPB = 256 / Pn;
   constexpr int kLeafLanes = LeafLanes<Pn>::value;
@@ -320,8 +334,10 @@ static void launch_ifgf(ifgf_plan* P, const zd* x, zd* y, hipStream_t st) {
   const LevelInfo& LD = t.lv[D];
   if (P->stored) {
     const long long M = P->leaf_items.n;
-    hipLaunchKernelGGL((k_leaf_items<Pn>), dim3((unsigned)((M * Pn * kLeafChunk + 255) / 256)), dim3(256), 0, st,
-                       (int)M, P->leaf_items.p, P->lf_val.p, P->rho.p, P->lpart.p);
+    const Z* lf = f32 ? (const Z*)P->lf_val32.p : (const Z*)P->lf_val.p;
+    Z* lpart = f32 ? (Z*)P->lpart32.p : (Z*)P->lpart.p;
+    hipLaunchKernelGGL((k_leaf_items<Pn, Z>), dim3((unsigned)((M * Pn * kLeafChunk + 255) / 256)), dim3(256), 0, st,
+                       (int)M, P->leaf_items.p, lf, P->rho.p, lpart);
   } else {
     constexpr int LCPB = 256 / (Pn * kLeafLanes);
     hipLaunchKernelGGL((k_leaf<F, Ps, Pang, kLeafLanes>), dim3((LD.ncone + LCPB - 1) / LCPB), dim3(256), 0, st,
@@ -335,7 +351,7 @@ static void launch_ifgf(ifgf_plan* P, const zd* x, zd* y, hipStream_t st) {
   // pull chain and one interpolation pass over all points
   mark(6);
   if (P->stored) {
-    launch_level_push<Ps, Pang>(P, st);
+    launch_level_push<Ps, Pang, R>(P, st);
   } else {
     for (int l = D; l >= 3; l--) {
       const LevelInfo& L = t.lv[l];
@@ -357,10 +373,12 @@ static void launch_ifgf(ifgf_plan* P, const zd* x, zd* y, hipStream_t st) {
   // correction, identity and unsort
   mark(10);
   const double ident = (F == kSingleLayer) ? 0.0 : 0.5;
-#define IFGF_FIN(S)                                                                                            \
-  hipLaunchKernelGGL(k_finalize<S>, dim3((N + 256 / S - 1) / (256 / S)), dim3(256), 0, st, N, P->geo.Np, ident, x, \
-                     P->corr_ptr.p, P->corr_patch.p, P->corr_w.p, P->inv.p, P->sol.p, P->near_item_off.p,           \
-                     P->stored ? P->npart.p : nullptr, P->it_ptr.p, P->stored ? P->pbuf.p + P->pbuf_ient : nullptr, y)
+  const Z* ient = nullptr;
+  if (P->stored) ient = f32 ? (const Z*)P->pbuf32.p + P->pbuf_ient : (const Z*)P->pbuf.p + P->pbuf_ient;
+#define IFGF_FIN(S)                                                                                                \
+  hipLaunchKernelGGL((k_finalize<S, Z>), dim3((N + 256 / S - 1) / (256 / S)), dim3(256), 0, st, N, P->geo.Np, ident, \
+                     x, P->corr_ptr.p, P->corr_patch.p, P->corr_w.p, P->inv.p, P->sol.p, P->near_item_off.p,       \
+                     P->stored ? P->npart.p : nullptr, P->it_ptr.p, ient, y)
   if (P->final_lanes == 4) IFGF_FIN(4);
   else if (P->final_lanes == 16) IFGF_FIN(16);
   else IFGF_FIN(8);
@@ -609,6 +627,39 @@ static void build_stored(ifgf_plan* P) {
   P->stored_bytes = (long long)(nf_goff[ngroups] + lf_off[LD.ncone] + (long long)t.n_cones * Pn + ncs + nit) *
                         (long long)sizeof(zd) +
                     (npr + nit) * (long long)(sizeof(Geo4) + sizeof(int2));
+  if (P->prm.precision == IFGF_MIXED) {
+    // FP32 copies: leaf operator, propagation / interpolation geometry
+    // (factors evaluated in FP64 above, rounded here), inverse node factors
+    auto round = [&](const void* in, void* out, long long npairs) {
+      if (npairs)
+        hipLaunchKernelGGL(k_round_f32, dim3((unsigned)((npairs + 255) / 256)), dim3(256), 0, st, npairs,
+                           (const dbl2*)in, (flt2*)out);
+    };
+    P->lf_val32.alloc(P->lf_val.n);
+    round(P->lf_val.p, P->lf_val32.p, (long long)P->lf_val.n);
+    P->push_geo32.alloc(P->push_geo.n);
+    round(P->push_geo.p, P->push_geo32.p, 2 * (long long)P->push_geo.n);
+    P->pn_ifac32.alloc(P->pn_ifac.n);
+    round(P->pn_ifac.p, P->pn_ifac32.p, (long long)P->pn_ifac.n);
+    P->pbuf32.alloc(P->pbuf.n);
+    HIP_TRY(hipMemsetAsync(P->pbuf32.p, 0, P->pbuf32.n * sizeof(zf), st));
+    P->lpart32.alloc(P->lpart.n);
+    ChebTablesF tf;
+    for (int q = 0; q < 16; q++) {
+      tf.xs[q] = (float)P->cheb.xs[q];
+      tf.xa[q] = (float)P->cheb.xa[q];
+    }
+    for (int q = 0; q < 256; q++) {
+      tf.Ts[q] = (float)P->cheb.Ts[q];
+      tf.Ta[q] = (float)P->cheb.Ta[q];
+    }
+    P->tabs32.upload(std::vector<ChebTablesF>{tf});
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(st));
+    P->stored_bytes += (long long)(P->lf_val32.n + P->pn_ifac32.n + P->pbuf32.n) * (long long)sizeof(zf) +
+                       (long long)P->push_geo32.n * (long long)sizeof(Geo4F);
+    P->mixed = true;
+  }
   P->stored = true;
 }
 
@@ -639,12 +690,14 @@ static bool build_stored_any(ifgf_plan* P) {
 }
 
 template <int F>
-static bool dispatch_p(ifgf_plan* P, const zd* x, zd* y, hipStream_t st) {
+static bool dispatch_p(ifgf_plan* P, const zd* x, zd* y, hipStream_t st, bool fp64) {
   const int a = P->prm.Ps, b = P->prm.Pang;
-#define IFGF_CASE(A, B)                       \
-  if (a == A && b == B) {                     \
-    launch_ifgf<F, A, B>(P, x, y, st);        \
-    return true;                              \
+  const bool f32 = P->mixed && !fp64;
+#define IFGF_CASE(A, B)                                   \
+  if (a == A && b == B) {                                 \
+    if (f32) launch_ifgf<F, A, B, float>(P, x, y, st);    \
+    else launch_ifgf<F, A, B, double>(P, x, y, st);       \
+    return true;                                          \
   }
   IFGF_CASE(5, 5)
   IFGF_CASE(3, 5)
@@ -656,11 +709,11 @@ static bool dispatch_p(ifgf_plan* P, const zd* x, zd* y, hipStream_t st) {
   return false;
 }
 
-static bool dispatch(ifgf_plan* P, const zd* x, zd* y, hipStream_t st) {
+static bool dispatch(ifgf_plan* P, const zd* x, zd* y, hipStream_t st, bool fp64 = false) {
   switch (P->prm.formulation) {
-    case kSingleLayer: return dispatch_p<kSingleLayer>(P, x, y, st);
-    case kDoubleLayer: return dispatch_p<kDoubleLayer>(P, x, y, st);
-    default: return dispatch_p<kCombined>(P, x, y, st);
+    case kSingleLayer: return dispatch_p<kSingleLayer>(P, x, y, st, fp64);
+    case kDoubleLayer: return dispatch_p<kDoubleLayer>(P, x, y, st, fp64);
+    default: return dispatch_p<kCombined>(P, x, y, st, fp64);
   }
 }
 
@@ -738,8 +791,8 @@ static void collect_profile(ifgf_plan* P) {
 }
 
 // one device-resident application on stream st (used by ifgf_apply and GMRES)
-int ifgf::apply_device(ifgf_plan* P, const zd* x, zd* y, hipStream_t st) {
-  if (!dispatch(P, x, y, st)) return fail(P, IFGF_ERR_ARG, "no kernel instantiation for this (Ps, Pang)");
+int ifgf::apply_device(ifgf_plan* P, const zd* x, zd* y, hipStream_t st, bool fp64) {
+  if (!dispatch(P, x, y, st, fp64)) return fail(P, IFGF_ERR_ARG, "no kernel instantiation for this (Ps, Pang)");
   // bound the number of outstanding event sets
   if (P->profile && P->ev_used >= (size_t)events_per_apply(P) * 4096) collect_profile(P);
   return IFGF_OK;
@@ -757,12 +810,13 @@ int ifgf_apply(ifgf_plan* P, const double* x, double* y, int nrhs, long long ld,
     for (int r = 0; r < nrhs; r++) {
       const zd* xr = (const zd*)x + (size_t)r * ld;
       zd* yr = (zd*)y + (size_t)r * ld;
+      const bool fp64 = (flags & IFGF_APPLY_FP64) != 0;
       if (flags & IFGF_PTR_DEVICE) {
-        int rc = apply_device(P, xr, yr, st);
+        int rc = apply_device(P, xr, yr, st, fp64);
         if (rc) return rc;
       } else {
         HIP_TRY(hipMemcpyAsync(P->xbuf.p, xr, N * sizeof(zd), hipMemcpyHostToDevice, st));
-        int rc = apply_device(P, P->xbuf.p, P->ybuf.p, st);
+        int rc = apply_device(P, P->xbuf.p, P->ybuf.p, st, fp64);
         if (rc) return rc;
         HIP_TRY(hipMemcpyAsync(yr, P->ybuf.p, N * sizeof(zd), hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));

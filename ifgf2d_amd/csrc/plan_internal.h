@@ -50,6 +50,7 @@ struct DevBuf {
 
 struct ifgf_plan {
   using zd = ifgf::zd;
+  using zf = ifgf::zf;
   std::string err;
   ifgf_params prm{};
   ifgf::Curve curve;
@@ -93,6 +94,13 @@ struct ifgf_plan {
   ifgf::DevBuf<ifgf::NearItem> near_items;  // near-field work items
   ifgf::DevBuf<int> near_item_off;          // items of target group g: [off[g], off[g+1])
   ifgf::DevBuf<zd> npart;                   // near-field partial sums [item][8]
+  // mixed precision (IFGF_MIXED): FP32 copies of the leaf operator, the
+  // propagation / interpolation geometry and inverse node factors, and FP32
+  // coefficient workspaces; the FP64 operator stays available (IFGF_APPLY_FP64)
+  bool mixed = false;
+  ifgf::DevBuf<zf> lf_val32, pn_ifac32, pbuf32, lpart32;
+  ifgf::DevBuf<ifgf::Geo4F> push_geo32;
+  ifgf::DevBuf<ifgf::ChebTablesF> tabs32;
   int nleaf = 0;
   // lanes per work item of the stored-path kernels
   int near_lanes = 8 /* kNearChunk */, leaf_lanes = 4 /* kLeafChunk */, final_lanes = 8;
@@ -117,8 +125,9 @@ struct ifgf_plan {
 namespace ifgf {
 
 
-// one device-resident application on stream st (plan.hip)
-int apply_device(ifgf_plan* P, const zd* x, zd* y, hipStream_t st);
+// one device-resident application on stream st (plan.hip); fp64 selects the
+// all-FP64 operator on a mixed-precision plan
+int apply_device(ifgf_plan* P, const zd* x, zd* y, hipStream_t st, bool fp64 = false);
 int plan_fail(ifgf_plan* p, int code, const std::string& msg);
 
 }  // namespace ifgf

@@ -14,12 +14,17 @@ struct ConeGeo {
   double pad;
 };
 
-// Chebyshev tables of the cone interpolant (Chebyshev.h:10-68), up to 16x16.
-struct ChebTables {
-  double xs[16], xa[16];    // ascending Chebyshev points in s and theta
-  double Ts[16 * 16];       // Ts[m*16 + k] = T_m(xs[k]) * alpha_m
-  double Ta[16 * 16];       // Ta[n*16 + l] = T_n(xa[l]) * alpha_n / (Ps*Pang)
+// Chebyshev tables of the cone interpolant (Chebyshev.h:10-68), up to 16x16,
+// in FP64 (plain plans) or rounded to FP32 (the cone transforms of a
+// mixed-precision plan).
+template <typename R>
+struct ChebTablesT {
+  R xs[16], xa[16];    // ascending Chebyshev points in s and theta
+  R Ts[16 * 16];       // Ts[m*16 + k] = T_m(xs[k]) * alpha_m
+  R Ta[16 * 16];       // Ta[n*16 + l] = T_n(xa[l]) * alpha_n / (Ps*Pang)
 };
+using ChebTables = ChebTablesT<double>;
+using ChebTablesF = ChebTablesT<float>;
 
 // One near-field work item: kNearItemChunks (or fewer) chunks of one target
 // group, starting at value offset off.
@@ -45,10 +50,14 @@ struct PushTask {
 };
 
 // Local Chebyshev coordinates (s, theta) of a point in a cone and the centre
-// factor exp(i k r)/sqrt(r) at its distance r from the cone's box centre.
-struct Geo4 {
-  double s, t;
-  double fre, fim;
+// factor exp(i k r)/sqrt(r) at its distance r from the cone's box centre
+// (evaluated in FP64; stored rounded to FP32 on a mixed-precision plan).
+template <typename R>
+struct Geo4T {
+  R s, t;
+  R fre, fim;
 };
+using Geo4 = Geo4T<double>;
+using Geo4F = Geo4T<float>;
 
 }  // namespace ifgf

@@ -41,4 +41,38 @@ IFGF_HD inline zd cdiv(zd a, zd b) {
   return zd{(a.x * b.x + a.y * b.y) * d, (a.y * b.x - a.x * b.y) * d};
 }
 
+// The same operations for either precision (Z = zd or zf), used by the kernels
+// that run in FP64 on a plain plan and in FP32 on a mixed-precision plan.
+template <typename R>
+struct complex_of;
+template <>
+struct complex_of<double> {
+  using type = zd;
+};
+template <>
+struct complex_of<float> {
+  using type = zf;
+};
+template <typename Z>
+IFGF_HD inline Z zadd(Z a, Z b) {
+  return Z{a.x + b.x, a.y + b.y};
+}
+template <typename Z>
+IFGF_HD inline Z zmul(Z a, Z b) {
+  return Z{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename Z>
+IFGF_HD inline void zfma(Z& a, Z b, Z c) {
+  a.x = fma(b.x, c.x, fma(-b.y, c.y, a.x));
+  a.y = fma(b.x, c.y, fma(b.y, c.x, a.y));
+}
+template <typename Z>
+IFGF_HD inline Z zcast(zd a) {
+  return Z{(decltype(Z::x))a.x, (decltype(Z::y))a.y};
+}
+template <typename Z>
+IFGF_HD inline zd zwiden(Z a) {
+  return zd{(double)a.x, (double)a.y};
+}
+
 }  // namespace ifgf

@@ -36,8 +36,10 @@ extern "C" {
 
 /* precision modes */
 #define IFGF_FP64 0   /* everything in FP64 */
-#define IFGF_MIXED 1  /* FP32 cone coefficients, propagation and interpolation;
-                         FP64 Hankel, centre factors, near field and correction */
+#define IFGF_MIXED 1  /* FP32 leaf operator, cone coefficients, propagation and interpolation;
+                         FP64 Hankel, centre factors (evaluated in FP64, applied rounded to
+                         FP32), near field, correction and final accumulation. The plan also
+                         keeps the FP64 operator (IFGF_APPLY_FP64). Needs stored operators. */
 
 /* curve types (Geometry/Curve.hpp:106-146; star has no reference class) */
 #define IFGF_CURVE_CIRCLE 0 /* params: cx, cy, R */
@@ -56,6 +58,8 @@ extern "C" {
 
 /* apply flags */
 #define IFGF_PTR_DEVICE 1     /* x / y are device pointers; the call is asynchronous on stream */
+#define IFGF_APPLY_FP64 2     /* on an IFGF_MIXED plan: apply the all-FP64 operator instead of the
+                                 mixed-precision one (no effect on an IFGF_FP64 plan) */
 
 typedef struct ifgf_plan ifgf_plan;
 
@@ -159,11 +163,28 @@ int ifgf_propagate(ifgf_plan* plan, const double* tx, const double* ty, long lon
  * re-orthogonalisation (FP64). Stops when the recursive residual estimate
  * |w(k)| / ||r0|| < tol (opGMRES.hpp:207) or after maxit iterations.
  * rhs / x are host pointers of n complex values; x holds the initial guess.
- * relres returns the true FP64 residual ||b - A x|| / ||b|| at exit. hist (may
- * be NULL) receives up to hist_cap residual estimates. Returns IFGF_OK,
- * IFGF_ERR_NOT_CONVERGED or an error. */
+ * relres returns the true residual ||b - A x|| / ||b|| at exit, always with
+ * the all-FP64 operator (on an IFGF_MIXED plan the iterations use the
+ * mixed-precision operator, whose true residual levels off near its FP32
+ * rounding; use ifgf_fgmres there). hist (may be NULL) receives up to
+ * hist_cap residual estimates. Returns IFGF_OK, IFGF_ERR_NOT_CONVERGED or an
+ * error. */
 int ifgf_gmres(ifgf_plan* plan, const double* rhs, double* x, int restart, double tol, int maxit,
                int* iters, double* relres, double* hist, int hist_cap);
+
+/* Flexible GMRES with FP64 refinement (no reference counterpart; the
+ * reference solves with opGMRES only, opGMRES.hpp:68-269): outer FGMRES(restart)
+ * in FP64 with the all-FP64 operator, right-preconditioned by an inner GMRES
+ * cycle (inner_restart iterations at most, x0 = 0) with the plan's default
+ * operator (mixed precision on an IFGF_MIXED plan). The inner relative
+ * tolerance for outer step j is min(0.5, max(inner_floor, tol / e_j)), e_j the
+ * outer relative residual estimate so far. Stops when the outer estimate
+ * |w(k)| / ||r0|| < tol or after maxit outer iterations. outer_iters /
+ * inner_iters return the FP64 Arnoldi steps and the inner operator
+ * applications' Arnoldi steps; relres the true FP64 relative residual. */
+int ifgf_fgmres(ifgf_plan* plan, const double* rhs, double* x, int restart, double tol, int maxit,
+                int inner_restart, double inner_floor, int* outer_iters, int* inner_iters,
+                double* relres, double* hist, int hist_cap);
 
 /* Generic host GMRES on any operator: the reference's Householder GMRES
  * (opGMRES.hpp:68-269) with the mapping of LinearOperator.hpp:54 as a C

@@ -1,0 +1,107 @@
+"""GPU: the mixed-precision operator (IFGF_MIXED) and FGMRES with FP64 refinement.
+
+A mixed plan applies the leaf operator, the cone coefficients, the propagation
+and the interpolation in FP32 (centre factors evaluated in FP64 and rounded);
+near field, correction and accumulation stay FP64. Bars, written per test:
+the mixed operator is within MIXED_TOL of the oracle's FP64 forward map
+(FP32 rounding of the far-field part only); the plan's FP64 operator
+(IFGF_APPLY_FP64) is bitwise the plain plan's; FGMRES (FP64 outer Arnoldi,
+inner mixed GMRES) reaches the FP64 true residual 1e-8 and the same density
+as plain FP64 GMRES.
+"""
+import numpy as np
+import pytest
+
+import ifgf2d_amd as I
+import oracle as O
+from ifgf2d_amd import configs
+
+pytestmark = pytest.mark.gpu
+
+MIXED_TOL = 2e-6  # relative 2-norm, FP32 far field
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def density(n, seed=0):
+    rng = np.random.default_rng(seed)
+    return rng.standard_normal(n) + 1j * rng.standard_normal(n)
+
+
+@pytest.mark.parametrize("curve,npatch,k,nlev,F", [
+    (I.circle(), 40, 10.0, 3, I.COMBINED),
+    (I.kite(), 400, 40.0, 6, I.COMBINED),
+    (I.star(a=[0, 0, 0, 0, 0, 0.3]), 800, 60.0, 7, I.SINGLE_LAYER),
+], ids=["circle-cfie", "kite-cfie", "star-slp"])
+def test_mixed_operator_against_oracle(curve, npatch, k, nlev, F):
+    lims = I.equal_parameter_lims(npatch)
+    delta = I.default_delta(curve, lims)
+    fm = I.ForwardMap(curve, lims, k, nlev, formulation=F, delta=delta, p=6.0, precision="mixed")
+    f64 = I.ForwardMap(curve, lims, k, nlev, formulation=F, delta=delta, p=6.0)
+    om = O.ForwardMap(curve[0], curve[1], lims, formulation=F, k=k, eta=k, delta=delta, p=6.0, nlevels=nlev)
+    print(f"mixed case N={fm.N} k={k} nlev={nlev} F={F}", flush=True)
+    x = density(fm.N, 21)
+    yo = om.apply(x)
+    ym = fm.compute_Ax_acc(x)
+    e = rel(ym, yo)
+    print(f"  mixed vs oracle {e:.3e}", flush=True)
+    assert 0 < e <= MIXED_TOL
+    # the FP64 operator of the mixed plan is the plain plan's, bit for bit
+    y64 = fm.compute_Ax_acc(x, fp64=True)
+    assert np.array_equal(y64, f64.compute_Ax_acc(x))
+    assert rel(y64, yo) <= 1e-12
+    # mixed applies are deterministic
+    assert np.array_equal(ym, fm.compute_Ax_acc(x))
+
+
+def test_mixed_full_size_kite():
+    """64k kite at k = 200: the mixed operator's distance from the FP64 one,
+    set against the IFGF approximation error itself (3e-4 against direct
+    summation), so the FP32 far field costs nothing in accuracy."""
+    kw, _ = configs.kite_k200()
+    fm = I.ForwardMap(**kw, precision="mixed")
+    x = density(fm.N, 22)
+    ym = fm.compute_Ax_acc(x)
+    y64 = fm.compute_Ax_acc(x, fp64=True)
+    e = rel(ym, y64)
+    print("kite 64k mixed vs fp64", e, flush=True)
+    assert 0 < e < MIXED_TOL
+
+
+def test_fgmres_circle_mie():
+    """Sound-soft unit circle, k = 10: FGMRES on a mixed plan reaches the FP64
+    true residual 1e-8 and the Mie series at r = 3, like FP64 GMRES."""
+    from test_gpu_parity import mie
+    kw, d = configs.circle_k10(Ps=9, Pang=9)
+    fm = I.ForwardMap(**kw, precision="mixed")
+    x, y, *_ = fm.points()
+    rhs = -configs.plane_wave(x, y, kw["k"], d)
+    dens, it, iit, relres, hist = fm.solve_fgmres(rhs, restart=20, tol=1e-8, inner_restart=50)
+    print("FGMRES outer", it, "inner", iit, "true relres", relres, flush=True)
+    assert relres < 1e-8
+    th = 2 * np.pi * np.arange(64) / 64
+    us = fm.propagate_unacc(3 * np.cos(th), 3 * np.sin(th), dens)
+    assert rel(us, mie(3.0, th, kw["k"])) < 1e-5
+    d64, it64, rr64, _ = I.ForwardMap(**kw).solve(rhs, restart=50, tol=1e-8)
+    assert rel(dens, d64) < 1e-6
+
+
+def test_fgmres_kite_against_fp64_gmres():
+    """Kite, 400 patches, k = 40: mixed-plan FGMRES against plain FP64 GMRES on
+    the same discrete system (true FP64 residual, densities), and mixed-plan
+    GMRES alone stalls above the FGMRES residual (what the refinement buys)."""
+    kw = configs.scaled(I.kite(), 400, 40.0, 6)
+    fm = I.ForwardMap(**kw, precision="mixed")
+    f64 = I.ForwardMap(**kw)
+    x, y, *_ = fm.points()
+    rhs = -configs.plane_wave(x, y, 40.0, (np.cos(np.pi / 4), np.sin(np.pi / 4)))
+    d64, it64, rr64, _ = f64.solve(rhs, restart=50, tol=1e-8, maxit=500)
+    dm, it, iit, relres, hist = fm.solve_fgmres(rhs, restart=20, tol=1e-8, inner_restart=50)
+    print("fp64 GMRES", it64, rr64, "| FGMRES outer", it, "inner", iit, relres, flush=True)
+    assert rr64 < 1e-8 and relres < 1e-8
+    assert rel(dm, d64) < 1e-6
+    # FGMRES on an FP64 plan (inner operator FP64) converges as well
+    d2, it2, iit2, rr2, _ = f64.solve_fgmres(rhs, restart=20, tol=1e-8, inner_restart=50)
+    assert rr2 < 1e-8 and rel(d2, d64) < 1e-6

@@ -103,6 +103,9 @@ def test_error_paths():
     # too many levels: a patch's singular points leave the leaf neighbourhood
     with pytest.raises(I.IFGFError, match="neighbourhood"):
         I.ForwardMap(I.circle(), lims, 10.0, 9, host_only=True)
+    # mixed precision needs the stored operators
+    with pytest.raises(I.IFGFError, match="mixed precision needs the stored operators"):
+        I.ForwardMap(I.circle(), lims, 10.0, 3, precision="mixed", on_the_fly=True, host_only=True)
     # compute on a host-only plan fails loudly (no CPU fallback)
     fm = I.ForwardMap(I.circle(), lims, 10.0, 3, host_only=True)
     with pytest.raises(I.IFGFError, match="host-only"):
