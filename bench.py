@@ -10,7 +10,8 @@ HBM. The near-field kernel values and the folded leaf operator depend only on
 the geometry and are evaluated once at plan creation (counted in
 plan_setup_s); the on-the-fly variant that re-evaluates them in every
 application is timed too (on_the_fly_matvecs_per_s). After the timed steps, one GMRES(50) solve to 1e-8 for the plane wave
-d = (cos pi/4, sin pi/4) gives the time to solve.
+d = (cos pi/4, sin pi/4) gives the time to solve. The mixed-precision operator
+(FP32 cone work) and its FGMRES solve with FP64 refinement are timed beside it.
 
 Prints ONE JSON line on stdout; progress goes to stderr.
 """
@@ -78,6 +79,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-on-the-fly", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 20 device copies of 256 MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -205,6 +207,7 @@ def main():
         roofline["traffic"] = json.load(open(traffic_file)).get("bytes_per_launch")
 
     solve = None
+    rhs = dens = None
     if not args.no_solve:
         xg, yg, *_ = fm.points()
         rhs = -configs.plane_wave(xg, yg, kw["k"], d)
@@ -215,6 +218,36 @@ def main():
         solve = {"time_to_solve_s": round(t_solve, 4), "gmres_iterations": it,
                  "true_relres": relres, "restart": 50, "tol": 1e-8}
         log(f"GMRES: {it} iterations, {t_solve:.3f} s, true relres {relres:.2e}")
+
+    # mixed precision: FP32 leaf operator, cone coefficients, propagation and
+    # interpolation (FP64 near field, correction, accumulation), and the
+    # FP64-outer FGMRES that recovers the 1e-8 FP64 true residual
+    mixed = None
+    if not args.no_mixed:
+        fmx = I.ForwardMap(**dict(kw, precision="mixed"), autotune=True)
+        for _ in range(args.warmup):
+            fmx.apply_device(xs.data_ptr(), ys.data_ptr())
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            fmx.apply_device(xs.data_ptr(), ys.data_ptr())
+        torch.cuda.synchronize()
+        mv = args.steps / (time.perf_counter() - t0)
+        ym = fmx.compute_Ax_acc(x_host)
+        y64 = fmx.compute_Ax_acc(x_host, fp64=True)
+        mixed = {"matvecs_per_s": round(mv, 3),
+                 "rel_diff_vs_fp64_operator": float(np.linalg.norm(ym - y64) / np.linalg.norm(y64))}
+        log(f"mixed precision: {mv:.1f} matvecs/s, operator rel diff vs FP64 {mixed['rel_diff_vs_fp64_operator']:.2e}")
+        if not args.no_solve:
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            dm, oit, iit, rr, _ = fmx.solve_fgmres(rhs, restart=20, tol=1e-8, inner_restart=60)
+            t_f = time.perf_counter() - t0
+            mixed["fgmres"] = {"time_to_solve_s": round(t_f, 4), "outer_iterations": oit,
+                               "inner_iterations": iit, "true_relres": rr,
+                               "density_rel_diff_vs_fp64_gmres": float(np.linalg.norm(dm - dens) / np.linalg.norm(dens))}
+            log(f"mixed FGMRES: {oit} outer + {iit} inner iterations, {t_f:.4f} s, true relres {rr:.2e}")
+        del fmx
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(kw, x_host)
 
@@ -234,6 +267,7 @@ def main():
         "autotuned_lanes": ext["lanes"],
         "on_the_fly_matvecs_per_s": otf,
         "solve": solve,
+        "mixed_precision": mixed,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

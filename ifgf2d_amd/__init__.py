@@ -23,7 +23,7 @@ import os
 import numpy as np
 
 __all__ = [
-    "ForwardMap", "LinearOperator", "gmres", "hankel01", "IFGFError",
+    "ForwardMap", "LinearOperator", "gmres", "hankel01", "morton_sort", "IFGFError",
     "circle", "kite", "star", "equal_parameter_lims", "SINGLE_LAYER", "DOUBLE_LAYER", "COMBINED",
 ]
 
@@ -39,6 +39,7 @@ _PREC = {"fp64": FP64, "mixed": MIXED, FP64: FP64, MIXED: MIXED}
 IFGF_OK, IFGF_ERR_NOT_CONVERGED = 0, 6
 IFGF_PLAN_HOST_ONLY, IFGF_PLAN_ON_THE_FLY, IFGF_PLAN_AUTOTUNE, IFGF_PTR_DEVICE = 1, 2, 4, 1
 IFGF_APPLY_FP64 = 2
+IFGF_PLAN_DEVICE_SORT = 8
 
 
 class IFGFError(RuntimeError):
@@ -68,6 +69,7 @@ EXPORTS = [
     "ifgf_plan_points", "ifgf_plan_stats", "ifgf_plan_sorting", "ifgf_plan_level_cones",
     "ifgf_plan_correction", "ifgf_apply", "ifgf_apply_direct", "ifgf_propagate", "ifgf_gmres",
     "ifgf_fgmres", "ifgf_gmres_host", "ifgf_hankel01", "ifgf_plan_profile", "ifgf_plan_stage_times",
+    "ifgf_morton_sort",
 ]
 
 _LIB = None
@@ -108,6 +110,7 @@ def lib():
         L.ifgf_gmres_host.argtypes = [APPLY_FN, ctypes.c_void_p, ctypes.c_longlong, _dp, _dp,
                                       ctypes.c_int, ctypes.c_double, ctypes.c_int, _ip, _dp]
         L.ifgf_hankel01.argtypes = [_dp, ctypes.c_longlong, _dp, _dp]
+        L.ifgf_morton_sort.argtypes = [_dp, _dp, ctypes.c_longlong, ctypes.c_int, _lp, _lp]
         L.ifgf_plan_profile.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.ifgf_plan_stage_times.argtypes = [ctypes.c_void_p, _dp, _lp]
         _LIB = L
@@ -177,11 +180,15 @@ class ForwardMap:
     on_the_fly: re-evaluate the near-field and leaf kernels in every application
     instead of storing them once at plan creation (less memory, slower).
     autotune: time the candidate lane counts of the stored-path kernels once.
+    precision: "fp64", or "mixed" (FP32 leaf operator, cone coefficients, propagation and
+    interpolation; the FP64 operator stays available through compute_Ax_acc(x, fp64=True)).
+    device_sort: Morton box assignment and point sort on the GPU (checked against the
+    host's stable sort).
     """
 
     def __init__(self, curve, patch_lims, k, nlevels, Np=10, formulation="combined", eta=None,
                  delta=None, p=4.0, Ps=5, Pang=5, precision="fp64", nrhs_max=1, host_only=False,
-                 on_the_fly=False, autotune=False):
+                 on_the_fly=False, autotune=False, device_sort=False):
         L = lib()
         self.curve = curve
         self.lims = np.ascontiguousarray(patch_lims, dtype=np.float64)
@@ -193,7 +200,7 @@ class ForwardMap:
         surf = _Surface(curve[0], _d(self._cp), self._cp.size, _d(self.lims), self.lims.size - 1,
                         Np, self.delta, float(p))
         flags = ((IFGF_PLAN_HOST_ONLY if host_only else 0) | (IFGF_PLAN_ON_THE_FLY if on_the_fly else 0)
-                 | (IFGF_PLAN_AUTOTUNE if autotune else 0))
+                 | (IFGF_PLAN_AUTOTUNE if autotune else 0) | (IFGF_PLAN_DEVICE_SORT if device_sort else 0))
         prm = _Params(self.formulation, _PREC[precision], self.k, self.eta, Ps, Pang, nlevels,
                       nrhs_max, flags)
         h = ctypes.c_void_p()
@@ -393,6 +400,18 @@ def gmres(op, rhs, restart=50, tol=1e-8, maxit=300, x0=None):
     if rc not in (IFGF_OK, IFGF_ERR_NOT_CONVERGED):
         raise IFGFError(lib().ifgf_last_error().decode())
     return x, it.value, err.value, rc == IFGF_OK
+
+
+def morton_sort(x, y, nlevels):
+    """BoxTree::SortBox on the GPU: (sorting, leaf codes) of the points for an nlevels tree."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    s = np.empty(x.size, dtype=np.int64)
+    c = np.empty(x.size, dtype=np.int64)
+    rc = lib().ifgf_morton_sort(_d(x), _d(y), x.size, nlevels, s.ctypes.data_as(_lp), c.ctypes.data_as(_lp))
+    if rc != IFGF_OK:
+        raise IFGFError(f"ifgf_morton_sort failed ({rc}): {lib().ifgf_last_error().decode()}")
+    return s, c
 
 
 def hankel01(x):

@@ -101,8 +101,14 @@ int ifgf_plan_create(const ifgf_surface* s, const ifgf_params* prm, ifgf_plan** 
       return fail(nullptr, IFGF_ERR_GEOMETRY, err);
     if (!build_correction(P->geo, P->curve, prm->formulation, prm->k, prm->eta, s->rp_p, P->corr, err))
       return fail(nullptr, IFGF_ERR_GEOMETRY, err);
+    std::vector<int> dperm;
+    if ((prm->flags & IFGF_PLAN_DEVICE_SORT) && !(prm->flags & IFGF_PLAN_HOST_ONLY)) {
+      if (prm->nlevels > 16) return fail(nullptr, IFGF_ERR_ARG, "nlevels above 16 (leaf codes are 32-bit)");
+      if (!device_morton_sort(P->geo.x.data(), P->geo.y.data(), P->geo.N, prm->nlevels, dperm, nullptr, nullptr))
+        return fail(nullptr, IFGF_ERR_GEOMETRY, "The 2D box index cannot be < 0 or larger than the number of boxes");
+    }
     if (!build_tree(P->geo.x.data(), P->geo.y.data(), P->geo.nx.data(), P->geo.ny.data(), P->geo.N,
-                    prm->nlevels, prm->k, prm->Ps, prm->Pang, P->tree, err))
+                    prm->nlevels, prm->k, prm->Ps, prm->Pang, P->tree, err, dperm.empty() ? nullptr : &dperm))
       return fail(nullptr, IFGF_ERR_GEOMETRY, err);
     if (!build_exclusions(P->geo, P->tree, P->patch_sorted, P->excl, P->n_near, err))
       return fail(nullptr, IFGF_ERR_GEOMETRY, err);

@@ -129,5 +129,9 @@ namespace ifgf {
 // all-FP64 operator on a mixed-precision plan
 int apply_device(ifgf_plan* P, const zd* x, zd* y, hipStream_t st, bool fp64 = false);
 int plan_fail(ifgf_plan* p, int code, const std::string& msg);
+// Leaf Morton codes and their stable sort on the GPU (morton.hip); false when
+// a point lies outside the leaf grid.
+bool device_morton_sort(const double* x, const double* y, long long n, int nlevels, std::vector<int>& perm,
+                        std::vector<long long>* codes, hipStream_t st);
 
 }  // namespace ifgf

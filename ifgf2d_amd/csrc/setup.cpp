@@ -469,7 +469,7 @@ struct LevelGeom {
 }  // namespace
 
 bool build_tree(const double* x, const double* y, const double* nx, const double* ny, long long N,
-                int nlev, double k, int Ps, int Pang, Tree& t, std::string& err) {
+                int nlev, double k, int Ps, int Pang, Tree& t, std::string& err, const std::vector<int>* perm_in) {
   if (nlev < 3) {
     err = "IFGF accelerator requires at least 3 levels.";
     return false;
@@ -513,8 +513,27 @@ bool build_tree(const double* x, const double* y, const double* nx, const double
     }
   }
   t.perm.resize(N);
-  std::iota(t.perm.begin(), t.perm.end(), 0);
-  std::stable_sort(t.perm.begin(), t.perm.end(), [&](int a, int b) { return code[a] < code[b]; });
+  if (perm_in) {
+    if ((long long)perm_in->size() != N) {
+      err = "device Morton permutation has the wrong length";
+      return false;
+    }
+    std::vector<char> seen(N, 0);
+    for (long long i = 0; i < N; i++) {
+      const int o = (*perm_in)[i];
+      const bool ordered = i == 0 || code[(*perm_in)[i - 1]] < code[o] ||
+                           (code[(*perm_in)[i - 1]] == code[o] && (*perm_in)[i - 1] < o);
+      if (o < 0 || o >= N || seen[o] || !ordered) {
+        err = "device Morton permutation differs from the stable sort of the leaf codes";
+        return false;
+      }
+      seen[o] = 1;
+    }
+    t.perm = *perm_in;
+  } else {
+    std::iota(t.perm.begin(), t.perm.end(), 0);
+    std::stable_sort(t.perm.begin(), t.perm.end(), [&](int a, int b) { return code[a] < code[b]; });
+  }
   t.inv.resize(N);
   t.sx.resize(N);
   t.sy.resize(N);

@@ -92,8 +92,12 @@ struct Tree {
   std::vector<int> point_box;                  // sorted point x level -> local box (N x (D+1))
   long long n_leaf = 0, n_prop = 0, n_interp = 0, n_cones = 0;
 };
+// perm_in (optional): the Morton permutation computed elsewhere (the device
+// sort); it is checked to be a permutation that orders the leaf codes with
+// ties in input order, i.e. exactly the host's stable sort.
 bool build_tree(const double* x, const double* y, const double* nx, const double* ny, long long N,
-                int nlev, double k, int Ps, int Pang, Tree& t, std::string& err);
+                int nlev, double k, int Ps, int Pang, Tree& t, std::string& err,
+                const std::vector<int>* perm_in = nullptr);
 
 // Near-field exclusion lists (init_sort_sing_point, ForwardMap.tpp:567-607) in
 // sorted order: own patch + patches the point is near-singular to, padded

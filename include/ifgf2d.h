@@ -55,6 +55,9 @@ extern "C" {
                                   streamed from HBM by every application) */
 #define IFGF_PLAN_AUTOTUNE 4   /* time the candidate lane counts of the stored-path
                                   kernels on this plan and keep the fastest */
+#define IFGF_PLAN_DEVICE_SORT 8 /* Morton box assignment and the stable sort of the points
+                                   on the GPU (ifgf_morton_sort) instead of the host; the
+                                   permutation is checked to equal the host's stable sort */
 
 /* apply flags */
 #define IFGF_PTR_DEVICE 1     /* x / y are device pointers; the call is asynchronous on stream */
@@ -119,6 +122,17 @@ int ifgf_plan_points(const ifgf_plan* plan, double* x, double* y, double* nx, do
  * evaluations per block of the per-level propagation/interpolation kernel,
  * [15] lanes per target of the correction kernel. */
 int ifgf_plan_stats(const ifgf_plan* plan, long long out[16]);
+
+/* Replaces BoxTree::SortBox (BoxTree.h:73-106) with ComputeBB's bounding box
+ * (BoxTree.h:179-202) and Level::Point2Morton (Level.h:145-161), on the GPU:
+ * the leaf boxes of an nlevels tree (leaf grid 2^(nlevels-1) per side,
+ * box index floor((x - min) / box size), x in the low bit of the interleaved
+ * code) and the stable sort of the points by code. sorting[i] = input index
+ * of the i-th point in Morton order (BoxTree::sorting_); codes (may be NULL)
+ * = leaf code of every input point. Host pointers. nlevels in [3, 16].
+ * Errors: IFGF_ERR_LEVELS, IFGF_ERR_GEOMETRY (a point outside the leaf grid). */
+int ifgf_morton_sort(const double* x, const double* y, long long n, int nlevels, long long* sorting,
+                     long long* codes);
 
 /* BoxTree::getSorting() (BoxTree.h:1163-1165): Morton position -> patch-order index. */
 int ifgf_plan_sorting(const ifgf_plan* plan, long long* sorting);

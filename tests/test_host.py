@@ -106,6 +106,11 @@ def test_error_paths():
     # mixed precision needs the stored operators
     with pytest.raises(I.IFGFError, match="mixed precision needs the stored operators"):
         I.ForwardMap(I.circle(), lims, 10.0, 3, precision="mixed", on_the_fly=True, host_only=True)
+    # the device sort validates its arguments before touching the GPU
+    with pytest.raises(I.IFGFError, match="at least 3 levels"):
+        I.morton_sort(np.zeros(4), np.zeros(4), 2)
+    with pytest.raises(I.IFGFError, match="above 16"):
+        I.morton_sort(np.zeros(4), np.zeros(4), 17)
     # compute on a host-only plan fails loudly (no CPU fallback)
     fm = I.ForwardMap(I.circle(), lims, 10.0, 3, host_only=True)
     with pytest.raises(I.IFGFError, match="host-only"):
