@@ -40,6 +40,7 @@ IFGF_OK, IFGF_ERR_NOT_CONVERGED = 0, 6
 IFGF_PLAN_HOST_ONLY, IFGF_PLAN_ON_THE_FLY, IFGF_PLAN_AUTOTUNE, IFGF_PTR_DEVICE = 1, 2, 4, 1
 IFGF_APPLY_FP64 = 2
 IFGF_PLAN_DEVICE_SORT = 8
+IFGF_PLAN_OVERLAP = 16
 
 
 class IFGFError(RuntimeError):
@@ -69,7 +70,7 @@ EXPORTS = [
     "ifgf_plan_points", "ifgf_plan_stats", "ifgf_plan_sorting", "ifgf_plan_level_cones",
     "ifgf_plan_correction", "ifgf_apply", "ifgf_apply_direct", "ifgf_propagate", "ifgf_gmres",
     "ifgf_fgmres", "ifgf_gmres_host", "ifgf_hankel01", "ifgf_plan_profile", "ifgf_plan_stage_times",
-    "ifgf_morton_sort",
+    "ifgf_morton_sort", "ifgf_apply_graph",
 ]
 
 _LIB = None
@@ -100,6 +101,8 @@ def lib():
         L.ifgf_plan_correction.restype = ctypes.c_longlong
         L.ifgf_apply.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
                                  ctypes.c_longlong, ctypes.c_int, ctypes.c_void_p]
+        L.ifgf_apply_graph.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                       ctypes.c_longlong, ctypes.c_int, ctypes.c_void_p]
         L.ifgf_apply_direct.argtypes = [ctypes.c_void_p, _dp, _lp, ctypes.c_longlong, _dp]
         L.ifgf_propagate.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_longlong, _dp, _dp]
         L.ifgf_gmres.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int, ctypes.c_double,
@@ -184,11 +187,13 @@ class ForwardMap:
     interpolation; the FP64 operator stays available through compute_Ax_acc(x, fp64=True)).
     device_sort: Morton box assignment and point sort on the GPU (checked against the
     host's stable sort).
+    overlap: run the stored near field on a second stream beside the leaf operator and the
+    level pushes (the autotuner also tries it).
     """
 
     def __init__(self, curve, patch_lims, k, nlevels, Np=10, formulation="combined", eta=None,
                  delta=None, p=4.0, Ps=5, Pang=5, precision="fp64", nrhs_max=1, host_only=False,
-                 on_the_fly=False, autotune=False, device_sort=False):
+                 on_the_fly=False, autotune=False, device_sort=False, overlap=False):
         L = lib()
         self.curve = curve
         self.lims = np.ascontiguousarray(patch_lims, dtype=np.float64)
@@ -200,7 +205,8 @@ class ForwardMap:
         surf = _Surface(curve[0], _d(self._cp), self._cp.size, _d(self.lims), self.lims.size - 1,
                         Np, self.delta, float(p))
         flags = ((IFGF_PLAN_HOST_ONLY if host_only else 0) | (IFGF_PLAN_ON_THE_FLY if on_the_fly else 0)
-                 | (IFGF_PLAN_AUTOTUNE if autotune else 0) | (IFGF_PLAN_DEVICE_SORT if device_sort else 0))
+                 | (IFGF_PLAN_AUTOTUNE if autotune else 0) | (IFGF_PLAN_DEVICE_SORT if device_sort else 0)
+                 | (IFGF_PLAN_OVERLAP if overlap else 0))
         prm = _Params(self.formulation, _PREC[precision], self.k, self.eta, Ps, Pang, nlevels,
                       nrhs_max, flags)
         h = ctypes.c_void_p()
@@ -234,7 +240,7 @@ class ForwardMap:
         return x, y, nx, ny, w
 
     def _stats(self):
-        s = np.empty(16, dtype=np.int64)
+        s = np.empty(20, dtype=np.int64)
         self._check(lib().ifgf_plan_stats(self._h, s.ctypes.data_as(_lp)), "stats")
         return s.tolist()
 
@@ -247,7 +253,9 @@ class ForwardMap:
         s = self._stats()
         return {"leaf_cones": s[8], "prop_cones": s[4] - s[8], "stored": bool(s[9]),
                 "near_stored": s[10], "stored_bytes": s[11],
-                "lanes": {"near": s[12], "leaf": s[13], "push_task_entries": s[14], "finalize": s[15]}}
+                "lanes": {"near": s[12], "leaf": s[13], "push_task_entries": s[14], "finalize": s[15],
+                          "near_on_second_stream": bool(s[16])},
+                "mixed": bool(s[17])}
 
     def sorting(self):
         s = np.empty(self.N, dtype=np.int64)
@@ -290,6 +298,13 @@ class ForwardMap:
         self._check(lib().ifgf_apply(self._h, ctypes.c_void_p(x_ptr), ctypes.c_void_p(y_ptr), nrhs,
                                      self.N if ld is None else ld, flags,
                                      ctypes.c_void_p(stream) if stream else None), "ifgf_apply")
+
+    def apply_graph(self, x_ptr, y_ptr, nrhs=1, ld=None, stream=None, fp64=False):
+        """apply_device replayed as one captured HIP graph (captured on the first call
+        with these pointers and shape)."""
+        self._check(lib().ifgf_apply_graph(self._h, ctypes.c_void_p(x_ptr), ctypes.c_void_p(y_ptr), nrhs,
+                                           self.N if ld is None else ld, IFGF_APPLY_FP64 if fp64 else 0,
+                                           ctypes.c_void_p(stream) if stream else None), "ifgf_apply_graph")
 
     def compute_Ax_unacc(self, x, targets):
         """Unaccelerated operator (direct sum + correction) at a target subset."""

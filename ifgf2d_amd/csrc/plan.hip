@@ -35,6 +35,9 @@ static void build_device_side(ifgf_plan* P) {
   const long long N = t.N;
   const int Pn = t.Ps * t.Pang;
   HIP_TRY(hipStreamCreate(&P->stream));
+  HIP_TRY(hipStreamCreate(&P->aux));
+  HIP_TRY(hipEventCreateWithFlags(&P->ev_fork, hipEventDisableTiming));
+  HIP_TRY(hipEventCreateWithFlags(&P->ev_join, hipEventDisableTiming));
   P->px.upload(t.sx);
   P->py.upload(t.sy);
   P->pnx.upload(t.snx);
@@ -154,6 +157,8 @@ int ifgf_plan_create(const ifgf_surface* s, const ifgf_params* prm, ifgf_plan** 
       build_device_side(P.get());
       if (!(prm->flags & IFGF_PLAN_ON_THE_FLY)) {
         if (!build_stored_any(P.get())) return fail(nullptr, IFGF_ERR_ARG, "no kernel instantiation for this (Ps, Pang)");
+        P->overlap = (prm->flags & IFGF_PLAN_OVERLAP) != 0;
+        if (prm->flags & IFGF_PLAN_AUTOTUNE) autotune(P.get());
       }
     }
   } catch (std::exception& e) {
@@ -178,7 +183,7 @@ int ifgf_plan_points(const ifgf_plan* p, double* x, double* y, double* nx, doubl
   return IFGF_OK;
 }
 
-int ifgf_plan_stats(const ifgf_plan* p, long long out[16]) {
+int ifgf_plan_stats(const ifgf_plan* p, long long out[20]) {
   if (!p || !out) return fail(nullptr, IFGF_ERR_ARG, "null argument");
   out[0] = p->n_near;
   out[1] = p->tree.n_leaf;
@@ -196,6 +201,10 @@ int ifgf_plan_stats(const ifgf_plan* p, long long out[16]) {
   out[13] = p->leaf_lanes;
   out[14] = p->push_emax;
   out[15] = p->final_lanes;
+  out[16] = p->overlap ? 1 : 0;
+  out[17] = p->mixed ? 1 : 0;
+  out[18] = 0;
+  out[19] = 0;
   return IFGF_OK;
 }
 
@@ -324,9 +333,16 @@ static void launch_ifgf(ifgf_plan* P, const zd* x, zd* y, hipStream_t st) {
   hipLaunchKernelGGL(k_gather, dim3(nb), dim3(256), 0, st, N, x, P->w.p, P->perm.p, P->rho.p);
   mark(1);
 
-  // near field
+  // near field (stored path with overlap: on the second stream, joined
+  // before the finalize; not while stage timing is recorded)
+  const bool fork = P->stored && P->overlap && !prof;
   mark(2);
-  if (P->stored) {
+  if (fork) {
+    HIP_TRY(hipEventRecord(P->ev_fork, st));
+    HIP_TRY(hipStreamWaitEvent(P->aux, P->ev_fork, 0));
+    launch_near_stored(P, P->aux);
+    HIP_TRY(hipEventRecord(P->ev_join, P->aux));
+  } else if (P->stored) {
     launch_near_stored(P, st);
   } else {
     hipLaunchKernelGGL((k_near<F, kNearLanes>), dim3((N + 256 / kNearLanes - 1) / (256 / kNearLanes)), dim3(256), 0,
@@ -377,6 +393,7 @@ static void launch_ifgf(ifgf_plan* P, const zd* x, zd* y, hipStream_t st) {
   mark(9);
 
   // correction, identity and unsort
+  if (fork) HIP_TRY(hipStreamWaitEvent(st, P->ev_join, 0));
   mark(10);
   const double ident = (F == kSingleLayer) ? 0.0 : 0.5;
   const Z* ient = nullptr;
@@ -749,7 +766,8 @@ static void autotune(ifgf_plan* P) {
     int* v;
     std::vector<int> cand;
   };
-  Knob knobs[] = {{&P->final_lanes, {4, 8, 16}}};
+  int ov = P->overlap ? 1 : 0;
+  Knob knobs[] = {{&P->final_lanes, {4, 8, 16}}, {&ov, {0, 1}}};
   // entries per level-push task: fewer give more, shorter blocks
   if (P->stored) {
     int best = P->push_emax;
@@ -769,6 +787,7 @@ static void autotune(ifgf_plan* P) {
     float tbest = 1e30f;
     for (int c : kb.cand) {
       *kb.v = c;
+      P->overlap = ov != 0;
       const float t = time_apply();
       if (t < tbest) {
         tbest = t;
@@ -777,6 +796,7 @@ static void autotune(ifgf_plan* P) {
     }
     *kb.v = best;
   }
+  P->overlap = ov != 0;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
 }
@@ -828,6 +848,47 @@ int ifgf_apply(ifgf_plan* P, const double* x, double* y, int nrhs, long long ld,
         HIP_TRY(hipStreamSynchronize(st));
       }
     }
+  } catch (std::exception& e) {
+    return fail(P, IFGF_ERR_HIP, e.what());
+  }
+  return IFGF_OK;
+}
+
+int ifgf_apply_graph(ifgf_plan* P, const double* x, double* y, int nrhs, long long ld, int flags, void* stream) {
+  if (!P || !x || !y) return fail(P, IFGF_ERR_ARG, "null argument");
+  if (!P->on_device) return fail(P, IFGF_ERR_NO_DEVICE, "plan was built host-only");
+  if (nrhs < 1 || ld < P->geo.N) return fail(P, IFGF_ERR_ARG, "nrhs < 1 or ld < n");
+  if (P->profile) return fail(P, IFGF_ERR_ARG, "stage profiling is on (its events cannot be captured)");
+  const int f = flags & IFGF_APPLY_FP64;
+  try {
+    if (!P->gexec || P->g_x != x || P->g_y != y || P->g_nrhs != nrhs || P->g_ld != ld || P->g_flags != f) {
+      if (P->gexec) HIP_TRY(hipGraphExecDestroy(P->gexec));
+      if (P->graph) HIP_TRY(hipGraphDestroy(P->graph));
+      P->gexec = nullptr;
+      P->graph = nullptr;
+      // capture on the plan's stream: every application launch of nrhs
+      // columns (kernels, and the fork / join events of the overlap path)
+      HIP_TRY(hipStreamSynchronize(P->stream));
+      HIP_TRY(hipStreamBeginCapture(P->stream, hipStreamCaptureModeThreadLocal));
+      int rc = IFGF_OK;
+      for (int r = 0; r < nrhs && rc == IFGF_OK; r++)
+        rc = apply_device(P, (const zd*)x + (size_t)r * ld, (zd*)y + (size_t)r * ld, P->stream, f != 0);
+      hipGraph_t g = nullptr;
+      const hipError_t ec = hipStreamEndCapture(P->stream, &g);
+      if (rc != IFGF_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+      HIP_TRY(ec);
+      P->graph = g;
+      HIP_TRY(hipGraphInstantiate(&P->gexec, g, nullptr, nullptr, 0));
+      P->g_x = x;
+      P->g_y = y;
+      P->g_nrhs = nrhs;
+      P->g_ld = ld;
+      P->g_flags = f;
+    }
+    HIP_TRY(hipGraphLaunch(P->gexec, stream ? (hipStream_t)stream : P->stream));
   } catch (std::exception& e) {
     return fail(P, IFGF_ERR_HIP, e.what());
   }

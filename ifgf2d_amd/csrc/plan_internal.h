@@ -64,6 +64,19 @@ struct ifgf_plan {
   ifgf::ChebTables cheb{};
   bool on_device = false;
   hipStream_t stream = nullptr;
+  // stored path: the near field on a second stream beside the leaf operator
+  // and the level-push chain (fork after the gather, join before finalize);
+  // chosen by the autotuner
+  bool overlap = false;
+  hipStream_t aux = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  // ifgf_apply_graph: one captured application sequence, keyed by its arguments
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  const void* g_x = nullptr;
+  void* g_y = nullptr;
+  int g_nrhs = 0, g_flags = 0;
+  long long g_ld = 0;
   // device tables
   ifgf::DevBuf<double> px, py, pnx, pny, w;
   ifgf::DevBuf<int> patch, perm, inv, leaf_of_point, leaf_begin, nb_ptr, nb_list;
@@ -118,6 +131,11 @@ struct ifgf_plan {
   ~ifgf_plan() {
     for (auto& e : ev_pool)
       if (e) (void)hipEventDestroy(e);
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
+    if (aux) (void)hipStreamDestroy(aux);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };

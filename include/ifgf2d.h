@@ -58,6 +58,8 @@ extern "C" {
 #define IFGF_PLAN_DEVICE_SORT 8 /* Morton box assignment and the stable sort of the points
                                    on the GPU (ifgf_morton_sort) instead of the host; the
                                    permutation is checked to equal the host's stable sort */
+#define IFGF_PLAN_OVERLAP 16   /* run the stored near field on a second stream beside the leaf
+                                  operator and the level pushes (IFGF_PLAN_AUTOTUNE also tries it) */
 
 /* apply flags */
 #define IFGF_PTR_DEVICE 1     /* x / y are device pointers; the call is asynchronous on stream */
@@ -120,8 +122,10 @@ int ifgf_plan_points(const ifgf_plan* plan, double* x, double* y, double* nx, do
  * included as zeros), [11] bytes of the stored operators, [12] and [13] lanes
  * per target / leaf-cone node of the near-field and leaf kernels, [14]
  * evaluations per block of the per-level propagation/interpolation kernel,
- * [15] lanes per target of the correction kernel. */
-int ifgf_plan_stats(const ifgf_plan* plan, long long out[16]);
+ * [15] lanes per target of the correction kernel, [16] 1 if the near field runs on a
+ * second stream beside the leaf / level-push chain, [17] 1 on a mixed-precision plan,
+ * [18..19] 0. */
+int ifgf_plan_stats(const ifgf_plan* plan, long long out[20]);
 
 /* Replaces BoxTree::SortBox (BoxTree.h:73-106) with ComputeBB's bounding box
  * (BoxTree.h:179-202) and Level::Point2Morton (Level.h:145-161), on the GPU:
@@ -160,6 +164,17 @@ long long ifgf_plan_correction(const ifgf_plan* plan, long long* ptr, int* patch
  * reentrant (like BoxTree's mutable buffers, BoxTree.h:48-51). */
 int ifgf_apply(ifgf_plan* plan, const double* x, double* y, int nrhs, long long ld, int flags,
                void* stream);
+
+/* ifgf_apply with IFGF_PTR_DEVICE as one HIP graph (no reference counterpart;
+ * for callers that replay the same application many times, e.g. a solver
+ * loop over fixed Krylov buffers). The first call with a given (x, y, nrhs,
+ * ld, IFGF_APPLY_FP64) captures the application's launches on the plan's
+ * stream into a graph; every call launches that graph on stream (NULL = the
+ * plan's stream), asynchronously. x / y are device pointers whose contents
+ * may change between calls; a call with other pointers recaptures. Fails
+ * while stage profiling is on. */
+int ifgf_apply_graph(ifgf_plan* plan, const double* x, double* y, int nrhs, long long ld, int flags,
+                     void* stream);
 
 /* Unaccelerated operator on a target subset: replaces compute_Ax_unacc
  * (ForwardMap.tpp:438-540) evaluated at targets[0..nt). Host pointers. */

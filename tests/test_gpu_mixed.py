@@ -105,3 +105,33 @@ def test_fgmres_kite_against_fp64_gmres():
     # FGMRES on an FP64 plan (inner operator FP64) converges as well
     d2, it2, iit2, rr2, _ = f64.solve_fgmres(rhs, restart=20, tol=1e-8, inner_restart=50)
     assert rr2 < 1e-8 and rel(d2, d64) < 1e-6
+
+
+def test_star_256k_fp64_and_mixed():
+    """Star r = 1 + 0.3 cos 5t at k = 500, 25,600 patches (N = 256,000), 12
+    levels, plane wave d = (1, 0): FP64 GMRES(50) and mixed-plan FGMRES both
+    reach the FP64 true residual 1e-8 and agree on the density; the mixed
+    operator stays within MIXED_TOL of the FP64 one; the IFGF error against
+    the direct sum on 2,000 targets is at the 5x5 level."""
+    kw, d = configs.star_k500()
+    f64 = I.ForwardMap(**kw)
+    print("star stats", f64.stats(), flush=True)
+    x = density(f64.N, 23)
+    y = f64.compute_Ax_acc(x)
+    tg = np.arange(0, f64.N, 128)
+    err = rel(y[tg], f64.compute_Ax_unacc(x, tg))
+    print("IFGF 5x5 vs direct on", tg.size, "targets", err, flush=True)
+    assert 1e-6 < err < 2e-3
+    xg, yg, *_ = f64.points()
+    rhs = -configs.plane_wave(xg, yg, kw["k"], d)
+    d64, it, rr, _ = f64.solve(rhs, restart=50, tol=1e-8, maxit=2000)
+    print("fp64 GMRES", it, rr, flush=True)
+    assert rr < 1e-8
+    del f64
+    fm = I.ForwardMap(**kw, precision="mixed")
+    e = rel(fm.compute_Ax_acc(x), y)
+    print("mixed vs fp64 operator", e, flush=True)
+    assert 0 < e < MIXED_TOL
+    dm, oit, iit, rrm, _ = fm.solve_fgmres(rhs, restart=20, tol=1e-8, inner_restart=60)
+    print("mixed FGMRES", oit, iit, rrm, "density diff", rel(dm, d64), flush=True)
+    assert rrm < 1e-8 and rel(dm, d64) < 1e-6
