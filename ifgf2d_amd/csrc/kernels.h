@@ -568,6 +568,7 @@ struct PushArgsT {
   const ChebTablesT<R>* tabs;
   Z* pbuf;               // child slots of every (cone, node), then interpolation entries
   int leaf;              // 1 at level D
+  int rs;                // batched kernels: columns per slot in pbuf (the widest block)
 };
 
 // EPT entries per thread: a task holds at most 256 * EPT entries, and every

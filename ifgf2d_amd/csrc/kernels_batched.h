@@ -1,0 +1,283 @@
+// Several right-hand sides per application (ifgf_apply with nrhs > 1 on a
+// stored-operator plan): the same stored near-field values, leaf operator and
+// push geometry as the single-column kernels in kernels.h, each loaded once
+// and applied to R columns. Densities and every intermediate are laid out
+// point-major with the R columns innermost (rho_b[i * R + r], partial sums
+// [item][..][r], slots [slot][r]), so a column block of one point is one
+// contiguous R x 16-byte (FP64) or R x 8-byte (FP32) segment.
+//
+// Each kernel performs, per column, the operations of its single-column
+// counterpart in the same order (the same summation order everywhere), so a
+// column of a batched application agrees with a single application of that
+// column to rounding (the compiler may fuse multiply-adds differently in the
+// two kernels), and batched applications are bitwise deterministic.
+#pragma once
+
+#include "kernels.h"
+
+namespace ifgf {
+
+// Cones per batched push task: two LDS windows of CB x P x R complex values
+// within 40 KB, and at most 256 threads in the cone phase.
+constexpr int batched_cones(int P, int R, int zbytes) {
+  const int c = 40960 / (2 * P * R * zbytes);
+  const int m = c < 256 / P ? c : 256 / P;
+  return m < 1 ? 1 : m;
+}
+
+// rho_b[i * R + r] = x[r * ld + perm[i]] * w[perm[i]]
+template <int R>
+__global__ void __launch_bounds__(256) k_gather_b(int N, const zd* __restrict__ x, long long ld,
+                                                  const double* __restrict__ w, const int* __restrict__ perm,
+                                                  zd* __restrict__ rho) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)N * R) return;
+  const int i = (int)(t / R), r = (int)(t % R);
+  const int o = perm[i];
+  rho[t] = cscale(x[(size_t)r * ld + o], w[o]);
+}
+
+// k_near_items with R columns: lane (t, sl) keeps R accumulators.
+template <int R>
+__global__ void __launch_bounds__(256)
+    k_near_items_b(int N, int nitems, const NearItem* __restrict__ items, const int4* __restrict__ nf_seg,
+                   const zd* __restrict__ nf_val, const zd* __restrict__ rho, zd* __restrict__ npart) {
+  const int w = xcd_block() * 4 + threadIdx.x / 64;
+  const int lane = threadIdx.x % 64;
+  const int t = lane / kNearChunk, sl = lane % kNearChunk;
+  zd acc[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) acc[r] = zd{0.0, 0.0};
+  if (w < nitems) {
+    const NearItem it = items[w];
+    const int i = it.g * kNearGroup + t;
+    if (i < N) {
+      const dbl2* __restrict__ v = reinterpret_cast<const dbl2*>(nf_val) + it.off + t * kNearChunk + sl;
+      dbl2 val[kNearItemChunks];
+#pragma unroll
+      for (int u = 0; u < kNearItemChunks; u++)
+        if (u < it.nc) val[u] = v[u * kNearGroup * kNearChunk];
+      const int4* sg = nf_seg + (size_t)i * (kNbSegInts / 4);
+      const int4 q0 = sg[0], q1 = sg[1], q2 = sg[2], q3 = sg[3], q4 = sg[4];
+      const int end[kMaxNb - 1] = {q0.x, q0.z, q1.x, q1.z, q2.x, q2.z, q3.x, q3.z};
+      const int sh[kMaxNb] = {q0.y, q0.w, q1.y, q1.w, q2.y, q2.w, q3.y, q3.w, q4.y};
+      const int len = q4.x;
+#pragma unroll
+      for (int u = 0; u < kNearItemChunks; u++) {
+        const int o = (it.u0 + u) * kNearChunk + sl;
+        if (u < it.nc && o < len) {
+          int src = o + sh[0];
+#pragma unroll
+          for (int q = 1; q < kMaxNb; q++) src = (o >= end[q - 1]) ? o + sh[q] : src;
+          const dbl2* __restrict__ rr = reinterpret_cast<const dbl2*>(rho) + (size_t)src * R;
+#pragma unroll
+          for (int r = 0; r < R; r++) {
+            const dbl2 x = rr[r];
+            cfma(acc[r], zd{val[u].x, val[u].y}, zd{x.x, x.y});
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < R; r++) acc[r] = lane_group_sum<kNearChunk>(acc[r]);
+  if (w < nitems && sl == 0) {
+#pragma unroll
+    for (int r = 0; r < R; r++) npart[((size_t)w * kNearGroup + t) * R + r] = acc[r];
+  }
+}
+
+// k_leaf_items with R columns.
+template <int P, int R, typename Z>
+__global__ void __launch_bounds__(256)
+    k_leaf_items_b(int nitems, const LeafItem* __restrict__ items, const Z* __restrict__ lf, const zd* __restrict__ rho,
+                   Z* __restrict__ lpart) {
+  constexpr int S = kLeafChunk;
+  using V2 = typename vec2_of<Z>::type;
+  const long long gt = (long long)xcd_block() * blockDim.x + threadIdx.x;
+  const long long item = gt / (P * S);
+  const int rm = (int)(gt % (P * S));
+  const int m = rm / S, sl = rm % S;
+  Z acc[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) acc[r] = Z{0, 0};
+  if (item < nitems) {
+    const LeafItem it = items[item];
+    const V2* __restrict__ L = reinterpret_cast<const V2*>(lf) + it.off + m * S + sl;
+    V2 val[kLeafItemTiles];
+#pragma unroll
+    for (int u = 0; u < kLeafItemTiles; u++)
+      if (u * S + sl < it.n) val[u] = L[u * S * P];
+#pragma unroll
+    for (int u = 0; u < kLeafItemTiles; u++) {
+      if (u * S + sl < it.n) {
+        const zd* __restrict__ rr = rho + (size_t)(it.s0 + u * S + sl) * R;
+#pragma unroll
+        for (int r = 0; r < R; r++) zfma(acc[r], Z{val[u].x, val[u].y}, zcast<Z>(rr[r]));
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    acc[r].x = lane_sum<S>(acc[r].x);
+    acc[r].y = lane_sum<S>(acc[r].y);
+  }
+  if (item < nitems && sl == 0) {
+#pragma unroll
+    for (int r = 0; r < R; r++) lpart[(item * P + m) * R + r] = acc[r];
+  }
+}
+
+// k_level_push with R columns. Tasks hold at most CB cones (their R-column
+// coefficients fit the LDS windows) and 256 entries (one per thread); the
+// cone phase runs on threads (cone, node) < CB * P. Every slot of pbuf holds
+// a.rs >= R columns whatever the block width, so the child slots no entry
+// writes (parents with fewer than four children) stay zero for every width.
+template <int Ps, int Pang, int R, int CB, typename RT>
+__global__ void __launch_bounds__(256) k_level_push_b(PushArgsT<RT> a) {
+  using Z = typename complex_of<RT>::type;
+  constexpr int P = Ps * Pang;
+  static_assert(CB * P <= 256, "cone phase needs CB * P threads");
+  __shared__ ChebTablesT<RT> T;
+  __shared__ Z buf[CB * P * R];
+  __shared__ Z tmp[CB * P * R];
+  const PushTask task = a.tasks[xcd_block()];
+  const int tid = threadIdx.x;
+  if (!a.leaf)
+    for (int q = tid; q < (int)(sizeof(ChebTablesT<RT>) / sizeof(RT)); q += blockDim.x)
+      ((RT*)&T)[q] = ((const RT*)a.tabs)[q];
+  const int cl = tid / P, j = tid % P;
+  const bool act = cl < task.nc && cl < CB;
+  const long long g = (long long)task.c0 + cl;
+  Geo4T<RT> G;
+  int2 d;
+  const bool has = tid < task.ne;
+  if (has) {
+    G = a.geo[task.e0 + tid];
+    d = a.dst[task.e0 + tid];
+  }
+  // cone inputs: column r of node j of cone cl at buf[(cl * P + j) * R + r]
+  if (act) {
+    if (a.leaf) {
+      const int c = (int)(g - a.leaf_cone_off);
+      const int w1 = a.lio[c + 1];
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        Z v{0, 0};
+        for (int w = a.lio[c]; w < w1; w++) v = zadd(v, a.lpart[((size_t)w * P + j) * R + r]);
+        buf[(cl * P + j) * R + r] = v;
+      }
+    } else {
+      const int rs = a.rs;
+      const Z* s = a.pbuf + (g * P + j) * 4 * rs;
+      const Z f = a.pn_ifac[g * P + j];
+#pragma unroll
+      for (int r = 0; r < R; r++)
+        buf[(cl * P + j) * R + r] = zmul(zadd(zadd(zadd(s[r], s[rs + r]), s[2 * rs + r]), s[3 * rs + r]), f);
+    }
+  }
+  __syncthreads();
+  if (!a.leaf) {
+    // separable transform per column (cone_transform's operations)
+    const int n = j / Ps, m = j % Ps;
+    if (act) {
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        Z q{0, 0};
+#pragma unroll
+        for (int l = 0; l < Pang; l++) {
+          const Z f = buf[(cl * P + l * Ps + m) * R + r];
+          q.x = fma(f.x, T.Ta[n * 16 + l], q.x);
+          q.y = fma(f.y, T.Ta[n * 16 + l], q.y);
+        }
+        tmp[(cl * P + j) * R + r] = q;
+      }
+    }
+    __syncthreads();
+    if (act) {
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        Z c{0, 0};
+#pragma unroll
+        for (int k = 0; k < Ps; k++) {
+          const Z v = tmp[(cl * P + n * Ps + k) * R + r];
+          c.x = fma(v.x, T.Ts[m * 16 + k], c.x);
+          c.y = fma(v.y, T.Ts[m * 16 + k], c.y);
+        }
+        buf[(cl * P + j) * R + r] = c;
+      }
+    }
+    __syncthreads();
+  }
+  if (has) {
+    // cone_eval's operations per column
+    RT ts[Ps], ta[Pang];
+    ts[0] = 1;
+    if (Ps > 1) ts[1] = G.s;
+#pragma unroll
+    for (int i = 2; i < Ps; i++) ts[i] = 2 * G.s * ts[i - 1] - ts[i - 2];
+    ta[0] = 1;
+    if (Pang > 1) ta[1] = G.t;
+#pragma unroll
+    for (int i = 2; i < Pang; i++) ta[i] = 2 * G.t * ta[i - 1] - ta[i - 2];
+    const Z* c = buf + (size_t)(d.y - task.c0) * P * R;
+    const Z f{G.fre, G.fim};
+    Z* out = a.pbuf + (size_t)d.x * a.rs;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      Z acc{0, 0};
+#pragma unroll
+      for (int aa = 0; aa < Pang; aa++) {
+        Z row{0, 0};
+#pragma unroll
+        for (int m = 0; m < Ps; m++) {
+          const Z v = c[(aa * Ps + m) * R + r];
+          row.x = fma(ts[m], v.x, row.x);
+          row.y = fma(ts[m], v.y, row.y);
+        }
+        acc.x = fma(ta[aa], row.x, acc.x);
+        acc.y = fma(ta[aa], row.y, acc.y);
+      }
+      out[r] = zmul(acc, f);
+    }
+  }
+}
+
+// k_finalize with R columns: lanes (target, column, slice) with S slices per
+// (target, column); y is column-major with stride ld; interpolation entry f
+// holds rs columns at ient[f * rs].
+template <int S, int R, typename ZI>
+__global__ void __launch_bounds__(256)
+    k_finalize_b(int N, int Np, double ident, const zd* __restrict__ x, long long ld,
+                 const long long* __restrict__ corr_ptr, const int* __restrict__ corr_patch,
+                 const zd* __restrict__ corr_w, const int* __restrict__ inv, const int* __restrict__ nio,
+                 const zd* __restrict__ npart, const long long* __restrict__ it_ptr, const ZI* __restrict__ ient,
+                 int rs, zd* __restrict__ y) {
+  const long long gt = (long long)xcd_block() * blockDim.x + threadIdx.x;
+  const long long ir = gt / S;
+  const int sl = (int)(gt % S);
+  const int i = (int)(ir / R), r = (int)(ir % R);
+  zd acc{0.0, 0.0}, near{0.0, 0.0}, far{0.0, 0.0};
+  if (i < N) {
+    const int si = inv[i];
+    const zd* xr = x + (size_t)r * ld;
+    const long long e0 = corr_ptr[i], e1 = corr_ptr[i + 1];
+    const long long nitem = (e1 - e0) * Np;
+    for (long long it = sl; it < nitem; it += S) {
+      const long long e = e0 + it / Np;
+      const int q = (int)(it % Np);
+      cfma(acc, corr_w[e * Np + q], xr[(size_t)corr_patch[e] * Np + q]);
+    }
+    const int g = si / kNearGroup, t = si % kNearGroup;
+    const int w1 = nio[g + 1];
+    for (int w = nio[g] + sl; w < w1; w += S) near = cadd(near, npart[((size_t)w * kNearGroup + t) * R + r]);
+    const long long f1 = it_ptr[si + 1];
+    for (long long f = it_ptr[si] + sl; f < f1; f += S) far = cadd(far, zwiden(ient[(size_t)f * rs + r]));
+  }
+  acc = lane_group_sum<S>(acc);
+  near = lane_group_sum<S>(near);
+  far = lane_group_sum<S>(far);
+  if (i < N && sl == 0) y[(size_t)r * ld + i] = cadd(cadd(cscale(x[(size_t)r * ld + i], ident), acc), cadd(near, far));
+}
+
+}  // namespace ifgf

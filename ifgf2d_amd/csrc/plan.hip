@@ -13,6 +13,7 @@
 
 #include "../../include/ifgf2d.h"
 #include "kernels.h"
+#include "kernels_batched.h"
 #include "plan_internal.h"
 #include "solver.h"
 
@@ -410,20 +411,79 @@ static void launch_ifgf(ifgf_plan* P, const zd* x, zd* y, hipStream_t st) {
   HIP_TRY(hipGetLastError());
 }
 
+// One application to a block of R columns (x, y column-major, stride ld) with
+// the batched kernels; RT = double on a plain plan, float on a mixed plan.
+template <int F, int Ps, int Pang, int R, typename RT>
+static void launch_batched(ifgf_plan* P, const zd* x, long long ld, zd* y, hipStream_t st) {
+  using Z = typename complex_of<RT>::type;
+  constexpr bool f32 = std::is_same<RT, float>::value;
+  constexpr int Pn = Ps * Pang;
+  constexpr int CB = batched_cones(Pn, R, (int)sizeof(Z));
+  constexpr int LR = R == 2 ? 1 : R == 4 ? 2 : 3;
+  const Tree& t = P->tree;
+  const int N = (int)t.N, D = t.D;
+  hipLaunchKernelGGL((k_gather_b<R>), dim3((unsigned)(((long long)N * R + 255) / 256)), dim3(256), 0, st, N, x, ld,
+                     P->w.p, P->perm.p, P->rho.p);
+  const int M = (int)P->near_items.n;
+  hipLaunchKernelGGL((k_near_items_b<R>), dim3((M + 3) / 4), dim3(256), 0, st, N, M, P->near_items.p, P->nf_seg.p,
+                     P->nf_val.p, P->rho.p, P->npart_b.p);
+  const long long ML = P->leaf_items.n;
+  const Z* lf = f32 ? (const Z*)P->lf_val32.p : (const Z*)P->lf_val.p;
+  Z* lpart = f32 ? (Z*)P->lpart_b32.p : (Z*)P->lpart_b.p;
+  hipLaunchKernelGGL((k_leaf_items_b<Pn, R, Z>), dim3((unsigned)((ML * Pn * kLeafChunk + 255) / 256)), dim3(256), 0,
+                     st, (int)ML, P->leaf_items.p, lf, P->rho.p, lpart);
+  Z* pbuf = f32 ? (Z*)P->pbuf_b32.p : (Z*)P->pbuf_b.p;
+  PushArgsT<RT> a{};
+  if constexpr (f32) {
+    a = PushArgsT<RT>{nullptr, P->push_geo32.p, P->push_dst.p, P->leaf_item_off.p, lpart,
+                      t.lv[D].cone_off, P->pn_ifac32.p, P->tabs32.p, pbuf, 0, P->rb_max};
+  } else {
+    a = PushArgsT<RT>{nullptr, P->push_geo.p, P->push_dst.p, P->leaf_item_off.p, lpart,
+                      t.lv[D].cone_off, P->pn_ifac.p, P->tabs.p, pbuf, 0, P->rb_max};
+  }
+  const std::vector<int>& lo = P->blevel_off[LR];
+  for (int li = 0; li + 1 < D; li++) {
+    const int t0 = lo[li], nt = lo[li + 1] - t0;
+    if (nt == 0) continue;
+    a.tasks = P->btasks[LR].p + t0;
+    a.leaf = li == 0;
+    hipLaunchKernelGGL((k_level_push_b<Ps, Pang, R, CB, RT>), dim3(nt), dim3(256), 0, st, a);
+  }
+  const double ident = (F == kSingleLayer) ? 0.0 : 0.5;
+  const Z* ient = pbuf + (size_t)P->pbuf_ient * P->rb_max;
+  const long long nthr = (long long)N * R;
+#define IFGF_FINB(S)                                                                                               \
+  hipLaunchKernelGGL((k_finalize_b<S, R, Z>), dim3((unsigned)((nthr * S + 255) / 256)), dim3(256), 0, st, N,         \
+                     P->geo.Np, ident, x, ld, P->corr_ptr.p, P->corr_patch.p, P->corr_w.p, P->inv.p,                 \
+                     P->near_item_off.p, P->npart_b.p, P->it_ptr.p, ient, P->rb_max, y)
+  if (P->final_lanes == 4) IFGF_FINB(4);
+  else if (P->final_lanes == 16) IFGF_FINB(16);
+  else IFGF_FINB(8);
+#undef IFGF_FINB
+  HIP_TRY(hipGetLastError());
+}
+
+template <int F, int Ps, int Pang, typename RT>
+static void launch_batched_r(ifgf_plan* P, int R, const zd* x, long long ld, zd* y, hipStream_t st) {
+  if (R == 8) launch_batched<F, Ps, Pang, 8, RT>(P, x, ld, y, st);
+  else if (R == 4) launch_batched<F, Ps, Pang, 4, RT>(P, x, ld, y, st);
+  else launch_batched<F, Ps, Pang, 2, RT>(P, x, ld, y, st);
+}
+
 // Evaluates the stored operators once (plan creation).
 // Push tasks per level l = D..2 (level index li = D - l): runs of consecutive
 // cones of the level (at most 256 / P cones, at most emax entries); a cone
 // with more than emax entries gets tasks of its own, emax entries each.
-static void build_push_tasks(ifgf_plan* P, int emax) {
-  if (emax != 256 && emax != 512 && emax != 1024 && emax != 2048) throw std::runtime_error("push task size");
+static void make_push_tasks(const ifgf_plan* P, int emax, int cmax, std::vector<PushTask>& tasks,
+                            std::vector<int>& level_off) {
   const Tree& t = P->tree;
-  const int D = t.D, Pn = t.Ps * t.Pang, cmax = 256 / Pn;
+  const int D = t.D;
   const std::vector<long long>& off = P->push_off;
-  std::vector<PushTask> tasks;
-  P->push_level_off.assign(D, 0);
+  tasks.clear();
+  level_off.assign(D, 0);
   for (int li = 0; li + 1 < D; li++) {
     const int l = D - li;
-    P->push_level_off[li] = (int)tasks.size();
+    level_off[li] = (int)tasks.size();
     PushTask cur{0, 0, 0, 0};
     auto flush = [&] {
       if (cur.nc) tasks.push_back(cur);
@@ -448,9 +508,45 @@ static void build_push_tasks(ifgf_plan* P, int emax) {
     }
     flush();
   }
-  P->push_level_off[D - 1] = (int)tasks.size();
+  level_off[D - 1] = (int)tasks.size();
+}
+
+static void build_push_tasks(ifgf_plan* P, int emax) {
+  if (emax != 256 && emax != 512 && emax != 1024 && emax != 2048) throw std::runtime_error("push task size");
+  std::vector<PushTask> tasks;
+  make_push_tasks(P, emax, 256 / (P->tree.Ps * P->tree.Pang), tasks, P->push_level_off);
   P->push_tasks.upload(tasks);
   P->push_emax = emax;
+}
+
+// Buffers and push tasks of the batched kernels (column blocks up to rb_max).
+template <int Ps, int Pang>
+static void build_batched(ifgf_plan* P, long long ncs, long long nit) {
+  constexpr int Pn = Ps * Pang;
+  const int nr = P->prm.nrhs_max;
+  P->rb_max = nr >= 8 ? 8 : nr >= 4 ? 4 : nr >= 2 ? 2 : 0;
+  if (!P->rb_max) return;
+  const size_t R = (size_t)P->rb_max;
+  P->npart_b.alloc(P->npart.n * R);
+  if (P->mixed) {
+    P->lpart_b32.alloc(P->lpart.n * R);
+    P->pbuf_b32.alloc((size_t)(ncs + nit) * R);
+    HIP_TRY(hipMemsetAsync(P->pbuf_b32.p, 0, P->pbuf_b32.n * sizeof(zf), P->stream));
+  } else {
+    P->lpart_b.alloc(P->lpart.n * R);
+    P->pbuf_b.alloc((size_t)(ncs + nit) * R);
+    HIP_TRY(hipMemsetAsync(P->pbuf_b.p, 0, P->pbuf_b.n * sizeof(zd), P->stream));
+  }
+  const int zb = P->mixed ? (int)sizeof(zf) : (int)sizeof(zd);
+  for (int lr = 1; lr <= 3; lr++) {
+    const int r = 1 << lr;
+    if (r > P->rb_max) break;
+    std::vector<PushTask> tasks;
+    make_push_tasks(P, 256, batched_cones(Pn, r, zb), tasks, P->blevel_off[lr]);
+    P->btasks[lr].upload(tasks);
+  }
+  P->stored_bytes += (long long)(P->npart_b.n + P->lpart_b.n + P->pbuf_b.n) * (long long)sizeof(zd) +
+                     (long long)(P->lpart_b32.n + P->pbuf_b32.n) * (long long)sizeof(zf);
 }
 
 template <int F, int Ps, int Pang>
@@ -683,6 +779,8 @@ static void build_stored(ifgf_plan* P) {
                        (long long)P->push_geo32.n * (long long)sizeof(Geo4F);
     P->mixed = true;
   }
+  build_batched<Ps, Pang>(P, ncs, nit);
+  HIP_TRY(hipStreamSynchronize(st));
   P->stored = true;
 }
 
@@ -730,6 +828,34 @@ static bool dispatch_p(ifgf_plan* P, const zd* x, zd* y, hipStream_t st, bool fp
   IFGF_CASE(11, 11)
 #undef IFGF_CASE
   return false;
+}
+
+template <int F>
+static bool dispatch_b(ifgf_plan* P, int R, const zd* x, long long ld, zd* y, hipStream_t st) {
+  const int a = P->prm.Ps, b = P->prm.Pang;
+#define IFGF_CASE(A, B)                                              \
+  if (a == A && b == B) {                                            \
+    if (P->mixed) launch_batched_r<F, A, B, float>(P, R, x, ld, y, st);  \
+    else launch_batched_r<F, A, B, double>(P, R, x, ld, y, st);      \
+    return true;                                                     \
+  }
+  IFGF_CASE(5, 5)
+  IFGF_CASE(3, 5)
+  IFGF_CASE(5, 7)
+  IFGF_CASE(7, 7)
+  IFGF_CASE(9, 9)
+  IFGF_CASE(11, 11)
+#undef IFGF_CASE
+  return false;
+}
+
+// one block of R (2, 4 or 8) columns through the batched kernels
+static bool dispatch_block(ifgf_plan* P, int R, const zd* x, long long ld, zd* y, hipStream_t st) {
+  switch (P->prm.formulation) {
+    case kSingleLayer: return dispatch_b<kSingleLayer>(P, R, x, ld, y, st);
+    case kDoubleLayer: return dispatch_b<kDoubleLayer>(P, R, x, ld, y, st);
+    default: return dispatch_b<kCombined>(P, R, x, ld, y, st);
+  }
 }
 
 static bool dispatch(ifgf_plan* P, const zd* x, zd* y, hipStream_t st, bool fp64 = false) {
@@ -833,20 +959,43 @@ int ifgf_apply(ifgf_plan* P, const double* x, double* y, int nrhs, long long ld,
   if (nrhs < 1 || ld < N) return fail(P, IFGF_ERR_ARG, "nrhs < 1 or ld < n");
   try {
     hipStream_t st = stream ? (hipStream_t)stream : P->stream;
-    for (int r = 0; r < nrhs; r++) {
+    const bool fp64 = (flags & IFGF_APPLY_FP64) != 0;
+    const bool batched_ok = P->stored && P->rb_max >= 2 && !(fp64 && P->mixed);
+    int r = 0;
+    while (r < nrhs) {
+      // column block: the widest batched width that fits, else one column
+      int R = 1;
+      if (batched_ok)
+        for (int c : {8, 4, 2})
+          if (c <= P->rb_max && c <= nrhs - r) {
+            R = c;
+            break;
+          }
       const zd* xr = (const zd*)x + (size_t)r * ld;
       zd* yr = (zd*)y + (size_t)r * ld;
-      const bool fp64 = (flags & IFGF_APPLY_FP64) != 0;
       if (flags & IFGF_PTR_DEVICE) {
-        int rc = apply_device(P, xr, yr, st, fp64);
-        if (rc) return rc;
+        if (R > 1) {
+          if (!dispatch_block(P, R, xr, ld, yr, st)) return fail(P, IFGF_ERR_ARG, "no kernel instantiation");
+        } else {
+          int rc = apply_device(P, xr, yr, st, fp64);
+          if (rc) return rc;
+        }
       } else {
-        HIP_TRY(hipMemcpyAsync(P->xbuf.p, xr, N * sizeof(zd), hipMemcpyHostToDevice, st));
-        int rc = apply_device(P, P->xbuf.p, P->ybuf.p, st, fp64);
-        if (rc) return rc;
-        HIP_TRY(hipMemcpyAsync(yr, P->ybuf.p, N * sizeof(zd), hipMemcpyDeviceToHost, st));
+        for (int q = 0; q < R; q++)
+          HIP_TRY(hipMemcpyAsync(P->xbuf.p + (size_t)q * N, xr + (size_t)q * ld, N * sizeof(zd), hipMemcpyHostToDevice,
+                                 st));
+        if (R > 1) {
+          if (!dispatch_block(P, R, P->xbuf.p, N, P->ybuf.p, st)) return fail(P, IFGF_ERR_ARG, "no kernel instantiation");
+        } else {
+          int rc = apply_device(P, P->xbuf.p, P->ybuf.p, st, fp64);
+          if (rc) return rc;
+        }
+        for (int q = 0; q < R; q++)
+          HIP_TRY(hipMemcpyAsync(yr + (size_t)q * ld, P->ybuf.p + (size_t)q * N, N * sizeof(zd), hipMemcpyDeviceToHost,
+                                 st));
         HIP_TRY(hipStreamSynchronize(st));
       }
+      r += R;
     }
   } catch (std::exception& e) {
     return fail(P, IFGF_ERR_HIP, e.what());

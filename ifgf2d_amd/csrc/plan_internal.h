@@ -107,6 +107,14 @@ struct ifgf_plan {
   ifgf::DevBuf<ifgf::NearItem> near_items;  // near-field work items
   ifgf::DevBuf<int> near_item_off;          // items of target group g: [off[g], off[g+1])
   ifgf::DevBuf<zd> npart;                   // near-field partial sums [item][8]
+  // several right-hand sides per application: column blocks of rb_max (2, 4
+  // or 8; 0 = none) through the batched kernels; push tasks per block width R
+  // (index log2 R) with the cone cap of their LDS windows
+  int rb_max = 0;
+  ifgf::DevBuf<zd> npart_b, lpart_b, pbuf_b;
+  ifgf::DevBuf<zf> lpart_b32, pbuf_b32;
+  ifgf::DevBuf<ifgf::PushTask> btasks[4];
+  std::vector<int> blevel_off[4];
   // mixed precision (IFGF_MIXED): FP32 copies of the leaf operator, the
   // propagation / interpolation geometry and inverse node factors, and FP32
   // coefficient workspaces; the FP64 operator stays available (IFGF_APPLY_FP64)

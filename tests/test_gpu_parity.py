@@ -142,12 +142,52 @@ def test_graph_replay_matches_eager(overlap, precision):
         assert np.array_equal(got, ref), rep
 
 
+# Batched column blocks run other kernels than single applications (the same
+# operations per column, but the compiler may fuse multiply-adds differently),
+# so a column agrees with its single application to rounding, not bitwise.
+BATCH_TOL = 1e-14
+
+
 def test_multiple_right_hand_sides():
-    fm, _ = build_pair(I.star(a=[0, 0, 0, 0, 0, 0.3]), 800, 60.0, 7, I.COMBINED, 5, 5)
+    fm, om = build_pair(I.star(a=[0, 0, 0, 0, 0, 0.3]), 800, 60.0, 7, I.COMBINED, 5, 5)
     X = np.stack([density(fm.N, s) for s in (5, 6, 7)], axis=1)
     Y = fm.compute_Ax_acc(X)
     for j in range(3):
-        assert np.array_equal(Y[:, j], fm.compute_Ax_acc(X[:, j]))
+        e = rel(Y[:, j], fm.compute_Ax_acc(X[:, j]))
+        print(f"column {j}: batched vs single {e:.3e}", flush=True)
+        assert e <= BATCH_TOL
+        assert rel(Y[:, j], om.apply(X[:, j])) <= PARITY_TOL
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batched_columns_equal_single_applications(precision):
+    """nrhs > 1 runs column blocks of 8, 4 and 2 through the batched kernels
+    (each stored value loaded once per block); every column equals a single
+    application to rounding, for host and device pointers and ld > n; batched
+    applications themselves are bitwise deterministic."""
+    torch = pytest.importorskip("torch")
+    kw = configs.scaled(I.star(a=[0, 0, 0, 0, 0, 0.3]), 800, 60.0, 7)
+    fm = I.ForwardMap(**kw, precision=precision, nrhs_max=8)
+    n = fm.N
+    X = np.stack([density(n, 50 + s) for s in range(11)], axis=1)
+    single = np.stack([fm.compute_Ax_acc(X[:, j]) for j in range(11)], axis=1)
+    tol = BATCH_TOL if precision == "fp64" else 1e-6
+    for nr in (2, 3, 5, 8, 11):
+        Y = fm.compute_Ax_acc(X[:, :nr])
+        e = max(rel(Y[:, j], single[:, j]) for j in range(nr))
+        print(f"{precision} nrhs {nr}: max column difference {e:.3e}", flush=True)
+        assert e <= tol, nr
+        assert np.array_equal(Y, fm.compute_Ax_acc(X[:, :nr]))
+    Y11 = fm.compute_Ax_acc(X)
+    ld = n + 7
+    xt = torch.zeros((11, ld), dtype=torch.complex128, device="cuda:0")
+    xt[:, :n] = torch.from_numpy(np.ascontiguousarray(X.T))
+    yt = torch.zeros_like(xt)
+    torch.cuda.synchronize()
+    fm.apply_device(xt.data_ptr(), yt.data_ptr(), nrhs=11, ld=ld)
+    torch.cuda.synchronize()
+    assert np.array_equal(yt[:, :n].cpu().numpy().T, Y11)
+    assert np.all(yt[:, n:].cpu().numpy() == 0)
 
 
 def test_device_pointer_apply_with_torch_stream():
