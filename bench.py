@@ -11,7 +11,8 @@ the geometry and are evaluated once at plan creation (counted in
 plan_setup_s); the on-the-fly variant that re-evaluates them in every
 application is timed too (on_the_fly_matvecs_per_s). After the timed steps, one GMRES(50) solve to 1e-8 for the plane wave
 d = (cos pi/4, sin pi/4) gives the time to solve. The mixed-precision operator
-(FP32 cone work) and its FGMRES solve with FP64 refinement are timed beside it.
+(FP32 cone work) and its FGMRES solve with FP64 refinement are timed beside it,
+and so are 64-column applications through the batched kernels.
 
 Prints ONE JSON line on stdout; progress goes to stderr.
 """
@@ -52,7 +53,7 @@ def stage_work(st, formulation, P, Ps, Pang, stored):
     return w
 
 
-def cpu_baseline(kw, x):
+def cpu_baseline(kw, x, name):
     """The oracle (single-threaded CPU restatement, Amos Hankel) on one
     forward-map application of the same workload; oracle setup excluded."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -66,9 +67,8 @@ def cpu_baseline(kw, x):
     dt = time.time() - t0
     log(f"cpu baseline: one oracle application {dt:.2f} s")
     return {"value": 1.0 / dt, "unit": "matvecs/s", "cores": 1, "kind": "port",
-            "sample": "1 forward-map application (combined field, 5x5) of the 64,000-point kite at "
-                      "k=200 by the single-threaded oracle with the reference's Amos Hankel; "
-                      "oracle setup excluded"}
+            "sample": f"1 forward-map application (combined field, 5x5) of the {name} by the "
+                      "single-threaded oracle with the reference's Amos Hankel; oracle setup excluded"}
 
 
 def main():
@@ -80,6 +80,9 @@ def main():
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--no-on-the-fly", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--workload", choices=("kite", "star"), default="kite",
+                    help="kite: the 64k kite at k=200 (default); star: the 256k star r=1+0.3cos5t at k=500")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 20 device copies of 256 MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -103,7 +106,7 @@ def main():
     import ifgf2d_amd as I
     from ifgf2d_amd import configs
 
-    kw, d = configs.kite_k200()
+    kw, d = configs.kite_k200() if args.workload == "kite" else configs.star_k500()
     t0 = time.time()
     fm = I.ForwardMap(**kw, autotune=True)
     t_setup = time.time() - t0
@@ -202,7 +205,8 @@ def main():
         otf = round(nrep / (time.perf_counter() - t0), 3)
         log(f"on-the-fly kernels: {otf} matvecs/s")
         del fo
-    traffic_file = os.path.join(ROOT, "profiles", f"traffic_{roofline['kernel']}_kite_k200.json")
+    tag = "kite_k200" if args.workload == "kite" else "star_k500"
+    traffic_file = os.path.join(ROOT, "profiles", f"traffic_{roofline['kernel']}_{tag}.json")
     if os.path.exists(traffic_file):
         roofline["traffic"] = json.load(open(traffic_file)).get("bytes_per_launch")
 
@@ -249,15 +253,40 @@ def main():
             log(f"mixed FGMRES: {oit} outer + {iit} inner iterations, {t_f:.4f} s, true relres {rr:.2e}")
         del fmx
 
-    cpu = None if args.no_cpu_baseline else cpu_baseline(kw, x_host)
+    # several right-hand sides per application: 64 columns in blocks of 8
+    # through the batched kernels, FP64 and mixed precision
+    batched = None
+    if not args.no_batched:
+        batched = {"nrhs": 64}
+        Xb = torch.from_numpy(rng.standard_normal((64, fm.N)) + 1j * rng.standard_normal((64, fm.N))).to(dev)
+        Yb = torch.empty_like(Xb)
+        for prec in ("fp64", "mixed"):
+            fb = I.ForwardMap(**dict(kw, precision=prec), nrhs_max=64, autotune=True)
+            fb.apply_device(Xb.data_ptr(), Yb.data_ptr(), nrhs=64)
+            torch.cuda.synchronize()
+            reps = max(2, args.steps // 100)
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                fb.apply_device(Xb.data_ptr(), Yb.data_ptr(), nrhs=64)
+            torch.cuda.synchronize()
+            dt = (time.perf_counter() - t0) / reps
+            batched[prec] = {"ms_per_application": round(dt * 1e3, 4), "column_matvecs_per_s": round(64 / dt, 1)}
+            log(f"batched {prec}: 64 columns in {dt * 1e3:.3f} ms -> {64 / dt:.1f} column-matvecs/s")
+            del fb
+        del Xb, Yb
+
+    cpu = None if args.no_cpu_baseline else cpu_baseline(
+        kw, x_host, "64,000-point kite at k=200" if args.workload == "kite" else "256,000-point star at k=500")
 
     out = {
         "metric": "time-to-solve (rel. residual 1e-8) + IFGF matvecs/s vs plain FP64 on GPU",
         "value": round(value, 3), "unit": "matvecs/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms_per_step, 5), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "kite k=200, 6400 patches x 10 Fejer nodes (N=64000), combined "
-                               "field eta=k, 5x5 cone nodes, 10 levels, FP64",
+        "config": {"workload": ("kite k=200, 6400 patches x 10 Fejer nodes (N=64000), combined "
+                                "field eta=k, 5x5 cone nodes, 10 levels, FP64") if args.workload == "kite" else
+                               ("star r=1+0.3cos5t k=500, 25600 patches x 10 Fejer nodes (N=256000), combined "
+                                "field eta=k, 5x5 cone nodes, 12 levels, FP64"),
                    "N": fm.N, "nrhs": 1, "nlevels": kw["nlevels"], "Ps": 5, "Pang": 5,
                    "parallelism": "single GPU"},
         "stages_ms": {k_: round(v, 5) for k_, v in stage_avg.items()},
@@ -268,6 +297,7 @@ def main():
         "on_the_fly_matvecs_per_s": otf,
         "solve": solve,
         "mixed_precision": mixed,
+        "batched_rhs": batched,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
