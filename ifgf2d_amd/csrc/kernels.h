@@ -543,33 +543,43 @@ __global__ void __launch_bounds__(256)
 // block first forms the cones' coefficients in LDS:
 //   l = D: the sum of the leaf operator items' partial coefficients (already
 //          transformed);
-//   l < D: node value v_j = (p_0 + p_1 + p_2 + p_3) / factor_j from the four
-//          child slots p_k = pbuf[(g P + j) 4 + k] that the level below wrote
-//          (empty slots stay 0), then the separable 2D transform.
-// It then evaluates those interpolants at every push entry (cone, s, theta,
-// factor, destination) of the task and writes value * factor to
-// pbuf[destination]: either the k-th child slot of a parent node at level
-// l-1 (propagation) or the slot of one (target, cousin cone) interpolation
-// entry (summed per target in fixed order by k_finalize). Every destination
-// slot has exactly one writer, so no atomics and a fixed summation order.
-// A level costs one dependent round trip (the four child slots) before the
-// LDS work, instead of the pull form's entry range -> child entries -> child
-// coefficients chain.
+//   l < D: node value v_j = (p_0 + p_1 + p_2 + p_3) / factor_j from the
+//          values its (up to four) relevant children pushed at the level
+//          below, read through cidx (absent children read the zero slot),
+//          then the separable 2D transform.
+// It then evaluates those interpolants at every push entry (s, theta,
+// factor) of the task and writes value * factor to slot 1 + q of pbuf for
+// push entry q, so consecutive threads write consecutive slots. A push
+// entry is either a parent node at level l-1 falling in the cone
+// (propagation) or a (target, cousin cone) pair (interpolation, summed per
+// target in fixed order by k_finalize). Every slot has exactly one writer,
+// so no atomics and a fixed summation order. The cone of an entry follows
+// from the cones' entry offsets coff (entries are grouped by cone).
 template <typename R>
 struct PushArgsT {
   using Z = typename complex_of<R>::type;
   const PushTask* tasks;
   const Geo4T<R>* geo;   // per push entry: (s, theta, centre factor)
-  const int2* dst;       // per push entry: (destination slot in pbuf, global cone id)
+  const int* coff;       // push entries of global cone g: [coff[g], coff[g+1])
+  const int4* cidx;      // per (cone, node): slots of its children's pushed values (0: none)
   const int* lio;        // level D: items of leaf cone c: [lio[c], lio[c+1])
   const Z* lpart;        // level D: leaf item partial coefficients [item][P]
   long long leaf_cone_off;
   const Z* pn_ifac;      // 1 / centre factor per (cone, node), levels < D
   const ChebTablesT<R>* tabs;
-  Z* pbuf;               // child slots of every (cone, node), then interpolation entries
+  Z* pbuf;               // slot 0 = 0, slot 1 + q = push entry q's value
   int leaf;              // 1 at level D
   int rs;                // batched kernels: columns per slot in pbuf (the widest block)
 };
+
+// Local cone (0 .. task.nc-1) of entry e of a task: entries are grouped by
+// cone in cone order.
+template <typename R>
+__device__ inline int push_entry_cone(const PushArgsT<R>& a, const PushTask& task, int e) {
+  int c = 0;
+  for (int q = 1; q < task.nc; q++) c += (e >= a.coff[task.c0 + q] - task.e0) ? 1 : 0;
+  return c;
+}
 
 // EPT entries per thread: a task holds at most 256 * EPT entries, and every
 // thread loads all of its entries together with the cones' inputs before the
@@ -594,14 +604,10 @@ __global__ void __launch_bounds__(256) k_level_push(PushArgsT<R> a) {
   const bool act = cl < task.nc;
   const long long g = (long long)task.c0 + cl;
   Geo4T<R> G[EPT];
-  int2 d[EPT];
 #pragma unroll
   for (int u = 0; u < EPT; u++) {
     const int e = tid + u * 256;
-    if (e < task.ne) {
-      G[u] = a.geo[task.e0 + e];
-      d[u] = a.dst[task.e0 + e];
-    }
+    if (e < task.ne) G[u] = a.geo[task.e0 + e];
   }
   Z v{0, 0};
   if (act) {
@@ -610,8 +616,8 @@ __global__ void __launch_bounds__(256) k_level_push(PushArgsT<R> a) {
       const int w1 = a.lio[c + 1];
       for (int w = a.lio[c]; w < w1; w++) v = zadd(v, a.lpart[(size_t)w * P + j]);
     } else {
-      const Z* s = a.pbuf + (g * P + j) * 4;
-      const Z s0 = s[0], s1 = s[1], s2 = s[2], s3 = s[3];
+      const int4 ci = a.cidx[g * P + j];
+      const Z s0 = a.pbuf[ci.x], s1 = a.pbuf[ci.y], s2 = a.pbuf[ci.z], s3 = a.pbuf[ci.w];
       v = zmul(zadd(zadd(zadd(s0, s1), s2), s3), a.pn_ifac[g * P + j]);
     }
     buf[tid] = v;
@@ -625,9 +631,11 @@ __global__ void __launch_bounds__(256) k_level_push(PushArgsT<R> a) {
   }
 #pragma unroll
   for (int u = 0; u < EPT; u++) {
-    if (tid + u * 256 < task.ne) {
-      const Z val = cone_eval<Ps, Pang>(buf + (d[u].y - task.c0) * P, G[u].s, G[u].t);
-      a.pbuf[d[u].x] = zmul(val, Z{G[u].fre, G[u].fim});
+    const int e = tid + u * 256;
+    if (e < task.ne) {
+      const int ce = push_entry_cone(a, task, e);
+      const Z val = cone_eval<Ps, Pang>(buf + ce * P, G[u].s, G[u].t);
+      a.pbuf[1 + task.e0 + e] = zmul(val, Z{G[u].fre, G[u].fim});
     }
   }
 }
@@ -736,15 +744,17 @@ __global__ void __launch_bounds__(256)
 // S lanes per target split the correction products, the target's near-field
 // item partials (stored path: npart[item * 8 + si % 8] for the items
 // [nio[si / 8], nio[si / 8 + 1]) of its group; on the fly: sol[si]) and its
-// interpolation entries (stored path: ient[it_ptr[si] .. it_ptr[si+1]), one
-// per (level, cousin cone)); lane 0 adds the identity.
+// interpolation entries (stored path: slots[it_slot[f]] for f in
+// [it_ptr[si], it_ptr[si+1]), one per (level, cousin cone), in that order);
+// lane 0 adds the identity.
 template <int S, typename ZI>
 __global__ void __launch_bounds__(256)
     k_finalize(int N, int Np, double ident, const zd* __restrict__ x,
                const long long* __restrict__ corr_ptr, const int* __restrict__ corr_patch,
                const zd* __restrict__ corr_w, const int* __restrict__ inv, const zd* __restrict__ sol,
                const int* __restrict__ nio, const zd* __restrict__ npart,
-               const long long* __restrict__ it_ptr, const ZI* __restrict__ ient, zd* __restrict__ y) {
+               const long long* __restrict__ it_ptr, const int* __restrict__ it_slot, const ZI* __restrict__ slots,
+               zd* __restrict__ y) {
   const int lb = xcd_block();
   const int gt = lb * blockDim.x + threadIdx.x;
   const int i = gt / S, sl = gt % S;
@@ -766,9 +776,9 @@ __global__ void __launch_bounds__(256)
     } else if (sl == 0) {
       near = sol[si];
     }
-    if (ient) {
+    if (slots) {
       const long long f1 = it_ptr[si + 1];
-      for (long long f = it_ptr[si] + sl; f < f1; f += S) far = cadd(far, zwiden(ient[f]));
+      for (long long f = it_ptr[si] + sl; f < f1; f += S) far = cadd(far, zwiden(slots[it_slot[f]]));
     }
   }
   acc = lane_group_sum<S>(acc);

@@ -131,8 +131,8 @@ __global__ void __launch_bounds__(256)
 // k_level_push with R columns. Tasks hold at most CB cones (their R-column
 // coefficients fit the LDS windows) and 256 entries (one per thread); the
 // cone phase runs on threads (cone, node) < CB * P. Every slot of pbuf holds
-// a.rs >= R columns whatever the block width, so the child slots no entry
-// writes (parents with fewer than four children) stay zero for every width.
+// a.rs >= R columns whatever the block width, so the zero slot that absent
+// children read stays zero for every width.
 template <int Ps, int Pang, int R, int CB, typename RT>
 __global__ void __launch_bounds__(256) k_level_push_b(PushArgsT<RT> a) {
   using Z = typename complex_of<RT>::type;
@@ -150,12 +150,8 @@ __global__ void __launch_bounds__(256) k_level_push_b(PushArgsT<RT> a) {
   const bool act = cl < task.nc && cl < CB;
   const long long g = (long long)task.c0 + cl;
   Geo4T<RT> G;
-  int2 d;
   const bool has = tid < task.ne;
-  if (has) {
-    G = a.geo[task.e0 + tid];
-    d = a.dst[task.e0 + tid];
-  }
+  if (has) G = a.geo[task.e0 + tid];
   // cone inputs: column r of node j of cone cl at buf[(cl * P + j) * R + r]
   if (act) {
     if (a.leaf) {
@@ -169,11 +165,15 @@ __global__ void __launch_bounds__(256) k_level_push_b(PushArgsT<RT> a) {
       }
     } else {
       const int rs = a.rs;
-      const Z* s = a.pbuf + (g * P + j) * 4 * rs;
+      const int4 ci = a.cidx[g * P + j];
+      const Z* s0 = a.pbuf + (size_t)ci.x * rs;
+      const Z* s1 = a.pbuf + (size_t)ci.y * rs;
+      const Z* s2 = a.pbuf + (size_t)ci.z * rs;
+      const Z* s3 = a.pbuf + (size_t)ci.w * rs;
       const Z f = a.pn_ifac[g * P + j];
 #pragma unroll
       for (int r = 0; r < R; r++)
-        buf[(cl * P + j) * R + r] = zmul(zadd(zadd(zadd(s[r], s[rs + r]), s[2 * rs + r]), s[3 * rs + r]), f);
+        buf[(cl * P + j) * R + r] = zmul(zadd(zadd(zadd(s0[r], s1[r]), s2[r]), s3[r]), f);
     }
   }
   __syncthreads();
@@ -220,9 +220,9 @@ __global__ void __launch_bounds__(256) k_level_push_b(PushArgsT<RT> a) {
     if (Pang > 1) ta[1] = G.t;
 #pragma unroll
     for (int i = 2; i < Pang; i++) ta[i] = 2 * G.t * ta[i - 1] - ta[i - 2];
-    const Z* c = buf + (size_t)(d.y - task.c0) * P * R;
+    const Z* c = buf + (size_t)push_entry_cone(a, task, tid) * P * R;
     const Z f{G.fre, G.fim};
-    Z* out = a.pbuf + (size_t)d.x * a.rs;
+    Z* out = a.pbuf + (size_t)(1 + task.e0 + tid) * a.rs;
 #pragma unroll
     for (int r = 0; r < R; r++) {
       Z acc{0, 0};
@@ -245,14 +245,14 @@ __global__ void __launch_bounds__(256) k_level_push_b(PushArgsT<RT> a) {
 
 // k_finalize with R columns: lanes (target, column, slice) with S slices per
 // (target, column); y is column-major with stride ld; interpolation entry f
-// holds rs columns at ient[f * rs].
+// holds rs columns at slots[it_slot[f] * rs].
 template <int S, int R, typename ZI>
 __global__ void __launch_bounds__(256)
     k_finalize_b(int N, int Np, double ident, const zd* __restrict__ x, long long ld,
                  const long long* __restrict__ corr_ptr, const int* __restrict__ corr_patch,
                  const zd* __restrict__ corr_w, const int* __restrict__ inv, const int* __restrict__ nio,
-                 const zd* __restrict__ npart, const long long* __restrict__ it_ptr, const ZI* __restrict__ ient,
-                 int rs, zd* __restrict__ y) {
+                 const zd* __restrict__ npart, const long long* __restrict__ it_ptr, const int* __restrict__ it_slot,
+                 const ZI* __restrict__ slots, int rs, zd* __restrict__ y) {
   const long long gt = (long long)xcd_block() * blockDim.x + threadIdx.x;
   const long long ir = gt / S;
   const int sl = (int)(gt % S);
@@ -272,7 +272,7 @@ __global__ void __launch_bounds__(256)
     const int w1 = nio[g + 1];
     for (int w = nio[g] + sl; w < w1; w += S) near = cadd(near, npart[((size_t)w * kNearGroup + t) * R + r]);
     const long long f1 = it_ptr[si + 1];
-    for (long long f = it_ptr[si] + sl; f < f1; f += S) far = cadd(far, zwiden(ient[(size_t)f * rs + r]));
+    for (long long f = it_ptr[si] + sl; f < f1; f += S) far = cadd(far, zwiden(slots[(size_t)it_slot[f] * rs + r]));
   }
   acc = lane_group_sum<S>(acc);
   near = lane_group_sum<S>(near);

@@ -280,10 +280,10 @@ static void launch_level_push_ept(ifgf_plan* P, hipStream_t st) {
   const int D = P->tree.D;
   PushArgsT<R> a{};
   if constexpr (std::is_same<R, float>::value) {
-    a = PushArgsT<R>{nullptr,     P->push_geo32.p, P->push_dst.p, P->leaf_item_off.p, P->lpart32.p,
+    a = PushArgsT<R>{nullptr,     P->push_geo32.p, P->push_coff.p, P->push_cidx.p, P->leaf_item_off.p, P->lpart32.p,
                      P->tree.lv[D].cone_off, P->pn_ifac32.p, P->tabs32.p, P->pbuf32.p, 0};
   } else {
-    a = PushArgsT<R>{nullptr,   P->push_geo.p, P->push_dst.p, P->leaf_item_off.p, P->lpart.p,
+    a = PushArgsT<R>{nullptr,   P->push_geo.p, P->push_coff.p, P->push_cidx.p, P->leaf_item_off.p, P->lpart.p,
                      P->tree.lv[D].cone_off, P->pn_ifac.p, P->tabs.p, P->pbuf.p, 0};
   }
   for (int li = 0; li + 1 < D; li++) {
@@ -397,12 +397,12 @@ static void launch_ifgf(ifgf_plan* P, const zd* x, zd* y, hipStream_t st) {
   if (fork) HIP_TRY(hipStreamWaitEvent(st, P->ev_join, 0));
   mark(10);
   const double ident = (F == kSingleLayer) ? 0.0 : 0.5;
-  const Z* ient = nullptr;
-  if (P->stored) ient = f32 ? (const Z*)P->pbuf32.p + P->pbuf_ient : (const Z*)P->pbuf.p + P->pbuf_ient;
+  const Z* slots = nullptr;
+  if (P->stored) slots = f32 ? (const Z*)P->pbuf32.p : (const Z*)P->pbuf.p;
 #define IFGF_FIN(S)                                                                                                \
   hipLaunchKernelGGL((k_finalize<S, Z>), dim3((N + 256 / S - 1) / (256 / S)), dim3(256), 0, st, N, P->geo.Np, ident, \
                      x, P->corr_ptr.p, P->corr_patch.p, P->corr_w.p, P->inv.p, P->sol.p, P->near_item_off.p,       \
-                     P->stored ? P->npart.p : nullptr, P->it_ptr.p, ient, y)
+                     P->stored ? P->npart.p : nullptr, P->it_ptr.p, P->it_slot.p, slots, y)
   if (P->final_lanes == 4) IFGF_FIN(4);
   else if (P->final_lanes == 16) IFGF_FIN(16);
   else IFGF_FIN(8);
@@ -435,10 +435,10 @@ static void launch_batched(ifgf_plan* P, const zd* x, long long ld, zd* y, hipSt
   Z* pbuf = f32 ? (Z*)P->pbuf_b32.p : (Z*)P->pbuf_b.p;
   PushArgsT<RT> a{};
   if constexpr (f32) {
-    a = PushArgsT<RT>{nullptr, P->push_geo32.p, P->push_dst.p, P->leaf_item_off.p, lpart,
+    a = PushArgsT<RT>{nullptr, P->push_geo32.p, P->push_coff.p, P->push_cidx.p, P->leaf_item_off.p, lpart,
                       t.lv[D].cone_off, P->pn_ifac32.p, P->tabs32.p, pbuf, 0, P->rb_max};
   } else {
-    a = PushArgsT<RT>{nullptr, P->push_geo.p, P->push_dst.p, P->leaf_item_off.p, lpart,
+    a = PushArgsT<RT>{nullptr, P->push_geo.p, P->push_coff.p, P->push_cidx.p, P->leaf_item_off.p, lpart,
                       t.lv[D].cone_off, P->pn_ifac.p, P->tabs.p, pbuf, 0, P->rb_max};
   }
   const std::vector<int>& lo = P->blevel_off[LR];
@@ -450,12 +450,12 @@ static void launch_batched(ifgf_plan* P, const zd* x, long long ld, zd* y, hipSt
     hipLaunchKernelGGL((k_level_push_b<Ps, Pang, R, CB, RT>), dim3(nt), dim3(256), 0, st, a);
   }
   const double ident = (F == kSingleLayer) ? 0.0 : 0.5;
-  const Z* ient = pbuf + (size_t)P->pbuf_ient * P->rb_max;
+
   const long long nthr = (long long)N * R;
 #define IFGF_FINB(S)                                                                                               \
   hipLaunchKernelGGL((k_finalize_b<S, R, Z>), dim3((unsigned)((nthr * S + 255) / 256)), dim3(256), 0, st, N,         \
                      P->geo.Np, ident, x, ld, P->corr_ptr.p, P->corr_patch.p, P->corr_w.p, P->inv.p,                 \
-                     P->near_item_off.p, P->npart_b.p, P->it_ptr.p, ient, P->rb_max, y)
+                     P->near_item_off.p, P->npart_b.p, P->it_ptr.p, P->it_slot.p, pbuf, P->rb_max, y)
   if (P->final_lanes == 4) IFGF_FINB(4);
   else if (P->final_lanes == 16) IFGF_FINB(16);
   else IFGF_FINB(8);
@@ -521,7 +521,7 @@ static void build_push_tasks(ifgf_plan* P, int emax) {
 
 // Buffers and push tasks of the batched kernels (column blocks up to rb_max).
 template <int Ps, int Pang>
-static void build_batched(ifgf_plan* P, long long ncs, long long nit) {
+static void build_batched(ifgf_plan* P) {
   constexpr int Pn = Ps * Pang;
   const int nr = P->prm.nrhs_max;
   P->rb_max = nr >= 8 ? 8 : nr >= 4 ? 4 : nr >= 2 ? 2 : 0;
@@ -530,11 +530,11 @@ static void build_batched(ifgf_plan* P, long long ncs, long long nit) {
   P->npart_b.alloc(P->npart.n * R);
   if (P->mixed) {
     P->lpart_b32.alloc(P->lpart.n * R);
-    P->pbuf_b32.alloc((size_t)(ncs + nit) * R);
+    P->pbuf_b32.alloc((size_t)P->nslots * R);
     HIP_TRY(hipMemsetAsync(P->pbuf_b32.p, 0, P->pbuf_b32.n * sizeof(zf), P->stream));
   } else {
     P->lpart_b.alloc(P->lpart.n * R);
-    P->pbuf_b.alloc((size_t)(ncs + nit) * R);
+    P->pbuf_b.alloc((size_t)P->nslots * R);
     HIP_TRY(hipMemsetAsync(P->pbuf_b.p, 0, P->pbuf_b.n * sizeof(zd), P->stream));
   }
   const int zb = P->mixed ? (int)sizeof(zf) : (int)sizeof(zd);
@@ -703,16 +703,19 @@ static void build_stored(ifgf_plan* P) {
   if (npr) HIP_TRY(hipMemcpyAsync(pr_cone.data(), d_pr_cone.p, npr * sizeof(int), hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
   // push entries grouped by the cone that evaluates them: first the parent
-  // nodes that fall in the cone (destination: the node's k-th child slot),
-  // then the (target, cousin) interpolation entries (destination: the entry)
-  const long long ncs = t.n_cones * (long long)Pn * 4;
-  if (ncs + nit > 2147483647LL) throw std::runtime_error("too many propagation slots and interpolation entries");
+  // nodes that fall in the cone, then the (target, cousin) interpolation
+  // entries. Push entry q writes slot 1 + q of pbuf (slot 0 stays zero), so
+  // a task's writes are one contiguous run; a parent node reads its k-th
+  // child's value through cidx[node][k] (0 for an absent child) and the
+  // finalize a target's interpolations through it_slot.
+  const long long npush = npr + nit;
+  if (npush + 1 > 2147483647LL) throw std::runtime_error("too many propagation and interpolation entries");
   std::vector<long long> off(t.n_cones + 1, 0);
   for (long long e = 0; e < npr; e++) off[pr_cone[e] + 1]++;
   for (long long e = 0; e < nit; e++) off[t.it_cone[e] + 1]++;
   for (long long g = 0; g < t.n_cones; g++) off[g + 1] += off[g];
-  std::vector<long long> pos(off.begin(), off.end() - 1), src(npr + nit);
-  std::vector<int2> dst(npr + nit);
+  std::vector<long long> pos(off.begin(), off.end() - 1), src(npush);
+  std::vector<int4> cidx((size_t)t.n_cones * Pn, make_int4(0, 0, 0, 0));
   for (long long g = 0; g < t.n_cones; g++) {
     const int nch = (int)((pr_ptr[g + 1] - pr_ptr[g]) / Pn);
     for (int kc = 0; kc < nch; kc++)
@@ -720,16 +723,23 @@ static void build_stored(ifgf_plan* P) {
         const long long e = pr_ptr[g] + (long long)kc * Pn + j;
         const long long q = pos[pr_cone[e]]++;
         src[q] = e;
-        dst[q] = make_int2((int)((g * Pn + j) * 4 + kc), pr_cone[e]);
+        int* c = &cidx[(size_t)g * Pn + j].x;
+        c[kc] = (int)(1 + q);
       }
   }
+  std::vector<int> it_slot(nit);
   for (long long e = 0; e < nit; e++) {
     const long long q = pos[t.it_cone[e]]++;
     src[q] = npr + e;
-    dst[q] = make_int2((int)(ncs + e), t.it_cone[e]);
+    it_slot[e] = (int)(1 + q);
   }
+  P->it_slot.upload(it_slot);
+  P->push_cidx.upload(cidx);
   P->push_off = off;
-  P->push_dst.upload(dst);
+  {
+    std::vector<int> coff(off.begin(), off.end());
+    P->push_coff.upload(coff);
+  }
   P->push_geo.alloc(npr + nit);
   {
     DevBuf<long long> d_src;
@@ -739,13 +749,14 @@ static void build_stored(ifgf_plan* P) {
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(st));
   }
-  P->pbuf.alloc(ncs + nit);
-  HIP_TRY(hipMemset(P->pbuf.p, 0, (ncs + nit) * sizeof(zd)));
-  P->pbuf_ient = ncs;
+  P->nslots = 1 + npush;
+  P->pbuf.alloc(P->nslots);
+  HIP_TRY(hipMemset(P->pbuf.p, 0, P->nslots * sizeof(zd)));
   build_push_tasks(P, P->push_emax);
-  P->stored_bytes = (long long)(nf_goff[ngroups] + lf_off[LD.ncone] + (long long)t.n_cones * Pn + ncs + nit) *
+  P->stored_bytes = (long long)(nf_goff[ngroups] + lf_off[LD.ncone] + (long long)t.n_cones * Pn + P->nslots) *
                         (long long)sizeof(zd) +
-                    (npr + nit) * (long long)(sizeof(Geo4) + sizeof(int2));
+                    npush * (long long)sizeof(Geo4) + nit * (long long)sizeof(int) +
+                    (long long)t.n_cones * Pn * (long long)sizeof(int4);
   if (P->prm.precision == IFGF_MIXED) {
     // FP32 copies: leaf operator, propagation / interpolation geometry
     // (factors evaluated in FP64 above, rounded here), inverse node factors
@@ -779,7 +790,7 @@ static void build_stored(ifgf_plan* P) {
                        (long long)P->push_geo32.n * (long long)sizeof(Geo4F);
     P->mixed = true;
   }
-  build_batched<Ps, Pang>(P, ncs, nit);
+  build_batched<Ps, Pang>(P);
   HIP_TRY(hipStreamSynchronize(st));
   P->stored = true;
 }

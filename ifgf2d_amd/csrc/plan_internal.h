@@ -94,9 +94,11 @@ struct ifgf_plan {
   // level-push propagation / interpolation
   ifgf::DevBuf<ifgf::PushTask> push_tasks;
   ifgf::DevBuf<ifgf::Geo4> push_geo;
-  ifgf::DevBuf<int2> push_dst;     // (destination slot in pbuf, global cone id)
-  ifgf::DevBuf<zd> pbuf;           // child slots [cone][node][4], then interpolation entries
-  long long pbuf_ient = 0;         // offset of the interpolation entries in pbuf
+  ifgf::DevBuf<int> push_coff;     // push entries of global cone g: [coff[g], coff[g+1])
+  ifgf::DevBuf<int4> push_cidx;    // per (cone, node): pbuf slots of its <= 4 children (0: none)
+  ifgf::DevBuf<zd> pbuf;           // slot 0 = 0, then slot 1 + q for push entry q
+  long long nslots = 0;            // 1 + push entries
+  ifgf::DevBuf<int> it_slot;       // slot of every interpolation entry (it_ptr order)
   std::vector<long long> push_off; // push entries of global cone g: [off[g], off[g+1])
   std::vector<int> push_level_off; // tasks of level index li: [off[li], off[li+1])
   int push_emax = 512;             // entries per push task (autotuned)
