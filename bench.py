@@ -155,11 +155,13 @@ def main():
     stored = ext["stored"]
     if stored:
         # one push launch per level carries propagation and interpolation;
-        # it streams 40 B of (s, theta, factor, destination) per evaluation and
-        # 64 B of child slots + 16 B factor per parent node, and writes 16 B per evaluation
+        # it streams 32 B of stored (s, theta, factor) per evaluation and writes
+        # its 16 B value; per parent-cone node it reads a 16 B child-slot index
+        # and a 16 B inverse factor, and the children's pushed values (16 B per
+        # propagation evaluation)
         flop = work["propagation"][1] + work["interpolation"][1]
         nev = st["prop"] + st["interp"]
-        byts = 56.0 * nev + 80.0 * 25 * (st["cones"] - ext["leaf_cones"])
+        byts = 48.0 * nev + 32.0 * 25 * (st["cones"] - ext["leaf_cones"]) + 16.0 * st["prop"]
         work["propagation"] = ("mixed", (flop, byts), "FLOP+B")
         work.pop("interpolation")
     dom = max(work, key=lambda s: stage_avg[s])
