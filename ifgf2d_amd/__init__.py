@@ -41,6 +41,7 @@ IFGF_PLAN_HOST_ONLY, IFGF_PLAN_ON_THE_FLY, IFGF_PLAN_AUTOTUNE, IFGF_PTR_DEVICE =
 IFGF_APPLY_FP64 = 2
 IFGF_PLAN_DEVICE_SORT = 8
 IFGF_PLAN_OVERLAP = 16
+IFGF_PLAN_DEVICE_PRECOMP = 32
 
 
 class IFGFError(RuntimeError):
@@ -189,11 +190,13 @@ class ForwardMap:
     host's stable sort).
     overlap: run the stored near field on a second stream beside the leaf operator and the
     level pushes (the autotuner also tries it).
+    device_precomp: build the rectangular-polar singular / near-singular correction on the
+    GPU instead of the host.
     """
 
     def __init__(self, curve, patch_lims, k, nlevels, Np=10, formulation="combined", eta=None,
                  delta=None, p=4.0, Ps=5, Pang=5, precision="fp64", nrhs_max=1, host_only=False,
-                 on_the_fly=False, autotune=False, device_sort=False, overlap=False):
+                 on_the_fly=False, autotune=False, device_sort=False, overlap=False, device_precomp=False):
         L = lib()
         self.curve = curve
         self.lims = np.ascontiguousarray(patch_lims, dtype=np.float64)
@@ -206,7 +209,7 @@ class ForwardMap:
                         Np, self.delta, float(p))
         flags = ((IFGF_PLAN_HOST_ONLY if host_only else 0) | (IFGF_PLAN_ON_THE_FLY if on_the_fly else 0)
                  | (IFGF_PLAN_AUTOTUNE if autotune else 0) | (IFGF_PLAN_DEVICE_SORT if device_sort else 0)
-                 | (IFGF_PLAN_OVERLAP if overlap else 0))
+                 | (IFGF_PLAN_OVERLAP if overlap else 0) | (IFGF_PLAN_DEVICE_PRECOMP if device_precomp else 0))
         prm = _Params(self.formulation, _PREC[precision], self.k, self.eta, Ps, Pang, nlevels,
                       nrhs_max, flags)
         h = ctypes.c_void_p()

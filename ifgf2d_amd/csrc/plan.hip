@@ -103,8 +103,13 @@ int ifgf_plan_create(const ifgf_surface* s, const ifgf_params* prm, ifgf_plan** 
       return fail(nullptr, IFGF_ERR_ARG, err);
     if (!build_geometry(P->curve, s->patch_lims, s->npatches, s->Np, s->delta, P->geo, err))
       return fail(nullptr, IFGF_ERR_GEOMETRY, err);
-    if (!build_correction(P->geo, P->curve, prm->formulation, prm->k, prm->eta, s->rp_p, P->corr, err))
+    if ((prm->flags & IFGF_PLAN_DEVICE_PRECOMP) && !(prm->flags & IFGF_PLAN_HOST_ONLY)) {
+      if (!build_correction_device(P->geo, P->curve, prm->formulation, prm->k, prm->eta, s->rp_p, P->corr, err,
+                                   nullptr))
+        return fail(nullptr, IFGF_ERR_ARG, err);
+    } else if (!build_correction(P->geo, P->curve, prm->formulation, prm->k, prm->eta, s->rp_p, P->corr, err)) {
       return fail(nullptr, IFGF_ERR_GEOMETRY, err);
+    }
     std::vector<int> dperm;
     if ((prm->flags & IFGF_PLAN_DEVICE_SORT) && !(prm->flags & IFGF_PLAN_HOST_ONLY)) {
       if (prm->nlevels > 16) return fail(nullptr, IFGF_ERR_ARG, "nlevels above 16 (leaf codes are 32-bit)");

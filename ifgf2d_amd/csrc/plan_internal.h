@@ -157,6 +157,10 @@ namespace ifgf {
 // all-FP64 operator on a mixed-precision plan
 int apply_device(ifgf_plan* P, const zd* x, zd* y, hipStream_t st, bool fp64 = false);
 int plan_fail(ifgf_plan* p, int code, const std::string& msg);
+// The RP correction built on the GPU (precomp.hip); false (err set) when Np
+// exceeds the device kernel's limit.
+bool build_correction_device(const Geometry& g, const Curve& c, int F, double k, double eta, double p,
+                             Correction& out, std::string& err, hipStream_t st);
 // Leaf Morton codes and their stable sort on the GPU (morton.hip); false when
 // a point lies outside the leaf grid.
 bool device_morton_sort(const double* x, const double* y, long long n, int nlevels, std::vector<int>& perm,

@@ -16,71 +16,6 @@
 namespace ifgf {
 
 // ---------------------------------------------------------------- curves ---
-double Curve::x(double t) const {
-  if (type == 0) return R * std::cos(t) + cx;
-  if (type == 1) return A * std::cos(t) + B * std::cos(2.0 * t) + C;
-  double r = 1.0;
-  for (size_t m = 0; m < am.size(); m++) r += am[m] * std::cos(m * t) + bm[m] * std::sin(m * t);
-  return cx + R * r * std::cos(t);
-}
-double Curve::y(double t) const {
-  if (type == 0) return R * std::sin(t) + cy;
-  if (type == 1) return D * std::sin(t);
-  double r = 1.0;
-  for (size_t m = 0; m < am.size(); m++) r += am[m] * std::cos(m * t) + bm[m] * std::sin(m * t);
-  return cy + R * r * std::sin(t);
-}
-static void star_rrp(const Curve& c, double t, double& r, double& rp) {
-  double a = 1.0, d = 0.0;
-  for (size_t m = 0; m < c.am.size(); m++) {
-    a += c.am[m] * std::cos(m * t) + c.bm[m] * std::sin(m * t);
-    d += -c.am[m] * m * std::sin(m * t) + c.bm[m] * m * std::cos(m * t);
-  }
-  r = c.R * a;
-  rp = c.R * d;
-}
-double Curve::xp(double t) const {
-  if (type == 0) return -R * std::sin(t);
-  if (type == 1) return -A * std::sin(t) - 2.0 * B * std::sin(2.0 * t);
-  double r, rp;
-  star_rrp(*this, t, r, rp);
-  return rp * std::cos(t) - r * std::sin(t);
-}
-double Curve::yp(double t) const {
-  if (type == 0) return R * std::cos(t);
-  if (type == 1) return D * std::cos(t);
-  double r, rp;
-  star_rrp(*this, t, r, rp);
-  return rp * std::sin(t) + r * std::cos(t);
-}
-
-void Curve::diff(double a, double d, double& dx, double& dy) const {
-  const double mid = a - 0.5 * d;
-  const double s1 = std::sin(0.5 * d);
-  if (type == 0) {
-    dx = -2.0 * R * std::sin(mid) * s1;
-    dy = 2.0 * R * std::cos(mid) * s1;
-    return;
-  }
-  if (type == 1) {
-    dx = -2.0 * A * std::sin(mid) * s1 - 2.0 * B * std::sin(2.0 * mid) * std::sin(d);
-    dy = 2.0 * D * std::cos(mid) * s1;
-    return;
-  }
-  // star: x = cx + r(t) cos t, r(a) - r(a - d) by the same identities
-  double ra = 1.0, drr = 0.0;
-  for (size_t m = 0; m < am.size(); m++) {
-    ra += am[m] * std::cos(m * a) + bm[m] * std::sin(m * a);
-    const double sm = std::sin(0.5 * m * d);
-    drr += -2.0 * am[m] * std::sin(m * mid) * sm + 2.0 * bm[m] * std::cos(m * mid) * sm;
-  }
-  ra *= R;
-  drr *= R;
-  const double b = a - d;
-  dx = ra * (-2.0 * std::sin(mid) * s1) + drr * std::cos(b);
-  dy = ra * (2.0 * std::cos(mid) * s1) + drr * std::sin(b);
-}
-
 bool make_curve(int type, const double* prm, int nprm, Curve& c, std::string& err) {
   c = Curve();
   c.type = type;
@@ -106,8 +41,15 @@ bool make_curve(int type, const double* prm, int nprm, Curve& c, std::string& er
       err = "star curve: parameter list shorter than 4 + 2*nmodes";
       return false;
     }
-    c.am.assign(prm + 4, prm + 4 + nm);
-    c.bm.assign(prm + 4 + nm, prm + 4 + 2 * nm);
+    if (nm > kMaxStarModes) {
+      err = "star curve: more than 64 modes";
+      return false;
+    }
+    c.nm = nm;
+    for (int m = 0; m < nm; m++) {
+      c.am[m] = prm[4 + m];
+      c.bm[m] = prm[4 + nm + m];
+    }
     return true;
   }
   err = "unknown curve type or too few parameters";
@@ -124,47 +66,6 @@ static void fejer(int n, std::vector<double>& x, std::vector<double>& w) {
     double s = 0.0;
     for (int j = 1; j <= n / 2; j++) s += std::cos(j * M_PI * (2.0 * i + 1.0) / n) / (4.0 * j * j - 1.0);
     w[i] = (2.0 / n) * (1.0 - 2.0 * s);
-  }
-}
-
-// RP change of variables w(t) and w'(t), utils/Quadrature.hpp:73-102.
-static double vfun(double t, double p) {
-  const double u = t / M_PI - 1.0;
-  return (0.5 - 1.0 / p) * u * u * u + u / p + 0.5;
-}
-static double vpfun(double t, double p) {
-  const double u = t / M_PI - 1.0;
-  return (3.0 / M_PI) * (0.5 - 1.0 / p) * u * u + 1.0 / (p * M_PI);
-}
-static double wcov(double t, double p) {
-  const double a = std::pow(vfun(t, p), p), b = std::pow(vfun(2 * M_PI - t, p), p);
-  return 2.0 * M_PI * a / (a + b);
-}
-static double wpcov(double t, double p) {
-  const double v1 = vfun(t, p), v2 = vfun(2 * M_PI - t, p);
-  const double a = std::pow(v1, p), b = std::pow(v2, p);
-  return 2.0 * M_PI * p * vpfun(t, p) * std::pow(v1, p - 1.0) * std::pow(v2, p - 1.0) / ((a + b) * (a + b));
-}
-static int sgnd(double v) { return (0.0 < v) - (v < 0.0); }
-
-// xi_cov_{center,left,right} and derivatives, utils/Quadrature.hpp:118-147.
-// off = xi - xi_s, the offset from the singular location xi_s (alpha, -1 or
-// +1), returned without cancellation.
-static void rp_map(int loc, double t, double alpha, double p, double& xi, double& dxi, double& off) {
-  if (loc == 0) {
-    off = ((sgnd(t) - alpha) / M_PI) * wcov(M_PI * std::abs(t), p);
-    xi = alpha + off;
-    dxi = (1.0 - alpha * sgnd(t)) * wpcov(M_PI * std::abs(t), p);
-  } else if (loc == -1) {
-    const double u = (t + 1.0) / 2.0;
-    off = (2.0 / M_PI) * wcov(M_PI * std::abs(u), p);
-    xi = -1.0 + off;
-    dxi = wpcov(M_PI * std::abs(u), p) * sgnd(u);
-  } else {
-    const double u = (t - 1.0) / 2.0;
-    off = -((2.0 / M_PI) * wcov(M_PI * std::abs(u), p));
-    xi = 1.0 + off;
-    dxi = -wpcov(M_PI * std::abs(u), p) * sgnd(u);
   }
 }
 
@@ -261,96 +162,47 @@ bool build_geometry(const Curve& c, const double* lims, int npatches, int Np, do
 }
 
 // ------------------------------------------------------------ correction ---
-// One column of the precomputation matrix: Np Chebyshev moments of the kernel
-// against the patch, integrated with the 40-node RP rule around tsing
-// (single_patch_point_compute_precomputations, ForwardMap.tpp:287-389).
-// The target is the curve point at parameter ttgt; the separation x - y is
-// formed from parameter differences (Curve::diff), so the double-layer
-// numerator (x - y).n_y keeps full relative accuracy as y -> x.
-static void rp_column(const Curve& c, int F, double k, double eta, double p, double t1, double t2,
-                      double tsing, double ttgt, int Np, const std::vector<double>& fn,
-                      const std::vector<double>& fw, zd* out) {
-  const double alpha = ((1.0 - -1.0) / (t2 - t1)) * (tsing - t1) + -1.0;
-  int loc = 0;
-  if (std::abs(-1.0 - alpha) < 1e-5) loc = -1;
-  else if (std::abs(1.0 - alpha) < 1e-5) loc = 1;
-  for (int n = 0; n < Np; n++) out[n] = zd{0.0, 0.0};
-  const int nq = (int)fn.size();
-  // parameter of the singular location the change of variables is centred on
-  const double ts = loc == 0 ? tsing : (loc < 0 ? t1 : t2);
-  double tt = ttgt - ts;  // target offset from it (wrapped to (-pi, pi])
-  if (tt > M_PI) tt -= 2.0 * M_PI;
-  if (tt < -M_PI) tt += 2.0 * M_PI;
-  for (int i = 0; i < nq; i++) {
-    double xi, dxi, off;
-    rp_map(loc, fn[i], alpha, p, xi, dxi, off);
-    const double g = ((t2 - t1) / 2.0) * (xi + 1.0) + t1;
-    const double n0 = c.yp(g), n1 = -c.xp(g);
-    const double cj = std::sqrt(n0 * n0 + n1 * n1);
-    double dx, dy;
-    c.diff(ttgt, tt - ((t2 - t1) / 2.0) * off, dx, dy);
-    const zd gr = green_d_rt(F, dx, dy, n0 / cj, n1 / cj, k, eta);
-    const double s = ((t2 - t1) / 2.0) * cj * dxi * fw[i];
-    const zd v{gr.x * s, gr.y * s};
-    // Chebyshev moments T_n(xi), n < Np (ForwardMap.tpp:373-385)
-    double tm = 1.0, tc = xi;
-    out[0] = cadd(out[0], v);
-    out[1] = cadd(out[1], cscale(v, tc));
-    for (int n = 2; n < Np; n++) {
-      const double tn = 2.0 * xi * tc - tm;
-      out[n] = cadd(out[n], cscale(v, tn));
-      tm = tc;
-      tc = tn;
+void rp_rule(std::vector<double>& fn, std::vector<double>& fw) { fejer(40, fn, fw); }
+
+void rp_columns(const Geometry& g, std::vector<RpColumn>& cols, std::vector<long long>& patch_col_off) {
+  const int Np = g.Np;
+  cols.clear();
+  patch_col_off.assign(g.npatches + 1, 0);
+  for (int pi = 0; pi < g.npatches; pi++) {
+    patch_col_off[pi] = (long long)cols.size();
+    for (int j = 0; j < Np; j++) {
+      const long long i = (long long)pi * Np + j;
+      cols.push_back(RpColumn{pi, g.t[i], g.t[i]});
     }
+    for (size_t j = 0; j < g.ns_idx[pi].size(); j++)
+      cols.push_back(RpColumn{pi, g.ns_t[pi][j], g.t[g.ns_idx[pi][j]]});
   }
+  patch_col_off[g.npatches] = (long long)cols.size();
 }
 
-bool build_correction(const Geometry& g, const Curve& c, int F, double k, double eta, double p,
-                      Correction& out, std::string& err) {
+std::vector<double> rp_fold_table(const Geometry& g) {
   const int Np = g.Np;
-  std::vector<double> fn, fw;
-  fejer(40, fn, fw);  // num_ns = 40 (ForwardMap.hpp:70)
-  // Chebyshev-coefficient map of Cheb1D::interp_1d<Np> (Chebyshev1d.hpp:134-163):
-  // c_n = (alpha_n / Np) sum_k rho_k T_n(x_k), x_k the Fejer nodes, alpha_0 = 1 else 2.
-  std::vector<double> T(Np * Np);
+  std::vector<double> T(Np * Np), F(Np * Np);
   for (int kk = 0; kk < Np; kk++) {
     const double x = g.fejer_nodes[kk];
     T[0 * Np + kk] = 1.0;
     T[1 * Np + kk] = x;
     for (int n = 2; n < Np; n++) T[n * Np + kk] = 2.0 * x * T[(n - 1) * Np + kk] - T[(n - 2) * Np + kk];
   }
-  // folded rows W_p[col][k] = sum_n (alpha_n/Np) T_n(x_k) M_p[n][col]
-  std::vector<std::vector<zd>> Wp(g.npatches);
-#pragma omp parallel for schedule(dynamic, 16)
-  for (int pi = 0; pi < g.npatches; pi++) {
-    const int ncol = Np + (int)g.ns_idx[pi].size();
-    std::vector<zd> M((size_t)ncol * Np);
-    for (int j = 0; j < Np; j++) {
-      const long long i = (long long)pi * Np + j;
-      rp_column(c, F, k, eta, p, g.t1[pi], g.t2[pi], g.t[i], g.t[i], Np, fn, fw, &M[(size_t)j * Np]);
-    }
-    for (size_t j = 0; j < g.ns_idx[pi].size(); j++) {
-      const long long q = g.ns_idx[pi][j];
-      rp_column(c, F, k, eta, p, g.t1[pi], g.t2[pi], g.ns_t[pi][j], g.t[q], Np, fn, fw,
-                &M[(size_t)(Np + j) * Np]);
-    }
-    std::vector<zd>& W = Wp[pi];
-    W.assign((size_t)ncol * Np, zd{0.0, 0.0});
-    for (int col = 0; col < ncol; col++)
-      for (int kk = 0; kk < Np; kk++) {
-        zd s{0.0, 0.0};
-        for (int n = 0; n < Np; n++) {
-          const double a = (n == 0 ? 1.0 : 2.0) / Np * T[n * Np + kk];
-          s = cadd(s, cscale(M[(size_t)col * Np + n], a));
-        }
-        W[(size_t)col * Np + kk] = s;
-      }
-  }
+  for (int n = 0; n < Np; n++)
+    for (int kk = 0; kk < Np; kk++) F[n * Np + kk] = (n == 0 ? 1.0 : 2.0) / Np * T[n * Np + kk];
+  return F;
+}
+
+void assemble_correction(const Geometry& g, const std::vector<zd>& W, const std::vector<long long>& patch_col_off,
+                         Correction& out) {
+  const int Np = g.Np;
   // target-major CSR: own patch first, then near-singular patches in patch order
-  std::vector<std::vector<std::pair<int, int>>> rows(g.N);  // (patch, column)
-  for (long long i = 0; i < g.N; i++) rows[i].push_back({(int)(i / Np), (int)(i % Np)});
+  std::vector<std::vector<std::pair<int, long long>>> rows(g.N);  // (patch, global column)
+  for (long long i = 0; i < g.N; i++) rows[i].push_back({(int)(i / Np), patch_col_off[i / Np] + i % Np});
   for (int pi = 0; pi < g.npatches; pi++)
-    for (size_t j = 0; j < g.ns_idx[pi].size(); j++) rows[g.ns_idx[pi][j]].push_back({pi, Np + (int)j});
+    for (size_t j = 0; j < g.ns_idx[pi].size(); j++)
+      rows[g.ns_idx[pi][j]].push_back({pi, patch_col_off[pi] + Np + (long long)j});
   out.ptr.assign(g.N + 1, 0);
   for (long long i = 0; i < g.N; i++) out.ptr[i + 1] = out.ptr[i] + (long long)rows[i].size();
   out.patch.resize(out.ptr[g.N]);
@@ -359,10 +211,38 @@ bool build_correction(const Geometry& g, const Curve& c, int F, double k, double
     for (size_t e = 0; e < rows[i].size(); e++) {
       const long long ee = out.ptr[i] + (long long)e;
       out.patch[ee] = rows[i][e].first;
-      const auto& W = Wp[rows[i][e].first];
       std::copy(W.begin() + (size_t)rows[i][e].second * Np, W.begin() + (size_t)(rows[i][e].second + 1) * Np,
                 out.w.begin() + (size_t)ee * Np);
     }
+}
+
+bool build_correction(const Geometry& g, const Curve& c, int F, double k, double eta, double p,
+                      Correction& out, std::string& err) {
+  const int Np = g.Np;
+  std::vector<double> fn, fw;
+  rp_rule(fn, fw);
+  const std::vector<double> Tf = rp_fold_table(g);
+  std::vector<RpColumn> cols;
+  std::vector<long long> off;
+  rp_columns(g, cols, off);
+  // folded columns W[col][k] = sum_n Tf[n][k] M[col][n]
+  std::vector<zd> W(cols.size() * Np);
+#pragma omp parallel
+  {
+    std::vector<zd> M(Np);
+#pragma omp for schedule(dynamic, 64)
+    for (long long col = 0; col < (long long)cols.size(); col++) {
+      const RpColumn& rc = cols[col];
+      rp_column(c, F, k, eta, p, g.t1[rc.patch], g.t2[rc.patch], rc.tsing, rc.ttgt, Np, fn.data(), fw.data(),
+                (int)fn.size(), M.data());
+      for (int kk = 0; kk < Np; kk++) {
+        zd s{0.0, 0.0};
+        for (int n = 0; n < Np; n++) s = cadd(s, cscale(M[n], Tf[n * Np + kk]));
+        W[(size_t)col * Np + kk] = s;
+      }
+    }
+  }
+  assemble_correction(g, W, off, out);
   (void)err;
   return true;
 }

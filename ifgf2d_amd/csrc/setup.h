@@ -11,27 +11,11 @@
 #include <string>
 #include <vector>
 
+#include "rp.h"
 #include "zmath.h"
 
 namespace ifgf {
 
-// Closed parametrised curve, t in [0, 2 pi) (Geometry/Curve.hpp:21-146).
-//   type 0 circle (cx, cy, R); 1 kite x = A cos t + B cos 2t + C, y = D sin t;
-//   2 star (cx, cy, R, nmodes, a[0..nmodes), b[0..nmodes)),
-//     r(t) = R (1 + sum_m a_m cos mt + b_m sin mt).
-struct Curve {
-  int type = 0;
-  double cx = 0, cy = 0, R = 1;
-  double A = 1, B = 0.65, C = -0.65, D = 1.5;
-  std::vector<double> am, bm;
-  double x(double t) const;
-  double y(double t) const;
-  double xp(double t) const;
-  double yp(double t) const;
-  // x(a) - x(a - d), y(a) - y(a - d) from trigonometric difference identities,
-  // accurate to rounding relative to the (small) difference itself.
-  void diff(double a, double d, double& dx, double& dy) const;
-};
 bool make_curve(int type, const double* prm, int nprm, Curve& c, std::string& err);
 
 // Nystrom discretisation in patch order (ForwardMap.tpp:37-222).
@@ -57,6 +41,22 @@ struct Correction {
 };
 bool build_correction(const Geometry& g, const Curve& c, int F, double k, double eta, double p,
                       Correction& out, std::string& err);
+// The RP columns of every patch in the order the correction uses them: the
+// patch's own Np nodes (tsing = ttgt = the node's parameter), then its
+// near-singular points (tsing = closest parameter, ttgt = the point's).
+struct RpColumn {
+  int patch;
+  double tsing, ttgt;
+};
+void rp_columns(const Geometry& g, std::vector<RpColumn>& cols, std::vector<long long>& patch_col_off);
+// Chebyshev-coefficient map folded into the columns: Tfold[n * Np + k] =
+// (alpha_n / Np) T_n(x_k) (Cheb1D::interp_1d<Np>, Chebyshev1d.hpp:134-163).
+std::vector<double> rp_fold_table(const Geometry& g);
+// Target-major CSR of the folded columns W (patch_col_off[p] * Np + col * Np + k).
+void assemble_correction(const Geometry& g, const std::vector<zd>& W, const std::vector<long long>& patch_col_off,
+                         Correction& out);
+// 40-node Fejer rule of the RP integrals (num_ns, ForwardMap.hpp:70).
+void rp_rule(std::vector<double>& fn, std::vector<double>& fw);
 
 struct LevelInfo {
   int level = 0;

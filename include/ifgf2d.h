@@ -60,6 +60,9 @@ extern "C" {
                                    permutation is checked to equal the host's stable sort */
 #define IFGF_PLAN_OVERLAP 16   /* run the stored near field on a second stream beside the leaf
                                   operator and the level pushes (IFGF_PLAN_AUTOTUNE also tries it) */
+#define IFGF_PLAN_DEVICE_PRECOMP 32 /* evaluate the rectangular-polar singular / near-singular
+                                       correction (ForwardMap.tpp:287-423) on the GPU, one thread
+                                       per column, instead of on the host (Np <= 16) */
 
 /* apply flags */
 #define IFGF_PTR_DEVICE 1     /* x / y are device pointers; the call is asynchronous on stream */
