@@ -12,7 +12,8 @@ plan_setup_s); the on-the-fly variant that re-evaluates them in every
 application is timed too (on_the_fly_matvecs_per_s). After the timed steps, one GMRES(50) solve to 1e-8 for the plane wave
 d = (cos pi/4, sin pi/4) gives the time to solve. The mixed-precision operator
 (FP32 cone work) and its FGMRES solve with FP64 refinement are timed beside it,
-and so are 64-column applications through the batched kernels.
+and so are 64-column applications through the batched kernels and sixteen random
+star obstacles applied one after another and as one captured group launch.
 
 Prints ONE JSON line on stdout; progress goes to stderr.
 """
@@ -81,6 +82,7 @@ def main():
     ap.add_argument("--no-on-the-fly", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-group", action="store_true")
     ap.add_argument("--workload", choices=("kite", "star"), default="kite",
                     help="kite: the 64k kite at k=200 (default); star: the 256k star r=1+0.3cos5t at k=500")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -277,6 +279,38 @@ def main():
             del fb
         del Xb, Yb
 
+    # sixteen seeded random star obstacles at k=100 (~10 nodes per wavelength):
+    # their forward maps one after another against one captured group launch
+    group = None
+    if not args.no_group:
+        maps = [I.ForwardMap(**kw_, autotune=True) for kw_ in configs.random_stars(16, k=100.0, seed=0)]
+        gx = [torch.from_numpy(rng.standard_normal(m.N) + 1j * rng.standard_normal(m.N)).to(dev) for m in maps]
+        gy = [torch.empty_like(v) for v in gx]
+        torch.cuda.synchronize()
+        reps = max(5, args.steps // 10)
+        for m, a_, b_ in zip(maps, gx, gy):
+            m.apply_device(a_.data_ptr(), b_.data_ptr())
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            for m, a_, b_ in zip(maps, gx, gy):
+                m.apply_device(a_.data_ptr(), b_.data_ptr())
+        torch.cuda.synchronize()
+        t_seq = (time.perf_counter() - t0) / reps
+        grp = I.ObstacleGroup(maps, [v.data_ptr() for v in gx], [v.data_ptr() for v in gy])
+        grp.launch()
+        grp.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            grp.launch()
+        grp.synchronize()
+        t_grp = (time.perf_counter() - t0) / reps
+        group = {"obstacles": len(maps), "N": [m.N for m in maps],
+                 "sequential_ms_per_matvec": round(t_seq * 1e3, 4), "group_ms_per_matvec": round(t_grp * 1e3, 4)}
+        log(f"16 random stars: sequential {t_seq * 1e3:.3f} ms, one group launch {t_grp * 1e3:.3f} ms per matvec")
+        grp.close()
+        del maps, gx, gy
+
     cpu = None if args.no_cpu_baseline else cpu_baseline(
         kw, x_host, "64,000-point kite at k=200" if args.workload == "kite" else "256,000-point star at k=500")
 
@@ -300,6 +334,7 @@ def main():
         "solve": solve,
         "mixed_precision": mixed,
         "batched_rhs": batched,
+        "obstacle_group": group,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -23,7 +23,7 @@ import os
 import numpy as np
 
 __all__ = [
-    "ForwardMap", "LinearOperator", "gmres", "hankel01", "morton_sort", "IFGFError",
+    "ForwardMap", "LinearOperator", "ObstacleGroup", "gmres", "hankel01", "morton_sort", "IFGFError",
     "circle", "kite", "star", "equal_parameter_lims", "SINGLE_LAYER", "DOUBLE_LAYER", "COMBINED",
 ]
 
@@ -71,7 +71,8 @@ EXPORTS = [
     "ifgf_plan_points", "ifgf_plan_stats", "ifgf_plan_sorting", "ifgf_plan_level_cones",
     "ifgf_plan_correction", "ifgf_apply", "ifgf_apply_direct", "ifgf_propagate", "ifgf_gmres",
     "ifgf_fgmres", "ifgf_gmres_host", "ifgf_hankel01", "ifgf_plan_profile", "ifgf_plan_stage_times",
-    "ifgf_morton_sort", "ifgf_apply_graph",
+    "ifgf_morton_sort", "ifgf_apply_graph", "ifgf_group_create", "ifgf_group_launch",
+    "ifgf_group_synchronize", "ifgf_group_destroy",
 ]
 
 _LIB = None
@@ -104,6 +105,12 @@ def lib():
                                  ctypes.c_longlong, ctypes.c_int, ctypes.c_void_p]
         L.ifgf_apply_graph.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
                                        ctypes.c_longlong, ctypes.c_int, ctypes.c_void_p]
+        L.ifgf_group_create.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int,
+                                        ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p),
+                                        ctypes.POINTER(ctypes.c_void_p)]
+        L.ifgf_group_launch.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        L.ifgf_group_synchronize.argtypes = [ctypes.c_void_p]
+        L.ifgf_group_destroy.argtypes = [ctypes.c_void_p]
         L.ifgf_apply_direct.argtypes = [ctypes.c_void_p, _dp, _lp, ctypes.c_longlong, _dp]
         L.ifgf_propagate.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_longlong, _dp, _dp]
         L.ifgf_gmres.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int, ctypes.c_double,
@@ -374,6 +381,44 @@ class ForwardMap:
         def mapping(x, out):
             out[:] = self.compute_Ax_acc(x)
         return LinearOperator(self.N, self.N, mapping)
+
+
+class ObstacleGroup:
+    """Several obstacles' forward maps as one captured launch (ifgf_group_*):
+    maps[i] applied to the device vector x_ptrs[i] into y_ptrs[i]."""
+
+    def __init__(self, maps, x_ptrs, y_ptrs):
+        n = len(maps)
+        self.maps = list(maps)  # the plans must outlive the group
+        P = (ctypes.c_void_p * n)(*[m._h.value for m in maps])
+        X = (ctypes.c_void_p * n)(*x_ptrs)
+        Y = (ctypes.c_void_p * n)(*y_ptrs)
+        h = ctypes.c_void_p()
+        rc = lib().ifgf_group_create(P, n, X, Y, ctypes.byref(h))
+        if rc != IFGF_OK:
+            raise IFGFError(f"ifgf_group_create failed ({rc}): {lib().ifgf_last_error().decode()}")
+        self._h = h
+
+    def launch(self, stream=None):
+        rc = lib().ifgf_group_launch(self._h, ctypes.c_void_p(stream) if stream else None)
+        if rc != IFGF_OK:
+            raise IFGFError(lib().ifgf_last_error().decode())
+
+    def synchronize(self):
+        rc = lib().ifgf_group_synchronize(self._h)
+        if rc != IFGF_OK:
+            raise IFGFError(lib().ifgf_last_error().decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().ifgf_group_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class LinearOperator:

@@ -29,6 +29,24 @@ def star_k500(**kw):
                 k=500.0, nlevels=12, p=6.0, **kw), (1.0, 0.0)
 
 
+def random_stars(count=16, k=100.0, seed=0, nodes_per_wavelength=10, **kw):
+    """Seeded random star obstacles r(t) = R (1 + sum_{m=2..8} a_m cos mt + b_m sin mt),
+    a_m, b_m ~ U(-0.05, 0.05), R ~ U(0.5, 4), at wavenumber k: Np = 10 Fejer nodes per
+    patch and patches of about one wavelength (~10 nodes per wavelength), nlevels for
+    ~30 points per leaf box. Returns a list of ForwardMap keyword dicts."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        R = rng.uniform(0.5, 4.0)
+        a = [0.0, 0.0] + list(rng.uniform(-0.05, 0.05, 7))
+        b = [0.0, 0.0] + list(rng.uniform(-0.05, 0.05, 7))
+        npatch = int(np.ceil(1.1 * R * k * 10 / nodes_per_wavelength))  # ~ perimeter / wavelength
+        nlev = max(3, int(np.round(np.log2(npatch * 10 / 100))) + 1)
+        out.append(dict(curve=star(R=R, a=a, b=b), patch_lims=equal_parameter_lims(npatch), k=k,
+                        nlevels=nlev, p=6.0, **kw))
+    return out
+
+
 def scaled(curve, npatches, k, nlevels, **kw):
     """Any curve at any size (used for reduced-size parity cases)."""
     return dict(curve=curve, patch_lims=equal_parameter_lims(npatches), k=k, nlevels=nlevels, p=6.0,

@@ -1060,6 +1060,96 @@ int ifgf_apply_graph(ifgf_plan* P, const double* x, double* y, int nrhs, long lo
   return IFGF_OK;
 }
 
+}  // extern "C"
+
+// A group of plans (obstacles) applied as one captured graph.
+struct ifgf_group {
+  std::vector<ifgf_plan*> plans;
+  hipStream_t stream = nullptr;
+  std::vector<hipEvent_t> events;  // per plan: its branch's end; last: the fork
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::string err;
+  ~ifgf_group() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (auto& e : events)
+      if (e) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+extern "C" {
+
+int ifgf_group_create(ifgf_plan* const* plans, int nplans, const double* const* x, double* const* y,
+                      ifgf_group** out) {
+  if (!plans || nplans < 1 || !x || !y || !out) return fail(nullptr, IFGF_ERR_ARG, "invalid argument");
+  *out = nullptr;
+  for (int i = 0; i < nplans; i++) {
+    if (!plans[i] || !x[i] || !y[i]) return fail(nullptr, IFGF_ERR_ARG, "null plan or vector");
+    if (!plans[i]->on_device) return fail(nullptr, IFGF_ERR_NO_DEVICE, "plan was built host-only");
+    if (plans[i]->profile) return fail(nullptr, IFGF_ERR_ARG, "stage profiling is on (its events cannot be captured)");
+    for (int j = 0; j < i; j++)
+      if (plans[j] == plans[i]) return fail(nullptr, IFGF_ERR_ARG, "a plan appears twice in the group");
+  }
+  std::unique_ptr<ifgf_group> G(new ifgf_group);
+  try {
+    HIP_TRY(hipStreamCreate(&G->stream));
+    for (int i = 0; i < nplans; i++) HIP_TRY(hipStreamSynchronize(plans[i]->stream));
+    // one branch per obstacle: each plan's application is captured on that
+    // plan's own stream, forked from and joined back into the group's stream,
+    // so the obstacles run side by side as they do when applied eagerly
+    G->events.resize(nplans + 1, nullptr);
+    for (auto& e : G->events) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    HIP_TRY(hipStreamBeginCapture(G->stream, hipStreamCaptureModeThreadLocal));
+    int rc = IFGF_OK;
+    hipError_t ec = hipEventRecord(G->events[nplans], G->stream);
+    for (int i = 0; i < nplans && rc == IFGF_OK && ec == hipSuccess; i++) {
+      ec = hipStreamWaitEvent(plans[i]->stream, G->events[nplans], 0);
+      if (ec == hipSuccess) rc = apply_device(plans[i], (const zd*)x[i], (zd*)y[i], plans[i]->stream, false);
+      if (ec == hipSuccess && rc == IFGF_OK) ec = hipEventRecord(G->events[i], plans[i]->stream);
+      if (ec == hipSuccess && rc == IFGF_OK) ec = hipStreamWaitEvent(G->stream, G->events[i], 0);
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t ee = hipStreamEndCapture(G->stream, &g);
+    if (ec == hipSuccess) ec = ee;
+    if (rc != IFGF_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return fail(nullptr, rc, g_last_error);
+    }
+    HIP_TRY(ec);
+    G->graph = g;
+    HIP_TRY(hipGraphInstantiate(&G->exec, g, nullptr, nullptr, 0));
+    G->plans.assign(plans, plans + nplans);
+  } catch (std::exception& e) {
+    return fail(nullptr, IFGF_ERR_HIP, e.what());
+  }
+  *out = G.release();
+  return IFGF_OK;
+}
+
+int ifgf_group_launch(ifgf_group* G, void* stream) {
+  if (!G) return fail(nullptr, IFGF_ERR_ARG, "null group");
+  try {
+    HIP_TRY(hipGraphLaunch(G->exec, stream ? (hipStream_t)stream : G->stream));
+  } catch (std::exception& e) {
+    return fail(nullptr, IFGF_ERR_HIP, e.what());
+  }
+  return IFGF_OK;
+}
+
+int ifgf_group_synchronize(ifgf_group* G) {
+  if (!G) return fail(nullptr, IFGF_ERR_ARG, "null group");
+  try {
+    HIP_TRY(hipStreamSynchronize(G->stream));
+  } catch (std::exception& e) {
+    return fail(nullptr, IFGF_ERR_HIP, e.what());
+  }
+  return IFGF_OK;
+}
+
+void ifgf_group_destroy(ifgf_group* G) { delete G; }
+
 int ifgf_apply_direct(ifgf_plan* P, const double* x, const long long* targets, long long nt, double* out) {
   if (!P || !x || !targets || !out) return fail(P, IFGF_ERR_ARG, "null argument");
   if (!P->on_device) return fail(P, IFGF_ERR_NO_DEVICE, "plan was built host-only");

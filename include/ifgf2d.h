@@ -70,6 +70,7 @@ extern "C" {
                                  mixed-precision one (no effect on an IFGF_FP64 plan) */
 
 typedef struct ifgf_plan ifgf_plan;
+typedef struct ifgf_group ifgf_group;
 
 /* Discretised closed curve: equal-weight Fejer first-rule nodes on each patch
  * [patch_lims[p], patch_lims[p+1]] (ForwardMap ctor, ForwardMap.tpp:15-222). */
@@ -178,6 +179,20 @@ int ifgf_apply(ifgf_plan* plan, const double* x, double* y, int nrhs, long long 
  * while stage profiling is on. */
 int ifgf_apply_graph(ifgf_plan* plan, const double* x, double* y, int nrhs, long long ld, int flags,
                      void* stream);
+
+/* Several obstacles' forward maps as one launch (no reference counterpart;
+ * the reference handles one ForwardMap at a time): captures the device
+ * applications y[i] = A_i x[i] of nplans distinct plans (device pointers,
+ * n_i complex values each, one column) into one HIP graph. Every
+ * ifgf_group_launch then runs all of them with one launch on stream (NULL =
+ * the group's own stream), asynchronously; x[i] / y[i] contents may change
+ * between launches, the pointers may not. Plans must outlive the group and
+ * are not applied elsewhere while a launch runs. */
+int ifgf_group_create(ifgf_plan* const* plans, int nplans, const double* const* x, double* const* y,
+                      ifgf_group** group);
+int ifgf_group_launch(ifgf_group* group, void* stream);
+int ifgf_group_synchronize(ifgf_group* group);  /* waits for the group's own stream */
+void ifgf_group_destroy(ifgf_group* group);
 
 /* Unaccelerated operator on a target subset: replaces compute_Ax_unacc
  * (ForwardMap.tpp:438-540) evaluated at targets[0..nt). Host pointers. */

@@ -86,6 +86,47 @@ def test_forward_map_parity_on_the_fly(name, curve, npatch, k, nlev, F, Ps, Pang
     assert e <= PARITY_TOL
 
 
+@pytest.mark.parametrize("Np,npatch,F", [(6, 400, I.COMBINED), (16, 250, I.DOUBLE_LAYER), (12, 300, I.SINGLE_LAYER)],
+                         ids=["Np6-cfie", "Np16-dlp", "Np12-slp"])
+def test_forward_map_parity_other_patch_orders(Np, npatch, F):
+    """The reference's template parameter Np (Fejer nodes per patch,
+    ForwardMap<Np, ...>) other than 10: the correction rows, the intensity
+    weights and the IFGF all follow Np (kite, k = 40)."""
+    curve = I.kite()
+    lims = I.equal_parameter_lims(npatch)
+    delta = I.default_delta(curve, lims)
+    fm = I.ForwardMap(curve, lims, 40.0, 6, Np=Np, formulation=F, delta=delta, p=6.0)
+    om = O.ForwardMap(curve[0], curve[1], lims, Np=Np, formulation=F, k=40.0, eta=40.0, delta=delta, p=6.0,
+                      nlevels=6)
+    x = density(fm.N, 70)
+    e = rel(fm.compute_Ax_acc(x), om.apply(x))
+    print(f"Np={Np} N={fm.N}: rel diff {e:.3e}", flush=True)
+    assert e <= PARITY_TOL
+
+
+def test_random_star_obstacles_parity():
+    """Seeded random stars r = R(1 + sum_{m=2..8} a_m cos mt + b_m sin mt),
+    a_m, b_m ~ U(-0.05, 0.05), R ~ U(0.5, 4) at k = 100 (the multi-obstacle
+    batch's geometry), about 10 nodes per wavelength: GPU vs oracle."""
+    rng = np.random.default_rng(2024)
+    for _ in range(3):
+        R = rng.uniform(0.5, 4.0)
+        a = [0.0, 0.0] + list(rng.uniform(-0.05, 0.05, 7))
+        b = [0.0, 0.0] + list(rng.uniform(-0.05, 0.05, 7))
+        curve = I.star(R=R, a=a, b=b)
+        npatch = int(np.ceil(2 * np.pi * R * 1.1 * 100.0 / (2 * np.pi)))  # ~ perimeter / wavelength
+        nlev = max(3, int(np.round(np.log2(npatch * 10 / 100))) + 1)  # ~30 points per leaf box
+        lims = I.equal_parameter_lims(npatch)
+        delta = I.default_delta(curve, lims)
+        fm = I.ForwardMap(curve, lims, 100.0, nlev, delta=delta, p=6.0)
+        om = O.ForwardMap(curve[0], curve[1], lims, formulation=O.COMBINED, k=100.0, eta=100.0, delta=delta, p=6.0,
+                          nlevels=nlev)
+        x = density(fm.N, 71)
+        e = rel(fm.compute_Ax_acc(x), om.apply(x))
+        print(f"random star R={R:.2f} N={fm.N} nlevels={nlev}: rel diff {e:.3e}", flush=True)
+        assert e <= PARITY_TOL
+
+
 def test_apply_is_deterministic_and_linear():
     fm, _ = build_pair(I.kite(), 400, 40.0, 6, I.COMBINED, 5, 5)
     x1, x2 = density(fm.N, 3), density(fm.N, 4)
