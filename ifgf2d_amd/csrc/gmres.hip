@@ -7,7 +7,12 @@
 // re-orthogonalisation pass (CGS2), all in FP64. The projections stay on the
 // device; one (j+2)-value copy per iteration brings the new Hessenberg column
 // to the host, where the Givens rotations and the residual estimate
-// |w(k)| / ||b - A x0|| (opGMRES.hpp:207) are updated.
+// |w(k)| / ||b - A x0|| (opGMRES.hpp:207) are updated. An iteration is six
+// kernels besides the operator (each projection's partial sums are reduced
+// by the kernel that consumes them), and the host applies the rotations of
+// iteration j while the device already runs iteration j+1 (which needs only
+// the new basis vector): at convergence that one speculative iteration is
+// discarded.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -21,17 +26,16 @@ using namespace ifgf;
 
 namespace {
 
-constexpr int kChunks = 512;
+constexpr int kDot = 64;       // partial sums per projection
+constexpr int kMaxRestart = 256;
 
-// part[col * kChunks + blockIdx.x] = sum over this block's rows of conj(V[:,col]) * w
+// part[col * kDot + blockIdx.x] = sum over this block's rows of conj(V[:,col]) * w
 __global__ void __launch_bounds__(256)
-    k_mdot(long long n, const zd* __restrict__ V, long long ldv, const zd* __restrict__ w,
-           zd* __restrict__ part) {
+    k_dots(long long n, const zd* __restrict__ V, long long ldv, const zd* __restrict__ w, zd* __restrict__ part) {
   const int col = blockIdx.y;
   const zd* v = V + (size_t)col * ldv;
   zd acc{0.0, 0.0};
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (long long)gridDim.x * blockDim.x) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
     const zd a = v[i], b = w[i];
     acc.x = fma(a.x, b.x, fma(a.y, b.y, acc.x));
     acc.y = fma(a.x, b.y, fma(-a.y, b.x, acc.y));
@@ -47,55 +51,66 @@ __global__ void __launch_bounds__(256)
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) part[(size_t)col * kChunks + blockIdx.x] = zd{sx[0], sy[0]};
+  if (threadIdx.x == 0) part[(size_t)col * kDot + blockIdx.x] = zd{sx[0], sy[0]};
 }
 
-// h[col] (+)= sum of the chunk partials; one block per column
-__global__ void __launch_bounds__(256) k_reduce(const zd* __restrict__ part, zd* __restrict__ h, int accumulate) {
-  const int col = blockIdx.x;
-  __shared__ double sx[256], sy[256];
-  double ax = 0.0, ay = 0.0;
-  for (int c = threadIdx.x; c < kChunks; c += 256) {
-    ax += part[(size_t)col * kChunks + c].x;
-    ay += part[(size_t)col * kChunks + c].y;
-  }
-  sx[threadIdx.x] = ax;
-  sy[threadIdx.x] = ay;
-  __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) {
-      sx[threadIdx.x] += sx[threadIdx.x + s];
-      sy[threadIdx.x] += sy[threadIdx.x + s];
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    if (accumulate) h[col] = cadd(h[col], zd{sx[0], sy[0]});
-    else h[col] = zd{sx[0], sy[0]};
-  }
+// Sum of the kDot partials of column c, in a fixed order (the same in every
+// block that needs it): four interleaved accumulators, then combined.
+__device__ inline zd sum_partials(const zd* __restrict__ part, int c) {
+  zd a[4] = {{0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}, {0.0, 0.0}};
+  for (int b = 0; b < kDot; b += 4)
+#pragma unroll
+    for (int q = 0; q < 4; q++) a[q] = cadd(a[q], part[(size_t)c * kDot + b + q]);
+  return cadd(cadd(a[0], a[1]), cadd(a[2], a[3]));
 }
 
-// w -= sign * sum_{j < ncol} V[:,j] h[j]
+// Every block reduces the projections h[c] = sum_b part[c][b] (c < ncol);
+// block 0 stores them into hio (or adds them, for the second Gram-Schmidt
+// pass); then each thread updates its row: w -= V h.
 __global__ void __launch_bounds__(256)
-    k_update(long long n, int ncol, const zd* __restrict__ V, long long ldv, const zd* __restrict__ h,
-             double sign, zd* __restrict__ w) {
+    k_project_out(long long n, int ncol, const zd* __restrict__ V, long long ldv, const zd* __restrict__ part,
+                  zd* __restrict__ hio, int accumulate, zd* __restrict__ w) {
+  __shared__ zd hs[kMaxRestart + 1];
+  for (int c = threadIdx.x; c < ncol; c += blockDim.x) hs[c] = sum_partials(part, c);
+  __syncthreads();
+  if (blockIdx.x == 0)
+    for (int c = threadIdx.x; c < ncol; c += blockDim.x) hio[c] = accumulate ? cadd(hio[c], hs[c]) : hs[c];
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   zd acc = w[i];
-  for (int j = 0; j < ncol; j++) {
-    const zd c = cscale(h[j], -sign);
-    cfma(acc, c, V[(size_t)j * ldv + i]);
-  }
+  for (int c = 0; c < ncol; c++) cfma(acc, cscale(hs[c], -1.0), V[(size_t)c * ldv + i]);
   w[i] = acc;
 }
 
-// dst = src / sqrt(|nrm|) (nrm = ||src||^2 computed on the device)
+// dst = src / ||src|| with ||src||^2 = sum of the kDot partials (part from
+// k_dots(src, src)); block 0 stores ||src|| into *nrm.
 __global__ void __launch_bounds__(256)
-    k_normalize(long long n, const zd* __restrict__ src, const zd* __restrict__ nrm, zd* __restrict__ dst) {
+    k_normalize(long long n, const zd* __restrict__ src, const zd* __restrict__ part, zd* __restrict__ dst,
+                zd* __restrict__ nrm) {
+  __shared__ double s;
+  if (threadIdx.x == 0) s = sum_partials(part, 0).x;
+  __syncthreads();
+  const double nn = sqrt(s);
+  if (blockIdx.x == 0 && threadIdx.x == 0 && nrm) *nrm = zd{nn, 0.0};
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const double s = 1.0 / sqrt(nrm[0].x);
-  dst[i] = cscale(src[i], s);
+  dst[i] = cscale(src[i], 1.0 / nn);
+}
+
+// out[0] = sum of the kDot partials of column 0 (a squared norm)
+__global__ void __launch_bounds__(64) k_sum0(const zd* __restrict__ part, zd* __restrict__ out) {
+  if (threadIdx.x == 0) out[0] = sum_partials(part, 0);
+}
+
+// x -= sign * sum_{j < ncol} B[:,j] y[j]
+__global__ void __launch_bounds__(256)
+    k_update(long long n, int ncol, const zd* __restrict__ B, long long ldb, const zd* __restrict__ y, double sign,
+             zd* __restrict__ x) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  zd acc = x[i];
+  for (int j = 0; j < ncol; j++) cfma(acc, cscale(y[j], -sign), B[(size_t)j * ldb + i]);
+  x[i] = acc;
 }
 
 // r = b - r
@@ -132,16 +147,30 @@ void apply_givens(const Givens& g, std::complex<double>& a, std::complex<double>
   a = t;
 }
 
-// Device workspace of one GMRES cycle of restart m on vectors of length n.
+// Device workspace of one GMRES cycle of restart m on vectors of length n,
+// and two pinned host slots (with their events) for the Hessenberg columns of
+// consecutive iterations.
 struct Krylov {
   DevBuf<zd> V, w, tmp, part, h;
-  Krylov(long long n, int m) {
+  zd* hpin = nullptr;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  int m;
+  Krylov(long long n, int m_) : m(m_) {
     V.alloc((size_t)n * (m + 1));
     w.alloc(n);
     tmp.alloc(n);
-    part.alloc((size_t)kChunks * (m + 2));
-    h.alloc(2 * (m + 2));  // [0, m+2): projections; [m+2, 2m+4): second CGS pass
+    part.alloc((size_t)kDot * (m + 2));
+    h.alloc(m + 2);
+    HIP_TRY(hipHostMalloc((void**)&hpin, 2 * (m + 2) * sizeof(zd), hipHostMallocDefault));
+    for (auto& e : ev) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   }
+  ~Krylov() {
+    for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+    if (hpin) (void)hipHostFree(hpin);
+  }
+  Krylov(const Krylov&) = delete;
+  Krylov& operator=(const Krylov&) = delete;
 };
 
 struct Solver {
@@ -155,42 +184,45 @@ struct Solver {
     const int rc = apply_device(P, x, y, st, fp64);
     if (rc) throw std::runtime_error(P->err);
   }
-  // ||v||^2 into dst[0].x (device)
-  void norm2(Krylov& K, const zd* v, zd* dst) {
-    hipLaunchKernelGGL(k_mdot, dim3(kChunks, 1), dim3(256), 0, st, n, v, n, v, K.part.p);
-    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, st, K.part.p, dst, 0);
-  }
+  // ||v|| (synchronises)
   double norm(Krylov& K, const zd* v) {
-    norm2(K, v, K.h.p);
-    std::vector<std::complex<double>> hv;
-    fetch(K.h.p, 1, hv);
-    return std::sqrt(hv[0].real());
-  }
-  void fetch(const zd* src, int cnt, std::vector<std::complex<double>>& out) {
-    out.resize(cnt);
-    HIP_TRY(hipMemcpyAsync(out.data(), src, cnt * sizeof(zd), hipMemcpyDeviceToHost, st));
+    hipLaunchKernelGGL(k_dots, dim3(kDot, 1), dim3(256), 0, st, n, v, n, v, K.part.p);
+    hipLaunchKernelGGL(k_sum0, dim3(1), dim3(64), 0, st, K.part.p, K.h.p);
+    zd h0;
+    HIP_TRY(hipMemcpyAsync(&h0, K.h.p, sizeof(zd), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
+    return std::sqrt(h0.x);
+  }
+  // dst = src / ||src|| (asynchronous)
+  void normalize(Krylov& K, const zd* src, zd* dst, zd* nrm) {
+    hipLaunchKernelGGL(k_dots, dim3(kDot, 1), dim3(256), 0, st, n, src, n, src, K.part.p);
+    hipLaunchKernelGGL(k_normalize, dim3(nb), dim3(256), 0, st, n, src, K.part.p, dst, nrm);
   }
   // r = b - A x
   void residual(const zd* b, const zd* x, zd* r, bool fp64) {
     apply(x, r, fp64);
     hipLaunchKernelGGL(k_residual, dim3(nb), dim3(256), 0, st, n, b, r);
   }
-  // Appends column j+1 of the Arnoldi basis: w is orthogonalised against
-  // V[:, 0..j] by CGS2 and normalised into V[:, j+1]; returns the Hessenberg
-  // column h[0..j+1] on the host.
-  void arnoldi_step(Krylov& K, int m, int j, std::vector<std::complex<double>>& hv) {
-    hipLaunchKernelGGL(k_mdot, dim3(kChunks, j + 1), dim3(256), 0, st, n, K.V.p, n, K.w.p, K.part.p);
-    hipLaunchKernelGGL(k_reduce, dim3(j + 1), dim3(256), 0, st, K.part.p, K.h.p, 0);
-    hipLaunchKernelGGL(k_update, dim3(nb), dim3(256), 0, st, n, j + 1, K.V.p, n, K.h.p, 1.0, K.w.p);
-    hipLaunchKernelGGL(k_mdot, dim3(kChunks, j + 1), dim3(256), 0, st, n, K.V.p, n, K.w.p, K.part.p);
-    hipLaunchKernelGGL(k_reduce, dim3(j + 1), dim3(256), 0, st, K.part.p, K.h.p + m + 2, 0);
-    hipLaunchKernelGGL(k_update, dim3(nb), dim3(256), 0, st, n, j + 1, K.V.p, n, K.h.p + m + 2, 1.0, K.w.p);
-    hipLaunchKernelGGL(k_reduce, dim3(j + 1), dim3(256), 0, st, K.part.p, K.h.p, 1);  // h += h2
-    norm2(K, K.w.p, K.h.p + j + 1);
-    hipLaunchKernelGGL(k_normalize, dim3(nb), dim3(256), 0, st, n, K.w.p, K.h.p + j + 1, K.V.p + (size_t)(j + 1) * n);
-    fetch(K.h.p, j + 2, hv);
-    hv[j + 1] = std::sqrt(hv[j + 1].real());
+  // Orthogonalises w against V[:, 0..j] (CGS2), normalises it into
+  // V[:, j+1] and copies the Hessenberg column h[0..j+1] into pinned slot
+  // j % 2, recording its event. Asynchronous.
+  void arnoldi_enqueue(Krylov& K, int j) {
+    const int nc = j + 1;
+    hipLaunchKernelGGL(k_dots, dim3(kDot, nc), dim3(256), 0, st, n, K.V.p, n, K.w.p, K.part.p);
+    hipLaunchKernelGGL(k_project_out, dim3(nb), dim3(256), 0, st, n, nc, K.V.p, n, K.part.p, K.h.p, 0, K.w.p);
+    hipLaunchKernelGGL(k_dots, dim3(kDot, nc), dim3(256), 0, st, n, K.V.p, n, K.w.p, K.part.p);
+    hipLaunchKernelGGL(k_project_out, dim3(nb), dim3(256), 0, st, n, nc, K.V.p, n, K.part.p, K.h.p, 1, K.w.p);
+    normalize(K, K.w.p, K.V.p + (size_t)(j + 1) * n, K.h.p + j + 1);
+    zd* slot = K.hpin + (size_t)(j % 2) * (K.m + 2);
+    HIP_TRY(hipMemcpyAsync(slot, K.h.p, (j + 2) * sizeof(zd), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipEventRecord(K.ev[j % 2], st));
+  }
+  // waits for iteration j's Hessenberg column
+  void arnoldi_wait(Krylov& K, int j, std::vector<std::complex<double>>& hv) {
+    HIP_TRY(hipEventSynchronize(K.ev[j % 2]));
+    const zd* slot = K.hpin + (size_t)(j % 2) * (K.m + 2);
+    hv.resize(j + 2);
+    for (int i = 0; i < j + 2; i++) hv[i] = std::complex<double>(slot[i].x, slot[i].y);
   }
   // x += B y for the first kk columns of B (ld n)
   void update(Krylov& K, const zd* B, int kk, const std::vector<std::complex<double>>& y, zd* x) {
@@ -227,22 +259,27 @@ int gmres_core(Solver& S, Krylov& K, bool fp64, const zd* b, zd* x, int m, doubl
   int it = 0;
   std::vector<C> H((size_t)(m + 1) * m), g(m + 1), hv;
   std::vector<Givens> rot(m);
+  auto enqueue = [&](int j) {
+    S.apply(K.V.p + (size_t)j * n, K.w.p, fp64);
+    S.arnoldi_enqueue(K, j);
+  };
   while (!converged && it < maxit) {
-    // V[:,0] = r / beta (norm recomputed on the device from K.h[0])
-    S.norm2(K, K.tmp.p, K.h.p);
-    hipLaunchKernelGGL(k_normalize, dim3(S.nb), dim3(256), 0, S.st, n, K.tmp.p, K.h.p, K.V.p);
+    S.normalize(K, K.tmp.p, K.V.p, nullptr);  // V[:,0] = r / ||r||
     std::fill(g.begin(), g.end(), C(0.0));
     g[0] = beta;
+    const int jmax = std::min(m, maxit - it);
+    enqueue(0);
     int j = 0;
-    for (; j < m && it < maxit; j++) {
-      it++;
-      S.apply(K.V.p + (size_t)j * n, K.w.p, fp64);
-      S.arnoldi_step(K, m, j, hv);
+    for (; j < jmax; j++) {
+      if (j + 1 < jmax) enqueue(j + 1);  // needs only V[:, j+1]
+      S.arnoldi_wait(K, j, hv);
+      hv[j + 1] = std::sqrt(std::norm(hv[j + 1]));
       for (int i = 0; i < j; i++) apply_givens(rot[i], hv[i], hv[i + 1]);
       rot[j] = make_givens(hv[j], hv[j + 1]);
       apply_givens(rot[j], hv[j], hv[j + 1]);
       apply_givens(rot[j], g[j], g[j + 1]);
       for (int i = 0; i <= j; i++) H[(size_t)i * m + j] = hv[i];
+      it++;
       est = std::abs(g[j + 1]) / r0n;
       if (hist && it - 1 < hist_cap) hist[it - 1] = est;
       if (est < tol) {
@@ -251,6 +288,8 @@ int gmres_core(Solver& S, Krylov& K, bool fp64, const zd* b, zd* x, int m, doubl
         break;
       }
     }
+    // x += V y (after the speculative iteration, if one is in flight: it only
+    // wrote w, V[:, j+1] and the Hessenberg buffers)
     S.update(K, K.V.p, j, back_substitute(H, m, g, j), x);
     if (!converged && it < maxit) {
       S.residual(b, x, K.tmp.p, fp64);
@@ -268,6 +307,7 @@ extern "C" int ifgf_gmres(ifgf_plan* P, const double* rhs, double* x, int restar
                           int* iters, double* relres, double* hist, int hist_cap) {
   if (!P || !rhs || !x || !iters || !relres || restart < 1 || maxit < 1)
     return plan_fail(P, IFGF_ERR_ARG, "invalid argument");
+  if (restart > kMaxRestart) return plan_fail(P, IFGF_ERR_ARG, "restart above 256");
   if (!P->on_device) return plan_fail(P, IFGF_ERR_NO_DEVICE, "plan was built host-only");
   const long long n = P->geo.N;
   *iters = 0;
@@ -305,6 +345,7 @@ extern "C" int ifgf_fgmres(ifgf_plan* P, const double* rhs, double* x, int resta
   if (!P || !rhs || !x || !outer_iters || !inner_iters || !relres || restart < 1 || maxit < 1 || inner_restart < 1 ||
       !(inner_floor > 0.0))
     return plan_fail(P, IFGF_ERR_ARG, "invalid argument");
+  if (restart > kMaxRestart || inner_restart > kMaxRestart) return plan_fail(P, IFGF_ERR_ARG, "restart above 256");
   if (!P->on_device) return plan_fail(P, IFGF_ERR_NO_DEVICE, "plan was built host-only");
   using C = std::complex<double>;
   const long long n = P->geo.N;
@@ -334,8 +375,7 @@ extern "C" int ifgf_fgmres(ifgf_plan* P, const double* rhs, double* x, int resta
     std::vector<C> H((size_t)(m + 1) * m), g(m + 1), hv;
     std::vector<Givens> rot(m);
     while (!converged && it < maxit) {
-      S.norm2(K, K.tmp.p, K.h.p);
-      hipLaunchKernelGGL(k_normalize, dim3(S.nb), dim3(256), 0, S.st, n, K.tmp.p, K.h.p, K.V.p);
+      S.normalize(K, K.tmp.p, K.V.p, nullptr);
       std::fill(g.begin(), g.end(), C(0.0));
       g[0] = beta;
       int j = 0;
@@ -354,7 +394,9 @@ extern "C" int ifgf_fgmres(ifgf_plan* P, const double* rhs, double* x, int resta
                                   nullptr, 0, &iconv);
         // w = A_fp64 z_j, orthogonalised against V (FP64 Arnoldi)
         S.apply(zj, K.w.p, true);
-        S.arnoldi_step(K, m, j, hv);
+        S.arnoldi_enqueue(K, j);
+        S.arnoldi_wait(K, j, hv);
+        hv[j + 1] = std::sqrt(std::norm(hv[j + 1]));
         for (int i = 0; i < j; i++) apply_givens(rot[i], hv[i], hv[i + 1]);
         rot[j] = make_givens(hv[j], hv[j + 1]);
         apply_givens(rot[j], hv[j], hv[j + 1]);
