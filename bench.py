@@ -276,6 +276,20 @@ def main():
             dt = (time.perf_counter() - t0) / reps
             batched[prec] = {"ms_per_application": round(dt * 1e3, 4), "column_matvecs_per_s": round(64 / dt, 1)}
             log(f"batched {prec}: 64 columns in {dt * 1e3:.3f} ms -> {64 / dt:.1f} column-matvecs/s")
+            if prec == "fp64" and not args.no_solve:
+                # 64 incidence directions 2 pi j / 64 solved together (lockstep GMRES(50))
+                xg, yg, *_ = fb.points()
+                th = 2 * np.pi * np.arange(64) / 64
+                B = np.stack([-configs.plane_wave(xg, yg, kw["k"], (np.cos(t_), np.sin(t_))) for t_ in th], axis=1)
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                _, its, rr = fb.solve_multi(B, restart=50, tol=1e-8, maxit=2000)
+                t_m = time.perf_counter() - t0
+                batched["solve_64_directions"] = {"time_to_solve_s": round(t_m, 4), "iterations_min": int(its.min()),
+                                                  "iterations_max": int(its.max()),
+                                                  "true_relres_max": float(rr.max())}
+                log(f"64 directions solved together: {t_m:.3f} s, iterations {its.min()}..{its.max()}, "
+                    f"max true relres {rr.max():.2e}")
             del fb
         del Xb, Yb
 

@@ -72,7 +72,7 @@ EXPORTS = [
     "ifgf_plan_correction", "ifgf_apply", "ifgf_apply_direct", "ifgf_propagate", "ifgf_gmres",
     "ifgf_fgmres", "ifgf_gmres_host", "ifgf_hankel01", "ifgf_plan_profile", "ifgf_plan_stage_times",
     "ifgf_morton_sort", "ifgf_apply_graph", "ifgf_group_create", "ifgf_group_launch",
-    "ifgf_group_synchronize", "ifgf_group_destroy",
+    "ifgf_group_synchronize", "ifgf_group_destroy", "ifgf_gmres_multi",
 ]
 
 _LIB = None
@@ -118,6 +118,8 @@ def lib():
         L.ifgf_fgmres.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int, ctypes.c_double,
                                   ctypes.c_int, ctypes.c_int, ctypes.c_double, _ip, _ip, _dp, _dp,
                                   ctypes.c_int]
+        L.ifgf_gmres_multi.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                       ctypes.c_int, _ip, _dp]
         L.ifgf_gmres_host.argtypes = [APPLY_FN, ctypes.c_void_p, ctypes.c_longlong, _dp, _dp,
                                       ctypes.c_int, ctypes.c_double, ctypes.c_int, _ip, _dp]
         L.ifgf_hankel01.argtypes = [_dp, ctypes.c_longlong, _dp, _dp]
@@ -348,6 +350,21 @@ class ForwardMap:
         if rc not in (IFGF_OK, IFGF_ERR_NOT_CONVERGED):
             self._check(rc, "ifgf_gmres")
         return x, it.value, rr.value, hist[: it.value]
+
+    def solve_multi(self, rhs, restart=50, tol=1e-8, maxit=1000, x0=None):
+        """GMRES for the columns of rhs (N x nrhs) at once, lockstep with batched
+        applications (ifgf_gmres_multi). Returns (X, iterations per column, true
+        relative residual per column)."""
+        B = np.asfortranarray(np.asarray(rhs, dtype=np.complex128).reshape(self.N, -1))
+        nr = B.shape[1]
+        X = np.zeros_like(B, order="F") if x0 is None else np.asfortranarray(np.array(x0, dtype=np.complex128))
+        its = np.zeros(nr, dtype=np.int32)
+        rr = np.zeros(nr)
+        rc = lib().ifgf_gmres_multi(self._h, B.ctypes.data_as(_dp), X.ctypes.data_as(_dp), nr, restart, tol, maxit,
+                                    its.ctypes.data_as(_ip), _d(rr))
+        if rc not in (IFGF_OK, IFGF_ERR_NOT_CONVERGED):
+            self._check(rc, "ifgf_gmres_multi")
+        return X, its, rr
 
     def solve_fgmres(self, rhs, restart=50, tol=1e-8, maxit=200, inner_restart=50, inner_floor=1e-5,
                      x0=None):

@@ -120,6 +120,111 @@ __global__ void __launch_bounds__(256) k_residual(long long n, const zd* __restr
   r[i] = csub(b[i], r[i]);
 }
 
+// ---- several systems in lockstep (ifgf_gmres_multi) ----------------------
+// Launch slot a works on system sys[a]: its basis at V + sys[a] * vs
+// (columns ld n), its work vector at W + a * n (the block the batched
+// operator wrote), its partials at part + a * ps, its Hessenberg column at
+// H + sys[a] * hs.
+
+__global__ void __launch_bounds__(256)
+    k_dots_ms(long long n, const zd* __restrict__ V, long long vs, const zd* __restrict__ W, zd* __restrict__ part,
+              long long ps, const int* __restrict__ sys) {
+  const int col = blockIdx.y, a = blockIdx.z;
+  const zd* v = V + (size_t)sys[a] * vs + (size_t)col * n;
+  const zd* w = W + (size_t)a * n;
+  zd acc{0.0, 0.0};
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+    const zd x = v[i], y = w[i];
+    acc.x = fma(x.x, y.x, fma(x.y, y.y, acc.x));
+    acc.y = fma(x.x, y.y, fma(-x.y, y.x, acc.y));
+  }
+  __shared__ double sx[256], sy[256];
+  sx[threadIdx.x] = acc.x;
+  sy[threadIdx.x] = acc.y;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      sx[threadIdx.x] += sx[threadIdx.x + s];
+      sy[threadIdx.x] += sy[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[(size_t)a * ps + (size_t)col * kDot + blockIdx.x] = zd{sx[0], sy[0]};
+}
+
+__global__ void __launch_bounds__(256)
+    k_project_out_ms(long long n, int ncol, const zd* __restrict__ V, long long vs, const zd* __restrict__ part,
+                     long long ps, zd* __restrict__ H, long long hs, int accumulate, zd* __restrict__ W,
+                     const int* __restrict__ sys) {
+  const int a = blockIdx.y, s = sys[a];
+  __shared__ zd hsh[kMaxRestart + 1];
+  for (int c = threadIdx.x; c < ncol; c += blockDim.x) hsh[c] = sum_partials(part + (size_t)a * ps, c);
+  __syncthreads();
+  zd* h = H + (size_t)s * hs;
+  if (blockIdx.x == 0)
+    for (int c = threadIdx.x; c < ncol; c += blockDim.x) h[c] = accumulate ? cadd(h[c], hsh[c]) : hsh[c];
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const zd* v = V + (size_t)s * vs;
+  zd* w = W + (size_t)a * n;
+  zd acc = w[i];
+  for (int c = 0; c < ncol; c++) cfma(acc, cscale(hsh[c], -1.0), v[(size_t)c * n + i]);
+  w[i] = acc;
+}
+
+// partials of ||W[:, a]||^2
+__global__ void __launch_bounds__(256)
+    k_norm_parts_ms(long long n, const zd* __restrict__ W, zd* __restrict__ part, long long ps) {
+  const int a = blockIdx.z;
+  const zd* w = W + (size_t)a * n;
+  double acc = 0.0;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    acc = fma(w[i].x, w[i].x, fma(w[i].y, w[i].y, acc));
+  __shared__ double sx[256];
+  sx[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sx[threadIdx.x] += sx[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[(size_t)a * ps + blockIdx.x] = zd{sx[0], 0.0};
+}
+
+// V[:, col] of system sys[a] = W[:, a] / ||W[:, a]||; H[sys[a]][col] = the norm
+__global__ void __launch_bounds__(256)
+    k_normalize_ms(long long n, const zd* __restrict__ W, const zd* __restrict__ part, long long ps,
+                   zd* __restrict__ V, long long vs, int col, zd* __restrict__ H, long long hs,
+                   const int* __restrict__ sys) {
+  const int a = blockIdx.y, s = sys[a];
+  __shared__ double nn2;
+  if (threadIdx.x == 0) nn2 = sum_partials(part + (size_t)a * ps, 0).x;
+  __syncthreads();
+  const double nn = sqrt(nn2);
+  if (blockIdx.x == 0 && threadIdx.x == 0 && H) H[(size_t)s * hs + col] = zd{nn, 0.0};
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  V[(size_t)s * vs + (size_t)col * n + i] = cscale(W[(size_t)a * n + i], 1.0 / nn);
+}
+
+// dst[:, a] = src_s[:, col] for s = sys[a] (src system stride ss, column stride n)
+__global__ void __launch_bounds__(256)
+    k_gather_ms(long long n, const zd* __restrict__ src, long long ss, int col, const int* __restrict__ sys,
+                zd* __restrict__ dst) {
+  const int a = blockIdx.y;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  dst[(size_t)a * n + i] = src[(size_t)sys[a] * ss + (size_t)col * n + i];
+}
+
+// W[:, a] = B_s - W[:, a] for s = sys[a]
+__global__ void __launch_bounds__(256)
+    k_residual_ms(long long n, const zd* __restrict__ B, const int* __restrict__ sys, zd* __restrict__ W) {
+  const int a = blockIdx.y;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  W[(size_t)a * n + i] = csub(B[(size_t)sys[a] * n + i], W[(size_t)a * n + i]);
+}
+
 struct Givens {
   std::complex<double> c, s;
 };
@@ -424,6 +529,176 @@ extern "C" int ifgf_fgmres(ifgf_plan* P, const double* rhs, double* x, int resta
     HIP_TRY(hipMemcpyAsync(x, xd.p, n * sizeof(zd), hipMemcpyDeviceToHost, S.st));
     HIP_TRY(hipStreamSynchronize(S.st));
     return converged ? IFGF_OK : IFGF_ERR_NOT_CONVERGED;
+  } catch (std::exception& e) {
+    return plan_fail(P, IFGF_ERR_HIP, e.what());
+  }
+}
+
+// Restarted GMRES for nrhs right-hand sides of the same operator, advanced in
+// lockstep: each iteration applies the operator to the active systems' new
+// basis vectors in one call (batched column blocks), then one set of CGS2
+// kernels covers all of them; a system leaves the active set when its own
+// estimate |w(k)| / ||r0|| < tol (opGMRES.hpp:207). Every system follows the
+// same algorithm as ifgf_gmres (same restart, same stopping rule); all
+// active systems restart together.
+extern "C" int ifgf_gmres_multi(ifgf_plan* P, const double* rhs, double* x, int nrhs, int restart, double tol,
+                                int maxit, int* iters, double* relres) {
+  if (!P || !rhs || !x || !iters || !relres || nrhs < 1 || restart < 1 || maxit < 1)
+    return plan_fail(P, IFGF_ERR_ARG, "invalid argument");
+  if (restart > kMaxRestart) return plan_fail(P, IFGF_ERR_ARG, "restart above 256");
+  if (!P->on_device) return plan_fail(P, IFGF_ERR_NO_DEVICE, "plan was built host-only");
+  using C = std::complex<double>;
+  const long long n = P->geo.N;
+  const int m = restart, R = nrhs;
+  const long long vs = n * (m + 1), hs = m + 2, ps = (long long)kDot * (m + 2);
+  const int nb = (int)((n + 255) / 256);
+  try {
+    hipStream_t st = P->stream;
+    DevBuf<zd> B, X, V, W, part, H;
+    DevBuf<int> sysd;
+    B.alloc((size_t)n * R);
+    X.alloc((size_t)n * R);
+    V.alloc((size_t)vs * R);
+    W.alloc((size_t)n * 2 * R);  // operator input block, then its output block
+    part.alloc((size_t)ps * R);
+    H.alloc((size_t)hs * R);
+    sysd.alloc(R);
+    HIP_TRY(hipMemcpyAsync(B.p, rhs, (size_t)n * R * sizeof(zd), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(X.p, x, (size_t)n * R * sizeof(zd), hipMemcpyHostToDevice, st));
+    std::vector<int> act(R);
+    for (int s = 0; s < R; s++) act[s] = s;
+    auto upload_act = [&]() { HIP_TRY(hipMemcpyAsync(sysd.p, act.data(), act.size() * sizeof(int), hipMemcpyHostToDevice, st)); };
+    // norms of W[:, 0..na) (synchronises)
+    auto norms = [&](int na, std::vector<double>& out) {
+      hipLaunchKernelGGL(k_norm_parts_ms, dim3(kDot, 1, na), dim3(256), 0, st, n, W.p, part.p, ps);
+      std::vector<zd> pt((size_t)ps * na);
+      HIP_TRY(hipMemcpyAsync(pt.data(), part.p, pt.size() * sizeof(zd), hipMemcpyDeviceToHost, st));
+      HIP_TRY(hipStreamSynchronize(st));
+      out.assign(na, 0.0);
+      for (int a = 0; a < na; a++) {
+        double acc[4] = {0, 0, 0, 0};
+        for (int b = 0; b < kDot; b += 4)
+          for (int q = 0; q < 4; q++) acc[q] += pt[(size_t)a * ps + b + q].x;
+        out[a] = std::sqrt((acc[0] + acc[1]) + (acc[2] + acc[3]));
+      }
+    };
+    // W[:, a] = b_s - A x_s for the active systems (x gathered into W[:, R + a])
+    auto residuals = [&](bool fp64) {
+      upload_act();
+      const int na = (int)act.size();
+      zd* in = W.p + (size_t)R * n;
+      hipLaunchKernelGGL(k_gather_ms, dim3(nb, na), dim3(256), 0, st, n, X.p, n, 0, sysd.p, in);
+      int rc = apply_columns(P, in, n, W.p, na, st, fp64);
+      if (rc) throw std::runtime_error(P->err);
+      hipLaunchKernelGGL(k_residual_ms, dim3(nb, na), dim3(256), 0, st, n, B.p, sysd.p, W.p);
+    };
+    // ||b||
+    std::vector<double> bn(R), r0n(R), beta(R), tmp;
+    {
+      HIP_TRY(hipMemcpyAsync(W.p, B.p, (size_t)n * R * sizeof(zd), hipMemcpyDeviceToDevice, st));
+      norms(R, bn);
+    }
+    for (int s = 0; s < R; s++) iters[s] = 0;
+    residuals(false);
+    norms(R, tmp);
+    std::vector<char> done(R, 0);
+    std::vector<std::vector<C>> Hm(R, std::vector<C>((size_t)(m + 1) * m)), g(R, std::vector<C>(m + 1));
+    std::vector<std::vector<Givens>> rot(R, std::vector<Givens>(m));
+    for (int s = 0; s < R; s++) {
+      r0n[s] = beta[s] = tmp[s];
+      if (tmp[s] == 0.0 || bn[s] == 0.0) done[s] = 1;
+    }
+    std::vector<C> y;
+    std::vector<zd> hh((size_t)hs * R);
+    int it = 0;
+    while (it < maxit) {
+      act.clear();
+      for (int s = 0; s < R; s++)
+        if (!done[s]) act.push_back(s);
+      if (act.empty()) break;
+      // W holds the active systems' residuals in act order: V_s[:, 0] = r / ||r||
+      upload_act();
+      int na = (int)act.size();
+      hipLaunchKernelGGL(k_norm_parts_ms, dim3(kDot, 1, na), dim3(256), 0, st, n, W.p, part.p, ps);
+      hipLaunchKernelGGL(k_normalize_ms, dim3(nb, na), dim3(256), 0, st, n, W.p, part.p, ps, V.p, vs, 0,
+                         (zd*)nullptr, hs, sysd.p);
+      for (int s : act) {
+        std::fill(g[s].begin(), g[s].end(), C(0.0));
+        g[s][0] = beta[s];
+      }
+      const int jmax = std::min(m, maxit - it);
+      std::vector<int> cols(R, 0);  // Arnoldi columns used by each system this cycle
+      int j = 0;
+      for (; j < jmax && na > 0; j++) {
+        // Wa = A V[:, j] for the active systems (inputs gathered into W[:, R + a])
+        upload_act();
+        zd* in = W.p + (size_t)R * n;
+        zd* Wa = W.p;
+        hipLaunchKernelGGL(k_gather_ms, dim3(nb, na), dim3(256), 0, st, n, V.p, vs, j, sysd.p, in);
+        int rc = apply_columns(P, in, n, Wa, na, st, false);
+        if (rc) return rc;
+        const int nc = j + 1;
+        hipLaunchKernelGGL(k_dots_ms, dim3(kDot, nc, na), dim3(256), 0, st, n, V.p, vs, Wa, part.p, ps, sysd.p);
+        hipLaunchKernelGGL(k_project_out_ms, dim3(nb, na), dim3(256), 0, st, n, nc, V.p, vs, part.p, ps, H.p, hs, 0, Wa,
+                           sysd.p);
+        hipLaunchKernelGGL(k_dots_ms, dim3(kDot, nc, na), dim3(256), 0, st, n, V.p, vs, Wa, part.p, ps, sysd.p);
+        hipLaunchKernelGGL(k_project_out_ms, dim3(nb, na), dim3(256), 0, st, n, nc, V.p, vs, part.p, ps, H.p, hs, 1, Wa,
+                           sysd.p);
+        hipLaunchKernelGGL(k_norm_parts_ms, dim3(kDot, 1, na), dim3(256), 0, st, n, Wa, part.p, ps);
+        hipLaunchKernelGGL(k_normalize_ms, dim3(nb, na), dim3(256), 0, st, n, Wa, part.p, ps, V.p, vs, j + 1, H.p, hs,
+                           sysd.p);
+        HIP_TRY(hipMemcpyAsync(hh.data(), H.p, hh.size() * sizeof(zd), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        it++;
+        std::vector<int> still;
+        for (int s : act) {
+          std::vector<C> hv(j + 2);
+          for (int i = 0; i < j + 2; i++) hv[i] = C(hh[(size_t)s * hs + i].x, hh[(size_t)s * hs + i].y);
+          for (int i = 0; i < j; i++) apply_givens(rot[s][i], hv[i], hv[i + 1]);
+          rot[s][j] = make_givens(hv[j], hv[j + 1]);
+          apply_givens(rot[s][j], hv[j], hv[j + 1]);
+          apply_givens(rot[s][j], g[s][j], g[s][j + 1]);
+          for (int i = 0; i <= j; i++) Hm[s][(size_t)i * m + j] = hv[i];
+          iters[s]++;
+          cols[s] = j + 1;
+          if (std::abs(g[s][j + 1]) / r0n[s] < tol) done[s] = 1;
+          else still.push_back(s);
+        }
+        // systems that converged now: x_s += V_s y_s
+        for (int s : act) {
+          if (!done[s]) continue;
+          y = back_substitute(Hm[s], m, g[s], cols[s]);
+          HIP_TRY(hipMemcpyAsync(H.p + (size_t)s * hs, y.data(), cols[s] * sizeof(zd), hipMemcpyHostToDevice, st));
+          hipLaunchKernelGGL(k_update, dim3(nb), dim3(256), 0, st, n, cols[s], V.p + (size_t)s * vs, n,
+                             H.p + (size_t)s * hs, -1.0, X.p + (size_t)s * n);
+        }
+        act = still;
+        na = (int)act.size();
+      }
+      // end of the cycle: the systems still active update x and restart
+      for (int s : act) {
+        y = back_substitute(Hm[s], m, g[s], cols[s]);
+        HIP_TRY(hipMemcpyAsync(H.p + (size_t)s * hs, y.data(), cols[s] * sizeof(zd), hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_update, dim3(nb), dim3(256), 0, st, n, cols[s], V.p + (size_t)s * vs, n,
+                           H.p + (size_t)s * hs, -1.0, X.p + (size_t)s * n);
+      }
+      if (!act.empty() && it < maxit) {
+        residuals(false);
+        norms((int)act.size(), tmp);
+        for (size_t a = 0; a < act.size(); a++) beta[act[a]] = tmp[a];
+      }
+    }
+    // true residuals with the all-FP64 operator
+    act.resize(R);
+    for (int s = 0; s < R; s++) act[s] = s;
+    residuals(true);
+    norms(R, tmp);
+    for (int s = 0; s < R; s++) relres[s] = bn[s] > 0.0 ? tmp[s] / bn[s] : 0.0;
+    HIP_TRY(hipMemcpyAsync(x, X.p, (size_t)n * R * sizeof(zd), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    bool all = true;
+    for (int s = 0; s < R; s++) all = all && done[s];
+    return all ? IFGF_OK : IFGF_ERR_NOT_CONVERGED;
   } catch (std::exception& e) {
     return plan_fail(P, IFGF_ERR_HIP, e.what());
   }

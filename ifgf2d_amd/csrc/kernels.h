@@ -318,7 +318,7 @@ __global__ void __launch_bounds__(256)
 // Rows vary from a few dozen to several hundred values, and equal items keep
 // every wave's dependent chain short (item -> values and segment table ->
 // densities), so the stream is not held up by the longest rows.
-constexpr int kNearItemChunks = 4;
+constexpr int kNearItemChunks = 8;
 
 __global__ void __launch_bounds__(256)
     k_near_items(int N, int nitems, const NearItem* __restrict__ items, const int4* __restrict__ nf_seg,

@@ -966,6 +966,32 @@ int ifgf::apply_device(ifgf_plan* P, const zd* x, zd* y, hipStream_t st, bool fp
   return IFGF_OK;
 }
 
+// nrhs device columns (stride ld) through the batched kernels in blocks of
+// 8 / 4 / 2 columns where the plan has them, single applications otherwise.
+int ifgf::apply_columns(ifgf_plan* P, const zd* x, long long ld, zd* y, int nrhs, hipStream_t st, bool fp64) {
+  const bool batched_ok = P->stored && P->rb_max >= 2 && !(fp64 && P->mixed);
+  int r = 0;
+  while (r < nrhs) {
+    int R = 1;
+    if (batched_ok)
+      for (int c : {8, 4, 2})
+        if (c <= P->rb_max && c <= nrhs - r) {
+          R = c;
+          break;
+        }
+    const zd* xr = x + (size_t)r * ld;
+    zd* yr = y + (size_t)r * ld;
+    if (R > 1) {
+      if (!dispatch_block(P, R, xr, ld, yr, st)) return fail(P, IFGF_ERR_ARG, "no kernel instantiation");
+    } else {
+      const int rc = apply_device(P, xr, yr, st, fp64);
+      if (rc) return rc;
+    }
+    r += R;
+  }
+  return IFGF_OK;
+}
+
 extern "C" {
 
 int ifgf_apply(ifgf_plan* P, const double* x, double* y, int nrhs, long long ld, int flags, void* stream) {

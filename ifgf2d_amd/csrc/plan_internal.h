@@ -156,6 +156,9 @@ namespace ifgf {
 // one device-resident application on stream st (plan.hip); fp64 selects the
 // all-FP64 operator on a mixed-precision plan
 int apply_device(ifgf_plan* P, const zd* x, zd* y, hipStream_t st, bool fp64 = false);
+// nrhs device columns (stride ld): batched blocks of 8 / 4 / 2 where the plan
+// has them (plan.hip)
+int apply_columns(ifgf_plan* P, const zd* x, long long ld, zd* y, int nrhs, hipStream_t st, bool fp64 = false);
 int plan_fail(ifgf_plan* p, int code, const std::string& msg);
 // The RP correction built on the GPU (precomp.hip); false (err set) when Np
 // exceeds the device kernel's limit.

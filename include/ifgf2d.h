@@ -219,6 +219,19 @@ int ifgf_propagate(ifgf_plan* plan, const double* tx, const double* ty, long lon
 int ifgf_gmres(ifgf_plan* plan, const double* rhs, double* x, int restart, double tol, int maxit,
                int* iters, double* relres, double* hist, int hist_cap);
 
+/* GMRES for nrhs right-hand sides of the plan's operator at once (no
+ * reference counterpart; the reference solves one right-hand side per
+ * opGMRES call): every system runs ifgf_gmres's algorithm (restart, the
+ * estimate |w(k)| / ||r0|| < tol per system), and the systems advance in
+ * lockstep so that each iteration applies the operator to all active systems'
+ * new basis vectors in one batched call (blocks of 8 columns when the plan's
+ * nrhs_max allows) and orthogonalises all of them with one set of kernels.
+ * rhs / x: n x nrhs, column-major, host pointers (x holds the initial
+ * guesses). iters[s] / relres[s]: iterations and true FP64 relative residual
+ * of system s. Returns IFGF_OK when every system converged. restart <= 256. */
+int ifgf_gmres_multi(ifgf_plan* plan, const double* rhs, double* x, int nrhs, int restart, double tol,
+                     int maxit, int* iters, double* relres);
+
 /* Flexible GMRES with FP64 refinement (no reference counterpart; the
  * reference solves with opGMRES only, opGMRES.hpp:68-269): outer FGMRES(restart)
  * in FP64 with the all-FP64 operator, right-preconditioned by an inner GMRES
