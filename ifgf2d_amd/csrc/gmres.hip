@@ -351,12 +351,14 @@ std::vector<std::complex<double>> back_substitute(const std::vector<std::complex
 // Restarted GMRES(m) on device vectors: x (initial guess in, solution out)
 // for A x = b with the plan's operator (fp64: the all-FP64 one). Stops when
 // |g(k)| / ||b - A x0|| < tol (opGMRES.hpp:207) or after maxit iterations.
-// Returns the iteration count; *est the final estimate.
+// Returns the iteration count; *est the final estimate. x0_zero: x is zero on
+// entry, so r0 = b without an application.
 int gmres_core(Solver& S, Krylov& K, bool fp64, const zd* b, zd* x, int m, double tol, int maxit, double* est_out,
-               double* hist, int hist_cap, bool* conv) {
+               double* hist, int hist_cap, bool* conv, bool x0_zero = false) {
   using C = std::complex<double>;
   const long long n = S.n;
-  S.residual(b, x, K.tmp.p, fp64);
+  if (x0_zero) HIP_TRY(hipMemcpyAsync(K.tmp.p, b, n * sizeof(zd), hipMemcpyDeviceToDevice, S.st));  // r0 = b
+  else S.residual(b, x, K.tmp.p, fp64);
   const double r0n = S.norm(K, K.tmp.p);
   double beta = r0n;
   bool converged = (r0n == 0.0);
@@ -432,7 +434,9 @@ extern "C" int ifgf_gmres(ifgf_plan* P, const double* rhs, double* x, int restar
     }
     double est = 0.0;
     bool converged = false;
-    *iters = gmres_core(S, K, false, b.p, xd.p, restart, tol, maxit, &est, hist, hist_cap, &converged);
+    bool zero = true;
+    for (long long i = 0; i < 2 * n && zero; i++) zero = x[i] == 0.0;
+    *iters = gmres_core(S, K, false, b.p, xd.p, restart, tol, maxit, &est, hist, hist_cap, &converged, zero);
     // true residual with the all-FP64 operator
     S.residual(b.p, xd.p, K.tmp.p, true);
     *relres = S.norm(K, K.tmp.p) / bnorm;
@@ -496,7 +500,7 @@ extern "C" int ifgf_fgmres(ifgf_plan* P, const double* rhs, double* x, int resta
         double iest = 0.0;
         bool iconv = false;
         inner_total += gmres_core(S, Ki, false, K.V.p + (size_t)j * n, zj, inner_restart, tau, inner_restart, &iest,
-                                  nullptr, 0, &iconv);
+                                  nullptr, 0, &iconv, true);
         // w = A_fp64 z_j, orthogonalised against V (FP64 Arnoldi)
         S.apply(zj, K.w.p, true);
         S.arnoldi_enqueue(K, j);

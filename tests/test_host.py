@@ -135,3 +135,20 @@ def test_host_gmres_through_linear_operator():
     # same iterates as the oracle's restatement (same algorithm, same data)
     xo, ito, erro, rc = O.gmres_dense(A, b, restart=15, tol=1e-12, maxit=400)
     assert ito == it and np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-12
+
+
+def test_c_abi_from_plain_c(tmp_path):
+    """The header compiles as C99 and the library links and runs from a C
+    program (host-only plan, stats, sorting, correction rows, error codes)."""
+    import shutil
+    import subprocess
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("gcc not available")
+    exe = tmp_path / "c_abi_check"
+    libdir = os.path.join(ROOT, "ifgf2d_amd")
+    subprocess.run([gcc, "-std=c99", "-Wall", "-Werror", "-O1", os.path.join(ROOT, "tests", "c_abi_check.c"),
+                    "-I", os.path.join(ROOT, "include"), "-L", libdir, "-lifgf2d", f"-Wl,-rpath,{libdir}", "-lm",
+                    "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0 and out.stdout.strip().endswith("ok"), out.stdout + out.stderr
