@@ -72,7 +72,8 @@ EXPORTS = [
     "ifgf_plan_correction", "ifgf_apply", "ifgf_apply_direct", "ifgf_propagate", "ifgf_gmres",
     "ifgf_fgmres", "ifgf_gmres_host", "ifgf_hankel01", "ifgf_plan_profile", "ifgf_plan_stage_times",
     "ifgf_morton_sort", "ifgf_apply_graph", "ifgf_group_create", "ifgf_group_launch",
-    "ifgf_group_synchronize", "ifgf_group_destroy", "ifgf_gmres_multi",
+    "ifgf_group_synchronize", "ifgf_group_destroy", "ifgf_gmres_multi", "ifgf_field_create",
+    "ifgf_field_apply", "ifgf_field_stats", "ifgf_field_destroy",
 ]
 
 _LIB = None
@@ -126,6 +127,12 @@ def lib():
         L.ifgf_morton_sort.argtypes = [_dp, _dp, ctypes.c_longlong, ctypes.c_int, _lp, _lp]
         L.ifgf_plan_profile.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.ifgf_plan_stage_times.argtypes = [ctypes.c_void_p, _dp, _lp]
+        L.ifgf_field_create.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_longlong, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+        L.ifgf_field_apply.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                       ctypes.c_void_p]
+        L.ifgf_field_stats.argtypes = [ctypes.c_void_p, _lp]
+        L.ifgf_field_destroy.argtypes = [ctypes.c_void_p]
         _LIB = L
     return _LIB
 
@@ -337,6 +344,14 @@ class ForwardMap:
                                          _d(out.view(np.float64))), "ifgf_propagate")
         return out
 
+    def field(self, tx, ty, nlevels=None, Ps=5, Pang=5, precision="fp64"):
+        """IFGF-accelerated field evaluator at the points (tx, ty): the sum of
+        propagate_unacc (ForwardMap.tpp:702-727) through an IFGF tree of its own.
+        nlevels defaults to this plan's levels plus the number of doublings from the
+        points' bounding box to that of the points and targets (the leaf size stays
+        about the same)."""
+        return Field(self, tx, ty, nlevels, Ps, Pang, precision)
+
     def solve(self, rhs, restart=50, tol=1e-8, maxit=1000, x0=None):
         """Device GMRES (ForwardMap::solve, ForwardMap.tpp:658-697).
 
@@ -398,6 +413,71 @@ class ForwardMap:
         def mapping(x, out):
             out[:] = self.compute_Ax_acc(x)
         return LinearOperator(self.N, self.N, mapping)
+
+
+class Field:
+    """IFGF-accelerated evaluation of a ForwardMap's layer potential at separate
+    target points (ifgf_field_*). evaluate(density) returns, per target,
+    sum_s G(target, x_s) density_s w_s over the plan's points (coincident points
+    contribute zero); G, k and eta are the plan's."""
+
+    def __init__(self, fm, tx, ty, nlevels=None, Ps=5, Pang=5, precision="fp64"):
+        self.tx = np.ascontiguousarray(tx, dtype=np.float64).ravel()
+        self.ty = np.ascontiguousarray(ty, dtype=np.float64).ravel()
+        if self.tx.size != self.ty.size:
+            raise IFGFError("tx and ty differ in length")
+        self.M, self.N = self.tx.size, fm.N
+        if nlevels is None:
+            x, y = fm.points()[:2]
+            s0 = max(np.ptp(x), np.ptp(y))
+            s1 = max(np.ptp(np.concatenate([x, self.tx])), np.ptp(np.concatenate([y, self.ty])))
+            nlevels = fm.nlevels + max(0, int(np.round(np.log2(s1 / s0))))
+        self.nlevels = int(nlevels)
+        h = ctypes.c_void_p()
+        rc = lib().ifgf_field_create(fm._h, _d(self.tx), _d(self.ty), self.M, self.nlevels, Ps, Pang,
+                                     _PREC[precision], ctypes.byref(h))
+        if rc != IFGF_OK:
+            raise IFGFError(f"ifgf_field_create failed ({rc}): {lib().ifgf_last_error().decode()}")
+        self._h = h
+
+    def evaluate(self, density, fp64=False):
+        d = _c(density)
+        if d.size != self.N:
+            raise IFGFError("density length differs from the plan's point count")
+        out = np.empty(self.M, dtype=np.complex128)
+        rc = lib().ifgf_field_apply(self._h, d.ctypes.data, out.ctypes.data, IFGF_APPLY_FP64 if fp64 else 0, None)
+        if rc != IFGF_OK:
+            raise IFGFError(f"ifgf_field_apply failed ({rc}): {lib().ifgf_last_error().decode()}")
+        return out
+
+    __call__ = evaluate
+
+    def apply_device(self, d_ptr, out_ptr, stream=None, fp64=False):
+        """Asynchronous evaluation on device pointers (density N, out M complex128)."""
+        flags = IFGF_PTR_DEVICE | (IFGF_APPLY_FP64 if fp64 else 0)
+        rc = lib().ifgf_field_apply(self._h, ctypes.c_void_p(d_ptr), ctypes.c_void_p(out_ptr), flags,
+                                    ctypes.c_void_p(stream) if stream else None)
+        if rc != IFGF_OK:
+            raise IFGFError(f"ifgf_field_apply failed ({rc}): {lib().ifgf_last_error().decode()}")
+
+    def stats(self):
+        s = np.empty(8, dtype=np.int64)
+        rc = lib().ifgf_field_stats(self._h, s.ctypes.data_as(_lp))
+        if rc != IFGF_OK:
+            raise IFGFError(lib().ifgf_last_error().decode())
+        return dict(zip(("targets", "near", "leaf", "prop", "interp", "cones", "nlevels", "stored_bytes"),
+                        s.tolist()))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().ifgf_field_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class ObstacleGroup:

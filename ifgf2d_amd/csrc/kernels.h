@@ -277,9 +277,14 @@ __device__ inline long long near_index(long long goff, int i, int o) {
          o % kNearChunk;
 }
 
+// N targets at (qx, qy) (the sorted sources themselves for the forward map);
+// leaf_of_point / nb_ptr / nb_list give each target's neighbouring source
+// leaf boxes. excl / patch may be null (no excluded pairs: a field at
+// separate targets, where coincident points give a zero kernel).
 template <int F, int S>
 __global__ void __launch_bounds__(256)
-    k_near_build(int N, const double* __restrict__ px, const double* __restrict__ py,
+    k_near_build(int N, const double* __restrict__ qx, const double* __restrict__ qy,
+                 const double* __restrict__ px, const double* __restrict__ py,
                  const double* __restrict__ pnx, const double* __restrict__ pny,
                  const int* __restrict__ patch, const int4* __restrict__ excl,
                  const int* __restrict__ leaf_of_point, const int* __restrict__ leaf_begin,
@@ -288,8 +293,9 @@ __global__ void __launch_bounds__(256)
   const int gt = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = gt / S, sl = gt % S;
   if (i >= N) return;
-  const double tx = px[i], ty = py[i];
-  const int4 e = excl[i];
+  const double tx = qx[i], ty = qy[i];
+  const bool ex = excl != nullptr;
+  const int4 e = ex ? excl[i] : make_int4(-1, -1, -1, -1);
   const int b = leaf_of_point[i];
   const long long goff = nf_goff[i / kNearGroup];
   int base = 0;
@@ -297,7 +303,7 @@ __global__ void __launch_bounds__(256)
     const int nb = nb_list[n];
     const int s0 = leaf_begin[nb], s1 = leaf_begin[nb + 1];
     for (int s = s0 + sl; s < s1; s += S) {
-      const int ps = patch[s];
+      const int ps = ex ? patch[s] : 0;
       zd g{0.0, 0.0};
       if (!(ps == e.x || ps == e.y || ps == e.z || ps == e.w))
         g = green_d<F>(tx - px[s], ty - py[s], pnx[s], pny[s], k, eta);
@@ -740,7 +746,8 @@ __global__ void __launch_bounds__(256)
   if (i < N && sl == 0) sol[i] = cadd(sol[i], acc);
 }
 
-// y[i] = ident x[i] + sum_e W_e x[patch_e] + near[si] + far[si], si = inv[i]:
+// y[i] = ident x[i] + sum_e W_e x[patch_e] + near[si] + far[si], si = inv[i]
+// (a field at separate targets passes x = corr_ptr = null: y[i] = near + far):
 // S lanes per target split the correction products, the target's near-field
 // item partials (stored path: npart[item * 8 + si % 8] for the items
 // [nio[si / 8], nio[si / 8 + 1]) of its group; on the fly: sol[si]) and its
@@ -762,7 +769,7 @@ __global__ void __launch_bounds__(256)
   int si = 0;
   if (i < N) {
     si = inv[i];
-    const long long e0 = corr_ptr[i], e1 = corr_ptr[i + 1];
+    const long long e0 = corr_ptr ? corr_ptr[i] : 0, e1 = corr_ptr ? corr_ptr[i + 1] : 0;
     const long long nitem = (e1 - e0) * Np;
     for (long long it = sl; it < nitem; it += S) {
       const long long e = e0 + it / Np;
@@ -784,7 +791,7 @@ __global__ void __launch_bounds__(256)
   acc = lane_group_sum<S>(acc);
   near = lane_group_sum<S>(near);
   far = lane_group_sum<S>(far);
-  if (i < N && sl == 0) y[i] = cadd(cadd(cscale(x[i], ident), acc), cadd(near, far));
+  if (i < N && sl == 0) y[i] = cadd(cadd(x ? cscale(x[i], ident) : zd{0.0, 0.0}, acc), cadd(near, far));
 }
 
 // Direct sum at a target subset, excluding the same pairs the correction

@@ -46,15 +46,25 @@ static void build_device_side(ifgf_plan* P) {
   P->w.upload(P->geo.w);
   P->patch.upload(P->patch_sorted);
   P->perm.upload(t.perm);
-  P->inv.upload(t.inv);
-  P->leaf_of_point.upload(t.leaf_of_point);
   P->leaf_begin.upload(t.leaf_begin);
-  P->nb_ptr.upload(t.nb_ptr);
-  P->nb_list.upload(t.nb_list);
-  std::vector<int4> ex(N);
-  for (long long i = 0; i < N; i++)
-    ex[i] = make_int4(P->excl[i * 4], P->excl[i * 4 + 1], P->excl[i * 4 + 2], P->excl[i * 4 + 3]);
-  P->excl_d.upload(ex);
+  if (P->field) {
+    // targets: inverse sort, leaf groups and their neighbouring source boxes
+    P->inv.upload(t.tinv);
+    P->leaf_of_point.upload(t.tleaf);
+    P->nb_ptr.upload(t.tnb_ptr);
+    P->nb_list.upload(t.tnb_list);
+    P->tpx.upload(t.tx);
+    P->tpy.upload(t.ty);
+  } else {
+    P->inv.upload(t.inv);
+    P->leaf_of_point.upload(t.leaf_of_point);
+    P->nb_ptr.upload(t.nb_ptr);
+    P->nb_list.upload(t.nb_list);
+    std::vector<int4> ex(N);
+    for (long long i = 0; i < N; i++)
+      ex[i] = make_int4(P->excl[i * 4], P->excl[i * 4 + 1], P->excl[i * 4 + 2], P->excl[i * 4 + 3]);
+    P->excl_d.upload(ex);
+  }
   P->cg.upload(P->cone_geo);
   P->cone_box.upload(t.cone_box);
   P->cone_qd.upload(P->cone_q);
@@ -72,8 +82,49 @@ static void build_device_side(ifgf_plan* P) {
   P->sol.alloc((size_t)N * R);
   P->coef.alloc((size_t)t.n_cones * Pn * R);
   P->xbuf.alloc((size_t)N * R);
-  P->ybuf.alloc((size_t)N * R);
+  P->ybuf.alloc((size_t)t.M * R);
   P->on_device = true;
+}
+
+// Per-cone geometry records and the Chebyshev tables (Chebyshev.h:10-68) with
+// the transform's alpha factors.
+static void build_cone_tables(ifgf_plan* P) {
+  const Tree& t = P->tree;
+  P->cone_geo.resize(t.n_cones);
+  P->cone_q.resize(t.n_cones);
+  for (long long g = 0; g < t.n_cones; g++) {
+    const LevelInfo& L = t.lv[t.cone_level[g]];
+    const int b = t.cone_box[g], posr = t.cone_posr[g], post = t.cone_post[g];
+    ConeGeo& G = P->cone_geo[g];
+    G.cx = t.box_cx[L.box_off + b];
+    G.cy = t.box_cy[L.box_off + b];
+    G.h = L.h;
+    G.rmin = L.h / t.radial[L.radial_off + posr + 1];
+    G.dr = L.h / t.radial[L.radial_off + posr] - G.rmin;
+    G.da = M_PI / L.nce;
+    G.th0 = G.da * post;
+    G.pad = 0;
+    P->cone_q[g] = post * L.nrad + posr;
+  }
+  const int Ps = P->prm.Ps, Pa = P->prm.Pang;
+  auto pts = [](int n, double* x) {
+    for (int i = 0; i < n; i++) x[n - 1 - i] = std::cos(M_PI * (2.0 * i + 1.0) / (2.0 * n));
+  };
+  pts(Ps, P->cheb.xs);
+  pts(Pa, P->cheb.xa);
+  auto tn = [](int n, const double* x, double* T, double scale) {
+    std::vector<double> t((size_t)n * n);
+    for (int i = 0; i < n; i++) {
+      t[i] = 1.0;
+      t[n + i] = x[i];
+    }
+    for (int m = 2; m < n; m++)
+      for (int i = 0; i < n; i++) t[m * n + i] = 2.0 * x[i] * t[(m - 1) * n + i] - t[(m - 2) * n + i];
+    for (int m = 0; m < n; m++)
+      for (int i = 0; i < n; i++) T[m * 16 + i] = t[m * n + i] * (m == 0 ? 1.0 : 2.0) * scale;
+  };
+  tn(Ps, P->cheb.xs, P->cheb.Ts, 1.0);
+  tn(Pa, P->cheb.xa, P->cheb.Ta, 1.0 / (Ps * Pa));
 }
 
 extern "C" {
@@ -121,44 +172,7 @@ int ifgf_plan_create(const ifgf_surface* s, const ifgf_params* prm, ifgf_plan** 
       return fail(nullptr, IFGF_ERR_GEOMETRY, err);
     if (!build_exclusions(P->geo, P->tree, P->patch_sorted, P->excl, P->n_near, err))
       return fail(nullptr, IFGF_ERR_GEOMETRY, err);
-    // per-cone geometry records
-    const Tree& t = P->tree;
-    P->cone_geo.resize(t.n_cones);
-    P->cone_q.resize(t.n_cones);
-    for (long long g = 0; g < t.n_cones; g++) {
-      const LevelInfo& L = t.lv[t.cone_level[g]];
-      const int b = t.cone_box[g], posr = t.cone_posr[g], post = t.cone_post[g];
-      ConeGeo& G = P->cone_geo[g];
-      G.cx = t.box_cx[L.box_off + b];
-      G.cy = t.box_cy[L.box_off + b];
-      G.h = L.h;
-      G.rmin = L.h / t.radial[L.radial_off + posr + 1];
-      G.dr = L.h / t.radial[L.radial_off + posr] - G.rmin;
-      G.da = M_PI / L.nce;
-      G.th0 = G.da * post;
-      G.pad = 0;
-      P->cone_q[g] = post * L.nrad + posr;
-    }
-    // Chebyshev tables (Chebyshev.h:10-68) with the transform's alpha factors
-    const int Ps = prm->Ps, Pa = prm->Pang;
-    auto pts = [](int n, double* x) {
-      for (int i = 0; i < n; i++) x[n - 1 - i] = std::cos(M_PI * (2.0 * i + 1.0) / (2.0 * n));
-    };
-    pts(Ps, P->cheb.xs);
-    pts(Pa, P->cheb.xa);
-    auto tn = [](int n, const double* x, double* T, double scale) {
-      std::vector<double> t((size_t)n * n);
-      for (int i = 0; i < n; i++) {
-        t[i] = 1.0;
-        t[n + i] = x[i];
-      }
-      for (int m = 2; m < n; m++)
-        for (int i = 0; i < n; i++) t[m * n + i] = 2.0 * x[i] * t[(m - 1) * n + i] - t[(m - 2) * n + i];
-      for (int m = 0; m < n; m++)
-        for (int i = 0; i < n; i++) T[m * 16 + i] = t[m * n + i] * (m == 0 ? 1.0 : 2.0) * scale;
-    };
-    tn(Ps, P->cheb.xs, P->cheb.Ts, 1.0);
-    tn(Pa, P->cheb.xa, P->cheb.Ta, 1.0 / (Ps * Pa));
+    build_cone_tables(P.get());
     if (!(prm->flags & IFGF_PLAN_HOST_ONLY)) {
       build_device_side(P.get());
       if (!(prm->flags & IFGF_PLAN_ON_THE_FLY)) {
@@ -275,7 +289,8 @@ static hipEvent_t* take_events(ifgf_plan* P) {
 }
 
 static void launch_near_stored(ifgf_plan* P, hipStream_t st) {
-  const int N = (int)P->tree.N, M = (int)P->near_items.n;
+  const int N = (int)P->tree.M, M = (int)P->near_items.n;
+  if (M == 0) return;  // a field whose targets have no source in their leaf neighbourhood
   hipLaunchKernelGGL(k_near_items, dim3((M + 3) / 4), dim3(256), 0, st, N, M, P->near_items.p, P->nf_seg.p,
                      P->nf_val.p, P->rho.p, P->npart.p);
 }
@@ -364,6 +379,7 @@ static void launch_ifgf(ifgf_plan* P, const zd* x, zd* y, hipStream_t st) {
     const long long M = P->leaf_items.n;
     const Z* lf = f32 ? (const Z*)P->lf_val32.p : (const Z*)P->lf_val.p;
     Z* lpart = f32 ? (Z*)P->lpart32.p : (Z*)P->lpart.p;
+    if (M > 0)
     hipLaunchKernelGGL((k_leaf_items<Pn, Z>), dim3((unsigned)((M * Pn * kLeafChunk + 255) / 256)), dim3(256), 0, st,
                        (int)M, P->leaf_items.p, lf, P->rho.p, lpart);
   } else {
@@ -398,15 +414,18 @@ static void launch_ifgf(ifgf_plan* P, const zd* x, zd* y, hipStream_t st) {
                        P->sol.p, k);
   mark(9);
 
-  // correction, identity and unsort
+  // correction, identity and unsort (a field: near + far at its targets)
   if (fork) HIP_TRY(hipStreamWaitEvent(st, P->ev_join, 0));
   mark(10);
   const double ident = (F == kSingleLayer) ? 0.0 : 0.5;
   const Z* slots = nullptr;
   if (P->stored) slots = f32 ? (const Z*)P->pbuf32.p : (const Z*)P->pbuf.p;
-#define IFGF_FIN(S)                                                                                                \
-  hipLaunchKernelGGL((k_finalize<S, Z>), dim3((N + 256 / S - 1) / (256 / S)), dim3(256), 0, st, N, P->geo.Np, ident, \
-                     x, P->corr_ptr.p, P->corr_patch.p, P->corr_w.p, P->inv.p, P->sol.p, P->near_item_off.p,       \
+  const int NT = (int)t.M;
+  const zd* fx = P->field ? nullptr : x;
+  const long long* fcorr = P->field ? nullptr : P->corr_ptr.p;
+#define IFGF_FIN(S)                                                                                                  \
+  hipLaunchKernelGGL((k_finalize<S, Z>), dim3((NT + 256 / S - 1) / (256 / S)), dim3(256), 0, st, NT, P->geo.Np, ident, \
+                     fx, fcorr, P->corr_patch.p, P->corr_w.p, P->inv.p, P->sol.p, P->near_item_off.p,               \
                      P->stored ? P->npart.p : nullptr, P->it_ptr.p, P->it_slot.p, slots, y)
   if (P->final_lanes == 4) IFGF_FIN(4);
   else if (P->final_lanes == 16) IFGF_FIN(16);
@@ -558,7 +577,16 @@ template <int F, int Ps, int Pang>
 static void build_stored(ifgf_plan* P) {
   constexpr int Pn = Ps * Pang;
   const Tree& t = P->tree;
-  const int N = (int)t.N, D = t.D;
+  const int D = t.D;
+  // N: targets (the sorted sources for the forward map, tree.M separate
+  // targets for a field); tleaf / tnb_*: each target's leaf group and the
+  // source leaf boxes around it
+  const int N = (int)t.M;
+  const std::vector<int>& tleaf = P->field ? t.tleaf : t.leaf_of_point;
+  const std::vector<int>& tnb_ptr = P->field ? t.tnb_ptr : t.nb_ptr;
+  const std::vector<int>& tnb_list = P->field ? t.tnb_list : t.nb_list;
+  const double* qx = P->field ? P->tpx.p : P->px.p;
+  const double* qy = P->field ? P->tpy.p : P->py.p;
   const double k = P->prm.k, eta = P->prm.eta;
   hipStream_t st = P->stream;
   // near field: segment length of target i = points of its leaf neighbours;
@@ -566,10 +594,10 @@ static void build_stored(ifgf_plan* P) {
   std::vector<int> nf_len(N);
   long long near_entries = 0;
   for (int i = 0; i < N; i++) {
-    const int b = t.leaf_of_point[i];
+    const int b = tleaf[i];
     int c = 0;
-    for (int n = t.nb_ptr[b]; n < t.nb_ptr[b + 1]; n++)
-      c += t.leaf_begin[t.nb_list[n] + 1] - t.leaf_begin[t.nb_list[n]];
+    for (int n = tnb_ptr[b]; n < tnb_ptr[b + 1]; n++)
+      c += t.leaf_begin[tnb_list[n] + 1] - t.leaf_begin[tnb_list[n]];
     nf_len[i] = c;
     near_entries += c;
   }
@@ -597,19 +625,21 @@ static void build_stored(ifgf_plan* P) {
     item_off[ngroups] = (int)items.size();
     P->near_items.upload(items);
     P->near_item_off.upload(item_off);
-    P->npart.alloc(items.size() * kNearGroup);
+    P->npart.alloc(std::max<size_t>(1, items.size()) * kNearGroup);  // never null: k_finalize tests it
   }
-  // per leaf box, its neighbour boxes as (exclusive end in the flattened
-  // segment, source index minus flat offset), padded to kMaxNb segments
+  // per target leaf group, its neighbour boxes as (exclusive end in the
+  // flattened segment, source index minus flat offset), padded to kMaxNb
+  // segments
   P->nleaf = (int)t.leaf_begin.size() - 1;
-  std::vector<int> seg((size_t)P->nleaf * kNbSegInts, 0);
-  for (int b = 0; b < P->nleaf; b++) {
-    const int nn = t.nb_ptr[b + 1] - t.nb_ptr[b];
+  const int ntg = (int)tnb_ptr.size() - 1;
+  std::vector<int> seg((size_t)ntg * kNbSegInts, 0);
+  for (int b = 0; b < ntg; b++) {
+    const int nn = tnb_ptr[b + 1] - tnb_ptr[b];
     if (nn > kMaxNb) throw std::runtime_error("a leaf box has more than 9 neighbour boxes");
     int* sg = &seg[(size_t)b * kNbSegInts];
     int cum = 0, q = 0;
-    for (int n = t.nb_ptr[b]; n < t.nb_ptr[b + 1]; n++, q++) {
-      const int s0 = t.leaf_begin[t.nb_list[n]], len = t.leaf_begin[t.nb_list[n] + 1] - s0;
+    for (int n = tnb_ptr[b]; n < tnb_ptr[b + 1]; n++, q++) {
+      const int s0 = t.leaf_begin[tnb_list[n]], len = t.leaf_begin[tnb_list[n] + 1] - s0;
       sg[2 * q + 1] = s0 - cum;
       cum += len;
       sg[2 * q] = cum;
@@ -619,14 +649,15 @@ static void build_stored(ifgf_plan* P) {
   {
     std::vector<int4> per_target((size_t)N * (kNbSegInts / 4));
     for (int i = 0; i < N; i++)
-      std::memcpy(&per_target[(size_t)i * (kNbSegInts / 4)], &seg[(size_t)t.leaf_of_point[i] * kNbSegInts],
+      std::memcpy(&per_target[(size_t)i * (kNbSegInts / 4)], &seg[(size_t)tleaf[i] * kNbSegInts],
                   kNbSegInts * sizeof(int));
     P->nf_seg.upload(per_target);
   }
   P->nf_val.alloc(nf_goff[ngroups]);
-  HIP_TRY(hipMemsetAsync(P->nf_val.p, 0, nf_goff[ngroups] * sizeof(zd), st));
+  if (nf_goff[ngroups]) HIP_TRY(hipMemsetAsync(P->nf_val.p, 0, nf_goff[ngroups] * sizeof(zd), st));
   hipLaunchKernelGGL((k_near_build<F, kNearLanes>), dim3((N + 256 / kNearLanes - 1) / (256 / kNearLanes)),
-                     dim3(256), 0, st, N, P->px.p, P->py.p, P->pnx.p, P->pny.p, P->patch.p, P->excl_d.p,
+                     dim3(256), 0, st, N, qx, qy, P->px.p, P->py.p, P->pnx.p, P->pny.p,
+                     P->field ? nullptr : P->patch.p, P->field ? nullptr : P->excl_d.p,
                      P->leaf_of_point.p, P->leaf_begin.p, P->nb_ptr.p, P->nb_list.p, P->nf_goff.p,
                      P->nf_val.p, k, eta);
   HIP_TRY(hipGetLastError());
@@ -661,11 +692,13 @@ static void build_stored(ifgf_plan* P) {
     DevBuf<zd> tmp;
     tmp.alloc(lf_off[LD.ncone]);
     const long long nitems = lf_off[LD.ncone];
+    if (nitems > 0) {
     hipLaunchKernelGGL((k_leaf_build_values<F, Ps, Pang>), dim3((unsigned)((nitems + 255) / 256)), dim3(256), 0,
                        st, LD.ncone, LD.cone_off, P->cg.p, P->cone_box.p, P->leaf_begin.p, P->lf_off.p,
                        P->lf_off.p, P->px.p, P->py.p, P->pnx.p, P->pny.p, P->tabs.p, tmp.p, nitems, k, eta);
     hipLaunchKernelGGL((k_leaf_build_transform<Ps, Pang>), dim3((unsigned)((nitems + 255) / 256)), dim3(256), 0,
                        st, nitems / Pn, tmp.p, P->tabs.p, P->lf_val.p);
+    }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(st));
   }
@@ -673,7 +706,7 @@ static void build_stored(ifgf_plan* P) {
   DevBuf<Geo4> it_geo;
   it_geo.alloc(t.it_ptr[N]);
   hipLaunchKernelGGL((k_interp_build<kInterpLanes>), dim3((N + 256 / kInterpLanes - 1) / (256 / kInterpLanes)),
-                     dim3(256), 0, st, N, P->px.p, P->py.p, P->it_ptr.p, P->it_cone.p, P->cg.p, it_geo.p, k);
+                     dim3(256), 0, st, N, qx, qy, P->it_ptr.p, P->it_cone.p, P->cg.p, it_geo.p, k);
   HIP_TRY(hipGetLastError());
   // propagation geometry per (parent node, relevant child): entry
   // pr_ptr[g] + k P + j holds node j of parent cone g in its k-th child
@@ -698,6 +731,7 @@ static void build_stored(ifgf_plan* P) {
   for (int l = D; l >= 3; l--) {
     const LevelInfo& L = t.lv[l];
     const LevelInfo& Pl = t.lv[l - 1];
+    if (Pl.ncone == 0) continue;  // a field level without cones
     hipLaunchKernelGGL((k_prop_build<Ps, Pang>), dim3((Pl.ncone * Pn + 255) / 256), dim3(256), 0, st, Pl.ncone,
                        Pl.cone_off, Pl.box_off, L.cone_off, L.nslot, P->cg.p, P->cone_box.p, P->cone_qd.p,
                        P->box_child.p, P->proto.p + L.proto_off, P->dense_map.p + L.map_off, P->tabs.p,
@@ -749,6 +783,7 @@ static void build_stored(ifgf_plan* P) {
   {
     DevBuf<long long> d_src;
     d_src.upload(src);
+    if (npr + nit > 0)
     hipLaunchKernelGGL(k_push_gather, dim3((unsigned)((npr + nit + 255) / 256)), dim3(256), 0, st, npr + nit, d_src.p,
                        npr, pr_geo.p, it_geo.p, P->push_geo.p);
     HIP_TRY(hipGetLastError());
@@ -1280,6 +1315,89 @@ int ifgf_propagate(ifgf_plan* P, const double* tx, const double* ty, long long n
   }
   return IFGF_OK;
 }
+
+}  // extern "C"
+
+struct ifgf_field {
+  std::unique_ptr<ifgf_plan> P;
+};
+
+extern "C" {
+
+int ifgf_field_create(const ifgf_plan* src, const double* tx, const double* ty, long long nt, int nlevels, int Ps,
+                      int Pang, int precision, ifgf_field** out) {
+  if (!src || !tx || !ty || !out) return fail(nullptr, IFGF_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (nt < 1 || nt > (1ll << 30)) return fail(nullptr, IFGF_ERR_ARG, "target count out of range");
+  if (nlevels < 3) return fail(nullptr, IFGF_ERR_LEVELS, "IFGF accelerator requires at least 3 levels.");
+  if (Ps < 2 || Ps > 16 || Pang < 2 || Pang > 16) return fail(nullptr, IFGF_ERR_ARG, "Ps and Pang must lie in [2, 16]");
+  if (precision != IFGF_FP64 && precision != IFGF_MIXED) return fail(nullptr, IFGF_ERR_ARG, "unsupported precision mode");
+  std::unique_ptr<ifgf_field> F(new ifgf_field);
+  F->P.reset(new ifgf_plan);
+  ifgf_plan* P = F->P.get();
+  P->field = true;
+  P->prm = src->prm;
+  P->prm.nlevels = nlevels;
+  P->prm.Ps = Ps;
+  P->prm.Pang = Pang;
+  P->prm.precision = precision;
+  P->prm.nrhs_max = 1;
+  P->prm.flags = 0;
+  P->geo = src->geo;
+  std::string err;
+  try {
+    const Geometry& g = P->geo;
+    if (!build_tree(g.x.data(), g.y.data(), g.nx.data(), g.ny.data(), g.N, nlevels, P->prm.k, Ps, Pang, P->tree, err,
+                    nullptr, tx, ty, nt))
+      return fail(nullptr, IFGF_ERR_GEOMETRY, err);
+    P->n_near = P->tree.n_near_t;
+    build_cone_tables(P);
+    if (src->on_device) {  // a field of a host-only plan holds the tables only
+      build_device_side(P);
+      if (!build_stored_any(P)) return fail(nullptr, IFGF_ERR_ARG, "no kernel instantiation for this (Ps, Pang)");
+    }
+  } catch (std::exception& e) {
+    return fail(nullptr, IFGF_ERR_HIP, e.what());
+  }
+  *out = F.release();
+  return IFGF_OK;
+}
+
+int ifgf_field_apply(ifgf_field* F, const double* density, double* out, int flags, void* stream) {
+  if (!F || !density || !out) return fail(nullptr, IFGF_ERR_ARG, "null argument");
+  ifgf_plan* P = F->P.get();
+  if (!P->on_device) return fail(P, IFGF_ERR_NO_DEVICE, "field of a host-only plan");
+  const long long N = P->geo.N, M = P->tree.M;
+  try {
+    hipStream_t st = stream ? (hipStream_t)stream : P->stream;
+    const bool fp64 = (flags & IFGF_APPLY_FP64) != 0;
+    if (flags & IFGF_PTR_DEVICE) return apply_device(P, (const zd*)density, (zd*)out, st, fp64);
+    HIP_TRY(hipMemcpyAsync(P->xbuf.p, density, N * sizeof(zd), hipMemcpyHostToDevice, st));
+    const int rc = apply_device(P, P->xbuf.p, P->ybuf.p, st, fp64);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(out, P->ybuf.p, M * sizeof(zd), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+  } catch (std::exception& e) {
+    return fail(P, IFGF_ERR_HIP, e.what());
+  }
+  return IFGF_OK;
+}
+
+int ifgf_field_stats(const ifgf_field* F, long long out[8]) {
+  if (!F || !out) return fail(nullptr, IFGF_ERR_ARG, "null argument");
+  const ifgf_plan* P = F->P.get();
+  out[0] = P->tree.M;
+  out[1] = P->tree.n_near_t;
+  out[2] = P->tree.n_leaf;
+  out[3] = P->tree.n_prop;
+  out[4] = P->tree.n_interp;
+  out[5] = P->tree.n_cones;
+  out[6] = P->tree.nlev;
+  out[7] = P->stored_bytes;
+  return IFGF_OK;
+}
+
+void ifgf_field_destroy(ifgf_field* F) { delete F; }
 
 int ifgf_hankel01(const double* x, long long n, double* h0, double* h1) {
   if (!x || !h0 || !h1 || n < 0) return fail(nullptr, IFGF_ERR_ARG, "null argument");

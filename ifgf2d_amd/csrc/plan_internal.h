@@ -63,6 +63,10 @@ struct ifgf_plan {
   std::vector<int> cone_q;
   ifgf::ChebTables cheb{};
   bool on_device = false;
+  // a field evaluator (ifgf_field_*): the tree holds separate targets; the
+  // application has no identity, correction or excluded pairs, and its
+  // output is one value per target (tree.M) in the targets' input order
+  bool field = false;
   hipStream_t stream = nullptr;
   // stored path: the near field on a second stream beside the leaf operator
   // and the level-push chain (fork after the gather, join before finalize);
@@ -79,6 +83,7 @@ struct ifgf_plan {
   long long g_ld = 0;
   // device tables
   ifgf::DevBuf<double> px, py, pnx, pny, w;
+  ifgf::DevBuf<double> tpx, tpy;  // field: sorted targets
   ifgf::DevBuf<int> patch, perm, inv, leaf_of_point, leaf_begin, nb_ptr, nb_list;
   ifgf::DevBuf<int4> excl_d;
   ifgf::DevBuf<ifgf::ConeGeo> cg;

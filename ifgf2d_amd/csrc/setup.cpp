@@ -349,13 +349,19 @@ struct LevelGeom {
 }  // namespace
 
 bool build_tree(const double* x, const double* y, const double* nx, const double* ny, long long N,
-                int nlev, double k, int Ps, int Pang, Tree& t, std::string& err, const std::vector<int>* perm_in) {
+                int nlev, double k, int Ps, int Pang, Tree& t, std::string& err, const std::vector<int>* perm_in,
+                const double* tx, const double* ty, long long M) {
   if (nlev < 3) {
     err = "IFGF accelerator requires at least 3 levels.";
     return false;
   }
   if (N < 1 || N > (1ll << 30)) {
     err = "point count out of range";
+    return false;
+  }
+  const bool sep = tx != nullptr;
+  if (sep && (!ty || M < 1 || M > (1ll << 30) || perm_in)) {
+    err = "target count out of range";
     return false;
   }
   t = Tree();
@@ -365,6 +371,8 @@ bool build_tree(const double* x, const double* y, const double* nx, const double
   t.k = k;
   t.Ps = Ps;
   t.Pang = Pang;
+  t.sep = sep;
+  t.M = sep ? M : N;
   const int D = t.D, P = Ps * Pang;
   // Bounding box +- 1e-10 and the level-1 box size (BoxTree.h:179-231).
   double mn0 = 1e308, mn1 = 1e308, mx0 = -1e308, mx1 = -1e308;
@@ -373,6 +381,16 @@ bool build_tree(const double* x, const double* y, const double* nx, const double
     mn1 = std::min(mn1, y[i]);
     mx0 = std::max(mx0, x[i]);
     mx1 = std::max(mx1, y[i]);
+  }
+  for (long long i = 0; sep && i < M; i++) {
+    mn0 = std::min(mn0, tx[i]);
+    mn1 = std::min(mn1, ty[i]);
+    mx0 = std::max(mx0, tx[i]);
+    mx1 = std::max(mx1, ty[i]);
+  }
+  if (!(mx0 - mn0 < 1e300) || !(mx1 - mn1 < 1e300)) {
+    err = "non-finite point coordinates";
+    return false;
   }
   mx0 += 1e-10;
   mx1 += 1e-10;
@@ -427,6 +445,30 @@ bool build_tree(const double* x, const double* y, const double* nx, const double
     t.snx[i] = nx[o];
     t.sny[i] = ny[o];
   }
+  if (sep) {
+    // targets: the same stable sort by leaf Morton code
+    std::vector<long long> tcode(M);
+    for (long long i = 0; i < M; i++) {
+      bool ok;
+      tcode[i] = G[D].point2morton(tx[i], ty[i], D, ok);
+      if (!ok) {
+        err = "The 2D box index cannot be < 0 or larger than the number of boxes";
+        return false;
+      }
+    }
+    t.tperm.resize(M);
+    std::iota(t.tperm.begin(), t.tperm.end(), 0);
+    std::stable_sort(t.tperm.begin(), t.tperm.end(), [&](int a, int b) { return tcode[a] < tcode[b]; });
+    t.tinv.resize(M);
+    t.tx.resize(M);
+    t.ty.resize(M);
+    for (long long i = 0; i < M; i++) {
+      const int o = t.tperm[i];
+      t.tinv[o] = (int)i;
+      t.tx[i] = tx[o];
+      t.ty[i] = ty[o];
+    }
+  }
   // Relevant boxes per level (BoxTree.h:111-174): leaf boxes from the sorted
   // codes, parents = codes / 4, each level ascending.
   std::vector<std::vector<long long>> boxes(D + 1);
@@ -479,76 +521,137 @@ bool build_tree(const double* x, const double* y, const double* nx, const double
         auto it = box_index[l + 1].find(boxes[l][b] * 4 + cp);
         if (it != box_index[l + 1].end()) t.box_child[(size_t)(t.lv[l].box_off + b) * 4 + cp] = it->second;
       }
-  // leaf neighbours (GetNeighbours, Level.h:378-404), ascending Morton
-  t.nb_ptr.assign(1, 0);
-  for (size_t b = 0; b < boxes[D].size(); b++) {
-    const long long bx = (long long)merge_from2((uint64_t)boxes[D][b]);
-    const long long by = (long long)merge_from2((uint64_t)boxes[D][b] >> 1);
+  // relevant (source) boxes of the 3x3 neighbourhood of the level-l box with
+  // Morton code m (GetNeighbours, Level.h:378-404) and its cousins (children
+  // of the parent's neighbours that are not neighbours, GetCousins,
+  // Level.h:261-289), as local box indices in ascending Morton order
+  auto neighbours_of = [&](int l, long long m, std::vector<int>& out) {
+    const long long bx = (long long)merge_from2((uint64_t)m);
+    const long long by = (long long)merge_from2((uint64_t)m >> 1);
     std::vector<std::pair<long long, int>> nb;
-    for (long long j = std::max(0ll, by - 1); j <= std::min(G[D].nbps - 1, by + 1); j++)
-      for (long long i = std::max(0ll, bx - 1); i <= std::min(G[D].nbps - 1, bx + 1); i++) {
-        auto it = box_index[D].find(morton(i, j, D));
-        if (it != box_index[D].end()) nb.push_back({it->first, it->second});
+    for (long long j = std::max(0ll, by - 1); j <= std::min(G[l].nbps - 1, by + 1); j++)
+      for (long long i = std::max(0ll, bx - 1); i <= std::min(G[l].nbps - 1, bx + 1); i++) {
+        auto it = box_index[l].find(morton(i, j, l));
+        if (it != box_index[l].end()) nb.push_back({it->first, it->second});
       }
     std::sort(nb.begin(), nb.end());
-    for (auto& e : nb) t.nb_list.push_back(e.second);
-    t.nb_ptr.push_back((int)t.nb_list.size());
-  }
-  // cousin lists per box and level (GetCousins, Level.h:261-289), ascending Morton
-  std::vector<std::vector<std::vector<int>>> cz(D + 1);
-  for (int l = 2; l <= D; l++) {
-    cz[l].resize(boxes[l].size());
-    for (size_t b = 0; b < boxes[l].size(); b++) {
-      const long long bx = (long long)merge_from2((uint64_t)boxes[l][b]);
-      const long long by = (long long)merge_from2((uint64_t)boxes[l][b] >> 1);
-      const long long xlo = std::max<long long>((bx / 2) * 2 - 2, 0), xup = std::min<long long>(G[l].nbps - 1, (bx / 2) * 2 + 3);
-      const long long ylo = std::max<long long>((by / 2) * 2 - 2, 0), yup = std::min<long long>(G[l].nbps - 1, (by / 2) * 2 + 3);
-      std::vector<std::pair<long long, int>> c;
-      for (long long j = ylo; j <= yup; j++)
-        for (long long i = xlo; i <= xup; i++) {
-          if (std::abs(i - bx) <= 1 && std::abs(j - by) <= 1) continue;
-          auto it = box_index[l].find(morton(i, j, l));
-          if (it != box_index[l].end()) c.push_back({it->first, it->second});
-        }
-      std::sort(c.begin(), c.end());
-      for (auto& e : c) cz[l][b].push_back(e.second);
-    }
-  }
-  // per sorted point: box at each level (Point2Morton at that level)
-  std::vector<int> pbox((size_t)N * (D + 1), -1);
-  bool bad = false;
-#pragma omp parallel for schedule(static) reduction(|| : bad)
-  for (long long i = 0; i < N; i++)
-    for (int l = 2; l <= D; l++) {
-      bool ok;
-      const long long m = G[l].point2morton(t.sx[i], t.sy[i], l, ok);
-      auto it = ok ? box_index[l].find(m) : box_index[l].end();
-      if (it == box_index[l].end()) {
-        bad = true;
-        continue;
+    out.clear();
+    for (auto& e : nb) out.push_back(e.second);
+  };
+  auto cousins_of = [&](int l, long long m, std::vector<int>& out) {
+    const long long bx = (long long)merge_from2((uint64_t)m);
+    const long long by = (long long)merge_from2((uint64_t)m >> 1);
+    const long long xlo = std::max<long long>((bx / 2) * 2 - 2, 0), xup = std::min<long long>(G[l].nbps - 1, (bx / 2) * 2 + 3);
+    const long long ylo = std::max<long long>((by / 2) * 2 - 2, 0), yup = std::min<long long>(G[l].nbps - 1, (by / 2) * 2 + 3);
+    std::vector<std::pair<long long, int>> c;
+    for (long long j = ylo; j <= yup; j++)
+      for (long long i = xlo; i <= xup; i++) {
+        if (std::abs(i - bx) <= 1 && std::abs(j - by) <= 1) continue;
+        auto it = box_index[l].find(morton(i, j, l));
+        if (it != box_index[l].end()) c.push_back({it->first, it->second});
       }
-      pbox[(size_t)i * (D + 1) + l] = it->second;
+    std::sort(c.begin(), c.end());
+    out.clear();
+    for (auto& e : c) out.push_back(e.second);
+  };
+  // leaf neighbours of every source leaf box
+  t.nb_ptr.assign(1, 0);
+  {
+    std::vector<int> nb;
+    for (size_t b = 0; b < boxes[D].size(); b++) {
+      neighbours_of(D, boxes[D][b], nb);
+      t.nb_list.insert(t.nb_list.end(), nb.begin(), nb.end());
+      t.nb_ptr.push_back((int)t.nb_list.size());
     }
-  if (bad) {
-    err = "point outside the relevant boxes of a level";
-    return false;
+  }
+  // Target groups per level: targets sharing their level-l box. For the
+  // forward map these are the relevant boxes themselves (every target is a
+  // source); for separate targets, the runs of equal level-l codes of the
+  // sorted targets (codes are ascending at every level). cz[l][grp] is the
+  // group's cousin list, pbox[i * (D+1) + l] the group of sorted target i.
+  const long long NT = t.M;
+  const double* TX = sep ? t.tx.data() : t.sx.data();
+  const double* TY = sep ? t.ty.data() : t.sy.data();
+  std::vector<std::vector<std::vector<int>>> cz(D + 1);
+  std::vector<int> pbox((size_t)NT * (D + 1), -1);
+  bool bad = false;
+  if (!sep) {
+    for (int l = 2; l <= D; l++) {
+      cz[l].resize(boxes[l].size());
+      for (size_t b = 0; b < boxes[l].size(); b++) cousins_of(l, boxes[l][b], cz[l][b]);
+    }
+    // per sorted point: box at each level (Point2Morton at that level)
+#pragma omp parallel for schedule(static) reduction(|| : bad)
+    for (long long i = 0; i < N; i++)
+      for (int l = 2; l <= D; l++) {
+        bool ok;
+        const long long m = G[l].point2morton(t.sx[i], t.sy[i], l, ok);
+        auto it = ok ? box_index[l].find(m) : box_index[l].end();
+        if (it == box_index[l].end()) {
+          bad = true;
+          continue;
+        }
+        pbox[(size_t)i * (D + 1) + l] = it->second;
+      }
+    if (bad) {
+      err = "point outside the relevant boxes of a level";
+      return false;
+    }
+  } else {
+    std::vector<long long> gcode;
+    for (int l = 2; l <= D; l++) {
+      gcode.clear();
+      for (long long i = 0; i < M; i++) {
+        bool ok;
+        const long long m = G[l].point2morton(TX[i], TY[i], l, ok);
+        if (!ok) {
+          err = "The 2D box index cannot be < 0 or larger than the number of boxes";
+          return false;
+        }
+        if (gcode.empty() || gcode.back() != m) gcode.push_back(m);
+        pbox[(size_t)i * (D + 1) + l] = (int)gcode.size() - 1;
+      }
+      cz[l].resize(gcode.size());
+      for (size_t gi = 0; gi < gcode.size(); gi++) cousins_of(l, gcode[gi], cz[l][gi]);
+      if (l == D) {
+        // near field of the targets: source leaf boxes around each leaf group
+        t.tnb_ptr.assign(1, 0);
+        std::vector<int> nb;
+        for (size_t gi = 0; gi < gcode.size(); gi++) {
+          neighbours_of(D, gcode[gi], nb);
+          t.tnb_list.insert(t.tnb_list.end(), nb.begin(), nb.end());
+          t.tnb_ptr.push_back((int)t.tnb_list.size());
+        }
+      }
+    }
+    t.tleaf.resize(M);
+    for (long long i = 0; i < M; i++) {
+      const int g = pbox[(size_t)i * (D + 1) + D];
+      t.tleaf[i] = g;
+      for (int n = t.tnb_ptr[g]; n < t.tnb_ptr[g + 1]; n++)
+        t.n_near_t += t.leaf_begin[t.tnb_list[n] + 1] - t.leaf_begin[t.tnb_list[n]];
+    }
   }
   // interpolation entries: per target, levels D..2, cousins ascending
   // (Interpolation, BoxTree.h:876-937); first the non-relevant cone ids.
-  t.it_ptr.assign(N + 1, 0);
-  for (long long i = 0; i < N; i++) {
+  t.it_ptr.assign(NT + 1, 0);
+  for (long long i = 0; i < NT; i++) {
     long long c = 0;
     for (int l = D; l >= 2; l--) c += (long long)cz[l][pbox[(size_t)i * (D + 1) + l]].size();
     t.it_ptr[i + 1] = t.it_ptr[i] + c;
   }
-  const long long nent = t.it_ptr[N];
+  const long long nent = t.it_ptr[NT];
+  if (nent > 2147483647LL) {
+    err = "too many interpolation entries";
+    return false;
+  }
   std::vector<int> ent_q(nent);
 #pragma omp parallel for schedule(dynamic, 256) reduction(|| : bad)
-  for (long long i = 0; i < N; i++) {
+  for (long long i = 0; i < NT; i++) {
     long long e = t.it_ptr[i];
     for (int l = D; l >= 2; l--)
       for (int c : cz[l][pbox[(size_t)i * (D + 1) + l]]) {
-        const long long q = G[l].nonrel(boxes[l][c], t.sx[i], t.sy[i]);
+        const long long q = G[l].nonrel(boxes[l][c], TX[i], TY[i]);
         if (q < 0) bad = true;
         ent_q[e++] = (int)q;
       }
@@ -567,7 +670,7 @@ bool build_tree(const double* x, const double* y, const double* nx, const double
   }
   t.dense_map.assign(map_total, -1);
   std::vector<char> mark(map_total, 0);
-  for (long long i = 0; i < N; i++) {
+  for (long long i = 0; i < NT; i++) {
     long long e = t.it_ptr[i];
     for (int l = D; l >= 2; l--)
       for (int c : cz[l][pbox[(size_t)i * (D + 1) + l]]) mark[t.lv[l].map_off + (long long)c * t.lv[l].nslot + ent_q[e++]] = 1;
@@ -623,7 +726,7 @@ bool build_tree(const double* x, const double* y, const double* nx, const double
         t.cone_post.push_back(q / L.nrad);
         t.cone_level.push_back((unsigned char)l);
       }
-    if (nc == 0) {
+    if (nc == 0 && !sep) {  // a field's coarse levels may have none (its targets are near every source)
       err = "The number of cone segments on any level cannot be 0";
       return false;
     }
@@ -636,7 +739,7 @@ bool build_tree(const double* x, const double* y, const double* nx, const double
   // final interpolation entries as global cone ids
   t.it_cone.resize(nent);
 #pragma omp parallel for schedule(static)
-  for (long long i = 0; i < N; i++) {
+  for (long long i = 0; i < NT; i++) {
     long long e = t.it_ptr[i];
     for (int l = D; l >= 2; l--)
       for (int c : cz[l][pbox[(size_t)i * (D + 1) + l]]) {
@@ -645,7 +748,7 @@ bool build_tree(const double* x, const double* y, const double* nx, const double
       }
   }
   t.n_interp = nent;
-  t.point_box.swap(pbox);
+  if (!sep) t.point_box.swap(pbox);
   return true;
 }
 

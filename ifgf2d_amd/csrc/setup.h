@@ -91,13 +91,30 @@ struct Tree {
   std::vector<int> it_cone;                    // global cone id per entry
   std::vector<int> point_box;                  // sorted point x level -> local box (N x (D+1))
   long long n_leaf = 0, n_prop = 0, n_interp = 0, n_cones = 0;
+  // Targets. The forward map's targets are its sorted sources (sep = false,
+  // M = N, the arrays below empty). A field tree (sep = true) holds M
+  // separate targets sorted by leaf Morton code (stable), the leaf group of
+  // every sorted target (targets sharing a leaf box) and, per group, the
+  // source leaf boxes of its 3x3 neighbourhood in ascending Morton order.
+  // it_ptr / it_cone then run over the sorted targets.
+  bool sep = false;
+  long long M = 0;
+  std::vector<int> tperm, tinv;                // sorted target -> original, original -> sorted
+  std::vector<double> tx, ty;                  // sorted targets
+  std::vector<int> tleaf, tnb_ptr, tnb_list;
+  long long n_near_t = 0;                      // (target, source) pairs of the field's near field
 };
 // perm_in (optional): the Morton permutation computed elsewhere (the device
 // sort); it is checked to be a permutation that orders the leaf codes with
 // ties in input order, i.e. exactly the host's stable sort.
+// tx, ty, M (optional): a separate target set (a field tree). The bounding
+// box then covers sources and targets; relevant cones are those the targets'
+// interpolations and the parents need, exactly as for the forward map with
+// the targets in place of the sources.
 bool build_tree(const double* x, const double* y, const double* nx, const double* ny, long long N,
                 int nlev, double k, int Ps, int Pang, Tree& t, std::string& err,
-                const std::vector<int>* perm_in = nullptr);
+                const std::vector<int>* perm_in = nullptr, const double* tx = nullptr,
+                const double* ty = nullptr, long long M = 0);
 
 // Near-field exclusion lists (init_sort_sing_point, ForwardMap.tpp:567-607) in
 // sorted order: own patch + patches the point is near-singular to, padded

@@ -518,9 +518,21 @@ struct Stats {
 };
 
 // BoxTree<Ps,Pang>, IFGF_Source/BoxTree.h:24-1171.
+//
+// Extension without a reference counterpart (the reference's targets are
+// always its sources): an optional separate target set tx, ty. The bounding
+// box then covers sources and targets, the targets are sorted by their leaf
+// Morton code like the sources, and the three loops over targets (cones due
+// to cousins, near field, interpolation) run over them instead of the
+// sources; the near field skips no index (the kernel is zero for r < 1e-14).
+// Without separate targets every step is the reference's.
 struct BoxTree {
   std::vector<double> x_, y_, nx_, ny_;
   std::vector<ll> sorting_, inverse_;
+  bool sep_ = false;                 // separate target set
+  std::vector<double> tx_, ty_;      // sorted targets (the sorted sources if !sep_)
+  std::vector<ll> tsorting_;
+  ll M_ = 0;
   int nlevels_;
   double k_, eta_;
   int F_;
@@ -531,17 +543,24 @@ struct BoxTree {
   Stats st_;
 
   BoxTree(const std::vector<double>& x, const std::vector<double>& y, const std::vector<double>& nx,
-          const std::vector<double>& ny, int nlev, double k, int F, double eta, int Ps, int Pang)
+          const std::vector<double>& ny, int nlev, double k, int F, double eta, int Ps, int Pang,
+          const std::vector<double>* tx = nullptr, const std::vector<double>* ty = nullptr)
       : x_(x), y_(y), nx_(nx), ny_(ny), inverse_(x.size()), nlevels_(nlev), k_(k), eta_(eta),
         F_(F), N_((ll)x.size()), ip_(Ps, Pang) {
     if (nlevels_ < 3) throw std::invalid_argument("IFGF accelerator requires at least 3 levels.");
+    if (tx && ty) {
+      sep_ = true;
+      tx_ = *tx;
+      ty_ = *ty;
+      M_ = (ll)tx_.size();
+    }
     InitializeBoxesAndLevels();
     InitializeRelevantConeSegments();
     ll mx = 0;
     for (int l = 2; l < nlevels_; l++) mx = std::max<ll>(mx, (ll)levels_[l]->rel2nonrel.size());
     cur_.assign(mx * ip_.P, 0.0);
     prev_.assign(mx * ip_.P, 0.0);
-    solution_.assign(N_, 0.0);
+    solution_.assign(M_, 0.0);
   }
 
   void InitializeBoxesAndLevels() {  // BoxTree.h:179-231
@@ -551,6 +570,12 @@ struct BoxTree {
       mn[1] = std::min(mn[1], y_[i]);
       mxv[0] = std::max(mxv[0], x_[i]);
       mxv[1] = std::max(mxv[1], y_[i]);
+    }
+    for (ll i = 0; i < M_; i++) {
+      mn[0] = std::min(mn[0], tx_[i]);
+      mn[1] = std::min(mn[1], ty_[i]);
+      mxv[0] = std::max(mxv[0], tx_[i]);
+      mxv[1] = std::max(mxv[1], ty_[i]);
     }
     mxv[0] += 1e-10;
     mxv[1] += 1e-10;
@@ -576,6 +601,27 @@ struct BoxTree {
     arrange(nx_);
     arrange(ny_);
     for (ll i = 0; i < N_; i++) inverse_[sorting_[i]] = i;
+    if (sep_) {
+      std::vector<ll> tcode(M_);
+      tsorting_.resize(M_);
+      for (ll i = 0; i < M_; i++) {
+        tsorting_[i] = i;
+        tcode[i] = point2morton(tx_[i], ty_[i], mn[0], mn[1], bs / (1ll << (nlevels_ - 1)), nlevels_ - 1);
+      }
+      std::stable_sort(tsorting_.begin(), tsorting_.end(), [&](ll a, ll b) { return tcode[a] < tcode[b]; });
+      std::vector<double> a(M_), b(M_);
+      for (ll i = 0; i < M_; i++) {
+        a[i] = tx_[tsorting_[i]];
+        b[i] = ty_[tsorting_[i]];
+      }
+      tx_.swap(a);
+      ty_.swap(b);
+    } else {
+      tx_ = x_;
+      ty_ = y_;
+      tsorting_ = sorting_;
+      M_ = N_;
+    }
     levels_.resize(nlevels_);
     for (int i = 1; i < nlevels_; i++)
       levels_[i].reset(new Level(i, nlevels_, mn[0], mn[1], bs / (1ll << i), k_));
@@ -610,13 +656,13 @@ struct BoxTree {
     ll old = -1;
     std::vector<ll> cz;
     std::map<ll, std::set<ll>> tmp;
-    for (ll i = 0; i < N_; i++) {
-      ll m = L.p2m(x_[i], y_[i]);
+    for (ll i = 0; i < M_; i++) {
+      ll m = L.p2m(tx_[i], ty_[i]);
       if (m != old) {
         cz = L.cousins(m);
         old = m;
       }
-      for (ll c : cz) tmp[c].insert(L.cart2nonrel(c, x_[i], y_[i]));
+      for (ll c : cz) tmp[c].insert(L.cart2nonrel(c, tx_[i], ty_[i]));
     }
     for (auto& kv : tmp) out[kv.first] = std::vector<ll>(kv.second.begin(), kv.second.end());
   }
@@ -675,7 +721,7 @@ struct BoxTree {
       std::map<ll, std::vector<ll>> ic, pc;
       ConesDueToCousins(level, ic);
       if (level > 2) ConesDueToParents(level, pc);
-      if (ic.empty() && pc.empty())
+      if (ic.empty() && pc.empty() && !sep_)  // separate targets: coarse levels may have none
         throw std::logic_error("The number of cone segments on any level cannot be 0");
       std::map<ll, std::set<ll>> merged;
       for (auto& kv : ic) merged[kv.first].insert(kv.second.begin(), kv.second.end());
@@ -696,8 +742,8 @@ struct BoxTree {
     Level& L = *levels_.back();
     ll old = -1;
     std::vector<ll> nb;
-    for (ll i = 0; i < N_; i++) {
-      ll m = L.p2m(x_[i], y_[i]);
+    for (ll i = 0; i < M_; i++) {
+      ll m = L.p2m(tx_[i], ty_[i]);
       cd res = solution_[i];
       if (m != old) {
         nb = L.neighbours(m);
@@ -707,9 +753,9 @@ struct BoxTree {
       for (ll b : nb) {
         const auto& pd = L.box2pts.at(b);
         for (ll s = pd[0]; s < pd[0] + pd[1]; s++) {
-          if (s == i) continue;
+          if (!sep_ && s == i) continue;
           if (excl && (*excl)[i].count(s)) continue;
-          res += kernel(F_, x_[i], y_[i], x_[s], y_[s], nx_[s], ny_[s], k_, eta_) * dens[s];
+          res += kernel(F_, tx_[i], ty_[i], x_[s], y_[s], nx_[s], ny_[s], k_, eta_) * dens[s];
           st_.near++;
         }
       }
@@ -786,8 +832,8 @@ struct BoxTree {
     Level& L = *levels_[level];
     ll old = -1;
     std::vector<ll> cz;
-    for (ll i = 0; i < N_; i++) {
-      const double x = x_[i], y = y_[i];
+    for (ll i = 0; i < M_; i++) {
+      const double x = tx_[i], y = ty_[i];
       cd v = solution_[i];
       ll m = L.p2m(x, y);
       if (m != old) {
@@ -807,7 +853,8 @@ struct BoxTree {
     }
   }
 
-  // Solve<F>, :1020-1071. dens in original order, overwritten with the result.
+  // Solve<F>, :1020-1071. dens in original order, overwritten with the result
+  // (with separate targets: resized to the targets, in their input order).
   void Solve(bool with_near, std::vector<cd>& dens, const std::vector<std::unordered_set<ll>>* excl) {
     st_.near = st_.leaf = st_.prop = st_.interp = 0;
     std::fill(solution_.begin(), solution_.end(), cd(0.0));
@@ -820,7 +867,8 @@ struct BoxTree {
       Interpolation(level);
       std::swap(cur_, prev_);
     }
-    for (ll i = 0; i < N_; i++) dens[sorting_[i]] = solution_[i];
+    dens.resize(M_);
+    for (ll i = 0; i < M_; i++) dens[tsorting_[i]] = solution_[i];
   }
 };
 
@@ -1587,6 +1635,35 @@ int oracle_boxtree_solve(const double* x, const double* y, const double* nx, con
     std::vector<cd> d((const cd*)dens, (const cd*)dens + N);
     b.Solve(true, d, nullptr);
     std::memcpy(out, d.data(), N * sizeof(cd));
+    if (stats) {
+      stats[0] = b.st_.near;
+      stats[1] = b.st_.leaf;
+      stats[2] = b.st_.prop;
+      stats[3] = b.st_.interp;
+      stats[4] = b.st_.cones;
+    }
+    return 0;
+  } catch (std::exception& e) {
+    g_err = e.what();
+    return 1;
+  }
+}
+
+// BoxTree::Solve with a separate target set (no reference counterpart; the
+// IFGF-accelerated form of propagate_unacc, ForwardMap.tpp:702-727, taking the
+// intensities): out[t] = sum_s G(target t, source s) dens[s], near field
+// included, zero for coincident points.
+int oracle_field_solve(const double* x, const double* y, const double* nx, const double* ny, long long N,
+                       const double* tx, const double* ty, long long M, int nlev, double k, int F, double eta,
+                       int Ps, int Pang, const double* dens, double* out, long long* stats) {
+  try {
+    const std::vector<double> vtx(tx, tx + M), vty(ty, ty + M);
+    BoxTree b(std::vector<double>(x, x + N), std::vector<double>(y, y + N),
+              std::vector<double>(nx, nx + N), std::vector<double>(ny, ny + N), nlev, k, F, eta,
+              Ps, Pang, &vtx, &vty);
+    std::vector<cd> d((const cd*)dens, (const cd*)dens + N);
+    b.Solve(true, d, nullptr);
+    std::memcpy(out, d.data(), M * sizeof(cd));
     if (stats) {
       stats[0] = b.st_.near;
       stats[1] = b.st_.leaf;

@@ -59,6 +59,10 @@ def lib():
         L.oracle_boxtree_solve.argtypes = [_dp, _dp, _dp, _dp, ctypes.c_longlong, ctypes.c_int,
                                            ctypes.c_double, ctypes.c_int, ctypes.c_double,
                                            ctypes.c_int, ctypes.c_int, _dp, _dp, _lp]
+        L.oracle_field_solve.argtypes = [_dp, _dp, _dp, _dp, ctypes.c_longlong, _dp, _dp,
+                                         ctypes.c_longlong, ctypes.c_int, ctypes.c_double,
+                                         ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                         _dp, _dp, _lp]
         L.oracle_direct.argtypes = [_dp, _dp, _dp, _dp, ctypes.c_longlong, ctypes.c_double,
                                     ctypes.c_int, ctypes.c_double, _dp, _lp, ctypes.c_longlong,
                                     _dp]
@@ -217,6 +221,24 @@ def boxtree_solve(x, y, nx, ny, dens, nlevels, k, formulation=SINGLE_LAYER, eta=
     rc = lib().oracle_boxtree_solve(*[_d(a) for a in arrs], arrs[0].size, nlevels, k, formulation,
                                     eta, Ps, Pang, d.ctypes.data_as(_dp), out.ctypes.data_as(_dp),
                                     _l(st))
+    if rc != 0:
+        raise RuntimeError(_err())
+    return out, dict(zip(("near", "leaf", "prop", "interp", "cones"), st.tolist()))
+
+
+def field_solve(x, y, nx, ny, dens, tx, ty, nlevels, k, formulation=SINGLE_LAYER, eta=0.0, Ps=5,
+                Pang=5):
+    """BoxTree solve with a separate target set (tx, ty): the field of the
+    intensities dens at the targets, near field included (coincident points
+    contribute zero). Returns (values in target order, counts)."""
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, nx, ny)]
+    t = [np.ascontiguousarray(a, dtype=np.float64) for a in (tx, ty)]
+    d = np.ascontiguousarray(dens, dtype=np.complex128)
+    out = np.empty(t[0].size, dtype=np.complex128)
+    st = np.empty(5, dtype=np.int64)
+    rc = lib().oracle_field_solve(*[_d(a) for a in arrs], arrs[0].size, _d(t[0]), _d(t[1]), t[0].size,
+                                  nlevels, k, formulation, eta, Ps, Pang, d.ctypes.data_as(_dp),
+                                  out.ctypes.data_as(_dp), _l(st))
     if rc != 0:
         raise RuntimeError(_err())
     return out, dict(zip(("near", "leaf", "prop", "interp", "cones"), st.tolist()))

@@ -117,6 +117,49 @@ def test_error_paths():
         fm.compute_Ax_acc(np.ones(fm.N))
 
 
+FIELD_TARGETS = {
+    "ring": lambda rng, x, y: (1.5 * np.cos(rng.uniform(0, 2 * np.pi, 300)),
+                               1.5 * np.sin(rng.uniform(0, 2 * np.pi, 300))),
+    "grid": lambda rng, x, y: tuple(a.ravel() for a in np.meshgrid(np.linspace(-2, 2, 21),
+                                                                    np.linspace(-2, 2, 21))),
+    "points": lambda rng, x, y: (x, y),
+    "far": lambda rng, x, y: (np.array([0.1, 9.0, -7.5]), np.array([0.2, 3.0, -8.0])),
+}
+
+
+@pytest.mark.parametrize("which", sorted(FIELD_TARGETS))
+def test_field_tables_match_oracle(which):
+    """The separate-target tree of ifgf_field_create (host tables of a host-only
+    plan) against the oracle's: identical work counts per target set, and the
+    default level count keeps the leaf size when the box grows."""
+    fm = I.ForwardMap(I.circle(), I.equal_parameter_lims(80), 10.0, 4, host_only=True)
+    x, y, nx, ny, w = fm.points()
+    tx, ty = FIELD_TARGETS[which](np.random.default_rng(4), x, y)
+    f = fm.field(tx, ty)
+    st = f.stats()
+    _, so = O.field_solve(x, y, nx, ny, np.ones(fm.N), tx, ty, f.nlevels, 10.0, O.COMBINED, 10.0)
+    for key in ("near", "leaf", "prop", "interp", "cones"):
+        assert st[key] == so[key], key
+    assert st["targets"] == tx.size
+    assert f.nlevels == {"ring": 5, "grid": 5, "points": 4, "far": 7}[which]
+    with pytest.raises(I.IFGFError, match="host-only"):
+        f.evaluate(np.ones(fm.N))
+
+
+def test_field_error_paths():
+    fm = I.ForwardMap(I.circle(), I.equal_parameter_lims(40), 10.0, 3, host_only=True)
+    with pytest.raises(I.IFGFError, match="target count"):
+        fm.field(np.zeros(0), np.zeros(0))
+    with pytest.raises(I.IFGFError, match="at least 3 levels"):
+        fm.field(np.ones(3), np.ones(3), nlevels=2)
+    with pytest.raises(I.IFGFError, match="differ in length"):
+        fm.field(np.ones(3), np.ones(2))
+    with pytest.raises(I.IFGFError, match="Ps and Pang"):
+        fm.field(np.ones(3), np.ones(3), Ps=17)
+    with pytest.raises(KeyError):
+        fm.field(np.ones(3), np.ones(3), precision="fp16")
+
+
 def test_host_gmres_through_linear_operator():
     """ifgf_gmres_host: the reference's Householder GMRES driving a callback
     LinearOperator (LinearOperator.cpp:14-23), against a dense solve."""

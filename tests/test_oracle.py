@@ -147,6 +147,49 @@ def test_boxtree_standalone_vs_direct():
     assert st["leaf"] > 0 and st["interp"] > 0 and st["prop"] > 0
 
 
+def test_field_solve_reduces_to_boxtree_at_the_points():
+    """The separate-target BoxTree (no reference counterpart) with the targets
+    equal to the points is the reference's BoxTree::Solve: same tree, same
+    cones, same sums (the self pair it adds has a zero kernel)."""
+    n = 1200
+    th = 2 * np.pi * (np.arange(n) + 0.25) / n
+    x, y = np.cos(th) + 0.3 * np.cos(2 * th), 1.2 * np.sin(th)
+    rng = np.random.default_rng(2)
+    d = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    for F, eta in ((O.SINGLE_LAYER, 0.0), (O.COMBINED, 25.0)):
+        a, sa = O.boxtree_solve(x, y, np.cos(th), np.sin(th), d, nlevels=5, k=25.0, formulation=F, eta=eta)
+        b, sb = O.field_solve(x, y, np.cos(th), np.sin(th), d, x, y, nlevels=5, k=25.0, formulation=F,
+                              eta=eta)
+        assert np.array_equal(a, b)
+        assert sb["near"] == sa["near"] + n
+        for key in ("leaf", "prop", "interp", "cones"):
+            assert sa[key] == sb[key], key
+
+
+@pytest.mark.parametrize("F", [O.SINGLE_LAYER, O.DOUBLE_LAYER, O.COMBINED])
+def test_field_solve_converges_to_propagate_unacc(F):
+    """IFGF at separate targets (a ring outside the circle and a grid through
+    it) against the direct field sum of propagate_unacc (ForwardMap.tpp:702-727):
+    the error falls geometrically with the interpolation order."""
+    om = circle_fm(80, 10.0, F, 4)
+    x, y, nx, ny, w = om.points()
+    rng = np.random.default_rng(3)
+    u = rng.standard_normal(om.N) + 1j * rng.standard_normal(om.N)
+    tt = rng.uniform(0, 2 * np.pi, 200)
+    gx, gy = np.meshgrid(np.linspace(-2.0, 2.0, 17), np.linspace(-2.0, 2.0, 17))
+    tx = np.concatenate([1.5 * np.cos(tt), gx.ravel()])
+    ty = np.concatenate([1.5 * np.sin(tt), gy.ravel()])
+    ref = om.propagate(tx, ty, u)
+    errs = []
+    for P in (5, 7, 9):
+        out, st = O.field_solve(x, y, nx, ny, u * w, tx, ty, nlevels=5, k=10.0, formulation=F, eta=10.0,
+                                Ps=P, Pang=P)
+        assert st["near"] > 0 and st["interp"] > 0 and st["prop"] > 0
+        errs.append(rel(out, ref))
+    assert 1e-5 < errs[0] < 1e-2
+    assert errs[1] < errs[0] / 5 and errs[2] < errs[1] / 5
+
+
 def test_gmres_dense_matches_direct_solve():
     rng = np.random.default_rng(2)
     n = 60
