@@ -83,6 +83,7 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-group", action="store_true")
+    ap.add_argument("--no-field", action="store_true")
     ap.add_argument("--workload", choices=("kite", "star"), default="kite",
                     help="kite: the 64k kite at k=200 (default); star: the 256k star r=1+0.3cos5t at k=500")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -325,6 +326,45 @@ def main():
         grp.close()
         del maps, gx, gy
 
+    # the scattered field on a 256 x 256 grid over the scatterer (65,536 targets)
+    # through the IFGF field evaluator (5x5 and 9x9) against the GPU direct sum
+    field = None
+    if not args.no_field and args.workload == "kite":
+        g = np.linspace(-2.0, 2.0, 256)
+        tx, ty = (a_.ravel().copy() for a_ in np.meshgrid(g, g))
+        u = torch.from_numpy(x_host).to(dev)
+        sub = np.random.default_rng(7).choice(tx.size, 2000, replace=False)
+        t0 = time.perf_counter()
+        direct_all = fm.propagate_unacc(tx, ty, x_host)
+        t_direct = time.perf_counter() - t0
+        field = {"targets": int(tx.size), "grid": "256 x 256 on [-2, 2]^2",
+                 "direct_sum_s": round(t_direct, 4)}
+        for P_ in (5, 9):
+            t0 = time.perf_counter()
+            f = fm.field(tx, ty, Ps=P_, Pang=P_)
+            t_create = time.perf_counter() - t0
+            out_d = torch.empty(f.M, dtype=torch.complex128, device=dev)
+            for _ in range(3):
+                f.apply_device(u.data_ptr(), out_d.data_ptr())
+            torch.cuda.synchronize()
+            reps = max(10, args.steps // 20)
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                f.apply_device(u.data_ptr(), out_d.data_ptr())
+            torch.cuda.synchronize()
+            t_eval = (time.perf_counter() - t0) / reps
+            vals = out_d.cpu().numpy()
+            err = float(np.linalg.norm(vals[sub] - direct_all[sub]) / np.linalg.norm(direct_all[sub]))
+            fs = f.stats()
+            field[f"{P_}x{P_}"] = {"ms_per_evaluation": round(t_eval * 1e3, 4), "create_s": round(t_create, 3),
+                                   "rel_err_vs_direct": err, "levels": fs["nlevels"], "cones": fs["cones"],
+                                   "stored_bytes": fs["stored_bytes"],
+                                   "speedup_vs_direct": round(t_direct / t_eval, 1)}
+            log(f"field {P_}x{P_}: {t_eval * 1e3:.3f} ms per evaluation of 65,536 targets "
+                f"(direct sum {t_direct:.3f} s), rel err {err:.2e}")
+            f.close()
+            del out_d
+
     cpu = None if args.no_cpu_baseline else cpu_baseline(
         kw, x_host, "64,000-point kite at k=200" if args.workload == "kite" else "256,000-point star at k=500")
 
@@ -349,6 +389,7 @@ def main():
         "mixed_precision": mixed,
         "batched_rhs": batched,
         "obstacle_group": group,
+        "field": field,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -45,6 +45,17 @@ int main(void) {
   double x[800] = {0}, y[800];
   CHECK(ifgf_apply(plan, x, y, 1, 400, 0, NULL) == IFGF_ERR_NO_DEVICE);
   CHECK(ifgf_plan_error(plan)[0] != '\0');
+  /* field of a host-only plan: tables and counts, no evaluation */
+  double tx[3] = {1.5, 0.0, -3.0}, ty[3] = {0.0, 1.5, 2.0};
+  ifgf_field* field = NULL;
+  CHECK(ifgf_field_create(plan, tx, ty, 3, 5, 5, 5, IFGF_FP64, &field) == IFGF_OK && field != NULL);
+  long long fs[8];
+  CHECK(ifgf_field_stats(field, fs) == IFGF_OK);
+  CHECK(fs[0] == 3 && fs[2] > 0 && fs[5] > 0 && fs[6] == 5);
+  CHECK(ifgf_field_apply(field, x, y, 0, NULL) == IFGF_ERR_NO_DEVICE);
+  ifgf_field_destroy(field);
+  CHECK(ifgf_field_create(plan, tx, ty, 0, 5, 5, 5, IFGF_FP64, &field) == IFGF_ERR_ARG);
+  CHECK(ifgf_field_create(plan, tx, ty, 3, 2, 5, 5, IFGF_FP64, &field) == IFGF_ERR_LEVELS);
   ifgf_plan_destroy(plan);
   /* argument errors */
   prm.nlevels = 2;
