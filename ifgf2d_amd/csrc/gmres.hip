@@ -3,16 +3,20 @@
 // Replaces ForwardMap::solve's Eigen::opGMRES<LinearOperator>
 // (ForwardMap.tpp:658-697; Householder Arnoldi in opGMRES.hpp:68-269) with an
 // Arnoldi process that keeps the Krylov basis V (n x (m+1) complex, column
-// major) in HBM and orthogonalises by classical Gram-Schmidt with one
-// re-orthogonalisation pass (CGS2), all in FP64. The projections stay on the
-// device; one (j+2)-value copy per iteration brings the new Hessenberg column
-// to the host, where the Givens rotations and the residual estimate
-// |w(k)| / ||b - A x0|| (opGMRES.hpp:207) are updated. An iteration is six
-// kernels besides the operator (each projection's partial sums are reduced
-// by the kernel that consumes them), and the host applies the rotations of
-// iteration j while the device already runs iteration j+1 (which needs only
-// the new basis vector): at convergence that one speculative iteration is
-// discarded.
+// major) in HBM and orthogonalises by classical Gram-Schmidt with selective
+// re-orthogonalisation, all in FP64: a second Gram-Schmidt pass runs when the
+// first removed more than half of the new vector's squared norm (the "twice
+// is enough" test of Daniel, Gragg, Kaufman and Stewart, 1976), which the
+// device decides from the norms the first pass already summed. The
+// projections stay on the device; the normalising kernel writes the new
+// Hessenberg column h[0..j+1] straight into pinned host memory, where the
+// Givens rotations and the residual estimate |w(k)| / ||b - A x0||
+// (opGMRES.hpp:207) are updated. An iteration is four kernels besides the
+// operator, six when the second pass runs (each projection's partial sums are
+// reduced by the kernel that consumes them), and the host applies the
+// rotations of iteration j while the device already runs iteration j+1
+// (which needs only the new basis vector): at convergence that one
+// speculative iteration is discarded.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -29,11 +33,34 @@ namespace {
 constexpr int kDot = 64;       // partial sums per projection
 constexpr int kMaxRestart = 256;
 
-// part[col * kDot + blockIdx.x] = sum over this block's rows of conj(V[:,col]) * w
+// Sum of the nb values x[0..nb) in a fixed order, by the first wave of the
+// block (lane l takes l, l + 64, ...; then a butterfly): every block that
+// calls it gets the same bits. All threads of the block must call it.
+__device__ inline double block_sum_fixed(const double* __restrict__ x, int nb) {
+  __shared__ double r;
+  if (threadIdx.x < 64) {
+    double a = 0.0;
+    for (int i = threadIdx.x; i < nb; i += 64) a += x[i];
+    for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
+    if (threadIdx.x == 0) r = a;
+  }
+  __syncthreads();
+  return r;
+}
+
+// part[col * kDot + blockIdx.x] = sum over this block's rows of conj(V[:,col]) * w;
+// the column col == wcol (if any) takes w itself (its squared norm).
+// gate_w / gate_n (second Gram-Schmidt pass only): skip unless
+// second_pass(gate_w, gate_n, ...) holds.
+__device__ inline bool second_pass(const zd* __restrict__ part_w, int wcol, const double* __restrict__ part_n, int nbn);
+
 __global__ void __launch_bounds__(256)
-    k_dots(long long n, const zd* __restrict__ V, long long ldv, const zd* __restrict__ w, zd* __restrict__ part) {
+    k_dots(long long n, const zd* __restrict__ V, long long ldv, const zd* __restrict__ w, zd* __restrict__ part,
+           int wcol = -1, const zd* __restrict__ gate_w = nullptr, const double* __restrict__ gate_n = nullptr,
+           int gate_nb = 0) {
+  if (gate_w && !second_pass(gate_w, wcol, gate_n, gate_nb)) return;
   const int col = blockIdx.y;
-  const zd* v = V + (size_t)col * ldv;
+  const zd* v = col == wcol ? w : V + (size_t)col * ldv;
   zd acc{0.0, 0.0};
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
     const zd a = v[i], b = w[i];
@@ -64,22 +91,68 @@ __device__ inline zd sum_partials(const zd* __restrict__ part, int c) {
   return cadd(cadd(a[0], a[1]), cadd(a[2], a[3]));
 }
 
+// Second Gram-Schmidt pass needed: ||w'||^2 (the nbn per-block partials of
+// the first pass's output) < ||w||^2 / 2 (column wcol of the first pass's
+// dot partials). Deterministic and the same in every block.
+__device__ inline bool second_pass(const zd* __restrict__ part_w, int wcol, const double* __restrict__ part_n,
+                                   int nbn) {
+  const double after = block_sum_fixed(part_n, nbn);
+  __shared__ double before;
+  if (threadIdx.x == 0) before = sum_partials(part_w, wcol).x;
+  __syncthreads();
+  return after < 0.5 * before;
+}
+
 // Every block reduces the projections h[c] = sum_b part[c][b] (c < ncol);
 // block 0 stores them into hio (or adds them, for the second Gram-Schmidt
-// pass); then each thread updates its row: w -= V h.
+// pass); then each thread updates its row, w -= V h, and the block writes
+// the partial sum of |w_i|^2 over its rows to part_n[blockIdx.x]. A second
+// pass (gate_w set) runs only when second_pass() holds.
 __global__ void __launch_bounds__(256)
     k_project_out(long long n, int ncol, const zd* __restrict__ V, long long ldv, const zd* __restrict__ part,
-                  zd* __restrict__ hio, int accumulate, zd* __restrict__ w) {
+                  zd* __restrict__ hio, int accumulate, zd* __restrict__ w, double* __restrict__ part_n,
+                  const zd* __restrict__ gate_w = nullptr, const double* __restrict__ gate_n = nullptr) {
+  if (gate_w && !second_pass(gate_w, ncol, gate_n, gridDim.x)) return;
   __shared__ zd hs[kMaxRestart + 1];
+  __shared__ double red[256];
   for (int c = threadIdx.x; c < ncol; c += blockDim.x) hs[c] = sum_partials(part, c);
   __syncthreads();
   if (blockIdx.x == 0)
     for (int c = threadIdx.x; c < ncol; c += blockDim.x) hio[c] = accumulate ? cadd(hio[c], hs[c]) : hs[c];
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  double sq = 0.0;
+  if (i < n) {
+    zd acc = w[i];
+    for (int c = 0; c < ncol; c++) cfma(acc, cscale(hs[c], -1.0), V[(size_t)c * ldv + i]);
+    w[i] = acc;
+    sq = fma(acc.x, acc.x, acc.y * acc.y);
+  }
+  red[threadIdx.x] = sq;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part_n[blockIdx.x] = red[0];
+}
+
+// Arnoldi step end: ||w||^2 = the first pass's partials part_n1, or the
+// second's (part_n2) when second_pass() held; V[:, j+1] = w / ||w||; block 0
+// stores ||w|| into h[ncol] and writes the column h[0..ncol] to hcol (pinned
+// host memory).
+__global__ void __launch_bounds__(256)
+    k_arnoldi_end(long long n, int ncol, const zd* __restrict__ w, const zd* __restrict__ part_w,
+                  const double* __restrict__ part_n1, const double* __restrict__ part_n2, zd* __restrict__ dst,
+                  zd* __restrict__ h, zd* __restrict__ hcol) {
+  const bool two = second_pass(part_w, ncol, part_n1, gridDim.x);
+  const double nn = sqrt(block_sum_fixed(two ? part_n2 : part_n1, gridDim.x));
+  if (blockIdx.x == 0) {
+    for (int c = threadIdx.x; c <= ncol; c += blockDim.x) hcol[c] = c < ncol ? h[c] : zd{nn, 0.0};
+    if (threadIdx.x == 0) h[ncol] = zd{nn, 0.0};
+  }
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  zd acc = w[i];
-  for (int c = 0; c < ncol; c++) cfma(acc, cscale(hs[c], -1.0), V[(size_t)c * ldv + i]);
-  w[i] = acc;
+  dst[i] = cscale(w[i], 1.0 / nn);
 }
 
 // dst = src / ||src|| with ||src||^2 = sum of the kDot partials (part from
@@ -257,16 +330,20 @@ void apply_givens(const Givens& g, std::complex<double>& a, std::complex<double>
 // consecutive iterations.
 struct Krylov {
   DevBuf<zd> V, w, tmp, part, h;
+  DevBuf<double> part_n;  // per-block squared norms of the two Gram-Schmidt passes
   zd* hpin = nullptr;
+  zd* hpin_dev = nullptr;  // hpin as a device pointer
   hipEvent_t ev[2] = {nullptr, nullptr};
   int m;
   Krylov(long long n, int m_) : m(m_) {
     V.alloc((size_t)n * (m + 1));
     w.alloc(n);
     tmp.alloc(n);
-    part.alloc((size_t)kDot * (m + 2));
+    part.alloc((size_t)kDot * (2 * m + 4));
     h.alloc(m + 2);
+    part_n.alloc((size_t)2 * ((n + 255) / 256));
     HIP_TRY(hipHostMalloc((void**)&hpin, 2 * (m + 2) * sizeof(zd), hipHostMallocDefault));
+    HIP_TRY(hipHostGetDevicePointer((void**)&hpin_dev, hpin, 0));
     for (auto& e : ev) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   }
   ~Krylov() {
@@ -308,18 +385,28 @@ struct Solver {
     apply(x, r, fp64);
     hipLaunchKernelGGL(k_residual, dim3(nb), dim3(256), 0, st, n, b, r);
   }
-  // Orthogonalises w against V[:, 0..j] (CGS2), normalises it into
-  // V[:, j+1] and copies the Hessenberg column h[0..j+1] into pinned slot
-  // j % 2, recording its event. Asynchronous.
+  // Orthogonalises w against V[:, 0..j] (classical Gram-Schmidt, a second
+  // pass when the first cancelled more than half of ||w||^2), normalises it
+  // into V[:, j+1] and writes the Hessenberg column h[0..j+1] into pinned
+  // slot j % 2, recording its event. Asynchronous. The second pass's dot
+  // partials go after the first's (columns nc+1 .. 2nc), so the first pass's
+  // ||w||^2 partials (column nc) stay readable for the test.
   void arnoldi_enqueue(Krylov& K, int j) {
     const int nc = j + 1;
-    hipLaunchKernelGGL(k_dots, dim3(kDot, nc), dim3(256), 0, st, n, K.V.p, n, K.w.p, K.part.p);
-    hipLaunchKernelGGL(k_project_out, dim3(nb), dim3(256), 0, st, n, nc, K.V.p, n, K.part.p, K.h.p, 0, K.w.p);
-    hipLaunchKernelGGL(k_dots, dim3(kDot, nc), dim3(256), 0, st, n, K.V.p, n, K.w.p, K.part.p);
-    hipLaunchKernelGGL(k_project_out, dim3(nb), dim3(256), 0, st, n, nc, K.V.p, n, K.part.p, K.h.p, 1, K.w.p);
-    normalize(K, K.w.p, K.V.p + (size_t)(j + 1) * n, K.h.p + j + 1);
-    zd* slot = K.hpin + (size_t)(j % 2) * (K.m + 2);
-    HIP_TRY(hipMemcpyAsync(slot, K.h.p, (j + 2) * sizeof(zd), hipMemcpyDeviceToHost, st));
+    zd* part2 = K.part.p + (size_t)(nc + 1) * kDot;
+    double* pn1 = K.part_n.p;
+    double* pn2 = K.part_n.p + nb;
+    hipLaunchKernelGGL(k_dots, dim3(kDot, nc + 1), dim3(256), 0, st, n, K.V.p, n, K.w.p, K.part.p, nc,
+                       (const zd*)nullptr, (const double*)nullptr, 0);
+    hipLaunchKernelGGL(k_project_out, dim3(nb), dim3(256), 0, st, n, nc, K.V.p, n, K.part.p, K.h.p, 0, K.w.p, pn1,
+                       (const zd*)nullptr, (const double*)nullptr);
+    hipLaunchKernelGGL(k_dots, dim3(kDot, nc), dim3(256), 0, st, n, K.V.p, n, K.w.p, part2, nc,
+                       (const zd*)K.part.p, (const double*)pn1, nb);
+    hipLaunchKernelGGL(k_project_out, dim3(nb), dim3(256), 0, st, n, nc, K.V.p, n, part2, K.h.p, 1, K.w.p, pn2,
+                       (const zd*)K.part.p, (const double*)pn1);
+    zd* slot = K.hpin_dev + (size_t)(j % 2) * (K.m + 2);
+    hipLaunchKernelGGL(k_arnoldi_end, dim3(nb), dim3(256), 0, st, n, nc, K.w.p, K.part.p, pn1, pn2,
+                       K.V.p + (size_t)(j + 1) * n, K.h.p, slot);
     HIP_TRY(hipEventRecord(K.ev[j % 2], st));
   }
   // waits for iteration j's Hessenberg column
