@@ -199,12 +199,25 @@ __global__ void __launch_bounds__(256) k_residual(long long n, const zd* __restr
 // operator wrote), its partials at part + a * ps, its Hessenberg column at
 // H + sys[a] * hs.
 
+// second[a] = second_pass() of slot a (one block per slot): the gated
+// kernels of the second pass then read one flag per system.
+__global__ void __launch_bounds__(64)
+    k_gate_ms(int ncol, const zd* __restrict__ part, long long ps, const double* __restrict__ part_n, long long pns,
+              int nbn, int* __restrict__ second) {
+  const int a = blockIdx.x;
+  const bool two = second_pass(part + (size_t)a * ps, ncol, part_n + (size_t)a * pns, nbn);
+  if (threadIdx.x == 0) second[a] = two ? 1 : 0;
+}
+
+// Column wcol takes W[:, a] itself; gate (second pass): runs for slot a only
+// when gate[a] is set.
 __global__ void __launch_bounds__(256)
     k_dots_ms(long long n, const zd* __restrict__ V, long long vs, const zd* __restrict__ W, zd* __restrict__ part,
-              long long ps, const int* __restrict__ sys) {
+              long long ps, const int* __restrict__ sys, int wcol, const int* __restrict__ gate) {
   const int col = blockIdx.y, a = blockIdx.z;
-  const zd* v = V + (size_t)sys[a] * vs + (size_t)col * n;
+  if (gate && !gate[a]) return;
   const zd* w = W + (size_t)a * n;
+  const zd* v = col == wcol ? w : V + (size_t)sys[a] * vs + (size_t)col * n;
   zd acc{0.0, 0.0};
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
     const zd x = v[i], y = w[i];
@@ -225,24 +238,54 @@ __global__ void __launch_bounds__(256)
   if (threadIdx.x == 0) part[(size_t)a * ps + (size_t)col * kDot + blockIdx.x] = zd{sx[0], sy[0]};
 }
 
+// k_project_out for slot a = blockIdx.y (system sys[a]); the per-block
+// squared norms go to part_n + a * pns.
 __global__ void __launch_bounds__(256)
     k_project_out_ms(long long n, int ncol, const zd* __restrict__ V, long long vs, const zd* __restrict__ part,
                      long long ps, zd* __restrict__ H, long long hs, int accumulate, zd* __restrict__ W,
-                     const int* __restrict__ sys) {
+                     const int* __restrict__ sys, double* __restrict__ part_n, long long pns,
+                     const int* __restrict__ gate) {
   const int a = blockIdx.y, s = sys[a];
+  if (gate && !gate[a]) return;
   __shared__ zd hsh[kMaxRestart + 1];
+  __shared__ double red[256];
   for (int c = threadIdx.x; c < ncol; c += blockDim.x) hsh[c] = sum_partials(part + (size_t)a * ps, c);
   __syncthreads();
   zd* h = H + (size_t)s * hs;
   if (blockIdx.x == 0)
     for (int c = threadIdx.x; c < ncol; c += blockDim.x) h[c] = accumulate ? cadd(h[c], hsh[c]) : hsh[c];
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  double sq = 0.0;
+  if (i < n) {
+    const zd* v = V + (size_t)s * vs;
+    zd* w = W + (size_t)a * n;
+    zd acc = w[i];
+    for (int c = 0; c < ncol; c++) cfma(acc, cscale(hsh[c], -1.0), v[(size_t)c * n + i]);
+    w[i] = acc;
+    sq = fma(acc.x, acc.x, acc.y * acc.y);
+  }
+  red[threadIdx.x] = sq;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part_n[(size_t)a * pns + blockIdx.x] = red[0];
+}
+
+// k_arnoldi_end for slot a = blockIdx.y: V_s[:, col] = W[:, a] / ||W[:, a]||,
+// H[s][col] = the norm (the second pass's when second_pass() held).
+__global__ void __launch_bounds__(256)
+    k_arnoldi_end_ms(long long n, int ncol, const zd* __restrict__ W, const int* __restrict__ second,
+                     const double* __restrict__ part_n1, const double* __restrict__ part_n2, long long pns,
+                     zd* __restrict__ V, long long vs, zd* __restrict__ H, long long hs, const int* __restrict__ sys) {
+  const int a = blockIdx.y, s = sys[a];
+  const bool two = second[a] != 0;
+  const double nn = sqrt(block_sum_fixed((two ? part_n2 : part_n1) + (size_t)a * pns, gridDim.x));
+  if (blockIdx.x == 0 && threadIdx.x == 0) H[(size_t)s * hs + ncol] = zd{nn, 0.0};
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const zd* v = V + (size_t)s * vs;
-  zd* w = W + (size_t)a * n;
-  zd acc = w[i];
-  for (int c = 0; c < ncol; c++) cfma(acc, cscale(hsh[c], -1.0), v[(size_t)c * n + i]);
-  w[i] = acc;
+  V[(size_t)s * vs + (size_t)ncol * n + i] = cscale(W[(size_t)a * n + i], 1.0 / nn);
 }
 
 // partials of ||W[:, a]||^2
@@ -641,17 +684,21 @@ extern "C" int ifgf_gmres_multi(ifgf_plan* P, const double* rhs, double* x, int 
   using C = std::complex<double>;
   const long long n = P->geo.N;
   const int m = restart, R = nrhs;
-  const long long vs = n * (m + 1), hs = m + 2, ps = (long long)kDot * (m + 2);
+  const long long vs = n * (m + 1), hs = m + 2, ps = (long long)kDot * (2 * m + 4);
   const int nb = (int)((n + 255) / 256);
+  const long long pns = 2 * (long long)nb;
   try {
     hipStream_t st = P->stream;
     DevBuf<zd> B, X, V, W, part, H;
-    DevBuf<int> sysd;
+    DevBuf<double> part_n;
+    DevBuf<int> sysd, gate;
     B.alloc((size_t)n * R);
     X.alloc((size_t)n * R);
     V.alloc((size_t)vs * R);
     W.alloc((size_t)n * 2 * R);  // operator input block, then its output block
     part.alloc((size_t)ps * R);
+    part_n.alloc((size_t)pns * R);
+    gate.alloc(R);
     H.alloc((size_t)hs * R);
     sysd.alloc(R);
     HIP_TRY(hipMemcpyAsync(B.p, rhs, (size_t)n * R * sizeof(zd), hipMemcpyHostToDevice, st));
@@ -728,16 +775,22 @@ extern "C" int ifgf_gmres_multi(ifgf_plan* P, const double* rhs, double* x, int 
         hipLaunchKernelGGL(k_gather_ms, dim3(nb, na), dim3(256), 0, st, n, V.p, vs, j, sysd.p, in);
         int rc = apply_columns(P, in, n, Wa, na, st, false);
         if (rc) return rc;
+        // Gram-Schmidt with selective re-orthogonalisation, as in arnoldi_enqueue
         const int nc = j + 1;
-        hipLaunchKernelGGL(k_dots_ms, dim3(kDot, nc, na), dim3(256), 0, st, n, V.p, vs, Wa, part.p, ps, sysd.p);
+        zd* part2 = part.p + (size_t)(nc + 1) * kDot;
+        double* pn1 = part_n.p;
+        double* pn2 = part_n.p + nb;
+        hipLaunchKernelGGL(k_dots_ms, dim3(kDot, nc + 1, na), dim3(256), 0, st, n, V.p, vs, Wa, part.p, ps, sysd.p, nc,
+                           (const int*)nullptr);
         hipLaunchKernelGGL(k_project_out_ms, dim3(nb, na), dim3(256), 0, st, n, nc, V.p, vs, part.p, ps, H.p, hs, 0, Wa,
-                           sysd.p);
-        hipLaunchKernelGGL(k_dots_ms, dim3(kDot, nc, na), dim3(256), 0, st, n, V.p, vs, Wa, part.p, ps, sysd.p);
-        hipLaunchKernelGGL(k_project_out_ms, dim3(nb, na), dim3(256), 0, st, n, nc, V.p, vs, part.p, ps, H.p, hs, 1, Wa,
-                           sysd.p);
-        hipLaunchKernelGGL(k_norm_parts_ms, dim3(kDot, 1, na), dim3(256), 0, st, n, Wa, part.p, ps);
-        hipLaunchKernelGGL(k_normalize_ms, dim3(nb, na), dim3(256), 0, st, n, Wa, part.p, ps, V.p, vs, j + 1, H.p, hs,
-                           sysd.p);
+                           sysd.p, pn1, pns, (const int*)nullptr);
+        hipLaunchKernelGGL(k_gate_ms, dim3(na), dim3(64), 0, st, nc, part.p, ps, pn1, pns, nb, gate.p);
+        hipLaunchKernelGGL(k_dots_ms, dim3(kDot, nc, na), dim3(256), 0, st, n, V.p, vs, Wa, part2, ps, sysd.p, nc,
+                           (const int*)gate.p);
+        hipLaunchKernelGGL(k_project_out_ms, dim3(nb, na), dim3(256), 0, st, n, nc, V.p, vs, part2, ps, H.p, hs, 1, Wa,
+                           sysd.p, pn2, pns, (const int*)gate.p);
+        hipLaunchKernelGGL(k_arnoldi_end_ms, dim3(nb, na), dim3(256), 0, st, n, nc, Wa, (const int*)gate.p, pn1, pn2,
+                           pns, V.p, vs, H.p, hs, sysd.p);
         HIP_TRY(hipMemcpyAsync(hh.data(), H.p, hh.size() * sizeof(zd), hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
         it++;
