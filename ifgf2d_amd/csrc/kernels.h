@@ -603,9 +603,7 @@ __global__ void __launch_bounds__(256) k_level_push(PushArgsT<R> a) {
   __shared__ Z tmp[CMAX * P];
   const PushTask task = a.tasks[xcd_block()];
   const int tid = threadIdx.x;
-  if (!a.leaf)
-    for (int q = tid; q < (int)(sizeof(ChebTablesT<R>) / sizeof(R)); q += blockDim.x)
-      ((R*)&T)[q] = ((const R*)a.tabs)[q];
+  if (!a.leaf) load_transform_rows<Ps, Pang>(T, a.tabs);
   const int cl = tid / P, j = tid % P;
   const bool act = cl < task.nc;
   const long long g = (long long)task.c0 + cl;
@@ -767,25 +765,60 @@ __global__ void __launch_bounds__(256)
   const int i = gt / S, sl = gt % S;
   zd acc{0.0, 0.0}, near{0.0, 0.0}, far{0.0, 0.0};
   int si = 0;
+  // Each lane walks its entries in the order it, it + S, ... as one sum; the
+  // loops below load kFinU entries' indices, then their operands, before
+  // adding them in that order, so a lane waits for one dependent round trip
+  // per kFinU entries instead of one per entry.
+  constexpr int kFinU = 4;
   if (i < N) {
     si = inv[i];
     const long long e0 = corr_ptr ? corr_ptr[i] : 0, e1 = corr_ptr ? corr_ptr[i + 1] : 0;
     const long long nitem = (e1 - e0) * Np;
-    for (long long it = sl; it < nitem; it += S) {
-      const long long e = e0 + it / Np;
-      const int q = (int)(it % Np);
-      cfma(acc, corr_w[e * Np + q], x[(size_t)corr_patch[e] * Np + q]);
+    for (long long ib = sl; ib < nitem; ib += (long long)S * kFinU) {
+      zd wv[kFinU];
+      long long src[kFinU];
+#pragma unroll
+      for (int u = 0; u < kFinU; u++) {
+        const long long it = ib + (long long)u * S;
+        if (it < nitem) {
+          const long long e = e0 + it / Np;
+          const int q = (int)(it % Np);
+          wv[u] = corr_w[e * Np + q];
+          src[u] = (long long)corr_patch[e] * Np + q;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kFinU; u++)
+        if (ib + (long long)u * S < nitem) cfma(acc, wv[u], x[src[u]]);
     }
     if (npart) {
       const int g = si / kNearGroup, t = si % kNearGroup;
       const int w1 = nio[g + 1];
-      for (int w = nio[g] + sl; w < w1; w += S) near = cadd(near, npart[(size_t)w * kNearGroup + t]);
+      for (int wb = nio[g] + sl; wb < w1; wb += S * kFinU) {
+        zd pv[kFinU];
+#pragma unroll
+        for (int u = 0; u < kFinU; u++)
+          if (wb + u * S < w1) pv[u] = npart[(size_t)(wb + u * S) * kNearGroup + t];
+#pragma unroll
+        for (int u = 0; u < kFinU; u++)
+          if (wb + u * S < w1) near = cadd(near, pv[u]);
+      }
     } else if (sl == 0) {
       near = sol[si];
     }
     if (slots) {
       const long long f1 = it_ptr[si + 1];
-      for (long long f = it_ptr[si] + sl; f < f1; f += S) far = cadd(far, zwiden(slots[it_slot[f]]));
+      for (long long fb = it_ptr[si] + sl; fb < f1; fb += (long long)S * kFinU) {
+        int idx[kFinU];
+#pragma unroll
+        for (int u = 0; u < kFinU; u++) idx[u] = fb + (long long)u * S < f1 ? it_slot[fb + (long long)u * S] : 0;
+        ZI sv[kFinU];
+#pragma unroll
+        for (int u = 0; u < kFinU; u++) sv[u] = slots[idx[u]];
+#pragma unroll
+        for (int u = 0; u < kFinU; u++)
+          if (fb + (long long)u * S < f1) far = cadd(far, zwiden(sv[u]));
+      }
     }
   }
   acc = lane_group_sum<S>(acc);

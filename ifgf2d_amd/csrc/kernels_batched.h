@@ -143,9 +143,7 @@ __global__ void __launch_bounds__(256) k_level_push_b(PushArgsT<RT> a) {
   __shared__ Z tmp[CB * P * R];
   const PushTask task = a.tasks[xcd_block()];
   const int tid = threadIdx.x;
-  if (!a.leaf)
-    for (int q = tid; q < (int)(sizeof(ChebTablesT<RT>) / sizeof(RT)); q += blockDim.x)
-      ((RT*)&T)[q] = ((const RT*)a.tabs)[q];
+  if (!a.leaf) load_transform_rows<Ps, Pang>(T, a.tabs);
   const int cl = tid / P, j = tid % P;
   const bool act = cl < task.nc && cl < CB;
   const long long g = (long long)task.c0 + cl;

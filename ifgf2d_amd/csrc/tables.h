@@ -24,6 +24,23 @@ struct ChebTablesT {
   R Ta[16 * 16];       // Ta[n*16 + l] = T_n(xa[l]) * alpha_n / (Ps*Pang)
 };
 using ChebTables = ChebTablesT<double>;
+
+// Copies the transform rows a (Ps, Pang) kernel uses -- Ts[m][k], m, k < Ps,
+// and Ta[n][l], n, l < Pang -- from src to the block's LDS copy dst (all
+// threads of the block call it; a barrier must follow before dst is read).
+template <int Ps, int Pang, typename R>
+__device__ inline void load_transform_rows(ChebTablesT<R>& dst, const ChebTablesT<R>* __restrict__ src) {
+  for (int q = threadIdx.x; q < Ps * Ps + Pang * Pang; q += blockDim.x) {
+    if (q < Ps * Ps) {
+      const int o = (q / Ps) * 16 + q % Ps;
+      dst.Ts[o] = src->Ts[o];
+    } else {
+      const int r = q - Ps * Ps;
+      const int o = (r / Pang) * 16 + r % Pang;
+      dst.Ta[o] = src->Ta[o];
+    }
+  }
+}
 using ChebTablesF = ChebTablesT<float>;
 
 // One near-field work item: kNearItemChunks (or fewer) chunks of one target
