@@ -274,7 +274,7 @@ class ForwardMap:
                 "near_stored": s[10], "stored_bytes": s[11],
                 "lanes": {"near": s[12], "leaf": s[13], "push_task_entries": s[14], "finalize": s[15],
                           "near_on_second_stream": bool(s[16])},
-                "mixed": bool(s[17])}
+                "mixed": bool(s[17]), "ptab_entries": s[18], "ifac_entries": s[19]}
 
     def sorting(self):
         s = np.empty(self.N, dtype=np.int64)

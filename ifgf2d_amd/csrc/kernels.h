@@ -561,22 +561,39 @@ __global__ void __launch_bounds__(256)
 // target in fixed order by k_finalize). Every slot has exactly one writer,
 // so no atomics and a fixed summation order. The cone of an entry follows
 // from the cones' entry offsets coff (entries are grouped by cone).
+// The (s, theta, centre factor) of a propagation entry -- parent node j of a
+// parent cone with non-relevant id q, evaluated in the cone of its child at
+// position cp -- is the same for every parent box (the geometry relative to
+// the child's centre depends only on q, cp and j; BoxTree.h:299-428 keys its
+// protobox on the same triple), so it is stored once per (level, q, cp, j) in
+// ptab; an interpolation entry's is its own, in igeo. einfo tells them apart.
+// Likewise the inverse centre factor of parent node j depends only on the
+// parent cone's level, radial interval and s node (j % Ps): ifac_tab.
 template <typename R>
 struct PushArgsT {
   using Z = typename complex_of<R>::type;
   const PushTask* tasks;
-  const Geo4T<R>* geo;   // per push entry: (s, theta, centre factor)
+  const int* einfo;      // per push entry: >= 0: igeo index; < 0: ptab index -1 - einfo
+  const Geo4T<R>* igeo;  // interpolation entries' (s, theta, centre factor), in push order
+  const Geo4T<R>* ptab;  // propagation geometry per (level, q, cp) and node
   const int* coff;       // push entries of global cone g: [coff[g], coff[g+1])
   const int4* cidx;      // per (cone, node): slots of its children's pushed values (0: none)
   const int* lio;        // level D: items of leaf cone c: [lio[c], lio[c+1])
   const Z* lpart;        // level D: leaf item partial coefficients [item][P]
   long long leaf_cone_off;
-  const Z* pn_ifac;      // 1 / centre factor per (cone, node), levels < D
+  const int* ifac_off;   // per cone (levels < D): its (level, radial interval) row of ifac_tab
+  const Z* ifac_tab;     // 1 / centre factor per (level, radial interval, s node)
   const ChebTablesT<R>* tabs;
   Z* pbuf;               // slot 0 = 0, slot 1 + q = push entry q's value
   int leaf;              // 1 at level D
   int rs;                // batched kernels: columns per slot in pbuf (the widest block)
 };
+
+template <typename R>
+__device__ inline Geo4T<R> push_geo(const PushArgsT<R>& a, long long q) {
+  const int inf = a.einfo[q];
+  return inf >= 0 ? a.igeo[inf] : a.ptab[-1 - inf];
+}
 
 // Local cone (0 .. task.nc-1) of entry e of a task: entries are grouped by
 // cone in cone order.
@@ -611,7 +628,7 @@ __global__ void __launch_bounds__(256) k_level_push(PushArgsT<R> a) {
 #pragma unroll
   for (int u = 0; u < EPT; u++) {
     const int e = tid + u * 256;
-    if (e < task.ne) G[u] = a.geo[task.e0 + e];
+    if (e < task.ne) G[u] = push_geo(a, (long long)task.e0 + e);
   }
   Z v{0, 0};
   if (act) {
@@ -622,7 +639,7 @@ __global__ void __launch_bounds__(256) k_level_push(PushArgsT<R> a) {
     } else {
       const int4 ci = a.cidx[g * P + j];
       const Z s0 = a.pbuf[ci.x], s1 = a.pbuf[ci.y], s2 = a.pbuf[ci.z], s3 = a.pbuf[ci.w];
-      v = zmul(zadd(zadd(zadd(s0, s1), s2), s3), a.pn_ifac[g * P + j]);
+      v = zmul(zadd(zadd(zadd(s0, s1), s2), s3), a.ifac_tab[a.ifac_off[g] + j % Ps]);
     }
     buf[tid] = v;
   }
@@ -653,15 +670,14 @@ __global__ void __launch_bounds__(256) k_round_f32(long long n, const dbl2* __re
   out[i] = flt2{(float)v.x, (float)v.y};
 }
 
-// Build: push entry q takes its geometry from the propagation (src < npr)
-// or the interpolation (src - npr) geometry.
+// Build: out[i] = in[src[i]] (the push tables gathered from the per-entry
+// geometry that k_prop_build and k_interp_build computed).
+template <typename T>
 __global__ void __launch_bounds__(256)
-    k_push_gather(long long n, const long long* __restrict__ src, long long npr, const Geo4* __restrict__ pr_geo,
-                  const Geo4* __restrict__ it_geo, Geo4* __restrict__ out) {
-  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= n) return;
-  const long long s = src[q];
-  out[q] = s < npr ? pr_geo[s] : it_geo[s - npr];
+    k_take(long long n, const long long* __restrict__ src, const T* __restrict__ in, T* __restrict__ out) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[i] = in[src[i]];
 }
 
 // Propagation into the parent cones of level l-1 from the child cones of

@@ -149,7 +149,7 @@ __global__ void __launch_bounds__(256) k_level_push_b(PushArgsT<RT> a) {
   const long long g = (long long)task.c0 + cl;
   Geo4T<RT> G;
   const bool has = tid < task.ne;
-  if (has) G = a.geo[task.e0 + tid];
+  if (has) G = push_geo(a, (long long)task.e0 + tid);
   // cone inputs: column r of node j of cone cl at buf[(cl * P + j) * R + r]
   if (act) {
     if (a.leaf) {
@@ -168,7 +168,7 @@ __global__ void __launch_bounds__(256) k_level_push_b(PushArgsT<RT> a) {
       const Z* s1 = a.pbuf + (size_t)ci.y * rs;
       const Z* s2 = a.pbuf + (size_t)ci.z * rs;
       const Z* s3 = a.pbuf + (size_t)ci.w * rs;
-      const Z f = a.pn_ifac[g * P + j];
+      const Z f = a.ifac_tab[a.ifac_off[g] + j % Ps];
 #pragma unroll
       for (int r = 0; r < R; r++)
         buf[(cl * P + j) * R + r] = zmul(zadd(zadd(zadd(s0[r], s1[r]), s2[r]), s3[r]), f);

@@ -223,8 +223,8 @@ int ifgf_plan_stats(const ifgf_plan* p, long long out[20]) {
   out[15] = p->final_lanes;
   out[16] = p->overlap ? 1 : 0;
   out[17] = p->mixed ? 1 : 0;
-  out[18] = 0;
-  out[19] = 0;
+  out[18] = (long long)p->push_ptab.n;
+  out[19] = (long long)p->ifac_tab.n;
   return IFGF_OK;
 }
 
@@ -300,11 +300,13 @@ static void launch_level_push_ept(ifgf_plan* P, hipStream_t st) {
   const int D = P->tree.D;
   PushArgsT<R> a{};
   if constexpr (std::is_same<R, float>::value) {
-    a = PushArgsT<R>{nullptr,     P->push_geo32.p, P->push_coff.p, P->push_cidx.p, P->leaf_item_off.p, P->lpart32.p,
-                     P->tree.lv[D].cone_off, P->pn_ifac32.p, P->tabs32.p, P->pbuf32.p, 0};
+    a = PushArgsT<R>{nullptr, P->push_einfo.p, P->push_igeo32.p, P->push_ptab32.p, P->push_coff.p, P->push_cidx.p,
+                     P->leaf_item_off.p, P->lpart32.p, P->tree.lv[D].cone_off, P->ifac_off.p, P->ifac_tab32.p,
+                     P->tabs32.p, P->pbuf32.p, 0};
   } else {
-    a = PushArgsT<R>{nullptr,   P->push_geo.p, P->push_coff.p, P->push_cidx.p, P->leaf_item_off.p, P->lpart.p,
-                     P->tree.lv[D].cone_off, P->pn_ifac.p, P->tabs.p, P->pbuf.p, 0};
+    a = PushArgsT<R>{nullptr, P->push_einfo.p, P->push_igeo.p, P->push_ptab.p, P->push_coff.p, P->push_cidx.p,
+                     P->leaf_item_off.p, P->lpart.p, P->tree.lv[D].cone_off, P->ifac_off.p, P->ifac_tab.p,
+                     P->tabs.p, P->pbuf.p, 0};
   }
   for (int li = 0; li + 1 < D; li++) {
     const int t0 = P->push_level_off[li], nt = P->push_level_off[li + 1] - t0;
@@ -459,11 +461,13 @@ static void launch_batched(ifgf_plan* P, const zd* x, long long ld, zd* y, hipSt
   Z* pbuf = f32 ? (Z*)P->pbuf_b32.p : (Z*)P->pbuf_b.p;
   PushArgsT<RT> a{};
   if constexpr (f32) {
-    a = PushArgsT<RT>{nullptr, P->push_geo32.p, P->push_coff.p, P->push_cidx.p, P->leaf_item_off.p, lpart,
-                      t.lv[D].cone_off, P->pn_ifac32.p, P->tabs32.p, pbuf, 0, P->rb_max};
+    a = PushArgsT<RT>{nullptr, P->push_einfo.p, P->push_igeo32.p, P->push_ptab32.p, P->push_coff.p, P->push_cidx.p,
+                      P->leaf_item_off.p, lpart, t.lv[D].cone_off, P->ifac_off.p, P->ifac_tab32.p, P->tabs32.p, pbuf,
+                      0, P->rb_max};
   } else {
-    a = PushArgsT<RT>{nullptr, P->push_geo.p, P->push_coff.p, P->push_cidx.p, P->leaf_item_off.p, lpart,
-                      t.lv[D].cone_off, P->pn_ifac.p, P->tabs.p, pbuf, 0, P->rb_max};
+    a = PushArgsT<RT>{nullptr, P->push_einfo.p, P->push_igeo.p, P->push_ptab.p, P->push_coff.p, P->push_cidx.p,
+                      P->leaf_item_off.p, lpart, t.lv[D].cone_off, P->ifac_off.p, P->ifac_tab.p, P->tabs.p, pbuf, 0,
+                      P->rb_max};
   }
   const std::vector<int>& lo = P->blevel_off[LR];
   for (int li = 0; li + 1 < D; li++) {
@@ -727,7 +731,8 @@ static void build_stored(ifgf_plan* P) {
   d_pr_ptr.upload(pr_ptr);
   d_pr_cone.alloc(npr);
   pr_geo.alloc(npr);
-  P->pn_ifac.alloc((size_t)t.n_cones * Pn);
+  DevBuf<zd> pn_ifac;
+  pn_ifac.alloc((size_t)t.n_cones * Pn);
   for (int l = D; l >= 3; l--) {
     const LevelInfo& L = t.lv[l];
     const LevelInfo& Pl = t.lv[l - 1];
@@ -735,7 +740,7 @@ static void build_stored(ifgf_plan* P) {
     hipLaunchKernelGGL((k_prop_build<Ps, Pang>), dim3((Pl.ncone * Pn + 255) / 256), dim3(256), 0, st, Pl.ncone,
                        Pl.cone_off, Pl.box_off, L.cone_off, L.nslot, P->cg.p, P->cone_box.p, P->cone_qd.p,
                        P->box_child.p, P->proto.p + L.proto_off, P->dense_map.p + L.map_off, P->tabs.p,
-                       d_pr_ptr.p, d_pr_cone.p, pr_geo.p, P->pn_ifac.p, k);
+                       d_pr_ptr.p, d_pr_cone.p, pr_geo.p, pn_ifac.p, k);
   }
   HIP_TRY(hipGetLastError());
   std::vector<int> pr_cone(npr);
@@ -753,24 +758,72 @@ static void build_stored(ifgf_plan* P) {
   for (long long e = 0; e < npr; e++) off[pr_cone[e] + 1]++;
   for (long long e = 0; e < nit; e++) off[t.it_cone[e] + 1]++;
   for (long long g = 0; g < t.n_cones; g++) off[g + 1] += off[g];
-  std::vector<long long> pos(off.begin(), off.end() - 1), src(npush);
+  std::vector<long long> pos(off.begin(), off.end() - 1);
+  std::vector<int> einfo(npush);
   std::vector<int4> cidx((size_t)t.n_cones * Pn, make_int4(0, 0, 0, 0));
+  // propagation geometry shared per (level, parent non-relevant id q, child
+  // position cp): pair ids in order of first use, each gathered from its
+  // first parent cone's entries
+  // (pairs of parent level l: [pair_base[l], pair_base[l + 1]), key q * 4 + cp)
+  std::vector<long long> pair_base(D + 1, 0);
+  for (int l = 2; l < D; l++) pair_base[l + 1] = pair_base[l] + (long long)t.lv[l].nslot * 4;
+  std::vector<int> pair_id(pair_base[D], -1);
+  std::vector<long long> ptab_src;
   for (long long g = 0; g < t.n_cones; g++) {
     const int nch = (int)((pr_ptr[g + 1] - pr_ptr[g]) / Pn);
-    for (int kc = 0; kc < nch; kc++)
+    const int l = t.cone_level[g];
+    int kc = 0;
+    for (int cp = 0; cp < 4 && kc < nch; cp++) {
+      if (t.box_child[(size_t)(t.lv[l].box_off + t.cone_box[g]) * 4 + cp] < 0) continue;
+      int& id = pair_id[pair_base[l] + (long long)P->cone_q[g] * 4 + cp];
+      if (id < 0) {
+        id = (int)(ptab_src.size() / Pn);
+        for (int j = 0; j < Pn; j++) ptab_src.push_back(pr_ptr[g] + (long long)kc * Pn + j);
+      }
       for (int j = 0; j < Pn; j++) {
         const long long e = pr_ptr[g] + (long long)kc * Pn + j;
         const long long q = pos[pr_cone[e]]++;
-        src[q] = e;
+        einfo[q] = -1 - (id * Pn + j);
         int* c = &cidx[(size_t)g * Pn + j].x;
         c[kc] = (int)(1 + q);
       }
+      kc++;
+    }
   }
   std::vector<int> it_slot(nit);
+  std::vector<long long> it_of_slot(npush, -1);
   for (long long e = 0; e < nit; e++) {
     const long long q = pos[t.it_cone[e]]++;
-    src[q] = npr + e;
+    it_of_slot[q] = e;
     it_slot[e] = (int)(1 + q);
+  }
+  // interpolation entries' geometry in push order
+  std::vector<long long> igeo_src;
+  igeo_src.reserve(nit);
+  for (long long q = 0; q < npush; q++)
+    if (it_of_slot[q] >= 0) {
+      einfo[q] = (int)igeo_src.size();
+      igeo_src.push_back(it_of_slot[q]);
+    }
+  std::vector<long long>().swap(it_of_slot);
+  // inverse node factors per (level, radial interval, s node), gathered from
+  // the first parent cone of each (level, radial interval)
+  std::vector<int> ifac_off(t.n_cones, 0);
+  std::vector<long long> ifac_src;
+  {
+    std::vector<long long> rbase(D + 1, 0);
+    for (int l = 2; l < D; l++) rbase[l + 1] = rbase[l] + t.lv[l].nrad;
+    std::vector<int> row(rbase[D], -1);
+    for (long long g = 0; g < t.n_cones; g++) {
+      const int l = t.cone_level[g];
+      if (l >= D) continue;
+      int& r = row[rbase[l] + t.cone_posr[g]];
+      if (r < 0) {
+        r = (int)ifac_src.size();
+        for (int js = 0; js < Ps; js++) ifac_src.push_back(g * Pn + js);
+      }
+      ifac_off[g] = r;
+    }
   }
   P->it_slot.upload(it_slot);
   P->push_cidx.upload(cidx);
@@ -779,23 +832,32 @@ static void build_stored(ifgf_plan* P) {
     std::vector<int> coff(off.begin(), off.end());
     P->push_coff.upload(coff);
   }
-  P->push_geo.alloc(npr + nit);
+  P->push_einfo.upload(einfo);
+  P->ifac_off.upload(ifac_off);
   {
-    DevBuf<long long> d_src;
-    d_src.upload(src);
-    if (npr + nit > 0)
-    hipLaunchKernelGGL(k_push_gather, dim3((unsigned)((npr + nit + 255) / 256)), dim3(256), 0, st, npr + nit, d_src.p,
-                       npr, pr_geo.p, it_geo.p, P->push_geo.p);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipStreamSynchronize(st));
+    auto take = [&](auto* in, auto& out, const std::vector<long long>& idx) {
+      out.alloc(idx.size());
+      if (idx.empty()) return;
+      DevBuf<long long> d_idx;
+      d_idx.upload(idx);
+      using T = std::remove_pointer_t<decltype(in)>;
+      hipLaunchKernelGGL(k_take<T>, dim3((unsigned)((idx.size() + 255) / 256)), dim3(256), 0, st,
+                         (long long)idx.size(), d_idx.p, (const T*)in, out.p);
+      HIP_TRY(hipGetLastError());
+      HIP_TRY(hipStreamSynchronize(st));
+    };
+    take(it_geo.p, P->push_igeo, igeo_src);
+    take(pr_geo.p, P->push_ptab, ptab_src);
+    take(pn_ifac.p, P->ifac_tab, ifac_src);
   }
   P->nslots = 1 + npush;
   P->pbuf.alloc(P->nslots);
   HIP_TRY(hipMemset(P->pbuf.p, 0, P->nslots * sizeof(zd)));
   build_push_tasks(P, P->push_emax);
-  P->stored_bytes = (long long)(nf_goff[ngroups] + lf_off[LD.ncone] + (long long)t.n_cones * Pn + P->nslots) *
+  P->stored_bytes = (long long)(nf_goff[ngroups] + lf_off[LD.ncone] + P->ifac_tab.n + P->nslots) *
                         (long long)sizeof(zd) +
-                    npush * (long long)sizeof(Geo4) + nit * (long long)sizeof(int) +
+                    (long long)(P->push_igeo.n + P->push_ptab.n) * (long long)sizeof(Geo4) +
+                    (npush + nit + t.n_cones) * (long long)sizeof(int) +
                     (long long)t.n_cones * Pn * (long long)sizeof(int4);
   if (P->prm.precision == IFGF_MIXED) {
     // FP32 copies: leaf operator, propagation / interpolation geometry
@@ -807,10 +869,12 @@ static void build_stored(ifgf_plan* P) {
     };
     P->lf_val32.alloc(P->lf_val.n);
     round(P->lf_val.p, P->lf_val32.p, (long long)P->lf_val.n);
-    P->push_geo32.alloc(P->push_geo.n);
-    round(P->push_geo.p, P->push_geo32.p, 2 * (long long)P->push_geo.n);
-    P->pn_ifac32.alloc(P->pn_ifac.n);
-    round(P->pn_ifac.p, P->pn_ifac32.p, (long long)P->pn_ifac.n);
+    P->push_igeo32.alloc(P->push_igeo.n);
+    round(P->push_igeo.p, P->push_igeo32.p, 2 * (long long)P->push_igeo.n);
+    P->push_ptab32.alloc(P->push_ptab.n);
+    round(P->push_ptab.p, P->push_ptab32.p, 2 * (long long)P->push_ptab.n);
+    P->ifac_tab32.alloc(P->ifac_tab.n);
+    round(P->ifac_tab.p, P->ifac_tab32.p, (long long)P->ifac_tab.n);
     P->pbuf32.alloc(P->pbuf.n);
     HIP_TRY(hipMemsetAsync(P->pbuf32.p, 0, P->pbuf32.n * sizeof(zf), st));
     P->lpart32.alloc(P->lpart.n);
@@ -826,8 +890,8 @@ static void build_stored(ifgf_plan* P) {
     P->tabs32.upload(std::vector<ChebTablesF>{tf});
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(st));
-    P->stored_bytes += (long long)(P->lf_val32.n + P->pn_ifac32.n + P->pbuf32.n) * (long long)sizeof(zf) +
-                       (long long)P->push_geo32.n * (long long)sizeof(Geo4F);
+    P->stored_bytes += (long long)(P->lf_val32.n + P->ifac_tab32.n + P->pbuf32.n) * (long long)sizeof(zf) +
+                       (long long)(P->push_igeo32.n + P->push_ptab32.n) * (long long)sizeof(Geo4F);
     P->mixed = true;
   }
   build_batched<Ps, Pang>(P);

@@ -95,10 +95,14 @@ struct ifgf_plan {
   bool stored = false;
   ifgf::DevBuf<long long> nf_goff, lf_off;
   long long near_entries = 0;  // stored near-field values without the group padding
-  ifgf::DevBuf<zd> nf_val, lf_val, pn_ifac;
+  ifgf::DevBuf<zd> nf_val, lf_val;
   // level-push propagation / interpolation
   ifgf::DevBuf<ifgf::PushTask> push_tasks;
-  ifgf::DevBuf<ifgf::Geo4> push_geo;
+  ifgf::DevBuf<int> push_einfo;      // per push entry: igeo index, or -1 - ptab index
+  ifgf::DevBuf<ifgf::Geo4> push_igeo; // interpolation entries' geometry (push order)
+  ifgf::DevBuf<ifgf::Geo4> push_ptab; // propagation geometry per (level, q, cp) and node
+  ifgf::DevBuf<int> ifac_off;         // per cone: row of ifac_tab (levels < D)
+  ifgf::DevBuf<zd> ifac_tab;          // 1 / centre factor per (level, radial interval, s node)
   ifgf::DevBuf<int> push_coff;     // push entries of global cone g: [coff[g], coff[g+1])
   ifgf::DevBuf<int4> push_cidx;    // per (cone, node): pbuf slots of its <= 4 children (0: none)
   ifgf::DevBuf<zd> pbuf;           // slot 0 = 0, then slot 1 + q for push entry q
@@ -126,8 +130,8 @@ struct ifgf_plan {
   // propagation / interpolation geometry and inverse node factors, and FP32
   // coefficient workspaces; the FP64 operator stays available (IFGF_APPLY_FP64)
   bool mixed = false;
-  ifgf::DevBuf<zf> lf_val32, pn_ifac32, pbuf32, lpart32;
-  ifgf::DevBuf<ifgf::Geo4F> push_geo32;
+  ifgf::DevBuf<zf> lf_val32, ifac_tab32, pbuf32, lpart32;
+  ifgf::DevBuf<ifgf::Geo4F> push_igeo32, push_ptab32;
   ifgf::DevBuf<ifgf::ChebTablesF> tabs32;
   int nleaf = 0;
   // lanes per work item of the stored-path kernels

@@ -128,7 +128,8 @@ int ifgf_plan_points(const ifgf_plan* plan, double* x, double* y, double* nx, do
  * evaluations per block of the per-level propagation/interpolation kernel,
  * [15] lanes per target of the correction kernel, [16] 1 if the near field runs on a
  * second stream beside the leaf / level-push chain, [17] 1 on a mixed-precision plan,
- * [18..19] 0. */
+ * [18] stored propagation-geometry records (one per level, parent cone
+ * direction, child position and node), [19] stored inverse node factors. */
 int ifgf_plan_stats(const ifgf_plan* plan, long long out[20]);
 
 /* Replaces BoxTree::SortBox (BoxTree.h:73-106) with ComputeBB's bounding box
