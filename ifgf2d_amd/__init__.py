@@ -42,6 +42,7 @@ IFGF_APPLY_FP64 = 2
 IFGF_PLAN_DEVICE_SORT = 8
 IFGF_PLAN_OVERLAP = 16
 IFGF_PLAN_DEVICE_PRECOMP = 32
+IFGF_PLAN_DEVICE_GEOMETRY = 64
 
 
 class IFGFError(RuntimeError):
@@ -73,7 +74,7 @@ EXPORTS = [
     "ifgf_fgmres", "ifgf_gmres_host", "ifgf_hankel01", "ifgf_plan_profile", "ifgf_plan_stage_times",
     "ifgf_morton_sort", "ifgf_apply_graph", "ifgf_group_create", "ifgf_group_launch",
     "ifgf_group_synchronize", "ifgf_group_destroy", "ifgf_gmres_multi", "ifgf_field_create",
-    "ifgf_field_apply", "ifgf_field_stats", "ifgf_field_destroy",
+    "ifgf_field_apply", "ifgf_field_stats", "ifgf_field_destroy", "ifgf_plan_setup_times",
 ]
 
 _LIB = None
@@ -127,6 +128,7 @@ def lib():
         L.ifgf_morton_sort.argtypes = [_dp, _dp, ctypes.c_longlong, ctypes.c_int, _lp, _lp]
         L.ifgf_plan_profile.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.ifgf_plan_stage_times.argtypes = [ctypes.c_void_p, _dp, _lp]
+        L.ifgf_plan_setup_times.argtypes = [ctypes.c_void_p, _dp]
         L.ifgf_field_create.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_longlong, ctypes.c_int, ctypes.c_int,
                                         ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
         L.ifgf_field_apply.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
@@ -208,11 +210,14 @@ class ForwardMap:
     level pushes (the autotuner also tries it).
     device_precomp: build the rectangular-polar singular / near-singular correction on the
     GPU instead of the host.
+    device_geometry: run the near-singular search of the discretisation on the GPU.
+    device_setup: all three device setup steps (sort, correction, near-singular search).
     """
 
     def __init__(self, curve, patch_lims, k, nlevels, Np=10, formulation="combined", eta=None,
                  delta=None, p=4.0, Ps=5, Pang=5, precision="fp64", nrhs_max=1, host_only=False,
-                 on_the_fly=False, autotune=False, device_sort=False, overlap=False, device_precomp=False):
+                 on_the_fly=False, autotune=False, device_sort=False, overlap=False, device_precomp=False,
+                 device_geometry=False, device_setup=False):
         L = lib()
         self.curve = curve
         self.lims = np.ascontiguousarray(patch_lims, dtype=np.float64)
@@ -225,7 +230,10 @@ class ForwardMap:
                         Np, self.delta, float(p))
         flags = ((IFGF_PLAN_HOST_ONLY if host_only else 0) | (IFGF_PLAN_ON_THE_FLY if on_the_fly else 0)
                  | (IFGF_PLAN_AUTOTUNE if autotune else 0) | (IFGF_PLAN_DEVICE_SORT if device_sort else 0)
-                 | (IFGF_PLAN_OVERLAP if overlap else 0) | (IFGF_PLAN_DEVICE_PRECOMP if device_precomp else 0))
+                 | (IFGF_PLAN_OVERLAP if overlap else 0) | (IFGF_PLAN_DEVICE_PRECOMP if device_precomp else 0)
+                 | (IFGF_PLAN_DEVICE_GEOMETRY if device_geometry else 0))
+        if device_setup:
+            flags |= IFGF_PLAN_DEVICE_SORT | IFGF_PLAN_DEVICE_PRECOMP | IFGF_PLAN_DEVICE_GEOMETRY
         prm = _Params(self.formulation, _PREC[precision], self.k, self.eta, Ps, Pang, nlevels,
                       nrhs_max, flags)
         h = ctypes.c_void_p()
@@ -407,6 +415,13 @@ class ForwardMap:
         self._check(lib().ifgf_plan_stage_times(self._h, _d(t), ctypes.byref(c)), "stage_times")
         return dict(zip(("gather", "near", "leaf", "propagation", "interpolation", "finalize"),
                         t.tolist())), c.value
+
+    def setup_times(self):
+        """Seconds spent in each plan-creation step."""
+        t = np.zeros(8)
+        self._check(lib().ifgf_plan_setup_times(self._h, _d(t)), "setup_times")
+        return dict(zip(("geometry", "correction", "tree", "exclusions", "device_tables", "stored_operators",
+                         "autotune", "total"), t.tolist()))
 
     def linear_operator(self):
         """The LinearOperator that ForwardMap::solve builds (ForwardMap.tpp:673-682)."""

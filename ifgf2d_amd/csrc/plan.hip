@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <complex>
 #include <cstring>
@@ -149,11 +150,22 @@ int ifgf_plan_create(const ifgf_surface* s, const ifgf_params* prm, ifgf_plan** 
   P->prm = *prm;
   if (P->prm.nrhs_max < 1) P->prm.nrhs_max = 1;
   std::string err;
+  const auto t_start = std::chrono::steady_clock::now();
+  auto t_last = t_start;
+  auto lap = [&](int step) {
+    const auto now = std::chrono::steady_clock::now();
+    P->setup_s[step] += std::chrono::duration<double>(now - t_last).count();
+    P->setup_s[7] = std::chrono::duration<double>(now - t_start).count();
+    t_last = now;
+  };
   try {
     if (!make_curve(s->curve_type, s->curve_params, s->ncurve_params, P->curve, err))
       return fail(nullptr, IFGF_ERR_ARG, err);
-    if (!build_geometry(P->curve, s->patch_lims, s->npatches, s->Np, s->delta, P->geo, err))
+    const bool dev_geo = (prm->flags & IFGF_PLAN_DEVICE_GEOMETRY) && !(prm->flags & IFGF_PLAN_HOST_ONLY);
+    if (!build_geometry(P->curve, s->patch_lims, s->npatches, s->Np, s->delta, P->geo, err,
+                        dev_geo ? near_singular_device : nullptr))
       return fail(nullptr, IFGF_ERR_GEOMETRY, err);
+    lap(0);
     if ((prm->flags & IFGF_PLAN_DEVICE_PRECOMP) && !(prm->flags & IFGF_PLAN_HOST_ONLY)) {
       if (!build_correction_device(P->geo, P->curve, prm->formulation, prm->k, prm->eta, s->rp_p, P->corr, err,
                                    nullptr))
@@ -161,6 +173,7 @@ int ifgf_plan_create(const ifgf_surface* s, const ifgf_params* prm, ifgf_plan** 
     } else if (!build_correction(P->geo, P->curve, prm->formulation, prm->k, prm->eta, s->rp_p, P->corr, err)) {
       return fail(nullptr, IFGF_ERR_GEOMETRY, err);
     }
+    lap(1);
     std::vector<int> dperm;
     if ((prm->flags & IFGF_PLAN_DEVICE_SORT) && !(prm->flags & IFGF_PLAN_HOST_ONLY)) {
       if (prm->nlevels > 16) return fail(nullptr, IFGF_ERR_ARG, "nlevels above 16 (leaf codes are 32-bit)");
@@ -170,15 +183,20 @@ int ifgf_plan_create(const ifgf_surface* s, const ifgf_params* prm, ifgf_plan** 
     if (!build_tree(P->geo.x.data(), P->geo.y.data(), P->geo.nx.data(), P->geo.ny.data(), P->geo.N,
                     prm->nlevels, prm->k, prm->Ps, prm->Pang, P->tree, err, dperm.empty() ? nullptr : &dperm))
       return fail(nullptr, IFGF_ERR_GEOMETRY, err);
+    lap(2);
     if (!build_exclusions(P->geo, P->tree, P->patch_sorted, P->excl, P->n_near, err))
       return fail(nullptr, IFGF_ERR_GEOMETRY, err);
     build_cone_tables(P.get());
+    lap(3);
     if (!(prm->flags & IFGF_PLAN_HOST_ONLY)) {
       build_device_side(P.get());
+      lap(4);
       if (!(prm->flags & IFGF_PLAN_ON_THE_FLY)) {
         if (!build_stored_any(P.get())) return fail(nullptr, IFGF_ERR_ARG, "no kernel instantiation for this (Ps, Pang)");
+        lap(5);
         P->overlap = (prm->flags & IFGF_PLAN_OVERLAP) != 0;
         if (prm->flags & IFGF_PLAN_AUTOTUNE) autotune(P.get());
+        lap(6);
       }
     }
   } catch (std::exception& e) {
@@ -1487,6 +1505,12 @@ int ifgf_plan_profile(ifgf_plan* P, int enable) {
   P->ev_used = 0;
   for (double& v : P->stage_ms) v = 0.0;
   P->prof_count = 0;
+  return IFGF_OK;
+}
+
+int ifgf_plan_setup_times(const ifgf_plan* P, double seconds[8]) {
+  if (!P || !seconds) return fail(nullptr, IFGF_ERR_ARG, "null argument");
+  for (int q = 0; q < 8; q++) seconds[q] = P->setup_s[q];
   return IFGF_OK;
 }
 

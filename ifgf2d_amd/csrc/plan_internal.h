@@ -146,6 +146,7 @@ struct ifgf_plan {
   std::vector<hipEvent_t> ev_pool;
   size_t ev_used = 0;
   double stage_ms[6] = {0, 0, 0, 0, 0, 0};
+  double setup_s[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // plan creation steps (ifgf_plan_setup_times)
   long long prof_count = 0;
   ~ifgf_plan() {
     for (auto& e : ev_pool)
@@ -173,6 +174,9 @@ int plan_fail(ifgf_plan* p, int code, const std::string& msg);
 // exceeds the device kernel's limit.
 bool build_correction_device(const Geometry& g, const Curve& c, int F, double k, double eta, double p,
                              Correction& out, std::string& err, hipStream_t st);
+// The near-singular search on the GPU (precomp.hip), a build_geometry
+// NsSearch: the same candidates, test and order as the host loop.
+bool near_singular_device(const Curve& c, double delta, Geometry& g, std::string& err);
 // Leaf Morton codes and their stable sort on the GPU (morton.hip); false when
 // a point lies outside the leaf grid.
 bool device_morton_sort(const double* x, const double* y, long long n, int nlevels, std::vector<int>& perm,

@@ -1,4 +1,6 @@
-// The rectangular-polar singular / near-singular correction built on the
+// The near-singular search (determine_patch_near_singular_points,
+// ForwardMap.tpp:224-285, one thread per candidate point) and the
+// rectangular-polar singular / near-singular correction built on the
 // GPU (the precomputation of ForwardMap::compute_precomputations,
 // ForwardMap.tpp:391-423, per column single_patch_point_compute_precomputations,
 // ForwardMap.tpp:287-389): one thread per column evaluates the 40-node RP
@@ -36,7 +38,56 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// One thread per (patch p, adjacent patch e, node j) candidate, in the host
+// loop's order: ts_out = the closest parameter, or NaN when the point is not
+// near-singular to the patch.
+__global__ void __launch_bounds__(256)
+    k_near_singular(int npatches, int Np, const double* __restrict__ t1, const double* __restrict__ t2,
+                    const double* __restrict__ x, const double* __restrict__ y, Curve c, double delta,
+                    double* __restrict__ ts_out) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long long)npatches * 2 * Np) return;
+  const int j = (int)(i % Np);
+  const int pe = (int)(i / Np);
+  const int p = pe / 2, e = pe % 2;
+  const long long q = (long long)near_singular_candidate_patch(p, e, npatches) * Np + j;
+  double ts = 0.0;
+  ts_out[i] = near_singular_param(c, t1[p], t2[p], x[q], y[q], delta, ts) ? ts : NAN;
+}
+
 }  // namespace
+
+bool ifgf::near_singular_device(const Curve& c, double delta, Geometry& g, std::string& err) {
+  try {
+    const int P = g.npatches, Np = g.Np;
+    const long long n = (long long)P * 2 * Np;
+    DevBuf<double> dt1, dt2, dx, dy, dts;
+    dt1.upload(g.t1);
+    dt2.upload(g.t2);
+    dx.upload(g.x);
+    dy.upload(g.y);
+    dts.alloc(n);
+    hipLaunchKernelGGL(k_near_singular, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, P, Np, dt1.p, dt2.p,
+                       dx.p, dy.p, c, delta, dts.p);
+    HIP_TRY(hipGetLastError());
+    std::vector<double> ts(n);
+    HIP_TRY(hipMemcpy(ts.data(), dts.p, n * sizeof(double), hipMemcpyDeviceToHost));
+    for (int p = 0; p < P; p++)
+      for (int e = 0; e < 2; e++) {
+        const long long base = (long long)near_singular_candidate_patch(p, e, P) * Np;
+        for (int j = 0; j < Np; j++) {
+          const double t = ts[((long long)p * 2 + e) * Np + j];
+          if (t != t) continue;  // NaN: not near-singular
+          g.ns_idx[p].push_back(base + j);
+          g.ns_t[p].push_back(t);
+        }
+      }
+  } catch (std::exception& e) {
+    err = e.what();
+    return false;
+  }
+  return true;
+}
 
 bool ifgf::build_correction_device(const Geometry& g, const Curve& c, int F, double k, double eta, double p,
                                    Correction& out, std::string& err, hipStream_t st) {

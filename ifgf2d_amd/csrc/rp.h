@@ -174,4 +174,47 @@ IFGF_HD inline void rp_column(const Curve& c, int F, double k, double eta, doubl
   }
 }
 
+// Near-singular test of the point (xq, yq) against the patch [a, b]
+// (determine_patch_near_singular_points, ForwardMap.tpp:224-285): true when
+// the point lies within delta of the patch, with ts the closest parameter --
+// a if |x(a) - x_q| < delta, else b if that holds, else the interior
+// minimiser of the distance by golden-section search to 1e-12 (the
+// reference's Boost Brent, BrentsMethod.hpp:42-156, finds the same minimum).
+// Host setup and the device search share this code.
+IFGF_HD inline bool near_singular_param(const Curve& c, double a, double b, double xq, double yq, double delta,
+                                        double& ts) {
+  auto dist = [&](double t) {
+    const double dx = c.x(t) - xq, dy = c.y(t) - yq;
+    return sqrt(fabs(dx * dx + dy * dy));
+  };
+  if (dist(a) < delta) {
+    ts = a;
+    return true;
+  }
+  if (dist(b) < delta) {
+    ts = b;
+    return true;
+  }
+  const double gr = 0.5 * (sqrt(5.0) - 1.0);
+  double lo = a, hi = b, c1 = hi - gr * (hi - lo), c2 = lo + gr * (hi - lo);
+  double f1 = dist(c1), f2 = dist(c2);
+  for (int it = 0; it < 200 && (hi - lo) > 1e-12 * (1.0 + fabs(lo)); it++) {
+    if (f1 < f2) {
+      hi = c2;
+      c2 = c1;
+      f2 = f1;
+      c1 = hi - gr * (hi - lo);
+      f1 = dist(c1);
+    } else {
+      lo = c1;
+      c1 = c2;
+      f1 = f2;
+      c2 = lo + gr * (hi - lo);
+      f2 = dist(c2);
+    }
+  }
+  ts = 0.5 * (lo + hi);
+  return dist(ts) < delta;
+}
+
 }  // namespace ifgf

@@ -71,7 +71,7 @@ static void fejer(int n, std::vector<double>& x, std::vector<double>& w) {
 
 // -------------------------------------------------------------- geometry ---
 bool build_geometry(const Curve& c, const double* lims, int npatches, int Np, double delta,
-                    Geometry& g, std::string& err) {
+                    Geometry& g, std::string& err, NsSearch ns_search) {
   if (npatches < 3 || Np < 2 || Np > 64) {
     err = "need at least 3 patches and 2 <= Np <= 64";
     return false;
@@ -114,45 +114,15 @@ bool build_geometry(const Curve& c, const double* lims, int npatches, int Np, do
   // else the interior minimiser of the distance (golden-section search).
   g.ns_idx.assign(npatches, {});
   g.ns_t.assign(npatches, {});
+  if (ns_search) return ns_search(c, delta, g, err);
 #pragma omp parallel for schedule(static)
   for (int p = 0; p < npatches; p++) {
-    const double a = g.t1[p], b = g.t2[p];
-    const int est[2] = {(p - 1 + npatches) % npatches, (p + 1) % npatches};
     for (int e = 0; e < 2; e++) {
+      const int pe = near_singular_candidate_patch(p, e, npatches);
       for (int j = 0; j < Np; j++) {
-        const long long q = (long long)est[e] * Np + j;
-        const double xq = g.x[q], yq = g.y[q];
-        auto dist = [&](double t) {
-          const double dx = c.x(t) - xq, dy = c.y(t) - yq;
-          return std::sqrt(std::abs(dx * dx + dy * dy));
-        };
+        const long long q = (long long)pe * Np + j;
         double ts;
-        if (dist(a) < delta) {
-          ts = a;
-        } else if (dist(b) < delta) {
-          ts = b;
-        } else {
-          const double gr = 0.5 * (std::sqrt(5.0) - 1.0);
-          double lo = a, hi = b, c1 = hi - gr * (hi - lo), c2 = lo + gr * (hi - lo);
-          double f1 = dist(c1), f2 = dist(c2);
-          for (int it = 0; it < 200 && (hi - lo) > 1e-12 * (1.0 + std::abs(lo)); it++) {
-            if (f1 < f2) {
-              hi = c2;
-              c2 = c1;
-              f2 = f1;
-              c1 = hi - gr * (hi - lo);
-              f1 = dist(c1);
-            } else {
-              lo = c1;
-              c1 = c2;
-              f1 = f2;
-              c2 = lo + gr * (hi - lo);
-              f2 = dist(c2);
-            }
-          }
-          ts = 0.5 * (lo + hi);
-          if (!(dist(ts) < delta)) continue;
-        }
+        if (!near_singular_param(c, g.t1[p], g.t2[p], g.x[q], g.y[q], delta, ts)) continue;
         g.ns_idx[p].push_back(q);
         g.ns_t[p].push_back(ts);
       }

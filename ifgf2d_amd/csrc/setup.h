@@ -29,8 +29,18 @@ struct Geometry {
   std::vector<std::vector<long long>> ns_idx;
   std::vector<std::vector<double>> ns_t;
 };
+// Near-singular points: the points of the two adjacent patches (e = 0: the
+// previous, 1: the next, cyclically -- the neighbour estimate of
+// ForwardMap.tpp:164-218 for one closed curve) closer than delta to the patch,
+// with their closest parameter (near_singular_param). ns_search (optional)
+// fills g.ns_idx / g.ns_t in that order instead of the host loop (the device
+// search).
+IFGF_HD inline int near_singular_candidate_patch(int p, int e, int npatches) {
+  return e == 0 ? (p - 1 + npatches) % npatches : (p + 1) % npatches;
+}
+using NsSearch = bool (*)(const Curve& c, double delta, Geometry& g, std::string& err);
 bool build_geometry(const Curve& c, const double* lims, int npatches, int Np, double delta,
-                    Geometry& g, std::string& err);
+                    Geometry& g, std::string& err, NsSearch ns_search = nullptr);
 
 // Folded correction W_p = M_p^T T_Np in target-major CSR: for point i (patch
 // order), entries e in [ptr[i], ptr[i+1]) add sum_k w[e*Np+k] * rho[patch[e]*Np+k].

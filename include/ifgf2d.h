@@ -63,6 +63,9 @@ extern "C" {
 #define IFGF_PLAN_DEVICE_PRECOMP 32 /* evaluate the rectangular-polar singular / near-singular
                                        correction (ForwardMap.tpp:287-423) on the GPU, one thread
                                        per column, instead of on the host (Np <= 16) */
+#define IFGF_PLAN_DEVICE_GEOMETRY 64 /* the near-singular search of the discretisation
+                                        (ForwardMap.tpp:224-285) on the GPU, one thread per
+                                        candidate point, instead of on the host */
 
 /* apply flags */
 #define IFGF_PTR_DEVICE 1     /* x / y are device pointers; the call is asynchronous on stream */
@@ -304,6 +307,14 @@ int ifgf_hankel01(const double* x, long long n, double* h0, double* h1);
  * throughput should run with profiling off. */
 int ifgf_plan_profile(ifgf_plan* plan, int enable);
 int ifgf_plan_stage_times(ifgf_plan* plan, double times_ms[6], long long* count);
+
+/* Wall-clock seconds of plan creation by step: [0] discretisation and
+ * near-singular detection, [1] singular / near-singular correction, [2] Morton
+ * sort and tree tables, [3] near-field exclusions and cone records, [4] device
+ * tables, [5] stored operators (kernel evaluations and the level-push
+ * tables), [6] autotuning, [7] total. No reference counterpart (the reference
+ * times nothing inside precomps_and_setup). */
+int ifgf_plan_setup_times(const ifgf_plan* plan, double seconds[8]);
 
 #ifdef __cplusplus
 }

@@ -63,3 +63,53 @@ def test_device_correction_full_size_star():
     err = (np.abs(wd - wh).max(axis=1) / np.abs(wh).max(axis=1)).max()
     print(f"star 256k: {wh.shape[0]} rows, max row difference {err:.3e}", flush=True)
     assert err <= ROW_TOL_DLP
+
+
+GEO_CASES = CASES + [("random-star-" + str(i), kw["curve"], len(kw["patch_lims"]) - 1, kw["k"], kw["nlevels"],
+                      I.COMBINED) for i, kw in enumerate(configs.random_stars(3, k=100.0, seed=3))]
+
+
+@pytest.mark.parametrize("name,curve,npatch,k,nlev,F", GEO_CASES, ids=[c[0] for c in GEO_CASES])
+def test_device_near_singular_search_matches_host(name, curve, npatch, k, nlev, F):
+    """IFGF_PLAN_DEVICE_GEOMETRY: the near-singular points found on the GPU are
+    the host's (the correction's target-major structure, which lists every
+    (target, own or near-singular patch) pair, is identical) and the closest
+    parameters agree (rows within 1e-12, the host computing the correction)."""
+    lims = I.equal_parameter_lims(npatch)
+    delta = I.default_delta(curve, lims)
+    host = I.ForwardMap(curve, lims, k, nlev, formulation=F, delta=delta, p=6.0, host_only=True)
+    dev = I.ForwardMap(curve, lims, k, nlev, formulation=F, delta=delta, p=6.0, device_geometry=True)
+    ph, pth, wh = host.correction()
+    pd, ptd, wd = dev.correction()
+    assert np.array_equal(ph, pd) and np.array_equal(pth, ptd)
+    err = (np.abs(wd - wh).max(axis=1) / np.abs(wh).max(axis=1)).max()
+    print(f"{name}: {wh.shape[0]} rows, {len(pth)} correction entries, max row difference {err:.3e}")
+    assert err <= 1e-12
+
+
+def test_device_setup_forward_map_against_oracle():
+    """All three device setup steps (sort, near-singular search, correction):
+    the forward map within the oracle parity bar."""
+    curve, npatch, k, nlev = I.kite(), 400, 40.0, 6
+    lims = I.equal_parameter_lims(npatch)
+    delta = I.default_delta(curve, lims)
+    dev = I.ForwardMap(curve, lims, k, nlev, delta=delta, p=6.0, device_setup=True)
+    om = O.ForwardMap(curve[0], curve[1], lims, formulation=I.COMBINED, k=k, eta=k, delta=delta, p=6.0,
+                      nlevels=nlev)
+    x = np.random.default_rng(8).standard_normal(dev.N) + 0j
+    yo = om.apply(x)
+    e = np.linalg.norm(dev.compute_Ax_acc(x) - yo) / np.linalg.norm(yo)
+    print("device setup forward map vs oracle", e, dev.setup_times())
+    assert e <= 1e-12
+
+
+def test_device_near_singular_search_full_size_star():
+    """The C3 star (N = 256,000): the same near-singular pairs as the host."""
+    kw, _ = configs.star_k500()
+    host = I.ForwardMap(**kw, host_only=True)
+    dev = I.ForwardMap(**kw, device_setup=True)
+    ph, pth, _ = host.correction()
+    pd, ptd, _ = dev.correction()
+    assert np.array_equal(ph, pd) and np.array_equal(pth, ptd)
+    print("star setup (host-only plan)", host.setup_times())
+    print("star setup (device setup)", dev.setup_times())
