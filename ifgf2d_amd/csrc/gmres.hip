@@ -50,15 +50,11 @@ __device__ inline double block_sum_fixed(const double* __restrict__ x, int nb) {
 
 // part[col * kDot + blockIdx.x] = sum over this block's rows of conj(V[:,col]) * w;
 // the column col == wcol (if any) takes w itself (its squared norm).
-// gate_w / gate_n (second Gram-Schmidt pass only): skip unless
-// second_pass(gate_w, gate_n, ...) holds.
-__device__ inline bool second_pass(const zd* __restrict__ part_w, int wcol, const double* __restrict__ part_n, int nbn);
-
+// gate (second Gram-Schmidt pass only): runs only when gate[0] is set.
 __global__ void __launch_bounds__(256)
     k_dots(long long n, const zd* __restrict__ V, long long ldv, const zd* __restrict__ w, zd* __restrict__ part,
-           int wcol = -1, const zd* __restrict__ gate_w = nullptr, const double* __restrict__ gate_n = nullptr,
-           int gate_nb = 0) {
-  if (gate_w && !second_pass(gate_w, wcol, gate_n, gate_nb)) return;
+           int wcol = -1, const int* __restrict__ gate = nullptr) {
+  if (gate && !gate[0]) return;
   const int col = blockIdx.y;
   const zd* v = col == wcol ? w : V + (size_t)col * ldv;
   zd acc{0.0, 0.0};
@@ -107,12 +103,12 @@ __device__ inline bool second_pass(const zd* __restrict__ part_w, int wcol, cons
 // block 0 stores them into hio (or adds them, for the second Gram-Schmidt
 // pass); then each thread updates its row, w -= V h, and the block writes
 // the partial sum of |w_i|^2 over its rows to part_n[blockIdx.x]. A second
-// pass (gate_w set) runs only when second_pass() holds.
+// pass (gate set) runs only when gate[0] is set.
 __global__ void __launch_bounds__(256)
     k_project_out(long long n, int ncol, const zd* __restrict__ V, long long ldv, const zd* __restrict__ part,
                   zd* __restrict__ hio, int accumulate, zd* __restrict__ w, double* __restrict__ part_n,
-                  const zd* __restrict__ gate_w = nullptr, const double* __restrict__ gate_n = nullptr) {
-  if (gate_w && !second_pass(gate_w, ncol, gate_n, gridDim.x)) return;
+                  const int* __restrict__ gate = nullptr) {
+  if (gate && !gate[0]) return;
   __shared__ zd hs[kMaxRestart + 1];
   __shared__ double red[256];
   for (int c = threadIdx.x; c < ncol; c += blockDim.x) hs[c] = sum_partials(part, c);
@@ -137,14 +133,14 @@ __global__ void __launch_bounds__(256)
 }
 
 // Arnoldi step end: ||w||^2 = the first pass's partials part_n1, or the
-// second's (part_n2) when second_pass() held; V[:, j+1] = w / ||w||; block 0
-// stores ||w|| into h[ncol] and writes the column h[0..ncol] to hcol (pinned
-// host memory).
+// second's (part_n2) when the second pass ran (second[0]); V[:, j+1] =
+// w / ||w||; block 0 stores ||w|| into h[ncol] and writes the column
+// h[0..ncol] to hcol (pinned host memory).
 __global__ void __launch_bounds__(256)
-    k_arnoldi_end(long long n, int ncol, const zd* __restrict__ w, const zd* __restrict__ part_w,
+    k_arnoldi_end(long long n, int ncol, const zd* __restrict__ w, const int* __restrict__ second,
                   const double* __restrict__ part_n1, const double* __restrict__ part_n2, zd* __restrict__ dst,
                   zd* __restrict__ h, zd* __restrict__ hcol) {
-  const bool two = second_pass(part_w, ncol, part_n1, gridDim.x);
+  const bool two = second[0] != 0;
   const double nn = sqrt(block_sum_fixed(two ? part_n2 : part_n1, gridDim.x));
   if (blockIdx.x == 0) {
     for (int c = threadIdx.x; c <= ncol; c += blockDim.x) hcol[c] = c < ncol ? h[c] : zd{nn, 0.0};
@@ -374,6 +370,7 @@ void apply_givens(const Givens& g, std::complex<double>& a, std::complex<double>
 struct Krylov {
   DevBuf<zd> V, w, tmp, part, h;
   DevBuf<double> part_n;  // per-block squared norms of the two Gram-Schmidt passes
+  DevBuf<int> gate;       // 1 when the second Gram-Schmidt pass runs
   zd* hpin = nullptr;
   zd* hpin_dev = nullptr;  // hpin as a device pointer
   hipEvent_t ev[2] = {nullptr, nullptr};
@@ -385,6 +382,7 @@ struct Krylov {
     part.alloc((size_t)kDot * (2 * m + 4));
     h.alloc(m + 2);
     part_n.alloc((size_t)2 * ((n + 255) / 256));
+    gate.alloc(1);
     HIP_TRY(hipHostMalloc((void**)&hpin, 2 * (m + 2) * sizeof(zd), hipHostMallocDefault));
     HIP_TRY(hipHostGetDevicePointer((void**)&hpin_dev, hpin, 0));
     for (auto& e : ev) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
@@ -440,15 +438,16 @@ struct Solver {
     double* pn1 = K.part_n.p;
     double* pn2 = K.part_n.p + nb;
     hipLaunchKernelGGL(k_dots, dim3(kDot, nc + 1), dim3(256), 0, st, n, K.V.p, n, K.w.p, K.part.p, nc,
-                       (const zd*)nullptr, (const double*)nullptr, 0);
+                       (const int*)nullptr);
     hipLaunchKernelGGL(k_project_out, dim3(nb), dim3(256), 0, st, n, nc, K.V.p, n, K.part.p, K.h.p, 0, K.w.p, pn1,
-                       (const zd*)nullptr, (const double*)nullptr);
+                       (const int*)nullptr);
+    hipLaunchKernelGGL(k_gate_ms, dim3(1), dim3(64), 0, st, nc, K.part.p, 0ll, pn1, 0ll, nb, K.gate.p);
     hipLaunchKernelGGL(k_dots, dim3(kDot, nc), dim3(256), 0, st, n, K.V.p, n, K.w.p, part2, nc,
-                       (const zd*)K.part.p, (const double*)pn1, nb);
+                       (const int*)K.gate.p);
     hipLaunchKernelGGL(k_project_out, dim3(nb), dim3(256), 0, st, n, nc, K.V.p, n, part2, K.h.p, 1, K.w.p, pn2,
-                       (const zd*)K.part.p, (const double*)pn1);
+                       (const int*)K.gate.p);
     zd* slot = K.hpin_dev + (size_t)(j % 2) * (K.m + 2);
-    hipLaunchKernelGGL(k_arnoldi_end, dim3(nb), dim3(256), 0, st, n, nc, K.w.p, K.part.p, pn1, pn2,
+    hipLaunchKernelGGL(k_arnoldi_end, dim3(nb), dim3(256), 0, st, n, nc, K.w.p, (const int*)K.gate.p, pn1, pn2,
                        K.V.p + (size_t)(j + 1) * n, K.h.p, slot);
     HIP_TRY(hipEventRecord(K.ev[j % 2], st));
   }
