@@ -90,13 +90,24 @@ __device__ inline zd sum_partials(const zd* __restrict__ part, int c) {
 // Second Gram-Schmidt pass needed: ||w'||^2 (the nbn per-block partials of
 // the first pass's output) < ||w||^2 / 2 (column wcol of the first pass's
 // dot partials). Deterministic and the same in every block.
+// Both sums are formed by the block's first wave in a fixed order (lane l
+// takes partials l, l + 64, ...; then a butterfly).
 __device__ inline bool second_pass(const zd* __restrict__ part_w, int wcol, const double* __restrict__ part_n,
                                    int nbn) {
-  const double after = block_sum_fixed(part_n, nbn);
-  __shared__ double before;
-  if (threadIdx.x == 0) before = sum_partials(part_w, wcol).x;
+  static_assert(kDot == 64, "one dot partial per lane");
+  __shared__ int two;
+  if (threadIdx.x < 64) {
+    double after = 0.0;
+    for (int i = threadIdx.x; i < nbn; i += 64) after += part_n[i];
+    double before = part_w[(size_t)wcol * kDot + threadIdx.x].x;
+    for (int o = 32; o > 0; o >>= 1) {
+      after += __shfl_xor(after, o, 64);
+      before += __shfl_xor(before, o, 64);
+    }
+    if (threadIdx.x == 0) two = after < 0.5 * before ? 1 : 0;
+  }
   __syncthreads();
-  return after < 0.5 * before;
+  return two != 0;
 }
 
 // Every block reduces the projections h[c] = sum_b part[c][b] (c < ncol);
