@@ -1424,7 +1424,7 @@ int ifgf_field_create(const ifgf_plan* src, const double* tx, const double* ty, 
   P->prm.Pang = Pang;
   P->prm.precision = precision;
   P->prm.nrhs_max = 1;
-  P->prm.flags = 0;
+  P->prm.flags = src->prm.flags & IFGF_PLAN_AUTOTUNE;  // a tuned plan's field is tuned too
   P->geo = src->geo;
   std::string err;
   try {
@@ -1437,6 +1437,7 @@ int ifgf_field_create(const ifgf_plan* src, const double* tx, const double* ty, 
     if (src->on_device) {  // a field of a host-only plan holds the tables only
       build_device_side(P);
       if (!build_stored_any(P)) return fail(nullptr, IFGF_ERR_ARG, "no kernel instantiation for this (Ps, Pang)");
+      if (P->prm.flags & IFGF_PLAN_AUTOTUNE) autotune(P);
     }
   } catch (std::exception& e) {
     return fail(nullptr, IFGF_ERR_HIP, e.what());

@@ -224,7 +224,8 @@ int ifgf_propagate(ifgf_plan* plan, const double* tx, const double* ty, long lon
  * precision: IFGF_FP64 or IFGF_MIXED. Errors: IFGF_ERR_ARG (nt < 1, unknown
  * (Ps, Pang) or precision), IFGF_ERR_LEVELS, IFGF_ERR_GEOMETRY, IFGF_ERR_HIP.
  * The field of a host-only plan holds the host tables only (stats work,
- * ifgf_field_apply returns IFGF_ERR_NO_DEVICE). The field keeps its own copy
+ * ifgf_field_apply returns IFGF_ERR_NO_DEVICE); the field of a plan created
+ * with IFGF_PLAN_AUTOTUNE is tuned the same way. The field keeps its own copy
  * of the points; the plan may be destroyed afterwards. */
 typedef struct ifgf_field ifgf_field;
 int ifgf_field_create(const ifgf_plan* plan, const double* tx, const double* ty, long long nt, int nlevels,

@@ -150,3 +150,18 @@ def test_field_kite_64k_against_direct_subset():
         print("64k kite field", P, f.stats(), "rel err vs direct", errs[P])
         assert np.array_equal(out, f.evaluate(u))
     assert errs[5] < 1e-2 and errs[9] < 1e-5 and errs[9] < errs[5] / 10
+
+
+def test_field_of_autotuned_plan():
+    """A tuned plan's field is tuned too (task sizes, lanes, near field on a
+    second stream); it agrees with the untuned field to rounding."""
+    curve, lims = I.kite(), I.equal_parameter_lims(400)
+    fa = I.ForwardMap(curve, lims, 40.0, 6)
+    fb = I.ForwardMap(curve, lims, 40.0, 6, autotune=True)
+    x, y = fa.points()[:2]
+    tx, ty = targets("grid", x, y)
+    u = density(fa.N, 13)
+    a = fa.field(tx, ty).evaluate(u)
+    b = fb.field(tx, ty).evaluate(u)
+    print("tuned field vs default", rel(b, a))
+    assert rel(b, a) < 1e-13
