@@ -516,9 +516,15 @@ int gmres_core(Solver& S, Krylov& K, bool fp64, const zd* b, zd* x, int m, doubl
     g[0] = beta;
     const int jmax = std::min(m, maxit - it);
     enqueue(0);
+    int queued = 1;  // iterations [0, queued) are enqueued
+    double e1 = 0.0, e2 = 0.0;  // the last two residual estimates of this cycle
     int j = 0;
     for (; j < jmax; j++) {
-      if (j + 1 < jmax) enqueue(j + 1);  // needs only V[:, j+1]
+      // iteration j + 1 needs only V[:, j+1]: enqueue it before waiting for
+      // j, unless the estimates' last ratio predicts that j converges (then
+      // it would be thrown away)
+      const bool likely_last = j >= 2 && e2 > 0.0 && e1 * (e1 / e2) < 1.5 * tol;
+      if (queued == j + 1 && j + 1 < jmax && !likely_last) enqueue(queued++);
       S.arnoldi_wait(K, j, hv);
       hv[j + 1] = std::sqrt(std::norm(hv[j + 1]));
       for (int i = 0; i < j; i++) apply_givens(rot[i], hv[i], hv[i + 1]);
@@ -534,6 +540,9 @@ int gmres_core(Solver& S, Krylov& K, bool fp64, const zd* b, zd* x, int m, doubl
         j++;
         break;
       }
+      e2 = e1;
+      e1 = est;
+      if (queued == j + 1 && j + 1 < jmax) enqueue(queued++);  // the prediction missed
     }
     // x += V y (after the speculative iteration, if one is in flight: it only
     // wrote w, V[:, j+1] and the Hessenberg buffers)
@@ -616,7 +625,10 @@ extern "C" int ifgf_fgmres(ifgf_plan* P, const double* rhs, double* x, int resta
       *relres = 0.0;
       return IFGF_OK;
     }
-    S.residual(b.p, xd.p, K.tmp.p, true);
+    bool zero = true;
+    for (long long i = 0; i < 2 * n && zero; i++) zero = x[i] == 0.0;
+    if (zero) HIP_TRY(hipMemcpyAsync(K.tmp.p, b.p, n * sizeof(zd), hipMemcpyDeviceToDevice, S.st));  // r0 = b
+    else S.residual(b.p, xd.p, K.tmp.p, true);
     const double r0n = S.norm(K, K.tmp.p);
     double beta = r0n;
     bool converged = (r0n == 0.0);
