@@ -589,10 +589,25 @@ struct PushArgsT {
   int rs;                // batched kernels: columns per slot in pbuf (the widest block)
 };
 
+// einfo == nullptr: igeo holds every push entry's record in push order
+// (mixed-precision and batched launches, which measured faster without the
+// dependent load in front of the geometry).
 template <typename R>
 __device__ inline Geo4T<R> push_geo(const PushArgsT<R>& a, long long q) {
+  if (!a.einfo) return a.igeo[q];
   const int inf = a.einfo[q];
   return inf >= 0 ? a.igeo[inf] : a.ptab[-1 - inf];
+}
+
+// Build: the per-entry record of every push entry from the shared tables.
+template <typename R>
+__global__ void __launch_bounds__(256)
+    k_geo_direct(long long n, const int* __restrict__ einfo, const Geo4T<R>* __restrict__ igeo,
+                 const Geo4T<R>* __restrict__ ptab, Geo4T<R>* __restrict__ out) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const int inf = einfo[q];
+  out[q] = inf >= 0 ? igeo[inf] : ptab[-1 - inf];
 }
 
 // Local cone (0 .. task.nc-1) of entry e of a task: entries are grouped by

@@ -318,7 +318,7 @@ static void launch_level_push_ept(ifgf_plan* P, hipStream_t st) {
   const int D = P->tree.D;
   PushArgsT<R> a{};
   if constexpr (std::is_same<R, float>::value) {
-    a = PushArgsT<R>{nullptr, P->push_einfo.p, P->push_igeo32.p, P->push_ptab32.p, P->push_coff.p, P->push_cidx.p,
+    a = PushArgsT<R>{nullptr, nullptr, P->push_geo32.p, nullptr, P->push_coff.p, P->push_cidx.p,
                      P->leaf_item_off.p, P->lpart32.p, P->tree.lv[D].cone_off, P->ifac_off.p, P->ifac_tab32.p,
                      P->tabs32.p, P->pbuf32.p, 0};
   } else {
@@ -479,11 +479,11 @@ static void launch_batched(ifgf_plan* P, const zd* x, long long ld, zd* y, hipSt
   Z* pbuf = f32 ? (Z*)P->pbuf_b32.p : (Z*)P->pbuf_b.p;
   PushArgsT<RT> a{};
   if constexpr (f32) {
-    a = PushArgsT<RT>{nullptr, P->push_einfo.p, P->push_igeo32.p, P->push_ptab32.p, P->push_coff.p, P->push_cidx.p,
+    a = PushArgsT<RT>{nullptr, nullptr, P->push_geo32.p, nullptr, P->push_coff.p, P->push_cidx.p,
                       P->leaf_item_off.p, lpart, t.lv[D].cone_off, P->ifac_off.p, P->ifac_tab32.p, P->tabs32.p, pbuf,
                       0, P->rb_max};
   } else {
-    a = PushArgsT<RT>{nullptr, P->push_einfo.p, P->push_igeo.p, P->push_ptab.p, P->push_coff.p, P->push_cidx.p,
+    a = PushArgsT<RT>{nullptr, nullptr, P->push_geo_b.p, nullptr, P->push_coff.p, P->push_cidx.p,
                       P->leaf_item_off.p, lpart, t.lv[D].cone_off, P->ifac_off.p, P->ifac_tab.p, P->tabs.p, pbuf, 0,
                       P->rb_max};
   }
@@ -582,6 +582,12 @@ static void build_batched(ifgf_plan* P) {
     P->lpart_b.alloc(P->lpart.n * R);
     P->pbuf_b.alloc((size_t)P->nslots * R);
     HIP_TRY(hipMemsetAsync(P->pbuf_b.p, 0, P->pbuf_b.n * sizeof(zd), P->stream));
+    const long long npush = P->nslots - 1;
+    P->push_geo_b.alloc(npush);
+    if (npush > 0)
+      hipLaunchKernelGGL(k_geo_direct<double>, dim3((unsigned)((npush + 255) / 256)), dim3(256), 0, P->stream, npush,
+                         P->push_einfo.p, P->push_igeo.p, P->push_ptab.p, P->push_geo_b.p);
+    HIP_TRY(hipGetLastError());
   }
   const int zb = P->mixed ? (int)sizeof(zf) : (int)sizeof(zd);
   for (int lr = 1; lr <= 3; lr++) {
@@ -592,7 +598,8 @@ static void build_batched(ifgf_plan* P) {
     P->btasks[lr].upload(tasks);
   }
   P->stored_bytes += (long long)(P->npart_b.n + P->lpart_b.n + P->pbuf_b.n) * (long long)sizeof(zd) +
-                     (long long)(P->lpart_b32.n + P->pbuf_b32.n) * (long long)sizeof(zf);
+                     (long long)(P->lpart_b32.n + P->pbuf_b32.n) * (long long)sizeof(zf) +
+                     (long long)P->push_geo_b.n * (long long)sizeof(Geo4);
 }
 
 template <int F, int Ps, int Pang>
@@ -891,6 +898,10 @@ static void build_stored(ifgf_plan* P) {
     round(P->push_igeo.p, P->push_igeo32.p, 2 * (long long)P->push_igeo.n);
     P->push_ptab32.alloc(P->push_ptab.n);
     round(P->push_ptab.p, P->push_ptab32.p, 2 * (long long)P->push_ptab.n);
+    P->push_geo32.alloc(npush);
+    if (npush > 0)
+      hipLaunchKernelGGL(k_geo_direct<float>, dim3((unsigned)((npush + 255) / 256)), dim3(256), 0, st, npush,
+                         P->push_einfo.p, P->push_igeo32.p, P->push_ptab32.p, P->push_geo32.p);
     P->ifac_tab32.alloc(P->ifac_tab.n);
     round(P->ifac_tab.p, P->ifac_tab32.p, (long long)P->ifac_tab.n);
     P->pbuf32.alloc(P->pbuf.n);
@@ -909,7 +920,7 @@ static void build_stored(ifgf_plan* P) {
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(st));
     P->stored_bytes += (long long)(P->lf_val32.n + P->ifac_tab32.n + P->pbuf32.n) * (long long)sizeof(zf) +
-                       (long long)(P->push_igeo32.n + P->push_ptab32.n) * (long long)sizeof(Geo4F);
+                       (long long)(P->push_igeo32.n + P->push_ptab32.n + P->push_geo32.n) * (long long)sizeof(Geo4F);
     P->mixed = true;
   }
   build_batched<Ps, Pang>(P);

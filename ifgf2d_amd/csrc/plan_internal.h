@@ -132,6 +132,10 @@ struct ifgf_plan {
   bool mixed = false;
   ifgf::DevBuf<zf> lf_val32, ifac_tab32, pbuf32, lpart32;
   ifgf::DevBuf<ifgf::Geo4F> push_igeo32, push_ptab32;
+  // per-entry push records in push order (no entry-record indirection): the
+  // FP32 ones on a mixed plan, the FP64 ones on a plan with batched blocks
+  ifgf::DevBuf<ifgf::Geo4F> push_geo32;
+  ifgf::DevBuf<ifgf::Geo4> push_geo_b;
   ifgf::DevBuf<ifgf::ChebTablesF> tabs32;
   int nleaf = 0;
   // lanes per work item of the stored-path kernels
