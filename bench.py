@@ -176,18 +176,17 @@ def main():
     stored = ext["stored"]
     if stored:
         # one push launch per level carries propagation and interpolation;
-        # per evaluation it reads a 4 B entry record and writes its 16 B value;
-        # an interpolation reads its own 32 B (s, theta, factor), propagations
-        # share the stored records of their (level, parent direction, child
-        # position, node) (each counted once); per parent-cone node it reads a
-        # 16 B child-slot index and the children's pushed values (16 B per
+        # per evaluation it reads its 32 B (s, theta, factor) record and
+        # writes its 16 B value; per parent-cone node it reads a 16 B
+        # child-slot index and the children's pushed values (16 B per
         # propagation evaluation); per parent cone a 4 B row of the inverse
-        # node factors (each 16 B factor counted once)
+        # node factors, stored once per (level, radial interval, s node), each
+        # 16 B factor counted once
         flop = work["propagation"][1] + work["interpolation"][1]
         nev = st["prop"] + st["interp"]
         pcones = st["cones"] - ext["leaf_cones"]
-        byts = (20.0 * nev + 32.0 * st["interp"] + 32.0 * ext["ptab_entries"] + 16.0 * 25 * pcones
-                + 16.0 * st["prop"] + 4.0 * pcones + 16.0 * ext["ifac_entries"])
+        byts = (48.0 * nev + 16.0 * 25 * pcones + 16.0 * st["prop"] + 4.0 * pcones
+                + 16.0 * ext["ifac_entries"])
         work["propagation"] = ("mixed", (flop, byts), "FLOP+B")
         work.pop("interpolation")
     dom = max(work, key=lambda s: stage_avg[s])

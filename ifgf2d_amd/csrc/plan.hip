@@ -322,7 +322,9 @@ static void launch_level_push_ept(ifgf_plan* P, hipStream_t st) {
                      P->leaf_item_off.p, P->lpart32.p, P->tree.lv[D].cone_off, P->ifac_off.p, P->ifac_tab32.p,
                      P->tabs32.p, P->pbuf32.p, 0};
   } else {
-    a = PushArgsT<R>{nullptr, P->push_einfo.p, P->push_igeo.p, P->push_ptab.p, P->push_coff.p, P->push_cidx.p,
+    const bool direct = P->push_geo_b.n > 0;
+    a = PushArgsT<R>{nullptr, direct ? nullptr : P->push_einfo.p, direct ? P->push_geo_b.p : P->push_igeo.p,
+                     direct ? nullptr : P->push_ptab.p, P->push_coff.p, P->push_cidx.p,
                      P->leaf_item_off.p, P->lpart.p, P->tree.lv[D].cone_off, P->ifac_off.p, P->ifac_tab.p,
                      P->tabs.p, P->pbuf.p, 0};
   }
@@ -583,11 +585,14 @@ static void build_batched(ifgf_plan* P) {
     P->pbuf_b.alloc((size_t)P->nslots * R);
     HIP_TRY(hipMemsetAsync(P->pbuf_b.p, 0, P->pbuf_b.n * sizeof(zd), P->stream));
     const long long npush = P->nslots - 1;
-    P->push_geo_b.alloc(npush);
-    if (npush > 0)
-      hipLaunchKernelGGL(k_geo_direct<double>, dim3((unsigned)((npush + 255) / 256)), dim3(256), 0, P->stream, npush,
-                         P->push_einfo.p, P->push_igeo.p, P->push_ptab.p, P->push_geo_b.p);
-    HIP_TRY(hipGetLastError());
+    if (P->push_geo_b.n != (size_t)npush) {  // a field with batched blocks
+      P->push_geo_b.alloc(npush);
+      if (npush > 0)
+        hipLaunchKernelGGL(k_geo_direct<double>, dim3((unsigned)((npush + 255) / 256)), dim3(256), 0, P->stream,
+                           npush, P->push_einfo.p, P->push_igeo.p, P->push_ptab.p, P->push_geo_b.p);
+      HIP_TRY(hipGetLastError());
+      P->stored_bytes += (long long)P->push_geo_b.n * (long long)sizeof(Geo4);
+    }
   }
   const int zb = P->mixed ? (int)sizeof(zf) : (int)sizeof(zd);
   for (int lr = 1; lr <= 3; lr++) {
@@ -598,8 +603,7 @@ static void build_batched(ifgf_plan* P) {
     P->btasks[lr].upload(tasks);
   }
   P->stored_bytes += (long long)(P->npart_b.n + P->lpart_b.n + P->pbuf_b.n) * (long long)sizeof(zd) +
-                     (long long)(P->lpart_b32.n + P->pbuf_b32.n) * (long long)sizeof(zf) +
-                     (long long)P->push_geo_b.n * (long long)sizeof(Geo4);
+                     (long long)(P->lpart_b32.n + P->pbuf_b32.n) * (long long)sizeof(zf);
 }
 
 template <int F, int Ps, int Pang>
@@ -922,6 +926,29 @@ static void build_stored(ifgf_plan* P) {
     P->stored_bytes += (long long)(P->lf_val32.n + P->ifac_tab32.n + P->pbuf32.n) * (long long)sizeof(zf) +
                        (long long)(P->push_igeo32.n + P->push_ptab32.n + P->push_geo32.n) * (long long)sizeof(Geo4F);
     P->mixed = true;
+  }
+  // Forward-map plans read per-entry FP64 records in push order (no
+  // entry-record indirection in front of the geometry: 3 us per application
+  // faster at the 64k kite); a field keeps the shared tables, whose
+  // propagation part is most of its entries.
+  if (!P->field) {
+    P->push_geo_b.alloc(npush);
+    if (npush > 0)
+      hipLaunchKernelGGL(k_geo_direct<double>, dim3((unsigned)((npush + 255) / 256)), dim3(256), 0, st, npush,
+                         P->push_einfo.p, P->push_igeo.p, P->push_ptab.p, P->push_geo_b.p);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(st));
+    P->stored_bytes += (long long)npush * (long long)sizeof(Geo4) -
+                       (long long)(P->push_igeo.n + P->push_ptab.n) * (long long)sizeof(Geo4) -
+                       npush * (long long)sizeof(int);
+    P->push_einfo.free();
+    P->push_igeo.free();
+    P->push_ptab.free();
+    if (P->mixed) {
+      P->stored_bytes -= (long long)(P->push_igeo32.n + P->push_ptab32.n) * (long long)sizeof(Geo4F);
+      P->push_igeo32.free();
+      P->push_ptab32.free();
+    }
   }
   build_batched<Ps, Pang>(P);
   HIP_TRY(hipStreamSynchronize(st));
