@@ -43,6 +43,7 @@ IFGF_PLAN_DEVICE_SORT = 8
 IFGF_PLAN_OVERLAP = 16
 IFGF_PLAN_DEVICE_PRECOMP = 32
 IFGF_PLAN_DEVICE_GEOMETRY = 64
+IFGF_PLAN_DEVICE_TREE = 128
 
 
 class IFGFError(RuntimeError):
@@ -211,13 +212,15 @@ class ForwardMap:
     device_precomp: build the rectangular-polar singular / near-singular correction on the
     GPU instead of the host.
     device_geometry: run the near-singular search of the discretisation on the GPU.
-    device_setup: all three device setup steps (sort, correction, near-singular search).
+    device_tree: build the tree's relevant cone segments (interpolation entries, protoboxes,
+        numbering) on the GPU; the tables equal the host's.
+    device_setup: all four device setup steps (sort, correction, near-singular search, cones).
     """
 
     def __init__(self, curve, patch_lims, k, nlevels, Np=10, formulation="combined", eta=None,
                  delta=None, p=4.0, Ps=5, Pang=5, precision="fp64", nrhs_max=1, host_only=False,
                  on_the_fly=False, autotune=False, device_sort=False, overlap=False, device_precomp=False,
-                 device_geometry=False, device_setup=False):
+                 device_geometry=False, device_setup=False, device_tree=False):
         L = lib()
         self.curve = curve
         self.lims = np.ascontiguousarray(patch_lims, dtype=np.float64)
@@ -231,9 +234,11 @@ class ForwardMap:
         flags = ((IFGF_PLAN_HOST_ONLY if host_only else 0) | (IFGF_PLAN_ON_THE_FLY if on_the_fly else 0)
                  | (IFGF_PLAN_AUTOTUNE if autotune else 0) | (IFGF_PLAN_DEVICE_SORT if device_sort else 0)
                  | (IFGF_PLAN_OVERLAP if overlap else 0) | (IFGF_PLAN_DEVICE_PRECOMP if device_precomp else 0)
-                 | (IFGF_PLAN_DEVICE_GEOMETRY if device_geometry else 0))
+                 | (IFGF_PLAN_DEVICE_GEOMETRY if device_geometry else 0)
+                 | (IFGF_PLAN_DEVICE_TREE if device_tree else 0))
         if device_setup:
-            flags |= IFGF_PLAN_DEVICE_SORT | IFGF_PLAN_DEVICE_PRECOMP | IFGF_PLAN_DEVICE_GEOMETRY
+            flags |= (IFGF_PLAN_DEVICE_SORT | IFGF_PLAN_DEVICE_PRECOMP | IFGF_PLAN_DEVICE_GEOMETRY
+                      | IFGF_PLAN_DEVICE_TREE)
         prm = _Params(self.formulation, _PREC[precision], self.k, self.eta, Ps, Pang, nlevels,
                       nrhs_max, flags)
         h = ctypes.c_void_p()

@@ -16,6 +16,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "cone_grid.h"
 #include "green.h"
 #include "tables.h"
 
@@ -48,18 +49,6 @@ template <>
 struct vec2_of<zf> {
   using type = flt2;
 };
-
-// Functions::Atan2 (PolarCoordinates.h:8-27): theta in [0, 2 pi).
-__device__ inline double atan2_02pi(double y, double x) {
-  if (x > 0 && y >= 0) return atan(y / x);
-  if (x > 0 && y < 0) return atan(y / x) + 2.0 * M_PI;
-  if (x < 0 && y > 0) return atan(y / x) + M_PI;
-  if (x < 0 && y == 0) return M_PI;
-  if (x < 0 && y < 0) return atan(y / x) + M_PI;
-  if (x == 0 && y > 0) return M_PI_2;
-  if (x == 0 && y < 0) return 3.0 * M_PI_2;
-  return 0.0;
-}
 
 // Level::Cheb2Cart + Cheb2Radius (Level.h:347-375, 406-411).
 __device__ inline void cone_node(const ConeGeo& g, double s, double t, double& x, double& y,

@@ -11,6 +11,7 @@
 #include <numeric>
 #include <unordered_map>
 
+#include "cone_grid.h"
 #include "green.h"
 
 namespace ifgf {
@@ -228,30 +229,9 @@ static uint64_t split_by2(uint64_t v, int level) {
   x = (x | (x << 1)) & 0x5555555555555555ull;
   return x;
 }
-static uint64_t merge_from2(uint64_t x) {
-  x &= 0x5555555555555555ull;
-  x = (x ^ (x >> 1)) & 0x3333333333333333ull;
-  x = (x ^ (x >> 2)) & 0xF0F0F0F0F0F0F0Full;
-  x = (x ^ (x >> 4)) & 0xFF00FF00FF00FFull;
-  x = (x ^ (x >> 8)) & 0xFFFF0000FFFFull;
-  x = (x ^ (x >> 16)) & 0xffffffffull;
-  return x;
-}
 static long long morton(long long px, long long py, int level) {
   return (long long)(split_by2((uint64_t)px, level) | (split_by2((uint64_t)py, level) << 1));
 }
-// Functions::Atan2, IFGF_Source/PolarCoordinates.h:8-27 (theta in [0, 2 pi)).
-static double atan2_02pi(double y, double x) {
-  if (x > 0 && y >= 0) return std::atan(y / x);
-  if (x > 0 && y < 0) return std::atan(y / x) + 2.0 * M_PI;
-  if (x < 0 && y > 0) return std::atan(y / x) + M_PI;
-  if (x < 0 && y == 0) return M_PI;
-  if (x < 0 && y < 0) return std::atan(y / x) + M_PI;
-  if (x == 0 && y > 0) return M_PI_2;
-  if (x == 0 && y < 0) return 3.0 * M_PI_2;
-  return 0.0;
-}
-
 namespace {
 struct LevelGeom {
   double min_x, min_y, bs, h;
@@ -282,45 +262,17 @@ struct LevelGeom {
     ok = !(px < 0 || py < 0 || px >= nbps || py >= nbps);
     return ok ? morton(px, py, level) : -1;
   }
-  void center(long long m, double& cx, double& cy) const {  // GetBoxCenter, Level.h:291-298
-    const long long px = (long long)merge_from2((uint64_t)m), py = (long long)merge_from2((uint64_t)m >> 1);
-    cx = min_x + px * bs + 0.5 * bs;
-    cy = min_y + py * bs + 0.5 * bs;
-  }
-  // Cart2Nonrelcone, Level.h:310-323 (returns -1 if outside the cone grid)
-  long long nonrel(long long m, double x, double y) const {
-    double cx, cy;
-    center(m, cx, cy);
-    x -= cx;
-    y -= cy;
-    const double r = std::sqrt(x * x + y * y);
-    if (r == 0.0) return -1;
-    const double th = atan2_02pi(y, x);
-    const long long posr = (long long)(std::upper_bound(radial.begin(), radial.end(), r) - radial.begin()) - 1;
-    const long long post = (long long)(th * nce / M_PI);
-    if (posr < 0 || posr >= nrad || post < 0 || post > 2 * nce) return -1;
-    return post * nrad + posr;
-  }
-  // Cheb2Cart, Level.h:347-375
+  ConeGrid grid() const { return ConeGrid{min_x, min_y, bs, h, (int)nce, (int)nrad, radial.data()}; }
+  long long nonrel(long long m, double x, double y) const { return grid_nonrel(grid(), m, x, y); }
   void cheb2cart(long long m, long long q, double s, double t, double& x, double& y) const {
-    const long long posr = q % nrad, post = q / nrad;
-    const double mn = radial[posr];
-    const double rmin = h / radial[posr + 1];
-    const double dr = h / mn - rmin;
-    const double r = h / ((s + 1.0) / 2.0 * dr + rmin);
-    const double da = M_PI / nce;
-    const double th = (t + 1.0) / 2.0 * da + da * post;
-    double cx, cy;
-    center(m, cx, cy);
-    x = r * std::cos(th) + cx;
-    y = r * std::sin(th) + cy;
+    grid_cheb2cart(grid(), m, q, s, t, x, y);
   }
 };
 }  // namespace
 
 bool build_tree(const double* x, const double* y, const double* nx, const double* ny, long long N,
                 int nlev, double k, int Ps, int Pang, Tree& t, std::string& err, const std::vector<int>* perm_in,
-                const double* tx, const double* ty, long long M) {
+                const double* tx, const double* ty, long long M, ConeBuilder cones) {
   if (nlev < 3) {
     err = "IFGF accelerator requires at least 3 levels.";
     return false;
@@ -477,7 +429,7 @@ bool build_tree(const double* x, const double* y, const double* nx, const double
     L.nslot = (int)((2 * G[l].nce + 1) * G[l].nrad);
     for (long long m : boxes[l]) {
       double cx, cy;
-      G[l].center(m, cx, cy);
+      grid_center(G[l].grid(), m, cx, cy);
       t.box_morton.push_back(m);
       t.box_cx.push_back(cx);
       t.box_cy.push_back(cy);
@@ -496,8 +448,8 @@ bool build_tree(const double* x, const double* y, const double* nx, const double
   // of the parent's neighbours that are not neighbours, GetCousins,
   // Level.h:261-289), as local box indices in ascending Morton order
   auto neighbours_of = [&](int l, long long m, std::vector<int>& out) {
-    const long long bx = (long long)merge_from2((uint64_t)m);
-    const long long by = (long long)merge_from2((uint64_t)m >> 1);
+    const long long bx = (long long)morton_merge2((uint64_t)m);
+    const long long by = (long long)morton_merge2((uint64_t)m >> 1);
     std::vector<std::pair<long long, int>> nb;
     for (long long j = std::max(0ll, by - 1); j <= std::min(G[l].nbps - 1, by + 1); j++)
       for (long long i = std::max(0ll, bx - 1); i <= std::min(G[l].nbps - 1, bx + 1); i++) {
@@ -509,8 +461,8 @@ bool build_tree(const double* x, const double* y, const double* nx, const double
     for (auto& e : nb) out.push_back(e.second);
   };
   auto cousins_of = [&](int l, long long m, std::vector<int>& out) {
-    const long long bx = (long long)merge_from2((uint64_t)m);
-    const long long by = (long long)merge_from2((uint64_t)m >> 1);
+    const long long bx = (long long)morton_merge2((uint64_t)m);
+    const long long by = (long long)morton_merge2((uint64_t)m >> 1);
     const long long xlo = std::max<long long>((bx / 2) * 2 - 2, 0), xup = std::min<long long>(G[l].nbps - 1, (bx / 2) * 2 + 3);
     const long long ylo = std::max<long long>((by / 2) * 2 - 2, 0), yup = std::min<long long>(G[l].nbps - 1, (by / 2) * 2 + 3);
     std::vector<std::pair<long long, int>> c;
@@ -615,108 +567,143 @@ bool build_tree(const double* x, const double* y, const double* nx, const double
     err = "too many interpolation entries";
     return false;
   }
-  std::vector<int> ent_q(nent);
-#pragma omp parallel for schedule(dynamic, 256) reduction(|| : bad)
-  for (long long i = 0; i < NT; i++) {
-    long long e = t.it_ptr[i];
-    for (int l = D; l >= 2; l--)
-      for (int c : cz[l][pbox[(size_t)i * (D + 1) + l]]) {
-        const long long q = G[l].nonrel(boxes[l][c], TX[i], TY[i]);
-        if (q < 0) bad = true;
-        ent_q[e++] = (int)q;
-      }
-  }
-  if (bad) {
-    err = "target outside the cone grid of a cousin box";
-    return false;
-  }
-  // mark relevant cones, level by level top-down, then number them (box
-  // ascending, non-relevant id ascending) -- InitializeRelevantConeSegments,
-  // BoxTree.h:553-583 with the merge of :430-459 and flattening of :461-514.
-  long long map_total = 0;
+  // Dense (box, non-relevant id) maps and protoboxes, level by level;
+  // interpolation points (Interpolator.h:37-41): s = xs[j % Ps], theta = xang[j / Ps].
+  long long map_total = 0, proto_total = 0;
   for (int l = 2; l <= D; l++) {
     t.lv[l].map_off = map_total;
     map_total += (long long)t.lv[l].nbox * t.lv[l].nslot;
+    if (l > 2) {
+      t.lv[l].proto_off = proto_total;
+      proto_total += (long long)t.lv[l - 1].nslot * 4 * P;
+    }
   }
-  t.dense_map.assign(map_total, -1);
-  std::vector<char> mark(map_total, 0);
-  for (long long i = 0; i < NT; i++) {
-    long long e = t.it_ptr[i];
-    for (int l = D; l >= 2; l--)
-      for (int c : cz[l][pbox[(size_t)i * (D + 1) + l]]) mark[t.lv[l].map_off + (long long)c * t.lv[l].nslot + ent_q[e++]] = 1;
-  }
-  // interpolation points (Interpolator.h:37-41): s = xs[j % Ps], theta = xang[j / Ps]
   std::vector<double> xs(Ps), xa(Pang);
   for (int i = 0; i < Ps; i++) xs[Ps - 1 - i] = std::cos(M_PI * (2.0 * i + 1.0) / (2.0 * Ps));
   for (int i = 0; i < Pang; i++) xa[Pang - 1 - i] = std::cos(M_PI * (2.0 * i + 1.0) / (2.0 * Pang));
-  int cone_off = 0;
-  for (int l = 2; l <= D; l++) {
-    LevelInfo& L = t.lv[l];
-    if (l > 2) {
-      // cones needed by the parents (GetRelevantConeSegmentsDueToParents,
-      // BoxTree.h:299-428): protobox[q][cp] is filled from the first parent
-      // cone with non-relevant id q that has a child at position cp.
-      const LevelInfo& Pl = t.lv[l - 1];
-      L.proto_off = (long long)t.proto.size();
-      t.proto.resize(t.proto.size() + (size_t)Pl.nslot * 4 * P, -1);
-      int* proto = &t.proto[L.proto_off];
-      for (int g = Pl.cone_off; g < Pl.cone_off + Pl.ncone; g++) {
-        const int pb = t.cone_box[g];
-        const long long q = (long long)t.cone_post[g] * Pl.nrad + t.cone_posr[g];
-        for (int cp = 0; cp < 4; cp++) {
-          const int cb = t.box_child[(size_t)(Pl.box_off + pb) * 4 + cp];
-          if (cb < 0) continue;
-          int* slot = proto + ((size_t)q * 4 + cp) * P;
-          if (slot[0] < 0) {
-            for (int j = 0; j < P; j++) {
-              double px, py;
-              G[l - 1].cheb2cart(boxes[l - 1][pb], q, xs[j % Ps], xa[j / Ps], px, py);
-              const long long cq = G[l].nonrel(boxes[l][cb], px, py);
-              if (cq < 0) {
-                err = "parent interpolation node outside the child cone grid";
-                return false;
-              }
-              slot[j] = (int)cq;
-            }
-          }
-          for (int j = 0; j < P; j++) mark[L.map_off + (long long)cb * L.nslot + slot[j]] = 1;
-          t.n_prop += P;
-        }
-      }
+  if (cones) {
+    ConeInput in;
+    in.tx = TX;
+    in.ty = TY;
+    in.nt = NT;
+    in.pbox = pbox.data();
+    in.xs = xs;
+    in.xa = xa;
+    in.grp_off.assign(D + 2, 0);
+    long long ngrp = 0, ncz = 0;
+    for (int l = 2; l <= D; l++) {
+      in.grp_off[l] = ngrp;
+      ngrp += (long long)cz[l].size();
+      for (const auto& v : cz[l]) ncz += (long long)v.size();
     }
-    L.cone_off = cone_off;
-    int nc = 0;
-    for (int b = 0; b < L.nbox; b++)
-      for (int q = 0; q < L.nslot; q++) {
-        const long long a = L.map_off + (long long)b * L.nslot + q;
-        if (!mark[a]) continue;
-        t.dense_map[a] = nc++;
-        t.cone_box.push_back(b);
-        t.cone_posr.push_back(q % L.nrad);
-        t.cone_post.push_back(q / L.nrad);
-        t.cone_level.push_back((unsigned char)l);
-      }
-    if (nc == 0 && !sep) {  // a field's coarse levels may have none (its targets are near every source)
-      err = "The number of cone segments on any level cannot be 0";
+    in.grp_off[D + 1] = ngrp;
+    if (ncz > 2147483647LL) {
+      err = "too many cousin entries";
       return false;
     }
-    L.ncone = nc;
-    cone_off += nc;
+    in.cz_ptr.reserve(ngrp + 1);
+    in.cz_list.reserve(ncz);
+    in.cz_ptr.push_back(0);
+    for (int l = 2; l <= D; l++)
+      for (const auto& v : cz[l]) {
+        in.cz_list.insert(in.cz_list.end(), v.begin(), v.end());
+        in.cz_ptr.push_back((int)in.cz_list.size());
+      }
+    if (!cones(in, t, err)) return false;
+  } else {
+    std::vector<int> ent_q(nent);
+#pragma omp parallel for schedule(dynamic, 256) reduction(|| : bad)
+    for (long long i = 0; i < NT; i++) {
+      long long e = t.it_ptr[i];
+      for (int l = D; l >= 2; l--)
+        for (int c : cz[l][pbox[(size_t)i * (D + 1) + l]]) {
+          const long long q = G[l].nonrel(boxes[l][c], TX[i], TY[i]);
+          if (q < 0) bad = true;
+          ent_q[e++] = (int)q;
+        }
+    }
+    if (bad) {
+      err = "target outside the cone grid of a cousin box";
+      return false;
+    }
+    // mark relevant cones, level by level top-down, then number them (box
+    // ascending, non-relevant id ascending) -- InitializeRelevantConeSegments,
+    // BoxTree.h:553-583 with the merge of :430-459 and flattening of :461-514.
+    t.dense_map.assign(map_total, -1);
+    t.proto.assign(proto_total, -1);
+    std::vector<char> mark(map_total, 0);
+    for (long long i = 0; i < NT; i++) {
+      long long e = t.it_ptr[i];
+      for (int l = D; l >= 2; l--)
+        for (int c : cz[l][pbox[(size_t)i * (D + 1) + l]]) mark[t.lv[l].map_off + (long long)c * t.lv[l].nslot + ent_q[e++]] = 1;
+    }
+    int cone_off = 0;
+    for (int l = 2; l <= D; l++) {
+      LevelInfo& L = t.lv[l];
+      if (l > 2) {
+        // cones needed by the parents (GetRelevantConeSegmentsDueToParents,
+        // BoxTree.h:299-428): protobox[q][cp] is filled from the first parent
+        // cone with non-relevant id q that has a child at position cp.
+        const LevelInfo& Pl = t.lv[l - 1];
+        int* proto = &t.proto[L.proto_off];
+        for (int g = Pl.cone_off; g < Pl.cone_off + Pl.ncone; g++) {
+          const int pb = t.cone_box[g];
+          const long long q = (long long)t.cone_post[g] * Pl.nrad + t.cone_posr[g];
+          for (int cp = 0; cp < 4; cp++) {
+            const int cb = t.box_child[(size_t)(Pl.box_off + pb) * 4 + cp];
+            if (cb < 0) continue;
+            int* slot = proto + ((size_t)q * 4 + cp) * P;
+            if (slot[0] < 0) {
+              for (int j = 0; j < P; j++) {
+                double px, py;
+                G[l - 1].cheb2cart(boxes[l - 1][pb], q, xs[j % Ps], xa[j / Ps], px, py);
+                const long long cq = G[l].nonrel(boxes[l][cb], px, py);
+                if (cq < 0) {
+                  err = "parent interpolation node outside the child cone grid";
+                  return false;
+                }
+                slot[j] = (int)cq;
+              }
+            }
+            for (int j = 0; j < P; j++) mark[L.map_off + (long long)cb * L.nslot + slot[j]] = 1;
+            t.n_prop += P;
+          }
+        }
+      }
+      L.cone_off = cone_off;
+      int nc = 0;
+      for (int b = 0; b < L.nbox; b++)
+        for (int q = 0; q < L.nslot; q++) {
+          const long long a = L.map_off + (long long)b * L.nslot + q;
+          if (!mark[a]) continue;
+          t.dense_map[a] = nc++;
+          t.cone_box.push_back(b);
+          t.cone_posr.push_back(q % L.nrad);
+          t.cone_post.push_back(q / L.nrad);
+          t.cone_level.push_back((unsigned char)l);
+        }
+      if (nc == 0 && !sep) {  // a field's coarse levels may have none (its targets are near every source)
+        err = "The number of cone segments on any level cannot be 0";
+        return false;
+      }
+      L.ncone = nc;
+      cone_off += nc;
+    }
+    t.n_cones = cone_off;
+    // final interpolation entries as global cone ids
+    t.it_cone.resize(nent);
+#pragma omp parallel for schedule(static)
+    for (long long i = 0; i < NT; i++) {
+      long long e = t.it_ptr[i];
+      for (int l = D; l >= 2; l--)
+        for (int c : cz[l][pbox[(size_t)i * (D + 1) + l]]) {
+          t.it_cone[e] = t.lv[l].cone_off + t.dense_map[t.lv[l].map_off + (long long)c * t.lv[l].nslot + ent_q[e]];
+          e++;
+        }
+    }
   }
-  t.n_cones = cone_off;
   for (int g = t.lv[D].cone_off; g < t.lv[D].cone_off + t.lv[D].ncone; g++)
     t.n_leaf += (long long)P * (t.leaf_begin[t.cone_box[g] + 1] - t.leaf_begin[t.cone_box[g]]);
-  // final interpolation entries as global cone ids
-  t.it_cone.resize(nent);
-#pragma omp parallel for schedule(static)
-  for (long long i = 0; i < NT; i++) {
-    long long e = t.it_ptr[i];
-    for (int l = D; l >= 2; l--)
-      for (int c : cz[l][pbox[(size_t)i * (D + 1) + l]]) {
-        t.it_cone[e] = t.lv[l].cone_off + t.dense_map[t.lv[l].map_off + (long long)c * t.lv[l].nslot + ent_q[e]];
-        e++;
-      }
-  }
   t.n_interp = nent;
   if (!sep) t.point_box.swap(pbox);
   return true;
@@ -748,8 +735,8 @@ bool build_exclusions(const Geometry& g, const Tree& t, std::vector<int>& patch_
   const int D = t.D;
   auto leaf_xy = [&](long long sorted, long long& bx, long long& by) {
     const long long m = t.box_morton[t.lv[D].box_off + t.leaf_of_point[sorted]];
-    bx = (long long)merge_from2((uint64_t)m);
-    by = (long long)merge_from2((uint64_t)m >> 1);
+    bx = (long long)morton_merge2((uint64_t)m);
+    by = (long long)morton_merge2((uint64_t)m >> 1);
   };
   for (long long i = 0; i < N; i++) {
     long long ax, ay;

@@ -113,7 +113,30 @@ struct Tree {
   std::vector<double> tx, ty;                  // sorted targets
   std::vector<int> tleaf, tnb_ptr, tnb_list;
   long long n_near_t = 0;                      // (target, source) pairs of the field's near field
+  // device cone builder: interpolation entries and protobox nodes within a
+  // relative 1e-9 of a cone boundary, decided on the host
+  long long near_boundary_entries = 0, near_boundary_nodes = 0;
 };
+// Inputs of the relevant-cone step (interpolation entries, protoboxes and the
+// numbering of the relevant cones) when build_tree hands it to a builder of
+// its own: the sorted targets, their group at every level, each group's
+// cousin boxes and the Chebyshev nodes.
+struct ConeInput {
+  const double* tx = nullptr;  // sorted targets
+  const double* ty = nullptr;
+  long long nt = 0;
+  const int* pbox = nullptr;      // nt x (D + 1): target group per level
+  std::vector<long long> grp_off;  // first global group id per level (D + 2 entries)
+  std::vector<int> cz_ptr, cz_list;  // cousin boxes (local ids, ascending) per global group
+  std::vector<double> xs, xa;        // Chebyshev nodes in s and theta
+};
+// A builder fills t.dense_map, t.proto, t.cone_box / posr / post / level,
+// lv[l].cone_off / ncone, t.n_cones, t.n_prop and t.it_cone exactly as the
+// host loop does (lv[l].map_off and proto_off are set before the call).
+using ConeBuilder = bool (*)(const ConeInput& in, Tree& t, std::string& err);
+// The builder on the GPU (cones.hip).
+bool relevant_cones_device(const ConeInput& in, Tree& t, std::string& err);
+
 // perm_in (optional): the Morton permutation computed elsewhere (the device
 // sort); it is checked to be a permutation that orders the leaf codes with
 // ties in input order, i.e. exactly the host's stable sort.
@@ -121,10 +144,11 @@ struct Tree {
 // box then covers sources and targets; relevant cones are those the targets'
 // interpolations and the parents need, exactly as for the forward map with
 // the targets in place of the sources.
+// cones (optional): builds the relevant cones in place of the host loop.
 bool build_tree(const double* x, const double* y, const double* nx, const double* ny, long long N,
                 int nlev, double k, int Ps, int Pang, Tree& t, std::string& err,
                 const std::vector<int>* perm_in = nullptr, const double* tx = nullptr,
-                const double* ty = nullptr, long long M = 0);
+                const double* ty = nullptr, long long M = 0, ConeBuilder cones = nullptr);
 
 // Near-field exclusion lists (init_sort_sing_point, ForwardMap.tpp:567-607) in
 // sorted order: own patch + patches the point is near-singular to, padded
