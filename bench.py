@@ -135,8 +135,9 @@ def main():
     st = fm.stats()
     log(f"plan: N={fm.N} setup {t_setup:.2f} s stats {st}")
 
-    rng = np.random.default_rng(0)
-    x_host = rng.standard_normal(fm.N) + 1j * rng.standard_normal(fm.N)
+    # seeded splitmix64 + Box-Muller density (configs.seeded_density; the oracle's
+    # C++ draws the same values)
+    x_host = configs.seeded_density(fm.N, seed=0)
     dev = torch.device("cuda:0")
     xs = torch.from_numpy(x_host).to(dev)
     ys = torch.empty_like(xs)
@@ -281,7 +282,7 @@ def main():
     batched = None
     if not args.no_batched:
         batched = {"nrhs": 64}
-        Xb = torch.from_numpy(rng.standard_normal((64, fm.N)) + 1j * rng.standard_normal((64, fm.N))).to(dev)
+        Xb = torch.from_numpy(configs.seeded_density(64 * fm.N, seed=1).reshape(64, fm.N)).to(dev)
         Yb = torch.empty_like(Xb)
         for prec in ("fp64", "mixed"):
             fb = I.ForwardMap(**dict(kw, precision=prec), nrhs_max=64, autotune=True)
@@ -316,7 +317,7 @@ def main():
     group = None
     if not args.no_group:
         maps = [I.ForwardMap(**kw_, autotune=True) for kw_ in configs.random_stars(16, k=100.0, seed=0)]
-        gx = [torch.from_numpy(rng.standard_normal(m.N) + 1j * rng.standard_normal(m.N)).to(dev) for m in maps]
+        gx = [torch.from_numpy(configs.seeded_density(m.N, seed=2 + j)).to(dev) for j, m in enumerate(maps)]
         gy = [torch.empty_like(v) for v in gx]
         torch.cuda.synchronize()
         reps = max(5, args.steps // 10)

@@ -56,3 +56,29 @@ def scaled(curve, npatches, k, nlevels, **kw):
 def plane_wave(x, y, k, d):
     """Incident field exp(i k d.x)."""
     return np.exp(1j * k * (d[0] * x + d[1] * y))
+
+
+_GOLDEN = np.uint64(0x9E3779B97F4A7C15)
+
+
+def splitmix64(seed, n, start=0):
+    """Counter-based splitmix64 (Vigna): value i = mix(seed + (start + i + 1) * golden
+    gamma), so any index range is reproducible on its own; the same generator runs in
+    the oracle's C++ (oracle_splitmix)."""
+    with np.errstate(over="ignore"):
+        z = np.uint64(seed) + (np.arange(start, start + n, dtype=np.uint64) + np.uint64(1)) * _GOLDEN
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return z ^ (z >> np.uint64(31))
+
+
+def seeded_density(n, seed=0, column=0):
+    """The benchmark's seeded complex density: entry i of column c takes the pair of
+    splitmix64 values (2m, 2m + 1), m = c * n + i, as uniforms u = (v >> 11 + 1) / 2^53
+    in (0, 1] and returns their Box-Muller normals r cos(2 pi u2) + i r sin(2 pi u2),
+    r = sqrt(-2 ln u1): real and imaginary parts iid N(0, 1)."""
+    v = splitmix64(seed, 2 * n, start=2 * column * n)
+    u = ((v >> np.uint64(11)) + np.uint64(1)).astype(np.float64) * (1.0 / 9007199254740992.0)
+    r = np.sqrt(-2.0 * np.log(u[0::2]))
+    a = 2.0 * np.pi * u[1::2]
+    return r * np.cos(a) + 1j * (r * np.sin(a))

@@ -1409,6 +1409,32 @@ extern "C" {
 
 const char* oracle_last_error() { return g_err.c_str(); }
 
+// The benchmark's counter-based generator: splitmix64 value i of a seed
+// (value i = mix(seed + (start + i + 1) * 0x9E3779B97F4A7C15)), and the
+// seeded complex density built from it by Box-Muller (entry i of column c
+// from values 2m, 2m + 1, m = c n + i; u = ((v >> 11) + 1) / 2^53).
+static unsigned long long splitmix_at(unsigned long long seed, unsigned long long i) {
+  unsigned long long z = seed + (i + 1) * 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+void oracle_splitmix(unsigned long long seed, long long n, long long start, unsigned long long* out) {
+  for (long long i = 0; i < n; i++) out[i] = splitmix_at(seed, (unsigned long long)(start + i));
+}
+
+void oracle_seeded_density(unsigned long long seed, long long n, int column, double* out) {
+  const unsigned long long base = 2ull * (unsigned long long)column * (unsigned long long)n;
+  for (long long i = 0; i < n; i++) {
+    const double u1 = (double)((splitmix_at(seed, base + 2 * i) >> 11) + 1) * (1.0 / 9007199254740992.0);
+    const double u2 = (double)((splitmix_at(seed, base + 2 * i + 1) >> 11) + 1) * (1.0 / 9007199254740992.0);
+    const double r = std::sqrt(-2.0 * std::log(u1)), a = 2.0 * M_PI * u2;
+    out[2 * i] = r * std::cos(a);
+    out[2 * i + 1] = r * std::sin(a);
+  }
+}
+
 void oracle_hankel(int n, const double* x, double* out, long long cnt) {
   for (ll i = 0; i < cnt; i++) {
     cd h = hankel1(n, x[i]);

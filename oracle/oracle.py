@@ -29,6 +29,9 @@ def lib():
         L = _LIB
         L.oracle_last_error.restype = ctypes.c_char_p
         L.oracle_hankel.argtypes = [ctypes.c_int, _dp, _dp, ctypes.c_longlong]
+        L.oracle_splitmix.argtypes = [ctypes.c_ulonglong, ctypes.c_longlong, ctypes.c_longlong,
+                                      ctypes.POINTER(ctypes.c_ulonglong)]
+        L.oracle_seeded_density.argtypes = [ctypes.c_ulonglong, ctypes.c_longlong, ctypes.c_int, _dp]
         L.oracle_fejer.argtypes = [ctypes.c_int, _dp, _dp]
         L.oracle_rp_cov.argtypes = [ctypes.c_int, _dp, ctypes.c_longlong, ctypes.c_double,
                                     ctypes.c_double, _dp, _dp]
@@ -251,4 +254,16 @@ def direct(x, y, nx, ny, dens, k, targets, formulation=SINGLE_LAYER, eta=0.0):
     out = np.empty(t.size, dtype=np.complex128)
     lib().oracle_direct(*[_d(a) for a in arrs], arrs[0].size, k, formulation, eta,
                         d.ctypes.data_as(_dp), _l(t), t.size, out.ctypes.data_as(_dp))
+    return out
+
+
+def splitmix64(seed, n, start=0):
+    out = np.empty(n, dtype=np.uint64)
+    lib().oracle_splitmix(seed, n, start, out.ctypes.data_as(ctypes.POINTER(ctypes.c_ulonglong)))
+    return out
+
+
+def seeded_density(n, seed=0, column=0):
+    out = np.empty(n, dtype=np.complex128)
+    lib().oracle_seeded_density(seed, n, column, out.ctypes.data_as(_dp))
     return out

@@ -250,3 +250,38 @@ def test_gmres_is_residual_minimising():
         true = np.linalg.norm(b - A @ x) / np.linalg.norm(b)
         assert abs(true - best[k - 1]) < 1e-10 * max(1.0, best[k - 1])
         assert abs(err - true) < 1e-10
+
+
+# ------------------------------------------------- benchmark input generator ---
+def test_seeded_density_generator_matches_fixture():
+    """The benchmark's counter-based generator (splitmix64 + Box-Muller), in the
+    oracle's C++ and in ifgf2d_amd.configs, against the fixture written from an
+    independent Python-integer / mpmath restatement, and splitmix64's published
+    first output for seed 0 (0xE220A8397B1DCDAF)."""
+    import json
+    import os
+
+    from ifgf2d_amd import configs
+
+    with open(os.path.join(os.path.dirname(__file__), "golden", "seeded_density.json")) as f:
+        g = json.load(f)
+    assert int(g["splitmix64"]["0"][0]) == 0xE220A8397B1DCDAF
+    for s, vals in g["splitmix64"].items():
+        want = np.array([int(v) for v in vals], dtype=np.uint64)
+        assert np.array_equal(O.splitmix64(int(s), 16), want)
+        assert np.array_equal(configs.splitmix64(int(s), 16), want)
+        assert np.array_equal(configs.splitmix64(int(s), 10, start=6), want[6:])
+    # FP64 evaluation of the 30-digit values: the angle 2 pi u2 carries ~eps * 2 pi,
+    # times the radius
+    want = np.array([complex(a, b) for a, b in g["density_seed0"]])
+    for got in (O.seeded_density(8), configs.seeded_density(8)):
+        assert np.all(np.abs(got - want) <= 2e-15 * np.maximum(1.0, np.abs(want)))
+    # C++ and numpy agree to the math libraries' last bits; column c of an n-entry
+    # block continues the stream at entry c n
+    a, b = O.seeded_density(5000, seed=3, column=2), configs.seeded_density(5000, seed=3, column=2)
+    assert np.all(np.abs(a - b) <= 2e-15 * np.maximum(1.0, np.abs(a)))
+    assert np.array_equal(configs.seeded_density(10, column=1), configs.seeded_density(20)[10:])
+    z = configs.seeded_density(200000, seed=1)
+    assert abs(z.real.mean()) < 0.01 and abs(z.imag.mean()) < 0.01
+    assert abs(z.real.var() - 1) < 0.02 and abs(z.imag.var() - 1) < 0.02
+    assert abs(np.mean(z.real * z.imag)) < 0.01
