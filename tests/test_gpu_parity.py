@@ -275,6 +275,10 @@ def test_device_gmres_circle_mie():
     dens, it, relres, hist = fm.solve(rhs, restart=50, tol=1e-8)
     print("GMRES iterations", it, "true relres", relres)
     assert relres < 5e-8 and hist[-1] < 1e-8
+    # the device reductions (and the second Gram-Schmidt pass decisions taken
+    # from them) are deterministic
+    dens2, it2, relres2, hist2 = fm.solve(rhs, restart=50, tol=1e-8)
+    assert it2 == it and np.array_equal(dens2, dens) and np.array_equal(hist2, hist)
     th = 2 * np.pi * np.arange(64) / 64
     us = fm.propagate_unacc(3 * np.cos(th), 3 * np.sin(th), dens)
     err = rel(us, mie(3.0, th, kw["k"]))
