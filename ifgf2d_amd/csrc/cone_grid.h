@@ -31,6 +31,21 @@ IFGF_HD inline uint64_t morton_merge2(uint64_t x) {
   return x;
 }
 
+// Morton interleave of box (px, py) at a level (Level.h:172-237): x in the
+// low bit, level + 1 bits per coordinate.
+IFGF_HD inline uint64_t morton_split2(uint64_t v, int level) {
+  uint64_t x = v & ((uint64_t(1) << (level + 1)) - 1);
+  x = (x | (x << 16)) & 0x0000FFFF0000FFFFull;
+  x = (x | (x << 8)) & 0x00FF00FF00FF00FFull;
+  x = (x | (x << 4)) & 0x0F0F0F0F0F0F0F0Full;
+  x = (x | (x << 2)) & 0x3333333333333333ull;
+  x = (x | (x << 1)) & 0x5555555555555555ull;
+  return x;
+}
+IFGF_HD inline long long morton_code(long long px, long long py, int level) {
+  return (long long)(morton_split2((uint64_t)px, level) | (morton_split2((uint64_t)py, level) << 1));
+}
+
 struct ConeGrid {
   double min_x, min_y, bs, h;  // bounding-box corner, box size, h = sqrt(2)/2 bs
   int nce, nrad;               // angular half-count (2 nce + 1 directions), radial intervals

@@ -1,7 +1,21 @@
-// Relevant cone segments on the GPU: the cone step of the tree setup
-// (BoxTree::InitializeRelevantConeSegments, IFGF_Source/BoxTree.h:553-583,
-// with GetRelevantConeSegmentsDueToParents, :299-428, the merge of :430-459
-// and the flattening of :461-514), producing the host loop's tables.
+// Tree setup on the GPU: the grouping step (each target's box at every level,
+// Level::Point2Morton, IFGF_Source/Level.h:145-161; the cousin lists of
+// Level::GetCousins, :261-289, and the neighbourhoods of GetNeighbours,
+// :378-404; the interpolation entry counts) and the cone step
+// (BoxTree::InitializeRelevantConeSegments, BoxTree.h:553-583, with
+// GetRelevantConeSegmentsDueToParents, :299-428, the merge of :430-459 and
+// the flattening of :461-514), producing the host loops' tables.
+//
+//   k_target_boxes    one thread per target: its box code at every level and,
+//                     for the forward map, the box's index (binary search in
+//                     the level's ascending box codes)
+//   k_group_runs      one workgroup per level of a field: the runs of equal
+//                     codes among the sorted targets are its groups
+//   k_window          per group: the boxes of its cousin window (6 x 6 less
+//                     the 3 x 3 neighbourhood) or of its 3 x 3 neighbourhood
+//                     that exist, counted, then listed in ascending order
+//   k_scan            one workgroup: exclusive prefix sums with a carry
+//   k_entry_counts    per target: its interpolation entries over the levels
 //
 //   k_interp_entries  one thread per target: the non-relevant cone of each of
 //                     its (level, cousin box) interpolation entries, marked
@@ -234,6 +248,180 @@ __global__ void __launch_bounds__(kThreads)
   it_cone[e] = lv[l].cone_off + dense_map[a];
 }
 
+// ------------------------------------------------------------- grouping ---
+struct GLevel {
+  double min_x, min_y, bs;
+  long long nbps;
+  long long grp_off;  // first global group id of the level
+  int box_off, nbox;  // the level's boxes in the global box arrays
+  int ngrp, pad;
+};
+
+// index of code in the ascending codes c[0..n), or -1
+__device__ inline int find_code(const long long* __restrict__ c, int n, long long code) {
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (c[mid] < code)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo < n && c[lo] == code ? lo : -1;
+}
+
+// flag bits: 1 a target outside the level's box grid (field), 2 a point
+// outside the relevant boxes of its level (forward map)
+__global__ void __launch_bounds__(kThreads)
+    k_target_boxes(long long nt, int D, int sep, const double* __restrict__ tx, const double* __restrict__ ty,
+                   const GLevel* __restrict__ glv, const long long* __restrict__ box_morton,
+                   long long* __restrict__ codes, int* __restrict__ pbox, int* __restrict__ flag) {
+  const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (i >= nt) return;
+  const double x = tx[i], y = ty[i];
+  pbox[i * (D + 1)] = -1;
+  pbox[i * (D + 1) + 1] = -1;
+  for (int l = 2; l <= D; l++) {
+    const GLevel L = glv[l];
+    const long long px = (long long)((x - L.min_x) / L.bs), py = (long long)((y - L.min_y) / L.bs);
+    const bool ok = !(px < 0 || py < 0 || px >= L.nbps || py >= L.nbps);
+    const long long m = ok ? morton_code(px, py, l) : -1;
+    codes[(size_t)l * nt + i] = m;
+    if (sep) {
+      if (!ok) atomicOr(flag, 1);
+      continue;
+    }
+    const int b = ok ? find_code(box_morton + L.box_off, L.nbox, m) : -1;
+    if (b < 0) atomicOr(flag, 2);
+    pbox[i * (D + 1) + l] = b;
+  }
+}
+
+// Field level l: group of sorted target i = number of code changes before
+// it; gcode[g] = the group's code; count = groups.
+__global__ void __launch_bounds__(kNumberThreads)
+    k_group_runs(long long nt, int D, int l, const long long* __restrict__ code, int* __restrict__ pbox,
+                 long long* __restrict__ gcode, int* __restrict__ count) {
+  constexpr int kW = kNumberThreads / 64;
+  __shared__ int wsum[kW];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int carry = 0;
+  for (long long base = 0; base < nt; base += kNumberThreads) {
+    const long long i = base + threadIdx.x;
+    const int f = i < nt ? (i == 0 || code[i] != code[i - 1]) : 0;
+    const unsigned long long bal = __ballot(f);
+    const int pre = __popcll(bal & ((2ull << lane) - 1ull));  // inclusive
+    if (lane == 0) wsum[w] = __popcll(bal);
+    __syncthreads();
+    int woff = 0, tot = 0;
+    for (int k = 0; k < kW; k++) {
+      woff += k < w ? wsum[k] : 0;
+      tot += wsum[k];
+    }
+    if (i < nt) {
+      const int g = carry + woff + pre - 1;
+      pbox[i * (D + 1) + l] = g;
+      if (f) gcode[g] = code[i];
+    }
+    carry += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *count = carry;
+}
+
+// Groups [g0, g0 + n) of one level, codes gcode: the existing boxes of the
+// cousin window (cousins = 1) or of the 3 x 3 neighbourhood (cousins = 0).
+// out == nullptr: cnt[g] = their number; else out[ptr[g] ..] = their local
+// box indices, ascending (ascending codes).
+__global__ void __launch_bounds__(kThreads)
+    k_window(int n, int l, int cousins, const long long* __restrict__ gcode, const GLevel* __restrict__ glv,
+             const long long* __restrict__ box_morton, int* __restrict__ cnt, const int* __restrict__ ptr,
+             int* __restrict__ out) {
+  const int g = blockIdx.x * kThreads + threadIdx.x;
+  if (g >= n) return;
+  const GLevel L = glv[l];
+  const long long* codes = box_morton + L.box_off;
+  const long long m = gcode[g];
+  const long long bx = (long long)morton_merge2((uint64_t)m), by = (long long)morton_merge2((uint64_t)m >> 1);
+  long long xlo, xup, ylo, yup;
+  if (cousins) {
+    xlo = (bx / 2) * 2 - 2 > 0 ? (bx / 2) * 2 - 2 : 0;
+    xup = (bx / 2) * 2 + 3 < L.nbps - 1 ? (bx / 2) * 2 + 3 : L.nbps - 1;
+    ylo = (by / 2) * 2 - 2 > 0 ? (by / 2) * 2 - 2 : 0;
+    yup = (by / 2) * 2 + 3 < L.nbps - 1 ? (by / 2) * 2 + 3 : L.nbps - 1;
+  } else {
+    xlo = bx - 1 > 0 ? bx - 1 : 0;
+    xup = bx + 1 < L.nbps - 1 ? bx + 1 : L.nbps - 1;
+    ylo = by - 1 > 0 ? by - 1 : 0;
+    yup = by + 1 < L.nbps - 1 ? by + 1 : L.nbps - 1;
+  }
+  int found[36];
+  int k = 0;
+  for (long long j = ylo; j <= yup; j++)
+    for (long long i = xlo; i <= xup; i++) {
+      if (cousins && (i - bx <= 1 && bx - i <= 1) && (j - by <= 1 && by - j <= 1)) continue;
+      const int b = find_code(codes, L.nbox, morton_code(i, j, l));
+      if (b >= 0 && k < 36) found[k++] = b;
+    }
+  if (!out) {
+    cnt[g] = k;
+    return;
+  }
+  for (int a = 1; a < k; a++) {  // insertion sort, at most 27 entries
+    const int v = found[a];
+    int b = a - 1;
+    while (b >= 0 && found[b] > v) {
+      found[b + 1] = found[b];
+      b--;
+    }
+    found[b + 1] = v;
+  }
+  int* o = out + ptr[g];
+  for (int a = 0; a < k; a++) o[a] = found[a];
+}
+
+// out[i] = sum of in[0..i) (exclusive), out[n] = the total; one workgroup.
+template <typename T>
+__global__ void __launch_bounds__(kNumberThreads) k_scan(long long n, const int* __restrict__ in, T* __restrict__ out) {
+  constexpr int kW = kNumberThreads / 64;
+  __shared__ T wsum[kW];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  T carry = 0;
+  for (long long base = 0; base < n; base += kNumberThreads) {
+    const long long i = base + threadIdx.x;
+    const T v = i < n ? (T)in[i] : (T)0;
+    T x = v;  // inclusive scan over the wave
+    for (int o = 1; o < 64; o <<= 1) {
+      const T y = __shfl_up(x, o, 64);
+      if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    T woff = 0, tot = 0;
+    for (int k = 0; k < kW; k++) {
+      woff += k < w ? wsum[k] : (T)0;
+      tot += wsum[k];
+    }
+    if (i < n) out[i] = carry + woff + x - v;
+    carry += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[n] = carry;
+}
+
+__global__ void __launch_bounds__(kThreads)
+    k_entry_counts(long long nt, int D, const int* __restrict__ pbox, const GLevel* __restrict__ glv,
+                   const int* __restrict__ cz_ptr, int* __restrict__ cnt) {
+  const long long i = (long long)blockIdx.x * kThreads + threadIdx.x;
+  if (i >= nt) return;
+  int c = 0;
+  for (int l = 2; l <= D; l++) {
+    const long long g = glv[l].grp_off + pbox[i * (D + 1) + l];
+    c += cz_ptr[g + 1] - cz_ptr[g];
+  }
+  cnt[i] = c;
+}
+
 template <typename T>
 void download(std::vector<T>& v, const DevBuf<T>& b, size_t n) {
   v.resize(n);
@@ -444,6 +632,123 @@ bool relevant_cones_device(const ConeInput& in, Tree& t, std::string& err) {
   t.cone_level.resize(cone_off);
   for (int l = 2; l <= D; l++)
     for (int g = t.lv[l].cone_off; g < t.lv[l].cone_off + t.lv[l].ncone; g++) t.cone_level[g] = (unsigned char)l;
+  return true;
+}
+
+}  // namespace ifgf
+
+namespace ifgf {
+
+bool tree_groups_device(const GroupJob& job, Tree& t, std::vector<int>& pbox, ConeInput& cin, std::string& err) {
+  const int D = t.D;
+  const long long nt = job.nt;
+  std::vector<GLevel> gh(D + 1);
+  for (int l = 2; l <= D; l++) {
+    const LevelInfo& L = t.lv[l];
+    gh[l] = GLevel{L.min_x, L.min_y, L.bs, job.nbps[l], 0, L.box_off, L.nbox, 0, 0};
+  }
+  DevBuf<GLevel> glv;
+  DevBuf<double> tx, ty;
+  DevBuf<long long> box_morton, codes, gcode, it_ptr;
+  DevBuf<int> pb, cnt, cz_ptr, cz_list, nb_ptr, nb_list, scal;
+  glv.upload(gh);
+  tx.upload(std::vector<double>(job.tx, job.tx + nt));
+  ty.upload(std::vector<double>(job.ty, job.ty + nt));
+  box_morton.upload(t.box_morton);
+  codes.alloc((size_t)(D + 1) * nt);
+  pb.alloc((size_t)(D + 1) * nt);
+  scal.alloc(1 + (size_t)D + 1);  // [0] error flags, [1 + l] groups of field level l
+  HIP_TRY(hipMemset(scal.p, 0, scal.n * sizeof(int)));
+  hipLaunchKernelGGL(k_target_boxes, dim3(grid_for(nt)), dim3(kThreads), 0, 0, nt, D, job.sep ? 1 : 0, tx.p, ty.p,
+                     glv.p, box_morton.p, codes.p, pb.p, scal.p);
+  HIP_TRY(hipGetLastError());
+  const int fl = read_one(scal.p);
+  if (fl & 1) {
+    err = "The 2D box index cannot be < 0 or larger than the number of boxes";
+    return false;
+  }
+  if (fl & 2) {
+    err = "point outside the relevant boxes of a level";
+    return false;
+  }
+  // groups: the forward map's are its boxes (global box ids), a field's the
+  // runs of equal codes of its sorted targets
+  long long ngrp = 0;
+  if (job.sep) gcode.alloc((size_t)(D + 1) * nt);  // at most nt groups per level
+  for (int l = 2; l <= D; l++) {
+    gh[l].grp_off = ngrp;
+    if (job.sep) {
+      hipLaunchKernelGGL(k_group_runs, dim3(1), dim3(kNumberThreads), 0, 0, nt, D, l, codes.p + (size_t)l * nt, pb.p,
+                         gcode.p + ngrp, scal.p + 1 + l);
+      HIP_TRY(hipGetLastError());
+      gh[l].ngrp = read_one(scal.p + 1 + l);
+    } else {
+      gh[l].ngrp = gh[l].nbox;
+    }
+    ngrp += gh[l].ngrp;
+  }
+  glv.upload(gh);
+  const long long* grp_code = job.sep ? gcode.p : box_morton.p;
+  // cousin lists of every group, level by level
+  cnt.alloc(std::max<long long>(ngrp, nt) + 1);
+  for (int l = 2; l <= D; l++)
+    if (gh[l].ngrp)
+      hipLaunchKernelGGL(k_window, dim3(grid_for(gh[l].ngrp)), dim3(kThreads), 0, 0, gh[l].ngrp, l, 1,
+                         grp_code + gh[l].grp_off, glv.p, box_morton.p, cnt.p + gh[l].grp_off, (const int*)nullptr,
+                         (int*)nullptr);
+  DevBuf<long long> tot;
+  tot.alloc(ngrp + 1);
+  hipLaunchKernelGGL(k_scan<long long>, dim3(1), dim3(kNumberThreads), 0, 0, ngrp, cnt.p, tot.p);
+  HIP_TRY(hipGetLastError());
+  const long long ncz = read_one(tot.p + ngrp);
+  if (ncz > 2147483647LL) {
+    err = "too many cousin entries";
+    return false;
+  }
+  cz_ptr.alloc(ngrp + 1);
+  hipLaunchKernelGGL(k_scan<int>, dim3(1), dim3(kNumberThreads), 0, 0, ngrp, cnt.p, cz_ptr.p);
+  cz_list.alloc(ncz);
+  for (int l = 2; l <= D; l++)
+    if (gh[l].ngrp)
+      hipLaunchKernelGGL(k_window, dim3(grid_for(gh[l].ngrp)), dim3(kThreads), 0, 0, gh[l].ngrp, l, 1,
+                         grp_code + gh[l].grp_off, glv.p, box_morton.p, (int*)nullptr, cz_ptr.p + gh[l].grp_off,
+                         cz_list.p);
+  HIP_TRY(hipGetLastError());
+  // leaf neighbourhoods: of every source leaf box, and of a field's leaf groups
+  auto neighbourhoods = [&](const long long* gc, int n, std::vector<int>& ptr, std::vector<int>& list) {
+    hipLaunchKernelGGL(k_window, dim3(grid_for(n)), dim3(kThreads), 0, 0, n, D, 0, gc, glv.p, box_morton.p, cnt.p,
+                       (const int*)nullptr, (int*)nullptr);
+    nb_ptr.alloc((size_t)n + 1);
+    hipLaunchKernelGGL(k_scan<int>, dim3(1), dim3(kNumberThreads), 0, 0, (long long)n, cnt.p, nb_ptr.p);
+    HIP_TRY(hipGetLastError());
+    const int nn = read_one(nb_ptr.p + n);
+    nb_list.alloc(nn);
+    hipLaunchKernelGGL(k_window, dim3(grid_for(n)), dim3(kThreads), 0, 0, n, D, 0, gc, glv.p, box_morton.p,
+                       (int*)nullptr, nb_ptr.p, nb_list.p);
+    HIP_TRY(hipGetLastError());
+    download(ptr, nb_ptr, (size_t)n + 1);
+    download(list, nb_list, (size_t)nn);
+  };
+  neighbourhoods(box_morton.p + gh[D].box_off, gh[D].nbox, t.nb_ptr, t.nb_list);
+  if (job.sep) neighbourhoods(gcode.p + gh[D].grp_off, gh[D].ngrp, t.tnb_ptr, t.tnb_list);
+  // interpolation entries per target
+  if (nt)
+    hipLaunchKernelGGL(k_entry_counts, dim3(grid_for(nt)), dim3(kThreads), 0, 0, nt, D, pb.p, glv.p, cz_ptr.p,
+                       cnt.p);
+  it_ptr.alloc(nt + 1);
+  hipLaunchKernelGGL(k_scan<long long>, dim3(1), dim3(kNumberThreads), 0, 0, nt, cnt.p, it_ptr.p);
+  HIP_TRY(hipGetLastError());
+  download(t.it_ptr, it_ptr, (size_t)nt + 1);
+  download(pbox, pb, (size_t)(D + 1) * nt);
+  download(cin.cz_ptr, cz_ptr, (size_t)ngrp + 1);
+  download(cin.cz_list, cz_list, (size_t)ncz);
+  cin.grp_off.assign(D + 2, 0);
+  for (int l = 2; l <= D; l++) cin.grp_off[l] = gh[l].grp_off;
+  cin.grp_off[D + 1] = ngrp;
+  if (job.sep) {
+    t.tleaf.resize(nt);
+    for (long long i = 0; i < nt; i++) t.tleaf[i] = pbox[(size_t)i * (D + 1) + D];
+  }
   return true;
 }
 

@@ -183,7 +183,8 @@ int ifgf_plan_create(const ifgf_surface* s, const ifgf_params* prm, ifgf_plan** 
     const bool dev_tree = (prm->flags & IFGF_PLAN_DEVICE_TREE) && !(prm->flags & IFGF_PLAN_HOST_ONLY);
     if (!build_tree(P->geo.x.data(), P->geo.y.data(), P->geo.nx.data(), P->geo.ny.data(), P->geo.N,
                     prm->nlevels, prm->k, prm->Ps, prm->Pang, P->tree, err, dperm.empty() ? nullptr : &dperm,
-                    nullptr, nullptr, 0, dev_tree ? relevant_cones_device : nullptr))
+                    nullptr, nullptr, 0, dev_tree ? relevant_cones_device : nullptr,
+                    dev_tree ? tree_groups_device : nullptr))
       return fail(nullptr, IFGF_ERR_GEOMETRY, err);
     lap(2);
     if (!build_exclusions(P->geo, P->tree, P->patch_sorted, P->excl, P->n_near, err))
@@ -1472,7 +1473,8 @@ int ifgf_field_create(const ifgf_plan* src, const double* tx, const double* ty, 
     const Geometry& g = P->geo;
     const bool dev_tree = (P->prm.flags & IFGF_PLAN_DEVICE_TREE) && src->on_device;
     if (!build_tree(g.x.data(), g.y.data(), g.nx.data(), g.ny.data(), g.N, nlevels, P->prm.k, Ps, Pang, P->tree, err,
-                    nullptr, tx, ty, nt, dev_tree ? relevant_cones_device : nullptr))
+                    nullptr, tx, ty, nt, dev_tree ? relevant_cones_device : nullptr,
+                    dev_tree ? tree_groups_device : nullptr))
       return fail(nullptr, IFGF_ERR_GEOMETRY, err);
     P->n_near = P->tree.n_near_t;
     build_cone_tables(P);

@@ -219,19 +219,8 @@ bool build_correction(const Geometry& g, const Curve& c, int F, double k, double
 }
 
 // ------------------------------------------------------------------ tree ---
-// Morton interleave, IFGF_Source/Level.h:172-237.
-static uint64_t split_by2(uint64_t v, int level) {
-  uint64_t x = v & ((uint64_t(1) << (level + 1)) - 1);
-  x = (x | (x << 16)) & 0xFFFF0000FFFFull;
-  x = (x | (x << 8)) & 0xFF00FF00FF00FFull;
-  x = (x | (x << 4)) & 0xF0F0F0F0F0F0F0Full;
-  x = (x | (x << 2)) & 0x3333333333333333ull;
-  x = (x | (x << 1)) & 0x5555555555555555ull;
-  return x;
-}
-static long long morton(long long px, long long py, int level) {
-  return (long long)(split_by2((uint64_t)px, level) | (split_by2((uint64_t)py, level) << 1));
-}
+static long long morton(long long px, long long py, int level) { return morton_code(px, py, level); }
+
 namespace {
 struct LevelGeom {
   double min_x, min_y, bs, h;
@@ -272,7 +261,7 @@ struct LevelGeom {
 
 bool build_tree(const double* x, const double* y, const double* nx, const double* ny, long long N,
                 int nlev, double k, int Ps, int Pang, Tree& t, std::string& err, const std::vector<int>* perm_in,
-                const double* tx, const double* ty, long long M, ConeBuilder cones) {
+                const double* tx, const double* ty, long long M, ConeBuilder cones, GroupBuilder groups) {
   if (nlev < 3) {
     err = "IFGF accelerator requires at least 3 levels.";
     return false;
@@ -476,9 +465,13 @@ bool build_tree(const double* x, const double* y, const double* nx, const double
     out.clear();
     for (auto& e : c) out.push_back(e.second);
   };
+  if (groups && !cones) {
+    err = "the grouping step on the device needs the device cone builder";
+    return false;
+  }
   // leaf neighbours of every source leaf box
-  t.nb_ptr.assign(1, 0);
-  {
+  if (!groups) {
+    t.nb_ptr.assign(1, 0);
     std::vector<int> nb;
     for (size_t b = 0; b < boxes[D].size(); b++) {
       neighbours_of(D, boxes[D][b], nb);
@@ -495,9 +488,25 @@ bool build_tree(const double* x, const double* y, const double* nx, const double
   const double* TX = sep ? t.tx.data() : t.sx.data();
   const double* TY = sep ? t.ty.data() : t.sy.data();
   std::vector<std::vector<std::vector<int>>> cz(D + 1);
-  std::vector<int> pbox((size_t)NT * (D + 1), -1);
+  std::vector<int> pbox;
+  ConeInput cin;
   bool bad = false;
-  if (!sep) {
+  if (groups) {
+    GroupJob job;
+    job.tx = TX;
+    job.ty = TY;
+    job.nt = NT;
+    job.sep = sep;
+    job.nbps.assign(D + 1, 0);
+    for (int l = 0; l <= D; l++) job.nbps[l] = G[l].nbps;
+    if (!groups(job, t, pbox, cin, err)) return false;
+    for (long long i = 0; sep && i < M; i++) {
+      const int g = t.tleaf[i];
+      for (int n = t.tnb_ptr[g]; n < t.tnb_ptr[g + 1]; n++)
+        t.n_near_t += t.leaf_begin[t.tnb_list[n] + 1] - t.leaf_begin[t.tnb_list[n]];
+    }
+  } else if (!sep) {
+    pbox.assign((size_t)NT * (D + 1), -1);
     for (int l = 2; l <= D; l++) {
       cz[l].resize(boxes[l].size());
       for (size_t b = 0; b < boxes[l].size(); b++) cousins_of(l, boxes[l][b], cz[l][b]);
@@ -520,6 +529,7 @@ bool build_tree(const double* x, const double* y, const double* nx, const double
       return false;
     }
   } else {
+    pbox.assign((size_t)NT * (D + 1), -1);
     std::vector<long long> gcode;
     for (int l = 2; l <= D; l++) {
       gcode.clear();
@@ -556,11 +566,13 @@ bool build_tree(const double* x, const double* y, const double* nx, const double
   }
   // interpolation entries: per target, levels D..2, cousins ascending
   // (Interpolation, BoxTree.h:876-937); first the non-relevant cone ids.
-  t.it_ptr.assign(NT + 1, 0);
-  for (long long i = 0; i < NT; i++) {
-    long long c = 0;
-    for (int l = D; l >= 2; l--) c += (long long)cz[l][pbox[(size_t)i * (D + 1) + l]].size();
-    t.it_ptr[i + 1] = t.it_ptr[i] + c;
+  if (!groups) {
+    t.it_ptr.assign(NT + 1, 0);
+    for (long long i = 0; i < NT; i++) {
+      long long c = 0;
+      for (int l = D; l >= 2; l--) c += (long long)cz[l][pbox[(size_t)i * (D + 1) + l]].size();
+      t.it_ptr[i + 1] = t.it_ptr[i] + c;
+    }
   }
   const long long nent = t.it_ptr[NT];
   if (nent > 2147483647LL) {
@@ -582,34 +594,35 @@ bool build_tree(const double* x, const double* y, const double* nx, const double
   for (int i = 0; i < Ps; i++) xs[Ps - 1 - i] = std::cos(M_PI * (2.0 * i + 1.0) / (2.0 * Ps));
   for (int i = 0; i < Pang; i++) xa[Pang - 1 - i] = std::cos(M_PI * (2.0 * i + 1.0) / (2.0 * Pang));
   if (cones) {
-    ConeInput in;
-    in.tx = TX;
-    in.ty = TY;
-    in.nt = NT;
-    in.pbox = pbox.data();
-    in.xs = xs;
-    in.xa = xa;
-    in.grp_off.assign(D + 2, 0);
-    long long ngrp = 0, ncz = 0;
-    for (int l = 2; l <= D; l++) {
-      in.grp_off[l] = ngrp;
-      ngrp += (long long)cz[l].size();
-      for (const auto& v : cz[l]) ncz += (long long)v.size();
-    }
-    in.grp_off[D + 1] = ngrp;
-    if (ncz > 2147483647LL) {
-      err = "too many cousin entries";
-      return false;
-    }
-    in.cz_ptr.reserve(ngrp + 1);
-    in.cz_list.reserve(ncz);
-    in.cz_ptr.push_back(0);
-    for (int l = 2; l <= D; l++)
-      for (const auto& v : cz[l]) {
-        in.cz_list.insert(in.cz_list.end(), v.begin(), v.end());
-        in.cz_ptr.push_back((int)in.cz_list.size());
+    cin.tx = TX;
+    cin.ty = TY;
+    cin.nt = NT;
+    cin.pbox = pbox.data();
+    cin.xs = xs;
+    cin.xa = xa;
+    if (!groups) {
+      cin.grp_off.assign(D + 2, 0);
+      long long ngrp = 0, ncz = 0;
+      for (int l = 2; l <= D; l++) {
+        cin.grp_off[l] = ngrp;
+        ngrp += (long long)cz[l].size();
+        for (const auto& v : cz[l]) ncz += (long long)v.size();
       }
-    if (!cones(in, t, err)) return false;
+      cin.grp_off[D + 1] = ngrp;
+      if (ncz > 2147483647LL) {
+        err = "too many cousin entries";
+        return false;
+      }
+      cin.cz_ptr.reserve(ngrp + 1);
+      cin.cz_list.reserve(ncz);
+      cin.cz_ptr.push_back(0);
+      for (int l = 2; l <= D; l++)
+        for (const auto& v : cz[l]) {
+          cin.cz_list.insert(cin.cz_list.end(), v.begin(), v.end());
+          cin.cz_ptr.push_back((int)cin.cz_list.size());
+        }
+    }
+    if (!cones(cin, t, err)) return false;
   } else {
     std::vector<int> ent_q(nent);
 #pragma omp parallel for schedule(dynamic, 256) reduction(|| : bad)

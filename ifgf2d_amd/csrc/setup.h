@@ -137,6 +137,25 @@ using ConeBuilder = bool (*)(const ConeInput& in, Tree& t, std::string& err);
 // The builder on the GPU (cones.hip).
 bool relevant_cones_device(const ConeInput& in, Tree& t, std::string& err);
 
+// Inputs of the grouping step (each target's box at every level, the target
+// groups, their cousin boxes, the leaf neighbourhoods and the interpolation
+// entry counts) when build_tree hands it to a builder of its own.
+struct GroupJob {
+  const double* tx = nullptr;  // sorted targets
+  const double* ty = nullptr;
+  long long nt = 0;
+  bool sep = false;             // separate targets (a field tree)
+  std::vector<long long> nbps;  // boxes per side, per level
+};
+// A builder fills pbox (nt x (D + 1), -1 below level 2), the cousin CSR of
+// cin (grp_off, cz_ptr, cz_list), t.it_ptr and the leaf neighbourhoods
+// (t.nb_ptr / nb_list of the source leaf boxes for the forward map;
+// t.tnb_ptr / tnb_list and t.tleaf for a field) exactly as the host loops do.
+using GroupBuilder = bool (*)(const GroupJob& job, Tree& t, std::vector<int>& pbox, ConeInput& cin,
+                              std::string& err);
+// The builder on the GPU (cones.hip).
+bool tree_groups_device(const GroupJob& job, Tree& t, std::vector<int>& pbox, ConeInput& cin, std::string& err);
+
 // perm_in (optional): the Morton permutation computed elsewhere (the device
 // sort); it is checked to be a permutation that orders the leaf codes with
 // ties in input order, i.e. exactly the host's stable sort.
@@ -145,10 +164,13 @@ bool relevant_cones_device(const ConeInput& in, Tree& t, std::string& err);
 // interpolations and the parents need, exactly as for the forward map with
 // the targets in place of the sources.
 // cones (optional): builds the relevant cones in place of the host loop.
+// groups (optional, with cones): builds the target groups, cousin and
+// neighbour lists and entry counts in place of the host loops.
 bool build_tree(const double* x, const double* y, const double* nx, const double* ny, long long N,
                 int nlev, double k, int Ps, int Pang, Tree& t, std::string& err,
                 const std::vector<int>* perm_in = nullptr, const double* tx = nullptr,
-                const double* ty = nullptr, long long M = 0, ConeBuilder cones = nullptr);
+                const double* ty = nullptr, long long M = 0, ConeBuilder cones = nullptr,
+                GroupBuilder groups = nullptr);
 
 // Near-field exclusion lists (init_sort_sing_point, ForwardMap.tpp:567-607) in
 // sorted order: own patch + patches the point is near-singular to, padded

@@ -66,9 +66,11 @@ extern "C" {
 #define IFGF_PLAN_DEVICE_GEOMETRY 64 /* the near-singular search of the discretisation
                                         (ForwardMap.tpp:224-285) on the GPU, one thread per
                                         candidate point, instead of on the host */
-#define IFGF_PLAN_DEVICE_TREE 128 /* the relevant cone segments of the tree (interpolation entries,
-                                     protoboxes, numbering; BoxTree.h:299-583) on the GPU instead
-                                     of the host; same tables as the host's loop */
+#define IFGF_PLAN_DEVICE_TREE 128 /* the tree's target groups, cousin and neighbour lists
+                                     (Level.h:145-161, 261-289, 378-404) and relevant cone segments
+                                     (interpolation entries, protoboxes, numbering;
+                                     BoxTree.h:299-583) on the GPU instead of the host; same tables
+                                     as the host's loops */
 
 /* apply flags */
 #define IFGF_PTR_DEVICE 1     /* x / y are device pointers; the call is asynchronous on stream */

@@ -1,6 +1,9 @@
-"""Relevant cone segments built on the GPU (IFGF_PLAN_DEVICE_TREE) against the host's.
+"""Tree tables built on the GPU (IFGF_PLAN_DEVICE_TREE) against the host's.
 
-The cone step of the tree setup (BoxTree::InitializeRelevantConeSegments,
+The grouping step (every target's box per level, Level::Point2Morton; the
+cousin lists, Level::GetCousins, and neighbourhoods, GetNeighbours; the
+interpolation entry counts) and the cone step of the tree setup
+(BoxTree::InitializeRelevantConeSegments,
 BoxTree.h:553-583, with the protoboxes of GetRelevantConeSegmentsDueToParents,
 :299-428) decides which cones every level keeps, in which order, which cone
 every interpolation entry reads and which child cone every parent node is
