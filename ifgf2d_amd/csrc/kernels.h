@@ -772,6 +772,10 @@ __global__ void __launch_bounds__(256)
 // interpolation entries (stored path: slots[it_slot[f]] for f in
 // [it_ptr[si], it_ptr[si+1]), one per (level, cousin cone), in that order);
 // lane 0 adds the identity.
+// stage 0: all of it. A split in two launches with the same bits: stage 1
+// (beside the level pushes, once the near field is done) writes
+// pre[2i] = ident x + correction and pre[2i+1] = near; stage 2 adds the
+// interpolation entries: y[i] = pre[2i] + (pre[2i+1] + far).
 template <int S, typename ZI>
 __global__ void __launch_bounds__(256)
     k_finalize(int N, int Np, double ident, const zd* __restrict__ x,
@@ -779,7 +783,7 @@ __global__ void __launch_bounds__(256)
                const zd* __restrict__ corr_w, const int* __restrict__ inv, const zd* __restrict__ sol,
                const int* __restrict__ nio, const zd* __restrict__ npart,
                const long long* __restrict__ it_ptr, const int* __restrict__ it_slot, const ZI* __restrict__ slots,
-               zd* __restrict__ y) {
+               zd* __restrict__ y, int stage = 0, zd* __restrict__ pre = nullptr) {
   const int lb = xcd_block();
   const int gt = lb * blockDim.x + threadIdx.x;
   const int i = gt / S, sl = gt % S;
@@ -792,7 +796,8 @@ __global__ void __launch_bounds__(256)
   constexpr int kFinU = 4;
   if (i < N) {
     si = inv[i];
-    const long long e0 = corr_ptr ? corr_ptr[i] : 0, e1 = corr_ptr ? corr_ptr[i + 1] : 0;
+    const bool own = stage != 2;
+    const long long e0 = own && corr_ptr ? corr_ptr[i] : 0, e1 = own && corr_ptr ? corr_ptr[i + 1] : 0;
     const long long nitem = (e1 - e0) * Np;
     for (long long ib = sl; ib < nitem; ib += (long long)S * kFinU) {
       zd wv[kFinU];
@@ -811,7 +816,8 @@ __global__ void __launch_bounds__(256)
       for (int u = 0; u < kFinU; u++)
         if (ib + (long long)u * S < nitem) cfma(acc, wv[u], x[src[u]]);
     }
-    if (npart) {
+    if (!own) {
+    } else if (npart) {
       const int g = si / kNearGroup, t = si % kNearGroup;
       const int w1 = nio[g + 1];
       for (int wb = nio[g] + sl; wb < w1; wb += S * kFinU) {
@@ -826,7 +832,7 @@ __global__ void __launch_bounds__(256)
     } else if (sl == 0) {
       near = sol[si];
     }
-    if (slots) {
+    if (slots && stage != 1) {
       const long long f1 = it_ptr[si + 1];
       for (long long fb = it_ptr[si] + sl; fb < f1; fb += (long long)S * kFinU) {
         int idx[kFinU];
@@ -844,7 +850,16 @@ __global__ void __launch_bounds__(256)
   acc = lane_group_sum<S>(acc);
   near = lane_group_sum<S>(near);
   far = lane_group_sum<S>(far);
-  if (i < N && sl == 0) y[i] = cadd(cadd(x ? cscale(x[i], ident) : zd{0.0, 0.0}, acc), cadd(near, far));
+  if (i < N && sl == 0) {
+    if (stage == 0) {
+      y[i] = cadd(cadd(x ? cscale(x[i], ident) : zd{0.0, 0.0}, acc), cadd(near, far));
+    } else if (stage == 1) {
+      pre[2 * (size_t)i] = cadd(x ? cscale(x[i], ident) : zd{0.0, 0.0}, acc);
+      pre[2 * (size_t)i + 1] = near;
+    } else {
+      y[i] = cadd(pre[2 * (size_t)i], cadd(pre[2 * (size_t)i + 1], far));
+    }
+  }
 }
 
 // Direct sum at a target subset, excluding the same pairs the correction

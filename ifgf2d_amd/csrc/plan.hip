@@ -382,11 +382,30 @@ static void launch_ifgf(ifgf_plan* P, const zd* x, zd* y, hipStream_t st) {
   // near field (stored path with overlap: on the second stream, joined
   // before the finalize; not while stage timing is recorded)
   const bool fork = P->stored && P->overlap && !prof;
+  const double ident = (F == kSingleLayer) ? 0.0 : 0.5;
+  const Z* slots = nullptr;
+  if (P->stored) slots = f32 ? (const Z*)P->pbuf32.p : (const Z*)P->pbuf.p;
+  const int NT = (int)t.M;
+  const zd* fx = P->field ? nullptr : x;
+  const long long* fcorr = P->field ? nullptr : P->corr_ptr.p;
+  // finalize stage 0 (all), 1 (identity, correction, near field) or 2
+  // (interpolation entries), with the plan's lanes per target
+  auto finalize = [&](int stage, hipStream_t s) {
+#define IFGF_FIN(S)                                                                                          \
+  hipLaunchKernelGGL((k_finalize<S, Z>), dim3((NT + 256 / S - 1) / (256 / S)), dim3(256), 0, s, NT, P->geo.Np, \
+                     ident, fx, fcorr, P->corr_patch.p, P->corr_w.p, P->inv.p, P->sol.p, P->near_item_off.p,  \
+                     P->stored ? P->npart.p : nullptr, P->it_ptr.p, P->it_slot.p, slots, y, stage, P->fin_pre.p)
+    if (P->final_lanes == 4) IFGF_FIN(4);
+    else if (P->final_lanes == 16) IFGF_FIN(16);
+    else IFGF_FIN(8);
+#undef IFGF_FIN
+  };
   mark(2);
   if (fork) {
     HIP_TRY(hipEventRecord(P->ev_fork, st));
     HIP_TRY(hipStreamWaitEvent(P->aux, P->ev_fork, 0));
     launch_near_stored(P, P->aux);
+    finalize(1, P->aux);
     HIP_TRY(hipEventRecord(P->ev_join, P->aux));
   } else if (P->stored) {
     launch_near_stored(P, st);
@@ -439,23 +458,11 @@ static void launch_ifgf(ifgf_plan* P, const zd* x, zd* y, hipStream_t st) {
                        P->sol.p, k);
   mark(9);
 
-  // correction, identity and unsort (a field: near + far at its targets)
+  // correction, identity and unsort (a field: near + far at its targets);
+  // with the near field on the second stream its first part ran there
   if (fork) HIP_TRY(hipStreamWaitEvent(st, P->ev_join, 0));
   mark(10);
-  const double ident = (F == kSingleLayer) ? 0.0 : 0.5;
-  const Z* slots = nullptr;
-  if (P->stored) slots = f32 ? (const Z*)P->pbuf32.p : (const Z*)P->pbuf.p;
-  const int NT = (int)t.M;
-  const zd* fx = P->field ? nullptr : x;
-  const long long* fcorr = P->field ? nullptr : P->corr_ptr.p;
-#define IFGF_FIN(S)                                                                                                  \
-  hipLaunchKernelGGL((k_finalize<S, Z>), dim3((NT + 256 / S - 1) / (256 / S)), dim3(256), 0, st, NT, P->geo.Np, ident, \
-                     fx, fcorr, P->corr_patch.p, P->corr_w.p, P->inv.p, P->sol.p, P->near_item_off.p,               \
-                     P->stored ? P->npart.p : nullptr, P->it_ptr.p, P->it_slot.p, slots, y)
-  if (P->final_lanes == 4) IFGF_FIN(4);
-  else if (P->final_lanes == 16) IFGF_FIN(16);
-  else IFGF_FIN(8);
-#undef IFGF_FIN
+  finalize(fork ? 2 : 0, st);
   mark(11);
   HIP_TRY(hipGetLastError());
 }
@@ -662,6 +669,7 @@ static void build_stored(ifgf_plan* P) {
     P->near_items.upload(items);
     P->near_item_off.upload(item_off);
     P->npart.alloc(std::max<size_t>(1, items.size()) * kNearGroup);  // never null: k_finalize tests it
+    P->fin_pre.alloc(2 * (size_t)t.M);
   }
   // per target leaf group, its neighbour boxes as (exclusive end in the
   // flattened segment, source index minus flat offset), padded to kMaxNb
