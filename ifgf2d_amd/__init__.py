@@ -207,8 +207,8 @@ class ForwardMap:
     interpolation; the FP64 operator stays available through compute_Ax_acc(x, fp64=True)).
     device_sort: Morton box assignment and point sort on the GPU (checked against the
     host's stable sort).
-    overlap: run the stored near field on a second stream beside the leaf operator and the
-    level pushes (the autotuner also tries it).
+    overlap: run the stored near field (and the identity / correction half of the final
+    sum) on a second stream beside the level pushes (the autotuner also tries it).
     device_precomp: build the rectangular-polar singular / near-singular correction on the
     GPU instead of the host.
     device_geometry: run the near-singular search of the discretisation on the GPU.
@@ -286,7 +286,7 @@ class ForwardMap:
         return {"leaf_cones": s[8], "prop_cones": s[4] - s[8], "stored": bool(s[9]),
                 "near_stored": s[10], "stored_bytes": s[11],
                 "lanes": {"near": s[12], "leaf": s[13], "push_task_entries": s[14], "finalize": s[15],
-                          "near_on_second_stream": bool(s[16])},
+                          "near_on_second_stream": bool(s[16]), "near_fork": s[16]},
                 "mixed": bool(s[17]), "ptab_entries": s[18], "ifac_entries": s[19]}
 
     def sorting(self):

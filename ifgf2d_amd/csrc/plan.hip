@@ -197,7 +197,7 @@ int ifgf_plan_create(const ifgf_surface* s, const ifgf_params* prm, ifgf_plan** 
       if (!(prm->flags & IFGF_PLAN_ON_THE_FLY)) {
         if (!build_stored_any(P.get())) return fail(nullptr, IFGF_ERR_ARG, "no kernel instantiation for this (Ps, Pang)");
         lap(5);
-        P->overlap = (prm->flags & IFGF_PLAN_OVERLAP) != 0;
+        P->overlap = (prm->flags & IFGF_PLAN_OVERLAP) ? 2 : 0;
         if (prm->flags & IFGF_PLAN_AUTOTUNE) autotune(P.get());
         lap(6);
       }
@@ -242,7 +242,7 @@ int ifgf_plan_stats(const ifgf_plan* p, long long out[20]) {
   out[13] = p->leaf_lanes;
   out[14] = p->push_emax;
   out[15] = p->final_lanes;
-  out[16] = p->overlap ? 1 : 0;
+  out[16] = p->overlap;
   out[17] = p->mixed ? 1 : 0;
   out[18] = (long long)p->push_ptab.n;
   out[19] = (long long)p->ifac_tab.n;
@@ -317,7 +317,7 @@ static void launch_near_stored(ifgf_plan* P, hipStream_t st) {
 }
 
 template <int Ps, int Pang, int EPT, typename R>
-static void launch_level_push_ept(ifgf_plan* P, hipStream_t st) {
+static void launch_level_push_ept(ifgf_plan* P, hipStream_t st, int li0, int li1) {
   const int D = P->tree.D;
   PushArgsT<R> a{};
   if constexpr (std::is_same<R, float>::value) {
@@ -331,7 +331,7 @@ static void launch_level_push_ept(ifgf_plan* P, hipStream_t st) {
                      P->leaf_item_off.p, P->lpart.p, P->tree.lv[D].cone_off, P->ifac_off.p, P->ifac_tab.p,
                      P->tabs.p, P->pbuf.p, 0};
   }
-  for (int li = 0; li + 1 < D; li++) {
+  for (int li = li0; li + 1 < D && li < li1; li++) {
     const int t0 = P->push_level_off[li], nt = P->push_level_off[li + 1] - t0;
     if (nt == 0) continue;
     a.tasks = P->push_tasks.p + t0;
@@ -342,12 +342,12 @@ static void launch_level_push_ept(ifgf_plan* P, hipStream_t st) {
 
 // tasks hold at most 256 * EPT entries (push_emax)
 template <int Ps, int Pang, typename R>
-static void launch_level_push(ifgf_plan* P, hipStream_t st) {
+static void launch_level_push(ifgf_plan* P, hipStream_t st, int li0 = 0, int li1 = 1 << 30) {
   switch (P->push_emax) {
-    case 256: launch_level_push_ept<Ps, Pang, 1, R>(P, st); break;
-    case 512: launch_level_push_ept<Ps, Pang, 2, R>(P, st); break;
-    case 1024: launch_level_push_ept<Ps, Pang, 4, R>(P, st); break;
-    default: launch_level_push_ept<Ps, Pang, 8, R>(P, st); break;
+    case 256: launch_level_push_ept<Ps, Pang, 1, R>(P, st, li0, li1); break;
+    case 512: launch_level_push_ept<Ps, Pang, 2, R>(P, st, li0, li1); break;
+    case 1024: launch_level_push_ept<Ps, Pang, 4, R>(P, st, li0, li1); break;
+    default: launch_level_push_ept<Ps, Pang, 8, R>(P, st, li0, li1); break;
   }
 }
 
@@ -402,11 +402,7 @@ static void launch_ifgf(ifgf_plan* P, const zd* x, zd* y, hipStream_t st) {
   };
   mark(2);
   if (fork) {
-    HIP_TRY(hipEventRecord(P->ev_fork, st));
-    HIP_TRY(hipStreamWaitEvent(P->aux, P->ev_fork, 0));
-    launch_near_stored(P, P->aux);
-    finalize(1, P->aux);
-    HIP_TRY(hipEventRecord(P->ev_join, P->aux));
+    // forked further down
   } else if (P->stored) {
     launch_near_stored(P, st);
   } else {
@@ -433,13 +429,30 @@ static void launch_ifgf(ifgf_plan* P, const zd* x, zd* y, hipStream_t st) {
                        P->pnx.p, P->pny.p, P->rho.p, P->coef.p, P->tabs.p, k, eta);
   }
   mark(5);
+  // the fork: after the leaf stage (overlap 1) or after the first level push
+  // (overlap 2), so that the near-field stream shares the bandwidth with the
+  // latency-bound level pushes rather than with the leaf stream
+  const int fork_li = P->overlap == 2 ? 1 : 0;
+  auto do_fork = [&]() {
+    HIP_TRY(hipEventRecord(P->ev_fork, st));
+    HIP_TRY(hipStreamWaitEvent(P->aux, P->ev_fork, 0));
+    launch_near_stored(P, P->aux);
+    finalize(1, P->aux);
+    HIP_TRY(hipEventRecord(P->ev_join, P->aux));
+  };
 
   // propagation and interpolation: stored path, one push launch per level
   // (both stages in it, timed as the propagation stage); on the fly, the
   // pull chain and one interpolation pass over all points
   mark(6);
   if (P->stored) {
-    launch_level_push<Ps, Pang, R>(P, st);
+    if (fork) {
+      launch_level_push<Ps, Pang, R>(P, st, 0, fork_li);
+      do_fork();
+      launch_level_push<Ps, Pang, R>(P, st, fork_li);
+    } else {
+      launch_level_push<Ps, Pang, R>(P, st);
+    }
   } else {
     for (int l = D; l >= 3; l--) {
       const LevelInfo& L = t.lv[l];
@@ -1074,8 +1087,7 @@ static void autotune(ifgf_plan* P) {
     int* v;
     std::vector<int> cand;
   };
-  int ov = P->overlap ? 1 : 0;
-  Knob knobs[] = {{&P->final_lanes, {4, 8, 16}}, {&ov, {0, 1}}};
+  Knob knobs[] = {{&P->final_lanes, {4, 8, 16}}, {&P->overlap, {0, 1, 2}}};
   // entries per level-push task: fewer give more, shorter blocks
   if (P->stored) {
     int best = P->push_emax;
@@ -1095,7 +1107,6 @@ static void autotune(ifgf_plan* P) {
     float tbest = 1e30f;
     for (int c : kb.cand) {
       *kb.v = c;
-      P->overlap = ov != 0;
       const float t = time_apply();
       if (t < tbest) {
         tbest = t;
@@ -1104,7 +1115,6 @@ static void autotune(ifgf_plan* P) {
     }
     *kb.v = best;
   }
-  P->overlap = ov != 0;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
 }

@@ -71,7 +71,7 @@ struct ifgf_plan {
   // stored path: the near field on a second stream beside the leaf operator
   // and the level-push chain (fork after the gather, join before finalize);
   // chosen by the autotuner
-  bool overlap = false;
+  int overlap = 0;  // near field on the aux stream: 0 no, 1 forked after the leaf stage, 2 after the first level push
   hipStream_t aux = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   // ifgf_apply_graph: one captured application sequence, keyed by its arguments

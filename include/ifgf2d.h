@@ -58,8 +58,9 @@ extern "C" {
 #define IFGF_PLAN_DEVICE_SORT 8 /* Morton box assignment and the stable sort of the points
                                    on the GPU (ifgf_morton_sort) instead of the host; the
                                    permutation is checked to equal the host's stable sort */
-#define IFGF_PLAN_OVERLAP 16   /* run the stored near field on a second stream beside the leaf
-                                  operator and the level pushes (IFGF_PLAN_AUTOTUNE also tries it) */
+#define IFGF_PLAN_OVERLAP 16   /* run the stored near field and the identity / correction half of
+                                  the final sum on a second stream beside the level pushes
+                                  (IFGF_PLAN_AUTOTUNE also tries it) */
 #define IFGF_PLAN_DEVICE_PRECOMP 32 /* evaluate the rectangular-polar singular / near-singular
                                        correction (ForwardMap.tpp:287-423) on the GPU, one thread
                                        per column, instead of on the host (Np <= 16) */
@@ -134,8 +135,10 @@ int ifgf_plan_points(const ifgf_plan* plan, double* x, double* y, double* nx, do
  * included as zeros), [11] bytes of the stored operators, [12] and [13] lanes
  * per target / leaf-cone node of the near-field and leaf kernels, [14]
  * evaluations per block of the per-level propagation/interpolation kernel,
- * [15] lanes per target of the correction kernel, [16] 1 if the near field runs on a
- * second stream beside the leaf / level-push chain, [17] 1 on a mixed-precision plan,
+ * [15] lanes per target of the correction kernel, [16] the near field on a second
+ * stream beside the level pushes: 0 no, 1 forked after the leaf stage, 2 after the
+ * first level push (the first half of the correction kernel runs there too),
+ * [17] 1 on a mixed-precision plan,
  * [18] stored propagation-geometry records (one per level, parent cone
  * direction, child position and node), [19] stored inverse node factors. */
 int ifgf_plan_stats(const ifgf_plan* plan, long long out[20]);
