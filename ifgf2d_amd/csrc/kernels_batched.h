@@ -243,14 +243,18 @@ __global__ void __launch_bounds__(256) k_level_push_b(PushArgsT<RT> a) {
 
 // k_finalize with R columns: lanes (target, column, slice) with S slices per
 // (target, column); y is column-major with stride ld; interpolation entry f
-// holds rs columns at slots[it_slot[f] * rs].
+// holds rs columns at slots[it_slot[f] * rs]. stage 0: all of it; stage 1
+// (beside the level pushes): pre[2 (i R + r)] = ident x + correction,
+// pre[2 (i R + r) + 1] = near; stage 2: y = pre[..] + (pre[.. + 1] + far),
+// the same bits as stage 0.
 template <int S, int R, typename ZI>
 __global__ void __launch_bounds__(256)
     k_finalize_b(int N, int Np, double ident, const zd* __restrict__ x, long long ld,
                  const long long* __restrict__ corr_ptr, const int* __restrict__ corr_patch,
                  const zd* __restrict__ corr_w, const int* __restrict__ inv, const int* __restrict__ nio,
                  const zd* __restrict__ npart, const long long* __restrict__ it_ptr, const int* __restrict__ it_slot,
-                 const ZI* __restrict__ slots, int rs, zd* __restrict__ y) {
+                 const ZI* __restrict__ slots, int rs, zd* __restrict__ y, int stage = 0,
+                 zd* __restrict__ pre = nullptr) {
   const long long gt = (long long)xcd_block() * blockDim.x + threadIdx.x;
   const long long ir = gt / S;
   const int sl = (int)(gt % S);
@@ -258,24 +262,39 @@ __global__ void __launch_bounds__(256)
   zd acc{0.0, 0.0}, near{0.0, 0.0}, far{0.0, 0.0};
   if (i < N) {
     const int si = inv[i];
-    const zd* xr = x + (size_t)r * ld;
-    const long long e0 = corr_ptr[i], e1 = corr_ptr[i + 1];
-    const long long nitem = (e1 - e0) * Np;
-    for (long long it = sl; it < nitem; it += S) {
-      const long long e = e0 + it / Np;
-      const int q = (int)(it % Np);
-      cfma(acc, corr_w[e * Np + q], xr[(size_t)corr_patch[e] * Np + q]);
+    if (stage != 2) {
+      const zd* xr = x + (size_t)r * ld;
+      const long long e0 = corr_ptr[i], e1 = corr_ptr[i + 1];
+      const long long nitem = (e1 - e0) * Np;
+      for (long long it = sl; it < nitem; it += S) {
+        const long long e = e0 + it / Np;
+        const int q = (int)(it % Np);
+        cfma(acc, corr_w[e * Np + q], xr[(size_t)corr_patch[e] * Np + q]);
+      }
+      const int g = si / kNearGroup, t = si % kNearGroup;
+      const int w1 = nio[g + 1];
+      for (int w = nio[g] + sl; w < w1; w += S) near = cadd(near, npart[((size_t)w * kNearGroup + t) * R + r]);
     }
-    const int g = si / kNearGroup, t = si % kNearGroup;
-    const int w1 = nio[g + 1];
-    for (int w = nio[g] + sl; w < w1; w += S) near = cadd(near, npart[((size_t)w * kNearGroup + t) * R + r]);
-    const long long f1 = it_ptr[si + 1];
-    for (long long f = it_ptr[si] + sl; f < f1; f += S) far = cadd(far, zwiden(slots[(size_t)it_slot[f] * rs + r]));
+    if (stage != 1) {
+      const long long f1 = it_ptr[si + 1];
+      for (long long f = it_ptr[si] + sl; f < f1; f += S)
+        far = cadd(far, zwiden(slots[(size_t)it_slot[f] * rs + r]));
+    }
   }
   acc = lane_group_sum<S>(acc);
   near = lane_group_sum<S>(near);
   far = lane_group_sum<S>(far);
-  if (i < N && sl == 0) y[(size_t)r * ld + i] = cadd(cadd(cscale(x[(size_t)r * ld + i], ident), acc), cadd(near, far));
+  if (i < N && sl == 0) {
+    const size_t pi = 2 * ((size_t)i * R + r);
+    if (stage == 0) {
+      y[(size_t)r * ld + i] = cadd(cadd(cscale(x[(size_t)r * ld + i], ident), acc), cadd(near, far));
+    } else if (stage == 1) {
+      pre[pi] = cadd(cscale(x[(size_t)r * ld + i], ident), acc);
+      pre[pi + 1] = near;
+    } else {
+      y[(size_t)r * ld + i] = cadd(pre[pi], cadd(pre[pi + 1], far));
+    }
+  }
 }
 
 }  // namespace ifgf

@@ -494,8 +494,29 @@ static void launch_batched(ifgf_plan* P, const zd* x, long long ld, zd* y, hipSt
   hipLaunchKernelGGL((k_gather_b<R>), dim3((unsigned)(((long long)N * R + 255) / 256)), dim3(256), 0, st, N, x, ld,
                      P->w.p, P->perm.p, P->rho.p);
   const int M = (int)P->near_items.n;
-  hipLaunchKernelGGL((k_near_items_b<R>), dim3((M + 3) / 4), dim3(256), 0, st, N, M, P->near_items.p, P->nf_seg.p,
-                     P->nf_val.p, P->rho.p, P->npart_b.p);
+  // with overlap: the near field and the identity / correction half of the
+  // final sum on the second stream, forked along the push chain as for one column
+  const bool fork = P->overlap != 0 && M > 0;
+  auto near_b = [&](hipStream_t s) {
+    if (M > 0)
+      hipLaunchKernelGGL((k_near_items_b<R>), dim3((M + 3) / 4), dim3(256), 0, s, N, M, P->near_items.p,
+                         P->nf_seg.p, P->nf_val.p, P->rho.p, P->npart_b.p);
+  };
+  const double ident = (F == kSingleLayer) ? 0.0 : 0.5;
+  const long long nthr = (long long)N * R;
+  Z* pbuf_ = f32 ? (Z*)P->pbuf_b32.p : (Z*)P->pbuf_b.p;
+  auto finalize_b = [&](int stage, hipStream_t s) {
+#define IFGF_FINB(S)                                                                                              \
+  hipLaunchKernelGGL((k_finalize_b<S, R, Z>), dim3((unsigned)((nthr * S + 255) / 256)), dim3(256), 0, s, N,         \
+                     P->geo.Np, ident, x, ld, P->corr_ptr.p, P->corr_patch.p, P->corr_w.p, P->inv.p,                \
+                     P->near_item_off.p, P->npart_b.p, P->it_ptr.p, P->it_slot.p, pbuf_, P->rb_max, y, stage,      \
+                     P->fin_pre_b.p)
+    if (P->final_lanes == 4) IFGF_FINB(4);
+    else if (P->final_lanes == 16) IFGF_FINB(16);
+    else IFGF_FINB(8);
+#undef IFGF_FINB
+  };
+  if (!fork) near_b(st);
   const long long ML = P->leaf_items.n;
   const Z* lf = f32 ? (const Z*)P->lf_val32.p : (const Z*)P->lf_val.p;
   Z* lpart = f32 ? (Z*)P->lpart_b32.p : (Z*)P->lpart_b.p;
@@ -513,24 +534,23 @@ static void launch_batched(ifgf_plan* P, const zd* x, long long ld, zd* y, hipSt
                       P->rb_max};
   }
   const std::vector<int>& lo = P->blevel_off[LR];
+  const int fork_li = P->overlap == 2 ? 1 : 0;
   for (int li = 0; li + 1 < D; li++) {
+    if (fork && li == fork_li) {
+      HIP_TRY(hipEventRecord(P->ev_fork, st));
+      HIP_TRY(hipStreamWaitEvent(P->aux, P->ev_fork, 0));
+      near_b(P->aux);
+      finalize_b(1, P->aux);
+      HIP_TRY(hipEventRecord(P->ev_join, P->aux));
+    }
     const int t0 = lo[li], nt = lo[li + 1] - t0;
     if (nt == 0) continue;
     a.tasks = P->btasks[LR].p + t0;
     a.leaf = li == 0;
     hipLaunchKernelGGL((k_level_push_b<Ps, Pang, R, CB, RT>), dim3(nt), dim3(256), 0, st, a);
   }
-  const double ident = (F == kSingleLayer) ? 0.0 : 0.5;
-
-  const long long nthr = (long long)N * R;
-#define IFGF_FINB(S)                                                                                               \
-  hipLaunchKernelGGL((k_finalize_b<S, R, Z>), dim3((unsigned)((nthr * S + 255) / 256)), dim3(256), 0, st, N,         \
-                     P->geo.Np, ident, x, ld, P->corr_ptr.p, P->corr_patch.p, P->corr_w.p, P->inv.p,                 \
-                     P->near_item_off.p, P->npart_b.p, P->it_ptr.p, P->it_slot.p, pbuf, P->rb_max, y)
-  if (P->final_lanes == 4) IFGF_FINB(4);
-  else if (P->final_lanes == 16) IFGF_FINB(16);
-  else IFGF_FINB(8);
-#undef IFGF_FINB
+  if (fork) HIP_TRY(hipStreamWaitEvent(st, P->ev_join, 0));
+  finalize_b(fork ? 2 : 0, st);
   HIP_TRY(hipGetLastError());
 }
 
@@ -599,6 +619,7 @@ static void build_batched(ifgf_plan* P) {
   if (!P->rb_max) return;
   const size_t R = (size_t)P->rb_max;
   P->npart_b.alloc(P->npart.n * R);
+  P->fin_pre_b.alloc(2 * (size_t)P->tree.M * R);
   if (P->mixed) {
     P->lpart_b32.alloc(P->lpart.n * R);
     P->pbuf_b32.alloc((size_t)P->nslots * R);

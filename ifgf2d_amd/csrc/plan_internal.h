@@ -119,6 +119,7 @@ struct ifgf_plan {
   ifgf::DevBuf<int> near_item_off;          // items of target group g: [off[g], off[g+1])
   ifgf::DevBuf<zd> npart;                   // near-field partial sums [item][8]
   ifgf::DevBuf<zd> fin_pre;                 // split finalize: identity + correction, near, per target
+  ifgf::DevBuf<zd> fin_pre_b;               // the same for the batched columns (widest block)
   // several right-hand sides per application: column blocks of rb_max (2, 4
   // or 8; 0 = none) through the batched kernels; push tasks per block width R
   // (index log2 R) with the cone cap of their LDS windows
