@@ -197,7 +197,7 @@ int ifgf_plan_create(const ifgf_surface* s, const ifgf_params* prm, ifgf_plan** 
       if (!(prm->flags & IFGF_PLAN_ON_THE_FLY)) {
         if (!build_stored_any(P.get())) return fail(nullptr, IFGF_ERR_ARG, "no kernel instantiation for this (Ps, Pang)");
         lap(5);
-        P->overlap = (prm->flags & IFGF_PLAN_OVERLAP) ? 2 : 0;
+        P->overlap = P->overlap_b = (prm->flags & IFGF_PLAN_OVERLAP) ? 2 : 0;
         if (prm->flags & IFGF_PLAN_AUTOTUNE) autotune(P.get());
         lap(6);
       }
@@ -496,7 +496,7 @@ static void launch_batched(ifgf_plan* P, const zd* x, long long ld, zd* y, hipSt
   const int M = (int)P->near_items.n;
   // with overlap: the near field and the identity / correction half of the
   // final sum on the second stream, forked along the push chain as for one column
-  const bool fork = P->overlap != 0 && M > 0;
+  const bool fork = P->overlap_b != 0 && M > 0;
   auto near_b = [&](hipStream_t s) {
     if (M > 0)
       hipLaunchKernelGGL((k_near_items_b<R>), dim3((M + 3) / 4), dim3(256), 0, s, N, M, P->near_items.p,
@@ -534,7 +534,7 @@ static void launch_batched(ifgf_plan* P, const zd* x, long long ld, zd* y, hipSt
                       P->rb_max};
   }
   const std::vector<int>& lo = P->blevel_off[LR];
-  const int fork_li = P->overlap == 2 ? 1 : 0;
+  const int fork_li = P->overlap_b == 2 ? 1 : 0;
   for (int li = 0; li + 1 < D; li++) {
     if (fork && li == fork_li) {
       HIP_TRY(hipEventRecord(P->ev_fork, st));
@@ -1140,6 +1140,42 @@ static void autotune(ifgf_plan* P) {
       }
     }
     *kb.v = best;
+  }
+  // the batched blocks' second stream, timed on one block of the widest width
+  P->overlap_b = P->overlap;
+  if (P->stored && P->rb_max > 0) {
+    const int R = P->rb_max;
+    DevBuf<zd> xb, yb;
+    xb.alloc((size_t)N * R);
+    yb.alloc((size_t)N * R);
+    for (int c = 0; c < R; c++)
+      HIP_TRY(hipMemcpy(xb.p + (size_t)c * N, xh.data(), N * sizeof(zd), hipMemcpyHostToDevice));
+    auto time_block = [&]() {
+      for (int w = 0; w < 2; w++) dispatch_block(P, R, xb.p, N, yb.p, st);
+      float best = 1e30f;
+      for (int round = 0; round < 3; round++) {
+        HIP_TRY(hipEventRecord(e0, st));
+        for (int r = 0; r < 3; r++) dispatch_block(P, R, xb.p, N, yb.p, st);
+        HIP_TRY(hipEventRecord(e1, st));
+        HIP_TRY(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+        best = std::min(best, ms);
+      }
+      return best;
+    };
+    int best = P->overlap_b;
+    float tbest = 1e30f;
+    for (int c : {0, 1, 2}) {
+      P->overlap_b = c;
+      const float t = time_block();
+      if (t < tbest) {
+        tbest = t;
+        best = c;
+      }
+    }
+    P->overlap_b = best;
+    HIP_TRY(hipStreamSynchronize(st));
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);

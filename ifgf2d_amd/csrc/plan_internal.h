@@ -72,6 +72,7 @@ struct ifgf_plan {
   // and the level-push chain (fork after the gather, join before finalize);
   // chosen by the autotuner
   int overlap = 0;  // near field on the aux stream: 0 no, 1 forked after the leaf stage, 2 after the first level push
+  int overlap_b = 0;  // the same for the batched column blocks
   hipStream_t aux = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   // ifgf_apply_graph: one captured application sequence, keyed by its arguments
