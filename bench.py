@@ -310,14 +310,26 @@ def main():
                 xg, yg, *_ = fb.points()
                 th = 2 * np.pi * np.arange(64) / 64
                 B = np.stack([-configs.plane_wave(xg, yg, kw["k"], (np.cos(t_), np.sin(t_))) for t_ in th], axis=1)
-                (_, its, rr), t_mc, t_m = timed_solve(lambda: fb.solve_multi(B, restart=50, tol=1e-8, maxit=2000),
-                                                      reps=2)
+                (X64, its, rr), t_mc, t_m = timed_solve(lambda: fb.solve_multi(B, restart=50, tol=1e-8, maxit=2000),
+                                                        reps=2)
                 batched["solve_64_directions"] = {"time_to_solve_s": round(t_m, 4), "first_call_s": round(t_mc, 4),
                                                   "iterations_min": int(its.min()),
                                                   "iterations_max": int(its.max()),
                                                   "true_relres_max": float(rr.max())}
                 log(f"64 directions solved together: {t_m:.3f} s, iterations {its.min()}..{its.max()}, "
                     f"max true relres {rr.max():.2e}")
+            if prec == "mixed" and not args.no_solve and "solve_64_directions" in batched:
+                # the same 64 directions: lockstep mixed-precision GMRES corrections,
+                # FP64 residuals (ifgf_refine_multi)
+                (Xm, outer, its, rr), t_mc, t_m = timed_solve(
+                    lambda: fb.solve_multi_refined(B, restart=50, tol=1e-8, maxit=2000), reps=2)
+                dmax = max(float(np.linalg.norm(Xm[:, j] - X64[:, j]) / np.linalg.norm(X64[:, j])) for j in range(64))
+                batched["solve_64_directions_mixed_refined"] = {
+                    "time_to_solve_s": round(t_m, 4), "first_call_s": round(t_mc, 4), "refinement_steps": outer,
+                    "inner_iterations_min": int(its.min()), "inner_iterations_max": int(its.max()),
+                    "true_relres_max": float(rr.max()), "density_rel_diff_vs_fp64_max": dmax}
+                log(f"64 directions, mixed refinement: {t_m:.3f} s, {outer} steps, inner iterations "
+                    f"{its.min()}..{its.max()}, max true relres {rr.max():.2e}, density diff {dmax:.1e}")
             del fb
         del Xb, Yb
 

@@ -76,6 +76,7 @@ EXPORTS = [
     "ifgf_morton_sort", "ifgf_apply_graph", "ifgf_group_create", "ifgf_group_launch",
     "ifgf_group_synchronize", "ifgf_group_destroy", "ifgf_gmres_multi", "ifgf_field_create",
     "ifgf_field_apply", "ifgf_field_stats", "ifgf_field_destroy", "ifgf_plan_setup_times",
+    "ifgf_refine_multi",
 ]
 
 _LIB = None
@@ -123,6 +124,8 @@ def lib():
                                   ctypes.c_int]
         L.ifgf_gmres_multi.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                        ctypes.c_int, _ip, _dp]
+        L.ifgf_refine_multi.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                        ctypes.c_int, ctypes.c_int, ctypes.c_double, _ip, _ip, _dp]
         L.ifgf_gmres_host.argtypes = [APPLY_FN, ctypes.c_void_p, ctypes.c_longlong, _dp, _dp,
                                       ctypes.c_int, ctypes.c_double, ctypes.c_int, _ip, _dp]
         L.ifgf_hankel01.argtypes = [_dp, ctypes.c_longlong, _dp, _dp]
@@ -393,6 +396,25 @@ class ForwardMap:
         if rc not in (IFGF_OK, IFGF_ERR_NOT_CONVERGED):
             self._check(rc, "ifgf_gmres_multi")
         return X, its, rr
+
+    def solve_multi_refined(self, rhs, restart=50, tol=1e-8, maxit=1000, max_outer=10, inner_floor=1e-5,
+                            x0=None):
+        """Mixed-precision refinement for the columns of rhs (ifgf_refine_multi):
+        lockstep GMRES corrections with the plan's default operator (mixed precision
+        on a mixed plan), FP64 residuals. Returns (X, refinement steps, inner
+        iterations per column, true FP64 relative residual per column)."""
+        B = np.asfortranarray(np.asarray(rhs, dtype=np.complex128).reshape(self.N, -1))
+        nr = B.shape[1]
+        X = np.zeros_like(B, order="F") if x0 is None else np.asfortranarray(np.array(x0, dtype=np.complex128))
+        its = np.zeros(nr, dtype=np.int32)
+        rr = np.zeros(nr)
+        outer = ctypes.c_int(0)
+        rc = lib().ifgf_refine_multi(self._h, B.ctypes.data_as(_dp), X.ctypes.data_as(_dp), nr, restart, tol,
+                                     maxit, max_outer, inner_floor, ctypes.byref(outer), its.ctypes.data_as(_ip),
+                                     _d(rr))
+        if rc not in (IFGF_OK, IFGF_ERR_NOT_CONVERGED):
+            self._check(rc, "ifgf_refine_multi")
+        return X, outer.value, its, rr
 
     def solve_fgmres(self, rhs, restart=50, tol=1e-8, maxit=200, inner_restart=50, inner_floor=1e-5,
                      x0=None):

@@ -697,12 +697,12 @@ extern "C" int ifgf_fgmres(ifgf_plan* P, const double* rhs, double* x, int resta
 // estimate |w(k)| / ||r0|| < tol (opGMRES.hpp:207). Every system follows the
 // same algorithm as ifgf_gmres (same restart, same stopping rule); all
 // active systems restart together.
-extern "C" int ifgf_gmres_multi(ifgf_plan* P, const double* rhs, double* x, int nrhs, int restart, double tol,
-                                int maxit, int* iters, double* relres) {
-  if (!P || !rhs || !x || !iters || !relres || nrhs < 1 || restart < 1 || maxit < 1)
-    return plan_fail(P, IFGF_ERR_ARG, "invalid argument");
-  if (restart > kMaxRestart) return plan_fail(P, IFGF_ERR_ARG, "restart above 256");
-  if (!P->on_device) return plan_fail(P, IFGF_ERR_NO_DEVICE, "plan was built host-only");
+// The lockstep GMRES on device columns: B (right-hand sides) and X (initial
+// guesses in, solutions out), n x nrhs each. true_res: relres[s] = the true
+// relative residual with the all-FP64 operator; else the residual norm at the
+// start of the last cycle over ||b_s||.
+static int gmres_multi_dev(ifgf_plan* P, const zd* Bp, zd* Xp, int nrhs, int restart, double tol, int maxit,
+                           int* iters, double* relres, bool true_res) {
   using C = std::complex<double>;
   const long long n = P->geo.N;
   const int m = restart, R = nrhs;
@@ -711,11 +711,9 @@ extern "C" int ifgf_gmres_multi(ifgf_plan* P, const double* rhs, double* x, int 
   const long long pns = 2 * (long long)nb;
   try {
     hipStream_t st = P->stream;
-    DevBuf<zd> B, X, V, W, part, H;
+    DevBuf<zd> V, W, part, H;
     DevBuf<double> part_n;
     DevBuf<int> sysd, gate;
-    B.alloc((size_t)n * R);
-    X.alloc((size_t)n * R);
     V.alloc((size_t)vs * R);
     W.alloc((size_t)n * 2 * R);  // operator input block, then its output block
     part.alloc((size_t)ps * R);
@@ -723,8 +721,6 @@ extern "C" int ifgf_gmres_multi(ifgf_plan* P, const double* rhs, double* x, int 
     gate.alloc(R);
     H.alloc((size_t)hs * R);
     sysd.alloc(R);
-    HIP_TRY(hipMemcpyAsync(B.p, rhs, (size_t)n * R * sizeof(zd), hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(X.p, x, (size_t)n * R * sizeof(zd), hipMemcpyHostToDevice, st));
     std::vector<int> act(R);
     for (int s = 0; s < R; s++) act[s] = s;
     auto upload_act = [&]() { HIP_TRY(hipMemcpyAsync(sysd.p, act.data(), act.size() * sizeof(int), hipMemcpyHostToDevice, st)); };
@@ -747,15 +743,15 @@ extern "C" int ifgf_gmres_multi(ifgf_plan* P, const double* rhs, double* x, int 
       upload_act();
       const int na = (int)act.size();
       zd* in = W.p + (size_t)R * n;
-      hipLaunchKernelGGL(k_gather_ms, dim3(nb, na), dim3(256), 0, st, n, X.p, n, 0, sysd.p, in);
+      hipLaunchKernelGGL(k_gather_ms, dim3(nb, na), dim3(256), 0, st, n, Xp, n, 0, sysd.p, in);
       int rc = apply_columns(P, in, n, W.p, na, st, fp64);
       if (rc) throw std::runtime_error(P->err);
-      hipLaunchKernelGGL(k_residual_ms, dim3(nb, na), dim3(256), 0, st, n, B.p, sysd.p, W.p);
+      hipLaunchKernelGGL(k_residual_ms, dim3(nb, na), dim3(256), 0, st, n, Bp, sysd.p, W.p);
     };
     // ||b||
     std::vector<double> bn(R), r0n(R), beta(R), tmp;
     {
-      HIP_TRY(hipMemcpyAsync(W.p, B.p, (size_t)n * R * sizeof(zd), hipMemcpyDeviceToDevice, st));
+      HIP_TRY(hipMemcpyAsync(W.p, Bp, (size_t)n * R * sizeof(zd), hipMemcpyDeviceToDevice, st));
       norms(R, bn);
     }
     for (int s = 0; s < R; s++) iters[s] = 0;
@@ -836,7 +832,7 @@ extern "C" int ifgf_gmres_multi(ifgf_plan* P, const double* rhs, double* x, int 
           y = back_substitute(Hm[s], m, g[s], cols[s]);
           HIP_TRY(hipMemcpyAsync(H.p + (size_t)s * hs, y.data(), cols[s] * sizeof(zd), hipMemcpyHostToDevice, st));
           hipLaunchKernelGGL(k_update, dim3(nb), dim3(256), 0, st, n, cols[s], V.p + (size_t)s * vs, n,
-                             H.p + (size_t)s * hs, -1.0, X.p + (size_t)s * n);
+                             H.p + (size_t)s * hs, -1.0, Xp + (size_t)s * n);
         }
         act = still;
         na = (int)act.size();
@@ -846,7 +842,7 @@ extern "C" int ifgf_gmres_multi(ifgf_plan* P, const double* rhs, double* x, int 
         y = back_substitute(Hm[s], m, g[s], cols[s]);
         HIP_TRY(hipMemcpyAsync(H.p + (size_t)s * hs, y.data(), cols[s] * sizeof(zd), hipMemcpyHostToDevice, st));
         hipLaunchKernelGGL(k_update, dim3(nb), dim3(256), 0, st, n, cols[s], V.p + (size_t)s * vs, n,
-                           H.p + (size_t)s * hs, -1.0, X.p + (size_t)s * n);
+                           H.p + (size_t)s * hs, -1.0, Xp + (size_t)s * n);
       }
       if (!act.empty() && it < maxit) {
         residuals(false);
@@ -854,17 +850,147 @@ extern "C" int ifgf_gmres_multi(ifgf_plan* P, const double* rhs, double* x, int 
         for (size_t a = 0; a < act.size(); a++) beta[act[a]] = tmp[a];
       }
     }
-    // true residuals with the all-FP64 operator
-    act.resize(R);
-    for (int s = 0; s < R; s++) act[s] = s;
-    residuals(true);
-    norms(R, tmp);
-    for (int s = 0; s < R; s++) relres[s] = bn[s] > 0.0 ? tmp[s] / bn[s] : 0.0;
-    HIP_TRY(hipMemcpyAsync(x, X.p, (size_t)n * R * sizeof(zd), hipMemcpyDeviceToHost, st));
+    if (true_res) {  // true residuals with the all-FP64 operator
+      act.resize(R);
+      for (int s = 0; s < R; s++) act[s] = s;
+      residuals(true);
+      norms(R, tmp);
+      for (int s = 0; s < R; s++) relres[s] = bn[s] > 0.0 ? tmp[s] / bn[s] : 0.0;
+    } else {
+      for (int s = 0; s < R; s++) relres[s] = bn[s] > 0.0 ? beta[s] / bn[s] : 0.0;
+    }
     HIP_TRY(hipStreamSynchronize(st));
     bool all = true;
     for (int s = 0; s < R; s++) all = all && done[s];
     return all ? IFGF_OK : IFGF_ERR_NOT_CONVERGED;
+  } catch (std::exception& e) {
+    return plan_fail(P, IFGF_ERR_HIP, e.what());
+  }
+}
+
+extern "C" int ifgf_gmres_multi(ifgf_plan* P, const double* rhs, double* x, int nrhs, int restart, double tol,
+                                int maxit, int* iters, double* relres) {
+  if (!P || !rhs || !x || !iters || !relres || nrhs < 1 || restart < 1 || maxit < 1)
+    return plan_fail(P, IFGF_ERR_ARG, "invalid argument");
+  if (restart > kMaxRestart) return plan_fail(P, IFGF_ERR_ARG, "restart above 256");
+  if (!P->on_device) return plan_fail(P, IFGF_ERR_NO_DEVICE, "plan was built host-only");
+  const long long n = P->geo.N;
+  try {
+    hipStream_t st = P->stream;
+    DevBuf<zd> B, X;
+    B.alloc((size_t)n * nrhs);
+    X.alloc((size_t)n * nrhs);
+    HIP_TRY(hipMemcpyAsync(B.p, rhs, (size_t)n * nrhs * sizeof(zd), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(X.p, x, (size_t)n * nrhs * sizeof(zd), hipMemcpyHostToDevice, st));
+    const int rc = gmres_multi_dev(P, B.p, X.p, nrhs, restart, tol, maxit, iters, relres, true);
+    if (rc != IFGF_OK && rc != IFGF_ERR_NOT_CONVERGED) return rc;
+    HIP_TRY(hipMemcpyAsync(x, X.p, (size_t)n * nrhs * sizeof(zd), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    return rc;
+  } catch (std::exception& e) {
+    return plan_fail(P, IFGF_ERR_HIP, e.what());
+  }
+}
+
+// X[:, sys[a]] += D[:, a]
+__global__ void __launch_bounds__(256)
+    k_add_ms(long long n, const zd* __restrict__ D, const int* __restrict__ sys, zd* __restrict__ X) {
+  const int a = blockIdx.y;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  zd& v = X[(size_t)sys[a] * n + i];
+  v = cadd(v, D[(size_t)a * n + i]);
+}
+
+// Mixed-precision refinement for many right-hand sides: the corrections come
+// from the lockstep GMRES with the plan's default operator, the residuals
+// from the all-FP64 operator; every vector stays on the device.
+extern "C" int ifgf_refine_multi(ifgf_plan* P, const double* rhs, double* x, int nrhs, int restart, double tol,
+                                 int maxit, int max_outer, double inner_floor, int* outer, int* iters,
+                                 double* relres) {
+  if (!P || !rhs || !x || !outer || !iters || !relres || nrhs < 1 || restart < 1 || maxit < 1 || max_outer < 1 ||
+      !(tol > 0.0) || !(inner_floor > 0.0))
+    return plan_fail(P, IFGF_ERR_ARG, "invalid argument");
+  if (restart > kMaxRestart) return plan_fail(P, IFGF_ERR_ARG, "restart above 256");
+  if (!P->on_device) return plan_fail(P, IFGF_ERR_NO_DEVICE, "plan was built host-only");
+  const long long n = P->geo.N;
+  const int R = nrhs, nb = (int)((n + 255) / 256);
+  const long long ps = kDot;
+  try {
+    hipStream_t st = P->stream;
+    DevBuf<zd> B, X, W, G, D, part;
+    DevBuf<int> sysd;
+    B.alloc((size_t)n * R);
+    X.alloc((size_t)n * R);
+    W.alloc((size_t)n * R);  // residual block of the active systems
+    G.alloc((size_t)n * R);  // their x, gathered
+    D.alloc((size_t)n * R);  // their corrections
+    part.alloc((size_t)ps * R);
+    sysd.alloc(R);
+    HIP_TRY(hipMemcpyAsync(B.p, rhs, (size_t)n * R * sizeof(zd), hipMemcpyHostToDevice, st));
+    HIP_TRY(hipMemcpyAsync(X.p, x, (size_t)n * R * sizeof(zd), hipMemcpyHostToDevice, st));
+    std::vector<double> bn(R);
+    {
+      const std::complex<double>* b = reinterpret_cast<const std::complex<double>*>(rhs);
+      for (int q = 0; q < R; q++) {
+        double acc = 0.0;
+        for (long long i = 0; i < n; i++) acc += std::norm(b[(size_t)q * n + i]);
+        bn[q] = std::sqrt(acc);
+        iters[q] = 0;
+      }
+    }
+    std::vector<int> act(R);
+    for (int q = 0; q < R; q++) act[q] = q;
+    // W[:, a] = b - A x (all-FP64 operator) for the systems in act; relres of each
+    auto residuals = [&]() {
+      const int na = (int)act.size();
+      HIP_TRY(hipMemcpyAsync(sysd.p, act.data(), na * sizeof(int), hipMemcpyHostToDevice, st));
+      hipLaunchKernelGGL(k_gather_ms, dim3(nb, na), dim3(256), 0, st, n, X.p, n, 0, sysd.p, G.p);
+      const int rc = apply_columns(P, G.p, n, W.p, na, st, true);
+      if (rc) throw std::runtime_error(P->err);
+      hipLaunchKernelGGL(k_residual_ms, dim3(nb, na), dim3(256), 0, st, n, B.p, sysd.p, W.p);
+      hipLaunchKernelGGL(k_norm_parts_ms, dim3(kDot, 1, na), dim3(256), 0, st, n, W.p, part.p, ps);
+      std::vector<zd> pt((size_t)ps * na);
+      HIP_TRY(hipMemcpyAsync(pt.data(), part.p, pt.size() * sizeof(zd), hipMemcpyDeviceToHost, st));
+      HIP_TRY(hipStreamSynchronize(st));
+      for (int a = 0; a < na; a++) {
+        double acc = 0.0;
+        for (int q = 0; q < kDot; q++) acc += pt[(size_t)a * ps + q].x;
+        relres[act[a]] = bn[act[a]] > 0.0 ? std::sqrt(acc) / bn[act[a]] : 0.0;
+      }
+    };
+    residuals();
+    int k = 0;
+    std::vector<int> it(R);
+    std::vector<double> rr(R);
+    while (k < max_outer) {
+      std::vector<int> next;
+      for (int q : act)
+        if (relres[q] >= tol) next.push_back(q);
+      if (next.size() != act.size()) {  // keep the residual block in step with act
+        act.swap(next);
+        if (act.empty()) break;
+        residuals();
+      }
+      const int na = (int)act.size();
+      double emax = 0.0;
+      for (int q : act) emax = std::max(emax, relres[q]);
+      const double tau = std::min(0.5, std::max(inner_floor, tol / emax));
+      HIP_TRY(hipMemsetAsync(D.p, 0, (size_t)n * na * sizeof(zd), st));
+      const int rc = gmres_multi_dev(P, W.p, D.p, na, restart, tau, maxit, it.data(), rr.data(), false);
+      if (rc != IFGF_OK && rc != IFGF_ERR_NOT_CONVERGED) return rc;
+      k++;
+      HIP_TRY(hipMemcpyAsync(sysd.p, act.data(), na * sizeof(int), hipMemcpyHostToDevice, st));
+      hipLaunchKernelGGL(k_add_ms, dim3(nb, na), dim3(256), 0, st, n, D.p, sysd.p, X.p);
+      for (int a = 0; a < na; a++) iters[act[a]] += it[a];
+      residuals();
+    }
+    HIP_TRY(hipMemcpyAsync(x, X.p, (size_t)n * R * sizeof(zd), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    *outer = k;
+    bool all = true;
+    for (int q = 0; q < R; q++) all = all && relres[q] < tol;
+    return all ? IFGF_OK : plan_fail(P, IFGF_ERR_NOT_CONVERGED, "refinement did not reach tol");
   } catch (std::exception& e) {
     return plan_fail(P, IFGF_ERR_HIP, e.what());
   }

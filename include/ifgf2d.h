@@ -277,6 +277,21 @@ int ifgf_gmres(ifgf_plan* plan, const double* rhs, double* x, int restart, doubl
 int ifgf_gmres_multi(ifgf_plan* plan, const double* rhs, double* x, int nrhs, int restart, double tol,
                      int maxit, int* iters, double* relres);
 
+/* Mixed-precision refinement for many right-hand sides (no reference
+ * counterpart): x <- x + d, d the lockstep GMRES(restart) solution of
+ * A d = b - A x (ifgf_gmres_multi, x0 = 0, the plan's default operator: mixed
+ * precision on an IFGF_MIXED plan) to relative tolerance
+ * min(0.5, max(inner_floor, tol / e)), e the largest true relative residual of
+ * the systems not yet converged; the residuals use the all-FP64 operator, one
+ * column at a time. A system stops once its true FP64 relative residual is
+ * below tol; at most max_outer refinement steps. rhs / x: n x nrhs,
+ * column-major, host pointers (x holds the initial guesses). outer: the
+ * refinement steps taken; iters[s]: system s's inner Arnoldi steps; relres[s]:
+ * its true FP64 relative residual. Returns IFGF_OK when every system
+ * converged, else IFGF_ERR_NOT_CONVERGED. */
+int ifgf_refine_multi(ifgf_plan* plan, const double* rhs, double* x, int nrhs, int restart, double tol,
+                      int maxit, int max_outer, double inner_floor, int* outer, int* iters, double* relres);
+
 /* Flexible GMRES with FP64 refinement (no reference counterpart; the
  * reference solves with opGMRES only, opGMRES.hpp:68-269): outer FGMRES(restart)
  * in FP64 with the all-FP64 operator, right-preconditioned by an inner GMRES

@@ -158,3 +158,27 @@ def test_multi_rhs_gmres(precision):
             assert rr[j] < 1e-8 and rel(X[:, j], d) < 1e-7
         else:  # the mixed operator alone levels off near its FP32 rounding, as single GMRES does
             assert rr[j] < 1e-6 and rel(X[:, j], d) < 1e-5
+
+
+def test_multi_rhs_mixed_refinement():
+    """ifgf_refine_multi on a mixed-precision plan: twelve incidence directions
+    reach the FP64 true residual 1e-8 through lockstep mixed-precision GMRES
+    corrections and FP64 residuals, with densities within 1e-7 of the FP64
+    plan's lockstep solutions."""
+    kw = configs.scaled(I.kite(), 400, 40.0, 6)
+    f64 = I.ForwardMap(**kw, nrhs_max=8)
+    fmx = I.ForwardMap(**kw, precision="mixed", nrhs_max=8)
+    x, y, *_ = f64.points()
+    th = 2 * np.pi * np.arange(12) / 12
+    B = np.stack([-configs.plane_wave(x, y, 40.0, (np.cos(t), np.sin(t))) for t in th], axis=1)
+    X64, its64, rr64 = f64.solve_multi(B, restart=30, tol=1e-8, maxit=500)
+    X, outer, its, rr = fmx.solve_multi_refined(B, restart=30, tol=1e-8, maxit=500)
+    print("refinement steps", outer, "inner iterations", its.tolist(), "relres max", rr.max(),
+          "fp64 lockstep iterations", its64.tolist(), flush=True)
+    assert 1 <= outer <= 4
+    assert np.all(rr < 1e-8)
+    for j in range(12):
+        # the true residual checked against the FP64 operator here as well
+        r = B[:, j] - f64.compute_Ax_acc(X[:, j])
+        assert np.linalg.norm(r) / np.linalg.norm(B[:, j]) < 1e-8
+        assert rel(X[:, j], X64[:, j]) < 1e-7
