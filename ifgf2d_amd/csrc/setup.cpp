@@ -727,21 +727,26 @@ bool build_exclusions(const Geometry& g, const Tree& t, std::vector<int>& patch_
                       std::vector<int>& excl, long long& n_near, std::string& err) {
   const long long N = g.N;
   const int Np = g.Np;
-  std::vector<std::vector<int>> ex(N);
-  for (long long i = 0; i < N; i++) ex[i].push_back((int)(i / Np));
+  // own patch, then the patches the point is near-singular to, in patch order
+  // (patch order index i); exn[i] entries in ex[i * kMaxExcl ..]
+  std::vector<int> ex((size_t)N * kMaxExcl, -1), exn(N, 1);
+  for (long long i = 0; i < N; i++) ex[(size_t)i * kMaxExcl] = (int)(i / Np);
   for (int p = 0; p < g.npatches; p++)
-    for (long long q : g.ns_idx[p])
-      if (std::find(ex[q].begin(), ex[q].end(), p) == ex[q].end()) ex[q].push_back(p);
+    for (long long q : g.ns_idx[p]) {
+      int* e = &ex[(size_t)q * kMaxExcl];
+      if (std::find(e, e + exn[q], p) != e + exn[q]) continue;
+      if (exn[q] == kMaxExcl) {
+        err = "a point is near-singular to more patches than the near-field exclusion list holds";
+        return false;
+      }
+      e[exn[q]++] = p;
+    }
   patch_of_sorted.resize(N);
   excl.assign((size_t)N * kMaxExcl, -1);
   for (long long i = 0; i < N; i++) {
     const int s = t.inv[i];
     patch_of_sorted[s] = (int)(i / Np);
-    if ((int)ex[i].size() > kMaxExcl) {
-      err = "a point is near-singular to more patches than the near-field exclusion list holds";
-      return false;
-    }
-    for (size_t e = 0; e < ex[i].size(); e++) excl[(size_t)s * kMaxExcl + e] = ex[i][e];
+    std::copy(&ex[(size_t)i * kMaxExcl], &ex[(size_t)i * kMaxExcl] + exn[i], &excl[(size_t)s * kMaxExcl]);
   }
   // every excluded pair must be a leaf-level neighbour pair
   // (CheckIfSingandNearSingInNeighborhood, BoxTree.h:1079-1161)
@@ -751,18 +756,23 @@ bool build_exclusions(const Geometry& g, const Tree& t, std::vector<int>& patch_
     bx = (long long)morton_merge2((uint64_t)m);
     by = (long long)morton_merge2((uint64_t)m >> 1);
   };
+  bool far = false;
+#pragma omp parallel for schedule(static) reduction(|| : far)
   for (long long i = 0; i < N; i++) {
     long long ax, ay;
     leaf_xy(t.inv[i], ax, ay);
-    for (int p : ex[i])
+    for (int e = 0; e < exn[i]; e++) {
+      const int p = ex[(size_t)i * kMaxExcl + e];
       for (long long j = (long long)p * Np; j < (long long)(p + 1) * Np; j++) {
         long long bx, by;
         leaf_xy(t.inv[j], bx, by);
-        if (std::abs(ax - bx) > 1 || std::abs(ay - by) > 1) {
-          err = "Singular or near singular point not in the leaf neighbourhood (use fewer levels)";
-          return false;
-        }
+        if (std::abs(ax - bx) > 1 || std::abs(ay - by) > 1) far = true;
       }
+    }
+  }
+  if (far) {
+    err = "Singular or near singular point not in the leaf neighbourhood (use fewer levels)";
+    return false;
   }
   // near-field pair count (SingularInteractions, BoxTree.h:614-683)
   long long cnt = 0;
