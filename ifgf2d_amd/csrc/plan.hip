@@ -1,6 +1,7 @@
 // C ABI of the MI355X IFGF forward map (include/ifgf2d.h): plan creation,
 // device tables, the apply sequence and the auxiliary entry points.
 #include <hip/hip_runtime.h>
+#include <omp.h>
 
 #include <algorithm>
 #include <chrono>
@@ -828,13 +829,63 @@ static void build_stored(ifgf_plan* P) {
   // finalize a target's interpolations through it_slot.
   const long long npush = npr + nit;
   if (npush + 1 > 2147483647LL) throw std::runtime_error("too many propagation and interpolation entries");
-  std::vector<long long> off(t.n_cones + 1, 0);
-  for (long long e = 0; e < npr; e++) off[pr_cone[e] + 1]++;
-  for (long long e = 0; e < nit; e++) off[t.it_cone[e] + 1]++;
-  for (long long g = 0; g < t.n_cones; g++) off[g + 1] += off[g];
-  std::vector<long long> pos(off.begin(), off.end() - 1);
+  const long long NC = t.n_cones;
+  // Stable counting sort of the push entries by the cone that evaluates them:
+  // a cone's propagation entries in entry order, then its interpolation
+  // entries in entry order. Chunks of entries per thread: per-chunk counts,
+  // their prefix over (cone, chunk), then every chunk scatters its own.
+  const int nth = std::max(1, std::min(omp_get_max_threads(), 64));
+  std::vector<int> qprop(npr), qint(nit);  // push position of every entry
+  std::vector<long long> off(NC + 1, 0), npro(NC, 0), nint(NC, 0);
+  {
+    std::vector<std::vector<int>> hp(nth, std::vector<int>(NC, 0)), hi(nth, std::vector<int>(NC, 0));
+    auto chunk = [](long long n, int T, int c, long long& b0, long long& b1) {
+      b0 = n * c / T;
+      b1 = n * (c + 1) / T;
+    };
+#pragma omp parallel num_threads(nth)
+    {
+      const int c = omp_get_thread_num();
+      long long b0, b1;
+      chunk(npr, nth, c, b0, b1);
+      for (long long e = b0; e < b1; e++) hp[c][pr_cone[e]]++;
+      chunk(nit, nth, c, b0, b1);
+      for (long long e = b0; e < b1; e++) hi[c][t.it_cone[e]]++;
+    }
+    for (long long g = 0; g < NC; g++) {
+      long long sp = 0, si = 0;
+      for (int c = 0; c < nth; c++) {
+        sp += hp[c][g];
+        si += hi[c][g];
+      }
+      npro[g] = sp;
+      nint[g] = si;
+      off[g + 1] = off[g] + sp + si;
+    }
+    // hp / hi become each chunk's first position in the cone's run
+#pragma omp parallel for schedule(static) num_threads(nth)
+    for (long long g = 0; g < NC; g++) {
+      long long rp = off[g], ri = off[g] + npro[g];
+      for (int c = 0; c < nth; c++) {
+        const long long cp = hp[c][g], ci = hi[c][g];
+        hp[c][g] = (int)rp;
+        hi[c][g] = (int)ri;
+        rp += cp;
+        ri += ci;
+      }
+    }
+#pragma omp parallel num_threads(nth)
+    {
+      const int c = omp_get_thread_num();
+      long long b0, b1;
+      chunk(npr, nth, c, b0, b1);
+      for (long long e = b0; e < b1; e++) qprop[e] = hp[c][pr_cone[e]]++;
+      chunk(nit, nth, c, b0, b1);
+      for (long long e = b0; e < b1; e++) qint[e] = hi[c][t.it_cone[e]]++;
+    }
+  }
   std::vector<int> einfo(npush);
-  std::vector<int4> cidx((size_t)t.n_cones * Pn, make_int4(0, 0, 0, 0));
+  std::vector<int4> cidx((size_t)NC * Pn, make_int4(0, 0, 0, 0));
   // propagation geometry shared per (level, parent non-relevant id q, child
   // position cp): pair ids in order of first use, each gathered from its
   // first parent cone's entries
@@ -843,7 +894,8 @@ static void build_stored(ifgf_plan* P) {
   for (int l = 2; l < D; l++) pair_base[l + 1] = pair_base[l] + (long long)t.lv[l].nslot * 4;
   std::vector<int> pair_id(pair_base[D], -1);
   std::vector<long long> ptab_src;
-  for (long long g = 0; g < t.n_cones; g++) {
+  std::vector<int> gid((size_t)NC * 4, -1);  // pair id of parent cone g's k-th child
+  for (long long g = 0; g < NC; g++) {
     const int nch = (int)((pr_ptr[g + 1] - pr_ptr[g]) / Pn);
     const int l = t.cone_level[g];
     int kc = 0;
@@ -854,32 +906,41 @@ static void build_stored(ifgf_plan* P) {
         id = (int)(ptab_src.size() / Pn);
         for (int j = 0; j < Pn; j++) ptab_src.push_back(pr_ptr[g] + (long long)kc * Pn + j);
       }
-      for (int j = 0; j < Pn; j++) {
-        const long long e = pr_ptr[g] + (long long)kc * Pn + j;
-        const long long q = pos[pr_cone[e]]++;
-        einfo[q] = -1 - (id * Pn + j);
-        int* c = &cidx[(size_t)g * Pn + j].x;
-        c[kc] = (int)(1 + q);
-      }
+      gid[(size_t)g * 4 + kc] = id;
       kc++;
     }
   }
-  std::vector<int> it_slot(nit);
-  std::vector<long long> it_of_slot(npush, -1);
-  for (long long e = 0; e < nit; e++) {
-    const long long q = pos[t.it_cone[e]]++;
-    it_of_slot[q] = e;
-    it_slot[e] = (int)(1 + q);
-  }
-  // interpolation entries' geometry in push order
-  std::vector<long long> igeo_src;
-  igeo_src.reserve(nit);
-  for (long long q = 0; q < npush; q++)
-    if (it_of_slot[q] >= 0) {
-      einfo[q] = (int)igeo_src.size();
-      igeo_src.push_back(it_of_slot[q]);
+#pragma omp parallel for schedule(static) num_threads(nth)
+  for (long long g = 0; g < NC; g++) {
+    const int nch = (int)((pr_ptr[g + 1] - pr_ptr[g]) / Pn);
+    for (int kc = 0; kc < nch; kc++) {
+      const int id = gid[(size_t)g * 4 + kc];
+      for (int j = 0; j < Pn; j++) {
+        const long long e = pr_ptr[g] + (long long)kc * Pn + j;
+        const int q = qprop[e];
+        einfo[q] = -1 - (id * Pn + j);
+        int* c = &cidx[(size_t)g * Pn + j].x;
+        c[kc] = 1 + q;
+      }
     }
-  std::vector<long long>().swap(it_of_slot);
+  }
+  // interpolation entries: slot of each, and their geometry in push order (a
+  // cone's interpolation entries follow its propagation entries, so an
+  // entry's rank among all interpolation slots is its cone's base plus its
+  // place in the cone's run)
+  std::vector<int> it_slot(nit);
+  std::vector<long long> ibase(NC + 1, 0);
+  for (long long g = 0; g < NC; g++) ibase[g + 1] = ibase[g] + nint[g];
+  std::vector<long long> igeo_src(nit);
+#pragma omp parallel for schedule(static) num_threads(nth)
+  for (long long e = 0; e < nit; e++) {
+    const int q = qint[e];
+    const int g = t.it_cone[e];
+    const long long r = ibase[g] + (q - off[g] - npro[g]);
+    it_slot[e] = 1 + q;
+    einfo[q] = (int)r;
+    igeo_src[r] = e;
+  }
   // inverse node factors per (level, radial interval, s node), gathered from
   // the first parent cone of each (level, radial interval)
   std::vector<int> ifac_off(t.n_cones, 0);
