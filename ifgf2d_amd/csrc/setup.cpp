@@ -87,12 +87,14 @@ bool build_geometry(const Curve& c, const double* lims, int npatches, int Np, do
   for (auto* v : {&g.x, &g.y, &g.nx, &g.ny, &g.w, &g.t}) v->resize(g.N);
   // ForwardMap.tpp:75-157: node t = ab2cd(fejer_j, -1, 1, t1, t2), normal (y', -x'),
   // intensity factor (t2 - t1)/2 * w_j * |x'(t)| (compute_intensities, :425-435).
-  for (int p = 0; p < npatches; p++) {
-    const double a = lims[p], b = lims[p + 1];
-    if (!(b > a)) {
+  for (int p = 0; p < npatches; p++)
+    if (!(lims[p + 1] > lims[p])) {
       err = "patch limits must be increasing";
       return false;
     }
+#pragma omp parallel for schedule(static)
+  for (int p = 0; p < npatches; p++) {
+    const double a = lims[p], b = lims[p + 1];
     g.t1[p] = a;
     g.t2[p] = b;
     for (int j = 0; j < Np; j++) {
