@@ -129,7 +129,8 @@ def main():
 
     kw, d = configs.kite_k200() if args.workload == "kite" else configs.star_k500()
     t0 = time.time()
-    fm = I.ForwardMap(**kw, autotune=True)
+    # every setup step on the GPU (sort, near-singular search, correction, tree)
+    fm = I.ForwardMap(**kw, autotune=True, device_setup=True)
     t_setup = time.time() - t0
     kw["delta"] = fm.delta
     st = fm.stats()
