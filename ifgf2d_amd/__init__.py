@@ -76,7 +76,7 @@ EXPORTS = [
     "ifgf_morton_sort", "ifgf_apply_graph", "ifgf_group_create", "ifgf_group_launch",
     "ifgf_group_synchronize", "ifgf_group_destroy", "ifgf_gmres_multi", "ifgf_field_create",
     "ifgf_field_apply", "ifgf_field_stats", "ifgf_field_destroy", "ifgf_plan_setup_times",
-    "ifgf_refine_multi",
+    "ifgf_refine_multi", "ifgf_curve_eval", "ifgf_curve_diff",
 ]
 
 _LIB = None
@@ -130,6 +130,9 @@ def lib():
                                       ctypes.c_int, ctypes.c_double, ctypes.c_int, _ip, _dp]
         L.ifgf_hankel01.argtypes = [_dp, ctypes.c_longlong, _dp, _dp]
         L.ifgf_morton_sort.argtypes = [_dp, _dp, ctypes.c_longlong, ctypes.c_int, _lp, _lp]
+        L.ifgf_curve_eval.argtypes = [ctypes.c_int, _dp, ctypes.c_int, _dp, ctypes.c_longlong, _dp, _dp, _dp,
+                                      _dp]
+        L.ifgf_curve_diff.argtypes = [ctypes.c_int, _dp, ctypes.c_int, _dp, _dp, ctypes.c_longlong, _dp, _dp]
         L.ifgf_plan_profile.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.ifgf_plan_stage_times.argtypes = [ctypes.c_void_p, _dp, _lp]
         L.ifgf_plan_setup_times.argtypes = [ctypes.c_void_p, _dp]
@@ -187,6 +190,31 @@ def _curve_eval(curve, t):
     for m in range(n):
         r = r + p[4 + m] * np.cos(m * t) + p[4 + n + m] * np.sin(m * t)
     return p[0] + p[2] * r * np.cos(t), p[1] + p[2] * r * np.sin(t)
+
+
+def curve_eval(curve, t):
+    """(x, y, x', y') of a curve at the parameters t (Curve::xt/yt/xpt/ypt, host)."""
+    typ, p = curve
+    prm = np.ascontiguousarray(p, dtype=np.float64)
+    t = np.ascontiguousarray(t, dtype=np.float64)
+    out = [np.empty_like(t) for _ in range(4)]
+    rc = lib().ifgf_curve_eval(typ, _d(prm), prm.size, _d(t), t.size, *[_d(o) for o in out])
+    if rc != IFGF_OK:
+        raise IFGFError(lib().ifgf_last_error().decode())
+    return tuple(out)
+
+
+def curve_diff(curve, a, d):
+    """(x(a) - x(a - d), y(a) - y(a - d)) from difference identities (host)."""
+    typ, p = curve
+    prm = np.ascontiguousarray(p, dtype=np.float64)
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    d = np.ascontiguousarray(d, dtype=np.float64)
+    dx, dy = np.empty_like(a), np.empty_like(a)
+    rc = lib().ifgf_curve_diff(typ, _d(prm), prm.size, _d(a), _d(d), a.size, _d(dx), _d(dy))
+    if rc != IFGF_OK:
+        raise IFGFError(lib().ifgf_last_error().decode())
+    return dx, dy
 
 
 def default_delta(curve, lims, frac=0.75):

@@ -843,9 +843,10 @@ static void build_stored(ifgf_plan* P) {
       b0 = n * c / T;
       b1 = n * (c + 1) / T;
     };
-#pragma omp parallel num_threads(nth)
-    {
-      const int c = omp_get_thread_num();
+    // chunk c is a loop index, not a thread number: every chunk is counted
+    // and scattered whatever team size the runtime grants
+#pragma omp parallel for schedule(static) num_threads(nth)
+    for (int c = 0; c < nth; c++) {
       long long b0, b1;
       chunk(npr, nth, c, b0, b1);
       for (long long e = b0; e < b1; e++) hp[c][pr_cone[e]]++;
@@ -874,9 +875,8 @@ static void build_stored(ifgf_plan* P) {
         ri += ci;
       }
     }
-#pragma omp parallel num_threads(nth)
-    {
-      const int c = omp_get_thread_num();
+#pragma omp parallel for schedule(static) num_threads(nth)
+    for (int c = 0; c < nth; c++) {
       long long b0, b1;
       chunk(npr, nth, c, b0, b1);
       for (long long e = b0; e < b1; e++) qprop[e] = hp[c][pr_cone[e]]++;
@@ -1682,6 +1682,31 @@ int ifgf_hankel01(const double* x, long long n, double* h0, double* h1) {
   } catch (std::exception& e) {
     return fail(nullptr, IFGF_ERR_HIP, e.what());
   }
+  return IFGF_OK;
+}
+
+int ifgf_curve_eval(int curve_type, const double* params, int nparams, const double* t, long long n, double* x,
+                    double* y, double* xp, double* yp) {
+  if (!params || (n > 0 && !t) || n < 0) return fail(nullptr, IFGF_ERR_ARG, "null argument");
+  Curve c;
+  std::string err;
+  if (!make_curve(curve_type, params, nparams, c, err)) return fail(nullptr, IFGF_ERR_ARG, err);
+  for (long long i = 0; i < n; i++) {
+    if (x) x[i] = c.x(t[i]);
+    if (y) y[i] = c.y(t[i]);
+    if (xp) xp[i] = c.xp(t[i]);
+    if (yp) yp[i] = c.yp(t[i]);
+  }
+  return IFGF_OK;
+}
+
+int ifgf_curve_diff(int curve_type, const double* params, int nparams, const double* a, const double* d,
+                    long long n, double* dx, double* dy) {
+  if (!params || n < 0 || (n > 0 && (!a || !d || !dx || !dy))) return fail(nullptr, IFGF_ERR_ARG, "null argument");
+  Curve c;
+  std::string err;
+  if (!make_curve(curve_type, params, nparams, c, err)) return fail(nullptr, IFGF_ERR_ARG, err);
+  for (long long i = 0; i < n; i++) c.diff(a[i], d[i], dx[i], dy[i]);
   return IFGF_OK;
 }
 

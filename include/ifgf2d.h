@@ -318,6 +318,20 @@ int ifgf_gmres_host(ifgf_apply_fn apply, void* ctx, long long n, const double* r
  * h0 / h1 receive n interleaved complex values. */
 int ifgf_hankel01(const double* x, long long n, double* h0, double* h1);
 
+/* The discretised curve's parametrisation on the host: replaces Curve::xt,
+ * yt, xpt, ypt (Geometry/Curve.hpp:97-146, Curve.cpp:105-138; the star has no
+ * reference class) at t[0..n); any output may be NULL. curve_type / params as
+ * in ifgf_surface. */
+int ifgf_curve_eval(int curve_type, const double* params, int nparams, const double* t, long long n, double* x,
+                    double* y, double* xp, double* yp);
+
+/* (x(a) - x(a - d), y(a) - y(a - d)) at n pairs, formed with trigonometric
+ * difference identities (accurate relative to the difference itself); the
+ * reference subtracts Cartesian coordinates in the RP precomputation
+ * (ForwardMap.tpp:355-365), which this replaces. */
+int ifgf_curve_diff(int curve_type, const double* params, int nparams, const double* a, const double* d,
+                    long long n, double* dx, double* dy);
+
 /* Per-stage GPU timing of applications: hipEvents recorded on the stream each
  * application's kernels run on, without host synchronisation (the hipEvent
  * timers that replace the reference's std::chrono prints,

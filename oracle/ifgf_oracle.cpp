@@ -1492,6 +1492,35 @@ static Curve make_curve(int type, const double* prm, int nprm) {
   return c;
 }
 
+// Curve::xt / yt / xpt / ypt and Curve::diff at n parameters (Geometry/Curve.hpp:97-146).
+int oracle_curve_eval(int type, const double* prm, int nprm, const double* t, long long n, double* x, double* y,
+                      double* xp, double* yp) {
+  try {
+    const Curve c = make_curve(type, prm, nprm);
+    for (ll i = 0; i < n; i++) {
+      x[i] = c.xt(t[i]);
+      y[i] = c.yt(t[i]);
+      xp[i] = c.xpt(t[i]);
+      yp[i] = c.ypt(t[i]);
+    }
+  } catch (std::exception& e) {
+    g_err = e.what();
+    return 1;
+  }
+  return 0;
+}
+int oracle_curve_diff(int type, const double* prm, int nprm, const double* a, const double* d, long long n,
+                      double* dx, double* dy) {
+  try {
+    const Curve c = make_curve(type, prm, nprm);
+    for (ll i = 0; i < n; i++) c.diff(a[i], d[i], dx[i], dy[i]);
+  } catch (std::exception& e) {
+    g_err = e.what();
+    return 1;
+  }
+  return 0;
+}
+
 int oracle_patch_lims(int type, const double* prm, int nprm, double wl_per_patch, double k,
                       double* out, int cap) {
   try {

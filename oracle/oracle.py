@@ -37,6 +37,10 @@ def lib():
                                     ctypes.c_double, _dp, _dp]
         L.oracle_patch_lims.argtypes = [ctypes.c_int, _dp, ctypes.c_int, ctypes.c_double,
                                         ctypes.c_double, _dp, ctypes.c_int]
+        L.oracle_curve_eval.argtypes = [ctypes.c_int, _dp, ctypes.c_int, _dp, ctypes.c_longlong, _dp, _dp,
+                                        _dp, _dp]
+        L.oracle_curve_diff.argtypes = [ctypes.c_int, _dp, ctypes.c_int, _dp, _dp, ctypes.c_longlong,
+                                        _dp, _dp]
         L.oracle_fm_create.argtypes = [ctypes.c_int, _dp, ctypes.c_int, _dp, ctypes.c_int,
                                        ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                        ctypes.c_double, ctypes.c_double, ctypes.c_double,
@@ -102,6 +106,27 @@ def rp_cov(which, t, alpha, p):
     xi, dxi = np.empty_like(t), np.empty_like(t)
     lib().oracle_rp_cov(which, _d(t), t.size, alpha, p, _d(xi), _d(dxi))
     return xi, dxi
+
+
+def curve_eval(curve_type, params, t):
+    """(x, y, x', y') of the oracle's Curve at the parameters t."""
+    prm = np.ascontiguousarray(params, dtype=np.float64)
+    t = np.ascontiguousarray(t, dtype=np.float64)
+    out = [np.empty_like(t) for _ in range(4)]
+    if lib().oracle_curve_eval(curve_type, _d(prm), prm.size, _d(t), t.size, *[_d(o) for o in out]):
+        raise RuntimeError(_err())
+    return tuple(out)
+
+
+def curve_diff(curve_type, params, a, d):
+    """(x(a) - x(a - d), y(a) - y(a - d)) by the oracle's Curve::diff."""
+    prm = np.ascontiguousarray(params, dtype=np.float64)
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    d = np.ascontiguousarray(d, dtype=np.float64)
+    dx, dy = np.empty_like(a), np.empty_like(a)
+    if lib().oracle_curve_diff(curve_type, _d(prm), prm.size, _d(a), _d(d), a.size, _d(dx), _d(dy)):
+        raise RuntimeError(_err())
+    return dx, dy
 
 
 def patch_lims(curve_type, params, wavelengths_per_patch, k):
