@@ -57,11 +57,11 @@ def stage_work(st, formulation, P, Ps, Pang, stored):
     return w
 
 
-def timed_solve(fn, reps=3):
-    """(result of the last call, cold seconds, median warm seconds): the first
-    call carries the one-time costs (first launches of the solver kernels, the
-    first Krylov workspace allocations); the median of the next reps calls is
-    the steady-state time to solve."""
+def timed_solve(fn, reps=5):
+    """(result of the last call, cold seconds, warm seconds): the first call
+    carries the one-time costs (first launches of the solver kernels, the
+    workspaces the plan then keeps); the next reps calls are warm. Warm
+    seconds: {"min", "median", "max"} over those calls."""
     import torch
     ts = []
     for _ in range(reps + 1):
@@ -69,25 +69,40 @@ def timed_solve(fn, reps=3):
         t0 = time.perf_counter()
         out = fn()
         ts.append(time.perf_counter() - t0)
-    return out, ts[0], sorted(ts[1:])[len(ts[1:]) // 2]
+    warm = sorted(ts[1:])
+    return out, ts[0], {"min": round(warm[0], 5), "median": round(warm[len(warm) // 2], 5),
+                        "max": round(warm[-1], 5)}
 
 
 def cpu_baseline(kw, x, name):
-    """The oracle (single-threaded CPU restatement, Amos Hankel) on one
-    forward-map application of the same workload; oracle setup excluded."""
+    """The oracle (CPU restatement, the reference's Amos Hankel) on one
+    forward-map application of the same workload, oracle setup excluded: once
+    on all the host threads OpenMP is given (OMP_NUM_THREADS; the count is
+    reported) and once on one core. Returns (record, the oracle's output):
+    the output is the full-size parity check of the GPU operator."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
+    threads = O.set_threads(0)
     t0 = time.time()
     om = O.ForwardMap(kw["curve"][0], kw["curve"][1], kw["patch_lims"], formulation=O.COMBINED,
                       k=kw["k"], eta=kw["k"], delta=kw["delta"], p=kw["p"], nlevels=kw["nlevels"])
-    log(f"cpu baseline: oracle setup {time.time() - t0:.1f} s")
+    log(f"cpu baseline: oracle setup {time.time() - t0:.1f} s on {threads} threads")
     t0 = time.time()
-    om.apply(x)
+    y = om.apply(x)
     dt = time.time() - t0
-    log(f"cpu baseline: one oracle application {dt:.2f} s")
-    return {"value": 1.0 / dt, "unit": "matvecs/s", "cores": 1, "kind": "port",
-            "sample": f"1 forward-map application (combined field, 5x5) of the {name} by the "
-                      "single-threaded oracle with the reference's Amos Hankel; oracle setup excluded"}
+    log(f"cpu baseline: one oracle application {dt:.2f} s on {threads} threads")
+    O.set_threads(1)
+    t0 = time.time()
+    y1 = om.apply(x)
+    dt1 = time.time() - t0
+    O.set_threads(threads)
+    log(f"cpu baseline: one oracle application {dt1:.2f} s on 1 thread (same bits: {bool((y1 == y).all())})")
+    rec = {"value": 1.0 / dt, "unit": "matvecs/s", "cores": threads, "kind": "port",
+           "sample": f"1 forward-map application (combined field, 5x5) of the {name} by the oracle "
+                     f"(OpenMP over targets and cones, {threads} threads) with the reference's Amos Hankel; "
+                     "oracle setup excluded",
+           "one_core": {"value": 1.0 / dt1, "cores": 1}}
+    return rec, y
 
 
 def main():
@@ -255,9 +270,10 @@ def main():
         rhs = -configs.plane_wave(xg, yg, kw["k"], d)
         (dens, it, relres, hist), t_cold, t_solve = timed_solve(
             lambda: fm.solve(rhs, restart=50, tol=1e-8, maxit=2000))
-        solve = {"time_to_solve_s": round(t_solve, 4), "first_call_s": round(t_cold, 4), "gmres_iterations": it,
-                 "true_relres": relres, "restart": 50, "tol": 1e-8}
-        log(f"GMRES: {it} iterations, {t_solve:.3f} s, true relres {relres:.2e}")
+        solve = {"time_to_solve_s": t_solve["median"], "warm_s": t_solve, "first_call_s": round(t_cold, 4),
+                 "gmres_iterations": it, "true_relres": relres, "restart": 50, "tol": 1e-8,
+                 "last_call_phases_s": {k_: round(v, 5) for k_, v in fm.solver_times().items()}}
+        log(f"GMRES: {it} iterations, {t_solve} s warm, true relres {relres:.2e}, phases {fm.solver_times()}")
 
     # mixed precision: FP32 leaf operator, cone coefficients, propagation and
     # interpolation (FP64 near field, correction, accumulation), and the
@@ -281,10 +297,12 @@ def main():
         if not args.no_solve:
             (dm, oit, iit, rr, _), t_fc, t_f = timed_solve(
                 lambda: fmx.solve_fgmres(rhs, restart=20, tol=1e-8, inner_restart=60))
-            mixed["fgmres"] = {"time_to_solve_s": round(t_f, 4), "first_call_s": round(t_fc, 4), "outer_iterations": oit,
-                               "inner_iterations": iit, "true_relres": rr,
-                               "density_rel_diff_vs_fp64_gmres": float(np.linalg.norm(dm - dens) / np.linalg.norm(dens))}
-            log(f"mixed FGMRES: {oit} outer + {iit} inner iterations, {t_f:.4f} s, true relres {rr:.2e}")
+            mixed["fgmres"] = {"time_to_solve_s": t_f["median"], "warm_s": t_f, "first_call_s": round(t_fc, 4),
+                               "outer_iterations": oit, "inner_iterations": iit, "true_relres": rr,
+                               "density_rel_diff_vs_fp64_gmres": float(np.linalg.norm(dm - dens) / np.linalg.norm(dens)),
+                               "last_call_phases_s": {k_: round(v, 5) for k_, v in fmx.solver_times().items()}}
+            log(f"mixed FGMRES: {oit} outer + {iit} inner iterations, {t_f} s warm, true relres {rr:.2e}, "
+                f"phases {fmx.solver_times()}")
         del fmx
 
     # several right-hand sides per application: 64 columns in blocks of 8
@@ -313,11 +331,12 @@ def main():
                 B = np.stack([-configs.plane_wave(xg, yg, kw["k"], (np.cos(t_), np.sin(t_))) for t_ in th], axis=1)
                 (X64, its, rr), t_mc, t_m = timed_solve(lambda: fb.solve_multi(B, restart=50, tol=1e-8, maxit=2000),
                                                         reps=2)
-                batched["solve_64_directions"] = {"time_to_solve_s": round(t_m, 4), "first_call_s": round(t_mc, 4),
+                batched["solve_64_directions"] = {"time_to_solve_s": t_m["median"], "warm_s": t_m,
+                                                  "first_call_s": round(t_mc, 4),
                                                   "iterations_min": int(its.min()),
                                                   "iterations_max": int(its.max()),
                                                   "true_relres_max": float(rr.max())}
-                log(f"64 directions solved together: {t_m:.3f} s, iterations {its.min()}..{its.max()}, "
+                log(f"64 directions solved together: {t_m} s, iterations {its.min()}..{its.max()}, "
                     f"max true relres {rr.max():.2e}")
             if prec == "mixed" and not args.no_solve and "solve_64_directions" in batched:
                 # the same 64 directions: lockstep mixed-precision GMRES corrections,
@@ -326,10 +345,11 @@ def main():
                     lambda: fb.solve_multi_refined(B, restart=50, tol=1e-8, maxit=2000), reps=2)
                 dmax = max(float(np.linalg.norm(Xm[:, j] - X64[:, j]) / np.linalg.norm(X64[:, j])) for j in range(64))
                 batched["solve_64_directions_mixed_refined"] = {
-                    "time_to_solve_s": round(t_m, 4), "first_call_s": round(t_mc, 4), "refinement_steps": outer,
+                    "time_to_solve_s": t_m["median"], "warm_s": t_m, "first_call_s": round(t_mc, 4),
+                    "refinement_steps": outer,
                     "inner_iterations_min": int(its.min()), "inner_iterations_max": int(its.max()),
                     "true_relres_max": float(rr.max()), "density_rel_diff_vs_fp64_max": dmax}
-                log(f"64 directions, mixed refinement: {t_m:.3f} s, {outer} steps, inner iterations "
+                log(f"64 directions, mixed refinement: {t_m} s, {outer} steps, inner iterations "
                     f"{its.min()}..{its.max()}, max true relres {rr.max():.2e}, density diff {dmax:.1e}")
             del fb
         del Xb, Yb
@@ -405,8 +425,14 @@ def main():
             f.close()
             del out_d
 
-    cpu = None if args.no_cpu_baseline else cpu_baseline(
-        kw, x_host, "64,000-point kite at k=200" if args.workload == "kite" else "256,000-point star at k=500")
+    cpu = parity = None
+    if not args.no_cpu_baseline:
+        cpu, y_oracle = cpu_baseline(
+            kw, x_host, "64,000-point kite at k=200" if args.workload == "kite" else "256,000-point star at k=500")
+        # the timed operator against the oracle at full size (same density)
+        parity = {"N": fm.N, "bar": 1e-12,
+                  "rel_diff": float(np.linalg.norm(y_acc - y_oracle) / np.linalg.norm(y_oracle))}
+        log(f"GPU FP64 operator vs oracle at N={fm.N}: {parity['rel_diff']:.2e}")
 
     out = {
         "metric": "time-to-solve (rel. residual 1e-8) + IFGF matvecs/s vs plain FP64 on GPU",
@@ -433,9 +459,11 @@ def main():
         "field": field,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "oracle_parity": parity,
     }
     if cpu:
         out["gpu_over_cpu"] = round(value / cpu["value"], 1)
+        out["gpu_over_cpu_one_core"] = round(value / cpu["one_core"]["value"], 1)
     print(json.dumps(out), flush=True)
 
 

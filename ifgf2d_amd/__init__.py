@@ -76,7 +76,8 @@ EXPORTS = [
     "ifgf_morton_sort", "ifgf_apply_graph", "ifgf_group_create", "ifgf_group_launch",
     "ifgf_group_synchronize", "ifgf_group_destroy", "ifgf_gmres_multi", "ifgf_field_create",
     "ifgf_field_apply", "ifgf_field_stats", "ifgf_field_destroy", "ifgf_plan_setup_times",
-    "ifgf_refine_multi", "ifgf_curve_eval", "ifgf_curve_diff",
+    "ifgf_refine_multi", "ifgf_curve_eval", "ifgf_curve_diff", "ifgf_plan_solver_times",
+    "ifgf_propagate_noalloc",
 ]
 
 _LIB = None
@@ -133,6 +134,9 @@ def lib():
         L.ifgf_curve_eval.argtypes = [ctypes.c_int, _dp, ctypes.c_int, _dp, ctypes.c_longlong, _dp, _dp, _dp,
                                       _dp]
         L.ifgf_curve_diff.argtypes = [ctypes.c_int, _dp, ctypes.c_int, _dp, _dp, ctypes.c_longlong, _dp, _dp]
+        L.ifgf_plan_solver_times.argtypes = [ctypes.c_void_p, _dp]
+        L.ifgf_propagate_noalloc.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_longlong,
+                                             ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         L.ifgf_plan_profile.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.ifgf_plan_stage_times.argtypes = [ctypes.c_void_p, _dp, _lp]
         L.ifgf_plan_setup_times.argtypes = [ctypes.c_void_p, _dp]
@@ -388,6 +392,14 @@ class ForwardMap:
                                          _d(out.view(np.float64))), "ifgf_propagate")
         return out
 
+    def propagate_noalloc_device(self, tx_ptr, ty_ptr, nt, density_ptr, out_ptr, stream=None):
+        """propagate_unacc on device pointers through the library's fixed scratch
+        (ifgf_propagate_noalloc: allocates nothing; asynchronous)."""
+        self._check(lib().ifgf_propagate_noalloc(self._h, ctypes.c_void_p(tx_ptr), ctypes.c_void_p(ty_ptr), nt,
+                                                 ctypes.c_void_p(density_ptr), ctypes.c_void_p(out_ptr),
+                                                 ctypes.c_void_p(stream) if stream else None),
+                    "ifgf_propagate_noalloc")
+
     def field(self, tx, ty, nlevels=None, Ps=5, Pang=5, precision="fp64"):
         """IFGF-accelerated field evaluator at the points (tx, ty): the sum of
         propagate_unacc (ForwardMap.tpp:702-727) through an IFGF tree of its own.
@@ -477,6 +489,12 @@ class ForwardMap:
         self._check(lib().ifgf_plan_setup_times(self._h, _d(t)), "setup_times")
         return dict(zip(("geometry", "correction", "tree", "exclusions", "device_tables", "stored_operators",
                          "autotune", "total"), t.tolist()))
+
+    def solver_times(self):
+        """Seconds of the last solve/solve_fgmres call by phase."""
+        t = np.zeros(4)
+        self._check(lib().ifgf_plan_solver_times(self._h, _d(t)), "solver_times")
+        return dict(zip(("workspace", "host_waits", "inner_cycles", "total"), t.tolist()))
 
     def linear_operator(self):
         """The LinearOperator that ForwardMap::solve builds (ForwardMap.tpp:673-682)."""

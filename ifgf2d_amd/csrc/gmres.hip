@@ -20,8 +20,10 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <complex>
+#include <memory>
 #include <vector>
 
 #include "plan_internal.h"
@@ -386,7 +388,8 @@ struct Krylov {
   zd* hpin_dev = nullptr;  // hpin as a device pointer
   hipEvent_t ev[2] = {nullptr, nullptr};
   int m;
-  Krylov(long long n, int m_) : m(m_) {
+  long long n;
+  Krylov(long long n_, int m_) : m(m_), n(n_) {
     V.alloc((size_t)n * (m + 1));
     w.alloc(n);
     tmp.alloc(n);
@@ -407,12 +410,23 @@ struct Krylov {
   Krylov& operator=(const Krylov&) = delete;
 };
 
+using Clock = std::chrono::steady_clock;
+inline double seconds_since(Clock::time_point t0) {
+  return std::chrono::duration<double>(Clock::now() - t0).count();
+}
+
 struct Solver {
   ifgf_plan* P;
   hipStream_t st;
   long long n;
   int nb;
+  double wait_s = 0.0;  // host time blocked on the device (event / stream synchronisations)
   Solver(ifgf_plan* p) : P(p), st(p->stream), n(p->geo.N), nb((int)((p->geo.N + 255) / 256)) {}
+  void sync() {
+    const auto t0 = Clock::now();
+    HIP_TRY(hipStreamSynchronize(st));
+    wait_s += seconds_since(t0);
+  }
 
   void apply(const zd* x, zd* y, bool fp64) {
     const int rc = apply_device(P, x, y, st, fp64);
@@ -424,7 +438,7 @@ struct Solver {
     hipLaunchKernelGGL(k_sum0, dim3(1), dim3(64), 0, st, K.part.p, K.h.p);
     zd h0;
     HIP_TRY(hipMemcpyAsync(&h0, K.h.p, sizeof(zd), hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
+    sync();
     return std::sqrt(h0.x);
   }
   // dst = src / ||src|| (asynchronous)
@@ -464,7 +478,9 @@ struct Solver {
   }
   // waits for iteration j's Hessenberg column
   void arnoldi_wait(Krylov& K, int j, std::vector<std::complex<double>>& hv) {
+    const auto t0 = Clock::now();
     HIP_TRY(hipEventSynchronize(K.ev[j % 2]));
+    wait_s += seconds_since(t0);
     const zd* slot = K.hpin + (size_t)(j % 2) * (K.m + 2);
     hv.resize(j + 2);
     for (int i = 0; i < j + 2; i++) hv[i] = std::complex<double>(slot[i].x, slot[i].y);
@@ -475,6 +491,43 @@ struct Solver {
     hipLaunchKernelGGL(k_update, dim3(nb), dim3(256), 0, st, n, kk, B, n, K.h.p, -1.0, x);
   }
 };
+
+}  // namespace
+
+// Device workspaces of the single-system solvers, kept by the plan between
+// calls so that a warm solve allocates nothing (hipMalloc / hipHostMalloc of
+// tens of MB per call cost milliseconds and varied from run to run): the
+// outer and inner Krylov spaces (reused while their restart is at least the
+// one asked for), the right-hand side, the iterate and FGMRES's
+// preconditioned directions Z.
+struct ifgf::SolverCache {
+  std::unique_ptr<Krylov> outer, inner;
+  DevBuf<zd> b, x, Z;
+  int zcols = 0;
+};
+
+void ifgf::destroy_solver_cache(SolverCache* c) { delete c; }
+
+namespace {
+
+Krylov& cached_krylov(std::unique_ptr<Krylov>& k, long long n, int m) {
+  if (!k || k->n != n || k->m < m) {
+    k.reset();
+    k.reset(new Krylov(n, m));
+  }
+  return *k;
+}
+
+SolverCache& solver_cache(ifgf_plan* P) {
+  if (!P->solver) P->solver = new SolverCache();
+  SolverCache& c = *P->solver;
+  const long long n = P->geo.N;
+  if ((long long)c.b.n != n) {
+    c.b.alloc(n);
+    c.x.alloc(n);
+  }
+  return c;
+}
 
 // Upper-triangular solve R y = g (R = the rotated Hessenberg, row stride m).
 std::vector<std::complex<double>> back_substitute(const std::vector<std::complex<double>>& H, int m,
@@ -567,12 +620,14 @@ extern "C" int ifgf_gmres(ifgf_plan* P, const double* rhs, double* x, int restar
   if (!P->on_device) return plan_fail(P, IFGF_ERR_NO_DEVICE, "plan was built host-only");
   const long long n = P->geo.N;
   *iters = 0;
+  const auto t_start = Clock::now();
   try {
     Solver S(P);
-    Krylov K(n, restart);
-    DevBuf<zd> b, xd;
-    b.alloc(n);
-    xd.alloc(n);
+    SolverCache& W = solver_cache(P);
+    Krylov& K = cached_krylov(W.outer, n, restart);
+    DevBuf<zd>& b = W.b;
+    DevBuf<zd>& xd = W.x;
+    P->solver_s[0] = seconds_since(t_start);
     HIP_TRY(hipMemcpyAsync(b.p, rhs, n * sizeof(zd), hipMemcpyHostToDevice, S.st));
     HIP_TRY(hipMemcpyAsync(xd.p, x, n * sizeof(zd), hipMemcpyHostToDevice, S.st));
     const double bnorm = S.norm(K, b.p);
@@ -590,7 +645,10 @@ extern "C" int ifgf_gmres(ifgf_plan* P, const double* rhs, double* x, int restar
     S.residual(b.p, xd.p, K.tmp.p, true);
     *relres = S.norm(K, K.tmp.p) / bnorm;
     HIP_TRY(hipMemcpyAsync(x, xd.p, n * sizeof(zd), hipMemcpyDeviceToHost, S.st));
-    HIP_TRY(hipStreamSynchronize(S.st));
+    S.sync();
+    P->solver_s[1] = S.wait_s;
+    P->solver_s[2] = 0.0;
+    P->solver_s[3] = seconds_since(t_start);
     return converged ? IFGF_OK : IFGF_ERR_NOT_CONVERGED;
   } catch (std::exception& e) {
     return plan_fail(P, IFGF_ERR_HIP, e.what());
@@ -610,13 +668,21 @@ extern "C" int ifgf_fgmres(ifgf_plan* P, const double* rhs, double* x, int resta
   const int m = restart;
   *outer_iters = 0;
   *inner_iters = 0;
+  const auto t_start = Clock::now();
   try {
     Solver S(P);
-    Krylov K(n, m), Ki(n, inner_restart);
-    DevBuf<zd> b, xd, Z;
-    b.alloc(n);
-    xd.alloc(n);
-    Z.alloc((size_t)n * m);
+    SolverCache& W = solver_cache(P);
+    Krylov& K = cached_krylov(W.outer, n, m);
+    Krylov& Ki = cached_krylov(W.inner, n, inner_restart);
+    DevBuf<zd>& b = W.b;
+    DevBuf<zd>& xd = W.x;
+    if (W.zcols < m || (long long)W.Z.n != (long long)n * W.zcols) {
+      W.Z.alloc((size_t)n * m);
+      W.zcols = m;
+    }
+    DevBuf<zd>& Z = W.Z;
+    P->solver_s[0] = seconds_since(t_start);
+    double inner_s = 0.0;
     HIP_TRY(hipMemcpyAsync(b.p, rhs, n * sizeof(zd), hipMemcpyHostToDevice, S.st));
     HIP_TRY(hipMemcpyAsync(xd.p, x, n * sizeof(zd), hipMemcpyHostToDevice, S.st));
     const double bnorm = S.norm(K, b.p);
@@ -651,8 +717,10 @@ extern "C" int ifgf_fgmres(ifgf_plan* P, const double* rhs, double* x, int resta
         HIP_TRY(hipMemsetAsync(zj, 0, n * sizeof(zd), S.st));
         double iest = 0.0;
         bool iconv = false;
+        const auto t_in = Clock::now();
         inner_total += gmres_core(S, Ki, false, K.V.p + (size_t)j * n, zj, inner_restart, tau, inner_restart, &iest,
                                   nullptr, 0, &iconv, true);
+        inner_s += seconds_since(t_in);
         // w = A_fp64 z_j, orthogonalised against V (FP64 Arnoldi)
         S.apply(zj, K.w.p, true);
         S.arnoldi_enqueue(K, j);
@@ -683,7 +751,10 @@ extern "C" int ifgf_fgmres(ifgf_plan* P, const double* rhs, double* x, int resta
     *outer_iters = it;
     *inner_iters = inner_total;
     HIP_TRY(hipMemcpyAsync(x, xd.p, n * sizeof(zd), hipMemcpyDeviceToHost, S.st));
-    HIP_TRY(hipStreamSynchronize(S.st));
+    S.sync();
+    P->solver_s[1] = S.wait_s;
+    P->solver_s[2] = inner_s;
+    P->solver_s[3] = seconds_since(t_start);
     return converged ? IFGF_OK : IFGF_ERR_NOT_CONVERGED;
   } catch (std::exception& e) {
     return plan_fail(P, IFGF_ERR_HIP, e.what());
@@ -994,4 +1065,10 @@ extern "C" int ifgf_refine_multi(ifgf_plan* P, const double* rhs, double* x, int
   } catch (std::exception& e) {
     return plan_fail(P, IFGF_ERR_HIP, e.what());
   }
+}
+
+extern "C" int ifgf_plan_solver_times(const ifgf_plan* P, double seconds[4]) {
+  if (!P || !seconds) return plan_fail(const_cast<ifgf_plan*>(P), IFGF_ERR_ARG, "null argument");
+  for (int i = 0; i < 4; i++) seconds[i] = P->solver_s[i];
+  return IFGF_OK;
 }

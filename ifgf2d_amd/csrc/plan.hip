@@ -9,6 +9,7 @@
 #include <complex>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -1682,6 +1683,84 @@ int ifgf_hankel01(const double* x, long long n, double* h0, double* h1) {
   } catch (std::exception& e) {
     return fail(nullptr, IFGF_ERR_HIP, e.what());
   }
+  return IFGF_OK;
+}
+
+}  // extern "C"
+
+// ifgf_propagate_noalloc's scratch: the intensity-weighted density of the
+// plan's points in Morton order, 2 doubles per point, for at most
+// kNoAllocMaxPoints points (16 MiB on every device the library runs on, held
+// until the process ends). Host code never passes this array's name to a
+// kernel: its device address comes from hipGetSymbolAddress, once per device.
+namespace {
+constexpr long long kNoAllocMaxPoints = 1ll << 20;
+__device__ __align__(256) double scratch[2 * kNoAllocMaxPoints];
+std::mutex g_scratch_mu;
+constexpr int kMaxDevices = 64;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The current device's address of scratch (looked up on the device's first call).
+int scratch_address(ifgf::zd** out, std::string& err) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+    err = std::string("hipGetDevice: ") + hipGetErrorString(e);
+    return IFGF_ERR_HIP;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+      return IFGF_ERR_HIP;
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  *out = (ifgf::zd*)g_scratch_ptr[dev];
+  return IFGF_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int ifgf_propagate_noalloc(ifgf_plan* P, const double* tx, const double* ty, long long nt, const double* density,
+                           double* out, void* stream) {
+  if (!P || (nt > 0 && (!tx || !ty || !out)) || !density) return fail(P, IFGF_ERR_ARG, "null argument");
+  if (nt < 0 || nt > 2147483647ll) return fail(P, IFGF_ERR_ARG, "target count out of range");
+  if (!P->on_device || P->field) return fail(P, IFGF_ERR_NO_DEVICE, "plan was built host-only or is a field");
+  const long long N = P->geo.N;
+  if (N > kNoAllocMaxPoints)
+    return fail(P, IFGF_ERR_ARG, "plan has " + std::to_string(N) + " points; ifgf_propagate_noalloc's scratch holds " +
+                                     std::to_string(kNoAllocMaxPoints));
+  if (nt == 0) return IFGF_OK;
+  ifgf::zd* rho = nullptr;
+  std::string err;
+  if (const int rc = scratch_address(&rho, err)) return fail(P, rc, err);
+  hipStream_t st = stream ? (hipStream_t)stream : P->stream;
+  // every scratch entry the direct sum reads is written by the gather first
+  hipLaunchKernelGGL(k_gather, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, (int)N, (const zd*)density, P->w.p,
+                     P->perm.p, rho);
+  const int nb = (int)((nt + 255) / 256);
+  const double k = P->prm.k, eta = P->prm.eta;
+  const zd* r = rho;
+  zd* o = (zd*)out;
+  switch (P->prm.formulation) {
+    case kSingleLayer:
+      hipLaunchKernelGGL(k_propagate<kSingleLayer>, dim3(nb), dim3(256), 0, st, (int)nt, tx, ty, (int)N, P->px.p,
+                         P->py.p, P->pnx.p, P->pny.p, r, o, k, eta);
+      break;
+    case kDoubleLayer:
+      hipLaunchKernelGGL(k_propagate<kDoubleLayer>, dim3(nb), dim3(256), 0, st, (int)nt, tx, ty, (int)N, P->px.p,
+                         P->py.p, P->pnx.p, P->pny.p, r, o, k, eta);
+      break;
+    default:
+      hipLaunchKernelGGL(k_propagate<kCombined>, dim3(nb), dim3(256), 0, st, (int)nt, tx, ty, (int)N, P->px.p,
+                         P->py.p, P->pnx.p, P->pny.p, r, o, k, eta);
+  }
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(P, IFGF_ERR_HIP, hipGetErrorString(e));
   return IFGF_OK;
 }
 

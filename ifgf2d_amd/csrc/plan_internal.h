@@ -46,6 +46,11 @@ struct DevBuf {
 };
 
 
+// Device workspaces of the single-system solvers (gmres.hip), kept by the
+// plan between calls.
+struct SolverCache;
+void destroy_solver_cache(SolverCache* c);
+
 }  // namespace ifgf
 
 struct ifgf_plan {
@@ -155,7 +160,12 @@ struct ifgf_plan {
   double stage_ms[6] = {0, 0, 0, 0, 0, 0};
   double setup_s[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // plan creation steps (ifgf_plan_setup_times)
   long long prof_count = 0;
+  // solvers: cached workspaces, and the last single-system solve's seconds
+  // ([0] workspace set-up, [1] host waits on the device, [2] inner cycles, [3] total)
+  ifgf::SolverCache* solver = nullptr;
+  double solver_s[4] = {0, 0, 0, 0};
   ~ifgf_plan() {
+    ifgf::destroy_solver_cache(solver);
     for (auto& e : ev_pool)
       if (e) (void)hipEventDestroy(e);
     if (gexec) (void)hipGraphExecDestroy(gexec);

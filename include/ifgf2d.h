@@ -216,6 +216,21 @@ int ifgf_apply_direct(ifgf_plan* plan, const double* x, const long long* targets
 int ifgf_propagate(ifgf_plan* plan, const double* tx, const double* ty, long long nt,
                    const double* density, double* out);
 
+/* ifgf_propagate on device pointers, allocating nothing: no reference
+ * counterpart; for callers that cannot call hipMalloc (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid allocation). tx, ty: nt
+ * target coordinates; density: the plan's N complex values in patch order;
+ * out: nt complex values; all device pointers. Asynchronous on stream (NULL =
+ * the plan's stream), with the plan's device current. The intensity-weighted
+ * density goes through a fixed device scratch array of the library (16 MiB
+ * per device, held until the process ends), so the plan may have at most
+ * 1,048,576 points (IFGF_ERR_ARG above that). Every call on one device shares
+ * that scratch: calls that overlap on one device (other streams, other
+ * threads) race and must be serialised by the caller. Same sum as
+ * ifgf_propagate, taken over the points in the plan's Morton order. */
+int ifgf_propagate_noalloc(ifgf_plan* plan, const double* tx, const double* ty, long long nt, const double* density,
+                           double* out, void* stream);
+
 /* IFGF-accelerated field at separate target points (the acceleration has no
  * reference counterpart: the reference sums the same field directly in
  * ForwardMap::propagate_unacc, ForwardMap.tpp:702-727, which ifgf_propagate
@@ -305,6 +320,15 @@ int ifgf_refine_multi(ifgf_plan* plan, const double* rhs, double* x, int nrhs, i
 int ifgf_fgmres(ifgf_plan* plan, const double* rhs, double* x, int restart, double tol, int maxit,
                 int inner_restart, double inner_floor, int* outer_iters, int* inner_iters,
                 double* relres, double* hist, int hist_cap);
+
+/* Wall-clock seconds of the plan's last ifgf_gmres / ifgf_fgmres call: [0]
+ * workspace set-up (the Krylov spaces, right-hand side, iterate and FGMRES's
+ * preconditioned directions are allocated on the first call and kept by the
+ * plan, so a warm call spends ~0 here), [1] host time blocked on the device
+ * (the Hessenberg columns and norms the host's Givens steps wait for), [2] the
+ * inner GMRES cycles (FGMRES; 0 for GMRES), [3] the whole call. No reference
+ * counterpart (opGMRES.hpp:68-269 times nothing). */
+int ifgf_plan_solver_times(const ifgf_plan* plan, double seconds[4]);
 
 /* Generic host GMRES on any operator: the reference's Householder GMRES
  * (opGMRES.hpp:68-269) with the mapping of LinearOperator.hpp:54 as a C

@@ -37,6 +37,8 @@ def lib():
                                     ctypes.c_double, _dp, _dp]
         L.oracle_patch_lims.argtypes = [ctypes.c_int, _dp, ctypes.c_int, ctypes.c_double,
                                         ctypes.c_double, _dp, ctypes.c_int]
+        L.oracle_set_threads.argtypes = [ctypes.c_int]
+        L.oracle_set_threads.restype = ctypes.c_int
         L.oracle_curve_eval.argtypes = [ctypes.c_int, _dp, ctypes.c_int, _dp, ctypes.c_longlong, _dp, _dp,
                                         _dp, _dp]
         L.oracle_curve_diff.argtypes = [ctypes.c_int, _dp, ctypes.c_int, _dp, _dp, ctypes.c_longlong,
@@ -106,6 +108,12 @@ def rp_cov(which, t, alpha, p):
     xi, dxi = np.empty_like(t), np.empty_like(t)
     lib().oracle_rp_cov(which, _d(t), t.size, alpha, p, _d(xi), _d(dxi))
     return xi, dxi
+
+
+def set_threads(n=0):
+    """Threads of the oracle's OpenMP loops (0: the runtime default); returns
+    the count in effect. Results do not depend on it."""
+    return lib().oracle_set_threads(int(n))
 
 
 def curve_eval(curve_type, params, t):

@@ -285,3 +285,27 @@ def test_seeded_density_generator_matches_fixture():
     assert abs(z.real.mean()) < 0.01 and abs(z.imag.mean()) < 0.01
     assert abs(z.real.var() - 1) < 0.02 and abs(z.imag.var() - 1) < 0.02
     assert abs(np.mean(z.real * z.imag)) < 0.01
+
+
+def test_oracle_threads_do_not_change_results():
+    """The oracle's OpenMP loops sum every output in the serial order: one
+    thread and all threads give the same bits (kite, N = 2,000, 5 levels)."""
+    import ifgf2d_amd as I
+    from ifgf2d_amd import configs
+    lims = I.equal_parameter_lims(200)
+    curve = I.kite()
+    n0 = O.set_threads(0)
+    try:
+        O.set_threads(1)
+        om = O.ForwardMap(curve[0], curve[1], lims, k=20.0, delta=I.default_delta(curve, lims), p=6.0,
+                          nlevels=5)
+        x = configs.seeded_density(om.N)
+        y1 = om.apply(x)
+        pts1 = om.precomputation(7, 10 + om.near_singular()[0][7])
+        O.set_threads(max(2, n0))
+        om2 = O.ForwardMap(curve[0], curve[1], lims, k=20.0, delta=I.default_delta(curve, lims), p=6.0,
+                           nlevels=5)
+        assert np.array_equal(om2.apply(x), y1)
+        assert np.array_equal(om2.precomputation(7, 10 + om2.near_singular()[0][7]), pts1)
+    finally:
+        O.set_threads(n0)
