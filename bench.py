@@ -105,6 +105,61 @@ def cpu_baseline(kw, x, name):
     return rec, y
 
 
+def north_star_block(I, configs, np, log, reps=1):
+    """The headline many-right-hand-side comparison on the 256,000-point star at
+    k = 500 (C3 geometry, C4 directions): 64 plane waves 2 pi j / 64 solved to
+    the FP64 true residual 1e-8 (a) by 64 single FP64 GMRES(50) solves, (b) by
+    the FP64 lockstep GMRES over batched applications and (c) by the
+    mixed-precision refinement on a mixed plan; wall time of each (warm), the
+    ratios, and the scattered field at 64 points on r = 3 of (b) and (c)
+    against (a)'s FP64 solutions."""
+    import torch
+    kw, _ = configs.star_k500(nrhs_max=64)
+    t0 = time.time()
+    f64 = I.ForwardMap(**kw, autotune=True, device_setup=True)
+    setup64 = time.time() - t0
+    x, y, *_ = f64.points()
+    th = 2 * np.pi * np.arange(64) / 64
+    B = np.stack([-configs.plane_wave(x, y, kw["k"], (np.cos(t_), np.sin(t_))) for t_ in th], axis=1)
+    tt = 2 * np.pi * np.arange(64) / 64
+    tx, ty = 3.0 * np.cos(tt), 3.0 * np.sin(tt)
+    f64.solve(B[:, 0], restart=50, tol=1e-8, maxit=2000)  # first call (kernel loads, workspaces)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    singles, its1 = [], []
+    for j in range(64):
+        d_, it_, rr_, _ = f64.solve(B[:, j], restart=50, tol=1e-8, maxit=2000)
+        singles.append(d_)
+        its1.append(it_)
+    t_single = time.perf_counter() - t0
+    log(f"north star: 64 single FP64 solves {t_single:.3f} s (iterations {min(its1)}..{max(its1)})")
+    far = [f64.propagate_unacc(tx, ty, d_) for d_ in singles]
+    (X64, its, rr), _, t_lock = timed_solve(lambda: f64.solve_multi(B, restart=50, tol=1e-8, maxit=2000), reps=reps)
+    e_lock = max(float(np.linalg.norm(f64.propagate_unacc(tx, ty, X64[:, j]) - far[j]) / np.linalg.norm(far[j]))
+                 for j in range(64))
+    log(f"north star: FP64 lockstep {t_lock} s, max true relres {rr.max():.2e}, far field vs single {e_lock:.1e}")
+    del f64
+    t0 = time.time()
+    fmx = I.ForwardMap(**dict(kw, precision="mixed"), autotune=True, device_setup=True)
+    setupm = time.time() - t0
+    (Xm, outer, itm, rrm), _, t_ref = timed_solve(
+        lambda: fmx.solve_multi_refined(B, restart=50, tol=1e-8, maxit=2000), reps=reps)
+    e_mix = max(float(np.linalg.norm(fmx.propagate_unacc(tx, ty, Xm[:, j]) - far[j]) / np.linalg.norm(far[j]))
+                for j in range(64))
+    log(f"north star: mixed refinement {t_ref} s, {outer} steps, max true relres {rrm.max():.2e}, "
+        f"far field vs FP64 {e_mix:.1e}")
+    del fmx
+    return {"workload": "star r=1+0.3cos5t, k=500, N=256000, 64 plane waves 2 pi j/64, CFIE 5x5, tol 1e-8",
+            "plan_setup_s": {"fp64": round(setup64, 3), "mixed": round(setupm, 3)},
+            "fp64_single_solves_s": round(t_single, 4), "single_iterations": [min(its1), max(its1)],
+            "fp64_lockstep_s": t_lock["median"], "fp64_lockstep_true_relres_max": float(rr.max()),
+            "fp64_lockstep_far_field_rel_diff_max": e_lock,
+            "mixed_refined_s": t_ref["median"], "mixed_refinement_steps": outer,
+            "mixed_true_relres_max": float(rrm.max()), "mixed_far_field_rel_diff_vs_fp64_max": e_mix,
+            "speedup_mixed_vs_fp64_single": round(t_single / t_ref["median"], 3),
+            "speedup_mixed_vs_fp64_lockstep": round(t_lock["median"] / t_ref["median"], 3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -117,6 +172,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-group", action="store_true")
     ap.add_argument("--no-field", action="store_true")
+    ap.add_argument("--no-north-star", action="store_true",
+                    help="skip the 64-direction comparison on the 256k star")
     ap.add_argument("--workload", choices=("kite", "star"), default="kite",
                     help="kite: the 64k kite at k=200 (default); star: the 256k star r=1+0.3cos5t at k=500")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -354,6 +411,10 @@ def main():
             del fb
         del Xb, Yb
 
+    north = None
+    if not args.no_north_star and not args.no_solve:
+        north = north_star_block(I, configs, np, log)
+
     # sixteen seeded random star obstacles at k=100 (~10 nodes per wavelength):
     # their forward maps one after another against one captured group launch
     group = None
@@ -455,6 +516,7 @@ def main():
         "solve": solve,
         "mixed_precision": mixed,
         "batched_rhs": batched,
+        "north_star": north,
         "obstacle_group": group,
         "field": field,
         "roofline": roofline,

@@ -182,3 +182,39 @@ def test_multi_rhs_mixed_refinement():
         r = B[:, j] - f64.compute_Ax_acc(X[:, j])
         assert np.linalg.norm(r) / np.linalg.norm(B[:, j]) < 1e-8
         assert rel(X[:, j], X64[:, j]) < 1e-7
+
+
+def test_star_256k_64_directions():
+    """C4 at its stated scale: 64 incidence directions 2 pi j / 64 on the
+    256,000-point star at k = 500, solved together by the FP64 lockstep GMRES
+    (ifgf_gmres_multi) and by the mixed-precision refinement
+    (ifgf_refine_multi): every true FP64 residual below 1e-8, four columns
+    within 1e-7 of single FP64 ifgf_gmres solves, and the scattered field on
+    r = 3 (64 points, propagate_unacc) of every mixed-refined column within
+    1e-6 of the FP64 solution's."""
+    kw, _ = configs.star_k500(nrhs_max=64)
+    f64 = I.ForwardMap(**kw, device_setup=True)
+    x, y, *_ = f64.points()
+    th = 2 * np.pi * np.arange(64) / 64
+    B = np.stack([-configs.plane_wave(x, y, kw["k"], (np.cos(t), np.sin(t))) for t in th], axis=1)
+    X64, its, rr = f64.solve_multi(B, restart=50, tol=1e-8, maxit=2000)
+    print(f"FP64 lockstep: iterations {its.min()}..{its.max()}, max true relres {rr.max():.2e}", flush=True)
+    assert np.all(rr < 1e-8)
+    for j in (0, 13, 32, 57):
+        d, it, r1, _ = f64.solve(B[:, j], restart=50, tol=1e-8, maxit=2000)
+        print(f"  column {j}: single {it} iterations vs lockstep {its[j]}, density diff {rel(X64[:, j], d):.1e}",
+              flush=True)
+        assert rel(X64[:, j], d) < 1e-7
+    tt = 2 * np.pi * np.arange(64) / 64
+    tx, ty = 3.0 * np.cos(tt), 3.0 * np.sin(tt)
+    far64 = [f64.propagate_unacc(tx, ty, X64[:, j]) for j in range(64)]
+    del f64
+    fmx = I.ForwardMap(**kw, precision="mixed", device_setup=True)
+    Xm, outer, itm, rrm = fmx.solve_multi_refined(B, restart=50, tol=1e-8, maxit=2000)
+    print(f"mixed refinement: {outer} steps, inner {itm.min()}..{itm.max()}, max true relres {rrm.max():.2e}",
+          flush=True)
+    assert np.all(rrm < 1e-8)
+    efar = max(rel(fmx.propagate_unacc(tx, ty, Xm[:, j]), far64[j]) for j in range(64))
+    eden = max(rel(Xm[:, j], X64[:, j]) for j in range(64))
+    print(f"  max far-field diff vs FP64 {efar:.1e}, max density diff {eden:.1e}", flush=True)
+    assert efar < 1e-6 and eden < 1e-6
