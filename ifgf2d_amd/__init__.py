@@ -23,7 +23,7 @@ import os
 import numpy as np
 
 __all__ = [
-    "ForwardMap", "LinearOperator", "ObstacleGroup", "gmres", "hankel01", "morton_sort", "IFGFError",
+    "ForwardMap", "LinearOperator", "ObstacleGroup", "LearnedPreconditioner", "fno_gemm_check", "gmres", "hankel01", "morton_sort", "IFGFError",
     "circle", "kite", "star", "equal_parameter_lims", "SINGLE_LAYER", "DOUBLE_LAYER", "COMBINED",
 ]
 
@@ -44,6 +44,7 @@ IFGF_PLAN_OVERLAP = 16
 IFGF_PLAN_DEVICE_PRECOMP = 32
 IFGF_PLAN_DEVICE_GEOMETRY = 64
 IFGF_PLAN_DEVICE_TREE = 128
+IFGF_FNO_GUESS, IFGF_FNO_PRECONDITION = 1, 2
 
 
 class IFGFError(RuntimeError):
@@ -77,7 +78,10 @@ EXPORTS = [
     "ifgf_group_synchronize", "ifgf_group_destroy", "ifgf_gmres_multi", "ifgf_field_create",
     "ifgf_field_apply", "ifgf_field_stats", "ifgf_field_destroy", "ifgf_plan_setup_times",
     "ifgf_refine_multi", "ifgf_curve_eval", "ifgf_curve_diff", "ifgf_plan_solver_times",
-    "ifgf_propagate_noalloc",
+    "ifgf_propagate_noalloc", "ifgf_fno_num_params", "ifgf_fno_create", "ifgf_fno_destroy", "ifgf_fno_error",
+    "ifgf_fno_get_params", "ifgf_fno_set_params", "ifgf_fno_add_sample", "ifgf_fno_num_samples",
+    "ifgf_fno_clear_samples", "ifgf_fno_sample_data", "ifgf_fno_loss_grad", "ifgf_fno_train", "ifgf_fno_apply",
+    "ifgf_fno_gemm_check", "ifgf_fno_gmres",
 ]
 
 _LIB = None
@@ -137,6 +141,28 @@ def lib():
         L.ifgf_plan_solver_times.argtypes = [ctypes.c_void_p, _dp]
         L.ifgf_propagate_noalloc.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_longlong,
                                              ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        _fp = ctypes.POINTER(ctypes.c_float)
+        L.ifgf_fno_num_params.restype = ctypes.c_longlong
+        L.ifgf_fno_create.argtypes = [ctypes.c_ulonglong, ctypes.POINTER(ctypes.c_void_p)]
+        L.ifgf_fno_destroy.argtypes = [ctypes.c_void_p]
+        L.ifgf_fno_error.argtypes = [ctypes.c_void_p]
+        L.ifgf_fno_error.restype = ctypes.c_char_p
+        L.ifgf_fno_get_params.argtypes = [ctypes.c_void_p, _fp]
+        L.ifgf_fno_set_params.argtypes = [ctypes.c_void_p, _fp]
+        L.ifgf_fno_add_sample.argtypes = [ctypes.c_void_p, ctypes.c_void_p, _dp, _dp]
+        L.ifgf_fno_num_samples.argtypes = [ctypes.c_void_p]
+        L.ifgf_fno_clear_samples.argtypes = [ctypes.c_void_p]
+        L.ifgf_fno_sample_data.argtypes = [ctypes.c_void_p, ctypes.c_int, _lp, _fp, _fp, _dp, _dp]
+        L.ifgf_fno_loss_grad.argtypes = [ctypes.c_void_p, _ip, ctypes.c_int, _dp, _fp]
+        L.ifgf_fno_train.argtypes = [ctypes.c_void_p, _ip, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                     ctypes.c_ulonglong, _dp]
+        L.ifgf_fno_apply.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                     ctypes.c_void_p]
+        L.ifgf_fno_gemm_check.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _fp, _fp,
+                                          _fp]
+        L.ifgf_fno_gmres.argtypes = [ctypes.c_void_p, ctypes.c_void_p, _dp, _dp, ctypes.c_int, ctypes.c_double,
+                                     ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, _ip, _ip, _dp, _dp,
+                                     ctypes.c_int]
         L.ifgf_plan_profile.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.ifgf_plan_stage_times.argtypes = [ctypes.c_void_p, _dp, _lp]
         L.ifgf_plan_setup_times.argtypes = [ctypes.c_void_p, _dp]
@@ -496,6 +522,24 @@ class ForwardMap:
         self._check(lib().ifgf_plan_solver_times(self._h, _d(t)), "solver_times")
         return dict(zip(("workspace", "host_waits", "inner_cycles", "total"), t.tolist()))
 
+    def solve_learned(self, rhs, fno, restart=50, tol=1e-8, maxit=1000, guess=True, precondition=False,
+                      window=8, ratio=0.5, x0=None):
+        """GMRES with the learned network (ifgf_fno_gmres): guess -> initial density
+        M(b); precondition -> flexible right preconditioner z = M(v), falling back to
+        plain GMRES when the estimate stops falling (ratio over window iterations).
+        Returns (x, Arnoldi steps, fell back, true FP64 relative residual, estimates)."""
+        b = _c(rhs)
+        x = np.zeros_like(b) if x0 is None else _c(x0).copy()
+        it, fb, rr = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        hist = np.zeros(maxit)
+        mode = (IFGF_FNO_GUESS if guess else 0) | (IFGF_FNO_PRECONDITION if precondition else 0)
+        rc = lib().ifgf_fno_gmres(self._h, fno._h, _d(b.view(np.float64)), _d(x.view(np.float64)), restart, tol,
+                                  maxit, mode, window, ratio, ctypes.byref(it), ctypes.byref(fb), ctypes.byref(rr),
+                                  _d(hist), hist.size)
+        if rc not in (IFGF_OK, IFGF_ERR_NOT_CONVERGED):
+            self._check(rc, "ifgf_fno_gmres")
+        return x, it.value, bool(fb.value), rr.value, hist[: it.value]
+
     def linear_operator(self):
         """The LinearOperator that ForwardMap::solve builds (ForwardMap.tpp:673-682)."""
         def mapping(x, out):
@@ -604,6 +648,116 @@ class ObstacleGroup:
             self.close()
         except Exception:
             pass
+
+
+class LearnedPreconditioner:
+    """The learned 1D FNO preconditioner (ifgf_fno_*): trained on the device on
+    (right-hand side, density) pairs the solver produced, applied as z = M(v) on a
+    ForwardMap's curve."""
+
+    def __init__(self, seed=0):
+        h = ctypes.c_void_p()
+        rc = lib().ifgf_fno_create(seed, ctypes.byref(h))
+        if rc != IFGF_OK:
+            raise IFGFError(f"ifgf_fno_create failed ({rc}): {lib().ifgf_fno_error(None).decode()}")
+        self._h = h
+        self.nparams = int(lib().ifgf_fno_num_params())
+
+    def _check(self, rc, what):
+        if rc != IFGF_OK:
+            raise IFGFError(f"{what} failed ({rc}): {lib().ifgf_fno_error(self._h).decode()}")
+
+    def get_params(self):
+        p = np.empty(self.nparams, dtype=np.float32)
+        self._check(lib().ifgf_fno_get_params(self._h, p.ctypes.data_as(ctypes.POINTER(ctypes.c_float))), "get_params")
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        if p.size != self.nparams:
+            raise IFGFError("parameter vector of the wrong length")
+        self._check(lib().ifgf_fno_set_params(self._h, p.ctypes.data_as(ctypes.POINTER(ctypes.c_float))), "set_params")
+
+    def add_sample(self, fm, v, density):
+        """Adds (v, density) on fm's curve; returns the sample index."""
+        v, d = _c(v), _c(density)
+        self._check(lib().ifgf_fno_add_sample(self._h, fm._h, _d(v.view(np.float64)), _d(d.view(np.float64))),
+                    "add_sample")
+        return int(lib().ifgf_fno_num_samples(self._h)) - 1
+
+    @property
+    def num_samples(self):
+        return int(lib().ifgf_fno_num_samples(self._h))
+
+    def clear_samples(self):
+        self._check(lib().ifgf_fno_clear_samples(self._h), "clear_samples")
+
+    def sample_data(self, s):
+        n = ctypes.c_longlong()
+        self._check(lib().ifgf_fno_sample_data(self._h, s, ctypes.byref(n), None, None, None, None), "sample_data")
+        N = n.value
+        X = np.empty((N, 10), dtype=np.float32)
+        m = np.empty(N, dtype=np.float32)
+        t, wt = np.empty(N), np.empty(N)
+        fp = ctypes.POINTER(ctypes.c_float)
+        self._check(lib().ifgf_fno_sample_data(self._h, s, ctypes.byref(n), X.ctypes.data_as(fp),
+                                               m.ctypes.data_as(fp), _d(t), _d(wt)), "sample_data")
+        return {"X": X, "mphys": m, "t": t, "wt": wt}
+
+    def loss_grad(self, ids, grad=True):
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        loss = ctypes.c_double()
+        g = np.empty(self.nparams, dtype=np.float32) if grad else None
+        self._check(lib().ifgf_fno_loss_grad(self._h, ids.ctypes.data_as(_ip), ids.size, ctypes.byref(loss),
+                                             g.ctypes.data_as(ctypes.POINTER(ctypes.c_float)) if grad else None),
+                    "loss_grad")
+        return loss.value, g
+
+    def train(self, ids, epochs=10, batch=4, lr=1e-3, seed=0):
+        """Adam over the samples ids; returns the mean loss of every epoch."""
+        ids = np.ascontiguousarray(ids, dtype=np.int32)
+        hist = np.zeros(epochs)
+        self._check(lib().ifgf_fno_train(self._h, ids.ctypes.data_as(_ip), ids.size, epochs, batch, lr, seed,
+                                         _d(hist)), "train")
+        return hist
+
+    def apply(self, fm, v):
+        """z = M(v) on fm's curve (host vectors)."""
+        v = _c(v)
+        z = np.empty_like(v)
+        self._check(lib().ifgf_fno_apply(self._h, fm._h, v.ctypes.data, z.ctypes.data, 0, None), "apply")
+        return z
+
+    def apply_device(self, fm, v_ptr, z_ptr, stream=None):
+        self._check(lib().ifgf_fno_apply(self._h, fm._h, ctypes.c_void_p(v_ptr), ctypes.c_void_p(z_ptr),
+                                         IFGF_PTR_DEVICE, ctypes.c_void_p(stream) if stream else None), "apply")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().ifgf_fno_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def fno_gemm_check(A, B, trans_a=False, trans_b=False):
+    """The network's BF16 MFMA GEMM on host arrays: A B (A given transposed when
+    trans_a, B when trans_b)."""
+    A = np.ascontiguousarray(A, dtype=np.float32)
+    B = np.ascontiguousarray(B, dtype=np.float32)
+    M, K = (A.shape[1], A.shape[0]) if trans_a else A.shape
+    N = B.shape[0] if trans_b else B.shape[1]
+    C = np.empty((M, N), dtype=np.float32)
+    fp = ctypes.POINTER(ctypes.c_float)
+    rc = lib().ifgf_fno_gemm_check(M, N, K, int(trans_a), int(trans_b), A.ctypes.data_as(fp), B.ctypes.data_as(fp),
+                                   C.ctypes.data_as(fp))
+    if rc != IFGF_OK:
+        raise IFGFError(lib().ifgf_fno_error(None).decode())
+    return C
 
 
 class LinearOperator:

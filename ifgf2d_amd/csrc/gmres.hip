@@ -546,16 +546,18 @@ std::vector<std::complex<double>> back_substitute(const std::vector<std::complex
 // |g(k)| / ||b - A x0|| < tol (opGMRES.hpp:207) or after maxit iterations.
 // Returns the iteration count; *est the final estimate. x0_zero: x is zero on
 // entry, so r0 = b without an application.
+// ref_norm > 0: the estimates are relative to ref_norm instead of ||b - A x0||.
 int gmres_core(Solver& S, Krylov& K, bool fp64, const zd* b, zd* x, int m, double tol, int maxit, double* est_out,
-               double* hist, int hist_cap, bool* conv, bool x0_zero = false) {
+               double* hist, int hist_cap, bool* conv, bool x0_zero = false, double ref_norm = 0.0) {
   using C = std::complex<double>;
   const long long n = S.n;
   if (x0_zero) HIP_TRY(hipMemcpyAsync(K.tmp.p, b, n * sizeof(zd), hipMemcpyDeviceToDevice, S.st));  // r0 = b
   else S.residual(b, x, K.tmp.p, fp64);
-  const double r0n = S.norm(K, K.tmp.p);
-  double beta = r0n;
-  bool converged = (r0n == 0.0);
-  double est = converged ? 0.0 : 1.0;
+  const double r0 = S.norm(K, K.tmp.p);
+  const double r0n = ref_norm > 0.0 ? ref_norm : r0;
+  double beta = r0;
+  bool converged = (r0 == 0.0) || r0 / r0n < tol;
+  double est = r0 == 0.0 ? 0.0 : r0 / r0n;
   int it = 0;
   std::vector<C> H((size_t)(m + 1) * m), g(m + 1), hv;
   std::vector<Givens> rot(m);
@@ -1071,4 +1073,136 @@ extern "C" int ifgf_plan_solver_times(const ifgf_plan* P, double seconds[4]) {
   if (!P || !seconds) return plan_fail(const_cast<ifgf_plan*>(P), IFGF_ERR_ARG, "null argument");
   for (int i = 0; i < 4; i++) seconds[i] = P->solver_s[i];
   return IFGF_OK;
+}
+
+// ---- learned preconditioner (ifgf_fno_gmres) -------------------------------
+namespace {
+
+// Flexible GMRES(m) with z_j = M(v_j) from the learned network, on the plan's
+// default operator: x (initial guess in, solution out). The estimates are
+// |w(k)| / ref_norm. Stops on est < tol, after maxit iterations, or on
+// stagnation: once more than `window` estimates exist, an estimate above
+// ratio x the one `window` iterations earlier (the caller then falls back to
+// plain GMRES from the current x). Returns the iterations taken.
+int fgmres_learned(Solver& S, Krylov& K, zd* Z, ifgf_fno* F, const zd* b, zd* x, int m, double tol, int maxit,
+                   double ref_norm, int window, double ratio, double* hist, int hist_cap, int hist_off, bool* conv,
+                   bool* stalled) {
+  using C = std::complex<double>;
+  const long long n = S.n;
+  S.residual(b, x, K.tmp.p, false);
+  double beta = S.norm(K, K.tmp.p);
+  bool converged = beta == 0.0 || beta / ref_norm < tol;
+  *stalled = false;
+  int it = 0;
+  std::vector<C> H((size_t)(m + 1) * m), g(m + 1), hv;
+  std::vector<Givens> rot(m);
+  std::vector<double> ests;
+  while (!converged && it < maxit && !*stalled) {
+    S.normalize(K, K.tmp.p, K.V.p, nullptr);
+    std::fill(g.begin(), g.end(), C(0.0));
+    g[0] = beta;
+    int j = 0;
+    for (; j < m && it < maxit; j++) {
+      it++;
+      zd* zj = Z + (size_t)j * n;
+      if (fno_apply_device(F, S.P, K.V.p + (size_t)j * n, zj, S.st)) throw std::runtime_error(S.P->err);
+      S.apply(zj, K.w.p, false);
+      S.arnoldi_enqueue(K, j);
+      S.arnoldi_wait(K, j, hv);
+      hv[j + 1] = std::sqrt(std::norm(hv[j + 1]));
+      for (int i = 0; i < j; i++) apply_givens(rot[i], hv[i], hv[i + 1]);
+      rot[j] = make_givens(hv[j], hv[j + 1]);
+      apply_givens(rot[j], hv[j], hv[j + 1]);
+      apply_givens(rot[j], g[j], g[j + 1]);
+      for (int i = 0; i <= j; i++) H[(size_t)i * m + j] = hv[i];
+      const double est = std::abs(g[j + 1]) / ref_norm;
+      if (hist && hist_off + it - 1 < hist_cap) hist[hist_off + it - 1] = est;
+      ests.push_back(est);
+      if (est < tol) {
+        converged = true;
+        j++;
+        break;
+      }
+      if ((int)ests.size() > window && est > ratio * ests[ests.size() - 1 - window]) {
+        *stalled = true;
+        j++;
+        break;
+      }
+    }
+    S.update(K, Z, j, back_substitute(H, m, g, j), x);
+    if (!converged && it < maxit && !*stalled) {
+      S.residual(b, x, K.tmp.p, false);
+      beta = S.norm(K, K.tmp.p);
+    }
+  }
+  *conv = converged;
+  return it;
+}
+
+}  // namespace
+
+extern "C" int ifgf_fno_gmres(ifgf_plan* P, ifgf_fno* F, const double* rhs, double* x, int restart, double tol,
+                              int maxit, int mode, int window, double ratio, int* iters, int* fallback,
+                              double* relres, double* hist, int hist_cap) {
+  if (!P || !F || !rhs || !x || !iters || !fallback || !relres || restart < 1 || maxit < 1 || window < 1 ||
+      !(ratio > 0.0))
+    return plan_fail(P, IFGF_ERR_ARG, "invalid argument");
+  if (restart > kMaxRestart) return plan_fail(P, IFGF_ERR_ARG, "restart above 256");
+  if (!P->on_device) return plan_fail(P, IFGF_ERR_NO_DEVICE, "plan was built host-only");
+  const long long n = P->geo.N;
+  *iters = 0;
+  *fallback = 0;
+  const auto t_start = Clock::now();
+  try {
+    Solver S(P);
+    SolverCache& W = solver_cache(P);
+    Krylov& K = cached_krylov(W.outer, n, restart);
+    if (W.zcols < restart || (long long)W.Z.n != (long long)n * W.zcols) {
+      W.Z.alloc((size_t)n * restart);
+      W.zcols = restart;
+    }
+    P->solver_s[0] = seconds_since(t_start);
+    DevBuf<zd>& b = W.b;
+    DevBuf<zd>& xd = W.x;
+    HIP_TRY(hipMemcpyAsync(b.p, rhs, n * sizeof(zd), hipMemcpyHostToDevice, S.st));
+    HIP_TRY(hipMemcpyAsync(xd.p, x, n * sizeof(zd), hipMemcpyHostToDevice, S.st));
+    const double bnorm = S.norm(K, b.p);
+    if (bnorm == 0.0) {
+      std::fill((std::complex<double>*)x, (std::complex<double>*)x + n, std::complex<double>(0.0));
+      *relres = 0.0;
+      return IFGF_OK;
+    }
+    bool zero = true;
+    for (long long i = 0; i < 2 * n && zero; i++) zero = x[i] == 0.0;
+    if ((mode & IFGF_FNO_GUESS) && zero) {
+      if (fno_apply_device(F, P, b.p, xd.p, S.st)) throw std::runtime_error(P->err);
+      zero = false;
+    }
+    bool converged = false;
+    int it = 0;
+    if (mode & IFGF_FNO_PRECONDITION) {
+      bool stalled = false;
+      it = fgmres_learned(S, K, W.Z.p, F, b.p, xd.p, restart, tol, maxit, bnorm, window, ratio, hist, hist_cap, 0,
+                          &converged, &stalled);
+      if (stalled && !converged && it < maxit) *fallback = 1;
+    }
+    if (!converged && it < maxit) {
+      // plain GMRES (IdentityPreconditioner, opGMRES.hpp:274) from the current
+      // iterate, with the same |w(k)| / ||b|| < tol rule
+      double est = 0.0;
+      it += gmres_core(S, K, false, b.p, xd.p, restart, tol, maxit - it, &est, hist ? hist + it : nullptr,
+                       hist ? std::max(0, hist_cap - it) : 0, &converged, zero, bnorm);
+    }
+    *iters = it;
+    S.residual(b.p, xd.p, K.tmp.p, true);
+    *relres = S.norm(K, K.tmp.p) / bnorm;
+    HIP_TRY(hipMemcpyAsync(x, xd.p, n * sizeof(zd), hipMemcpyDeviceToHost, S.st));
+    S.sync();
+    P->solver_s[1] = S.wait_s;
+    P->solver_s[2] = 0.0;
+    P->solver_s[3] = seconds_since(t_start);
+    return converged ? IFGF_OK : IFGF_ERR_NOT_CONVERGED;
+  } catch (std::exception& e) {
+    return plan_fail(P, IFGF_ERR_HIP, e.what());
+  }
 }

@@ -50,6 +50,9 @@ struct DevBuf {
 // plan between calls.
 struct SolverCache;
 void destroy_solver_cache(SolverCache* c);
+// The learned preconditioner's per-curve data (fno.hip), kept by the plan.
+struct FnoGeo;
+void destroy_fno_geo(FnoGeo* g);
 
 }  // namespace ifgf
 
@@ -164,8 +167,10 @@ struct ifgf_plan {
   // ([0] workspace set-up, [1] host waits on the device, [2] inner cycles, [3] total)
   ifgf::SolverCache* solver = nullptr;
   double solver_s[4] = {0, 0, 0, 0};
+  ifgf::FnoGeo* fno_geo = nullptr;
   ~ifgf_plan() {
     ifgf::destroy_solver_cache(solver);
+    ifgf::destroy_fno_geo(fno_geo);
     for (auto& e : ev_pool)
       if (e) (void)hipEventDestroy(e);
     if (gexec) (void)hipGraphExecDestroy(gexec);
@@ -187,6 +192,9 @@ int apply_device(ifgf_plan* P, const zd* x, zd* y, hipStream_t st, bool fp64 = f
 // has them (plan.hip)
 int apply_columns(ifgf_plan* P, const zd* x, long long ld, zd* y, int nrhs, hipStream_t st, bool fp64 = false);
 int plan_fail(ifgf_plan* p, int code, const std::string& msg);
+// z = M(v): the learned preconditioner of fno.hip on device vectors of a plan's
+// curve (patch order), stream-ordered on st
+int fno_apply_device(ifgf_fno* F, ifgf_plan* P, const zd* v, zd* z, hipStream_t st);
 // The RP correction built on the GPU (precomp.hip); false (err set) when Np
 // exceeds the device kernel's limit.
 bool build_correction_device(const Geometry& g, const Curve& c, int F, double k, double eta, double p,

@@ -330,6 +330,65 @@ int ifgf_fgmres(ifgf_plan* plan, const double* rhs, double* x, int restart, doub
  * counterpart (opGMRES.hpp:68-269 times nothing). */
 int ifgf_plan_solver_times(const ifgf_plan* plan, double seconds[4]);
 
+/* ---- learned preconditioner: a 1D Fourier neural operator ------------------
+ * No reference counterpart. The reference's GMRES applies a preconditioner
+ * object at opGMRES.hpp:103,153,251 (precond.solve), IdentityPreconditioner
+ * by default (:274); these entry points put a learned one there. The network
+ * (4 Fourier layers, width 64, 32 modes on the boundary parameter t, BF16
+ * weights and activations into FP32-accumulating MFMA, FP32 master weights)
+ * maps per-node features of a vector v on the curve (geometry, |v|, its local
+ * tangential wavenumber) to a complex multiplier field m and returns
+ * z = M(v) = m v: the initial density for v = b, a flexible right
+ * preconditioner for Krylov vectors. It is trained on the device (Adam) on
+ * (right-hand side, density) pairs that the solver produced. One network
+ * object is not reentrant (its workspaces are shared by its calls). */
+typedef struct ifgf_fno ifgf_fno;
+#define IFGF_FNO_GUESS 1        /* ifgf_fno_gmres: initial density M(b) (when x is zero) */
+#define IFGF_FNO_PRECONDITION 2 /* ifgf_fno_gmres: flexible right preconditioner z_j = M(v_j) */
+
+long long ifgf_fno_num_params(void);
+/* seeded uniform initialisation (splitmix64) */
+int ifgf_fno_create(unsigned long long seed, ifgf_fno** out);
+void ifgf_fno_destroy(ifgf_fno* fno);
+const char* ifgf_fno_error(const ifgf_fno* fno); /* NULL: the thread's last error */
+/* FP32 master weights, flat (ifgf_fno_num_params() values) */
+int ifgf_fno_get_params(const ifgf_fno* fno, float* out);
+int ifgf_fno_set_params(ifgf_fno* fno, const float* in);
+/* Adds a training sample: right-hand side v and the solved density (host,
+ * N complex each, patch order) of a plan's curve; its features are formed on
+ * the device and kept with the network. */
+int ifgf_fno_add_sample(ifgf_fno* fno, ifgf_plan* plan, const double* v, const double* density);
+int ifgf_fno_num_samples(const ifgf_fno* fno);
+int ifgf_fno_clear_samples(ifgf_fno* fno);
+/* Sample s's features X [N][10], prior multiplier [N], node parameters t [N]
+ * and parameter quadrature weights [N] (host; any may be NULL). */
+int ifgf_fno_sample_data(const ifgf_fno* fno, int s, long long* N, float* X, float* mphys, double* t, double* wt);
+/* Mean over samples ids[0..n) of ||M(v) - density||^2 / ||density||^2, and
+ * (grad != NULL) its gradient with respect to the master weights. */
+int ifgf_fno_loss_grad(ifgf_fno* fno, const int* ids, int n, double* loss, float* grad);
+/* Adam (lr, 0.9, 0.999, 1e-8) over samples ids[0..n) for `epochs` seeded
+ * shuffles in minibatches of `batch`; loss_hist[e] = epoch e's mean loss. */
+int ifgf_fno_train(ifgf_fno* fno, const int* ids, int n, int epochs, int batch, double lr, unsigned long long seed,
+                   double* loss_hist);
+/* z = M(v) on a plan's curve; flags IFGF_PTR_DEVICE (device pointers,
+ * asynchronous on stream) or host pointers (synchronous). */
+int ifgf_fno_apply(ifgf_fno* fno, ifgf_plan* plan, const double* v, double* z, int flags, void* stream);
+/* BF16 MFMA GEMM of the network (C = A B, FP32 row-major host arrays; A
+ * given as K x M when trans_a, B as N x K when trans_b), for tests. */
+int ifgf_fno_gemm_check(int M, int N, int K, int trans_a, int trans_b, const float* A, const float* B, float* C);
+/* GMRES with the learned network (the plan's default operator; host
+ * pointers; x holds the initial guess): mode IFGF_FNO_GUESS replaces a zero
+ * x by M(b); IFGF_FNO_PRECONDITION runs flexible GMRES(restart) with
+ * z_j = M(v_j), and on stagnation (an estimate above ratio x the one `window`
+ * iterations earlier) falls back to plain GMRES (the identity preconditioner)
+ * from the current iterate (*fallback = 1). Every estimate is |w(k)| / ||b||
+ * (the reference divides by ||b - A x0||, opGMRES.hpp:105,207, which would
+ * discard what the initial density gains); stops below tol or after maxit
+ * Arnoldi steps in all. relres: the true FP64 ||b - A x|| / ||b||. */
+int ifgf_fno_gmres(ifgf_plan* plan, ifgf_fno* fno, const double* rhs, double* x, int restart, double tol, int maxit,
+                   int mode, int window, double ratio, int* iters, int* fallback, double* relres, double* hist,
+                   int hist_cap);
+
 /* Generic host GMRES on any operator: the reference's Householder GMRES
  * (opGMRES.hpp:68-269) with the mapping of LinearOperator.hpp:54 as a C
  * callback. Returns IFGF_OK / IFGF_ERR_NOT_CONVERGED; err is the estimate. */
