@@ -44,7 +44,7 @@ IFGF_PLAN_OVERLAP = 16
 IFGF_PLAN_DEVICE_PRECOMP = 32
 IFGF_PLAN_DEVICE_GEOMETRY = 64
 IFGF_PLAN_DEVICE_TREE = 128
-IFGF_FNO_GUESS, IFGF_FNO_PRECONDITION = 1, 2
+IFGF_FNO_GUESS, IFGF_FNO_PRECONDITION, IFGF_FNO_FIXED_MULTIPLIER = 1, 2, 4
 
 
 class IFGFError(RuntimeError):
@@ -523,7 +523,7 @@ class ForwardMap:
         return dict(zip(("workspace", "host_waits", "inner_cycles", "total"), t.tolist()))
 
     def solve_learned(self, rhs, fno, restart=50, tol=1e-8, maxit=1000, guess=True, precondition=False,
-                      window=8, ratio=0.5, x0=None):
+                      window=8, ratio=0.5, x0=None, fixed_multiplier=False):
         """GMRES with the learned network (ifgf_fno_gmres): guess -> initial density
         M(b); precondition -> flexible right preconditioner z = M(v), falling back to
         plain GMRES when the estimate stops falling (ratio over window iterations).
@@ -532,7 +532,8 @@ class ForwardMap:
         x = np.zeros_like(b) if x0 is None else _c(x0).copy()
         it, fb, rr = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
         hist = np.zeros(maxit)
-        mode = (IFGF_FNO_GUESS if guess else 0) | (IFGF_FNO_PRECONDITION if precondition else 0)
+        mode = ((IFGF_FNO_GUESS if guess else 0) | (IFGF_FNO_PRECONDITION if precondition else 0)
+                | (IFGF_FNO_FIXED_MULTIPLIER if fixed_multiplier else 0))
         rc = lib().ifgf_fno_gmres(self._h, fno._h, _d(b.view(np.float64)), _d(x.view(np.float64)), restart, tol,
                                   maxit, mode, window, ratio, ctypes.byref(it), ctypes.byref(fb), ctypes.byref(rr),
                                   _d(hist), hist.size)
@@ -696,7 +697,7 @@ class LearnedPreconditioner:
         n = ctypes.c_longlong()
         self._check(lib().ifgf_fno_sample_data(self._h, s, ctypes.byref(n), None, None, None, None), "sample_data")
         N = n.value
-        X = np.empty((N, 10), dtype=np.float32)
+        X = np.empty((N, 11), dtype=np.float32)
         m = np.empty(N, dtype=np.float32)
         t, wt = np.empty(N), np.empty(N)
         fp = ctypes.POINTER(ctypes.c_float)

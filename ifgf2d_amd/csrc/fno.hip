@@ -7,7 +7,7 @@
 //
 // Network (width 64, 32 Fourier modes, 4 Fourier layers; BF16 weights and
 // activations into FP32-accumulating MFMA, FP32 master weights):
-//   features X (per node, 10 channels padded to 32)
+//   features X (per node, 11 channels padded to 32)
 //   H0 = X Wlift + blift
 //   P_l = IDFT(mix_l(DFT(H_l))) + H_l W_l + b_l,  H_{l+1} = GELU(P_l) (l < 3), H_4 = P_3
 //   Q = GELU(H_4 Wp1 + bp1),  O = Q Wp2 + bp2 (2 of 16 padded columns used)
@@ -22,10 +22,12 @@
 // Inputs of a vector v on the curve (the right-hand side -u_inc of a plane
 // wave when the network supplies the initial density, a Krylov vector when it
 // preconditions): x, y (centred, scaled), the normal, the curvature, the
-// arclength over the perimeter, k L / 2000, |v| / rms(v), and the local
+// arclength over the perimeter, k L / 2000, |v| / rms(v), the local
 // tangential wavenumber w = conj(v) dv/ds / (i k (|v|^2 + eps rms^2)) (real
 // part: d.tau for a plane wave of direction d; its derivative per patch by
-// Chebyshev differentiation on the Fejer nodes). The network outputs a
+// Chebyshev differentiation on the Fejer nodes), and e.n with e the direction
+// that fits Re w = e.tau best over the curve (least squares; e = d for a plane
+// wave, so e.n < 0 marks the lit side and > 0 the shadow). The network outputs a
 // complex multiplier field, so z is linear in v for fixed features and
 // equivariant under v -> a v (a complex); m_phys = 2c / (1 + c), c =
 // sqrt(1 - Re(w)^2), is the inverse symbol of the combined-field operator
@@ -51,7 +53,7 @@ constexpr int kW = 64;       // width
 constexpr int kModes = 32;   // Fourier modes
 constexpr int kS = 2 * kModes;  // spectral rows (re, im per mode)
 constexpr int kL = 4;        // Fourier layers
-constexpr int kCin = 10;     // input features
+constexpr int kCin = 11;     // input features
 constexpr int kCinPad = 32;  // padded to the MFMA K step
 constexpr int kOutPad = 16;  // output columns (2 used)
 
@@ -351,6 +353,53 @@ __global__ void __launch_bounds__(256)
   for (int c = kCin; c < kCinPad; c++) x[c] = 0.f;
   const double cc = sqrt(fmax(0.0, 1.0 - wr * wr));
   mphys[i] = (float)(2.0 * cc / (1.0 + cc));
+}
+
+// Least-squares direction e of Re w = e.tau over the nodes (tau = (-n_y,
+// n_x)): one block sums the normal equations in a fixed order, then every
+// node gets X[i][10] = e.n (|e| capped at 1).
+__global__ void __launch_bounds__(256)
+    k_direction_sums(long long N, const float* __restrict__ geo, const float* __restrict__ X,
+                     double* __restrict__ out) {
+  __shared__ double red[5][256];
+  double a[5] = {0, 0, 0, 0, 0};
+  for (long long i = threadIdx.x; i < N; i += 256) {
+    const double tx = -geo[i * 7 + 3], ty = geo[i * 7 + 2];
+    const double w = X[i * kCinPad + 8];
+    a[0] += tx * tx;
+    a[1] += tx * ty;
+    a[2] += ty * ty;
+    a[3] += w * tx;
+    a[4] += w * ty;
+  }
+  for (int q = 0; q < 5; q++) red[q][threadIdx.x] = a[q];
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st)
+      for (int q = 0; q < 5; q++) red[q][threadIdx.x] += red[q][threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double det = red[0][0] * red[2][0] - red[1][0] * red[1][0];
+    double ex = 0.0, ey = 0.0;
+    if (det > 1e-12 * (red[0][0] + red[2][0]) * (red[0][0] + red[2][0])) {
+      ex = (red[2][0] * red[3][0] - red[1][0] * red[4][0]) / det;
+      ey = (red[0][0] * red[4][0] - red[1][0] * red[3][0]) / det;
+    }
+    const double e = sqrt(ex * ex + ey * ey);
+    if (e > 1.0) {
+      ex /= e;
+      ey /= e;
+    }
+    out[1] = ex;
+    out[2] = ey;
+  }
+}
+
+__global__ void __launch_bounds__(256)
+    k_direction_feature(long long N, const float* __restrict__ geo, const double* __restrict__ e, float* __restrict__ X) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < N) X[i * kCinPad + 10] = (float)(e[1] * geo[i * 7 + 2] + e[2] * geo[i * 7 + 3]);
 }
 
 // out[0] = sum |v|^2 (one block, fixed order)
@@ -772,6 +821,8 @@ struct ifgf_fno {
     hipLaunchKernelGGL(k_sumsq, dim3(1), dim3(256), 0, st, N, v, stats.p);
     hipLaunchKernelGGL(k_features, dim3(nblk(N)), dim3(256), 0, st, N, F.Np, F.geo.p, F.dfac.p, F.D.p, v, stats.p,
                        F.k, X.p, mph.p);
+    hipLaunchKernelGGL(k_direction_sums, dim3(1), dim3(256), 0, st, N, F.geo.p, X.p, stats.p);
+    hipLaunchKernelGGL(k_direction_feature, dim3(nblk(N)), dim3(256), 0, st, N, F.geo.p, stats.p, X.p);
     const long long off = 0;
     const bf16_t* T = F.T.p;
     const bf16_t* Tw = F.Tw.p;
@@ -905,6 +956,8 @@ int ifgf_fno_add_sample(ifgf_fno* F, ifgf_plan* P, const double* v, const double
     hipLaunchKernelGGL(k_sumsq, dim3(1), dim3(256), 0, F->st, N, S->v.p, F->stats.p);
     hipLaunchKernelGGL(k_features, dim3(nblk(N)), dim3(256), 0, F->st, N, G.Np, G.geo.p, G.dfac.p, G.D.p, S->v.p,
                        F->stats.p, G.k, S->X.p, S->mphys.p);
+    hipLaunchKernelGGL(k_direction_sums, dim3(1), dim3(256), 0, F->st, N, G.geo.p, S->X.p, F->stats.p);
+    hipLaunchKernelGGL(k_direction_feature, dim3(nblk(N)), dim3(256), 0, F->st, N, G.geo.p, F->stats.p, S->X.p);
     HIP_TRY(hipMemcpyAsync(S->T.p, G.T.p, (size_t)N * kS * sizeof(bf16_t), hipMemcpyDeviceToDevice, F->st));
     HIP_TRY(hipMemcpyAsync(S->Tw.p, G.Tw.p, (size_t)N * kS * sizeof(bf16_t), hipMemcpyDeviceToDevice, F->st));
     HIP_TRY(hipGetLastError());
@@ -927,7 +980,7 @@ int ifgf_fno_clear_samples(ifgf_fno* F) {
 }
 
 // Features, multiplier prior, node parameters and parameter quadrature
-// weights of sample s (host copies; any output may be NULL): X [N][10],
+// weights of sample s (host copies; any output may be NULL): X [N][11],
 // mphys [N], t [N], wt [N].
 int ifgf_fno_sample_data(const ifgf_fno* F, int s, long long* N, float* X, float* mphys, double* t, double* wt) {
   if (!F || s < 0 || s >= (int)F->samples.size() || !N) return fno_fail(nullptr, IFGF_ERR_ARG, "invalid argument");

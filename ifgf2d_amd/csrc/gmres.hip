@@ -23,6 +23,7 @@
 #include <chrono>
 #include <cmath>
 #include <complex>
+#include <functional>
 #include <memory>
 #include <vector>
 
@@ -1084,7 +1085,27 @@ namespace {
 // stagnation: once more than `window` estimates exist, an estimate above
 // ratio x the one `window` iterations earlier (the caller then falls back to
 // plain GMRES from the current x). Returns the iterations taken.
-int fgmres_learned(Solver& S, Krylov& K, zd* Z, ifgf_fno* F, const zd* b, zd* x, int m, double tol, int maxit,
+using Precond = std::function<void(const zd* v, zd* z)>;
+
+// z = m v elementwise (the fixed multiplier field of IFGF_FNO_FIXED_MULTIPLIER)
+__global__ void __launch_bounds__(256)
+    k_cmul_field(long long n, const zd* __restrict__ m, const zd* __restrict__ v, zd* __restrict__ z) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) z[i] = cmul(m[i], v[i]);
+}
+
+// m = z / v where |v|^2 > 1e-6 mean |v|^2 (else 1): the multiplier the
+// network applied to v (z = M(v) = m v)
+__global__ void __launch_bounds__(256)
+    k_field_ratio(long long n, const zd* __restrict__ z, const zd* __restrict__ v, const double* __restrict__ vnorm2,
+                  zd* __restrict__ m) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double a = v[i].x * v[i].x + v[i].y * v[i].y;
+  m[i] = a > 1e-6 * vnorm2[0] / (double)n ? cdiv(z[i], v[i]) : zd{1.0, 0.0};
+}
+
+int fgmres_learned(Solver& S, Krylov& K, zd* Z, const Precond& M, const zd* b, zd* x, int m, double tol, int maxit,
                    double ref_norm, int window, double ratio, double* hist, int hist_cap, int hist_off, bool* conv,
                    bool* stalled) {
   using C = std::complex<double>;
@@ -1105,7 +1126,7 @@ int fgmres_learned(Solver& S, Krylov& K, zd* Z, ifgf_fno* F, const zd* b, zd* x,
     for (; j < m && it < maxit; j++) {
       it++;
       zd* zj = Z + (size_t)j * n;
-      if (fno_apply_device(F, S.P, K.V.p + (size_t)j * n, zj, S.st)) throw std::runtime_error(S.P->err);
+      M(K.V.p + (size_t)j * n, zj);
       S.apply(zj, K.w.p, false);
       S.arnoldi_enqueue(K, j);
       S.arnoldi_wait(K, j, hv);
@@ -1182,9 +1203,27 @@ extern "C" int ifgf_fno_gmres(ifgf_plan* P, ifgf_fno* F, const double* rhs, doub
     int it = 0;
     if (mode & IFGF_FNO_PRECONDITION) {
       bool stalled = false;
-      it = fgmres_learned(S, K, W.Z.p, F, b.p, xd.p, restart, tol, maxit, bnorm, window, ratio, hist, hist_cap, 0,
+      DevBuf<zd> mfield;
+      Precond M = [&](const zd* v, zd* z) {
+        if (fno_apply_device(F, P, v, z, S.st)) throw std::runtime_error(P->err);
+      };
+      if (mode & IFGF_FNO_FIXED_MULTIPLIER) {
+        // the multiplier the network applies to b, kept for every Krylov vector
+        mfield.alloc(n);
+        if (fno_apply_device(F, P, b.p, K.tmp.p, S.st)) throw std::runtime_error(P->err);
+        hipLaunchKernelGGL(k_dots, dim3(kDot, 1), dim3(256), 0, S.st, n, b.p, n, b.p, K.part.p, -1,
+                           (const int*)nullptr);
+        hipLaunchKernelGGL(k_sum0, dim3(1), dim3(64), 0, S.st, K.part.p, K.h.p);
+        hipLaunchKernelGGL(k_field_ratio, dim3(S.nb), dim3(256), 0, S.st, n, K.tmp.p, b.p, (const double*)K.h.p,
+                           mfield.p);
+        M = [&](const zd* v, zd* z) {
+          hipLaunchKernelGGL(k_cmul_field, dim3(S.nb), dim3(256), 0, S.st, n, (const zd*)mfield.p, v, z);
+        };
+      }
+      it = fgmres_learned(S, K, W.Z.p, M, b.p, xd.p, restart, tol, maxit, bnorm, window, ratio, hist, hist_cap, 0,
                           &converged, &stalled);
       if (stalled && !converged && it < maxit) *fallback = 1;
+      zero = false;  // the flexible iterations have updated x
     }
     if (!converged && it < maxit) {
       // plain GMRES (IdentityPreconditioner, opGMRES.hpp:274) from the current

@@ -345,6 +345,8 @@ int ifgf_plan_solver_times(const ifgf_plan* plan, double seconds[4]);
 typedef struct ifgf_fno ifgf_fno;
 #define IFGF_FNO_GUESS 1        /* ifgf_fno_gmres: initial density M(b) (when x is zero) */
 #define IFGF_FNO_PRECONDITION 2 /* ifgf_fno_gmres: flexible right preconditioner z_j = M(v_j) */
+#define IFGF_FNO_FIXED_MULTIPLIER 4 /* with IFGF_FNO_PRECONDITION: z_j = m v_j with the multiplier
+                                        field m = M(b) / b the network gives the right-hand side */
 
 long long ifgf_fno_num_params(void);
 /* seeded uniform initialisation (splitmix64) */
@@ -360,7 +362,7 @@ int ifgf_fno_set_params(ifgf_fno* fno, const float* in);
 int ifgf_fno_add_sample(ifgf_fno* fno, ifgf_plan* plan, const double* v, const double* density);
 int ifgf_fno_num_samples(const ifgf_fno* fno);
 int ifgf_fno_clear_samples(ifgf_fno* fno);
-/* Sample s's features X [N][10], prior multiplier [N], node parameters t [N]
+/* Sample s's features X [N][11], prior multiplier [N], node parameters t [N]
  * and parameter quadrature weights [N] (host; any may be NULL). */
 int ifgf_fno_sample_data(const ifgf_fno* fno, int s, long long* N, float* X, float* mphys, double* t, double* wt);
 /* Mean over samples ids[0..n) of ||M(v) - density||^2 / ||density||^2, and

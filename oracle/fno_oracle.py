@@ -5,7 +5,7 @@ and backward kernels (ifgf2d_amd/csrc/fno.hip), never by the product. The
 network has no reference counterpart (the reference's GMRES takes an identity
 preconditioner, ForwardMap/opGMRES.hpp:103,153,251,274).
 
-forward/backward restate the device network exactly: lifting of the 10
+forward/backward restate the device network exactly: lifting of the 11
 features (padded to 32), four Fourier layers P_l = IDFT(mix_l(DFT(H_l))) +
 H_l W_l + b_l with GELU between them, projection to the complex multiplier m =
 m_phys + O_0 + i O_1, output z = m v, loss ||z - target||^2 / ||target||^2
@@ -18,7 +18,7 @@ by central finite differences of the float64 loss (tests/test_fno_oracle.py).
 import numpy as np
 from scipy.special import erf
 
-W, MODES, S, L, CIN, CINPAD, OUTPAD = 64, 32, 64, 4, 10, 32, 16
+W, MODES, S, L, CIN, CINPAD, OUTPAD = 64, 32, 64, 4, 11, 32, 16
 
 
 def layout():
@@ -136,7 +136,7 @@ def forward_sample(P, X, T, Tw, emulate):
 
 
 def loss_grad(params, samples, emulate=False, grad=True):
-    """samples: list of dicts with X (N x 10), mphys, t, wt, v, target.
+    """samples: list of dicts with X (N x 11), mphys, t, wt, v, target.
     Returns (mean loss, flat gradient or None)."""
     P = unpack(params)
     q = bf16 if emulate else (lambda a: a)
