@@ -441,10 +441,51 @@ def main():
             grp.launch()
         grp.synchronize()
         t_grp = (time.perf_counter() - t0) / reps
-        group = {"obstacles": len(maps), "N": [m.N for m in maps],
-                 "sequential_ms_per_matvec": round(t_seq * 1e3, 4), "group_ms_per_matvec": round(t_grp * 1e3, 4)}
-        log(f"16 random stars: sequential {t_seq * 1e3:.3f} ms, one group launch {t_grp * 1e3:.3f} ms per matvec")
+        gst = grp.stats()
+        group = {"obstacles": len(maps), "N": [m.N for m in maps], "nodes_per_wavelength": 10,
+                 "sequential_ms_per_matvec": round(t_seq * 1e3, 4), "group_ms_per_matvec": round(t_grp * 1e3, 4),
+                 "speedup": round(t_seq / t_grp, 2), "group_launches": gst["launches"],
+                 "group_workgroups": gst["workgroups"]}
+        log(f"16 random stars: sequential {t_seq * 1e3:.3f} ms, one group launch {t_grp * 1e3:.3f} ms per matvec "
+            f"({gst['launches']} fused kernel launches)")
         grp.close()
+        # cone orders chosen per obstacle by the autotuner (relative error 1e-4
+        # against the unaccelerated operator), fused per class of equal order
+        t0 = time.time()
+        orders = []
+        for kw_ in configs.random_stars(16, k=100.0, seed=0):
+            ps_, pa_, e_, ok_ = I.autotune_cone_order(kw_["curve"], kw_["patch_lims"], kw_["k"], kw_["nlevels"],
+                                                      1e-4, p=kw_["p"])
+            orders.append((ps_, pa_, e_))
+        t_tune = time.time() - t0
+        maps2 = [I.ForwardMap(**kw_, Ps=o_[0], Pang=o_[1], autotune=True)
+                 for kw_, o_ in zip(configs.random_stars(16, k=100.0, seed=0), orders)]
+        for m, a_, b_ in zip(maps2, gx, gy):
+            m.apply_device(a_.data_ptr(), b_.data_ptr())
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            for m, a_, b_ in zip(maps2, gx, gy):
+                m.apply_device(a_.data_ptr(), b_.data_ptr())
+        torch.cuda.synchronize()
+        t_seq2 = (time.perf_counter() - t0) / reps
+        grp2 = I.ObstacleGroup(maps2, [v.data_ptr() for v in gx], [v.data_ptr() for v in gy])
+        grp2.launch()
+        grp2.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            grp2.launch()
+        grp2.synchronize()
+        t_grp2 = (time.perf_counter() - t0) / reps
+        group["autotuned_cone_orders"] = {
+            "tol": 1e-4, "orders": [f"{o_[0]}x{o_[1]}" for o_ in orders],
+            "errors_max": max(o_[2] for o_ in orders), "autotune_s": round(t_tune, 3),
+            "classes": grp2.stats()["classes"], "sequential_ms_per_matvec": round(t_seq2 * 1e3, 4),
+            "group_ms_per_matvec": round(t_grp2 * 1e3, 4)}
+        log(f"16 random stars, autotuned cone orders (1e-4): {sorted(set(group['autotuned_cone_orders']['orders']))}, "
+            f"sequential {t_seq2 * 1e3:.3f} ms, group {t_grp2 * 1e3:.3f} ms")
+        grp2.close()
+        del maps2
         del maps, gx, gy
 
     # the scattered field on a 256 x 256 grid over the scatterer (65,536 targets)

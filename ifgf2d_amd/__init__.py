@@ -81,7 +81,7 @@ EXPORTS = [
     "ifgf_propagate_noalloc", "ifgf_fno_num_params", "ifgf_fno_create", "ifgf_fno_destroy", "ifgf_fno_error",
     "ifgf_fno_get_params", "ifgf_fno_set_params", "ifgf_fno_add_sample", "ifgf_fno_num_samples",
     "ifgf_fno_clear_samples", "ifgf_fno_sample_data", "ifgf_fno_loss_grad", "ifgf_fno_train", "ifgf_fno_apply",
-    "ifgf_fno_gemm_check", "ifgf_fno_gmres",
+    "ifgf_fno_gemm_check", "ifgf_fno_gmres", "ifgf_group_stats", "ifgf_autotune_cone_order",
 ]
 
 _LIB = None
@@ -163,6 +163,9 @@ def lib():
         L.ifgf_fno_gmres.argtypes = [ctypes.c_void_p, ctypes.c_void_p, _dp, _dp, ctypes.c_int, ctypes.c_double,
                                      ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double, _ip, _ip, _dp, _dp,
                                      ctypes.c_int]
+        L.ifgf_group_stats.argtypes = [ctypes.c_void_p, _lp]
+        L.ifgf_autotune_cone_order.argtypes = [ctypes.POINTER(_Surface), ctypes.POINTER(_Params), ctypes.c_double, _ip,
+                                               _ip, _dp]
         L.ifgf_plan_profile.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.ifgf_plan_stage_times.argtypes = [ctypes.c_void_p, _dp, _lp]
         L.ifgf_plan_setup_times.argtypes = [ctypes.c_void_p, _dp]
@@ -634,6 +637,14 @@ class ObstacleGroup:
         if rc != IFGF_OK:
             raise IFGFError(lib().ifgf_last_error().decode())
 
+    def stats(self):
+        """Plans, classes, kernel launches and workgroups per group application."""
+        s = np.empty(4, dtype=np.int64)
+        rc = lib().ifgf_group_stats(self._h, s.ctypes.data_as(_lp))
+        if rc != IFGF_OK:
+            raise IFGFError(lib().ifgf_last_error().decode())
+        return dict(zip(("plans", "classes", "launches", "workgroups"), s.tolist()))
+
     def synchronize(self):
         rc = lib().ifgf_group_synchronize(self._h)
         if rc != IFGF_OK:
@@ -803,6 +814,27 @@ def gmres(op, rhs, restart=50, tol=1e-8, maxit=300, x0=None):
     if rc not in (IFGF_OK, IFGF_ERR_NOT_CONVERGED):
         raise IFGFError(lib().ifgf_last_error().decode())
     return x, it.value, err.value, rc == IFGF_OK
+
+
+def autotune_cone_order(curve, patch_lims, k, nlevels, tol, Np=10, formulation="combined", eta=None, delta=None,
+                        p=4.0, precision="fp64", device_setup=True):
+    """The cone-order autotuner (ifgf_autotune_cone_order): the cheapest (Ps, Pang)
+    whose forward map is within tol of the unaccelerated operator. Returns
+    (Ps, Pang, error, reached)."""
+    lims = np.ascontiguousarray(patch_lims, dtype=np.float64)
+    cp = np.ascontiguousarray(curve[1], dtype=np.float64)
+    delta = default_delta(curve, lims) if delta is None else float(delta)
+    surf = _Surface(curve[0], _d(cp), cp.size, _d(lims), lims.size - 1, Np, delta, float(p))
+    flags = (IFGF_PLAN_DEVICE_SORT | IFGF_PLAN_DEVICE_PRECOMP | IFGF_PLAN_DEVICE_GEOMETRY
+             | IFGF_PLAN_DEVICE_TREE) if device_setup else 0
+    prm = _Params(_FORMS[formulation], _PREC[precision], float(k), float(k if eta is None else eta), 5, 5, nlevels, 1,
+                  flags)
+    a, b, e = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+    rc = lib().ifgf_autotune_cone_order(ctypes.byref(surf), ctypes.byref(prm), tol, ctypes.byref(a), ctypes.byref(b),
+                                        ctypes.byref(e))
+    if rc not in (IFGF_OK, IFGF_ERR_NOT_CONVERGED):
+        raise IFGFError(f"ifgf_autotune_cone_order failed ({rc}): {lib().ifgf_last_error().decode()}")
+    return a.value, b.value, e.value, rc == IFGF_OK
 
 
 def morton_sort(x, y, nlevels):

@@ -134,13 +134,23 @@ __device__ inline Z cone_transform(const Z* buf, Z* tmp, const TT& T, int j, boo
 }
 
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_gather(int N, const zd* __restrict__ x,
-                                                const double* __restrict__ w,
-                                                const int* __restrict__ perm, zd* __restrict__ rho) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// Every stored-path kernel below is a __device__ body that takes its logical
+// block index lb explicitly, and a __global__ wrapper that passes xcd_block()
+// (or blockIdx.x): the wrappers run one plan, the fused launches of an
+// obstacle group (plan.hip) run every plan's blocks in one grid, each block
+// calling the body with its plan's arguments and its block index within the
+// plan, so both do the same operations per block.
+__device__ inline void gather_body(int lb, int N, const zd* __restrict__ x, const double* __restrict__ w,
+                                   const int* __restrict__ perm, zd* __restrict__ rho) {
+  const int i = lb * blockDim.x + threadIdx.x;
   if (i >= N) return;
   const int o = perm[i];
   rho[i] = cscale(x[o], w[o]);
+}
+__global__ void __launch_bounds__(256) k_gather(int N, const zd* __restrict__ x,
+                                                const double* __restrict__ w,
+                                                const int* __restrict__ perm, zd* __restrict__ rho) {
+  gather_body(blockIdx.x, N, x, w, perm, rho);
 }
 
 // Sum of S lanes' partial results (lanes i*S .. i*S+S-1 of a wave), fixed order.
@@ -315,10 +325,10 @@ __global__ void __launch_bounds__(256)
 // densities), so the stream is not held up by the longest rows.
 constexpr int kNearItemChunks = 8;
 
-__global__ void __launch_bounds__(256)
-    k_near_items(int N, int nitems, const NearItem* __restrict__ items, const int4* __restrict__ nf_seg,
-                 const zd* __restrict__ nf_val, const zd* __restrict__ rho, zd* __restrict__ npart) {
-  const int w = xcd_block() * 4 + threadIdx.x / 64;
+__device__ inline void near_items_body(int lb, int N, int nitems, const NearItem* __restrict__ items,
+                                       const int4* __restrict__ nf_seg, const zd* __restrict__ nf_val,
+                                       const zd* __restrict__ rho, zd* __restrict__ npart) {
+  const int w = lb * 4 + threadIdx.x / 64;
   const int lane = threadIdx.x % 64;
   const int t = lane / kNearChunk, sl = lane % kNearChunk;
   zd acc{0.0, 0.0};
@@ -350,6 +360,11 @@ __global__ void __launch_bounds__(256)
   }
   acc = lane_group_sum<kNearChunk>(acc);
   if (w < nitems && sl == 0) npart[(size_t)w * kNearGroup + t] = acc;
+}
+__global__ void __launch_bounds__(256)
+    k_near_items(int N, int nitems, const NearItem* __restrict__ items, const int4* __restrict__ nf_seg,
+                 const zd* __restrict__ nf_val, const zd* __restrict__ rho, zd* __restrict__ npart) {
+  near_items_body(xcd_block(), N, nitems, items, nf_seg, nf_val, rho, npart);
 }
 
 // Leaf operator L_c (P x n_src per leaf cone c). Storage: the sources of a
@@ -441,12 +456,11 @@ __global__ void __launch_bounds__(256)
 constexpr int kLeafItemTiles = 4;
 
 template <int P, typename Z>
-__global__ void __launch_bounds__(256)
-    k_leaf_items(int nitems, const LeafItem* __restrict__ items, const Z* __restrict__ lf,
-                 const zd* __restrict__ rho, Z* __restrict__ lpart) {
+__device__ inline void leaf_items_body(int lb, int nitems, const LeafItem* __restrict__ items,
+                                       const Z* __restrict__ lf, const zd* __restrict__ rho, Z* __restrict__ lpart) {
   constexpr int S = kLeafChunk;
   using V2 = typename vec2_of<Z>::type;
-  const long long gt = (long long)xcd_block() * blockDim.x + threadIdx.x;
+  const long long gt = (long long)lb * blockDim.x + threadIdx.x;
   const long long item = gt / (P * S);
   const int r = (int)(gt % (P * S));
   const int m = r / S, sl = r % S;
@@ -470,6 +484,12 @@ __global__ void __launch_bounds__(256)
   acc.x = lane_sum<S>(acc.x);
   acc.y = lane_sum<S>(acc.y);
   if (item < nitems && sl == 0) lpart[item * P + m] = acc;
+}
+template <int P, typename Z>
+__global__ void __launch_bounds__(256)
+    k_leaf_items(int nitems, const LeafItem* __restrict__ items, const Z* __restrict__ lf,
+                 const zd* __restrict__ rho, Z* __restrict__ lpart) {
+  leaf_items_body<P, Z>(xcd_block(), nitems, items, lf, rho, lpart);
 }
 
 // Stored interpolation and propagation geometry: (s, theta, centre factor)
@@ -614,15 +634,22 @@ __device__ inline int push_entry_cone(const PushArgsT<R>& a, const PushTask& tas
 // R = double on a plain plan; float (FP32 coefficients, transforms and
 // evaluations, FP64-evaluated centre factors rounded to FP32) on a
 // mixed-precision plan.
+template <int Ps, int Pang, typename R>
+struct PushShared {
+  using Z = typename complex_of<R>::type;
+  ChebTablesT<R> T;
+  Z buf[(256 / (Ps * Pang)) * Ps * Pang];
+  Z tmp[(256 / (Ps * Pang)) * Ps * Pang];
+};
+
 template <int Ps, int Pang, int EPT, typename R>
-__global__ void __launch_bounds__(256) k_level_push(PushArgsT<R> a) {
+__device__ inline void level_push_body(int lb, const PushArgsT<R>& a, PushShared<Ps, Pang, R>& sh) {
   using Z = typename complex_of<R>::type;
   constexpr int P = Ps * Pang;
-  constexpr int CMAX = 256 / P;
-  __shared__ ChebTablesT<R> T;
-  __shared__ Z buf[CMAX * P];
-  __shared__ Z tmp[CMAX * P];
-  const PushTask task = a.tasks[xcd_block()];
+  ChebTablesT<R>& T = sh.T;
+  Z* buf = sh.buf;
+  Z* tmp = sh.tmp;
+  const PushTask task = a.tasks[lb];
   const int tid = threadIdx.x;
   if (!a.leaf) load_transform_rows<Ps, Pang>(T, a.tabs);
   const int cl = tid / P, j = tid % P;
@@ -663,6 +690,11 @@ __global__ void __launch_bounds__(256) k_level_push(PushArgsT<R> a) {
       a.pbuf[1 + task.e0 + e] = zmul(val, Z{G[u].fre, G[u].fim});
     }
   }
+}
+template <int Ps, int Pang, int EPT, typename R>
+__global__ void __launch_bounds__(256) k_level_push(PushArgsT<R> a) {
+  __shared__ PushShared<Ps, Pang, R> sh;
+  level_push_body<Ps, Pang, EPT, R>(xcd_block(), a, sh);
 }
 
 // FP64 -> FP32 copies of the stored operators of a mixed-precision plan
@@ -777,14 +809,13 @@ __global__ void __launch_bounds__(256)
 // pre[2i] = ident x + correction and pre[2i+1] = near; stage 2 adds the
 // interpolation entries: y[i] = pre[2i] + (pre[2i+1] + far).
 template <int S, typename ZI>
-__global__ void __launch_bounds__(256)
-    k_finalize(int N, int Np, double ident, const zd* __restrict__ x,
-               const long long* __restrict__ corr_ptr, const int* __restrict__ corr_patch,
-               const zd* __restrict__ corr_w, const int* __restrict__ inv, const zd* __restrict__ sol,
-               const int* __restrict__ nio, const zd* __restrict__ npart,
-               const long long* __restrict__ it_ptr, const int* __restrict__ it_slot, const ZI* __restrict__ slots,
-               zd* __restrict__ y, int stage = 0, zd* __restrict__ pre = nullptr) {
-  const int lb = xcd_block();
+__device__ inline void finalize_body(int lb, int N, int Np, double ident, const zd* __restrict__ x,
+                                     const long long* __restrict__ corr_ptr, const int* __restrict__ corr_patch,
+                                     const zd* __restrict__ corr_w, const int* __restrict__ inv,
+                                     const zd* __restrict__ sol, const int* __restrict__ nio,
+                                     const zd* __restrict__ npart, const long long* __restrict__ it_ptr,
+                                     const int* __restrict__ it_slot, const ZI* __restrict__ slots,
+                                     zd* __restrict__ y, int stage, zd* __restrict__ pre) {
   const int gt = lb * blockDim.x + threadIdx.x;
   const int i = gt / S, sl = gt % S;
   zd acc{0.0, 0.0}, near{0.0, 0.0}, far{0.0, 0.0};
@@ -860,6 +891,17 @@ __global__ void __launch_bounds__(256)
       y[i] = cadd(pre[2 * (size_t)i], cadd(pre[2 * (size_t)i + 1], far));
     }
   }
+}
+template <int S, typename ZI>
+__global__ void __launch_bounds__(256)
+    k_finalize(int N, int Np, double ident, const zd* __restrict__ x,
+               const long long* __restrict__ corr_ptr, const int* __restrict__ corr_patch,
+               const zd* __restrict__ corr_w, const int* __restrict__ inv, const zd* __restrict__ sol,
+               const int* __restrict__ nio, const zd* __restrict__ npart,
+               const long long* __restrict__ it_ptr, const int* __restrict__ it_slot, const ZI* __restrict__ slots,
+               zd* __restrict__ y, int stage = 0, zd* __restrict__ pre = nullptr) {
+  finalize_body<S, ZI>(xcd_block(), N, Np, ident, x, corr_ptr, corr_patch, corr_w, inv, sol, nio, npart, it_ptr,
+                       it_slot, slots, y, stage, pre);
 }
 
 // Direct sum at a target subset, excluding the same pairs the correction

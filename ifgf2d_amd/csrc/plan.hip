@@ -1389,21 +1389,296 @@ int ifgf_apply_graph(ifgf_plan* P, const double* x, double* y, int nrhs, long lo
 }  // extern "C"
 
 // A group of plans (obstacles) applied as one captured graph.
+// ---- obstacle groups: every stage of every plan in one launch --------------
+// A fused launch runs the blocks of all plans of a class (same (Ps, Pang) and
+// precision; the stored-path kernels do not depend on the formulation) in one
+// grid: block b finds its plan i from the block offsets boff (the largest i
+// with boff[i] <= b), loads that plan's arguments and runs the single-plan
+// kernel's body with block index b - boff[i]. A group application is then
+// gather, near field, leaf operator, one push launch per tree depth (depth d
+// of a plan with D levels is its level D - d) and the finalize (one launch per
+// finalize lane count in the class), all captured into one HIP graph.
+namespace {
+
+struct GatherArgs {
+  int N;
+  const zd* x;
+  const double* w;
+  const int* perm;
+  zd* rho;
+};
+struct NearArgs {
+  int N, nitems;
+  const NearItem* items;
+  const int4* nf_seg;
+  const zd* nf_val;
+  const zd* rho;
+  zd* npart;
+};
+template <typename Z>
+struct LeafArgs {
+  int nitems;
+  const LeafItem* items;
+  const Z* lf;
+  const zd* rho;
+  Z* lpart;
+};
+template <typename Z>
+struct FinArgs {
+  int N, Np;
+  double ident;
+  const zd* x;
+  const long long* corr_ptr;
+  const int* corr_patch;
+  const zd* corr_w;
+  const int* inv;
+  const zd* sol;
+  const int* nio;
+  const zd* npart;
+  const long long* it_ptr;
+  const int* it_slot;
+  const Z* slots;
+  zd* y;
+};
+
+__device__ inline int group_plan(const int* __restrict__ boff, int n, int b) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (boff[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ void __launch_bounds__(256) k_group_gather(const GatherArgs* __restrict__ a, const int* __restrict__ boff,
+                                                      int n) {
+  const int b = xcd_block(), i = group_plan(boff, n, b);
+  const GatherArgs A = a[i];
+  gather_body(b - boff[i], A.N, A.x, A.w, A.perm, A.rho);
+}
+__global__ void __launch_bounds__(256) k_group_near(const NearArgs* __restrict__ a, const int* __restrict__ boff,
+                                                    int n) {
+  const int b = xcd_block(), i = group_plan(boff, n, b);
+  const NearArgs A = a[i];
+  near_items_body(b - boff[i], A.N, A.nitems, A.items, A.nf_seg, A.nf_val, A.rho, A.npart);
+}
+template <int P, typename Z>
+__global__ void __launch_bounds__(256) k_group_leaf(const LeafArgs<Z>* __restrict__ a, const int* __restrict__ boff,
+                                                    int n) {
+  const int b = xcd_block(), i = group_plan(boff, n, b);
+  const LeafArgs<Z> A = a[i];
+  leaf_items_body<P, Z>(b - boff[i], A.nitems, A.items, A.lf, A.rho, A.lpart);
+}
+template <int Ps, int Pang, int EPT, typename R>
+__global__ void __launch_bounds__(256) k_group_push(const PushArgsT<R>* __restrict__ a, const int* __restrict__ boff,
+                                                    int n) {
+  __shared__ PushShared<Ps, Pang, R> sh;
+  const int b = xcd_block(), i = group_plan(boff, n, b);
+  const PushArgsT<R> A = a[i];
+  level_push_body<Ps, Pang, EPT, R>(b - boff[i], A, sh);
+}
+template <int S, typename Z>
+__global__ void __launch_bounds__(256) k_group_fin(const FinArgs<Z>* __restrict__ a, const int* __restrict__ boff,
+                                                   int n) {
+  const int b = xcd_block(), i = group_plan(boff, n, b);
+  const FinArgs<Z> A = a[i];
+  finalize_body<S, Z>(b - boff[i], A.N, A.Np, A.ident, A.x, A.corr_ptr, A.corr_patch, A.corr_w, A.inv, A.sol, A.nio,
+                      A.npart, A.it_ptr, A.it_slot, A.slots, A.y, 0, nullptr);
+}
+
+// One fused stage: per-plan arguments and block offsets on the device.
+struct Stage {
+  DevBuf<unsigned char> args;
+  DevBuf<int> boff;
+  int n = 0, blocks = 0;
+  template <typename A>
+  void set(const std::vector<A>& a, const std::vector<int>& nb) {
+    n = (int)a.size();
+    std::vector<int> off(n + 1, 0);
+    for (int i = 0; i < n; i++) off[i + 1] = off[i] + nb[i];
+    blocks = off[n];
+    std::vector<unsigned char> raw(sizeof(A) * a.size());
+    if (!a.empty()) std::memcpy(raw.data(), a.data(), raw.size());
+    args.upload(raw);
+    boff.upload(off);
+  }
+  template <typename A>
+  const A* p() const {
+    return reinterpret_cast<const A*>(args.p);
+  }
+};
+
+// Plans of one class and its fused stages.
+struct GroupClass {
+  int Ps = 0, Pang = 0, ept = 1;
+  bool f32 = false;
+  std::vector<int> plans;  // indices into the group's plans
+  Stage gather, near, leaf;
+  std::vector<Stage> push;  // per depth
+  Stage fin[3];             // finalize lanes 4, 8, 16
+};
+
+}  // namespace
+
 struct ifgf_group {
   std::vector<ifgf_plan*> plans;
+  std::vector<GroupClass> classes;
   hipStream_t stream = nullptr;
-  std::vector<hipEvent_t> events;  // per plan: its branch's end; last: the fork
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   std::string err;
   ~ifgf_group() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
-    for (auto& e : events)
-      if (e) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
+
+namespace {
+
+template <typename R>
+PushArgsT<R> push_args_of(ifgf_plan* P, int li) {
+  const int D = P->tree.D;
+  PushArgsT<R> a{};
+  if constexpr (std::is_same<R, float>::value) {
+    a = PushArgsT<R>{nullptr, nullptr, P->push_geo32.p, nullptr, P->push_coff.p, P->push_cidx.p,
+                     P->leaf_item_off.p, P->lpart32.p, P->tree.lv[D].cone_off, P->ifac_off.p, P->ifac_tab32.p,
+                     P->tabs32.p, P->pbuf32.p, 0};
+  } else {
+    const bool direct = P->push_geo_b.n > 0;
+    a = PushArgsT<R>{nullptr, direct ? nullptr : P->push_einfo.p, direct ? P->push_geo_b.p : P->push_igeo.p,
+                     direct ? nullptr : P->push_ptab.p, P->push_coff.p, P->push_cidx.p,
+                     P->leaf_item_off.p, P->lpart.p, P->tree.lv[D].cone_off, P->ifac_off.p, P->ifac_tab.p,
+                     P->tabs.p, P->pbuf.p, 0};
+  }
+  a.tasks = P->push_tasks.p + P->push_level_off[li];
+  a.leaf = li == 0;
+  return a;
+}
+
+// Builds the fused stages of class C (plans of precision R, P = Ps * Pang).
+template <int P_, typename R>
+void build_class(GroupClass& C, const std::vector<ifgf_plan*>& plans, const double* const* x, double* const* y) {
+  using Z = typename complex_of<R>::type;
+  constexpr bool f32 = std::is_same<R, float>::value;
+  std::vector<GatherArgs> ga;
+  std::vector<NearArgs> na;
+  std::vector<LeafArgs<Z>> la;
+  std::vector<int> gb, nbk, lb;
+  int maxd = 0;
+  C.ept = 1;
+  for (int q : C.plans) {
+    ifgf_plan* P = plans[q];
+    const int N = (int)P->tree.N;
+    ga.push_back(GatherArgs{N, (const zd*)x[q], P->w.p, P->perm.p, P->rho.p});
+    gb.push_back((N + 255) / 256);
+    const int M = (int)P->near_items.n;
+    na.push_back(NearArgs{(int)P->tree.M, M, P->near_items.p, P->nf_seg.p, P->nf_val.p, P->rho.p, P->npart.p});
+    nbk.push_back((M + 3) / 4);
+    const long long L = P->leaf_items.n;
+    la.push_back(LeafArgs<Z>{(int)L, P->leaf_items.p, f32 ? (const Z*)P->lf_val32.p : (const Z*)P->lf_val.p,
+                             P->rho.p, f32 ? (Z*)P->lpart32.p : (Z*)P->lpart.p});
+    lb.push_back((int)((L * P_ * kLeafChunk + 255) / 256));
+    maxd = std::max(maxd, P->tree.D - 1);
+    C.ept = std::max(C.ept, P->push_emax / 256);
+  }
+  C.gather.set(ga, gb);
+  C.near.set(na, nbk);
+  C.leaf.set(la, lb);
+  C.push.resize(maxd);
+  for (int li = 0; li < maxd; li++) {
+    std::vector<PushArgsT<R>> pa;
+    std::vector<int> pb;
+    for (int q : C.plans) {
+      ifgf_plan* P = plans[q];
+      const bool has = li + 1 < P->tree.D;
+      pa.push_back(has ? push_args_of<R>(P, li) : PushArgsT<R>{});
+      pb.push_back(has ? P->push_level_off[li + 1] - P->push_level_off[li] : 0);
+    }
+    C.push[li].set(pa, pb);
+  }
+  for (int si = 0; si < 3; si++) {
+    const int S = 4 << si;
+    std::vector<FinArgs<Z>> fa;
+    std::vector<int> fb;
+    for (int q : C.plans) {
+      ifgf_plan* P = plans[q];
+      const bool mine = P->final_lanes == S;
+      const int NT = (int)P->tree.M;
+      const double ident = P->prm.formulation == kSingleLayer ? 0.0 : 0.5;
+      fa.push_back(FinArgs<Z>{NT, P->geo.Np, ident, (const zd*)x[q], P->corr_ptr.p, P->corr_patch.p, P->corr_w.p,
+                              P->inv.p, P->sol.p, P->near_item_off.p, P->npart.p, P->it_ptr.p, P->it_slot.p,
+                              f32 ? (const Z*)P->pbuf32.p : (const Z*)P->pbuf.p, (zd*)y[q]});
+      fb.push_back(mine ? (NT + 256 / S - 1) / (256 / S) : 0);
+    }
+    C.fin[si].set(fa, fb);
+  }
+}
+
+template <int Ps, int Pang, int EPT, typename R>
+void launch_class_push(const GroupClass& C, hipStream_t st) {
+  for (const Stage& S : C.push)
+    if (S.blocks)
+      hipLaunchKernelGGL((k_group_push<Ps, Pang, EPT, R>), dim3(S.blocks), dim3(256), 0, st, S.p<PushArgsT<R>>(),
+                         S.boff.p, S.n);
+}
+
+template <int Ps, int Pang, typename R>
+void launch_class(const GroupClass& C, hipStream_t st) {
+  using Z = typename complex_of<R>::type;
+  constexpr int P = Ps * Pang;
+  if (C.gather.blocks)
+    hipLaunchKernelGGL(k_group_gather, dim3(C.gather.blocks), dim3(256), 0, st, C.gather.p<GatherArgs>(),
+                       C.gather.boff.p, C.gather.n);
+  if (C.near.blocks)
+    hipLaunchKernelGGL(k_group_near, dim3(C.near.blocks), dim3(256), 0, st, C.near.p<NearArgs>(), C.near.boff.p,
+                       C.near.n);
+  if (C.leaf.blocks)
+    hipLaunchKernelGGL((k_group_leaf<P, Z>), dim3(C.leaf.blocks), dim3(256), 0, st, C.leaf.p<LeafArgs<Z>>(),
+                       C.leaf.boff.p, C.leaf.n);
+  switch (C.ept) {
+    case 1: launch_class_push<Ps, Pang, 1, R>(C, st); break;
+    case 2: launch_class_push<Ps, Pang, 2, R>(C, st); break;
+    case 4: launch_class_push<Ps, Pang, 4, R>(C, st); break;
+    default: launch_class_push<Ps, Pang, 8, R>(C, st); break;
+  }
+  if (C.fin[0].blocks)
+    hipLaunchKernelGGL((k_group_fin<4, Z>), dim3(C.fin[0].blocks), dim3(256), 0, st, C.fin[0].p<FinArgs<Z>>(),
+                       C.fin[0].boff.p, C.fin[0].n);
+  if (C.fin[1].blocks)
+    hipLaunchKernelGGL((k_group_fin<8, Z>), dim3(C.fin[1].blocks), dim3(256), 0, st, C.fin[1].p<FinArgs<Z>>(),
+                       C.fin[1].boff.p, C.fin[1].n);
+  if (C.fin[2].blocks)
+    hipLaunchKernelGGL((k_group_fin<16, Z>), dim3(C.fin[2].blocks), dim3(256), 0, st, C.fin[2].p<FinArgs<Z>>(),
+                       C.fin[2].boff.p, C.fin[2].n);
+}
+
+// (Ps, Pang) dispatch of the class builders and launchers
+bool class_dispatch(GroupClass& C, const std::vector<ifgf_plan*>& plans, const double* const* x, double* const* y,
+                    hipStream_t st, bool build) {
+#define IFGF_GCASE(A, B)                                                          \
+  if (C.Ps == A && C.Pang == B) {                                                 \
+    if (build) {                                                                  \
+      if (C.f32) build_class<A * B, float>(C, plans, x, y);                       \
+      else build_class<A * B, double>(C, plans, x, y);                            \
+    } else {                                                                      \
+      if (C.f32) launch_class<A, B, float>(C, st);                                \
+      else launch_class<A, B, double>(C, st);                                     \
+    }                                                                             \
+    return true;                                                                  \
+  }
+  IFGF_GCASE(5, 5)
+  IFGF_GCASE(3, 5)
+  IFGF_GCASE(5, 7)
+  IFGF_GCASE(7, 7)
+  IFGF_GCASE(9, 9)
+  IFGF_GCASE(11, 11)
+#undef IFGF_GCASE
+  return false;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -1414,39 +1689,43 @@ int ifgf_group_create(ifgf_plan* const* plans, int nplans, const double* const* 
   for (int i = 0; i < nplans; i++) {
     if (!plans[i] || !x[i] || !y[i]) return fail(nullptr, IFGF_ERR_ARG, "null plan or vector");
     if (!plans[i]->on_device) return fail(nullptr, IFGF_ERR_NO_DEVICE, "plan was built host-only");
-    if (plans[i]->profile) return fail(nullptr, IFGF_ERR_ARG, "stage profiling is on (its events cannot be captured)");
+    if (!plans[i]->stored || plans[i]->field)
+      return fail(nullptr, IFGF_ERR_ARG, "group plans need stored operators (not on the fly, not a field)");
     for (int j = 0; j < i; j++)
       if (plans[j] == plans[i]) return fail(nullptr, IFGF_ERR_ARG, "a plan appears twice in the group");
   }
   std::unique_ptr<ifgf_group> G(new ifgf_group);
   try {
-    HIP_TRY(hipStreamCreate(&G->stream));
+    G->plans.assign(plans, plans + nplans);
     for (int i = 0; i < nplans; i++) HIP_TRY(hipStreamSynchronize(plans[i]->stream));
-    // one branch per obstacle: each plan's application is captured on that
-    // plan's own stream, forked from and joined back into the group's stream,
-    // so the obstacles run side by side as they do when applied eagerly
-    G->events.resize(nplans + 1, nullptr);
-    for (auto& e : G->events) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    // classes: plans of equal (Ps, Pang) and precision, in order of first appearance
+    for (int i = 0; i < nplans; i++) {
+      const ifgf_plan* P = plans[i];
+      GroupClass* C = nullptr;
+      for (auto& c : G->classes)
+        if (c.Ps == P->prm.Ps && c.Pang == P->prm.Pang && c.f32 == P->mixed) C = &c;
+      if (!C) {
+        G->classes.emplace_back();
+        C = &G->classes.back();
+        C->Ps = P->prm.Ps;
+        C->Pang = P->prm.Pang;
+        C->f32 = P->mixed;
+      }
+      C->plans.push_back(i);
+    }
+    for (auto& C : G->classes)
+      if (!class_dispatch(C, G->plans, x, y, nullptr, true))
+        return fail(nullptr, IFGF_ERR_ARG, "no kernel instantiation for a plan's (Ps, Pang)");
+    HIP_TRY(hipStreamCreate(&G->stream));
     HIP_TRY(hipStreamBeginCapture(G->stream, hipStreamCaptureModeThreadLocal));
-    int rc = IFGF_OK;
-    hipError_t ec = hipEventRecord(G->events[nplans], G->stream);
-    for (int i = 0; i < nplans && rc == IFGF_OK && ec == hipSuccess; i++) {
-      ec = hipStreamWaitEvent(plans[i]->stream, G->events[nplans], 0);
-      if (ec == hipSuccess) rc = apply_device(plans[i], (const zd*)x[i], (zd*)y[i], plans[i]->stream, false);
-      if (ec == hipSuccess && rc == IFGF_OK) ec = hipEventRecord(G->events[i], plans[i]->stream);
-      if (ec == hipSuccess && rc == IFGF_OK) ec = hipStreamWaitEvent(G->stream, G->events[i], 0);
-    }
+    for (auto& C : G->classes) class_dispatch(C, G->plans, x, y, G->stream, false);
     hipGraph_t g = nullptr;
+    const hipError_t ec = hipGetLastError();
     const hipError_t ee = hipStreamEndCapture(G->stream, &g);
-    if (ec == hipSuccess) ec = ee;
-    if (rc != IFGF_OK) {
-      if (g) (void)hipGraphDestroy(g);
-      return fail(nullptr, rc, g_last_error);
-    }
     HIP_TRY(ec);
+    HIP_TRY(ee);
     G->graph = g;
     HIP_TRY(hipGraphInstantiate(&G->exec, g, nullptr, nullptr, 0));
-    G->plans.assign(plans, plans + nplans);
   } catch (std::exception& e) {
     return fail(nullptr, IFGF_ERR_HIP, e.what());
   }
@@ -1471,6 +1750,79 @@ int ifgf_group_synchronize(ifgf_group* G) {
   } catch (std::exception& e) {
     return fail(nullptr, IFGF_ERR_HIP, e.what());
   }
+  return IFGF_OK;
+}
+
+int ifgf_autotune_cone_order(const ifgf_surface* surface, const ifgf_params* params, double tol, int* Ps, int* Pang,
+                             double* err) {
+  if (!surface || !params || !Ps || !Pang || !err || !(tol > 0.0)) return fail(nullptr, IFGF_ERR_ARG, "invalid argument");
+  static const int cand[][2] = {{3, 5}, {5, 5}, {5, 7}, {7, 7}, {9, 9}, {11, 11}};
+  double best = 1e300;
+  int bp = 0;
+  for (int c = 0; c < 6; c++) {
+    ifgf_params prm = *params;
+    prm.Ps = cand[c][0];
+    prm.Pang = cand[c][1];
+    prm.nrhs_max = 1;
+    ifgf_plan* P = nullptr;
+    int rc = ifgf_plan_create(surface, &prm, &P);
+    if (rc != IFGF_OK) return rc;
+    const long long N = P->geo.N;
+    // seeded complex density (splitmix64 uniforms in [-1, 1)) and a spread of
+    // at most 256 targets: the interpolation error against the direct sum
+    std::vector<double> x(2 * N), y(2 * N);
+    for (long long i = 0; i < 2 * N; i++) {
+      uint64_t z = 0x5eedull + (uint64_t)(i + 1) * 0x9E3779B97F4A7C15ull;
+      z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+      z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+      z ^= z >> 31;
+      x[i] = (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+    }
+    const long long nt = std::min<long long>(N, 256);
+    std::vector<long long> tg(nt);
+    for (long long t = 0; t < nt; t++) tg[t] = t * N / nt;
+    std::vector<double> yd(2 * nt);
+    rc = ifgf_apply(P, x.data(), y.data(), 1, N, 0, nullptr);
+    if (rc == IFGF_OK) rc = ifgf_apply_direct(P, x.data(), tg.data(), nt, yd.data());
+    ifgf_plan_destroy(P);
+    if (rc != IFGF_OK) return rc;
+    double num = 0.0, den = 0.0;
+    for (long long t = 0; t < nt; t++) {
+      const double dr = y[2 * tg[t]] - yd[2 * t], di = y[2 * tg[t] + 1] - yd[2 * t + 1];
+      num += dr * dr + di * di;
+      den += yd[2 * t] * yd[2 * t] + yd[2 * t + 1] * yd[2 * t + 1];
+    }
+    const double e = std::sqrt(num / den);
+    if (e < best) {
+      best = e;
+      bp = c;
+    }
+    if (e <= tol) break;
+  }
+  *Ps = cand[bp][0];
+  *Pang = cand[bp][1];
+  *err = best;
+  return best <= tol ? IFGF_OK : IFGF_ERR_NOT_CONVERGED;
+}
+
+int ifgf_group_stats(const ifgf_group* G, long long out[4]) {
+  if (!G || !out) return fail(nullptr, IFGF_ERR_ARG, "null argument");
+  long long launches = 0, blocks = 0;
+  for (const auto& C : G->classes) {
+    const Stage* st[] = {&C.gather, &C.near, &C.leaf, &C.fin[0], &C.fin[1], &C.fin[2]};
+    for (const Stage* s : st) {
+      launches += s->blocks > 0;
+      blocks += s->blocks;
+    }
+    for (const auto& s : C.push) {
+      launches += s.blocks > 0;
+      blocks += s.blocks;
+    }
+  }
+  out[0] = (long long)G->plans.size();
+  out[1] = (long long)G->classes.size();
+  out[2] = launches;
+  out[3] = blocks;
   return IFGF_OK;
 }
 

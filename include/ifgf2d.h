@@ -193,17 +193,36 @@ int ifgf_apply_graph(ifgf_plan* plan, const double* x, double* y, int nrhs, long
                      void* stream);
 
 /* Several obstacles' forward maps as one launch (no reference counterpart;
- * the reference handles one ForwardMap at a time): captures the device
- * applications y[i] = A_i x[i] of nplans distinct plans (device pointers,
- * n_i complex values each, one column) into one HIP graph. Every
- * ifgf_group_launch then runs all of them with one launch on stream (NULL =
- * the group's own stream), asynchronously; x[i] / y[i] contents may change
- * between launches, the pointers may not. Plans must outlive the group and
- * are not applied elsewhere while a launch runs. */
+ * the reference handles one ForwardMap at a time): y[i] = A_i x[i] for nplans
+ * distinct plans with stored operators (device pointers, n_i complex values
+ * each, one column). Every stage of the application runs as ONE kernel launch
+ * over all plans of a class (plans of equal (Ps, Pang) and precision; the
+ * formulation may differ): gather, near field, leaf operator, one push
+ * launch per tree depth, finalize; each workgroup finds its plan from a table
+ * of per-plan block offsets, so every plan computes exactly what its own
+ * application computes (bitwise). The launches are captured into one HIP
+ * graph; every ifgf_group_launch runs it on stream (NULL = the group's own
+ * stream), asynchronously. x[i] / y[i] contents may change between launches,
+ * the pointers may not. Plans must outlive the group and are not applied
+ * elsewhere while a launch runs. */
 int ifgf_group_create(ifgf_plan* const* plans, int nplans, const double* const* x, double* const* y,
                       ifgf_group** group);
 int ifgf_group_launch(ifgf_group* group, void* stream);
 int ifgf_group_synchronize(ifgf_group* group);  /* waits for the group's own stream */
+/* Cone-order autotuner (no reference counterpart: the reference fixes Ps
+ * and Pang at compile time, ForwardMap.hpp:49, BoxTree.h:24): builds the
+ * surface's plan for (Ps, Pang) = (3, 5), (5, 5), (5, 7), (7, 7), (9, 9),
+ * (11, 11) in that order of cost and returns the first whose application of
+ * a seeded random density is within tol (relative 2-norm) of the
+ * unaccelerated operator (ifgf_apply_direct) at up to 256 targets spread
+ * over the points; params gives everything else. err: that order's error.
+ * IFGF_ERR_NOT_CONVERGED (with the most accurate order) when none reaches tol. */
+int ifgf_autotune_cone_order(const ifgf_surface* surface, const ifgf_params* params, double tol, int* Ps, int* Pang,
+                             double* err);
+
+/* out[4]: plans, classes (plans of equal (Ps, Pang) and precision share every
+ * launch), kernel launches per group application, workgroups per application. */
+int ifgf_group_stats(const ifgf_group* group, long long out[4]);
 void ifgf_group_destroy(ifgf_group* group);
 
 /* Unaccelerated operator on a target subset: replaces compute_Ax_unacc
