@@ -369,8 +369,9 @@ def main():
         batched = {"nrhs": 64}
         Xb = torch.from_numpy(configs.seeded_density(64 * fm.N, seed=1).reshape(64, fm.N)).to(dev)
         Yb = torch.empty_like(Xb)
-        for prec in ("fp64", "mixed"):
-            fb = I.ForwardMap(**dict(kw, precision=prec), nrhs_max=64, autotune=True)
+        for prec in ("fp64", "mixed", "fp64_valu", "mixed_valu"):
+            fb = I.ForwardMap(**dict(kw, precision=prec.split("_")[0]), nrhs_max=64, autotune=True,
+                              mfma=not prec.endswith("valu"))
             fb.apply_device(Xb.data_ptr(), Yb.data_ptr(), nrhs=64)
             torch.cuda.synchronize()
             reps = max(2, args.steps // 100)

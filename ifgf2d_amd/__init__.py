@@ -44,6 +44,7 @@ IFGF_PLAN_OVERLAP = 16
 IFGF_PLAN_DEVICE_PRECOMP = 32
 IFGF_PLAN_DEVICE_GEOMETRY = 64
 IFGF_PLAN_DEVICE_TREE = 128
+IFGF_PLAN_NO_MFMA = 256
 IFGF_FNO_GUESS, IFGF_FNO_PRECONDITION, IFGF_FNO_FIXED_MULTIPLIER = 1, 2, 4
 
 
@@ -279,12 +280,14 @@ class ForwardMap:
     device_tree: build the tree's relevant cone segments (interpolation entries, protoboxes,
         numbering) on the GPU; the tables equal the host's.
     device_setup: all four device setup steps (sort, correction, near-singular search, cones).
+    mfma: blocks of 8 columns (nrhs_max >= 8) run the near field and leaf operator on the
+        matrix cores (False: the VALU batched kernels).
     """
 
     def __init__(self, curve, patch_lims, k, nlevels, Np=10, formulation="combined", eta=None,
                  delta=None, p=4.0, Ps=5, Pang=5, precision="fp64", nrhs_max=1, host_only=False,
                  on_the_fly=False, autotune=False, device_sort=False, overlap=False, device_precomp=False,
-                 device_geometry=False, device_setup=False, device_tree=False):
+                 device_geometry=False, device_setup=False, device_tree=False, mfma=True):
         L = lib()
         self.curve = curve
         self.lims = np.ascontiguousarray(patch_lims, dtype=np.float64)
@@ -299,7 +302,7 @@ class ForwardMap:
                  | (IFGF_PLAN_AUTOTUNE if autotune else 0) | (IFGF_PLAN_DEVICE_SORT if device_sort else 0)
                  | (IFGF_PLAN_OVERLAP if overlap else 0) | (IFGF_PLAN_DEVICE_PRECOMP if device_precomp else 0)
                  | (IFGF_PLAN_DEVICE_GEOMETRY if device_geometry else 0)
-                 | (IFGF_PLAN_DEVICE_TREE if device_tree else 0))
+                 | (IFGF_PLAN_DEVICE_TREE if device_tree else 0) | (0 if mfma else IFGF_PLAN_NO_MFMA))
         if device_setup:
             flags |= (IFGF_PLAN_DEVICE_SORT | IFGF_PLAN_DEVICE_PRECOMP | IFGF_PLAN_DEVICE_GEOMETRY
                       | IFGF_PLAN_DEVICE_TREE)

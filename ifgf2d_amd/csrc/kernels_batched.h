@@ -13,6 +13,8 @@
 // two kernels), and batched applications are bitwise deterministic.
 #pragma once
 
+#include <type_traits>
+
 #include "kernels.h"
 
 namespace ifgf {
@@ -241,6 +243,219 @@ __global__ void __launch_bounds__(256) k_level_push_b(PushArgsT<RT> a) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Blocks of 8 columns on the matrix cores. The near field of the targets of
+// one leaf box (a group) is a dense block V (targets x the box's
+// neighbourhood sources, zeros at excluded pairs), and the leaf operators of
+// a leaf box's cones form a dense block L ((cone, node) rows x the box's
+// sources); both multiply the 8-column block X of the sources' densities. A
+// complex product is one real MFMA product:
+//   [C_re | C_im] (16 x 16) = [V_re, -V_im] (16 x 2K) [[X_re, X_im], [X_im, -X_re]] (2K x 16),
+// source j giving the K rows 2j (V_re / X_re, X_im) and 2j + 1 (-V_im / X_im, -X_re).
+// The A operand is stored in MFMA order: tile (row tile rt, K chunk q) holds
+// the 64 values lane l reads (row l % 16, K index 4q + l / 16), so a wave
+// loads 64 consecutive values per instruction. V_MFMA_F64_16X16X4F64 (FP64
+// plans; the near field on mixed plans too) and V_MFMA_F32_16X16X4F32 (the
+// leaf operator of mixed plans): lane l supplies A[l % 16][l / 16] and
+// B[l / 16][l % 16] and receives C[row][l % 16], v < 4, with row = 4 (l / 16)
+// + v for the FP32 instruction and row = l / 16 + 4 v for the FP64 one
+// (checked on the device with unit-vector operands).
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+typedef float flt4 __attribute__((ext_vector_type(4)));
+
+__device__ inline dbl4 mfma_16x16x4(double a, double b, dbl4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ inline flt4 mfma_16x16x4(float a, float b, flt4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+// output row of accumulator element v in lane group kk = lane / 16
+__device__ inline int mfma_row(double, int kk, int v) { return kk + 4 * v; }
+__device__ inline int mfma_row(float, int kk, int v) { return 4 * kk + v; }
+
+// Source index of flat source offset o in a segment table (kMaxNb (end, shift) pairs)
+__device__ inline int seg_source(const int* __restrict__ sg, int o) {
+  int src = o + sg[1];
+#pragma unroll
+  for (int q = 1; q < kMaxNb; q++) src = (o >= sg[2 * q - 2]) ? o + sg[2 * q + 1] : src;
+  return src;
+}
+
+// Build: the A tiles of the near blocks from the stored near-field values.
+// One thread per tile value; units give each group's tiles in order.
+__global__ void __launch_bounds__(256)
+    k_near_mfma_build(long long n, int ngroups, const MfmaGroup* __restrict__ grp, const long long* __restrict__ nf_goff,
+                      const zd* __restrict__ nf_val, double* __restrict__ A) {
+  const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  int lo = 0, hi = ngroups - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (grp[mid].aoff <= v) lo = mid;
+    else hi = mid - 1;
+  }
+  const MfmaGroup G = grp[lo];
+  const long long local = v - G.aoff;
+  const int Q = (G.nsrc + 1) / 2;
+  const long long tile = local / 64;
+  const int lane = (int)(local % 64);
+  const int rt = (int)(tile / Q), q = (int)(tile % Q);
+  const int m = lane % 16, kk = lane / 16;
+  const int row = rt * 16 + m, o = 2 * q + kk / 2;
+  double val = 0.0;
+  if (row < G.rows && o < G.nsrc) {
+    const int i = G.row0 + row;
+    const zd z = nf_val[near_index(nf_goff[i / kNearGroup], i, o)];
+    val = (kk % 2 == 0) ? z.x : -z.y;
+  }
+  A[v] = val;
+}
+
+// Build: the A tiles of the leaf blocks from the stored leaf operator (rows
+// (cone, node) of the group's cones in order, K over the box's sources).
+template <int P, typename Z>
+__global__ void __launch_bounds__(256)
+    k_leaf_mfma_build(long long n, int ngroups, const MfmaGroup* __restrict__ grp, const int* __restrict__ gcone0,
+                      const long long* __restrict__ lf_off, const Z* __restrict__ lf,
+                      decltype(Z::x)* __restrict__ A) {
+  const long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  int lo = 0, hi = ngroups - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if (grp[mid].aoff <= v) lo = mid;
+    else hi = mid - 1;
+  }
+  const MfmaGroup G = grp[lo];
+  const long long local = v - G.aoff;
+  const int Q = (G.nsrc + 1) / 2;
+  const long long tile = local / 64;
+  const int lane = (int)(local % 64);
+  const int rt = (int)(tile / Q), q = (int)(tile % Q);
+  const int m = lane % 16, kk = lane / 16;
+  const int row = rt * 16 + m, o = 2 * q + kk / 2;
+  decltype(Z::x) val = 0;
+  if (row < G.rows && o < G.nsrc) {
+    const int c = gcone0[lo] + row / P, node = row % P;
+    const Z z = lf[leaf_index<P>(lf_off[c] / P + o, node)];
+    val = (kk % 2 == 0) ? z.x : -z.y;
+  }
+  A[v] = val;
+}
+
+// Near field of 8-column blocks: one block of 4 waves per unit (group, row
+// tile); wave w sums the K chunks [w Q / 4, (w + 1) Q / 4), the waves' tiles
+// are added in wave order; out[(row0 + row) * 8 + r] = the target's near sum.
+__global__ void __launch_bounds__(256)
+    k_near_mfma(int nunits, const MfmaUnit* __restrict__ units, const MfmaGroup* __restrict__ grp,
+                const int* __restrict__ segs, const double* __restrict__ A, const zd* __restrict__ rho,
+                zd* __restrict__ out) {
+  __shared__ double red[3][64][4];
+  const int u = xcd_block();
+  if (u >= nunits) return;
+  const MfmaUnit U = units[u];
+  const MfmaGroup G = grp[U.g];
+  const int* sg = segs + (size_t)G.seg * kNbSegInts;
+  const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
+  const int Q = (G.nsrc + 1) / 2;
+  const int q0 = w * Q / 4, q1 = (w + 1) * Q / 4;
+  const int kk = lane / 16, n = lane % 16;
+  const int c = kk % 2, r = n % 8;
+  const bool im = n >= 8;
+  // the one density component this lane feeds: rows 2j (X_re | X_im), rows
+  // 2j + 1 (X_im | -X_re)
+  const int part = (c == 0) == im ? 1 : 0;
+  const double sgn = (c == 1 && im) ? -1.0 : 1.0;
+  const double* __restrict__ x = reinterpret_cast<const double*>(rho) + r * 2 + part;
+  const double* __restrict__ a = A + G.aoff + (long long)U.rt * Q * 64 + lane;
+  // 8 K chunks per round: their loads first, then the products, on two
+  // accumulator chains (added at the end)
+  constexpr int kU = 8;
+  dbl4 acc = {0.0, 0.0, 0.0, 0.0}, acc2 = {0.0, 0.0, 0.0, 0.0};
+  for (int qb = q0; qb < q1; qb += kU) {
+    double av[kU], bv[kU];
+#pragma unroll
+    for (int t = 0; t < kU; t++) {
+      const int q = qb + t;
+      av[t] = 0.0;
+      bv[t] = 0.0;
+      if (q < q1) {
+        av[t] = a[(long long)q * 64];
+        const int o = 2 * q + kk / 2;
+        if (o < G.nsrc) bv[t] = sgn * x[(size_t)seg_source(sg, o) * 16];
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < kU; t += 2) {
+      acc = mfma_16x16x4(av[t], bv[t], acc);
+      acc2 = mfma_16x16x4(av[t + 1], bv[t + 1], acc2);
+    }
+  }
+  acc += acc2;
+  if (w > 0)
+    for (int v = 0; v < 4; v++) red[w - 1][lane][v] = acc[v];
+  __syncthreads();
+  if (w == 0) {
+    for (int s = 0; s < 3; s++)
+      for (int v = 0; v < 4; v++) acc[v] += red[s][lane][v];
+    double* o = reinterpret_cast<double*>(out);
+    for (int v = 0; v < 4; v++) {
+      const int row = U.rt * 16 + mfma_row(0.0, kk, v);
+      if (row < G.rows) o[((size_t)(G.row0 + row) * 8 + r) * 2 + (im ? 1 : 0)] = acc[v];
+    }
+  }
+}
+
+// Leaf operator of 8-column blocks: one wave per unit (group, row tile), 4
+// units per block; out[((cone) * P + node) * 8 + r] = the leaf cone's
+// transformed coefficients for column r (row0 = the group's first cone x P).
+template <typename RT>
+__global__ void __launch_bounds__(256)
+    k_leaf_mfma(int nunits, const MfmaUnit* __restrict__ units, const MfmaGroup* __restrict__ grp,
+                const RT* __restrict__ A, const zd* __restrict__ rho, typename complex_of<RT>::type* __restrict__ out) {
+  using V4 = typename std::conditional<std::is_same<RT, double>::value, dbl4, flt4>::type;
+  const int u = xcd_block() * 4 + threadIdx.x / 64;
+  if (u >= nunits) return;
+  const MfmaUnit U = units[u];
+  const MfmaGroup G = grp[U.g];
+  const int lane = threadIdx.x % 64;
+  const int Q = (G.nsrc + 1) / 2;
+  const int kk = lane / 16, n = lane % 16;
+  const int c = kk % 2, r = n % 8;
+  const bool im = n >= 8;
+  const int part = (c == 0) == im ? 1 : 0;
+  const RT sgn = (c == 1 && im) ? RT(-1) : RT(1);
+  const double* __restrict__ x = reinterpret_cast<const double*>(rho) + r * 2 + part;
+  const RT* __restrict__ a = A + G.aoff + (long long)U.rt * Q * 64 + lane;
+  constexpr int kU = 8;
+  V4 acc = {0, 0, 0, 0}, acc2 = {0, 0, 0, 0};
+  for (int qb = 0; qb < Q; qb += kU) {
+    RT av[kU], bv[kU];
+#pragma unroll
+    for (int t = 0; t < kU; t++) {
+      const int q = qb + t;
+      av[t] = 0;
+      bv[t] = 0;
+      if (q < Q) {
+        av[t] = a[(long long)q * 64];
+        const int o = 2 * q + kk / 2;
+        if (o < G.nsrc) bv[t] = sgn * (RT)x[(size_t)(G.seg + o) * 16];
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < kU; t += 2) {
+      acc = mfma_16x16x4(av[t], bv[t], acc);
+      acc2 = mfma_16x16x4(av[t + 1], bv[t + 1], acc2);
+    }
+  }
+  acc += acc2;
+  RT* o = reinterpret_cast<RT*>(out);
+  for (int v = 0; v < 4; v++) {
+    const int row = U.rt * 16 + mfma_row(RT(0), kk, v);
+    if (row < G.rows) o[((size_t)(G.row0 + row) * 8 + r) * 2 + (im ? 1 : 0)] = acc[v];
+  }
+}
+
 // k_finalize with R columns: lanes (target, column, slice) with S slices per
 // (target, column); y is column-major with stride ld; interpolation entry f
 // holds rs columns at slots[it_slot[f] * rs]. stage 0: all of it; stage 1
@@ -254,7 +469,7 @@ __global__ void __launch_bounds__(256)
                  const zd* __restrict__ corr_w, const int* __restrict__ inv, const int* __restrict__ nio,
                  const zd* __restrict__ npart, const long long* __restrict__ it_ptr, const int* __restrict__ it_slot,
                  const ZI* __restrict__ slots, int rs, zd* __restrict__ y, int stage = 0,
-                 zd* __restrict__ pre = nullptr) {
+                 zd* __restrict__ pre = nullptr, const zd* __restrict__ nearb = nullptr) {
   const long long gt = (long long)xcd_block() * blockDim.x + threadIdx.x;
   const long long ir = gt / S;
   const int sl = (int)(gt % S);
@@ -271,9 +486,14 @@ __global__ void __launch_bounds__(256)
         const int q = (int)(it % Np);
         cfma(acc, corr_w[e * Np + q], xr[(size_t)corr_patch[e] * Np + q]);
       }
-      const int g = si / kNearGroup, t = si % kNearGroup;
-      const int w1 = nio[g + 1];
-      for (int w = nio[g] + sl; w < w1; w += S) near = cadd(near, npart[((size_t)w * kNearGroup + t) * R + r]);
+      if (nearb) {
+        // the near sum of the 8-column MFMA blocks, one value per (target, column)
+        if (sl == 0) near = nearb[(size_t)si * R + r];
+      } else {
+        const int g = si / kNearGroup, t = si % kNearGroup;
+        const int w1 = nio[g + 1];
+        for (int w = nio[g] + sl; w < w1; w += S) near = cadd(near, npart[((size_t)w * kNearGroup + t) * R + r]);
+      }
     }
     if (stage != 1) {
       const long long f1 = it_ptr[si + 1];

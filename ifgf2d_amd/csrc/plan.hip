@@ -200,6 +200,7 @@ int ifgf_plan_create(const ifgf_surface* s, const ifgf_params* prm, ifgf_plan** 
         if (!build_stored_any(P.get())) return fail(nullptr, IFGF_ERR_ARG, "no kernel instantiation for this (Ps, Pang)");
         lap(5);
         P->overlap = P->overlap_b = (prm->flags & IFGF_PLAN_OVERLAP) ? 2 : 0;
+        P->use_mfma8 = !(prm->flags & IFGF_PLAN_NO_MFMA);
         if (prm->flags & IFGF_PLAN_AUTOTUNE) autotune(P.get());
         lap(6);
       }
@@ -499,10 +500,16 @@ static void launch_batched(ifgf_plan* P, const zd* x, long long ld, zd* y, hipSt
   // with overlap: the near field and the identity / correction half of the
   // final sum on the second stream, forked along the push chain as for one column
   const bool fork = P->overlap_b != 0 && M > 0;
+  // blocks of 8 columns on the matrix cores (near field and leaf operator)
+  const bool mf = R == 8 && P->mfma8 && P->use_mfma8;
   auto near_b = [&](hipStream_t s) {
-    if (M > 0)
+    if (mf) {
+      hipLaunchKernelGGL(k_near_mfma, dim3((unsigned)P->nm_units.n), dim3(256), 0, s, (int)P->nm_units.n,
+                         P->nm_units.p, P->nm_grp.p, P->nm_segs.p, P->nm_A.p, P->rho.p, P->nearb8.p);
+    } else if (M > 0) {
       hipLaunchKernelGGL((k_near_items_b<R>), dim3((M + 3) / 4), dim3(256), 0, s, N, M, P->near_items.p,
                          P->nf_seg.p, P->nf_val.p, P->rho.p, P->npart_b.p);
+    }
   };
   const double ident = (F == kSingleLayer) ? 0.0 : 0.5;
   const long long nthr = (long long)N * R;
@@ -512,7 +519,7 @@ static void launch_batched(ifgf_plan* P, const zd* x, long long ld, zd* y, hipSt
   hipLaunchKernelGGL((k_finalize_b<S, R, Z>), dim3((unsigned)((nthr * S + 255) / 256)), dim3(256), 0, s, N,         \
                      P->geo.Np, ident, x, ld, P->corr_ptr.p, P->corr_patch.p, P->corr_w.p, P->inv.p,                \
                      P->near_item_off.p, P->npart_b.p, P->it_ptr.p, P->it_slot.p, pbuf_, P->rb_max, y, stage,      \
-                     P->fin_pre_b.p)
+                     P->fin_pre_b.p, mf ? P->nearb8.p : nullptr)
     if (P->final_lanes == 4) IFGF_FINB(4);
     else if (P->final_lanes == 16) IFGF_FINB(16);
     else IFGF_FINB(8);
@@ -522,17 +529,29 @@ static void launch_batched(ifgf_plan* P, const zd* x, long long ld, zd* y, hipSt
   const long long ML = P->leaf_items.n;
   const Z* lf = f32 ? (const Z*)P->lf_val32.p : (const Z*)P->lf_val.p;
   Z* lpart = f32 ? (Z*)P->lpart_b32.p : (Z*)P->lpart_b.p;
-  hipLaunchKernelGGL((k_leaf_items_b<Pn, R, Z>), dim3((unsigned)((ML * Pn * kLeafChunk + 255) / 256)), dim3(256), 0,
-                     st, (int)ML, P->leaf_items.p, lf, P->rho.p, lpart);
+  const int* lio = P->leaf_item_off.p;
+  if (mf) {
+    // the leaf cones' coefficients for 8 columns, one "item" per cone
+    lpart = f32 ? (Z*)P->lcoef8_32.p : (Z*)P->lcoef8.p;
+    lio = P->lio_id.p;
+    const RT* A = f32 ? (const RT*)P->lm_A32.p : (const RT*)P->lm_A.p;
+    const int nu = (int)P->lm_units.n;
+    if (nu > 0)
+      hipLaunchKernelGGL((k_leaf_mfma<RT>), dim3((unsigned)((nu + 3) / 4)), dim3(256), 0, st, nu, P->lm_units.p,
+                         P->lm_grp.p, A, P->rho.p, (Z*)lpart);
+  } else {
+    hipLaunchKernelGGL((k_leaf_items_b<Pn, R, Z>), dim3((unsigned)((ML * Pn * kLeafChunk + 255) / 256)), dim3(256),
+                       0, st, (int)ML, P->leaf_items.p, lf, P->rho.p, lpart);
+  }
   Z* pbuf = f32 ? (Z*)P->pbuf_b32.p : (Z*)P->pbuf_b.p;
   PushArgsT<RT> a{};
   if constexpr (f32) {
     a = PushArgsT<RT>{nullptr, nullptr, P->push_geo32.p, nullptr, P->push_coff.p, P->push_cidx.p,
-                      P->leaf_item_off.p, lpart, t.lv[D].cone_off, P->ifac_off.p, P->ifac_tab32.p, P->tabs32.p, pbuf,
+                      lio, lpart, t.lv[D].cone_off, P->ifac_off.p, P->ifac_tab32.p, P->tabs32.p, pbuf,
                       0, P->rb_max};
   } else {
     a = PushArgsT<RT>{nullptr, nullptr, P->push_geo_b.p, nullptr, P->push_coff.p, P->push_cidx.p,
-                      P->leaf_item_off.p, lpart, t.lv[D].cone_off, P->ifac_off.p, P->ifac_tab.p, P->tabs.p, pbuf, 0,
+                      lio, lpart, t.lv[D].cone_off, P->ifac_off.p, P->ifac_tab.p, P->tabs.p, pbuf, 0,
                       P->rb_max};
   }
   const std::vector<int>& lo = P->blevel_off[LR];
@@ -650,6 +669,55 @@ static void build_batched(ifgf_plan* P) {
   }
   P->stored_bytes += (long long)(P->npart_b.n + P->lpart_b.n + P->pbuf_b.n) * (long long)sizeof(zd) +
                      (long long)(P->lpart_b32.n + P->pbuf_b32.n) * (long long)sizeof(zf);
+  // leaf-operator A tiles of the 8-column MFMA blocks: one group per leaf box
+  // with cones, rows (cone, node) of its cones in order, K over its sources
+  if (P->rb_max == 8 && P->nm_units.n > 0) {
+    const Tree& t = P->tree;
+    const LevelInfo& LD = t.lv[t.D];
+    std::vector<MfmaGroup> grp;
+    std::vector<MfmaUnit> units;
+    std::vector<int> cone0;
+    long long aoff = 0;
+    for (int c0 = 0; c0 < LD.ncone;) {
+      const int b = t.cone_box[LD.cone_off + c0];
+      int c1 = c0;
+      while (c1 < LD.ncone && t.cone_box[LD.cone_off + c1] == b) c1++;
+      const int nsrc = t.leaf_begin[b + 1] - t.leaf_begin[b];
+      const int rows = (c1 - c0) * Pn;
+      const int g = (int)grp.size();
+      grp.push_back(MfmaGroup{aoff, rows, nsrc, c0 * Pn, t.leaf_begin[b]});
+      cone0.push_back(c0);
+      const int rt = (rows + 15) / 16;
+      for (int r = 0; r < rt; r++) units.push_back(MfmaUnit{g, r});
+      aoff += (long long)rt * ((nsrc + 1) / 2) * 64;
+      c0 = c1;
+    }
+    P->lm_grp.upload(grp);
+    P->lm_units.upload(units);
+    P->lm_cone0.upload(cone0);
+    std::vector<int> id(LD.ncone + 1);
+    for (int c = 0; c <= LD.ncone; c++) id[c] = c;
+    P->lio_id.upload(id);
+    const unsigned nb = (unsigned)((aoff + 255) / 256);
+    if (P->mixed) {
+      P->lm_A32.alloc(aoff);
+      if (aoff > 0)
+        hipLaunchKernelGGL((k_leaf_mfma_build<Pn, zf>), dim3(nb), dim3(256), 0, P->stream, aoff, (int)grp.size(),
+                           P->lm_grp.p, P->lm_cone0.p, P->lf_off.p, P->lf_val32.p, P->lm_A32.p);
+      P->lcoef8_32.alloc((size_t)LD.ncone * Pn * 8);
+      P->stored_bytes += aoff * (long long)sizeof(float);
+    } else {
+      P->lm_A.alloc(aoff);
+      if (aoff > 0)
+        hipLaunchKernelGGL((k_leaf_mfma_build<Pn, zd>), dim3(nb), dim3(256), 0, P->stream, aoff, (int)grp.size(),
+                           P->lm_grp.p, P->lm_cone0.p, P->lf_off.p, P->lf_val.p, P->lm_A.p);
+      P->lcoef8.alloc((size_t)LD.ncone * Pn * 8);
+      P->stored_bytes += aoff * (long long)sizeof(double);
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(P->stream));
+    P->mfma8 = true;
+  }
 }
 
 template <int F, int Ps, int Pang>
@@ -741,6 +809,37 @@ static void build_stored(ifgf_plan* P) {
                      P->leaf_of_point.p, P->leaf_begin.p, P->nb_ptr.p, P->nb_list.p, P->nf_goff.p,
                      P->nf_val.p, k, eta);
   HIP_TRY(hipGetLastError());
+  // near-field A tiles of the 8-column MFMA blocks: one group per leaf box
+  // (its targets are consecutive and share the neighbourhood's sources)
+  bool contiguous = true;
+  for (int i = 1; i < N && contiguous; i++) contiguous = tleaf[i] >= tleaf[i - 1];
+  if (P->prm.nrhs_max >= 8 && !P->field && contiguous && N > 0) {
+    std::vector<MfmaGroup> grp;
+    std::vector<MfmaUnit> units;
+    long long aoff = 0;
+    for (int i0 = 0; i0 < N;) {
+      int i1 = i0;
+      while (i1 < N && tleaf[i1] == tleaf[i0]) i1++;
+      const int b = tleaf[i0];
+      const int nsrc = seg[(size_t)b * kNbSegInts + 2 * (kMaxNb - 1)];
+      const int g = (int)grp.size();
+      grp.push_back(MfmaGroup{aoff, i1 - i0, nsrc, i0, b});
+      const int rt = (i1 - i0 + 15) / 16;
+      for (int r = 0; r < rt; r++) units.push_back(MfmaUnit{g, r});
+      aoff += (long long)rt * ((nsrc + 1) / 2) * 64;
+      i0 = i1;
+    }
+    P->nm_grp.upload(grp);
+    P->nm_units.upload(units);
+    P->nm_segs.upload(seg);
+    P->nm_A.alloc(aoff);
+    if (aoff > 0)
+      hipLaunchKernelGGL(k_near_mfma_build, dim3((unsigned)((aoff + 255) / 256)), dim3(256), 0, st, aoff,
+                         (int)grp.size(), P->nm_grp.p, P->nf_goff.p, P->nf_val.p, P->nm_A.p);
+    HIP_TRY(hipGetLastError());
+    P->nearb8.alloc((size_t)N * 8);
+    P->stored_bytes += aoff * (long long)sizeof(double);
+  }
   // leaf operator: P x n_src per leaf cone
   const LevelInfo& LD = t.lv[D];
   // (sources padded to a multiple of kLeafChunk; lf_src = (first source, count))

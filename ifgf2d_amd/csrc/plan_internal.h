@@ -148,6 +148,19 @@ struct ifgf_plan {
   ifgf::DevBuf<ifgf::Geo4F> push_geo32;
   ifgf::DevBuf<ifgf::Geo4> push_geo_b;
   ifgf::DevBuf<ifgf::ChebTablesF> tabs32;
+  // 8-column blocks on the matrix cores: near-field and leaf-operator A tiles,
+  // their groups and units, the near sums per (target, column), the leaf
+  // coefficients per (leaf cone, node, column) and the identity item map the
+  // leaf-level push reads them through
+  bool mfma8 = false;      // the MFMA tiles are built (nrhs_max >= 8)
+  bool use_mfma8 = true;   // 8-column blocks take them (else the VALU batched kernels)
+  ifgf::DevBuf<ifgf::MfmaGroup> nm_grp, lm_grp;
+  ifgf::DevBuf<ifgf::MfmaUnit> nm_units, lm_units;
+  ifgf::DevBuf<int> nm_segs, lm_cone0, lio_id;
+  ifgf::DevBuf<double> nm_A, lm_A;
+  ifgf::DevBuf<float> lm_A32;
+  ifgf::DevBuf<zd> nearb8, lcoef8;
+  ifgf::DevBuf<zf> lcoef8_32;
   int nleaf = 0;
   // lanes per work item of the stored-path kernels
   int near_lanes = 8 /* kNearChunk */, leaf_lanes = 4 /* kLeafChunk */, final_lanes = 8;

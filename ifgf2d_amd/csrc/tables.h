@@ -66,6 +66,19 @@ struct PushTask {
   int e0, ne;
 };
 
+// 8-column MFMA blocks (kernels_batched.h). A group is a leaf box: its
+// targets (near field) or its leaf cones' (cone, node) rows (leaf operator)
+// against its sources; aoff = offset of its A tiles, rows, nsrc = sources,
+// row0 = first output row, seg = the near group's segment table index (the
+// leaf group's first source point). A unit is one row tile rt of group g.
+struct MfmaGroup {
+  long long aoff;
+  int rows, nsrc, row0, seg;
+};
+struct MfmaUnit {
+  int g, rt;
+};
+
 // Local Chebyshev coordinates (s, theta) of a point in a cone and the centre
 // factor exp(i k r)/sqrt(r) at its distance r from the cone's box centre
 // (evaluated in FP64; stored rounded to FP32 on a mixed-precision plan).

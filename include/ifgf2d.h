@@ -73,6 +73,9 @@ extern "C" {
                                      BoxTree.h:299-583) on the GPU instead of the host; same tables
                                      as the host's loops */
 
+#define IFGF_PLAN_NO_MFMA 256 /* blocks of 8 columns (nrhs_max >= 8) use the VALU batched near-field and
+                                 leaf kernels instead of the matrix-core (MFMA) ones */
+
 /* apply flags */
 #define IFGF_PTR_DEVICE 1     /* x / y are device pointers; the call is asynchronous on stream */
 #define IFGF_APPLY_FP64 2     /* on an IFGF_MIXED plan: apply the all-FP64 operator instead of the
