@@ -27,16 +27,20 @@ constexpr int batched_cones(int P, int R, int zbytes) {
   return m < 1 ? 1 : m;
 }
 
-// rho_b[i * R + r] = x[r * ld + perm[i]] * w[perm[i]]
+// rho_b[i * R + r] = x[r * ld + perm[i]] * w[perm[i]], and the columns
+// transposed in patch order for the finalize's correction: xt[o * R + r] =
+// x[r * ld + o], o = perm[i]
 template <int R>
 __global__ void __launch_bounds__(256) k_gather_b(int N, const zd* __restrict__ x, long long ld,
                                                   const double* __restrict__ w, const int* __restrict__ perm,
-                                                  zd* __restrict__ rho) {
+                                                  zd* __restrict__ rho, zd* __restrict__ xt) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long long)N * R) return;
   const int i = (int)(t / R), r = (int)(t % R);
   const int o = perm[i];
-  rho[t] = cscale(x[(size_t)r * ld + o], w[o]);
+  const zd v = x[(size_t)r * ld + o];
+  rho[t] = cscale(v, w[o]);
+  xt[(size_t)o * R + r] = v;
 }
 
 // k_near_items with R columns: lane (t, sl) keeps R accumulators.
@@ -456,63 +460,92 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// k_finalize with R columns: lanes (target, column, slice) with S slices per
-// (target, column); y is column-major with stride ld; interpolation entry f
-// holds rs columns at slots[it_slot[f] * rs]. stage 0: all of it; stage 1
-// (beside the level pushes): pre[2 (i R + r)] = ident x + correction,
-// pre[2 (i R + r) + 1] = near; stage 2: y = pre[..] + (pre[.. + 1] + far),
-// the same bits as stage 0.
+// k_finalize with R columns: lanes (target, column chunk, slice), S slices
+// per (target, chunk), each lane summing its entries for the kFinCols (or R,
+// if fewer) columns of its chunk, so a correction weight, an interpolation
+// slot index and a near-field partial are loaded once per chunk instead of
+// once per column; per column the sums run in the single-column kernel's
+// order. xt: the columns transposed in patch order (xt[n * R + r] = x[r * ld
+// + n], written by k_gather_b) for the correction's reads; y is column-major
+// with stride ld; interpolation entry f holds rs columns at slots[it_slot[f] *
+// rs]. stage 0: all of it; stage 1 (beside the level pushes): pre[2 (i R +
+// r)] = ident x + correction, pre[2 (i R + r) + 1] = near; stage 2: y = pre[..]
+// + (pre[.. + 1] + far), the same bits as stage 0.
+constexpr int kFinCols = 4;
+
 template <int S, int R, typename ZI>
 __global__ void __launch_bounds__(256)
-    k_finalize_b(int N, int Np, double ident, const zd* __restrict__ x, long long ld,
+    k_finalize_b(int N, int Np, double ident, const zd* __restrict__ x, long long ld, const zd* __restrict__ xt,
                  const long long* __restrict__ corr_ptr, const int* __restrict__ corr_patch,
                  const zd* __restrict__ corr_w, const int* __restrict__ inv, const int* __restrict__ nio,
                  const zd* __restrict__ npart, const long long* __restrict__ it_ptr, const int* __restrict__ it_slot,
                  const ZI* __restrict__ slots, int rs, zd* __restrict__ y, int stage = 0,
                  zd* __restrict__ pre = nullptr, const zd* __restrict__ nearb = nullptr) {
+  constexpr int CH = R < kFinCols ? R : kFinCols;
+  constexpr int NCH = R / CH;
   const long long gt = (long long)xcd_block() * blockDim.x + threadIdx.x;
-  const long long ir = gt / S;
+  const long long ic = gt / S;
   const int sl = (int)(gt % S);
-  const int i = (int)(ir / R), r = (int)(ir % R);
-  zd acc{0.0, 0.0}, near{0.0, 0.0}, far{0.0, 0.0};
+  const int i = (int)(ic / NCH), r0 = (int)(ic % NCH) * CH;
+  zd acc[CH], near[CH], far[CH];
+#pragma unroll
+  for (int c = 0; c < CH; c++) acc[c] = near[c] = far[c] = zd{0.0, 0.0};
   if (i < N) {
     const int si = inv[i];
     if (stage != 2) {
-      const zd* xr = x + (size_t)r * ld;
       const long long e0 = corr_ptr[i], e1 = corr_ptr[i + 1];
       const long long nitem = (e1 - e0) * Np;
       for (long long it = sl; it < nitem; it += S) {
         const long long e = e0 + it / Np;
         const int q = (int)(it % Np);
-        cfma(acc, corr_w[e * Np + q], xr[(size_t)corr_patch[e] * Np + q]);
+        const zd w = corr_w[e * Np + q];
+        const zd* xs = xt + ((size_t)corr_patch[e] * Np + q) * R + r0;
+#pragma unroll
+        for (int c = 0; c < CH; c++) cfma(acc[c], w, xs[c]);
       }
       if (nearb) {
         // the near sum of the 8-column MFMA blocks, one value per (target, column)
-        if (sl == 0) near = nearb[(size_t)si * R + r];
+        if (sl == 0)
+#pragma unroll
+          for (int c = 0; c < CH; c++) near[c] = nearb[(size_t)si * R + r0 + c];
       } else {
         const int g = si / kNearGroup, t = si % kNearGroup;
         const int w1 = nio[g + 1];
-        for (int w = nio[g] + sl; w < w1; w += S) near = cadd(near, npart[((size_t)w * kNearGroup + t) * R + r]);
+        for (int w = nio[g] + sl; w < w1; w += S) {
+          const zd* np_ = npart + ((size_t)w * kNearGroup + t) * R + r0;
+#pragma unroll
+          for (int c = 0; c < CH; c++) near[c] = cadd(near[c], np_[c]);
+        }
       }
     }
     if (stage != 1) {
       const long long f1 = it_ptr[si + 1];
-      for (long long f = it_ptr[si] + sl; f < f1; f += S)
-        far = cadd(far, zwiden(slots[(size_t)it_slot[f] * rs + r]));
+      for (long long f = it_ptr[si] + sl; f < f1; f += S) {
+        const ZI* sv = slots + (size_t)it_slot[f] * rs + r0;
+#pragma unroll
+        for (int c = 0; c < CH; c++) far[c] = cadd(far[c], zwiden(sv[c]));
+      }
     }
   }
-  acc = lane_group_sum<S>(acc);
-  near = lane_group_sum<S>(near);
-  far = lane_group_sum<S>(far);
+#pragma unroll
+  for (int c = 0; c < CH; c++) {
+    acc[c] = lane_group_sum<S>(acc[c]);
+    near[c] = lane_group_sum<S>(near[c]);
+    far[c] = lane_group_sum<S>(far[c]);
+  }
   if (i < N && sl == 0) {
-    const size_t pi = 2 * ((size_t)i * R + r);
-    if (stage == 0) {
-      y[(size_t)r * ld + i] = cadd(cadd(cscale(x[(size_t)r * ld + i], ident), acc), cadd(near, far));
-    } else if (stage == 1) {
-      pre[pi] = cadd(cscale(x[(size_t)r * ld + i], ident), acc);
-      pre[pi + 1] = near;
-    } else {
-      y[(size_t)r * ld + i] = cadd(pre[pi], cadd(pre[pi + 1], far));
+#pragma unroll
+    for (int c = 0; c < CH; c++) {
+      const int r = r0 + c;
+      const size_t pi = 2 * ((size_t)i * R + r);
+      if (stage == 0) {
+        y[(size_t)r * ld + i] = cadd(cadd(cscale(x[(size_t)r * ld + i], ident), acc[c]), cadd(near[c], far[c]));
+      } else if (stage == 1) {
+        pre[pi] = cadd(cscale(x[(size_t)r * ld + i], ident), acc[c]);
+        pre[pi + 1] = near[c];
+      } else {
+        y[(size_t)r * ld + i] = cadd(pre[pi], cadd(pre[pi + 1], far[c]));
+      }
     }
   }
 }

@@ -495,7 +495,7 @@ static void launch_batched(ifgf_plan* P, const zd* x, long long ld, zd* y, hipSt
   const Tree& t = P->tree;
   const int N = (int)t.N, D = t.D;
   hipLaunchKernelGGL((k_gather_b<R>), dim3((unsigned)(((long long)N * R + 255) / 256)), dim3(256), 0, st, N, x, ld,
-                     P->w.p, P->perm.p, P->rho.p);
+                     P->w.p, P->perm.p, P->rho.p, P->xt_b.p);
   const int M = (int)P->near_items.n;
   // with overlap: the near field and the identity / correction half of the
   // final sum on the second stream, forked along the push chain as for one column
@@ -512,12 +512,12 @@ static void launch_batched(ifgf_plan* P, const zd* x, long long ld, zd* y, hipSt
     }
   };
   const double ident = (F == kSingleLayer) ? 0.0 : 0.5;
-  const long long nthr = (long long)N * R;
+  const long long nthr = (long long)N * (R / (R < kFinCols ? R : kFinCols));
   Z* pbuf_ = f32 ? (Z*)P->pbuf_b32.p : (Z*)P->pbuf_b.p;
   auto finalize_b = [&](int stage, hipStream_t s) {
 #define IFGF_FINB(S)                                                                                              \
   hipLaunchKernelGGL((k_finalize_b<S, R, Z>), dim3((unsigned)((nthr * S + 255) / 256)), dim3(256), 0, s, N,         \
-                     P->geo.Np, ident, x, ld, P->corr_ptr.p, P->corr_patch.p, P->corr_w.p, P->inv.p,                \
+                     P->geo.Np, ident, x, ld, P->xt_b.p, P->corr_ptr.p, P->corr_patch.p, P->corr_w.p, P->inv.p,     \
                      P->near_item_off.p, P->npart_b.p, P->it_ptr.p, P->it_slot.p, pbuf_, P->rb_max, y, stage,      \
                      P->fin_pre_b.p, mf ? P->nearb8.p : nullptr)
     if (P->final_lanes == 4) IFGF_FINB(4);
@@ -641,6 +641,7 @@ static void build_batched(ifgf_plan* P) {
   const size_t R = (size_t)P->rb_max;
   P->npart_b.alloc(P->npart.n * R);
   P->fin_pre_b.alloc(2 * (size_t)P->tree.M * R);
+  P->xt_b.alloc((size_t)P->tree.N * R);
   if (P->mixed) {
     P->lpart_b32.alloc(P->lpart.n * R);
     P->pbuf_b32.alloc((size_t)P->nslots * R);

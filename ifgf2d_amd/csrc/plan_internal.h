@@ -133,7 +133,7 @@ struct ifgf_plan {
   // or 8; 0 = none) through the batched kernels; push tasks per block width R
   // (index log2 R) with the cone cap of their LDS windows
   int rb_max = 0;
-  ifgf::DevBuf<zd> npart_b, lpart_b, pbuf_b;
+  ifgf::DevBuf<zd> npart_b, lpart_b, pbuf_b, xt_b;
   ifgf::DevBuf<zf> lpart_b32, pbuf_b32;
   ifgf::DevBuf<ifgf::PushTask> btasks[4];
   std::vector<int> blevel_off[4];
