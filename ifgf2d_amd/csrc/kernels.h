@@ -102,6 +102,57 @@ __device__ inline Z cone_eval(const Z* __restrict__ c, decltype(Z::x) s, decltyp
   return acc;
 }
 
+// cone_eval at two points of the same cone: every coefficient is read once
+// for both (half the LDS reads); each value takes cone_eval's operations in
+// cone_eval's order.
+template <int Ps, int Pang, typename Z>
+__device__ inline void cone_eval2(const Z* __restrict__ c, decltype(Z::x) s0, decltype(Z::x) t0,
+                                  decltype(Z::x) s1, decltype(Z::x) t1, Z& v0, Z& v1) {
+  using R = decltype(Z::x);
+  R ts0[Ps], ta0[Pang], ts1[Ps], ta1[Pang];
+  ts0[0] = 1;
+  ts1[0] = 1;
+  if (Ps > 1) {
+    ts0[1] = s0;
+    ts1[1] = s1;
+  }
+#pragma unroll
+  for (int i = 2; i < Ps; i++) {
+    ts0[i] = 2 * s0 * ts0[i - 1] - ts0[i - 2];
+    ts1[i] = 2 * s1 * ts1[i - 1] - ts1[i - 2];
+  }
+  ta0[0] = 1;
+  ta1[0] = 1;
+  if (Pang > 1) {
+    ta0[1] = t0;
+    ta1[1] = t1;
+  }
+#pragma unroll
+  for (int i = 2; i < Pang; i++) {
+    ta0[i] = 2 * t0 * ta0[i - 1] - ta0[i - 2];
+    ta1[i] = 2 * t1 * ta1[i - 1] - ta1[i - 2];
+  }
+  Z acc0{0, 0}, acc1{0, 0};
+#pragma unroll
+  for (int a = 0; a < Pang; a++) {
+    Z row0{0, 0}, row1{0, 0};
+#pragma unroll
+    for (int m = 0; m < Ps; m++) {
+      const Z v = c[a * Ps + m];
+      row0.x = fma(ts0[m], v.x, row0.x);
+      row0.y = fma(ts0[m], v.y, row0.y);
+      row1.x = fma(ts1[m], v.x, row1.x);
+      row1.y = fma(ts1[m], v.y, row1.y);
+    }
+    acc0.x = fma(ta0[a], row0.x, acc0.x);
+    acc0.y = fma(ta0[a], row0.y, acc0.y);
+    acc1.x = fma(ta1[a], row1.x, acc1.x);
+    acc1.y = fma(ta1[a], row1.y, acc1.y);
+  }
+  v0 = acc0;
+  v1 = acc1;
+}
+
 // Separable 2D Chebyshev transform of the P values of one cone held in LDS
 // (Interpolator::GenerateInterpolant, Interpolator.h:81-120). Thread j of the
 // cone computes output j; buf and tmp are this cone's P-slot LDS windows.
@@ -655,10 +706,15 @@ __device__ inline void level_push_body(int lb, const PushArgsT<R>& a, PushShared
   const int cl = tid / P, j = tid % P;
   const bool act = cl < task.nc;
   const long long g = (long long)task.c0 + cl;
+  // entries: EPT >= 2, thread tid takes pairs of consecutive entries
+  // 2 (tid + 256 v), + 1 (v < EPT / 2; a task's entries and every cone's run
+  // are even in number, so a pair lies in one cone); EPT = 1, entry tid
+  constexpr bool kPairs = EPT >= 2;
+  auto entry_of = [&](int u) { return kPairs ? 2 * (tid + (u / 2) * 256) + (u & 1) : tid + u * 256; };
   Geo4T<R> G[EPT];
 #pragma unroll
   for (int u = 0; u < EPT; u++) {
-    const int e = tid + u * 256;
+    const int e = entry_of(u);
     if (e < task.ne) G[u] = push_geo(a, (long long)task.e0 + e);
   }
   Z v{0, 0};
@@ -681,13 +737,27 @@ __device__ inline void level_push_body(int lb, const PushArgsT<R>& a, PushShared
     if (act) buf[tid] = c;
     __syncthreads();
   }
+  if constexpr (kPairs) {
 #pragma unroll
-  for (int u = 0; u < EPT; u++) {
-    const int e = tid + u * 256;
-    if (e < task.ne) {
-      const int ce = push_entry_cone(a, task, e);
-      const Z val = cone_eval<Ps, Pang>(buf + ce * P, G[u].s, G[u].t);
-      a.pbuf[1 + task.e0 + e] = zmul(val, Z{G[u].fre, G[u].fim});
+    for (int u = 0; u < EPT; u += 2) {
+      const int e = entry_of(u);
+      if (e < task.ne) {
+        const int ce = push_entry_cone(a, task, e);
+        Z v0, v1;
+        cone_eval2<Ps, Pang>(buf + ce * P, G[u].s, G[u].t, G[u + 1].s, G[u + 1].t, v0, v1);
+        a.pbuf[1 + task.e0 + e] = zmul(v0, Z{G[u].fre, G[u].fim});
+        a.pbuf[1 + task.e0 + e + 1] = zmul(v1, Z{G[u + 1].fre, G[u + 1].fim});
+      }
+    }
+  } else {
+#pragma unroll
+    for (int u = 0; u < EPT; u++) {
+      const int e = tid + u * 256;
+      if (e < task.ne) {
+        const int ce = push_entry_cone(a, task, e);
+        const Z val = cone_eval<Ps, Pang>(buf + ce * P, G[u].s, G[u].t);
+        a.pbuf[1 + task.e0 + e] = zmul(val, Z{G[u].fre, G[u].fim});
+      }
     }
   }
 }

@@ -924,12 +924,15 @@ static void build_stored(ifgf_plan* P) {
   HIP_TRY(hipStreamSynchronize(st));
   // push entries grouped by the cone that evaluates them: first the parent
   // nodes that fall in the cone, then the (target, cousin) interpolation
-  // entries. Push entry q writes slot 1 + q of pbuf (slot 0 stays zero), so
-  // a task's writes are one contiguous run; a parent node reads its k-th
-  // child's value through cidx[node][k] (0 for an absent child) and the
-  // finalize a target's interpolations through it_slot.
-  const long long npush = npr + nit;
-  if (npush + 1 > 2147483647LL) throw std::runtime_error("too many propagation and interpolation entries");
+  // entries, then one padding entry if the cone's run is odd, so that every
+  // run has even length and the push kernels' pairs of consecutive entries
+  // never span two cones (a padding entry is evaluated with a valid record
+  // and writes a slot nobody reads). Push entry q writes slot 1 + q of pbuf
+  // (slot 0 stays zero), so a task's writes are one contiguous run; a parent
+  // node reads its k-th child's value through cidx[node][k] (0 for an absent
+  // child) and the finalize a target's interpolations through it_slot.
+  if (npr + nit + t.n_cones + 1 > 2147483647LL)
+    throw std::runtime_error("too many propagation and interpolation entries");
   const long long NC = t.n_cones;
   // Stable counting sort of the push entries by the cone that evaluates them:
   // a cone's propagation entries in entry order, then its interpolation
@@ -962,7 +965,7 @@ static void build_stored(ifgf_plan* P) {
       }
       npro[g] = sp;
       nint[g] = si;
-      off[g + 1] = off[g] + sp + si;
+      off[g + 1] = off[g] + sp + si + ((sp + si) & 1);
     }
     // hp / hi become each chunk's first position in the cone's run
 #pragma omp parallel for schedule(static) num_threads(nth)
@@ -985,7 +988,9 @@ static void build_stored(ifgf_plan* P) {
       for (long long e = b0; e < b1; e++) qint[e] = hi[c][t.it_cone[e]]++;
     }
   }
-  std::vector<int> einfo(npush);
+  const long long npush = off[NC];
+  // padding entries read the first interpolation record (or propagation row)
+  std::vector<int> einfo(npush, nit > 0 ? 0 : -1);
   std::vector<int4> cidx((size_t)NC * Pn, make_int4(0, 0, 0, 0));
   // propagation geometry shared per (level, parent non-relevant id q, child
   // position cp): pair ids in order of first use, each gathered from its
