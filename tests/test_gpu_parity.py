@@ -293,6 +293,26 @@ def test_device_gmres_circle_mie():
     assert rel(dens, do) < 1e-7
 
 
+def test_device_gmres_circle_mie_5x5():
+    """The C1 configuration itself (5 x 5 cones): device GMRES to 1e-8 and the
+    field on r = 3 against the Mie series; the bar is set by the 5 x 5 cone
+    interpolation error (the oracle's own solve: 1.3e-4; at 9 x 9, 1e-7), and
+    the density agrees with the oracle's Householder GMRES."""
+    kw, d = configs.circle_k10()
+    fm = I.ForwardMap(**kw)
+    x, y, *_ = fm.points()
+    rhs = -configs.plane_wave(x, y, kw["k"], d)
+    dens, it, relres, hist = fm.solve(rhs, restart=50, tol=1e-8)
+    th = 2 * np.pi * np.arange(64) / 64
+    err = rel(fm.propagate_unacc(3 * np.cos(th), 3 * np.sin(th), dens), mie(3.0, th, kw["k"]))
+    om = O.ForwardMap(O.CIRCLE, [0.0, 0.0, 1.0], kw["patch_lims"], formulation=O.COMBINED, k=10.0, eta=10.0,
+                      delta=fm.delta, p=6.0, nlevels=3)
+    do, ito, erro, _ = om.gmres(rhs, restart=50, tol=1e-8)
+    print(f"5x5: GMRES {it} iterations (oracle {ito}), true relres {relres:.1e}, Mie error {err:.3e}, "
+          f"density vs oracle {rel(dens, do):.1e}", flush=True)
+    assert relres < 5e-8 and err < 5e-4 and abs(ito - it) <= 2 and rel(dens, do) < 1e-7
+
+
 def test_full_size_kite_properties():
     """Kite at k = 200, 6400 patches (N = 64,000), nlevels 10: determinism,
     linearity, and IFGF accuracy against the GPU direct sum on 2,000 targets."""
