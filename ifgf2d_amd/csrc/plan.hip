@@ -486,7 +486,7 @@ static void launch_ifgf(ifgf_plan* P, const zd* x, zd* y, hipStream_t st) {
 // One application to a block of R columns (x, y column-major, stride ld) with
 // the batched kernels; RT = double on a plain plan, float on a mixed plan.
 template <int F, int Ps, int Pang, int R, typename RT>
-static void launch_batched(ifgf_plan* P, const zd* x, long long ld, zd* y, hipStream_t st) {
+static void launch_batched(ifgf_plan* P, const zd* x, long long ld, zd* y, hipStream_t st, int lane) {
   using Z = typename complex_of<RT>::type;
   constexpr bool f32 = std::is_same<RT, float>::value;
   constexpr int Pn = Ps * Pang;
@@ -494,8 +494,20 @@ static void launch_batched(ifgf_plan* P, const zd* x, long long ld, zd* y, hipSt
   constexpr int LR = R == 2 ? 1 : R == 4 ? 2 : 3;
   const Tree& t = P->tree;
   const int N = (int)t.N, D = t.D;
+  // the lane's buffers, second stream and events (lane 1: P->lane2)
+  BatchLane* L2 = lane ? &P->lane2 : nullptr;
+  zd* rho = L2 ? L2->rho.p : P->rho.p;
+  zd* xt_b = L2 ? L2->xt_b.p : P->xt_b.p;
+  zd* npart_b = L2 ? L2->npart_b.p : P->npart_b.p;
+  zd* fin_pre_b = L2 ? L2->fin_pre_b.p : P->fin_pre_b.p;
+  zd* nearb8 = L2 ? L2->nearb8.p : P->nearb8.p;
+  Z* pbuf_lane = f32 ? (Z*)(L2 ? L2->pbuf_b32.p : P->pbuf_b32.p) : (Z*)(L2 ? L2->pbuf_b.p : P->pbuf_b.p);
+  Z* lpart_lane = f32 ? (Z*)(L2 ? L2->lpart_b32.p : P->lpart_b32.p) : (Z*)(L2 ? L2->lpart_b.p : P->lpart_b.p);
+  Z* lcoef_lane = f32 ? (Z*)(L2 ? L2->lcoef8_32.p : P->lcoef8_32.p) : (Z*)(L2 ? L2->lcoef8.p : P->lcoef8.p);
+  hipStream_t aux = L2 ? L2->aux : P->aux;
+  hipEvent_t ev_fork = L2 ? L2->ev_fork : P->ev_fork, ev_join = L2 ? L2->ev_join : P->ev_join;
   hipLaunchKernelGGL((k_gather_b<R>), dim3((unsigned)(((long long)N * R + 255) / 256)), dim3(256), 0, st, N, x, ld,
-                     P->w.p, P->perm.p, P->rho.p, P->xt_b.p);
+                     P->w.p, P->perm.p, rho, xt_b);
   const int M = (int)P->near_items.n;
   // with overlap: the near field and the identity / correction half of the
   // final sum on the second stream, forked along the push chain as for one column
@@ -505,21 +517,21 @@ static void launch_batched(ifgf_plan* P, const zd* x, long long ld, zd* y, hipSt
   auto near_b = [&](hipStream_t s) {
     if (mf) {
       hipLaunchKernelGGL(k_near_mfma, dim3((unsigned)P->nm_units.n), dim3(256), 0, s, (int)P->nm_units.n,
-                         P->nm_units.p, P->nm_grp.p, P->nm_segs.p, P->nm_A.p, P->rho.p, P->nearb8.p);
+                         P->nm_units.p, P->nm_grp.p, P->nm_segs.p, P->nm_A.p, rho, nearb8);
     } else if (M > 0) {
       hipLaunchKernelGGL((k_near_items_b<R>), dim3((M + 3) / 4), dim3(256), 0, s, N, M, P->near_items.p,
-                         P->nf_seg.p, P->nf_val.p, P->rho.p, P->npart_b.p);
+                         P->nf_seg.p, P->nf_val.p, rho, npart_b);
     }
   };
   const double ident = (F == kSingleLayer) ? 0.0 : 0.5;
   const long long nthr = (long long)N * (R / (R < kFinCols ? R : kFinCols));
-  Z* pbuf_ = f32 ? (Z*)P->pbuf_b32.p : (Z*)P->pbuf_b.p;
+  Z* pbuf_ = pbuf_lane;
   auto finalize_b = [&](int stage, hipStream_t s) {
 #define IFGF_FINB(S)                                                                                              \
   hipLaunchKernelGGL((k_finalize_b<S, R, Z>), dim3((unsigned)((nthr * S + 255) / 256)), dim3(256), 0, s, N,         \
-                     P->geo.Np, ident, x, ld, P->xt_b.p, P->corr_ptr.p, P->corr_patch.p, P->corr_w.p, P->inv.p,     \
-                     P->near_item_off.p, P->npart_b.p, P->it_ptr.p, P->it_slot.p, pbuf_, P->rb_max, y, stage,      \
-                     P->fin_pre_b.p, mf ? P->nearb8.p : nullptr)
+                     P->geo.Np, ident, x, ld, xt_b, P->corr_ptr.p, P->corr_patch.p, P->corr_w.p, P->inv.p,     \
+                     P->near_item_off.p, npart_b, P->it_ptr.p, P->it_slot.p, pbuf_, P->rb_max, y, stage,      \
+                     fin_pre_b, mf ? nearb8 : nullptr)
     if (P->final_lanes == 4) IFGF_FINB(4);
     else if (P->final_lanes == 16) IFGF_FINB(16);
     else IFGF_FINB(8);
@@ -528,22 +540,22 @@ static void launch_batched(ifgf_plan* P, const zd* x, long long ld, zd* y, hipSt
   if (!fork) near_b(st);
   const long long ML = P->leaf_items.n;
   const Z* lf = f32 ? (const Z*)P->lf_val32.p : (const Z*)P->lf_val.p;
-  Z* lpart = f32 ? (Z*)P->lpart_b32.p : (Z*)P->lpart_b.p;
+  Z* lpart = lpart_lane;
   const int* lio = P->leaf_item_off.p;
   if (mf) {
     // the leaf cones' coefficients for 8 columns, one "item" per cone
-    lpart = f32 ? (Z*)P->lcoef8_32.p : (Z*)P->lcoef8.p;
+    lpart = lcoef_lane;
     lio = P->lio_id.p;
     const RT* A = f32 ? (const RT*)P->lm_A32.p : (const RT*)P->lm_A.p;
     const int nu = (int)P->lm_units.n;
     if (nu > 0)
       hipLaunchKernelGGL((k_leaf_mfma<RT>), dim3((unsigned)((nu + 3) / 4)), dim3(256), 0, st, nu, P->lm_units.p,
-                         P->lm_grp.p, A, P->rho.p, (Z*)lpart);
+                         P->lm_grp.p, A, rho, (Z*)lpart);
   } else {
     hipLaunchKernelGGL((k_leaf_items_b<Pn, R, Z>), dim3((unsigned)((ML * Pn * kLeafChunk + 255) / 256)), dim3(256),
-                       0, st, (int)ML, P->leaf_items.p, lf, P->rho.p, lpart);
+                       0, st, (int)ML, P->leaf_items.p, lf, rho, lpart);
   }
-  Z* pbuf = f32 ? (Z*)P->pbuf_b32.p : (Z*)P->pbuf_b.p;
+  Z* pbuf = pbuf_lane;
   PushArgsT<RT> a{};
   if constexpr (f32) {
     a = PushArgsT<RT>{nullptr, nullptr, P->push_geo32.p, nullptr, P->push_coff.p, P->push_cidx.p,
@@ -558,11 +570,11 @@ static void launch_batched(ifgf_plan* P, const zd* x, long long ld, zd* y, hipSt
   const int fork_li = P->overlap_b == 2 ? 1 : 0;
   for (int li = 0; li + 1 < D; li++) {
     if (fork && li == fork_li) {
-      HIP_TRY(hipEventRecord(P->ev_fork, st));
-      HIP_TRY(hipStreamWaitEvent(P->aux, P->ev_fork, 0));
-      near_b(P->aux);
-      finalize_b(1, P->aux);
-      HIP_TRY(hipEventRecord(P->ev_join, P->aux));
+      HIP_TRY(hipEventRecord(ev_fork, st));
+      HIP_TRY(hipStreamWaitEvent(aux, ev_fork, 0));
+      near_b(aux);
+      finalize_b(1, aux);
+      HIP_TRY(hipEventRecord(ev_join, aux));
     }
     const int t0 = lo[li], nt = lo[li + 1] - t0;
     if (nt == 0) continue;
@@ -570,16 +582,16 @@ static void launch_batched(ifgf_plan* P, const zd* x, long long ld, zd* y, hipSt
     a.leaf = li == 0;
     hipLaunchKernelGGL((k_level_push_b<Ps, Pang, R, CB, RT>), dim3(nt), dim3(256), 0, st, a);
   }
-  if (fork) HIP_TRY(hipStreamWaitEvent(st, P->ev_join, 0));
+  if (fork) HIP_TRY(hipStreamWaitEvent(st, ev_join, 0));
   finalize_b(fork ? 2 : 0, st);
   HIP_TRY(hipGetLastError());
 }
 
 template <int F, int Ps, int Pang, typename RT>
-static void launch_batched_r(ifgf_plan* P, int R, const zd* x, long long ld, zd* y, hipStream_t st) {
-  if (R == 8) launch_batched<F, Ps, Pang, 8, RT>(P, x, ld, y, st);
-  else if (R == 4) launch_batched<F, Ps, Pang, 4, RT>(P, x, ld, y, st);
-  else launch_batched<F, Ps, Pang, 2, RT>(P, x, ld, y, st);
+static void launch_batched_r(ifgf_plan* P, int R, const zd* x, long long ld, zd* y, hipStream_t st, int lane) {
+  if (R == 8) launch_batched<F, Ps, Pang, 8, RT>(P, x, ld, y, st, lane);
+  else if (R == 4) launch_batched<F, Ps, Pang, 4, RT>(P, x, ld, y, st, lane);
+  else launch_batched<F, Ps, Pang, 2, RT>(P, x, ld, y, st, lane);
 }
 
 // Evaluates the stored operators once (plan creation).
@@ -718,6 +730,35 @@ static void build_batched(ifgf_plan* P) {
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(P->stream));
     P->mfma8 = true;
+  }
+  // the second lane of 8-column blocks (plans for 16 or more columns)
+  if (P->rb_max == 8 && nr >= 16) {
+    BatchLane& L = P->lane2;
+    L.rho.alloc((size_t)P->tree.N * 8);
+    L.xt_b.alloc(P->xt_b.n);
+    L.npart_b.alloc(P->npart_b.n);
+    L.fin_pre_b.alloc(P->fin_pre_b.n);
+    if (P->nearb8.n) L.nearb8.alloc(P->nearb8.n);
+    if (P->mixed) {
+      L.lpart_b32.alloc(P->lpart_b32.n);
+      L.pbuf_b32.alloc(P->pbuf_b32.n);
+      HIP_TRY(hipMemsetAsync(L.pbuf_b32.p, 0, L.pbuf_b32.n * sizeof(zf), P->stream));
+      if (P->lcoef8_32.n) L.lcoef8_32.alloc(P->lcoef8_32.n);
+    } else {
+      L.lpart_b.alloc(P->lpart_b.n);
+      L.pbuf_b.alloc(P->pbuf_b.n);
+      HIP_TRY(hipMemsetAsync(L.pbuf_b.p, 0, L.pbuf_b.n * sizeof(zd), P->stream));
+      if (P->lcoef8.n) L.lcoef8.alloc(P->lcoef8.n);
+    }
+    if (!L.st) HIP_TRY(hipStreamCreate(&L.st));
+    if (!L.aux) HIP_TRY(hipStreamCreate(&L.aux));
+    for (hipEvent_t* e : {&L.ev_fork, &L.ev_join, &L.ev_start, &L.ev_done})
+      if (!*e) HIP_TRY(hipEventCreateWithFlags(e, hipEventDisableTiming));
+    HIP_TRY(hipStreamSynchronize(P->stream));
+    P->stored_bytes += (long long)(L.rho.n + L.xt_b.n + L.npart_b.n + L.fin_pre_b.n + L.nearb8.n + L.lpart_b.n +
+                                   L.pbuf_b.n + L.lcoef8.n) * (long long)sizeof(zd) +
+                       (long long)(L.lpart_b32.n + L.pbuf_b32.n + L.lcoef8_32.n) * (long long)sizeof(zf);
+    L.ready = true;
   }
 }
 
@@ -1214,12 +1255,12 @@ static bool dispatch_p(ifgf_plan* P, const zd* x, zd* y, hipStream_t st, bool fp
 }
 
 template <int F>
-static bool dispatch_b(ifgf_plan* P, int R, const zd* x, long long ld, zd* y, hipStream_t st) {
+static bool dispatch_b(ifgf_plan* P, int R, const zd* x, long long ld, zd* y, hipStream_t st, int lane) {
   const int a = P->prm.Ps, b = P->prm.Pang;
 #define IFGF_CASE(A, B)                                              \
   if (a == A && b == B) {                                            \
-    if (P->mixed) launch_batched_r<F, A, B, float>(P, R, x, ld, y, st);  \
-    else launch_batched_r<F, A, B, double>(P, R, x, ld, y, st);      \
+    if (P->mixed) launch_batched_r<F, A, B, float>(P, R, x, ld, y, st, lane);  \
+    else launch_batched_r<F, A, B, double>(P, R, x, ld, y, st, lane);      \
     return true;                                                     \
   }
   IFGF_CASE(5, 5)
@@ -1233,11 +1274,11 @@ static bool dispatch_b(ifgf_plan* P, int R, const zd* x, long long ld, zd* y, hi
 }
 
 // one block of R (2, 4 or 8) columns through the batched kernels
-static bool dispatch_block(ifgf_plan* P, int R, const zd* x, long long ld, zd* y, hipStream_t st) {
+static bool dispatch_block(ifgf_plan* P, int R, const zd* x, long long ld, zd* y, hipStream_t st, int lane = 0) {
   switch (P->prm.formulation) {
-    case kSingleLayer: return dispatch_b<kSingleLayer>(P, R, x, ld, y, st);
-    case kDoubleLayer: return dispatch_b<kDoubleLayer>(P, R, x, ld, y, st);
-    default: return dispatch_b<kCombined>(P, R, x, ld, y, st);
+    case kSingleLayer: return dispatch_b<kSingleLayer>(P, R, x, ld, y, st, lane);
+    case kDoubleLayer: return dispatch_b<kDoubleLayer>(P, R, x, ld, y, st, lane);
+    default: return dispatch_b<kCombined>(P, R, x, ld, y, st, lane);
   }
 }
 
@@ -1373,9 +1414,19 @@ int ifgf::apply_device(ifgf_plan* P, const zd* x, zd* y, hipStream_t st, bool fp
 
 // nrhs device columns (stride ld) through the batched kernels in blocks of
 // 8 / 4 / 2 columns where the plan has them, single applications otherwise.
+// With two or more 8-column blocks and a second buffer set (nrhs_max >= 16)
+// the 8-column blocks alternate between st and the second lane's stream, so
+// one block's latency-bound level pushes overlap the other's bandwidth-bound
+// near field, leaf operator and final sum; st waits for both lanes.
 int ifgf::apply_columns(ifgf_plan* P, const zd* x, long long ld, zd* y, int nrhs, hipStream_t st, bool fp64) {
   const bool batched_ok = P->stored && P->rb_max >= 2 && !(fp64 && P->mixed);
-  int r = 0;
+  BatchLane& L2 = P->lane2;
+  const bool two = batched_ok && P->rb_max == 8 && nrhs >= 16 && L2.ready;
+  if (two) {
+    HIP_TRY(hipEventRecord(L2.ev_start, st));
+    HIP_TRY(hipStreamWaitEvent(L2.st, L2.ev_start, 0));
+  }
+  int r = 0, blk = 0;
   while (r < nrhs) {
     int R = 1;
     if (batched_ok)
@@ -1387,12 +1438,18 @@ int ifgf::apply_columns(ifgf_plan* P, const zd* x, long long ld, zd* y, int nrhs
     const zd* xr = x + (size_t)r * ld;
     zd* yr = y + (size_t)r * ld;
     if (R > 1) {
-      if (!dispatch_block(P, R, xr, ld, yr, st)) return fail(P, IFGF_ERR_ARG, "no kernel instantiation");
+      const int lane = (two && R == 8 && (blk++ & 1)) ? 1 : 0;
+      if (!dispatch_block(P, R, xr, ld, yr, lane ? L2.st : st, lane))
+        return fail(P, IFGF_ERR_ARG, "no kernel instantiation");
     } else {
       const int rc = apply_device(P, xr, yr, st, fp64);
       if (rc) return rc;
     }
     r += R;
+  }
+  if (two) {
+    HIP_TRY(hipEventRecord(L2.ev_done, L2.st));
+    HIP_TRY(hipStreamWaitEvent(st, L2.ev_done, 0));
   }
   return IFGF_OK;
 }
@@ -1407,6 +1464,7 @@ int ifgf_apply(ifgf_plan* P, const double* x, double* y, int nrhs, long long ld,
   try {
     hipStream_t st = stream ? (hipStream_t)stream : P->stream;
     const bool fp64 = (flags & IFGF_APPLY_FP64) != 0;
+    if (flags & IFGF_PTR_DEVICE) return ifgf::apply_columns(P, (const zd*)x, ld, (zd*)y, nrhs, st, fp64);
     const bool batched_ok = P->stored && P->rb_max >= 2 && !(fp64 && P->mixed);
     int r = 0;
     while (r < nrhs) {
@@ -1420,28 +1478,20 @@ int ifgf_apply(ifgf_plan* P, const double* x, double* y, int nrhs, long long ld,
           }
       const zd* xr = (const zd*)x + (size_t)r * ld;
       zd* yr = (zd*)y + (size_t)r * ld;
-      if (flags & IFGF_PTR_DEVICE) {
-        if (R > 1) {
-          if (!dispatch_block(P, R, xr, ld, yr, st)) return fail(P, IFGF_ERR_ARG, "no kernel instantiation");
-        } else {
-          int rc = apply_device(P, xr, yr, st, fp64);
-          if (rc) return rc;
-        }
+      // host columns: staged through xbuf / ybuf one block at a time
+      for (int q = 0; q < R; q++)
+        HIP_TRY(hipMemcpyAsync(P->xbuf.p + (size_t)q * N, xr + (size_t)q * ld, N * sizeof(zd), hipMemcpyHostToDevice,
+                               st));
+      if (R > 1) {
+        if (!dispatch_block(P, R, P->xbuf.p, N, P->ybuf.p, st)) return fail(P, IFGF_ERR_ARG, "no kernel instantiation");
       } else {
-        for (int q = 0; q < R; q++)
-          HIP_TRY(hipMemcpyAsync(P->xbuf.p + (size_t)q * N, xr + (size_t)q * ld, N * sizeof(zd), hipMemcpyHostToDevice,
-                                 st));
-        if (R > 1) {
-          if (!dispatch_block(P, R, P->xbuf.p, N, P->ybuf.p, st)) return fail(P, IFGF_ERR_ARG, "no kernel instantiation");
-        } else {
-          int rc = apply_device(P, P->xbuf.p, P->ybuf.p, st, fp64);
-          if (rc) return rc;
-        }
-        for (int q = 0; q < R; q++)
-          HIP_TRY(hipMemcpyAsync(yr + (size_t)q * ld, P->ybuf.p + (size_t)q * N, N * sizeof(zd), hipMemcpyDeviceToHost,
-                                 st));
-        HIP_TRY(hipStreamSynchronize(st));
+        int rc = apply_device(P, P->xbuf.p, P->ybuf.p, st, fp64);
+        if (rc) return rc;
       }
+      for (int q = 0; q < R; q++)
+        HIP_TRY(hipMemcpyAsync(yr + (size_t)q * ld, P->ybuf.p + (size_t)q * N, N * sizeof(zd), hipMemcpyDeviceToHost,
+                               st));
+      HIP_TRY(hipStreamSynchronize(st));
       r += R;
     }
   } catch (std::exception& e) {

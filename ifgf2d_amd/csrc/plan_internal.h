@@ -46,6 +46,25 @@ struct DevBuf {
 };
 
 
+// A second set of the 8-column blocks' per-call buffers, streams and events
+// (plans with nrhs_max >= 16): the 8-column blocks of one call alternate
+// between the plan's own set on the caller's stream and this one on st, so
+// one block's latency-bound level pushes run beside another block's
+// bandwidth-bound near field, leaf operator and final sums.
+struct BatchLane {
+  DevBuf<zd> rho, xt_b, npart_b, fin_pre_b, nearb8, lpart_b, pbuf_b, lcoef8;
+  DevBuf<zf> lpart_b32, pbuf_b32, lcoef8_32;
+  hipStream_t st = nullptr, aux = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_start = nullptr, ev_done = nullptr;
+  bool ready = false;
+  ~BatchLane() {
+    for (hipEvent_t e : {ev_fork, ev_join, ev_start, ev_done})
+      if (e) (void)hipEventDestroy(e);
+    if (aux) (void)hipStreamDestroy(aux);
+    if (st) (void)hipStreamDestroy(st);
+  }
+};
+
 // Device workspaces of the single-system solvers (gmres.hip), kept by the
 // plan between calls.
 struct SolverCache;
@@ -137,6 +156,7 @@ struct ifgf_plan {
   ifgf::DevBuf<zf> lpart_b32, pbuf_b32;
   ifgf::DevBuf<ifgf::PushTask> btasks[4];
   std::vector<int> blevel_off[4];
+  ifgf::BatchLane lane2;  // the second set of 8-column buffers (nrhs_max >= 16)
   // mixed precision (IFGF_MIXED): FP32 copies of the leaf operator, the
   // propagation / interpolation geometry and inverse node factors, and FP32
   // coefficient workspaces; the FP64 operator stays available (IFGF_APPLY_FP64)

@@ -231,6 +231,43 @@ def test_batched_columns_equal_single_applications(precision):
     assert np.all(yt[:, n:].cpu().numpy() == 0)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_two_lane_column_blocks(precision):
+    """A plan for 32 columns keeps a second set of 8-column buffers: device
+    calls with two or more 8-column blocks alternate them between the
+    caller's stream and the second lane's stream. 27 columns (blocks 8, 8, 8,
+    2, 1) through device pointers (ld > n, on torch's current stream) equal
+    the host-pointer call bitwise (one lane, the same blocks), which equals
+    single applications to rounding; repeated device calls are bitwise
+    deterministic."""
+    torch = pytest.importorskip("torch")
+    kw = configs.scaled(I.star(a=[0, 0, 0, 0, 0, 0.3]), 800, 60.0, 7)
+    fm = I.ForwardMap(**kw, precision=precision, nrhs_max=32)
+    n, nr = fm.N, 27
+    X = np.stack([density(n, 90 + s) for s in range(nr)], axis=1)
+    Yh = fm.compute_Ax_acc(X)
+    tol = BATCH_TOL if precision == "fp64" else 1e-6
+    e = max(rel(Yh[:, j], fm.compute_Ax_acc(X[:, j])) for j in (0, 9, 17, 25, 26))
+    print(f"{precision}: host blocks vs single applications {e:.3e}", flush=True)
+    assert e <= tol
+    ld = n + 5
+    xt = torch.zeros((nr, ld), dtype=torch.complex128, device="cuda:0")
+    xt[:, :n] = torch.from_numpy(np.ascontiguousarray(X.T))
+    s = torch.cuda.current_stream()
+    outs = []
+    for _ in range(2):
+        yt = torch.zeros_like(xt)
+        torch.cuda.synchronize()
+        fm.apply_device(xt.data_ptr(), yt.data_ptr(), nrhs=nr, ld=ld, stream=s.cuda_stream)
+        # ordered on the caller's stream: a copy queued behind it sees every lane's output
+        outs.append(yt.cpu().numpy())
+    for Yd in outs:
+        d = np.abs(Yd[:, :n].T - Yh).max()
+        print(f"{precision}: two-lane device call vs host call max |diff| {d:.3e}", flush=True)
+        assert np.array_equal(Yd[:, :n].T, Yh)
+        assert np.all(Yd[:, n:] == 0)
+
+
 def test_device_pointer_apply_with_torch_stream():
     torch = pytest.importorskip("torch")
     fm, _ = build_pair(I.kite(), 400, 40.0, 6, I.COMBINED, 5, 5)
