@@ -711,25 +711,53 @@ __device__ inline void level_push_body(int lb, const PushArgsT<R>& a, PushShared
   // are even in number, so a pair lies in one cone); EPT = 1, entry tid
   constexpr bool kPairs = EPT >= 2;
   auto entry_of = [&](int u) { return kPairs ? 2 * (tid + (u / 2) * 256) + (u & 1) : tid + u * 256; };
-  Geo4T<R> G[EPT];
-#pragma unroll
-  for (int u = 0; u < EPT; u++) {
+  // every thread loads EPT records (indices clamped into the task, so the
+  // loads are unconditional and their outstanding count is the same on every
+  // path; the extra ones are never used). With entry records (einfo) all EPT
+  // record indices are loaded first and each record's address is a select,
+  // so the records cost one dependent round trip, not one per entry.
+  auto rec_index = [&](int u) -> long long {
     const int e = entry_of(u);
-    if (e < task.ne) G[u] = push_geo(a, (long long)task.e0 + e);
-  }
+    return (long long)task.e0 + (e < task.ne ? e : task.ne - 1);
+  };
+  int inf[EPT];
+  auto load_info = [&] {
+    if (a.einfo) {
+#pragma unroll
+      for (int u = 0; u < EPT; u++) inf[u] = a.einfo[rec_index(u)];
+    }
+  };
+  Geo4T<R> G[EPT];
+  auto load_geo = [&] {
+#pragma unroll
+    for (int u = 0; u < EPT; u++) {
+      const Geo4T<R>* src =
+          a.einfo ? (inf[u] >= 0 ? a.igeo + inf[u] : a.ptab + (-1 - inf[u])) : a.igeo + rec_index(u);
+      G[u] = *src;
+    }
+  };
   Z v{0, 0};
-  if (act) {
-    if (a.leaf) {
+  if (a.leaf) {
+    load_info();
+    load_geo();
+    if (act) {
       const int c = (int)(g - a.leaf_cone_off);
       const int w1 = a.lio[c + 1];
       for (int w = a.lio[c]; w < w1; w++) v = zadd(v, a.lpart[(size_t)w * P + j]);
-    } else {
-      const int4 ci = a.cidx[g * P + j];
-      const Z s0 = a.pbuf[ci.x], s1 = a.pbuf[ci.y], s2 = a.pbuf[ci.z], s3 = a.pbuf[ci.w];
-      v = zmul(zadd(zadd(zadd(s0, s1), s2), s3), a.ifac_tab[a.ifac_off[g] + j % Ps]);
     }
-    buf[tid] = v;
+  } else {
+    // the children's values first, the entries' records behind them: the
+    // records (most of the level's bytes) are still arriving while the
+    // block sums the children and runs the transform
+    load_info();
+    const long long gg = act ? g : (long long)task.c0;
+    const int4 ci = a.cidx[gg * P + j];
+    const Z fac = a.ifac_tab[a.ifac_off[gg] + j % Ps];
+    const Z s0 = a.pbuf[ci.x], s1 = a.pbuf[ci.y], s2 = a.pbuf[ci.z], s3 = a.pbuf[ci.w];
+    load_geo();
+    v = zmul(zadd(zadd(zadd(s0, s1), s2), s3), fac);
   }
+  if (act) buf[tid] = v;
   __syncthreads();
   if (!a.leaf) {
     const Z c = cone_transform<Ps, Pang>(buf + (act ? cl * P : 0), tmp + (act ? cl * P : 0), T, act ? j : 0, act);

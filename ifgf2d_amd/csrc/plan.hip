@@ -609,8 +609,10 @@ static void make_push_tasks(const ifgf_plan* P, int emax, int cmax, std::vector<
     const int l = D - li;
     level_off[li] = (int)tasks.size();
     PushTask cur{0, 0, 0, 0};
+    // a task without entries writes nothing and is dropped (the kernels
+    // read records at indices clamped into [e0, e0 + ne))
     auto flush = [&] {
-      if (cur.nc) tasks.push_back(cur);
+      if (cur.nc && cur.ne) tasks.push_back(cur);
       cur = PushTask{0, 0, 0, 0};
     };
     const long long c0 = t.lv[l].cone_off, c1 = c0 + t.lv[l].ncone;
