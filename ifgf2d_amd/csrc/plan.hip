@@ -8,6 +8,7 @@
 #include <cmath>
 #include <complex>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -1319,38 +1320,32 @@ static void autotune(ifgf_plan* P) {
     }
     return best;
   };
-  struct Knob {
-    int* v;
-    std::vector<int> cand;
+  // Two interleaved sweeps over a knob's candidates, the faster time of each
+  // kept; a candidate replaces the knob's default only when it is faster by
+  // more than 2% (near-ties otherwise flip with timing noise between runs).
+  auto pick = [&](int def, const std::vector<int>& cand, const std::function<void(int)>& set) {
+    std::vector<float> tm(cand.size(), 1e30f);
+    for (int sweep = 0; sweep < 2; sweep++)
+      for (size_t q = 0; q < cand.size(); q++) {
+        set(cand[q]);
+        tm[q] = std::min(tm[q], time_apply());
+      }
+    float tdef = 1e30f;
+    for (size_t q = 0; q < cand.size(); q++)
+      if (cand[q] == def) tdef = tm[q];
+    int best = def;
+    float tb = tdef;
+    for (size_t q = 0; q < cand.size(); q++)
+      if (tm[q] < 0.98f * tdef && tm[q] < tb) {
+        tb = tm[q];
+        best = cand[q];
+      }
+    set(best);
   };
-  Knob knobs[] = {{&P->final_lanes, {4, 8, 16}}, {&P->overlap, {0, 1, 2}}};
   // entries per level-push task: fewer give more, shorter blocks
-  if (P->stored) {
-    int best = P->push_emax;
-    float tbest = 1e30f;
-    for (int c : {256, 512, 1024, 2048}) {
-      build_push_tasks(P, c);
-      const float t = time_apply();
-      if (t < tbest) {
-        tbest = t;
-        best = c;
-      }
-    }
-    if (best != P->push_emax) build_push_tasks(P, best);
-  }
-  for (auto& kb : knobs) {
-    int best = *kb.v;
-    float tbest = 1e30f;
-    for (int c : kb.cand) {
-      *kb.v = c;
-      const float t = time_apply();
-      if (t < tbest) {
-        tbest = t;
-        best = c;
-      }
-    }
-    *kb.v = best;
-  }
+  if (P->stored) pick(P->push_emax, {256, 512, 1024, 2048}, [&](int c) { build_push_tasks(P, c); });
+  pick(P->final_lanes, {4, 8, 16}, [&](int c) { P->final_lanes = c; });
+  pick(P->overlap, {0, 1, 2}, [&](int c) { P->overlap = c; });
   // the batched blocks' second stream, timed on one block of the widest width
   P->overlap_b = P->overlap;
   if (P->stored && P->rb_max > 0) {
