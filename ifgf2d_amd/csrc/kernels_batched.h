@@ -135,18 +135,32 @@ __global__ void __launch_bounds__(256)
 }
 
 // k_level_push with R columns. Tasks hold at most CB cones (their R-column
-// coefficients fit the LDS windows) and 256 entries (one per thread); the
-// cone phase runs on threads (cone, node) < CB * P. Every slot of pbuf holds
-// a.rs >= R columns whatever the block width, so the zero slot that absent
-// children read stays zero for every width.
+// coefficients fit the LDS windows) and 256 entries (one record per thread);
+// the cone phase runs on threads (cone, node) < CB * P. The evaluation runs
+// on units: every cone's run of entries is cut into pieces of at most K
+// entries (K chosen per task so that all pieces fit 256 / R units), and the
+// R threads of a unit, one per column, hold their column's P coefficients
+// in registers and walk the piece's entries, whose records the block keeps
+// in LDS: one LDS read of each coefficient per piece instead of one per
+// entry (cones of up to 35 nodes; larger ones evaluate one entry per thread
+// from the LDS coefficients). Every slot of pbuf holds a.rs >= R columns whatever the block
+// width, so the zero slot that absent children read stays zero for every
+// width.
 template <int Ps, int Pang, int R, int CB, typename RT>
 __global__ void __launch_bounds__(256) k_level_push_b(PushArgsT<RT> a) {
   using Z = typename complex_of<RT>::type;
   constexpr int P = Ps * Pang;
   static_assert(CB * P <= 256, "cone phase needs CB * P threads");
+  // unit evaluation while a column's P coefficients fit the registers
+  constexpr bool kUnitEval = P <= 35;
+  constexpr int kUnits = 256 / R;
+  static_assert(!kUnitEval || kUnits > CB, "every cone needs a unit");
+  // tmp is free after the transform and then holds the task's records
+  constexpr int kGeoZ = (int)(256 * sizeof(Geo4T<RT>) / sizeof(Z));
+  constexpr int kTmp = CB * P * R > kGeoZ ? CB * P * R : kGeoZ;
   __shared__ ChebTablesT<RT> T;
   __shared__ Z buf[CB * P * R];
-  __shared__ Z tmp[CB * P * R];
+  __shared__ Z tmp[kTmp];
   const PushTask task = a.tasks[xcd_block()];
   const int tid = threadIdx.x;
   if (!a.leaf) load_transform_rows<Ps, Pang>(T, a.tabs);
@@ -213,36 +227,91 @@ __global__ void __launch_bounds__(256) k_level_push_b(PushArgsT<RT> a) {
     }
     __syncthreads();
   }
-  if (has) {
-    // cone_eval's operations per column
-    RT ts[Ps], ta[Pang];
-    ts[0] = 1;
-    if (Ps > 1) ts[1] = G.s;
+  if constexpr (!kUnitEval) {
+    // larger cones: one entry per thread, the coefficients read from LDS
+    if (has) {
+      RT ts[Ps], ta[Pang];
+      ts[0] = 1;
+      if (Ps > 1) ts[1] = G.s;
 #pragma unroll
-    for (int i = 2; i < Ps; i++) ts[i] = 2 * G.s * ts[i - 1] - ts[i - 2];
-    ta[0] = 1;
-    if (Pang > 1) ta[1] = G.t;
+      for (int i = 2; i < Ps; i++) ts[i] = 2 * G.s * ts[i - 1] - ts[i - 2];
+      ta[0] = 1;
+      if (Pang > 1) ta[1] = G.t;
 #pragma unroll
-    for (int i = 2; i < Pang; i++) ta[i] = 2 * G.t * ta[i - 1] - ta[i - 2];
-    const Z* c = buf + (size_t)push_entry_cone(a, task, tid) * P * R;
-    const Z f{G.fre, G.fim};
-    Z* out = a.pbuf + (size_t)(1 + task.e0 + tid) * a.rs;
+      for (int i = 2; i < Pang; i++) ta[i] = 2 * G.t * ta[i - 1] - ta[i - 2];
+      const Z* c = buf + (size_t)push_entry_cone(a, task, tid) * P * R;
+      const Z f{G.fre, G.fim};
+      Z* out = a.pbuf + (size_t)(1 + task.e0 + tid) * a.rs;
 #pragma unroll
-    for (int r = 0; r < R; r++) {
+      for (int r = 0; r < R; r++) {
+        Z acc{0, 0};
+#pragma unroll
+        for (int aa = 0; aa < Pang; aa++) {
+          Z row{0, 0};
+#pragma unroll
+          for (int m = 0; m < Ps; m++) {
+            const Z v = c[(aa * Ps + m) * R + r];
+            row.x = fma(ts[m], v.x, row.x);
+            row.y = fma(ts[m], v.y, row.y);
+          }
+          acc.x = fma(ta[aa], row.x, acc.x);
+          acc.y = fma(ta[aa], row.y, acc.y);
+        }
+        out[r] = zmul(acc, f);
+      }
+    }
+    return;
+  }
+  // the records into LDS (tmp's storage; every thread is past the transform)
+  Geo4T<RT>* Gs = reinterpret_cast<Geo4T<RT>*>(tmp);
+  if (has) Gs[tid] = G;
+  __syncthreads();
+  // evaluation units: thread tid = column r of unit u
+  const int u = tid / R, r = tid % R;
+  const int K = (task.ne + (kUnits - task.nc) - 1) / (kUnits - task.nc);
+  int uc = -1, lo = 0, hi = 0, ub = 0;
+  for (int q = 0; q < task.nc; q++) {
+    const int b0 = max(a.coff[task.c0 + q] - task.e0, 0);
+    const int b1 = min(a.coff[task.c0 + q + 1] - task.e0, task.ne);
+    const int nu = b1 > b0 ? (b1 - b0 + K - 1) / K : 0;
+    if (u >= ub && u < ub + nu) {
+      uc = q;
+      lo = b0 + (u - ub) * K;
+      hi = min(lo + K, b1);
+    }
+    ub += nu;
+  }
+  if (uc >= 0) {
+    Z cr[P];
+#pragma unroll
+    for (int q = 0; q < P; q++) cr[q] = buf[((size_t)uc * P + q) * R + r];
+    Z* out = a.pbuf + (size_t)(1 + task.e0) * a.rs + r;
+    for (int e = lo; e < hi; e++) {
+      const Geo4T<RT> g4 = Gs[e];
+      // cone_eval's operations
+      RT ts[Ps], ta[Pang];
+      ts[0] = 1;
+      if (Ps > 1) ts[1] = g4.s;
+#pragma unroll
+      for (int i = 2; i < Ps; i++) ts[i] = 2 * g4.s * ts[i - 1] - ts[i - 2];
+      ta[0] = 1;
+      if (Pang > 1) ta[1] = g4.t;
+#pragma unroll
+      for (int i = 2; i < Pang; i++) ta[i] = 2 * g4.t * ta[i - 1] - ta[i - 2];
       Z acc{0, 0};
 #pragma unroll
       for (int aa = 0; aa < Pang; aa++) {
         Z row{0, 0};
 #pragma unroll
         for (int m = 0; m < Ps; m++) {
-          const Z v = c[(aa * Ps + m) * R + r];
+          const Z v = cr[aa * Ps + m];
           row.x = fma(ts[m], v.x, row.x);
           row.y = fma(ts[m], v.y, row.y);
         }
         acc.x = fma(ta[aa], row.x, acc.x);
         acc.y = fma(ta[aa], row.y, acc.y);
       }
-      out[r] = zmul(acc, f);
+      out[(size_t)e * a.rs] = zmul(acc, Z{g4.fre, g4.fim});
     }
   }
 }
