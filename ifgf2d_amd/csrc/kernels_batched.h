@@ -21,6 +21,10 @@ namespace ifgf {
 
 // Cones per batched push task: two LDS windows of CB x P x R complex values
 // within 40 KB, and at most 256 threads in the cone phase.
+// entries per batched push task with unit evaluation (cones of <= 35 nodes)
+constexpr int kBatchedEntries = 512;
+constexpr int batched_entries(int P) { return P <= 35 ? kBatchedEntries : 256; }
+
 constexpr int batched_cones(int P, int R, int zbytes) {
   const int c = 40960 / (2 * P * R * zbytes);
   const int m = c < 256 / P ? c : 256 / P;
@@ -135,7 +139,7 @@ __global__ void __launch_bounds__(256)
 }
 
 // k_level_push with R columns. Tasks hold at most CB cones (their R-column
-// coefficients fit the LDS windows) and 256 entries (one record per thread);
+// coefficients fit the LDS windows) and batched_entries(P) entries;
 // the cone phase runs on threads (cone, node) < CB * P. The evaluation runs
 // on units: every cone's run of entries is cut into pieces of at most K
 // entries (K chosen per task so that all pieces fit 256 / R units), and the
@@ -156,7 +160,8 @@ __global__ void __launch_bounds__(256) k_level_push_b(PushArgsT<RT> a) {
   constexpr int kUnits = 256 / R;
   static_assert(!kUnitEval || kUnits > CB, "every cone needs a unit");
   // tmp is free after the transform and then holds the task's records
-  constexpr int kGeoZ = (int)(256 * sizeof(Geo4T<RT>) / sizeof(Z));
+  constexpr int kRec = kUnitEval ? kBatchedEntries / 256 : 1;  // records per thread
+  constexpr int kGeoZ = (int)(256 * kRec * sizeof(Geo4T<RT>) / sizeof(Z));
   constexpr int kTmp = CB * P * R > kGeoZ ? CB * P * R : kGeoZ;
   __shared__ ChebTablesT<RT> T;
   __shared__ Z buf[CB * P * R];
@@ -167,9 +172,11 @@ __global__ void __launch_bounds__(256) k_level_push_b(PushArgsT<RT> a) {
   const int cl = tid / P, j = tid % P;
   const bool act = cl < task.nc && cl < CB;
   const long long g = (long long)task.c0 + cl;
-  Geo4T<RT> G;
+  Geo4T<RT> G[kRec];
   const bool has = tid < task.ne;
-  if (has) G = push_geo(a, (long long)task.e0 + tid);
+#pragma unroll
+  for (int q = 0; q < kRec; q++)
+    if (tid + q * 256 < task.ne) G[q] = push_geo(a, (long long)task.e0 + tid + q * 256);
   // cone inputs: column r of node j of cone cl at buf[(cl * P + j) * R + r]
   if (act) {
     if (a.leaf) {
@@ -232,15 +239,15 @@ __global__ void __launch_bounds__(256) k_level_push_b(PushArgsT<RT> a) {
     if (has) {
       RT ts[Ps], ta[Pang];
       ts[0] = 1;
-      if (Ps > 1) ts[1] = G.s;
+      if (Ps > 1) ts[1] = G[0].s;
 #pragma unroll
-      for (int i = 2; i < Ps; i++) ts[i] = 2 * G.s * ts[i - 1] - ts[i - 2];
+      for (int i = 2; i < Ps; i++) ts[i] = 2 * G[0].s * ts[i - 1] - ts[i - 2];
       ta[0] = 1;
-      if (Pang > 1) ta[1] = G.t;
+      if (Pang > 1) ta[1] = G[0].t;
 #pragma unroll
-      for (int i = 2; i < Pang; i++) ta[i] = 2 * G.t * ta[i - 1] - ta[i - 2];
+      for (int i = 2; i < Pang; i++) ta[i] = 2 * G[0].t * ta[i - 1] - ta[i - 2];
       const Z* c = buf + (size_t)push_entry_cone(a, task, tid) * P * R;
-      const Z f{G.fre, G.fim};
+      const Z f{G[0].fre, G[0].fim};
       Z* out = a.pbuf + (size_t)(1 + task.e0 + tid) * a.rs;
 #pragma unroll
       for (int r = 0; r < R; r++) {
@@ -264,7 +271,9 @@ __global__ void __launch_bounds__(256) k_level_push_b(PushArgsT<RT> a) {
   }
   // the records into LDS (tmp's storage; every thread is past the transform)
   Geo4T<RT>* Gs = reinterpret_cast<Geo4T<RT>*>(tmp);
-  if (has) Gs[tid] = G;
+#pragma unroll
+  for (int q = 0; q < kRec; q++)
+    if (tid + q * 256 < task.ne) Gs[tid + q * 256] = G[q];
   __syncthreads();
   // evaluation units: thread tid = column r of unit u
   const int u = tid / R, r = tid % R;
