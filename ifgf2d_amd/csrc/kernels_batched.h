@@ -22,7 +22,7 @@ namespace ifgf {
 // Cones per batched push task: two LDS windows of CB x P x R complex values
 // within 40 KB, and at most 256 threads in the cone phase.
 // entries per batched push task with unit evaluation (cones of <= 35 nodes)
-constexpr int kBatchedEntries = 512;
+constexpr int kBatchedEntries = 1024;
 constexpr int batched_entries(int P) { return P <= 35 ? kBatchedEntries : 256; }
 
 constexpr int batched_cones(int P, int R, int zbytes) {
