@@ -288,8 +288,8 @@ void ifgf_field_destroy(ifgf_field* field);
 /* Restarted GMRES on the GPU with the plan's forward map: replaces
  * ForwardMap::solve's Eigen::opGMRES<LinearOperator> (ForwardMap.tpp:658-697,
  * opGMRES.hpp:68-269). Arnoldi by classical Gram-Schmidt with a second
- * pass when the first cancels more than half of the new vector's squared
- * norm (FP64). Stops when the recursive residual estimate
+ * pass when the first leaves less than a tenth of the new vector's norm
+ * (FP64). Stops when the recursive residual estimate
  * |w(k)| / ||r0|| < tol (opGMRES.hpp:207) or after maxit iterations.
  * rhs / x are host pointers of n complex values; x holds the initial guess.
  * relres returns the true residual ||b - A x|| / ||b|| at exit, always with
