@@ -189,14 +189,16 @@ __global__ void __launch_bounds__(64) k_sum0(const zd* __restrict__ part, zd* __
   if (threadIdx.x == 0) out[0] = sum_partials(part, 0);
 }
 
-// x -= sign * sum_{j < ncol} B[:,j] y[j]
+// x -= sign * sum_{j < ncol} B[:,j] y[j] (B in FP64 or, for an FP32 Krylov
+// basis, FP32 widened to FP64)
+template <typename VT>
 __global__ void __launch_bounds__(256)
-    k_update(long long n, int ncol, const zd* __restrict__ B, long long ldb, const zd* __restrict__ y, double sign,
+    k_update(long long n, int ncol, const VT* __restrict__ B, long long ldb, const zd* __restrict__ y, double sign,
              zd* __restrict__ x) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   zd acc = x[i];
-  for (int j = 0; j < ncol; j++) cfma(acc, cscale(y[j], -sign), B[(size_t)j * ldb + i]);
+  for (int j = 0; j < ncol; j++) cfma(acc, cscale(y[j], -sign), zwiden(B[(size_t)j * ldb + i]));
   x[i] = acc;
 }
 
@@ -224,17 +226,20 @@ __global__ void __launch_bounds__(64)
 }
 
 // Column wcol takes W[:, a] itself; gate (second pass): runs for slot a only
-// when gate[a] is set.
+// when gate[a] is set. The basis V in FP64 or FP32 (widened; sums in FP64).
+template <typename VT>
 __global__ void __launch_bounds__(256)
-    k_dots_ms(long long n, const zd* __restrict__ V, long long vs, const zd* __restrict__ W, zd* __restrict__ part,
+    k_dots_ms(long long n, const VT* __restrict__ V, long long vs, const zd* __restrict__ W, zd* __restrict__ part,
               long long ps, const int* __restrict__ sys, int wcol, const int* __restrict__ gate) {
   const int col = blockIdx.y, a = blockIdx.z;
   if (gate && !gate[a]) return;
   const zd* w = W + (size_t)a * n;
-  const zd* v = col == wcol ? w : V + (size_t)sys[a] * vs + (size_t)col * n;
+  const VT* v = V + (size_t)sys[a] * vs + (size_t)col * n;
+  const bool self = col == wcol;
   zd acc{0.0, 0.0};
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-    const zd x = v[i], y = w[i];
+    const zd y = w[i];
+    const zd x = self ? y : zwiden(v[i]);
     acc.x = fma(x.x, y.x, fma(x.y, y.y, acc.x));
     acc.y = fma(x.x, y.y, fma(-x.y, y.x, acc.y));
   }
@@ -254,8 +259,9 @@ __global__ void __launch_bounds__(256)
 
 // k_project_out for slot a = blockIdx.y (system sys[a]); the per-block
 // squared norms go to part_n + a * pns.
+template <typename VT>
 __global__ void __launch_bounds__(256)
-    k_project_out_ms(long long n, int ncol, const zd* __restrict__ V, long long vs, const zd* __restrict__ part,
+    k_project_out_ms(long long n, int ncol, const VT* __restrict__ V, long long vs, const zd* __restrict__ part,
                      long long ps, zd* __restrict__ H, long long hs, int accumulate, zd* __restrict__ W,
                      const int* __restrict__ sys, double* __restrict__ part_n, long long pns,
                      const int* __restrict__ gate) {
@@ -271,10 +277,10 @@ __global__ void __launch_bounds__(256)
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   double sq = 0.0;
   if (i < n) {
-    const zd* v = V + (size_t)s * vs;
+    const VT* v = V + (size_t)s * vs;
     zd* w = W + (size_t)a * n;
     zd acc = w[i];
-    for (int c = 0; c < ncol; c++) cfma(acc, cscale(hsh[c], -1.0), v[(size_t)c * n + i]);
+    for (int c = 0; c < ncol; c++) cfma(acc, cscale(hsh[c], -1.0), zwiden(v[(size_t)c * n + i]));
     w[i] = acc;
     sq = fma(acc.x, acc.x, acc.y * acc.y);
   }
@@ -289,17 +295,18 @@ __global__ void __launch_bounds__(256)
 
 // k_arnoldi_end for slot a = blockIdx.y: V_s[:, col] = W[:, a] / ||W[:, a]||,
 // H[s][col] = the norm (the second pass's when second_pass() held).
+template <typename VT>
 __global__ void __launch_bounds__(256)
     k_arnoldi_end_ms(long long n, int ncol, const zd* __restrict__ W, const int* __restrict__ second,
                      const double* __restrict__ part_n1, const double* __restrict__ part_n2, long long pns,
-                     zd* __restrict__ V, long long vs, zd* __restrict__ H, long long hs, const int* __restrict__ sys) {
+                     VT* __restrict__ V, long long vs, zd* __restrict__ H, long long hs, const int* __restrict__ sys) {
   const int a = blockIdx.y, s = sys[a];
   const bool two = second[a] != 0;
   const double nn = sqrt(block_sum_fixed((two ? part_n2 : part_n1) + (size_t)a * pns, gridDim.x));
   if (blockIdx.x == 0 && threadIdx.x == 0) H[(size_t)s * hs + ncol] = zd{nn, 0.0};
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  V[(size_t)s * vs + (size_t)ncol * n + i] = cscale(W[(size_t)a * n + i], 1.0 / nn);
+  V[(size_t)s * vs + (size_t)ncol * n + i] = zcast<VT>(cscale(W[(size_t)a * n + i], 1.0 / nn));
 }
 
 // partials of ||W[:, a]||^2
@@ -321,9 +328,10 @@ __global__ void __launch_bounds__(256)
 }
 
 // V[:, col] of system sys[a] = W[:, a] / ||W[:, a]||; H[sys[a]][col] = the norm
+template <typename VT>
 __global__ void __launch_bounds__(256)
     k_normalize_ms(long long n, const zd* __restrict__ W, const zd* __restrict__ part, long long ps,
-                   zd* __restrict__ V, long long vs, int col, zd* __restrict__ H, long long hs,
+                   VT* __restrict__ V, long long vs, int col, zd* __restrict__ H, long long hs,
                    const int* __restrict__ sys) {
   const int a = blockIdx.y, s = sys[a];
   __shared__ double nn2;
@@ -333,17 +341,18 @@ __global__ void __launch_bounds__(256)
   if (blockIdx.x == 0 && threadIdx.x == 0 && H) H[(size_t)s * hs + col] = zd{nn, 0.0};
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  V[(size_t)s * vs + (size_t)col * n + i] = cscale(W[(size_t)a * n + i], 1.0 / nn);
+  V[(size_t)s * vs + (size_t)col * n + i] = zcast<VT>(cscale(W[(size_t)a * n + i], 1.0 / nn));
 }
 
 // dst[:, a] = src_s[:, col] for s = sys[a] (src system stride ss, column stride n)
+template <typename VT>
 __global__ void __launch_bounds__(256)
-    k_gather_ms(long long n, const zd* __restrict__ src, long long ss, int col, const int* __restrict__ sys,
+    k_gather_ms(long long n, const VT* __restrict__ src, long long ss, int col, const int* __restrict__ sys,
                 zd* __restrict__ dst) {
   const int a = blockIdx.y;
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  dst[(size_t)a * n + i] = src[(size_t)sys[a] * ss + (size_t)col * n + i];
+  dst[(size_t)a * n + i] = zwiden(src[(size_t)sys[a] * ss + (size_t)col * n + i]);
 }
 
 // W[:, a] = B_s - W[:, a] for s = sys[a]
@@ -493,7 +502,7 @@ struct Solver {
   // x += B y for the first kk columns of B (ld n)
   void update(Krylov& K, const zd* B, int kk, const std::vector<std::complex<double>>& y, zd* x) {
     HIP_TRY(hipMemcpyAsync(K.h.p, y.data(), kk * sizeof(zd), hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_update, dim3(nb), dim3(256), 0, st, n, kk, B, n, K.h.p, -1.0, x);
+    hipLaunchKernelGGL(k_update<zd>, dim3(nb), dim3(256), 0, st, n, kk, B, n, K.h.p, -1.0, x);
   }
 };
 
@@ -778,7 +787,11 @@ extern "C" int ifgf_fgmres(ifgf_plan* P, const double* rhs, double* x, int resta
 // The lockstep GMRES on device columns: B (right-hand sides) and X (initial
 // guesses in, solutions out), n x nrhs each. true_res: relres[s] = the true
 // relative residual with the all-FP64 operator; else the residual norm at the
-// start of the last cycle over ||b_s||.
+// start of the last cycle over ||b_s||. VT: the Krylov basis in FP64 (zd) or
+// FP32 (zf, for the inner corrections of the mixed-precision refinement,
+// whose tolerance is 1e-5 or looser and whose operator is FP32; projections,
+// norms and the work vectors stay FP64).
+template <typename VT>
 static int gmres_multi_dev(ifgf_plan* P, const zd* Bp, zd* Xp, int nrhs, int restart, double tol, int maxit,
                            int* iters, double* relres, bool true_res) {
   using C = std::complex<double>;
@@ -789,7 +802,8 @@ static int gmres_multi_dev(ifgf_plan* P, const zd* Bp, zd* Xp, int nrhs, int res
   const long long pns = 2 * (long long)nb;
   try {
     hipStream_t st = P->stream;
-    DevBuf<zd> V, W, part, H;
+    DevBuf<VT> V;
+    DevBuf<zd> W, part, H;
     DevBuf<double> part_n;
     DevBuf<int> sysd, gate;
     V.alloc((size_t)vs * R);
@@ -821,7 +835,7 @@ static int gmres_multi_dev(ifgf_plan* P, const zd* Bp, zd* Xp, int nrhs, int res
       upload_act();
       const int na = (int)act.size();
       zd* in = W.p + (size_t)R * n;
-      hipLaunchKernelGGL(k_gather_ms, dim3(nb, na), dim3(256), 0, st, n, Xp, n, 0, sysd.p, in);
+      hipLaunchKernelGGL(k_gather_ms<zd>, dim3(nb, na), dim3(256), 0, st, n, (const zd*)Xp, n, 0, sysd.p, in);
       int rc = apply_columns(P, in, n, W.p, na, st, fp64);
       if (rc) throw std::runtime_error(P->err);
       hipLaunchKernelGGL(k_residual_ms, dim3(nb, na), dim3(256), 0, st, n, Bp, sysd.p, W.p);
@@ -854,7 +868,7 @@ static int gmres_multi_dev(ifgf_plan* P, const zd* Bp, zd* Xp, int nrhs, int res
       upload_act();
       int na = (int)act.size();
       hipLaunchKernelGGL(k_norm_parts_ms, dim3(kDot, 1, na), dim3(256), 0, st, n, W.p, part.p, ps);
-      hipLaunchKernelGGL(k_normalize_ms, dim3(nb, na), dim3(256), 0, st, n, W.p, part.p, ps, V.p, vs, 0,
+      hipLaunchKernelGGL(k_normalize_ms<VT>, dim3(nb, na), dim3(256), 0, st, n, W.p, part.p, ps, V.p, vs, 0,
                          (zd*)nullptr, hs, sysd.p);
       for (int s : act) {
         std::fill(g[s].begin(), g[s].end(), C(0.0));
@@ -868,7 +882,7 @@ static int gmres_multi_dev(ifgf_plan* P, const zd* Bp, zd* Xp, int nrhs, int res
         upload_act();
         zd* in = W.p + (size_t)R * n;
         zd* Wa = W.p;
-        hipLaunchKernelGGL(k_gather_ms, dim3(nb, na), dim3(256), 0, st, n, V.p, vs, j, sysd.p, in);
+        hipLaunchKernelGGL(k_gather_ms<VT>, dim3(nb, na), dim3(256), 0, st, n, (const VT*)V.p, vs, j, sysd.p, in);
         int rc = apply_columns(P, in, n, Wa, na, st, false);
         if (rc) return rc;
         // Gram-Schmidt with selective re-orthogonalisation, as in arnoldi_enqueue
@@ -876,16 +890,16 @@ static int gmres_multi_dev(ifgf_plan* P, const zd* Bp, zd* Xp, int nrhs, int res
         zd* part2 = part.p + (size_t)(nc + 1) * kDot;
         double* pn1 = part_n.p;
         double* pn2 = part_n.p + nb;
-        hipLaunchKernelGGL(k_dots_ms, dim3(kDot, nc + 1, na), dim3(256), 0, st, n, V.p, vs, Wa, part.p, ps, sysd.p, nc,
+        hipLaunchKernelGGL(k_dots_ms<VT>, dim3(kDot, nc + 1, na), dim3(256), 0, st, n, (const VT*)V.p, vs, Wa, part.p, ps, sysd.p, nc,
                            (const int*)nullptr);
-        hipLaunchKernelGGL(k_project_out_ms, dim3(nb, na), dim3(256), 0, st, n, nc, V.p, vs, part.p, ps, H.p, hs, 0, Wa,
+        hipLaunchKernelGGL(k_project_out_ms<VT>, dim3(nb, na), dim3(256), 0, st, n, nc, (const VT*)V.p, vs, part.p, ps, H.p, hs, 0, Wa,
                            sysd.p, pn1, pns, (const int*)nullptr);
         hipLaunchKernelGGL(k_gate_ms, dim3(na), dim3(64), 0, st, nc, part.p, ps, pn1, pns, nb, gate.p);
-        hipLaunchKernelGGL(k_dots_ms, dim3(kDot, nc, na), dim3(256), 0, st, n, V.p, vs, Wa, part2, ps, sysd.p, nc,
+        hipLaunchKernelGGL(k_dots_ms<VT>, dim3(kDot, nc, na), dim3(256), 0, st, n, (const VT*)V.p, vs, Wa, part2, ps, sysd.p, nc,
                            (const int*)gate.p);
-        hipLaunchKernelGGL(k_project_out_ms, dim3(nb, na), dim3(256), 0, st, n, nc, V.p, vs, part2, ps, H.p, hs, 1, Wa,
+        hipLaunchKernelGGL(k_project_out_ms<VT>, dim3(nb, na), dim3(256), 0, st, n, nc, (const VT*)V.p, vs, part2, ps, H.p, hs, 1, Wa,
                            sysd.p, pn2, pns, (const int*)gate.p);
-        hipLaunchKernelGGL(k_arnoldi_end_ms, dim3(nb, na), dim3(256), 0, st, n, nc, Wa, (const int*)gate.p, pn1, pn2,
+        hipLaunchKernelGGL(k_arnoldi_end_ms<VT>, dim3(nb, na), dim3(256), 0, st, n, nc, Wa, (const int*)gate.p, pn1, pn2,
                            pns, V.p, vs, H.p, hs, sysd.p);
         HIP_TRY(hipMemcpyAsync(hh.data(), H.p, hh.size() * sizeof(zd), hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
@@ -909,7 +923,7 @@ static int gmres_multi_dev(ifgf_plan* P, const zd* Bp, zd* Xp, int nrhs, int res
           if (!done[s]) continue;
           y = back_substitute(Hm[s], m, g[s], cols[s]);
           HIP_TRY(hipMemcpyAsync(H.p + (size_t)s * hs, y.data(), cols[s] * sizeof(zd), hipMemcpyHostToDevice, st));
-          hipLaunchKernelGGL(k_update, dim3(nb), dim3(256), 0, st, n, cols[s], V.p + (size_t)s * vs, n,
+          hipLaunchKernelGGL(k_update<VT>, dim3(nb), dim3(256), 0, st, n, cols[s], (const VT*)V.p + (size_t)s * vs, n,
                              H.p + (size_t)s * hs, -1.0, Xp + (size_t)s * n);
         }
         act = still;
@@ -919,7 +933,7 @@ static int gmres_multi_dev(ifgf_plan* P, const zd* Bp, zd* Xp, int nrhs, int res
       for (int s : act) {
         y = back_substitute(Hm[s], m, g[s], cols[s]);
         HIP_TRY(hipMemcpyAsync(H.p + (size_t)s * hs, y.data(), cols[s] * sizeof(zd), hipMemcpyHostToDevice, st));
-        hipLaunchKernelGGL(k_update, dim3(nb), dim3(256), 0, st, n, cols[s], V.p + (size_t)s * vs, n,
+        hipLaunchKernelGGL(k_update<VT>, dim3(nb), dim3(256), 0, st, n, cols[s], (const VT*)V.p + (size_t)s * vs, n,
                            H.p + (size_t)s * hs, -1.0, Xp + (size_t)s * n);
       }
       if (!act.empty() && it < maxit) {
@@ -960,7 +974,7 @@ extern "C" int ifgf_gmres_multi(ifgf_plan* P, const double* rhs, double* x, int 
     X.alloc((size_t)n * nrhs);
     HIP_TRY(hipMemcpyAsync(B.p, rhs, (size_t)n * nrhs * sizeof(zd), hipMemcpyHostToDevice, st));
     HIP_TRY(hipMemcpyAsync(X.p, x, (size_t)n * nrhs * sizeof(zd), hipMemcpyHostToDevice, st));
-    const int rc = gmres_multi_dev(P, B.p, X.p, nrhs, restart, tol, maxit, iters, relres, true);
+    const int rc = gmres_multi_dev<zd>(P, B.p, X.p, nrhs, restart, tol, maxit, iters, relres, true);
     if (rc != IFGF_OK && rc != IFGF_ERR_NOT_CONVERGED) return rc;
     HIP_TRY(hipMemcpyAsync(x, X.p, (size_t)n * nrhs * sizeof(zd), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
@@ -1023,7 +1037,7 @@ extern "C" int ifgf_refine_multi(ifgf_plan* P, const double* rhs, double* x, int
     auto residuals = [&]() {
       const int na = (int)act.size();
       HIP_TRY(hipMemcpyAsync(sysd.p, act.data(), na * sizeof(int), hipMemcpyHostToDevice, st));
-      hipLaunchKernelGGL(k_gather_ms, dim3(nb, na), dim3(256), 0, st, n, X.p, n, 0, sysd.p, G.p);
+      hipLaunchKernelGGL(k_gather_ms<zd>, dim3(nb, na), dim3(256), 0, st, n, (const zd*)X.p, n, 0, sysd.p, G.p);
       const int rc = apply_columns(P, G.p, n, W.p, na, st, true);
       if (rc) throw std::runtime_error(P->err);
       hipLaunchKernelGGL(k_residual_ms, dim3(nb, na), dim3(256), 0, st, n, B.p, sysd.p, W.p);
@@ -1055,7 +1069,9 @@ extern "C" int ifgf_refine_multi(ifgf_plan* P, const double* rhs, double* x, int
       for (int q : act) emax = std::max(emax, relres[q]);
       const double tau = std::min(0.5, std::max(inner_floor, tol / emax));
       HIP_TRY(hipMemsetAsync(D.p, 0, (size_t)n * na * sizeof(zd), st));
-      const int rc = gmres_multi_dev(P, W.p, D.p, na, restart, tau, maxit, it.data(), rr.data(), false);
+      // corrections with the plan's default operator; an FP32 basis with the FP32 operator
+      const int rc = P->mixed ? gmres_multi_dev<zf>(P, W.p, D.p, na, restart, tau, maxit, it.data(), rr.data(), false)
+                              : gmres_multi_dev<zd>(P, W.p, D.p, na, restart, tau, maxit, it.data(), rr.data(), false);
       if (rc != IFGF_OK && rc != IFGF_ERR_NOT_CONVERGED) return rc;
       k++;
       HIP_TRY(hipMemcpyAsync(sysd.p, act.data(), na * sizeof(int), hipMemcpyHostToDevice, st));

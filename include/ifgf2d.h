@@ -317,7 +317,7 @@ int ifgf_gmres_multi(ifgf_plan* plan, const double* rhs, double* x, int nrhs, in
 /* Mixed-precision refinement for many right-hand sides (no reference
  * counterpart): x <- x + d, d the lockstep GMRES(restart) solution of
  * A d = b - A x (ifgf_gmres_multi, x0 = 0, the plan's default operator: mixed
- * precision on an IFGF_MIXED plan) to relative tolerance
+ * precision, with an FP32 Krylov basis, on an IFGF_MIXED plan) to relative tolerance
  * min(0.5, max(inner_floor, tol / e)), e the largest true relative residual of
  * the systems not yet converged; the residuals use the all-FP64 operator, one
  * column at a time. A system stops once its true FP64 relative residual is
