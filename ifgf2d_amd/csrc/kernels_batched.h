@@ -450,22 +450,38 @@ __global__ void __launch_bounds__(256)
   const double sgn = (c == 1 && im) ? -1.0 : 1.0;
   const double* __restrict__ x = reinterpret_cast<const double*>(rho) + r * 2 + part;
   const double* __restrict__ a = A + G.aoff + (long long)U.rt * Q * 64 + lane;
-  // 8 K chunks per round: their loads first, then the products, on two
-  // accumulator chains (added at the end)
+  // the group's segment table in registers (uniform)
+  int se[kMaxNb], sh[kMaxNb];
+#pragma unroll
+  for (int q = 0; q < kMaxNb; q++) {
+    se[q] = sg[2 * q];
+    sh[q] = sg[2 * q + 1];
+  }
+  // 8 K chunks per round: their loads first (unconditional, at indices
+  // clamped into range, so that no load waits on another), then the
+  // products on two accumulator chains (added at the end); chunks past the
+  // wave's range and sources past the group's take 0
   constexpr int kU = 8;
   dbl4 acc = {0.0, 0.0, 0.0, 0.0}, acc2 = {0.0, 0.0, 0.0, 0.0};
   for (int qb = q0; qb < q1; qb += kU) {
     double av[kU], bv[kU];
+    bool ok[kU];
 #pragma unroll
     for (int t = 0; t < kU; t++) {
-      const int q = qb + t;
-      av[t] = 0.0;
-      bv[t] = 0.0;
-      if (q < q1) {
-        av[t] = a[(long long)q * 64];
-        const int o = 2 * q + kk / 2;
-        if (o < G.nsrc) bv[t] = sgn * x[(size_t)seg_source(sg, o) * 16];
-      }
+      const int q = qb + t < q1 ? qb + t : q1 - 1;
+      const int o = 2 * q + kk / 2;
+      ok[t] = qb + t < q1 && o < G.nsrc;
+      const int oc = o < G.nsrc ? o : G.nsrc - 1;
+      int src = oc + sh[0];
+#pragma unroll
+      for (int k = 1; k < kMaxNb; k++) src = (oc >= se[k - 1]) ? oc + sh[k] : src;
+      av[t] = a[(long long)q * 64];
+      bv[t] = x[(size_t)src * 16];
+    }
+#pragma unroll
+    for (int t = 0; t < kU; t++) {
+      av[t] = qb + t < q1 ? av[t] : 0.0;
+      bv[t] = ok[t] ? sgn * bv[t] : 0.0;
     }
 #pragma unroll
     for (int t = 0; t < kU; t += 2) {
@@ -513,16 +529,21 @@ __global__ void __launch_bounds__(256)
   V4 acc = {0, 0, 0, 0}, acc2 = {0, 0, 0, 0};
   for (int qb = 0; qb < Q; qb += kU) {
     RT av[kU], bv[kU];
+    bool ok[kU];
+    double xv[kU];
+    // unconditional loads at clamped indices, then the zeros (as k_near_mfma)
 #pragma unroll
     for (int t = 0; t < kU; t++) {
-      const int q = qb + t;
-      av[t] = 0;
-      bv[t] = 0;
-      if (q < Q) {
-        av[t] = a[(long long)q * 64];
-        const int o = 2 * q + kk / 2;
-        if (o < G.nsrc) bv[t] = sgn * (RT)x[(size_t)(G.seg + o) * 16];
-      }
+      const int q = qb + t < Q ? qb + t : Q - 1;
+      const int o = 2 * q + kk / 2;
+      ok[t] = qb + t < Q && o < G.nsrc;
+      av[t] = a[(long long)q * 64];
+      xv[t] = x[(size_t)(G.seg + (o < G.nsrc ? o : G.nsrc - 1)) * 16];
+    }
+#pragma unroll
+    for (int t = 0; t < kU; t++) {
+      av[t] = qb + t < Q ? av[t] : RT(0);
+      bv[t] = ok[t] ? sgn * (RT)xv[t] : RT(0);
     }
 #pragma unroll
     for (int t = 0; t < kU; t += 2) {

@@ -232,6 +232,26 @@ def test_batched_columns_equal_single_applications(precision):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_mfma_blocks_match_valu_blocks(precision):
+    """8-column blocks with the near field and leaf operator on the matrix
+    cores (default) against the VALU batched kernels (mfma=False): the same
+    columns to rounding (1e-14 FP64, 1e-6 mixed), both within the same bar of
+    single applications, both bitwise repeatable; 13 columns (one 8-column
+    block, then 4 and 1)."""
+    kw = configs.scaled(I.kite(), 600, 50.0, 7)
+    fm = I.ForwardMap(**kw, precision=precision, nrhs_max=8)
+    fv = I.ForwardMap(**kw, precision=precision, nrhs_max=8, mfma=False)
+    X = np.stack([density(fm.N, 70 + s) for s in range(13)], axis=1)
+    Ym, Yv = fm.compute_Ax_acc(X), fv.compute_Ax_acc(X)
+    tol = BATCH_TOL if precision == "fp64" else 1e-6
+    e = max(rel(Ym[:, j], Yv[:, j]) for j in range(13))
+    es = max(rel(Ym[:, j], fm.compute_Ax_acc(X[:, j])) for j in (0, 5, 7, 8, 12))
+    print(f"{precision}: MFMA vs VALU blocks {e:.3e}, MFMA blocks vs single {es:.3e}", flush=True)
+    assert e <= tol and es <= tol
+    assert np.array_equal(Ym, fm.compute_Ax_acc(X)) and np.array_equal(Yv, fv.compute_Ax_acc(X))
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_two_lane_column_blocks(precision):
     """A plan for 32 columns keeps a second set of 8-column buffers: device
     calls with two or more 8-column blocks alternate them between the
