@@ -928,22 +928,28 @@ __device__ inline void finalize_body(int lb, int N, int Np, double ident, const 
     const bool own = stage != 2;
     const long long e0 = own && corr_ptr ? corr_ptr[i] : 0, e1 = own && corr_ptr ? corr_ptr[i + 1] : 0;
     const long long nitem = (e1 - e0) * Np;
-    for (long long ib = sl; ib < nitem; ib += (long long)S * kFinU) {
-      zd wv[kFinU];
+    // (the loads are unconditional, at indices clamped into the lane's
+    // range, and the entries past it are skipped in the sums: a load inside a
+    // conditional would have to complete where the branches meet)
+    // (a target's items are few: 32-bit indices within its rows)
+    const int ni = (int)nitem;
+    for (int ib = sl; ib < ni; ib += S * kFinU) {
+      zd wv[kFinU], xv[kFinU];
       long long src[kFinU];
 #pragma unroll
       for (int u = 0; u < kFinU; u++) {
-        const long long it = ib + (long long)u * S;
-        if (it < nitem) {
-          const long long e = e0 + it / Np;
-          const int q = (int)(it % Np);
-          wv[u] = corr_w[e * Np + q];
-          src[u] = (long long)corr_patch[e] * Np + q;
-        }
+        const int it0 = ib + u * S;
+        const int it = it0 < ni ? it0 : ni - 1;
+        const long long e = e0 + it / Np;
+        const int q = it % Np;
+        wv[u] = corr_w[e * Np + q];
+        src[u] = (long long)corr_patch[e] * Np + q;
       }
 #pragma unroll
+      for (int u = 0; u < kFinU; u++) xv[u] = x[src[u]];
+#pragma unroll
       for (int u = 0; u < kFinU; u++)
-        if (ib + (long long)u * S < nitem) cfma(acc, wv[u], x[src[u]]);
+        if (ib + u * S < ni) cfma(acc, wv[u], xv[u]);
     }
     if (!own) {
     } else if (npart) {
@@ -953,7 +959,7 @@ __device__ inline void finalize_body(int lb, int N, int Np, double ident, const 
         zd pv[kFinU];
 #pragma unroll
         for (int u = 0; u < kFinU; u++)
-          if (wb + u * S < w1) pv[u] = npart[(size_t)(wb + u * S) * kNearGroup + t];
+          pv[u] = npart[(size_t)(wb + u * S < w1 ? wb + u * S : w1 - 1) * kNearGroup + t];
 #pragma unroll
         for (int u = 0; u < kFinU; u++)
           if (wb + u * S < w1) near = cadd(near, pv[u]);
@@ -966,7 +972,10 @@ __device__ inline void finalize_body(int lb, int N, int Np, double ident, const 
       for (long long fb = it_ptr[si] + sl; fb < f1; fb += (long long)S * kFinU) {
         int idx[kFinU];
 #pragma unroll
-        for (int u = 0; u < kFinU; u++) idx[u] = fb + (long long)u * S < f1 ? it_slot[fb + (long long)u * S] : 0;
+        for (int u = 0; u < kFinU; u++) {
+          const long long f = fb + (long long)u * S;
+          idx[u] = it_slot[f < f1 ? f : f1 - 1];
+        }
         ZI sv[kFinU];
 #pragma unroll
         for (int u = 0; u < kFinU; u++) sv[u] = slots[idx[u]];
