@@ -1305,13 +1305,13 @@ static void autotune(ifgf_plan* P) {
   hipEvent_t e0, e1;
   HIP_TRY(hipEventCreate(&e0));
   HIP_TRY(hipEventCreate(&e1));
-  // the fastest of three rounds of five applications, after two warm-up ones
+  // the fastest of three rounds of ten applications, after two warm-up ones
   auto time_apply = [&]() {
     for (int w = 0; w < 2; w++) dispatch(P, P->xbuf.p, P->ybuf.p, st);
     float best = 1e30f;
     for (int round = 0; round < 3; round++) {
       HIP_TRY(hipEventRecord(e0, st));
-      for (int r = 0; r < 5; r++) dispatch(P, P->xbuf.p, P->ybuf.p, st);
+      for (int r = 0; r < 10; r++) dispatch(P, P->xbuf.p, P->ybuf.p, st);
       HIP_TRY(hipEventRecord(e1, st));
       HIP_TRY(hipEventSynchronize(e1));
       float ms = 0.f;
@@ -1321,8 +1321,9 @@ static void autotune(ifgf_plan* P) {
     return best;
   };
   // Two interleaved sweeps over a knob's candidates, the faster time of each
-  // kept; a candidate replaces the knob's default only when it is faster by
-  // more than 2% (near-ties otherwise flip with timing noise between runs).
+  // kept; starting from the knob's default, a candidate (in list order)
+  // replaces the current choice only when it is more than 1% faster than
+  // it (near-ties otherwise flip with timing noise between runs).
   auto pick = [&](int def, const std::vector<int>& cand, const std::function<void(int)>& set) {
     std::vector<float> tm(cand.size(), 1e30f);
     for (int sweep = 0; sweep < 2; sweep++)
@@ -1336,7 +1337,7 @@ static void autotune(ifgf_plan* P) {
     int best = def;
     float tb = tdef;
     for (size_t q = 0; q < cand.size(); q++)
-      if (tm[q] < 0.98f * tdef && tm[q] < tb) {
+      if (cand[q] != def && tm[q] < 0.99f * tb) {
         tb = tm[q];
         best = cand[q];
       }
