@@ -180,13 +180,26 @@ __global__ void __launch_bounds__(256) k_level_push_b(PushArgsT<RT> a) {
   // cone inputs: column r of node j of cone cl at buf[(cl * P + j) * R + r]
   if (act) {
     if (a.leaf) {
+      // items outer, columns inner: kRC of an item's values load together
+      // (each column still sums its items in item order); FP64 takes the
+      // columns four at a time to stay within 3 waves' registers
+      constexpr int kRC = sizeof(Z) == 16 && R > 4 ? 4 : R;
       const int c = (int)(g - a.leaf_cone_off);
-      const int w1 = a.lio[c + 1];
+      const int w0 = a.lio[c], w1 = a.lio[c + 1];
 #pragma unroll
-      for (int r = 0; r < R; r++) {
-        Z v{0, 0};
-        for (int w = a.lio[c]; w < w1; w++) v = zadd(v, a.lpart[((size_t)w * P + j) * R + r]);
-        buf[(cl * P + j) * R + r] = v;
+      for (int r0 = 0; r0 < R; r0 += kRC) {
+        Z v[kRC];
+#pragma unroll
+        for (int r = 0; r < kRC; r++) v[r] = Z{0, 0};
+        for (int w = w0; w < w1; w++) {
+          Z t[kRC];
+#pragma unroll
+          for (int r = 0; r < kRC; r++) t[r] = a.lpart[((size_t)w * P + j) * R + r0 + r];
+#pragma unroll
+          for (int r = 0; r < kRC; r++) v[r] = zadd(v[r], t[r]);
+        }
+#pragma unroll
+        for (int r = 0; r < kRC; r++) buf[(cl * P + j) * R + r0 + r] = v[r];
       }
     } else {
       const int rs = a.rs;
