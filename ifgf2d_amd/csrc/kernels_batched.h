@@ -22,8 +22,9 @@ namespace ifgf {
 // Cones per batched push task: two LDS windows of CB x P x R complex values
 // within 40 KB, and at most 256 threads in the cone phase.
 // entries per batched push task with unit evaluation (cones of <= 35 nodes)
-constexpr int kBatchedEntries = 1024;
-constexpr int batched_entries(int P) { return P <= 35 ? kBatchedEntries : 256; }
+// (FP64: 512, so that the staged records, 32 B each, fit the transform's
+// LDS window and three blocks fit a CU; FP32: 1,024)
+constexpr int batched_entries(int P, int zbytes) { return P <= 35 ? (zbytes == 16 ? 512 : 1024) : 256; }
 
 constexpr int batched_cones(int P, int R, int zbytes) {
   const int c = 40960 / (2 * P * R * zbytes);
@@ -139,7 +140,7 @@ __global__ void __launch_bounds__(256)
 }
 
 // k_level_push with R columns. Tasks hold at most CB cones (their R-column
-// coefficients fit the LDS windows) and batched_entries(P) entries;
+// coefficients fit the LDS windows) and batched_entries(P, sizeof(Z)) entries;
 // the cone phase runs on threads (cone, node) < CB * P. The evaluation runs
 // on units: every cone's run of entries is cut into pieces of at most K
 // entries (K chosen per task so that all pieces fit 256 / R units), and the
@@ -160,7 +161,7 @@ __global__ void __launch_bounds__(256) k_level_push_b(PushArgsT<RT> a) {
   constexpr int kUnits = 256 / R;
   static_assert(!kUnitEval || kUnits > CB, "every cone needs a unit");
   // tmp is free after the transform and then holds the task's records
-  constexpr int kRec = kUnitEval ? kBatchedEntries / 256 : 1;  // records per thread
+  constexpr int kRec = batched_entries(P, (int)sizeof(Z)) / 256;  // records per thread
   constexpr int kGeoZ = (int)(256 * kRec * sizeof(Geo4T<RT>) / sizeof(Z));
   constexpr int kTmp = CB * P * R > kGeoZ ? CB * P * R : kGeoZ;
   __shared__ ChebTablesT<RT> T;

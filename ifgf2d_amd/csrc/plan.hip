@@ -680,7 +680,7 @@ static void build_batched(ifgf_plan* P) {
     const int r = 1 << lr;
     if (r > P->rb_max) break;
     std::vector<PushTask> tasks;
-    make_push_tasks(P, batched_entries(Pn), batched_cones(Pn, r, zb), tasks, P->blevel_off[lr]);
+    make_push_tasks(P, batched_entries(Pn, zb), batched_cones(Pn, r, zb), tasks, P->blevel_off[lr]);
     P->btasks[lr].upload(tasks);
   }
   P->stored_bytes += (long long)(P->npart_b.n + P->lpart_b.n + P->pbuf_b.n) * (long long)sizeof(zd) +
