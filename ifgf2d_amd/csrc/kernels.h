@@ -376,6 +376,12 @@ __global__ void __launch_bounds__(256)
 // densities), so the stream is not held up by the longest rows.
 constexpr int kNearItemChunks = 8;
 
+// kPaced: the form that runs beside the level pushes on the second stream.
+// Issuing all of a lane's chunk loads at once made the near stage 41.0 ->
+// 38.8 us on the 64k kite but the whole application 1-2% slower there (the
+// near field then takes more of the bandwidth the pushes' load phases need);
+// on the critical path (one stream, obstacle groups) the unpaced form runs.
+template <bool kPaced>
 __device__ inline void near_items_body(int lb, int N, int nitems, const NearItem* __restrict__ items,
                                        const int4* __restrict__ nf_seg, const zd* __restrict__ nf_val,
                                        const zd* __restrict__ rho, zd* __restrict__ npart) {
@@ -388,23 +394,46 @@ __device__ inline void near_items_body(int lb, int N, int nitems, const NearItem
     const int i = it.g * kNearGroup + t;
     if (i < N) {
       const dbl2* __restrict__ v = reinterpret_cast<const dbl2*>(nf_val) + it.off + t * kNearChunk + sl;
-      dbl2 val[kNearItemChunks];
-#pragma unroll
-      for (int u = 0; u < kNearItemChunks; u++)
-        if (u < it.nc) val[u] = v[u * kNearGroup * kNearChunk];
       const int4* sg = nf_seg + (size_t)i * (kNbSegInts / 4);
       const int4 q0 = sg[0], q1 = sg[1], q2 = sg[2], q3 = sg[3], q4 = sg[4];
       const int end[kMaxNb - 1] = {q0.x, q0.z, q1.x, q1.z, q2.x, q2.z, q3.x, q3.z};
       const int sh[kMaxNb] = {q0.y, q0.w, q1.y, q1.w, q2.y, q2.w, q3.y, q3.w, q4.y};
       const int len = q4.x;
+      dbl2 val[kNearItemChunks];
+      if constexpr (!kPaced) {
+        // every load unconditional (chunks past the item read its first one,
+        // sources past the target's segment read rho[0]) and the entries past
+        // the ranges skipped in the sum: all chunks' loads in flight at once
+        bool ok[kNearItemChunks];
+        zd rv[kNearItemChunks];
 #pragma unroll
-      for (int u = 0; u < kNearItemChunks; u++) {
-        const int o = (it.u0 + u) * kNearChunk + sl;
-        if (u < it.nc && o < len) {
+        for (int u = 0; u < kNearItemChunks; u++) {
+          val[u] = v[(u < it.nc ? u : 0) * kNearGroup * kNearChunk];
+          const int o = (it.u0 + u) * kNearChunk + sl;
+          ok[u] = u < it.nc && o < len;
           int src = o + sh[0];
 #pragma unroll
           for (int q = 1; q < kMaxNb; q++) src = (o >= end[q - 1]) ? o + sh[q] : src;
-          cfma(acc, zd{val[u].x, val[u].y}, rho[src]);
+          rv[u] = rho[ok[u] ? src : 0];
+        }
+#pragma unroll
+        for (int u = 0; u < kNearItemChunks; u++)
+          if (ok[u]) cfma(acc, zd{val[u].x, val[u].y}, rv[u]);
+      } else {
+        // loads inside the conditionals: each chunk's density waits for the
+        // one before (the same sums in the same order)
+#pragma unroll
+        for (int u = 0; u < kNearItemChunks; u++)
+          if (u < it.nc) val[u] = v[u * kNearGroup * kNearChunk];
+#pragma unroll
+        for (int u = 0; u < kNearItemChunks; u++) {
+          const int o = (it.u0 + u) * kNearChunk + sl;
+          if (u < it.nc && o < len) {
+            int src = o + sh[0];
+#pragma unroll
+            for (int q = 1; q < kMaxNb; q++) src = (o >= end[q - 1]) ? o + sh[q] : src;
+            cfma(acc, zd{val[u].x, val[u].y}, rho[src]);
+          }
         }
       }
     }
@@ -412,10 +441,11 @@ __device__ inline void near_items_body(int lb, int N, int nitems, const NearItem
   acc = lane_group_sum<kNearChunk>(acc);
   if (w < nitems && sl == 0) npart[(size_t)w * kNearGroup + t] = acc;
 }
+template <bool kPaced>
 __global__ void __launch_bounds__(256)
     k_near_items(int N, int nitems, const NearItem* __restrict__ items, const int4* __restrict__ nf_seg,
                  const zd* __restrict__ nf_val, const zd* __restrict__ rho, zd* __restrict__ npart) {
-  near_items_body(xcd_block(), N, nitems, items, nf_seg, nf_val, rho, npart);
+  near_items_body<kPaced>(xcd_block(), N, nitems, items, nf_seg, nf_val, rho, npart);
 }
 
 // Leaf operator L_c (P x n_src per leaf cone c). Storage: the sources of a

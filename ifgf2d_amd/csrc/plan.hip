@@ -313,11 +313,16 @@ static hipEvent_t* take_events(ifgf_plan* P) {
   return ev;
 }
 
-static void launch_near_stored(ifgf_plan* P, hipStream_t st) {
+// paced: the near field beside the level pushes (second stream)
+static void launch_near_stored(ifgf_plan* P, hipStream_t st, bool paced = false) {
   const int N = (int)P->tree.M, M = (int)P->near_items.n;
   if (M == 0) return;  // a field whose targets have no source in their leaf neighbourhood
-  hipLaunchKernelGGL(k_near_items, dim3((M + 3) / 4), dim3(256), 0, st, N, M, P->near_items.p, P->nf_seg.p,
-                     P->nf_val.p, P->rho.p, P->npart.p);
+  if (paced)
+    hipLaunchKernelGGL(k_near_items<true>, dim3((M + 3) / 4), dim3(256), 0, st, N, M, P->near_items.p, P->nf_seg.p,
+                       P->nf_val.p, P->rho.p, P->npart.p);
+  else
+    hipLaunchKernelGGL(k_near_items<false>, dim3((M + 3) / 4), dim3(256), 0, st, N, M, P->near_items.p, P->nf_seg.p,
+                       P->nf_val.p, P->rho.p, P->npart.p);
 }
 
 template <int Ps, int Pang, int EPT, typename R>
@@ -440,7 +445,7 @@ static void launch_ifgf(ifgf_plan* P, const zd* x, zd* y, hipStream_t st) {
   auto do_fork = [&]() {
     HIP_TRY(hipEventRecord(P->ev_fork, st));
     HIP_TRY(hipStreamWaitEvent(P->aux, P->ev_fork, 0));
-    launch_near_stored(P, P->aux);
+    launch_near_stored(P, P->aux, true);
     finalize(1, P->aux);
     HIP_TRY(hipEventRecord(P->ev_join, P->aux));
   };
@@ -1614,7 +1619,7 @@ __global__ void __launch_bounds__(256) k_group_near(const NearArgs* __restrict__
                                                     int n) {
   const int b = xcd_block(), i = group_plan(boff, n, b);
   const NearArgs A = a[i];
-  near_items_body(b - boff[i], A.N, A.nitems, A.items, A.nf_seg, A.nf_val, A.rho, A.npart);
+  near_items_body<false>(b - boff[i], A.N, A.nitems, A.items, A.nf_seg, A.nf_val, A.rho, A.npart);
 }
 template <int P, typename Z>
 __global__ void __launch_bounds__(256) k_group_leaf(const LeafArgs<Z>* __restrict__ a, const int* __restrict__ boff,
