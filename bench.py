@@ -105,7 +105,7 @@ def cpu_baseline(kw, x, name):
     return rec, y
 
 
-def north_star_block(I, configs, np, log, reps=1):
+def star_directions_block(I, configs, np, log, reps=1):
     """The headline many-right-hand-side comparison on the 256,000-point star at
     k = 500 (C3 geometry, C4 directions): 64 plane waves 2 pi j / 64 solved to
     the FP64 true residual 1e-8 (a) by 64 single FP64 GMRES(50) solves, (b) by
@@ -132,12 +132,12 @@ def north_star_block(I, configs, np, log, reps=1):
         singles.append(d_)
         its1.append(it_)
     t_single = time.perf_counter() - t0
-    log(f"north star: 64 single FP64 solves {t_single:.3f} s (iterations {min(its1)}..{max(its1)})")
+    log(f"64 star directions: 64 single FP64 solves {t_single:.3f} s (iterations {min(its1)}..{max(its1)})")
     far = [f64.propagate_unacc(tx, ty, d_) for d_ in singles]
     (X64, its, rr), _, t_lock = timed_solve(lambda: f64.solve_multi(B, restart=50, tol=1e-8, maxit=2000), reps=reps)
     e_lock = max(float(np.linalg.norm(f64.propagate_unacc(tx, ty, X64[:, j]) - far[j]) / np.linalg.norm(far[j]))
                  for j in range(64))
-    log(f"north star: FP64 lockstep {t_lock} s, max true relres {rr.max():.2e}, far field vs single {e_lock:.1e}")
+    log(f"64 star directions: FP64 lockstep {t_lock} s, max true relres {rr.max():.2e}, far field vs single {e_lock:.1e}")
     del f64
     t0 = time.time()
     fmx = I.ForwardMap(**dict(kw, precision="mixed"), autotune=True, device_setup=True)
@@ -146,7 +146,7 @@ def north_star_block(I, configs, np, log, reps=1):
         lambda: fmx.solve_multi_refined(B, restart=50, tol=1e-8, maxit=2000), reps=reps)
     e_mix = max(float(np.linalg.norm(fmx.propagate_unacc(tx, ty, Xm[:, j]) - far[j]) / np.linalg.norm(far[j]))
                 for j in range(64))
-    log(f"north star: mixed refinement {t_ref} s, {outer} steps, max true relres {rrm.max():.2e}, "
+    log(f"64 star directions: mixed refinement {t_ref} s, {outer} steps, max true relres {rrm.max():.2e}, "
         f"far field vs FP64 {e_mix:.1e}")
     del fmx
     return {"workload": "star r=1+0.3cos5t, k=500, N=256000, 64 plane waves 2 pi j/64, CFIE 5x5, tol 1e-8",
@@ -172,7 +172,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-group", action="store_true")
     ap.add_argument("--no-field", action="store_true")
-    ap.add_argument("--no-north-star", action="store_true",
+    ap.add_argument("--no-star-directions", action="store_true",
                     help="skip the 64-direction comparison on the 256k star")
     ap.add_argument("--workload", choices=("kite", "star"), default="kite",
                     help="kite: the 64k kite at k=200 (default); star: the 256k star r=1+0.3cos5t at k=500")
@@ -189,8 +189,8 @@ def main():
         log("copy calibration: 20 copies of 268435456 bytes")
         return
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the forward map is "
-              "not sharded across GPUs in this build", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the forward map "
+              "runs on a single GPU (no sharding across GPUs)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
@@ -412,9 +412,9 @@ def main():
             del fb
         del Xb, Yb
 
-    north = None
-    if not args.no_north_star and not args.no_solve:
-        north = north_star_block(I, configs, np, log)
+    star_dirs = None
+    if not args.no_star_directions and not args.no_solve:
+        star_dirs = star_directions_block(I, configs, np, log)
 
     # sixteen seeded random star obstacles at k=100 (~10 nodes per wavelength):
     # their forward maps one after another against one captured group launch
@@ -558,7 +558,7 @@ def main():
         "solve": solve,
         "mixed_precision": mixed,
         "batched_rhs": batched,
-        "north_star": north,
+        "star_64_directions": star_dirs,
         "obstacle_group": group,
         "field": field,
         "roofline": roofline,
