@@ -138,7 +138,8 @@ def star_directions_block(I, configs, np, log, reps=1):
     t_single = time.perf_counter() - t0
     log(f"64 star directions: 64 single FP64 solves {t_single:.3f} s (iterations {min(its1)}..{max(its1)})")
     far = [f64.propagate_unacc(tx, ty, d_) for d_ in singles]
-    (X64, its, rr), _, t_lock = timed_solve(lambda: f64.solve_multi(B, restart=50, tol=1e-8, maxit=2000), reps=reps)
+    (X64, its, rr), _, t_lock = timed_solve(lambda: f64.solve_multi(B, restart=50, tol=1e-8, maxit=2000), reps=reps,
+                                            settle=1)
     e_lock = max(float(np.linalg.norm(f64.propagate_unacc(tx, ty, X64[:, j]) - far[j]) / np.linalg.norm(far[j]))
                  for j in range(64))
     log(f"64 star directions: FP64 lockstep {t_lock} s, max true relres {rr.max():.2e}, far field vs single {e_lock:.1e}")
@@ -147,7 +148,7 @@ def star_directions_block(I, configs, np, log, reps=1):
     fmx = I.ForwardMap(**dict(kw, precision="mixed"), autotune=True, device_setup=True)
     setupm = time.time() - t0
     (Xm, outer, itm, rrm), _, t_ref = timed_solve(
-        lambda: fmx.solve_multi_refined(B, restart=50, tol=1e-8, maxit=2000), reps=reps)
+        lambda: fmx.solve_multi_refined(B, restart=50, tol=1e-8, maxit=2000), reps=reps, settle=1)
     e_mix = max(float(np.linalg.norm(fmx.propagate_unacc(tx, ty, Xm[:, j]) - far[j]) / np.linalg.norm(far[j]))
                 for j in range(64))
     log(f"64 star directions: mixed refinement {t_ref} s, {outer} steps, max true relres {rrm.max():.2e}, "
