@@ -572,7 +572,9 @@ static void launch_batched(ifgf_plan* P, const zd* x, long long ld, zd* y, hipSt
                       lio, lpart, t.lv[D].cone_off, P->ifac_off.p, P->ifac_tab.p, P->tabs.p, pbuf, 0,
                       P->rb_max};
   }
-  const std::vector<int>& lo = P->blevel_off[LR];
+  // FP64 blocks on a mixed plan take their own task list
+  const bool own64 = !f32 && P->mixed;
+  const std::vector<int>& lo = own64 ? P->blevel_off64[LR] : P->blevel_off[LR];
   const int fork_li = P->overlap_b == 2 ? 1 : 0;
   for (int li = 0; li + 1 < D; li++) {
     if (fork && li == fork_li) {
@@ -584,7 +586,7 @@ static void launch_batched(ifgf_plan* P, const zd* x, long long ld, zd* y, hipSt
     }
     const int t0 = lo[li], nt = lo[li + 1] - t0;
     if (nt == 0) continue;
-    a.tasks = P->btasks[LR].p + t0;
+    a.tasks = (own64 ? P->btasks64[LR].p : P->btasks[LR].p) + t0;
     a.leaf = li == 0;
     hipLaunchKernelGGL((k_level_push_b<Ps, Pang, R, CB, RT>), dim3(nt), dim3(256), 0, st, a);
   }
@@ -666,10 +668,13 @@ static void build_batched(ifgf_plan* P) {
     P->lpart_b32.alloc(P->lpart.n * R);
     P->pbuf_b32.alloc((size_t)P->nslots * R);
     HIP_TRY(hipMemsetAsync(P->pbuf_b32.p, 0, P->pbuf_b32.n * sizeof(zf), P->stream));
-  } else {
-    P->lpart_b.alloc(P->lpart.n * R);
-    P->pbuf_b.alloc((size_t)P->nslots * R);
-    HIP_TRY(hipMemsetAsync(P->pbuf_b.p, 0, P->pbuf_b.n * sizeof(zd), P->stream));
+  }
+  // the FP64 blocks (on a mixed plan too: its IFGF_APPLY_FP64 columns, e.g. the
+  // refinement's residuals)
+  P->lpart_b.alloc(P->lpart.n * R);
+  P->pbuf_b.alloc((size_t)P->nslots * R);
+  HIP_TRY(hipMemsetAsync(P->pbuf_b.p, 0, P->pbuf_b.n * sizeof(zd), P->stream));
+  {
     const long long npush = P->nslots - 1;
     if (P->push_geo_b.n != (size_t)npush) {  // a field with batched blocks
       P->push_geo_b.alloc(npush);
@@ -687,6 +692,11 @@ static void build_batched(ifgf_plan* P) {
     std::vector<PushTask> tasks;
     make_push_tasks(P, batched_entries(Pn, zb), batched_cones(Pn, r, zb), tasks, P->blevel_off[lr]);
     P->btasks[lr].upload(tasks);
+    if (P->mixed) {
+      const int z64 = (int)sizeof(zd);
+      make_push_tasks(P, batched_entries(Pn, z64), batched_cones(Pn, r, z64), tasks, P->blevel_off64[lr]);
+      P->btasks64[lr].upload(tasks);
+    }
   }
   P->stored_bytes += (long long)(P->npart_b.n + P->lpart_b.n + P->pbuf_b.n) * (long long)sizeof(zd) +
                      (long long)(P->lpart_b32.n + P->pbuf_b32.n) * (long long)sizeof(zf);
@@ -727,14 +737,13 @@ static void build_batched(ifgf_plan* P) {
                            P->lm_grp.p, P->lm_cone0.p, P->lf_off.p, P->lf_val32.p, P->lm_A32.p);
       P->lcoef8_32.alloc((size_t)LD.ncone * Pn * 8);
       P->stored_bytes += aoff * (long long)sizeof(float);
-    } else {
-      P->lm_A.alloc(aoff);
-      if (aoff > 0)
-        hipLaunchKernelGGL((k_leaf_mfma_build<Pn, zd>), dim3(nb), dim3(256), 0, P->stream, aoff, (int)grp.size(),
-                           P->lm_grp.p, P->lm_cone0.p, P->lf_off.p, P->lf_val.p, P->lm_A.p);
-      P->lcoef8.alloc((size_t)LD.ncone * Pn * 8);
-      P->stored_bytes += aoff * (long long)sizeof(double);
     }
+    P->lm_A.alloc(aoff);
+    if (aoff > 0)
+      hipLaunchKernelGGL((k_leaf_mfma_build<Pn, zd>), dim3(nb), dim3(256), 0, P->stream, aoff, (int)grp.size(),
+                         P->lm_grp.p, P->lm_cone0.p, P->lf_off.p, P->lf_val.p, P->lm_A.p);
+    P->lcoef8.alloc((size_t)LD.ncone * Pn * 8);
+    P->stored_bytes += aoff * (long long)sizeof(double);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(P->stream));
     P->mfma8 = true;
@@ -752,12 +761,11 @@ static void build_batched(ifgf_plan* P) {
       L.pbuf_b32.alloc(P->pbuf_b32.n);
       HIP_TRY(hipMemsetAsync(L.pbuf_b32.p, 0, L.pbuf_b32.n * sizeof(zf), P->stream));
       if (P->lcoef8_32.n) L.lcoef8_32.alloc(P->lcoef8_32.n);
-    } else {
-      L.lpart_b.alloc(P->lpart_b.n);
-      L.pbuf_b.alloc(P->pbuf_b.n);
-      HIP_TRY(hipMemsetAsync(L.pbuf_b.p, 0, L.pbuf_b.n * sizeof(zd), P->stream));
-      if (P->lcoef8.n) L.lcoef8.alloc(P->lcoef8.n);
     }
+    L.lpart_b.alloc(P->lpart_b.n);
+    L.pbuf_b.alloc(P->pbuf_b.n);
+    HIP_TRY(hipMemsetAsync(L.pbuf_b.p, 0, L.pbuf_b.n * sizeof(zd), P->stream));
+    if (P->lcoef8.n) L.lcoef8.alloc(P->lcoef8.n);
     if (!L.st) HIP_TRY(hipStreamCreate(&L.st));
     if (!L.aux) HIP_TRY(hipStreamCreate(&L.aux));
     for (hipEvent_t* e : {&L.ev_fork, &L.ev_join, &L.ev_start, &L.ev_done})
@@ -1263,11 +1271,11 @@ static bool dispatch_p(ifgf_plan* P, const zd* x, zd* y, hipStream_t st, bool fp
 }
 
 template <int F>
-static bool dispatch_b(ifgf_plan* P, int R, const zd* x, long long ld, zd* y, hipStream_t st, int lane) {
+static bool dispatch_b(ifgf_plan* P, int R, const zd* x, long long ld, zd* y, hipStream_t st, int lane, bool fp64) {
   const int a = P->prm.Ps, b = P->prm.Pang;
 #define IFGF_CASE(A, B)                                              \
   if (a == A && b == B) {                                            \
-    if (P->mixed) launch_batched_r<F, A, B, float>(P, R, x, ld, y, st, lane);  \
+    if (P->mixed && !fp64) launch_batched_r<F, A, B, float>(P, R, x, ld, y, st, lane);  \
     else launch_batched_r<F, A, B, double>(P, R, x, ld, y, st, lane);      \
     return true;                                                     \
   }
@@ -1282,11 +1290,12 @@ static bool dispatch_b(ifgf_plan* P, int R, const zd* x, long long ld, zd* y, hi
 }
 
 // one block of R (2, 4 or 8) columns through the batched kernels
-static bool dispatch_block(ifgf_plan* P, int R, const zd* x, long long ld, zd* y, hipStream_t st, int lane = 0) {
+static bool dispatch_block(ifgf_plan* P, int R, const zd* x, long long ld, zd* y, hipStream_t st, int lane = 0,
+                           bool fp64 = false) {
   switch (P->prm.formulation) {
-    case kSingleLayer: return dispatch_b<kSingleLayer>(P, R, x, ld, y, st, lane);
-    case kDoubleLayer: return dispatch_b<kDoubleLayer>(P, R, x, ld, y, st, lane);
-    default: return dispatch_b<kCombined>(P, R, x, ld, y, st, lane);
+    case kSingleLayer: return dispatch_b<kSingleLayer>(P, R, x, ld, y, st, lane, fp64);
+    case kDoubleLayer: return dispatch_b<kDoubleLayer>(P, R, x, ld, y, st, lane, fp64);
+    default: return dispatch_b<kCombined>(P, R, x, ld, y, st, lane, fp64);
   }
 }
 
@@ -1422,7 +1431,7 @@ int ifgf::apply_device(ifgf_plan* P, const zd* x, zd* y, hipStream_t st, bool fp
 // one block's latency-bound level pushes overlap the other's bandwidth-bound
 // near field, leaf operator and final sum; st waits for both lanes.
 int ifgf::apply_columns(ifgf_plan* P, const zd* x, long long ld, zd* y, int nrhs, hipStream_t st, bool fp64) {
-  const bool batched_ok = P->stored && P->rb_max >= 2 && !(fp64 && P->mixed);
+  const bool batched_ok = P->stored && P->rb_max >= 2;
   BatchLane& L2 = P->lane2;
   const bool two = batched_ok && P->rb_max == 8 && nrhs >= 16 && L2.ready;
   if (two) {
@@ -1442,7 +1451,7 @@ int ifgf::apply_columns(ifgf_plan* P, const zd* x, long long ld, zd* y, int nrhs
     zd* yr = y + (size_t)r * ld;
     if (R > 1) {
       const int lane = (two && R == 8 && (blk++ & 1)) ? 1 : 0;
-      if (!dispatch_block(P, R, xr, ld, yr, lane ? L2.st : st, lane))
+      if (!dispatch_block(P, R, xr, ld, yr, lane ? L2.st : st, lane, fp64))
         return fail(P, IFGF_ERR_ARG, "no kernel instantiation");
     } else {
       const int rc = apply_device(P, xr, yr, st, fp64);
@@ -1468,7 +1477,7 @@ int ifgf_apply(ifgf_plan* P, const double* x, double* y, int nrhs, long long ld,
     hipStream_t st = stream ? (hipStream_t)stream : P->stream;
     const bool fp64 = (flags & IFGF_APPLY_FP64) != 0;
     if (flags & IFGF_PTR_DEVICE) return ifgf::apply_columns(P, (const zd*)x, ld, (zd*)y, nrhs, st, fp64);
-    const bool batched_ok = P->stored && P->rb_max >= 2 && !(fp64 && P->mixed);
+    const bool batched_ok = P->stored && P->rb_max >= 2;
     int r = 0;
     while (r < nrhs) {
       // column block: the widest batched width that fits, else one column
@@ -1486,7 +1495,8 @@ int ifgf_apply(ifgf_plan* P, const double* x, double* y, int nrhs, long long ld,
         HIP_TRY(hipMemcpyAsync(P->xbuf.p + (size_t)q * N, xr + (size_t)q * ld, N * sizeof(zd), hipMemcpyHostToDevice,
                                st));
       if (R > 1) {
-        if (!dispatch_block(P, R, P->xbuf.p, N, P->ybuf.p, st)) return fail(P, IFGF_ERR_ARG, "no kernel instantiation");
+        if (!dispatch_block(P, R, P->xbuf.p, N, P->ybuf.p, st, 0, fp64))
+          return fail(P, IFGF_ERR_ARG, "no kernel instantiation");
       } else {
         int rc = apply_device(P, P->xbuf.p, P->ybuf.p, st, fp64);
         if (rc) return rc;

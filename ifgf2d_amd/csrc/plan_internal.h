@@ -156,6 +156,10 @@ struct ifgf_plan {
   ifgf::DevBuf<zf> lpart_b32, pbuf_b32;
   ifgf::DevBuf<ifgf::PushTask> btasks[4];
   std::vector<int> blevel_off[4];
+  // mixed-precision plans: the FP64 batched blocks' tasks too (their cones per
+  // task follow the FP64 LDS windows; IFGF_APPLY_FP64 blocks)
+  ifgf::DevBuf<ifgf::PushTask> btasks64[4];
+  std::vector<int> blevel_off64[4];
   ifgf::BatchLane lane2;  // the second set of 8-column buffers (nrhs_max >= 16)
   // mixed precision (IFGF_MIXED): FP32 copies of the leaf operator, the
   // propagation / interpolation geometry and inverse node factors, and FP32

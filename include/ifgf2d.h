@@ -79,7 +79,8 @@ extern "C" {
 /* apply flags */
 #define IFGF_PTR_DEVICE 1     /* x / y are device pointers; the call is asynchronous on stream */
 #define IFGF_APPLY_FP64 2     /* on an IFGF_MIXED plan: apply the all-FP64 operator instead of the
-                                 mixed-precision one (no effect on an IFGF_FP64 plan) */
+                                 mixed-precision one, in the same column blocks (no effect on an
+                                 IFGF_FP64 plan) */
 
 typedef struct ifgf_plan ifgf_plan;
 typedef struct ifgf_group ifgf_group;

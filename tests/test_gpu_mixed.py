@@ -70,6 +70,48 @@ def test_mixed_full_size_kite():
     assert 0 < e < MIXED_TOL
 
 
+BATCH_TOL = 1e-14  # relative 2-norm, FP64 column blocks against single columns
+
+
+def test_fp64_column_blocks_on_mixed_plan():
+    """IFGF_APPLY_FP64 with several columns on a mixed plan runs FP64 column
+    blocks (8, 8, 8, 2, 1 for 27 columns; two lanes through device pointers):
+    every column within BATCH_TOL of the same plan's single FP64 application
+    and of a plain FP64 plan's blocks, within 1e-12 of the oracle; the device
+    call equals the host call bitwise and repeats bitwise; the FP32 blocks of
+    the same plan stay within MIXED_TOL of the FP64 ones."""
+    torch = pytest.importorskip("torch")
+    curve, npatch, k, nlev = I.star(a=[0, 0, 0, 0, 0, 0.3]), 800, 60.0, 7
+    lims = I.equal_parameter_lims(npatch)
+    delta = I.default_delta(curve, lims)
+    kw = configs.scaled(curve, npatch, k, nlev, delta=delta)
+    fm = I.ForwardMap(**kw, precision="mixed", nrhs_max=32)
+    f64 = I.ForwardMap(**kw, nrhs_max=32)
+    n, nr = fm.N, 27
+    X = np.stack([density(n, 120 + s) for s in range(nr)], axis=1)
+    Y = fm.compute_Ax_acc(X, fp64=True)
+    e1 = max(rel(Y[:, j], fm.compute_Ax_acc(X[:, j], fp64=True)) for j in (0, 9, 17, 24, 26))
+    e2 = max(rel(Y[:, j], Yp) for j, Yp in enumerate(f64.compute_Ax_acc(X).T))
+    Ym = fm.compute_Ax_acc(X)
+    e3 = max(rel(Ym[:, j], Y[:, j]) for j in range(nr))
+    om = O.ForwardMap(curve[0], curve[1], lims, formulation=I.COMBINED, k=k, eta=k, delta=delta, p=6.0,
+                      nlevels=nlev)
+    e4 = max(rel(Y[:, j], om.apply(X[:, j])) for j in (3, 26))
+    print(f"FP64 blocks vs single {e1:.3e}, vs plain plan {e2:.3e}, FP32 blocks vs FP64 {e3:.3e}, "
+          f"vs oracle {e4:.3e}", flush=True)
+    assert e1 <= BATCH_TOL and e2 <= BATCH_TOL and 0 < e3 <= MIXED_TOL and e4 <= 1e-12
+    ld = n + 3
+    xt = torch.zeros((nr, ld), dtype=torch.complex128, device="cuda:0")
+    xt[:, :n] = torch.from_numpy(np.ascontiguousarray(X.T))
+    s = torch.cuda.current_stream()
+    for _ in range(2):
+        yt = torch.zeros_like(xt)
+        torch.cuda.synchronize()
+        fm.apply_device(xt.data_ptr(), yt.data_ptr(), nrhs=nr, ld=ld, stream=s.cuda_stream, fp64=True)
+        Yd = yt.cpu().numpy()
+        assert np.array_equal(Yd[:, :n].T, Y) and np.all(Yd[:, n:] == 0)
+
+
 def test_fgmres_circle_mie():
     """Sound-soft unit circle, k = 10: FGMRES on a mixed plan reaches the FP64
     true residual 1e-8 and the Mie series at r = 3, like FP64 GMRES."""
