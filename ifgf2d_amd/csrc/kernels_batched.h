@@ -603,18 +603,41 @@ __global__ void __launch_bounds__(256)
   zd acc[CH], near[CH], far[CH];
 #pragma unroll
   for (int c = 0; c < CH; c++) acc[c] = near[c] = far[c] = zd{0.0, 0.0};
+  // Each lane sums its entries in the order it, it + S, ...; the loops load
+  // kFinBU entries' indices, then their operands, before adding them in that
+  // order (one dependent round trip per kFinBU entries, not per entry). The
+  // loads are unconditional, at indices clamped into the lane's range, and
+  // the entries past it are skipped in the sums: a load inside a conditional
+  // would have to complete where the branches meet.
+  constexpr int kFinBU = 4;
   if (i < N) {
     const int si = inv[i];
     if (stage != 2) {
       const long long e0 = corr_ptr[i], e1 = corr_ptr[i + 1];
-      const long long nitem = (e1 - e0) * Np;
-      for (long long it = sl; it < nitem; it += S) {
-        const long long e = e0 + it / Np;
-        const int q = (int)(it % Np);
-        const zd w = corr_w[e * Np + q];
-        const zd* xs = xt + ((size_t)corr_patch[e] * Np + q) * R + r0;
+      // (a target's items are few: 32-bit indices within its rows)
+      const int ni = (int)((e1 - e0) * Np);
+      for (int ib = sl; ib < ni; ib += S * kFinBU) {
+        zd wv[kFinBU];
+        size_t src[kFinBU];
 #pragma unroll
-        for (int c = 0; c < CH; c++) cfma(acc[c], w, xs[c]);
+        for (int u = 0; u < kFinBU; u++) {
+          const int it0 = ib + u * S;
+          const int it = it0 < ni ? it0 : ni - 1;
+          const long long e = e0 + it / Np;
+          const int q = it % Np;
+          wv[u] = corr_w[e * Np + q];
+          src[u] = ((size_t)corr_patch[e] * Np + q) * R + r0;
+        }
+        zd xv[kFinBU][CH];
+#pragma unroll
+        for (int u = 0; u < kFinBU; u++)
+#pragma unroll
+          for (int c = 0; c < CH; c++) xv[u][c] = xt[src[u] + c];
+#pragma unroll
+        for (int u = 0; u < kFinBU; u++)
+          if (ib + u * S < ni)
+#pragma unroll
+            for (int c = 0; c < CH; c++) cfma(acc[c], wv[u], xv[u][c]);
       }
       if (nearb) {
         // the near sum of the 8-column MFMA blocks, one value per (target, column)
@@ -624,19 +647,44 @@ __global__ void __launch_bounds__(256)
       } else {
         const int g = si / kNearGroup, t = si % kNearGroup;
         const int w1 = nio[g + 1];
-        for (int w = nio[g] + sl; w < w1; w += S) {
-          const zd* np_ = npart + ((size_t)w * kNearGroup + t) * R + r0;
+        for (int wb = nio[g] + sl; wb < w1; wb += S * kFinBU) {
+          zd pv[kFinBU][CH];
 #pragma unroll
-          for (int c = 0; c < CH; c++) near[c] = cadd(near[c], np_[c]);
+          for (int u = 0; u < kFinBU; u++) {
+            const int w = wb + u * S < w1 ? wb + u * S : w1 - 1;
+            const zd* np_ = npart + ((size_t)w * kNearGroup + t) * R + r0;
+#pragma unroll
+            for (int c = 0; c < CH; c++) pv[u][c] = np_[c];
+          }
+#pragma unroll
+          for (int u = 0; u < kFinBU; u++)
+            if (wb + u * S < w1)
+#pragma unroll
+              for (int c = 0; c < CH; c++) near[c] = cadd(near[c], pv[u][c]);
         }
       }
     }
     if (stage != 1) {
       const long long f1 = it_ptr[si + 1];
-      for (long long f = it_ptr[si] + sl; f < f1; f += S) {
-        const ZI* sv = slots + (size_t)it_slot[f] * rs + r0;
+      for (long long fb = it_ptr[si] + sl; fb < f1; fb += (long long)S * kFinBU) {
+        int idx[kFinBU];
 #pragma unroll
-        for (int c = 0; c < CH; c++) far[c] = cadd(far[c], zwiden(sv[c]));
+        for (int u = 0; u < kFinBU; u++) {
+          const long long f = fb + (long long)u * S;
+          idx[u] = it_slot[f < f1 ? f : f1 - 1];
+        }
+        ZI sv[kFinBU][CH];
+#pragma unroll
+        for (int u = 0; u < kFinBU; u++) {
+          const ZI* sp = slots + (size_t)idx[u] * rs + r0;
+#pragma unroll
+          for (int c = 0; c < CH; c++) sv[u][c] = sp[c];
+        }
+#pragma unroll
+        for (int u = 0; u < kFinBU; u++)
+          if (fb + (long long)u * S < f1)
+#pragma unroll
+            for (int c = 0; c < CH; c++) far[c] = cadd(far[c], zwiden(sv[u][c]));
       }
     }
   }
