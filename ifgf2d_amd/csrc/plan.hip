@@ -500,8 +500,8 @@ static void launch_batched(ifgf_plan* P, const zd* x, long long ld, zd* y, hipSt
   constexpr int LR = R == 2 ? 1 : R == 4 ? 2 : 3;
   const Tree& t = P->tree;
   const int N = (int)t.N, D = t.D;
-  // the lane's buffers, second stream and events (lane 1: P->lane2)
-  BatchLane* L2 = lane ? &P->lane2 : nullptr;
+  // the lane's buffers, second stream and events (lane l >= 1: P->xlane[l - 1])
+  BatchLane* L2 = lane ? &P->xlane[lane - 1] : nullptr;
   zd* rho = L2 ? L2->rho.p : P->rho.p;
   zd* xt_b = L2 ? L2->xt_b.p : P->xt_b.p;
   zd* npart_b = L2 ? L2->npart_b.p : P->npart_b.p;
@@ -748,9 +748,10 @@ static void build_batched(ifgf_plan* P) {
     HIP_TRY(hipStreamSynchronize(P->stream));
     P->mfma8 = true;
   }
-  // the second lane of 8-column blocks (plans for 16 or more columns)
-  if (P->rb_max == 8 && nr >= 16) {
-    BatchLane& L = P->lane2;
+  // further lanes of 8-column blocks (plans for 16, 24 or more columns)
+  for (int xl = 0; xl < 2; xl++) {
+    if (P->rb_max != 8 || nr < 16 + 8 * xl) break;
+    BatchLane& L = P->xlane[xl];
     L.rho.alloc((size_t)P->tree.N * 8);
     L.xt_b.alloc(P->xt_b.n);
     L.npart_b.alloc(P->npart_b.n);
@@ -1338,12 +1339,14 @@ static void autotune(ifgf_plan* P) {
   // kept; starting from the knob's default, a candidate (in list order)
   // replaces the current choice only when it is more than 1% faster than
   // it (near-ties otherwise flip with timing noise between runs).
-  auto pick = [&](int def, const std::vector<int>& cand, const std::function<void(int)>& set) {
+  // (timer: one-column applications unless given)
+  auto pick = [&](int def, const std::vector<int>& cand, const std::function<void(int)>& set,
+                  const std::function<float()>& timer = {}) {
     std::vector<float> tm(cand.size(), 1e30f);
     for (int sweep = 0; sweep < 2; sweep++)
       for (size_t q = 0; q < cand.size(); q++) {
         set(cand[q]);
-        tm[q] = std::min(tm[q], time_apply());
+        tm[q] = std::min(tm[q], timer ? timer() : time_apply());
       }
     float tdef = 1e30f;
     for (size_t q = 0; q < cand.size(); q++)
@@ -1396,6 +1399,29 @@ static void autotune(ifgf_plan* P) {
     }
     P->overlap_b = best;
     HIP_TRY(hipStreamSynchronize(st));
+    // a third lane, timed on a call of 24 columns (three 8-column blocks)
+    if (P->xlane[1].ready) {
+      DevBuf<zd> x3, y3;
+      x3.alloc((size_t)N * 24);
+      y3.alloc((size_t)N * 24);
+      for (int c = 0; c < 24; c++)
+        HIP_TRY(hipMemcpy(x3.p + (size_t)c * N, xh.data(), N * sizeof(zd), hipMemcpyHostToDevice));
+      pick(2, {2, 3}, [&](int c) { P->max_lanes = c; }, [&]() {
+        float best3 = 1e30f;
+        apply_columns(P, x3.p, N, y3.p, 24, st, false);
+        for (int round = 0; round < 2; round++) {
+          HIP_TRY(hipEventRecord(e0, st));
+          for (int r = 0; r < 2; r++) apply_columns(P, x3.p, N, y3.p, 24, st, false);
+          HIP_TRY(hipEventRecord(e1, st));
+          HIP_TRY(hipEventSynchronize(e1));
+          float ms = 0.f;
+          HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+          best3 = std::min(best3, ms);
+        }
+        return best3;
+      });
+      HIP_TRY(hipStreamSynchronize(st));
+    }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
@@ -1432,11 +1458,15 @@ int ifgf::apply_device(ifgf_plan* P, const zd* x, zd* y, hipStream_t st, bool fp
 // near field, leaf operator and final sum; st waits for both lanes.
 int ifgf::apply_columns(ifgf_plan* P, const zd* x, long long ld, zd* y, int nrhs, hipStream_t st, bool fp64) {
   const bool batched_ok = P->stored && P->rb_max >= 2;
-  BatchLane& L2 = P->lane2;
-  const bool two = batched_ok && P->rb_max == 8 && nrhs >= 16 && L2.ready;
-  if (two) {
-    HIP_TRY(hipEventRecord(L2.ev_start, st));
-    HIP_TRY(hipStreamWaitEvent(L2.st, L2.ev_start, 0));
+  // lanes used: 1 + the extra lanes that are built and that the call's
+  // 8-column blocks fill
+  int nl = 1;
+  if (batched_ok && P->rb_max == 8)
+    while (nl < P->max_lanes && P->xlane[nl - 1].ready && nrhs >= 8 * (nl + 1)) nl++;
+  for (int l = 1; l < nl; l++) {
+    BatchLane& L = P->xlane[l - 1];
+    HIP_TRY(hipEventRecord(L.ev_start, st));
+    HIP_TRY(hipStreamWaitEvent(L.st, L.ev_start, 0));
   }
   int r = 0, blk = 0;
   while (r < nrhs) {
@@ -1450,8 +1480,8 @@ int ifgf::apply_columns(ifgf_plan* P, const zd* x, long long ld, zd* y, int nrhs
     const zd* xr = x + (size_t)r * ld;
     zd* yr = y + (size_t)r * ld;
     if (R > 1) {
-      const int lane = (two && R == 8 && (blk++ & 1)) ? 1 : 0;
-      if (!dispatch_block(P, R, xr, ld, yr, lane ? L2.st : st, lane, fp64))
+      const int lane = R == 8 ? blk++ % nl : 0;
+      if (!dispatch_block(P, R, xr, ld, yr, lane ? P->xlane[lane - 1].st : st, lane, fp64))
         return fail(P, IFGF_ERR_ARG, "no kernel instantiation");
     } else {
       const int rc = apply_device(P, xr, yr, st, fp64);
@@ -1459,9 +1489,10 @@ int ifgf::apply_columns(ifgf_plan* P, const zd* x, long long ld, zd* y, int nrhs
     }
     r += R;
   }
-  if (two) {
-    HIP_TRY(hipEventRecord(L2.ev_done, L2.st));
-    HIP_TRY(hipStreamWaitEvent(st, L2.ev_done, 0));
+  for (int l = 1; l < nl; l++) {
+    BatchLane& L = P->xlane[l - 1];
+    HIP_TRY(hipEventRecord(L.ev_done, L.st));
+    HIP_TRY(hipStreamWaitEvent(st, L.ev_done, 0));
   }
   return IFGF_OK;
 }

@@ -160,7 +160,10 @@ struct ifgf_plan {
   // task follow the FP64 LDS windows; IFGF_APPLY_FP64 blocks)
   ifgf::DevBuf<ifgf::PushTask> btasks64[4];
   std::vector<int> blevel_off64[4];
-  ifgf::BatchLane lane2;  // the second set of 8-column buffers (nrhs_max >= 16)
+  // the further sets of 8-column buffers: xlane[0] for nrhs_max >= 16,
+  // xlane[1] for nrhs_max >= 24 (lane l >= 1 of a call is xlane[l - 1])
+  ifgf::BatchLane xlane[2];
+  int max_lanes = 2;  // lanes a call's 8-column blocks may use (autotuned: 2 or 3)
   // mixed precision (IFGF_MIXED): FP32 copies of the leaf operator, the
   // propagation / interpolation geometry and inverse node factors, and FP32
   // coefficient workspaces; the FP64 operator stays available (IFGF_APPLY_FP64)

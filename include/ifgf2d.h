@@ -105,7 +105,8 @@ typedef struct {
   double eta;     /* combined-field coupling */
   int Ps, Pang;   /* Chebyshev nodes per cone in s and theta (BoxTree<Ps,Pang>) */
   int nlevels;    /* tree levels, >= 3 */
-  int nrhs_max;   /* largest nrhs passed to ifgf_apply (>= 16: a second lane of 8-column buffers) */
+  int nrhs_max;   /* largest nrhs passed to ifgf_apply (>= 16: a second lane of 8-column buffers;
+                     >= 24: a third, used where the autotuner finds it faster) */
   int flags;      /* IFGF_PLAN_HOST_ONLY | IFGF_PLAN_ON_THE_FLY | IFGF_PLAN_AUTOTUNE */
 } ifgf_params;
 
