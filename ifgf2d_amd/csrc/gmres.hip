@@ -793,7 +793,7 @@ extern "C" int ifgf_fgmres(ifgf_plan* P, const double* rhs, double* x, int resta
 // norms and the work vectors stay FP64).
 template <typename VT>
 static int gmres_multi_dev(ifgf_plan* P, const zd* Bp, zd* Xp, int nrhs, int restart, double tol, int maxit,
-                           int* iters, double* relres, bool true_res) {
+                           int* iters, double* relres, bool true_res, bool x0_zero = false) {
   using C = std::complex<double>;
   const long long n = P->geo.N;
   const int m = restart, R = nrhs;
@@ -847,8 +847,13 @@ static int gmres_multi_dev(ifgf_plan* P, const zd* Bp, zd* Xp, int nrhs, int res
       norms(R, bn);
     }
     for (int s = 0; s < R; s++) iters[s] = 0;
-    residuals(false);
-    norms(R, tmp);
+    if (x0_zero) {
+      // A 0 = 0: the initial residuals are the right-hand sides, which W holds
+      tmp = bn;
+    } else {
+      residuals(false);
+      norms(R, tmp);
+    }
     std::vector<char> done(R, 0);
     std::vector<std::vector<C>> Hm(R, std::vector<C>((size_t)(m + 1) * m)), g(R, std::vector<C>(m + 1));
     std::vector<std::vector<Givens>> rot(R, std::vector<Givens>(m));
@@ -960,6 +965,14 @@ static int gmres_multi_dev(ifgf_plan* P, const zd* Bp, zd* Xp, int nrhs, int res
   }
 }
 
+// True when every one of the count doubles is zero (either sign; a NaN is not).
+static bool host_all_zero(const double* v, long long count) {
+  int nz = 0;
+#pragma omp parallel for reduction(| : nz) schedule(static)
+  for (long long i = 0; i < count; i++) nz |= v[i] != 0.0;
+  return nz == 0;
+}
+
 extern "C" int ifgf_gmres_multi(ifgf_plan* P, const double* rhs, double* x, int nrhs, int restart, double tol,
                                 int maxit, int* iters, double* relres) {
   if (!P || !rhs || !x || !iters || !relres || nrhs < 1 || restart < 1 || maxit < 1)
@@ -973,8 +986,10 @@ extern "C" int ifgf_gmres_multi(ifgf_plan* P, const double* rhs, double* x, int 
     B.alloc((size_t)n * nrhs);
     X.alloc((size_t)n * nrhs);
     HIP_TRY(hipMemcpyAsync(B.p, rhs, (size_t)n * nrhs * sizeof(zd), hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(X.p, x, (size_t)n * nrhs * sizeof(zd), hipMemcpyHostToDevice, st));
-    const int rc = gmres_multi_dev<zd>(P, B.p, X.p, nrhs, restart, tol, maxit, iters, relres, true);
+    const bool x0_zero = host_all_zero(x, 2 * n * (long long)nrhs);
+    if (x0_zero) HIP_TRY(hipMemsetAsync(X.p, 0, (size_t)n * nrhs * sizeof(zd), st));
+    else HIP_TRY(hipMemcpyAsync(X.p, x, (size_t)n * nrhs * sizeof(zd), hipMemcpyHostToDevice, st));
+    const int rc = gmres_multi_dev<zd>(P, B.p, X.p, nrhs, restart, tol, maxit, iters, relres, true, x0_zero);
     if (rc != IFGF_OK && rc != IFGF_ERR_NOT_CONVERGED) return rc;
     HIP_TRY(hipMemcpyAsync(x, X.p, (size_t)n * nrhs * sizeof(zd), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
@@ -1020,10 +1035,14 @@ extern "C" int ifgf_refine_multi(ifgf_plan* P, const double* rhs, double* x, int
     part.alloc((size_t)ps * R);
     sysd.alloc(R);
     HIP_TRY(hipMemcpyAsync(B.p, rhs, (size_t)n * R * sizeof(zd), hipMemcpyHostToDevice, st));
-    HIP_TRY(hipMemcpyAsync(X.p, x, (size_t)n * R * sizeof(zd), hipMemcpyHostToDevice, st));
+    bool x_zero = host_all_zero(x, 2 * n * (long long)R);
+    if (x_zero) HIP_TRY(hipMemsetAsync(X.p, 0, (size_t)n * R * sizeof(zd), st));
+    else HIP_TRY(hipMemcpyAsync(X.p, x, (size_t)n * R * sizeof(zd), hipMemcpyHostToDevice, st));
     std::vector<double> bn(R);
     {
+      // (one column per thread, each summed in order)
       const std::complex<double>* b = reinterpret_cast<const std::complex<double>*>(rhs);
+#pragma omp parallel for schedule(static)
       for (int q = 0; q < R; q++) {
         double acc = 0.0;
         for (long long i = 0; i < n; i++) acc += std::norm(b[(size_t)q * n + i]);
@@ -1033,14 +1052,20 @@ extern "C" int ifgf_refine_multi(ifgf_plan* P, const double* rhs, double* x, int
     }
     std::vector<int> act(R);
     for (int q = 0; q < R; q++) act[q] = q;
-    // W[:, a] = b - A x (all-FP64 operator) for the systems in act; relres of each
+    // W[:, a] = b - A x (all-FP64 operator) for the systems in act; relres of
+    // each (x_zero: x = 0 and act every system in order, so W = b)
     auto residuals = [&]() {
       const int na = (int)act.size();
-      HIP_TRY(hipMemcpyAsync(sysd.p, act.data(), na * sizeof(int), hipMemcpyHostToDevice, st));
-      hipLaunchKernelGGL(k_gather_ms<zd>, dim3(nb, na), dim3(256), 0, st, n, (const zd*)X.p, n, 0, sysd.p, G.p);
-      const int rc = apply_columns(P, G.p, n, W.p, na, st, true);
-      if (rc) throw std::runtime_error(P->err);
-      hipLaunchKernelGGL(k_residual_ms, dim3(nb, na), dim3(256), 0, st, n, B.p, sysd.p, W.p);
+      if (x_zero) {
+        HIP_TRY(hipMemcpyAsync(W.p, B.p, (size_t)n * na * sizeof(zd), hipMemcpyDeviceToDevice, st));
+        x_zero = false;
+      } else {
+        HIP_TRY(hipMemcpyAsync(sysd.p, act.data(), na * sizeof(int), hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_gather_ms<zd>, dim3(nb, na), dim3(256), 0, st, n, (const zd*)X.p, n, 0, sysd.p, G.p);
+        const int rc = apply_columns(P, G.p, n, W.p, na, st, true);
+        if (rc) throw std::runtime_error(P->err);
+        hipLaunchKernelGGL(k_residual_ms, dim3(nb, na), dim3(256), 0, st, n, B.p, sysd.p, W.p);
+      }
       hipLaunchKernelGGL(k_norm_parts_ms, dim3(kDot, 1, na), dim3(256), 0, st, n, W.p, part.p, ps);
       std::vector<zd> pt((size_t)ps * na);
       HIP_TRY(hipMemcpyAsync(pt.data(), part.p, pt.size() * sizeof(zd), hipMemcpyDeviceToHost, st));
@@ -1070,8 +1095,10 @@ extern "C" int ifgf_refine_multi(ifgf_plan* P, const double* rhs, double* x, int
       const double tau = std::min(0.5, std::max(inner_floor, tol / emax));
       HIP_TRY(hipMemsetAsync(D.p, 0, (size_t)n * na * sizeof(zd), st));
       // corrections with the plan's default operator; an FP32 basis with the FP32 operator
-      const int rc = P->mixed ? gmres_multi_dev<zf>(P, W.p, D.p, na, restart, tau, maxit, it.data(), rr.data(), false)
-                              : gmres_multi_dev<zd>(P, W.p, D.p, na, restart, tau, maxit, it.data(), rr.data(), false);
+      // (D = 0: the initial residuals are W)
+      const int rc = P->mixed
+                         ? gmres_multi_dev<zf>(P, W.p, D.p, na, restart, tau, maxit, it.data(), rr.data(), false, true)
+                         : gmres_multi_dev<zd>(P, W.p, D.p, na, restart, tau, maxit, it.data(), rr.data(), false, true);
       if (rc != IFGF_OK && rc != IFGF_ERR_NOT_CONVERGED) return rc;
       k++;
       HIP_TRY(hipMemcpyAsync(sysd.p, act.data(), na * sizeof(int), hipMemcpyHostToDevice, st));

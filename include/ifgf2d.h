@@ -311,7 +311,8 @@ int ifgf_gmres(ifgf_plan* plan, const double* rhs, double* x, int restart, doubl
  * new basis vectors in one batched call (blocks of 8 columns when the plan's
  * nrhs_max allows) and orthogonalises all of them with one set of kernels.
  * rhs / x: n x nrhs, column-major, host pointers (x holds the initial
- * guesses). iters[s] / relres[s]: iterations and true FP64 relative residual
+ * guesses; an all-zero x is not copied or applied: the first residuals are
+ * rhs). iters[s] / relres[s]: iterations and true FP64 relative residual
  * of system s. Returns IFGF_OK when every system converged. restart <= 256. */
 int ifgf_gmres_multi(ifgf_plan* plan, const double* rhs, double* x, int nrhs, int restart, double tol,
                      int maxit, int* iters, double* relres);
@@ -321,10 +322,11 @@ int ifgf_gmres_multi(ifgf_plan* plan, const double* rhs, double* x, int nrhs, in
  * A d = b - A x (ifgf_gmres_multi, x0 = 0, the plan's default operator: mixed
  * precision, with an FP32 Krylov basis, on an IFGF_MIXED plan) to relative tolerance
  * min(0.5, max(inner_floor, tol / e)), e the largest true relative residual of
- * the systems not yet converged; the residuals use the all-FP64 operator, one
- * column at a time. A system stops once its true FP64 relative residual is
- * below tol; at most max_outer refinement steps. rhs / x: n x nrhs,
- * column-major, host pointers (x holds the initial guesses). outer: the
+ * the systems not yet converged; the residuals use the all-FP64 operator, in
+ * the same column blocks. A system stops once its true FP64 relative residual
+ * is below tol; at most max_outer refinement steps. rhs / x: n x nrhs,
+ * column-major, host pointers (x holds the initial guesses; an all-zero x is
+ * not copied or applied: the first residuals are rhs). outer: the
  * refinement steps taken; iters[s]: system s's inner Arnoldi steps; relres[s]:
  * its true FP64 relative residual. Returns IFGF_OK when every system
  * converged, else IFGF_ERR_NOT_CONVERGED. */
