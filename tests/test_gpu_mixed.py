@@ -226,6 +226,41 @@ def test_multi_rhs_mixed_refinement():
         assert rel(X[:, j], X64[:, j]) < 1e-7
 
 
+def test_multi_rhs_nonzero_initial_guess():
+    """A nonzero initial guess (a zero one is neither copied nor applied),
+    from a rough solution (the FP64 lockstep solution at 1e-3, one column left
+    at zero). The lockstep FP64 GMRES stops on |w(k)| / ||r0|| < tol per
+    system, so from the smaller r0 it goes further below ||b|| (every true
+    residual below 1e-8 x the start's, the zero column to within one
+    iteration of its zero-start count); the mixed refinement, which stops on
+    the true residual against ||b||, takes fewer inner iterations on the
+    warm-started columns than from zero. Every solution within 1e-7 of the
+    zero-start ones."""
+    kw = configs.scaled(I.kite(), 400, 40.0, 6)
+    f64 = I.ForwardMap(**kw, nrhs_max=8)
+    fmx = I.ForwardMap(**kw, precision="mixed", nrhs_max=8)
+    x, y, *_ = f64.points()
+    th = 2 * np.pi * np.arange(9) / 9
+    B = np.stack([-configs.plane_wave(x, y, 40.0, (np.cos(t), np.sin(t))) for t in th], axis=1)
+    X0, its0, rr0 = f64.solve_multi(B, restart=30, tol=1e-8, maxit=500)
+    G, _, rrg = f64.solve_multi(B, restart=30, tol=1e-3, maxit=500)
+    G[:, 4] = 0.0
+    rrg[4] = 1.0
+    X1, its1, rr1 = f64.solve_multi(B, restart=30, tol=1e-8, maxit=500, x0=G)
+    Xz, outz, itz, rrz = fmx.solve_multi_refined(B, restart=30, tol=1e-8, maxit=500)
+    Xr, outer, itr, rrr = fmx.solve_multi_refined(B, restart=30, tol=1e-8, maxit=500, x0=G)
+    print("zero start", its0.tolist(), "| warm start", its1.tolist(), "| refined from zero", outz, itz.tolist(),
+          "| from the warm start", outer, itr.tolist(), flush=True)
+    assert np.all(rr0 < 1e-8) and np.all(rrz < 1e-8) and np.all(rrr < 1e-8)
+    assert np.all(rr1 < 1e-8 * rrg * 10)
+    assert abs(int(its1[4]) - int(its0[4])) <= 1
+    assert outer <= outz and all(itr[j] < itz[j] for j in range(9) if j != 4)
+    for j in range(9):
+        assert rel(X1[:, j], X0[:, j]) < 1e-7 and rel(Xr[:, j], X0[:, j]) < 1e-7
+        r = B[:, j] - f64.compute_Ax_acc(Xr[:, j])
+        assert np.linalg.norm(r) / np.linalg.norm(B[:, j]) < 1e-8
+
+
 def test_star_256k_64_directions():
     """C4 at its stated scale: 64 incidence directions 2 pi j / 64 on the
     256,000-point star at k = 500, solved together by the FP64 lockstep GMRES
