@@ -351,8 +351,8 @@ __global__ void __launch_bounds__(256) k_level_push_b(PushArgsT<RT> a) {
 // The A operand is stored in MFMA order: tile (row tile rt, K chunk q) holds
 // the 64 values lane l reads (row l % 16, K index 4q + l / 16), so a wave
 // loads 64 consecutive values per instruction. V_MFMA_F64_16X16X4F64 (FP64
-// plans; the near field on mixed plans too) and V_MFMA_F32_16X16X4F32 (the
-// leaf operator of mixed plans): lane l supplies A[l % 16][l / 16] and
+// blocks) and V_MFMA_F32_16X16X4F32 (the mixed-precision blocks of mixed
+// plans): lane l supplies A[l % 16][l / 16] and
 // B[l / 16][l % 16] and receives C[row][l % 16], v < 4, with row = 4 (l / 16)
 // + v for the FP32 instruction and row = l / 16 + 4 v for the FP64 one
 // (checked on the device with unit-vector operands).
@@ -441,12 +441,16 @@ __global__ void __launch_bounds__(256)
 
 // Near field of 8-column blocks: one block of 4 waves per unit (group, row
 // tile); wave w sums the K chunks [w Q / 4, (w + 1) Q / 4), the waves' tiles
-// are added in wave order; out[(row0 + row) * 8 + r] = the target's near sum.
+// are added in wave order; out[(row0 + row) * 8 + r] = the target's near sum
+// (RT = double: FP64 tiles and products; float: the tiles rounded to FP32,
+// FP32 products and sums, widened into out).
+template <typename RT>
 __global__ void __launch_bounds__(256)
     k_near_mfma(int nunits, const MfmaUnit* __restrict__ units, const MfmaGroup* __restrict__ grp,
-                const int* __restrict__ segs, const double* __restrict__ A, const zd* __restrict__ rho,
+                const int* __restrict__ segs, const RT* __restrict__ A, const zd* __restrict__ rho,
                 zd* __restrict__ out) {
-  __shared__ double red[3][64][4];
+  using V4 = typename std::conditional<std::is_same<RT, double>::value, dbl4, flt4>::type;
+  __shared__ RT red[3][64][4];
   const int u = xcd_block();
   if (u >= nunits) return;
   const MfmaUnit U = units[u];
@@ -461,9 +465,9 @@ __global__ void __launch_bounds__(256)
   // the one density component this lane feeds: rows 2j (X_re | X_im), rows
   // 2j + 1 (X_im | -X_re)
   const int part = (c == 0) == im ? 1 : 0;
-  const double sgn = (c == 1 && im) ? -1.0 : 1.0;
+  const RT sgn = (c == 1 && im) ? RT(-1) : RT(1);
   const double* __restrict__ x = reinterpret_cast<const double*>(rho) + r * 2 + part;
-  const double* __restrict__ a = A + G.aoff + (long long)U.rt * Q * 64 + lane;
+  const RT* __restrict__ a = A + G.aoff + (long long)U.rt * Q * 64 + lane;
   // the group's segment table in registers (uniform)
   int se[kMaxNb], sh[kMaxNb];
 #pragma unroll
@@ -476,9 +480,9 @@ __global__ void __launch_bounds__(256)
   // products on two accumulator chains (added at the end); chunks past the
   // wave's range and sources past the group's take 0
   constexpr int kU = 8;
-  dbl4 acc = {0.0, 0.0, 0.0, 0.0}, acc2 = {0.0, 0.0, 0.0, 0.0};
+  V4 acc = {0, 0, 0, 0}, acc2 = {0, 0, 0, 0};
   for (int qb = q0; qb < q1; qb += kU) {
-    double av[kU], bv[kU];
+    RT av[kU], bv[kU];
     bool ok[kU];
 #pragma unroll
     for (int t = 0; t < kU; t++) {
@@ -490,12 +494,12 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
       for (int k = 1; k < kMaxNb; k++) src = (oc >= se[k - 1]) ? oc + sh[k] : src;
       av[t] = a[(long long)q * 64];
-      bv[t] = x[(size_t)src * 16];
+      bv[t] = (RT)x[(size_t)src * 16];
     }
 #pragma unroll
     for (int t = 0; t < kU; t++) {
-      av[t] = qb + t < q1 ? av[t] : 0.0;
-      bv[t] = ok[t] ? sgn * bv[t] : 0.0;
+      av[t] = qb + t < q1 ? av[t] : RT(0);
+      bv[t] = ok[t] ? sgn * bv[t] : RT(0);
     }
 #pragma unroll
     for (int t = 0; t < kU; t += 2) {
@@ -512,8 +516,8 @@ __global__ void __launch_bounds__(256)
       for (int v = 0; v < 4; v++) acc[v] += red[s][lane][v];
     double* o = reinterpret_cast<double*>(out);
     for (int v = 0; v < 4; v++) {
-      const int row = U.rt * 16 + mfma_row(0.0, kk, v);
-      if (row < G.rows) o[((size_t)(G.row0 + row) * 8 + r) * 2 + (im ? 1 : 0)] = acc[v];
+      const int row = U.rt * 16 + mfma_row(RT(0), kk, v);
+      if (row < G.rows) o[((size_t)(G.row0 + row) * 8 + r) * 2 + (im ? 1 : 0)] = (double)acc[v];
     }
   }
 }

@@ -522,8 +522,12 @@ static void launch_batched(ifgf_plan* P, const zd* x, long long ld, zd* y, hipSt
   const bool mf = R == 8 && P->mfma8 && P->use_mfma8;
   auto near_b = [&](hipStream_t s) {
     if (mf) {
-      hipLaunchKernelGGL(k_near_mfma, dim3((unsigned)P->nm_units.n), dim3(256), 0, s, (int)P->nm_units.n,
-                         P->nm_units.p, P->nm_grp.p, P->nm_segs.p, P->nm_A.p, rho, nearb8);
+      if constexpr (f32)
+        hipLaunchKernelGGL(k_near_mfma<float>, dim3((unsigned)P->nm_units.n), dim3(256), 0, s, (int)P->nm_units.n,
+                           P->nm_units.p, P->nm_grp.p, P->nm_segs.p, P->nm_A32.p, rho, nearb8);
+      else
+        hipLaunchKernelGGL(k_near_mfma<double>, dim3((unsigned)P->nm_units.n), dim3(256), 0, s, (int)P->nm_units.n,
+                           P->nm_units.p, P->nm_grp.p, P->nm_segs.p, P->nm_A.p, rho, nearb8);
     } else if (M > 0) {
       hipLaunchKernelGGL((k_near_items_b<R>), dim3((M + 3) / 4), dim3(256), 0, s, N, M, P->near_items.p,
                          P->nf_seg.p, P->nf_val.p, rho, npart_b);
@@ -1178,6 +1182,11 @@ static void build_stored(ifgf_plan* P) {
                          P->push_einfo.p, P->push_igeo32.p, P->push_ptab32.p, P->push_geo32.p);
     P->ifac_tab32.alloc(P->ifac_tab.n);
     round(P->ifac_tab.p, P->ifac_tab32.p, (long long)P->ifac_tab.n);
+    if (P->nm_A.n) {  // the 8-column near tiles (a multiple of 64 values)
+      P->nm_A32.alloc(P->nm_A.n);
+      round(P->nm_A.p, P->nm_A32.p, (long long)P->nm_A.n / 2);
+      P->stored_bytes += (long long)P->nm_A32.n * (long long)sizeof(float);
+    }
     P->pbuf32.alloc(P->pbuf.n);
     HIP_TRY(hipMemsetAsync(P->pbuf32.p, 0, P->pbuf32.n * sizeof(zf), st));
     P->lpart32.alloc(P->lpart.n);

@@ -185,7 +185,7 @@ struct ifgf_plan {
   ifgf::DevBuf<ifgf::MfmaUnit> nm_units, lm_units;
   ifgf::DevBuf<int> nm_segs, lm_cone0, lio_id;
   ifgf::DevBuf<double> nm_A, lm_A;
-  ifgf::DevBuf<float> lm_A32;
+  ifgf::DevBuf<float> lm_A32, nm_A32;  // mixed plans: the tiles rounded to FP32
   ifgf::DevBuf<zd> nearb8, lcoef8;
   ifgf::DevBuf<zf> lcoef8_32;
   int nleaf = 0;

@@ -38,8 +38,10 @@ extern "C" {
 #define IFGF_FP64 0   /* everything in FP64 */
 #define IFGF_MIXED 1  /* FP32 leaf operator, cone coefficients, propagation and interpolation;
                          FP64 Hankel, centre factors (evaluated in FP64, applied rounded to
-                         FP32), near field, correction and final accumulation. The plan also
-                         keeps the FP64 operator (IFGF_APPLY_FP64). Needs stored operators. */
+                         FP32), near field, correction and final accumulation (8-column
+                         blocks: the near field on FP32 matrix cores, its sums widened to
+                         FP64). The plan also keeps the FP64 operator (IFGF_APPLY_FP64).
+                         Needs stored operators. */
 
 /* curve types (Geometry/Curve.hpp:106-146; star has no reference class) */
 #define IFGF_CURVE_CIRCLE 0 /* params: cx, cy, R */

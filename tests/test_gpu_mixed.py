@@ -2,7 +2,9 @@
 
 A mixed plan applies the leaf operator, the cone coefficients, the propagation
 and the interpolation in FP32 (centre factors evaluated in FP64 and rounded);
-near field, correction and accumulation stay FP64. Bars, written per test:
+near field, correction and accumulation stay FP64 (in 8-column blocks the
+near field runs on FP32 matrix cores, its sums widened to FP64). Bars,
+written per test:
 the mixed operator is within MIXED_TOL of the oracle's FP64 forward map
 (FP32 rounding of the far-field part only); the plan's FP64 operator
 (IFGF_APPLY_FP64) is bitwise the plain plan's; FGMRES (FP64 outer Arnoldi,
