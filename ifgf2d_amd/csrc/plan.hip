@@ -901,7 +901,6 @@ static void build_stored(ifgf_plan* P) {
                          (int)grp.size(), P->nm_grp.p, P->nf_goff.p, P->nf_val.p, P->nm_A.p);
     HIP_TRY(hipGetLastError());
     P->nearb8.alloc((size_t)N * 8);
-    P->stored_bytes += aoff * (long long)sizeof(double);
   }
   // leaf operator: P x n_src per leaf cone
   const LevelInfo& LD = t.lv[D];
@@ -1162,6 +1161,7 @@ static void build_stored(ifgf_plan* P) {
                     (long long)(P->push_igeo.n + P->push_ptab.n) * (long long)sizeof(Geo4) +
                     (npush + nit + t.n_cones) * (long long)sizeof(int) +
                     (long long)t.n_cones * Pn * (long long)sizeof(int4);
+  P->stored_bytes += (long long)P->nm_A.n * (long long)sizeof(double);  // the 8-column near tiles
   if (P->prm.precision == IFGF_MIXED) {
     // FP32 copies: leaf operator, propagation / interpolation geometry
     // (factors evaluated in FP64 above, rounded here), inverse node factors
