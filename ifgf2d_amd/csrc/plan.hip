@@ -1415,7 +1415,7 @@ static void autotune(ifgf_plan* P) {
       y3.alloc((size_t)N * 24);
       for (int c = 0; c < 24; c++)
         HIP_TRY(hipMemcpy(x3.p + (size_t)c * N, xh.data(), N * sizeof(zd), hipMemcpyHostToDevice));
-      pick(2, {2, 3}, [&](int c) { P->max_lanes = c; }, [&]() {
+      pick(P->max_lanes, {2, 3}, [&](int c) { P->max_lanes = c; }, [&]() {
         float best3 = 1e30f;
         apply_columns(P, x3.p, N, y3.p, 24, st, false);
         for (int round = 0; round < 2; round++) {

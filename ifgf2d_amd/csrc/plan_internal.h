@@ -163,7 +163,7 @@ struct ifgf_plan {
   // the further sets of 8-column buffers: xlane[0] for nrhs_max >= 16,
   // xlane[1] for nrhs_max >= 24 (lane l >= 1 of a call is xlane[l - 1])
   ifgf::BatchLane xlane[2];
-  int max_lanes = 2;  // lanes a call's 8-column blocks may use (autotuned: 2 or 3)
+  int max_lanes = 3;  // lanes a call's 8-column blocks may use (autotuned: 2 or 3)
   // mixed precision (IFGF_MIXED): FP32 copies of the leaf operator, the
   // propagation / interpolation geometry and inverse node factors, and FP32
   // coefficient workspaces; the FP64 operator stays available (IFGF_APPLY_FP64)

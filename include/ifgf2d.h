@@ -108,7 +108,7 @@ typedef struct {
   int Ps, Pang;   /* Chebyshev nodes per cone in s and theta (BoxTree<Ps,Pang>) */
   int nlevels;    /* tree levels, >= 3 */
   int nrhs_max;   /* largest nrhs passed to ifgf_apply (>= 16: a second lane of 8-column buffers;
-                     >= 24: a third, used where the autotuner finds it faster) */
+                     >= 24: a third, which IFGF_PLAN_AUTOTUNE drops where two time faster) */
   int flags;      /* IFGF_PLAN_HOST_ONLY | IFGF_PLAN_ON_THE_FLY | IFGF_PLAN_AUTOTUNE */
 } ifgf_params;
 

@@ -253,13 +253,14 @@ def test_mfma_blocks_match_valu_blocks(precision):
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_two_lane_column_blocks(precision):
-    """A plan for 32 columns keeps a second set of 8-column buffers: device
-    calls with two or more 8-column blocks alternate them between the
-    caller's stream and the second lane's stream. 27 columns (blocks 8, 8, 8,
-    2, 1) through device pointers (ld > n, on torch's current stream) equal
-    the host-pointer call bitwise (one lane, the same blocks), which equals
-    single applications to rounding; repeated device calls are bitwise
-    deterministic."""
+    """A plan for 32 columns keeps a second and a third set of 8-column
+    buffers: device calls with two or more 8-column blocks deal them out
+    between the caller's stream and the further lanes' streams (three lanes
+    on a plan that is not autotuned). 27 columns (blocks 8, 8, 8, 2, 1, the
+    8-column ones on lanes 0, 1, 2) through device pointers (ld > n, on
+    torch's current stream) equal the host-pointer call bitwise (one lane,
+    the same blocks), which equals single applications to rounding; repeated
+    device calls are bitwise deterministic."""
     torch = pytest.importorskip("torch")
     kw = configs.scaled(I.star(a=[0, 0, 0, 0, 0, 0.3]), 800, 60.0, 7)
     fm = I.ForwardMap(**kw, precision=precision, nrhs_max=32)
