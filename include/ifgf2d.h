@@ -192,7 +192,9 @@ int ifgf_apply(ifgf_plan* plan, const double* x, double* y, int nrhs, long long 
  * for callers that replay the same application many times, e.g. a solver
  * loop over fixed Krylov buffers). The first call with a given (x, y, nrhs,
  * ld, IFGF_APPLY_FP64) captures the application's launches on the plan's
- * stream into a graph; every call launches that graph on stream (NULL = the
+ * stream into a graph (one single-column application per column: the batched
+ * column blocks and their lanes are not captured); every call launches that
+ * graph on stream (NULL = the
  * plan's stream), asynchronously. x / y are device pointers whose contents
  * may change between calls; a call with other pointers recaptures. Fails
  * while stage profiling is on. */
